@@ -1,0 +1,89 @@
+// Periodic-box Voronoi meshes and the Charles ideal-gas operator arrays built
+// on them (host C++, setup time only).
+//
+// Conventions (all indices are owner cell indices in Morton order):
+//  * a compact face f joins cv0[f] < cv1[f]; the cv1 side is the periodic image
+//    x_cv[cv1] + shift[f]*L, n_fa[f] is the area vector pointing from cv0 to that
+//    image, x_fa[f] is the face centroid in cv0's frame.
+//  * an extended pair p joins lo[p] < hi[p] (image shift pshift[p]); the first
+//    nfa pairs are the compact faces in face order, the remaining pairs follow
+//    sorted by (lo, hi, shift).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cgmesh {
+
+struct Voronoi {
+  int ncv = 0;
+  double L[3] = {0, 0, 0};
+  int lattice[3] = {0, 0, 0};   // generator lattice (0 when built from raw arrays)
+  std::vector<double> x_vv;     // generators [ncv][3]
+  std::vector<double> x_cv;     // centroids  [ncv][3]
+  std::vector<double> vol;      // [ncv]
+  int nfa = 0;
+  std::vector<int> cvofa;       // [nfa][2]
+  std::vector<int8_t> shift;    // [nfa][3]
+  std::vector<double> n_fa;     // [nfa][3]
+  std::vector<double> x_fa;     // [nfa][3]
+  int ndropped = 0;             // one-sided slivers removed during the merge
+};
+
+struct OperatorParams {
+  double interp_factor = 1.0 / 3.0;   // EF_OPS.INTERP_FACTOR
+  double sigma_thresh = 0.45;         // SIGMA_THRESH
+  int group_member_threshold = 100;   // GROUP_MEMBER_THRESHOLD
+  bool disable_grouping = false;      // DISABLE_FA_GROUPING
+  bool zero_c_ef = false;             // ZERO_C_EF
+};
+
+// Everything the time step consumes.
+struct Operators {
+  int ncv = 0, nfa = 0, nef = 0;
+  // per cell
+  std::vector<double> vol, inv_vol, x_cv, sa_to_vol, dcmag_hat, cmag;
+  // extended pairs (raw operator, before grouping substitution)
+  std::vector<int> lo, hi;             // [nef]
+  std::vector<int8_t> pshift;          // [nef][3]
+  std::vector<double> n_ef, c_ef;      // [nef][3]
+  // geometry the flux uses (grouping applied: grouped pairs take the
+  // representative's n and carry c = 0, i.e. no symmetric flux)
+  std::vector<double> n_use, c_use;    // [nef][3]
+  std::vector<int> group_ef;           // [nef]
+  // compact faces (pair index == face index)
+  std::vector<double> area, area_over_delta, unit_n;    // raw [nfa], [nfa][3]
+  std::vector<double> area_use, aod_use, unit_n_use;    // grouping applied
+  std::vector<int> group_fa;           // [nfa]
+  // gradient operator: per cell, diagonal first then neighbours in face order
+  std::vector<int> cvocv_i, cvocv_v;   // CSR [ncv+1], [nnz]
+  std::vector<double> grad_coeff;      // [nnz][3] (1/vol folded in)
+  std::vector<int> grad_flag;          // [ncv] 1 if a singular value was clipped
+  // per-cell lists of compact faces (face order) and of extended pairs (pair order)
+  std::vector<int> faocv_i, faocv_v;   // CSR over compact faces
+  std::vector<int> efocv_i, efocv_v;   // CSR over extended pairs
+  int ngrouped_ef = 0, ngrouped_fa = 0, nthresholded = 0;
+  double max_gcl = 0.0;                // max |sum_f n_fa| / vol^(2/3)
+};
+
+// Builders. Return an empty string on success, else an error message.
+std::string build_lattice(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz);
+std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                           double jitter, uint64_t seed, int nthreads);
+std::string build_from_arrays(Voronoi& m, int ncv, const double* x_cv, const double* vol,
+                              int nfa, const int* cvofa, const int* shift, const double* n_fa,
+                              const double* x_fa, const double L[3]);
+std::string build_operators(const Voronoi& m, const OperatorParams& p, Operators& op, int nthreads);
+
+// Morton code of a lattice index (10 bits per axis).
+uint64_t morton3(uint32_t i, uint32_t j, uint32_t k);
+
+// Counter-based uniform double in [0,1) (splitmix64 of seed and counter).
+double hash_uniform(uint64_t seed, uint64_t counter);
+
+// One-sided Jacobi SVD of a 3x3 column-major matrix: A = U diag(s) V^T.
+void svd33(const double A[9], double U[9], double V[9], double s[3]);
+
+int default_threads();
+
+}  // namespace cgmesh

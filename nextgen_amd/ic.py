@@ -1,0 +1,85 @@
+"""Initial fields and input decks of the periodic-box workloads.
+
+Taylor-Green vortex on [0, 2*pi]^3 (the TGV deck of the ideal-gas solver,
+evaluated at the cell centroids the way an ``INIT`` expression or
+``initialHook`` would set it, StaticSolver.hpp:1402-1470, charles_ig.cpp:209-222):
+
+    u =  U0 sin x cos y cos z,  v = -U0 cos x sin y cos z,  w = 0
+    p = p0 + rho0 U0^2 / 16 (cos 2x + cos 2y)(cos 2z + 2)
+    rho = p / (R T0),  rhoE = p/(gamma-1) + rho |u|^2 / 2   (initComplete, IdealGasSolver.hpp:282-285)
+
+Nondimensionalisation: U0 = 1, rho0 = 1, T0 = T_REF = 1, c0 = U0/Ma,
+P_REF = p0 = rho0 c0^2 / gamma, RHO_REF = 1 (so R = p0), MU_REF = rho0 U0 / Re,
+MU_POWER_LAW 0.75 (the reference default).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def tgv_params(Re=1600.0, Ma=0.1, gamma=1.4, sgs="NONE", cfl=0.8):
+    """charles.in text of the TGV deck."""
+    c0 = 1.0 / Ma
+    p0 = c0 * c0 / gamma
+    mu = 1.0 / Re
+    return (
+        f"GAMMA {gamma!r}\nP_REF {p0!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF {mu!r}\n"
+        f"MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL {sgs}\nTIMESTEP CFL {cfl!r}\n"
+    )
+
+
+def tgv_state(x_cv, Ma=0.1, gamma=1.4):
+    x, y, z = x_cv[:, 0], x_cv[:, 1], x_cv[:, 2]
+    c0 = 1.0 / Ma
+    p0 = c0 * c0 / gamma
+    R = p0  # RHO_REF = T_REF = 1
+    u = np.empty((len(x), 3))
+    u[:, 0] = np.sin(x) * np.cos(y) * np.cos(z)
+    u[:, 1] = -np.cos(x) * np.sin(y) * np.cos(z)
+    u[:, 2] = 0.0
+    p = p0 + (1.0 / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    rho = p / (R * 1.0)
+    rhoE = p / (gamma - 1.0) + 0.5 * rho * (u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2])
+    return rho, u, rhoE
+
+
+def k4_state(x_cv, seed=1, k0=4.0, Mt=0.3, gamma=1.4, kmax=None):
+    """Solenoidal random field with E(k) ~ k^4 exp(-2 (k/k0)^2), random phases,
+    scaled so u_rms'/c0 = Mt with c0 = 1/Mt... (uniform rho = 1, p = p0).
+
+    Direct Fourier sum over integer wave vectors |k| <= kmax evaluated at the
+    centroids (small meshes: cost ~ ncv * nmodes).
+    """
+    rng = np.random.default_rng(seed)
+    kmax = kmax if kmax is not None else int(np.ceil(2.5 * k0))
+    ks = []
+    for a in range(-kmax, kmax + 1):
+        for b in range(-kmax, kmax + 1):
+            for c in range(0, kmax + 1):
+                if c == 0 and (b < 0 or (b == 0 and a <= 0)):
+                    continue
+                if a * a + b * b + c * c <= kmax * kmax:
+                    ks.append((a, b, c))
+    ks = np.array(ks, dtype=np.float64)
+    kmag = np.sqrt((ks ** 2).sum(1))
+    # modes per unit shell ~ 2 pi k^2 (half space): amplitude^2 ~ E(k) / (2 pi k^2)
+    E = kmag ** 4 * np.exp(-2.0 * (kmag / k0) ** 2)
+    amp = np.sqrt(E / (2.0 * np.pi * kmag ** 2))
+    # random direction perpendicular to k
+    r = rng.normal(size=ks.shape)
+    d = np.cross(ks, r)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    phase = rng.uniform(0.0, 2.0 * np.pi, size=len(ks))
+    u = np.zeros((len(x_cv), 3))
+    for i in range(len(ks)):
+        arg = x_cv @ ks[i] + phase[i]
+        u += np.outer(np.cos(arg), amp[i] * d[i])
+    u -= u.mean(axis=0)
+    urms = np.sqrt((u ** 2).sum(1).mean() / 3.0)
+    c0 = 1.0
+    u *= Mt * c0 / urms
+    p0 = c0 * c0 / gamma
+    rho = np.ones(len(x_cv))
+    p = np.full(len(x_cv), p0)
+    rhoE = p / (gamma - 1.0) + 0.5 * rho * (u ** 2).sum(1)
+    return rho, u, rhoE

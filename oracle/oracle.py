@@ -1,0 +1,185 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests, the smoke check and the CPU-baseline leg of the benchmark import
+this module, and only as the checker: it loads ``oracle/liboracle.so`` (the C
+restatement in ig_oracle.c) and, when present, ``oracle/_ref/libref_pin.so``
+(the reference's own header code, used to pin the restatement).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_pin.so")
+
+
+class og_mesh(C.Structure):
+    _fields_ = [("ncv", C.c_int), ("nfa", C.c_int), ("nef", C.c_int)] + [
+        (n, C.c_void_p) for n in ("vol", "inv_vol", "sa_to_vol", "dcmag_hat", "lo", "hi", "n_use", "c_use",
+                                  "unit_n_use", "area_use", "aod_use", "n_fa", "cvocv_i", "cvocv_v", "grad_coeff")
+    ]
+
+
+class og_params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("p_ref", C.c_double), ("rho_ref", C.c_double), ("T_ref", C.c_double),
+        ("mu_ref", C.c_double), ("mu_power_law", C.c_double), ("Pr_lam", C.c_double), ("sgs_model", C.c_int),
+        ("vreman_coeff", C.c_double), ("a_sgs_coeff", C.c_double), ("a_sgs_coeff2", C.c_double),
+        ("lambda_dil_coeff", C.c_double), ("Pr_t", C.c_double), ("limit_asgs", C.c_int), ("coeff_p", C.c_double),
+        ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
+        ("dt_data", C.c_double * 2), ("check_interval", C.c_int),
+    ]
+
+
+class og_diag(C.Structure):
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("dt", C.c_double), ("ke", C.c_double),
+                ("enstrophy", C.c_double), ("sums", C.c_double * 5), ("cfl_max", C.c_double),
+                ("nan_flag", C.c_int)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so missing: run __graft_entry__.build()")
+        L = C.CDLL(ORACLE_SO)
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        L.og_create.restype = vp
+        L.og_create.argtypes = [C.POINTER(og_mesh), C.POINTER(og_params)]
+        L.og_destroy.argtypes = [vp]
+        L.og_set_state.argtypes = [vp, dp, dp, dp]
+        L.og_get_state.argtypes = [vp, dp, dp, dp]
+        L.og_get_field.argtypes = [vp, C.c_char_p, dp]
+        L.og_get_field.restype = C.c_int
+        for f in ("og_update_state", "og_sgs_props", "og_init_complete", "og_store", "og_restore"):
+            getattr(L, f).argtypes = [vp]
+        L.og_calc_rhs.argtypes = [vp, dp]
+        L.og_calc_cfl.argtypes = [vp, C.c_double]
+        L.og_calc_cfl.restype = C.c_double
+        L.og_step.argtypes = [vp]
+        L.og_step.restype = C.c_int
+        L.og_diag_get.argtypes = [vp, C.POINTER(og_diag)]
+        for f in ("og_flux_skew", "og_flux_symmetric", "og_flux_compact"):
+            getattr(L, f).restype = None
+        L.og_flux_skew.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp]
+        L.og_flux_symmetric.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.og_flux_compact.argtypes = [dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp]
+        L.og_fast_pow_posx.restype = C.c_double
+        L.og_fast_pow_posx.argtypes = [C.c_double, C.c_double]
+        L.og_gr_func.restype = C.c_double
+        L.og_gr_func.argtypes = [C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own header code (None when it was not built)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        R = C.CDLL(REF_SO)
+        dp = C.POINTER(C.c_double)
+        R.ref_flux_skew.argtypes = [dp, dp, dp, dp, dp]
+        R.ref_flux_symmetric.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        R.ref_flux_compact.argtypes = [dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp]
+        R.ref_fast_pow_posx.restype = C.c_double
+        R.ref_fast_pow_posx.argtypes = [C.c_double, C.c_double]
+        R.ref_svd33.restype = C.c_int
+        R.ref_svd33.argtypes = [dp, dp, dp, dp, dp]
+        R.ref_erk_wgt3.argtypes = [dp]
+        _ref = R
+    return _ref
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def params_from(cgp) -> og_params:
+    p = og_params()
+    for name, _ in og_params._fields_:
+        if name == "dt_data":
+            p.dt_data[0] = cgp.dt_data[0]
+            p.dt_data[1] = cgp.dt_data[1]
+        else:
+            setattr(p, name, getattr(cgp, name))
+    return p
+
+
+class OracleSolver:
+    """The oracle step on the operator arrays of a product mesh."""
+
+    def __init__(self, mesh, cgparams):
+        self._keep = {}
+        m = og_mesh()
+        m.ncv, m.nfa, m.nef = mesh.ncv, mesh.nfa, mesh.nef
+        names = {"vol": "vol", "inv_vol": "inv_vol", "sa_to_vol": "sa_to_vol", "dcmag_hat": "dcmag_hat",
+                 "lo": "lo", "hi": "hi", "n_use": "n_use", "c_use": "c_use", "unit_n_use": "unit_n_use",
+                 "area_use": "area_use", "aod_use": "aod_use", "n_fa": "n_fa", "cvocv_i": "cvocv_i",
+                 "cvocv_v": "cvocv_v", "grad_coeff": "grad_coeff"}
+        for field, arr in names.items():
+            a = np.ascontiguousarray(mesh.get(arr))
+            self._keep[field] = a
+            setattr(m, field, a.ctypes.data)
+        self._m = m
+        self._p = params_from(cgparams)
+        self.ncv = mesh.ncv
+        self._h = lib().og_create(C.byref(self._m), C.byref(self._p))
+
+    def set_state(self, rho, u, rhoE):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.ncv, 3)
+        rhoE = np.ascontiguousarray(rhoE, dtype=np.float64)
+        lib().og_set_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
+
+    def get_state(self):
+        rho, u, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
+        lib().og_get_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
+        return rho, u, rhoE
+
+    def get_field(self, name):
+        out = np.empty((self.ncv, 3, 3) if name == "dudx" else self.ncv)
+        if lib().og_get_field(self._h, name.encode(), dptr(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def init_complete(self):
+        lib().og_init_complete(self._h)
+
+    def calc_rhs(self):
+        out = np.empty((self.ncv, 5))
+        lib().og_calc_rhs(self._h, dptr(out))
+        return out
+
+    def calc_cfl(self, dt):
+        return lib().og_calc_cfl(self._h, dt)
+
+    def step(self, n=1):
+        rc = 0
+        for _ in range(n):
+            rc = lib().og_step(self._h)
+        return rc
+
+    def diag(self):
+        d = og_diag()
+        lib().og_diag_get(self._h, C.byref(d))
+        return {"step": d.step, "time": d.time, "dt": d.dt, "ke": d.ke, "enstrophy": d.enstrophy,
+                "sums": list(d.sums), "cfl_max": d.cfl_max, "nan_flag": d.nan_flag}
+
+    def close(self):
+        if self._h:
+            lib().og_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
