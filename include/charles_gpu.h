@@ -172,10 +172,12 @@ int cg_step_history(cg_solver* s, int nsteps, cg_diag* diag);
 /* storeData / restoreData (IdealGasSolver.hpp:333-405). */
 int cg_store(cg_solver* s);
 int cg_restore(cg_solver* s);
-/* Host-side time of nsteps steps measured with HIP events on the solver's
- * stream (warm state, inputs resident); also returns the average duration of
- * each kernel family when kernel_ms is non-null (kernel_ms[0..3] = flux+RK,
- * gradient, properties, total per step). */
+/* Time of nsteps steps measured with HIP events on the solver's stream (warm
+ * state, inputs resident in device memory). When kernel_ms is non-null, a
+ * second run of nsteps steps records events around every launch and returns
+ * the average duration per launch of kernel_ms[0] compact-face flux,
+ * [1] extended-face flux + RK3 + EOS, [2] gradient, [3] properties, and
+ * [4] the sum of all kernel durations per step. */
 int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms);
 const char* cg_last_error(const cg_solver* s);
 void cg_destroy(cg_solver* s);

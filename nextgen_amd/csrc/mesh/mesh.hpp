@@ -49,7 +49,9 @@ struct Operators {
   std::vector<double> n_ef, c_ef;      // [nef][3]
   // geometry the flux uses (grouping applied: grouped pairs take the
   // representative's n and carry c = 0, i.e. no symmetric flux)
-  std::vector<double> n_use, c_use;    // [nef][3]
+  std::vector<double> n_use, c_use;    // [nef][3], empty when nothing is grouped
+  const std::vector<double>& nU() const { return n_use.empty() ? n_ef : n_use; }
+  const std::vector<double>& cU() const { return c_use.empty() ? c_ef : c_use; }
   std::vector<int> group_ef;           // [nef]
   // compact faces (pair index == face index)
   std::vector<double> area, area_over_delta, unit_n;    // raw [nfa], [nfa][3]

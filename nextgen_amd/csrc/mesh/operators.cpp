@@ -536,12 +536,14 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
     }
     group_faces(m.n_fa, candf, P.group_member_threshold / 2, op.group_fa);
   }
-  op.n_use = op.n_ef;
-  op.c_use = op.c_ef;
   {
     std::vector<std::pair<int, int>> gl;
     for (int p = 0; p < nef; ++p)
       if (op.group_ef[p] >= 0) gl.push_back({op.group_ef[p], p});
+    if (!gl.empty()) {
+      op.n_use = op.n_ef;
+      op.c_use = op.c_ef;
+    }
     std::stable_sort(gl.begin(), gl.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
     int cur_rep = 0;
     for (size_t i = 0; i < gl.size(); ++i) {
@@ -573,37 +575,6 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
     }
   }
 
-  // ---- per-cell measures of the symmetric operator (FlowSolver.hpp:702-758) ----
-  op.cmag.assign(ncv, 0.0);
-  op.dcmag_hat.assign(ncv, 0.0);
-  for (int p = 0; p < nef; ++p) {
-    const double cm = mag3(&op.c_ef[3 * size_t(p)]);
-    const int l = op.lo[p], h = op.hi[p];
-    double d2 = 0.0;
-    for (int i = 0; i < 3; ++i) {
-      const double dd = (m.x_cv[3 * h + i] + op.pshift[3 * size_t(p) + i] * L[i]) - m.x_cv[3 * l + i];
-      d2 += std::pow(dd, 2);
-    }
-    const double delta = std::sqrt(d2);
-    op.cmag[l] += cm;
-    op.cmag[h] += cm;
-    op.dcmag_hat[l] += delta * cm;
-    op.dcmag_hat[h] += delta * cm;
-  }
-  op.sa_to_vol.assign(ncv, 0.0);
-  for (int c = 0; c < ncv; ++c) {
-    double area_sum = 0.0;
-    for (int foc = op.faocv_i[c]; foc < op.faocv_i[c + 1]; ++foc) area_sum += mag3(&m.n_fa[3 * op.faocv_v[foc]]);
-    if (!(area_sum > 0.0)) return "cell without faces";
-    op.dcmag_hat[c] /= area_sum;
-  }
-  for (int f = 0; f < nfa; ++f) {
-    const double nm = mag3(&m.n_fa[3 * f]);
-    op.sa_to_vol[m.cvofa[2 * f]] += nm;
-    op.sa_to_vol[m.cvofa[2 * f + 1]] += nm;
-  }
-  for (int c = 0; c < ncv; ++c) op.sa_to_vol[c] = op.sa_to_vol[c] / m.vol[c];
-
   // ---- extended pairs of each cell, in pair order ----
   op.efocv_i.assign(ncv + 1, 0);
   for (int p = 0; p < nef; ++p) {
@@ -619,6 +590,45 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
       op.efocv_v[pos[op.hi[p]]++] = p;
     }
   }
+
+  // ---- per-cell measures of the symmetric operator (FlowSolver.hpp:702-758) ----
+  // gathered per cell over its pairs in pair order (the order of a serial pair loop)
+  op.cmag.assign(ncv, 0.0);
+  op.dcmag_hat.assign(ncv, 0.0);
+  parallel_blocks(ncv, nthreads, [&](int, int b, int e) {
+    for (int c = b; c < e; ++c) {
+      double cms = 0.0, dcs = 0.0;
+      for (int k = op.efocv_i[c]; k < op.efocv_i[c + 1]; ++k) {
+        const int p = op.efocv_v[k];
+        const double cm = mag3(&op.c_ef[3 * size_t(p)]);
+        const int l = op.lo[p], h = op.hi[p];
+        double d2 = 0.0;
+        for (int i = 0; i < 3; ++i) {
+          const double dd = (m.x_cv[3 * h + i] + op.pshift[3 * size_t(p) + i] * L[i]) - m.x_cv[3 * l + i];
+          d2 += std::pow(dd, 2);
+        }
+        const double delta = std::sqrt(d2);
+        cms += cm;
+        dcs += delta * cm;
+      }
+      op.cmag[c] = cms;
+      op.dcmag_hat[c] = dcs;
+    }
+  });
+  op.sa_to_vol.assign(ncv, 0.0);
+  for (int c = 0; c < ncv; ++c) {
+    double area_sum = 0.0;
+    for (int foc = op.faocv_i[c]; foc < op.faocv_i[c + 1]; ++foc) area_sum += mag3(&m.n_fa[3 * op.faocv_v[foc]]);
+    if (!(area_sum > 0.0)) return "cell without faces";
+    op.dcmag_hat[c] /= area_sum;
+  }
+  for (int f = 0; f < nfa; ++f) {
+    const double nm = mag3(&m.n_fa[3 * f]);
+    op.sa_to_vol[m.cvofa[2 * f]] += nm;
+    op.sa_to_vol[m.cvofa[2 * f + 1]] += nm;
+  }
+  for (int c = 0; c < ncv; ++c) op.sa_to_vol[c] = op.sa_to_vol[c] / m.vol[c];
+
   return "";
 }
 

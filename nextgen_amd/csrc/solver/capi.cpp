@@ -12,11 +12,14 @@
 
 #include <algorithm>
 #include <cctype>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/charles_gpu.h"
@@ -50,8 +53,9 @@ struct cg_solver {
   cgdev::State st[2]{};
   int cur = 0;
   cgdev::Work w{};
-  double* store_buf = nullptr;     // older snapshot [5][ncv]
-  double* store_recent = nullptr;  // newer snapshot [5][ncv]
+  double* store_buf = nullptr;     // older snapshot
+  double* store_recent = nullptr;  // newer snapshot
+  size_t snap_n = 0;               // doubles per snapshot
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
   bool have_store = false;
@@ -179,7 +183,11 @@ int finish_mesh(cg_mesh* m, const cg_params* p, cg_mesh** out) {
     cg_params_default(&def);
     p = &def;
   }
+  const auto t0 = std::chrono::steady_clock::now();
   std::string e = cgmesh::build_operators(m->vor, op_params(p), m->op, cgmesh::default_threads());
+  if (std::getenv("CG_VERBOSE"))
+    std::fprintf(stderr, "build_operators: %.2f s\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   if (!e.empty()) {
     delete m;
     return fail(CG_EMESH, e);
@@ -202,14 +210,16 @@ int refresh(cg_solver* s) {
   return 0;
 }
 
+// snapshot layout: light records [ncv][8] followed by rhoE [ncv]
 int snapshot_copy(cg_solver* s, double* dst, bool to_snapshot) {
   const cgdev::State& st = s->st[s->cur];
-  double* f[5] = {st.rho, st.u0, st.u1, st.u2, st.rhoE};
-  for (int q = 0; q < 5; ++q) {
-    if (to_snapshot)
-      HIPCHK(s, hipMemcpyAsync(dst + (size_t)q * s->ncv, f[q], sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
-    else
-      HIPCHK(s, hipMemcpyAsync(f[q], dst + (size_t)q * s->ncv, sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
+  const size_t nl = (size_t)s->ncv * cgdev::L_N;
+  if (to_snapshot) {
+    HIPCHK(s, hipMemcpyAsync(dst, st.light, sizeof(double) * nl, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(dst + nl, st.rhoE, sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
+  } else {
+    HIPCHK(s, hipMemcpyAsync(st.light, dst, sizeof(double) * nl, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(st.rhoE, dst + nl, sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
   }
   return 0;
 }
@@ -259,7 +269,7 @@ int do_store(cg_solver* s) {
   if (int rc = read_info(s)) return rc;
   if (s->info_host->nan) return 0;  // never snapshot a NaN state
   if (s->have_store) {
-    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * 5 * s->ncv, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * s->snap_n, hipMemcpyDeviceToDevice, s->stream));
     s->store_step = s->store_step_recent;
     s->store_time = s->store_time_recent;
   }
@@ -267,7 +277,7 @@ int do_store(cg_solver* s) {
   s->store_step_recent = s->info_host->step;
   s->store_time_recent = s->info_host->time;
   if (!s->have_store) {
-    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * 5 * s->ncv, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * s->snap_n, hipMemcpyDeviceToDevice, s->stream));
     s->store_step = s->store_step_recent;
     s->store_time = s->store_time_recent;
     s->have_store = true;
@@ -283,6 +293,16 @@ int get_soa(cg_solver* s, double* const dev[], int ncomp, double* out) {
     HIPCHK(s, hipStreamSynchronize(s->stream));
     for (int c = 0; c < s->ncv; ++c) out[(size_t)c * ncomp + q] = tmp[c];
   }
+  return 0;
+}
+
+// copy fields [f0, f0+nf) of a device record array [ncv][stride] to out[ncv][nf]
+int get_rec(cg_solver* s, const double* rec, int stride, int f0, int nf, double* out) {
+  std::vector<double> tmp((size_t)s->ncv * stride);
+  HIPCHK(s, hipMemcpyAsync(tmp.data(), rec, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (int c = 0; c < s->ncv; ++c)
+    for (int q = 0; q < nf; ++q) out[(size_t)c * nf + q] = tmp[(size_t)c * stride + f0 + q];
   return 0;
 }
 
@@ -430,7 +450,11 @@ int cg_mesh_jittered(int nx, int ny, int nz, double Lx, double Ly, double Lz, do
   if (!out) return fail(CG_EINVAL, "null output");
   if (!(Lx > 0 && Ly > 0 && Lz > 0)) return fail(CG_EINVAL, "box lengths must be positive");
   cg_mesh* m = new cg_mesh();
+  const auto t0 = std::chrono::steady_clock::now();
   std::string e = cgmesh::build_jittered(m->vor, nx, ny, nz, Lx, Ly, Lz, jitter, seed, cgmesh::default_threads());
+  if (std::getenv("CG_VERBOSE"))
+    std::fprintf(stderr, "cg_mesh_jittered: voronoi %.2f s\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   if (!e.empty()) {
     delete m;
     return fail(CG_EMESH, e);
@@ -482,7 +506,7 @@ int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes) {
   if (false) {
   }
   ARR("x_cv", v.x_cv) ARR("x_vv", v.x_vv) ARR("vol", v.vol) ARR("cvofa", v.cvofa) ARR("n_fa", v.n_fa) ARR("x_fa", v.x_fa)
-  ARR("lo", o.lo) ARR("hi", o.hi) ARR("n_ef", o.n_ef) ARR("c_ef", o.c_ef) ARR("n_use", o.n_use) ARR("c_use", o.c_use)
+  ARR("lo", o.lo) ARR("hi", o.hi) ARR("n_ef", o.n_ef) ARR("c_ef", o.c_ef) ARR("n_use", o.nU()) ARR("c_use", o.cU())
   ARR("group_ef", o.group_ef) ARR("group_fa", o.group_fa) ARR("area", o.area) ARR("area_over_delta", o.area_over_delta)
   ARR("unit_n", o.unit_n) ARR("area_use", o.area_use) ARR("aod_use", o.aod_use) ARR("unit_n_use", o.unit_n_use)
   ARR("cvocv_i", o.cvocv_i) ARR("cvocv_v", o.cvocv_v) ARR("grad_coeff", o.grad_coeff) ARR("inv_vol", o.inv_vol)
@@ -555,47 +579,109 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
   rc |= upload(s, &g.dx23, dx23);
   if (rc) return bail(rc);
 
-  // extended half-faces
+  // extended half-faces (per-slot geometry), compact ones first in each cell
   {
-    std::vector<int> cnt(ncv);
-    for (int c = 0; c < ncv; ++c) cnt[c] = o.efocv_i[c + 1] - o.efocv_i[c];
-    Sell sl;
+    std::vector<int> cnt(ncv), ccnt(ncv);
+    for (int c = 0; c < ncv; ++c) {
+      cnt[c] = o.efocv_i[c + 1] - o.efocv_i[c];
+      int nc = 0;
+      while (nc < cnt[c] && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
+      ccnt[c] = nc;
+      for (int k = nc; k < cnt[c]; ++k)
+        if (o.efocv_v[o.efocv_i[c] + k] < nfa) return bail(s->set(CG_EMESH, "compact pairs must precede extended ones"));
+    }
+    Sell sl, cs;
     sl.build(ncv, cnt);
-    std::vector<int> nbr(sl.nslots, -1), code(sl.nslots, 0);
+    cs.build(ncv, ccnt);
+    const std::vector<double>& nU = o.nU();
+    const std::vector<double>& cU = o.cU();
+    const std::vector<double>& nf = m->vor.n_fa;
+    std::vector<int> nbr(sl.nslots, 0);
+    std::vector<double> g6[6];
+    for (auto& v : g6) v.assign(sl.nslots, 0.0);
+    std::vector<double> c8[8];
+    for (auto& v : c8) v.assign(cs.nslots, 0.0);
     for (int c = 0; c < ncv; ++c)
       for (int k = 0; k < cnt[c]; ++k) {
         const int pidx = o.efocv_v[o.efocv_i[c] + k];
         const bool hi = (o.hi[pidx] == c);
         const int64_t sidx = sl.slot(c, k);
-        nbr[sidx] = hi ? o.lo[pidx] : o.hi[pidx];
-        code[sidx] = int(uint32_t(pidx) | (hi ? 0x80000000u : 0u));
+        nbr[sidx] = int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
+        for (int d = 0; d < 3; ++d) {
+          g6[d][sidx] = nU[3 * (size_t)pidx + d];
+          g6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+        }
+        if (k < ccnt[c]) {
+          const int64_t cidx = cs.slot(c, k);
+          for (int d = 0; d < 3; ++d) {
+            c8[d][cidx] = o.unit_n_use[3 * (size_t)pidx + d];
+            c8[5 + d][cidx] = nf[3 * (size_t)pidx + d];
+          }
+          c8[3][cidx] = o.area_use[pidx];
+          c8[4][cidx] = o.aod_use[pidx];
+        }
       }
+    // per-block halos for LDS staging of the extended-face pass
+    {
+      const int nblk = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
+      std::vector<std::vector<int>> halos(nblk);
+      std::vector<int16_t> loc(sl.nslots, 0);
+      std::vector<std::string> herr(1);
+      const int nth = cgmesh::default_threads();
+      std::vector<std::thread> th;
+      for (int t = 0; t < nth; ++t)
+        th.emplace_back([&, t] {
+          std::vector<int> v;
+          for (int b = t; b < nblk; b += nth) {
+            const int c0 = b * cgdev::kBlock, c1 = std::min(ncv, c0 + cgdev::kBlock);
+            v.clear();
+            for (int c = c0; c < c1; ++c) {
+              v.push_back(c);
+              for (int k = 0; k < cnt[c]; ++k) v.push_back(nbr[sl.slot(c, k)] & 0x7fffffff);
+            }
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            if (v.size() >= 0x8000) { herr[0] = "block halo too large"; return; }
+            for (int c = c0; c < c1; ++c)
+              for (int k = 0; k < cnt[c]; ++k) {
+                const int64_t sidx = sl.slot(c, k);
+                const int nb = nbr[sidx] & 0x7fffffff;
+                const int pos = int(std::lower_bound(v.begin(), v.end(), nb) - v.begin());
+                loc[sidx] = int16_t(pos | ((nbr[sidx] < 0) ? 0x8000 : 0));
+              }
+            halos[b] = v;
+          }
+        });
+      for (auto& x : th) x.join();
+      int hmax = 0;
+      std::vector<int> hptr(nblk + 1, 0), hv;
+      for (int b = 0; b < nblk; ++b) {
+        hptr[b + 1] = hptr[b] + int(halos[b].size());
+        hmax = std::max(hmax, int(halos[b].size()));
+      }
+      hv.reserve(hptr[nblk]);
+      for (auto& h : halos) hv.insert(hv.end(), h.begin(), h.end());
+      int dev = 0, lds_max = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      const bool use_lds = herr[0].empty() && (size_t)hmax * 48 <= (size_t)lds_max && !std::getenv("CG_NO_LDS");
+      if (use_lds) {
+        rc |= upload(s, &g.halo_ptr, hptr);
+        rc |= upload(s, &g.halo_v, hv);
+        rc |= upload(s, &g.ef_loc, loc);
+        g.halo_max = hmax;
+      }
+    }
     rc |= upload(s, &g.ef_ptr, sl.ptr);
     rc |= upload(s, &g.ef_deg, sl.deg);
+    rc |= upload(s, &g.ef_nc, ccnt);
     rc |= upload(s, &g.ef_nbr, nbr);
-    rc |= upload(s, &g.ef_code, code);
-    if (rc) return bail(rc);
-  }
-  {
-    std::vector<double> a0(nef), a1(nef), a2(nef), b0(nef), b1(nef), b2(nef);
-    for (int q = 0; q < nef; ++q) {
-      a0[q] = o.n_use[3 * (size_t)q]; a1[q] = o.n_use[3 * (size_t)q + 1]; a2[q] = o.n_use[3 * (size_t)q + 2];
-      b0[q] = o.c_use[3 * (size_t)q]; b1[q] = o.c_use[3 * (size_t)q + 1]; b2[q] = o.c_use[3 * (size_t)q + 2];
-    }
-    rc |= upload(s, &g.pn0, a0); rc |= upload(s, &g.pn1, a1); rc |= upload(s, &g.pn2, a2);
-    rc |= upload(s, &g.pc0, b0); rc |= upload(s, &g.pc1, b1); rc |= upload(s, &g.pc2, b2);
-    if (rc) return bail(rc);
-  }
-  {
-    std::vector<double> u0(nfa), u1(nfa), u2(nfa), f0(nfa), f1(nfa), f2(nfa);
-    const std::vector<double>& nf = m->vor.n_fa;
-    for (int f = 0; f < nfa; ++f) {
-      u0[f] = o.unit_n_use[3 * f]; u1[f] = o.unit_n_use[3 * f + 1]; u2[f] = o.unit_n_use[3 * f + 2];
-      f0[f] = nf[3 * f]; f1[f] = nf[3 * f + 1]; f2[f] = nf[3 * f + 2];
-    }
-    rc |= upload(s, &g.un0, u0); rc |= upload(s, &g.un1, u1); rc |= upload(s, &g.un2, u2);
-    rc |= upload(s, &g.area, o.area_use); rc |= upload(s, &g.aod, o.aod_use);
-    rc |= upload(s, &g.nf0, f0); rc |= upload(s, &g.nf1, f1); rc |= upload(s, &g.nf2, f2);
+    rc |= upload(s, &g.sn0, g6[0]); rc |= upload(s, &g.sn1, g6[1]); rc |= upload(s, &g.sn2, g6[2]);
+    rc |= upload(s, &g.sc0, g6[3]); rc |= upload(s, &g.sc1, g6[4]); rc |= upload(s, &g.sc2, g6[5]);
+    rc |= upload(s, &g.cf_ptr, cs.ptr);
+    rc |= upload(s, &g.cu0, c8[0]); rc |= upload(s, &g.cu1, c8[1]); rc |= upload(s, &g.cu2, c8[2]);
+    rc |= upload(s, &g.carea, c8[3]); rc |= upload(s, &g.caod, c8[4]);
+    rc |= upload(s, &g.cn0, c8[5]); rc |= upload(s, &g.cn1, c8[6]); rc |= upload(s, &g.cn2, c8[7]);
     if (rc) return bail(rc);
   }
   // gradient stencil without the diagonal
@@ -623,35 +709,37 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
   }
   // state buffers
   for (int b = 0; b < 2; ++b) {
-    cgdev::State& st = s->st[b];
-    double** f[9] = {&st.rho, &st.u0, &st.u1, &st.u2, &st.rhoE, &st.spv, &st.p, &st.h, &st.sos};
-    for (auto pp : f) rc |= dev_alloc(s, pp, ncv);
+    rc |= dev_alloc(s, &s->st[b].light, (size_t)ncv * cgdev::L_N);
+    rc |= dev_alloc(s, &s->st[b].rhoE, ncv);
   }
   cgdev::Work& w = s->w;
-  rc |= dev_alloc(s, &w.dudx, 9 * (size_t)ncv);
-  double** wf[] = {&w.mu_total, &w.loc_total, &w.a_sgs_c, &w.fgr, &w.symh, &w.symp, &w.T, &w.logp, &w.invRT,
-                   &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
+  rc |= dev_alloc(s, &w.aux, (size_t)ncv * cgdev::A_N);
+  double** wf[] = {&w.logp, &w.T, &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
   for (auto pp : wf) rc |= dev_alloc(s, pp, ncv);
   rc |= dev_alloc(s, &w.rhs0, 5 * (size_t)ncv);
   rc |= dev_alloc(s, &w.rhs1, 5 * (size_t)ncv);
+  rc |= dev_alloc(s, &w.racc, 5 * (size_t)ncv);
   rc |= dev_alloc(s, &w.info, 1);
   rc |= dev_alloc(s, &w.partial, 8 * (size_t)s->nblocks);
   rc |= dev_alloc(s, &w.diag, 8);
-  rc |= dev_alloc(s, &s->store_buf, 5 * (size_t)ncv);
-  rc |= dev_alloc(s, &s->store_recent, 5 * (size_t)ncv);
+  s->snap_n = (size_t)ncv * (cgdev::L_N + 1);
+  rc |= dev_alloc(s, &s->store_buf, s->snap_n);
+  rc |= dev_alloc(s, &s->store_recent, s->snap_n);
   rc |= dev_alloc(s, &s->rhs_tmp, 5 * (size_t)ncv);
   if (rc) return bail(rc);
   if (hipMemcpy(w.info, s->info_host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice) != hipSuccess)
     return bail(s->set(CG_EHIP, "info upload failed"));
   // IdealGasSolver::initData reference state (IdealGasSolver.cpp:58-62)
-  for (int b = 0; b < 2; ++b) {
-    rc |= cgk::launch_fill(s->st[b].rho, p->rho_ref, ncv, s->stream);
-    rc |= cgk::launch_fill(s->st[b].u0, 0.0, ncv, s->stream);
-    rc |= cgk::launch_fill(s->st[b].u1, 0.0, ncv, s->stream);
-    rc |= cgk::launch_fill(s->st[b].u2, 0.0, ncv, s->stream);
-    rc |= cgk::launch_fill(s->st[b].rhoE, p->p_ref / (p->gamma - 1.0), ncv, s->stream);
+  {
+    std::vector<double> light((size_t)ncv * cgdev::L_N, 0.0), rhoE((size_t)ncv, p->p_ref / (p->gamma - 1.0));
+    for (int c = 0; c < ncv; ++c) light[(size_t)c * cgdev::L_N + cgdev::L_RHO] = p->rho_ref;
+    for (int b = 0; b < 2; ++b) {
+      if (hipMemcpy(s->st[b].light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->st[b].rhoE, rhoE.data(), sizeof(double) * rhoE.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(s->set(CG_EHIP, "initial state upload failed"));
+    }
   }
-  rc |= cgk::launch_fill(w.dudx, 0.0, 9 * (int64_t)ncv, s->stream);
+  rc |= cgk::launch_fill(w.aux, 0.0, (int64_t)ncv * cgdev::A_N, s->stream);
   rc |= cgk::launch_fill(w.rhs0, 0.0, 5 * (int64_t)ncv, s->stream);
   rc |= cgk::launch_fill(w.rhs1, 0.0, 5 * (int64_t)ncv, s->stream);
   if (rc || hipStreamSynchronize(s->stream) != hipSuccess) return bail(s->set(CG_EHIP, "initial fill failed"));
@@ -663,13 +751,15 @@ int cg_set_state(cg_solver* s, const double* rho, const double* u, const double*
   if (!s || !rho || !u || !rhoE) return fail(CG_EINVAL, "null argument");
   const cgdev::State& st = s->st[s->cur];
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  std::vector<double> tmp((size_t)s->ncv);
-  HIPCHK(s, hipMemcpy(st.rho, rho, sizeof(double) * s->ncv, hipMemcpyHostToDevice));
-  double* uc[3] = {st.u0, st.u1, st.u2};
-  for (int d = 0; d < 3; ++d) {
-    for (int c = 0; c < s->ncv; ++c) tmp[c] = u[3 * (size_t)c + d];
-    HIPCHK(s, hipMemcpy(uc[d], tmp.data(), sizeof(double) * s->ncv, hipMemcpyHostToDevice));
+  std::vector<double> light((size_t)s->ncv * cgdev::L_N, 0.0);
+  for (int c = 0; c < s->ncv; ++c) {
+    double* L = &light[(size_t)c * cgdev::L_N];
+    L[cgdev::L_U0] = u[3 * (size_t)c];
+    L[cgdev::L_U1] = u[3 * (size_t)c + 1];
+    L[cgdev::L_U2] = u[3 * (size_t)c + 2];
+    L[cgdev::L_RHO] = rho[c];
   }
+  HIPCHK(s, hipMemcpy(st.light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice));
   HIPCHK(s, hipMemcpy(st.rhoE, rhoE, sizeof(double) * s->ncv, hipMemcpyHostToDevice));
   s->ready = false;
   return CG_OK;
@@ -678,14 +768,10 @@ int cg_set_state(cg_solver* s, const double* rho, const double* u, const double*
 int cg_get_state(cg_solver* s, double* rho, double* u, double* rhoE) {
   if (!s) return fail(CG_EINVAL, "null solver");
   const cgdev::State& st = s->st[s->cur];
-  if (rho) {
-    double* f[1] = {st.rho};
-    if (int rc = get_soa(s, f, 1, rho)) return rc;
-  }
-  if (u) {
-    double* f[3] = {st.u0, st.u1, st.u2};
-    if (int rc = get_soa(s, f, 3, u)) return rc;
-  }
+  if (rho)
+    if (int rc = get_rec(s, st.light, cgdev::L_N, cgdev::L_RHO, 1, rho)) return rc;
+  if (u)
+    if (int rc = get_rec(s, st.light, cgdev::L_N, cgdev::L_U0, 3, u)) return rc;
   if (rhoE) {
     double* f[1] = {st.rhoE};
     if (int rc = get_soa(s, f, 1, rhoE)) return rc;
@@ -710,28 +796,25 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
   const cgdev::Work& w = s->w;
   if (nm == "dudx") {
     if (n != 9 * (int64_t)s->ncv) return s->set(CG_EINVAL, "dudx needs 9*ncv values");
-    double* f[9];
-    for (int k = 0; k < 9; ++k) f[k] = w.dudx + (size_t)k * s->ncv;
-    return get_soa(s, f, 9, out);
+    return get_rec(s, w.aux, cgdev::A_N, cgdev::A_D, 9, out);
   }
   if (n != s->ncv) return s->set(CG_EINVAL, "field needs ncv values");
-  // refresh the per-cell fields of the current state
+  struct { const char* nm; int light; int f; } recf[] = {
+      {"p", 1, cgdev::L_P}, {"sos", 1, cgdev::L_SOS}, {"h", 1, cgdev::L_H}, {"sp_vol", 1, cgdev::L_SPV},
+      {"mu_total", 0, cgdev::A_MU}, {"loc_total", 0, cgdev::A_LOC}, {"fgr", 0, cgdev::A_FGR},
+      {"symh_fax", 0, cgdev::A_SYMH}, {"symp_fax", 0, cgdev::A_SYMP}};
+  for (auto& r : recf)
+    if (nm == r.nm) return r.light ? get_rec(s, st.light, cgdev::L_N, r.f, 1, out) : get_rec(s, w.aux, cgdev::A_N, r.f, 1, out);
+  // per-step diagnostic fields: recompute the properties of the current state
   if (cgk::launch_props(s->g, st, s->w, s->P, 1, s->stream)) return s->set(CG_EHIP, "props launch failed");
   double* src = nullptr;
-  if (nm == "p") src = st.p;
-  else if (nm == "sos") src = st.sos;
-  else if (nm == "T") src = w.T;
+  if (nm == "T") src = w.T;
   else if (nm == "mu_lam") src = w.mu_lam;
   else if (nm == "mu_sgs") src = w.mu_sgs;
   else if (nm == "loc_lam") src = w.loc_lam;
   else if (nm == "loc_sgs") src = w.loc_sgs;
   else if (nm == "a_sgs") src = w.a_sgs;
   else if (nm == "vv2") src = w.vv2;
-  else if (nm == "fgr") src = w.fgr;
-  else if (nm == "mu_total") src = w.mu_total;
-  else if (nm == "loc_total") src = w.loc_total;
-  else if (nm == "symp_fax") src = w.symp;
-  else if (nm == "symh_fax") src = w.symh;
   else if (nm == "cfl") {
     if (read_info(s)) return CG_EHIP;
     if (cgk::launch_cfl(s->g, st, s->w, s->info_host->dt, s->stream)) return s->set(CG_EHIP, "cfl launch failed");
@@ -853,43 +936,47 @@ int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms)
   s->host_step += nsteps;
   if (kernel_ms) {
     // per-kernel durations: events bracket every launch of a second run
-    const int nk = 7;
+    // kernel_ms: 0 compact-face flux, 1 extended-face flux + RK3 + EOS, 2 gradient,
+    // 3 properties (per launch), 4 sum of all kernels per step
+    const int nk = 10;
     std::vector<hipEvent_t> ev(2 * nk);
     for (auto& e : ev) HIPCHK(s, hipEventCreate(&e));
-    double acc[3] = {0, 0, 0};
+    double acc[4] = {0, 0, 0, 0};
     double tot = 0.0;
+    int fam[nk];
     for (int i = 0; i < nsteps; ++i) {
       if (cgk::launch_begin_step(s->w, s->P, s->stream)) return s->set(CG_EHIP, "launch failed");
       int k = 0;
+      auto rec = [&](int family, auto&& launch) -> int {
+        HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
+        if (launch()) return s->set(CG_EHIP, "launch failed");
+        HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
+        fam[k++] = family;
+        return 0;
+      };
       for (int stage = 1; stage <= 3; ++stage) {
         const int nxt = 1 - s->cur;
-        HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
-        cgk::launch_flux(stage, 1, s->g, s->st[s->cur], s->st[nxt], s->w, s->P, nullptr, s->stream);
-        HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
-        ++k;
+        const cgdev::State in = s->st[s->cur], out = s->st[nxt];
+        if (int rc = rec(0, [&] { return cgk::launch_flux_compact(s->g, in, s->w, s->stream); })) return rc;
+        if (int rc = rec(1, [&] { return cgk::launch_flux_ext(stage, 1, s->g, in, out, s->w, s->P, nullptr, s->stream); })) return rc;
         s->cur = nxt;
-        HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
-        cgk::launch_grad(s->g, s->st[s->cur], s->w, s->stream);
-        HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
-        ++k;
+        if (int rc = rec(2, [&] { return cgk::launch_grad(s->g, s->st[s->cur], s->w, s->stream); })) return rc;
       }
-      HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
-      cgk::launch_props(s->g, s->st[s->cur], s->w, s->P, 0, s->stream);
-      HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
-      HIPCHK(s, hipEventSynchronize(ev[2 * k + 1]));
-      for (int q = 0; q < nk; ++q) {
+      if (int rc = rec(3, [&] { return cgk::launch_props(s->g, s->st[s->cur], s->w, s->P, 0, s->stream); })) return rc;
+      HIPCHK(s, hipEventSynchronize(ev[2 * (k - 1) + 1]));
+      for (int q = 0; q < k; ++q) {
         float t = 0.f;
         HIPCHK(s, hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
-        const int fam = (q == 6) ? 2 : (q % 2);
-        acc[fam] += t;
+        acc[fam[q]] += t;
         tot += t;
       }
     }
     s->host_step += nsteps;
     kernel_ms[0] = acc[0] / (3.0 * nsteps);
     kernel_ms[1] = acc[1] / (3.0 * nsteps);
-    kernel_ms[2] = acc[2] / nsteps;
-    kernel_ms[3] = tot / nsteps;
+    kernel_ms[2] = acc[2] / (3.0 * nsteps);
+    kernel_ms[3] = acc[3] / nsteps;
+    kernel_ms[4] = tot / nsteps;
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
   (void)hipEventDestroy(e0);

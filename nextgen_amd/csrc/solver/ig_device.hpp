@@ -1,4 +1,13 @@
-// Device-side data of the ideal-gas step (structure-of-arrays, Morton cell order).
+// Device-side data of the ideal-gas step (Morton cell order).
+//
+// Per-cell data that neighbours gather is packed in records so that one
+// neighbour costs one cache line per record instead of one line per field:
+//   light[c][8]  u0 u1 u2 sp_vol p h rho sos        (IdealGasState + rho, sos; per stage)
+//   aux[c][16]   mu_total loc_total a_sgs fgr symh_fax symp_fax - dudx[9]
+//                (AuxGasProp's lagged coefficients, per step; the velocity
+//                gradient, per stage)
+// Data a cell only reads for itself (rhoE, geometry, RK registers) stays
+// structure-of-arrays so a wavefront reads it coalesced.
 #pragma once
 #include <cstdint>
 
@@ -6,28 +15,45 @@ namespace cgdev {
 
 constexpr int kSlice = 64;   // cells per SELL slice == one wavefront
 constexpr int kBlock = 256;  // threads per block (4 slices)
+constexpr int kXcd = 8;      // XCDs a launch is spread over (round-robin by block id)
+
+// light record fields
+enum { L_U0 = 0, L_U1, L_U2, L_SPV, L_P, L_H, L_RHO, L_SOS, L_N = 8 };
+// aux record fields
+enum { A_MU = 0, A_LOC, A_ASGS, A_FGR, A_SYMH, A_SYMP, A_PAD, A_D = 7, A_N = 16 };
 
 // Mesh / operator arrays, all device pointers.
+//
+// Each cell's half-faces (its extended pairs, in pair order: the compact ones
+// first) sit in SELL-64 layout: slot(k, lane) = ptr[slice] + k*64 + lane for
+// k < deg[cell], so lane-consecutive cells read consecutive slots. The pair
+// geometry is stored per slot (both cells of a pair hold a copy), which keeps
+// every geometry read coalesced.
 struct Geom {
   int ncv, nfa, nef;
   const double *inv_vol, *vol, *sa_to_vol, *dcmag, *dx13, *dx23;
-  // extended half-faces in SELL-64 layout: slot(k, lane) = ef_ptr[slice] + k*64 + lane,
-  // k < ef_deg[cell]; ef_code = pair index | (cell is the pair's hi side) << 31
-  const int *ef_ptr, *ef_deg, *ef_nbr, *ef_code;
-  // pair geometry used by the flux (grouping applied)
-  const double *pn0, *pn1, *pn2, *pc0, *pc1, *pc2;
-  // compact geometry of the first nfa pairs (grouping applied) and raw normals
-  const double *un0, *un1, *un2, *area, *aod;
-  const double *nf0, *nf1, *nf2;
+  const int *ef_ptr, *ef_deg, *ef_nc;  // per slice / cell: slot base, half-faces, compact half-faces
+  const int* ef_nbr;                   // neighbour | (cell is the pair's hi side) << 31
+  const double *sn0, *sn1, *sn2, *sc0, *sc1, *sc2;  // pair n and c (grouping applied) per slot
+  // compact half-faces (k < ef_nc) in their own SELL-64 layout
+  const int* cf_ptr;
+  const double *cu0, *cu1, *cu2, *carea, *caod;     // unit normal, area, area/delta (grouping applied)
+  const double *cn0, *cn1, *cn2;                    // raw area vector (CFL)
+  // per-block halo for LDS staging (null when staging is off): the sorted
+  // cells block b's half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
+  // ef_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15
+  const int *halo_ptr, *halo_v;
+  const int16_t* ef_loc;
+  int halo_max;
   // gradient stencil without the diagonal, SELL-64 layout
   const int *gr_ptr, *gr_deg, *gr_nbr;
   const double *gc0, *gc1, *gc2;
 };
 
-// Conserved state and the per-stage primitives derived from it.
+// One buffer of the ping-pong state.
 struct State {
-  double *rho, *u0, *u1, *u2, *rhoE;
-  double *spv, *p, *h, *sos;
+  double* light;  // [ncv][8]
+  double* rhoE;   // [ncv]
 };
 
 struct StepInfo {
@@ -39,14 +65,13 @@ struct StepInfo {
 };
 
 struct Work {
-  double* dudx;      // [9][ncv], component i*3+j = du_i/dx_j
-  // lagged coefficients of AuxGasProp, refreshed once per step
-  double *mu_total, *loc_total, *a_sgs_c, *fgr, *symh, *symp;
-  // per-step fields
-  double *T, *logp, *invRT;
-  double *mu_lam, *mu_sgs, *loc_lam, *loc_sgs, *a_sgs, *vv2, *cfl;
+  double* aux;       // [ncv][16]
+  double* logp;      // [ncv] log(p/p_ref) of the end-of-step state
+  // per-step fields (diagnostic copies)
+  double *T, *mu_lam, *mu_sgs, *loc_lam, *loc_sgs, *a_sgs, *vv2, *cfl;
   double* rhs0;      // [5][ncv]
   double* rhs1;      // [5][ncv]
+  double* racc;      // [5][ncv] compact-face part of the stage RHS
   StepInfo* info;
   double* partial;   // reduction scratch [nblocks][8]
   double* diag;      // [8]
@@ -66,6 +91,9 @@ struct Params {
 namespace cgk {
 int launch_flux(int stage, int mode, const cgdev::Geom& g, const cgdev::State& in, const cgdev::State& out,
                 const cgdev::Work& w, const cgdev::Params& P, double* rhs_out, void* stream);
+int launch_flux_compact(const cgdev::Geom& g, const cgdev::State& in, const cgdev::Work& w, void* stream);
+int launch_flux_ext(int stage, int mode, const cgdev::Geom& g, const cgdev::State& in, const cgdev::State& out,
+                    const cgdev::Work& w, const cgdev::Params& P, double* rhs_out, void* stream);
 int launch_grad(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, void* stream);
 int launch_eos(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, const cgdev::Params& P,
                void* stream);
@@ -75,4 +103,6 @@ int launch_begin_step(const cgdev::Work& w, const cgdev::Params& P, void* stream
 int launch_cfl(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, double dt, void* stream);
 int launch_diag(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, int nblocks, void* stream);
 int launch_fill(double* p, double v, int64_t n, void* stream);
+// gather a light / aux field into a SoA array
+int launch_unpack(const double* rec, int stride, int field, int64_t n, double* out, void* stream);
 }  // namespace cgk

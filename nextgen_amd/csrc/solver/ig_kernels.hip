@@ -9,6 +9,10 @@
 // are no atomics in the RHS, so the result is deterministic and equals a
 // sequential face loop over the same pair order.
 //
+// Blocks are numbered so that each XCD (workgroups are dealt round-robin over
+// the 8 XCDs) works on one contiguous stretch of the Morton curve: a spatial
+// sub-box whose neighbour records stay in that XCD's L2.
+//
 // This file is compiled with -ffp-contract=off: every product and sum is
 // rounded on its own, in the order of the reference expressions.
 #include <hip/hip_runtime.h>
@@ -22,48 +26,66 @@ namespace {
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 
-struct Light {  // IdealGasState
-  double u[3], spv, p, h;
-};
-struct Aux {    // AuxGasProp
-  double mu_total, loc_total, half_usq, rho, a_sgs, sos, fgr, symh, symp;
-};
-
-__device__ __forceinline__ Light load_light(const State& s, int c) {
-  Light L;
-  L.u[0] = s.u0[c]; L.u[1] = s.u1[c]; L.u[2] = s.u2[c];
-  L.spv = s.spv[c]; L.p = s.p[c]; L.h = s.h[c];
-  return L;
+// logical block index: XCD x gets blocks [x*q + min(x,r), ...) of the Morton order
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, w = blockIdx.x;
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int x = w % kXcd, j = w / kXcd;
+  return x * q + (x < r ? x : r) + j;
 }
 
-__device__ __forceinline__ Aux load_aux(const State& s, const Work& w, const Light& L, int c) {
-  Aux a;
+// streamed once per launch: keep them from evicting reusable neighbour records
+template <class T>
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+
+struct Light {  // IdealGasState (+ rho, sos)
+  double u[3], spv, p, h, rho, sos;
+};
+
+__device__ __forceinline__ Light load_light(const double* __restrict__ L, int c) {
+  const double2* q = reinterpret_cast<const double2*>(L + (size_t)c * L_N);
+  const double2 a = q[0], b = q[1], d = q[2], e = q[3];
+  Light r;
+  r.u[0] = a.x; r.u[1] = a.y; r.u[2] = b.x; r.spv = b.y; r.p = d.x; r.h = d.y; r.rho = e.x; r.sos = e.y;
+  return r;
+}
+
+__device__ __forceinline__ void store_light(double* __restrict__ L, int c, const Light& r) {
+  double2* q = reinterpret_cast<double2*>(L + (size_t)c * L_N);
+  q[0] = make_double2(r.u[0], r.u[1]);
+  q[1] = make_double2(r.u[2], r.spv);
+  q[2] = make_double2(r.p, r.h);
+  q[3] = make_double2(r.rho, r.sos);
+}
+
+struct Aux {  // AuxGasProp lagged part + velocity gradient
+  double mu_total, loc_total, a_sgs, fgr, symh, symp;
+  double g[9];
+};
+
+__device__ __forceinline__ Aux load_aux(const double* __restrict__ A, int c) {
+  const double2* q = reinterpret_cast<const double2*>(A + (size_t)c * A_N);
+  const double2 a = q[0], b = q[1], d = q[2], e = q[3], f = q[4], h = q[5], i = q[6], j = q[7];
+  Aux r;
+  r.mu_total = a.x; r.loc_total = a.y; r.a_sgs = b.x; r.fgr = b.y; r.symh = d.x; r.symp = d.y;
+  r.g[0] = e.y; r.g[1] = f.x; r.g[2] = f.y; r.g[3] = h.x; r.g[4] = h.y; r.g[5] = i.x; r.g[6] = i.y;
+  r.g[7] = j.x; r.g[8] = j.y;
+  return r;
+}
+
+__device__ __forceinline__ double half_usq(const Light& L) {
   double usq = 0.0;
   usq += L.u[0] * L.u[0];
   usq += L.u[1] * L.u[1];
   usq += L.u[2] * L.u[2];
-  a.half_usq = 0.5 * usq;
-  a.rho = s.rho[c];
-  a.sos = s.sos[c];
-  a.mu_total = w.mu_total[c];
-  a.loc_total = w.loc_total[c];
-  a.a_sgs = w.a_sgs_c[c];
-  a.fgr = w.fgr[c];
-  a.symh = w.symh[c];
-  a.symp = w.symp[c];
-  return a;
-}
-
-__device__ __forceinline__ void load_grad(const Work& w, int ncv, int c, double g[9]) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) g[k] = w.dudx[(size_t)k * ncv + c];
+  return 0.5 * usq;
 }
 
 // calcInternalFluxNew, IdealGasSolverFlux.hpp:9-39
 __device__ __forceinline__ void flux_skew(double F[5], const double n[3], const Light& a, const Light& b,
-                                          double u_avg[3], double& inv_v_avg, double& h_stag, double& p_avg) {
+                                          double u_avg[3], double& inv_v_avg, double& h_stag) {
   inv_v_avg = 2.0 / (a.spv + b.spv);
-  p_avg = 0.5 * (b.p + a.p);
+  const double p_avg = 0.5 * (b.p + a.p);
   double un_avg = 0.0, u0u1 = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -111,14 +133,13 @@ __device__ __forceinline__ void flux_symmetric(double cF[5], const double c[3], 
 // calcInternalCompactFluxNewGrad, IdealGasSolverFlux.hpp:93-215
 __device__ __forceinline__ void flux_compact(double F[5], const double un[3], double area, double aod,
                                              const Light& a, const Light& b, const Aux& A0, const Aux& A1,
-                                             const double u_avg[3], double h_stag, const double g0[9],
-                                             const double g1[9]) {
+                                             const double u_avg[3], double h_stag) {
   const double aod_half = 0.5 * aod;
   const double delta = area / aod;
   const double mu_tmp = 0.5 * (A0.mu_total + A1.mu_total) * area;
   const double mu_total_c = (A0.mu_total + A1.mu_total) * aod_half;
   const double loc_coeff = (A0.loc_total + A1.loc_total) * aod_half;
-  const double dhalf_usq = A1.half_usq - A0.half_usq;
+  const double dhalf_usq = half_usq(b) - half_usq(a);
   const double dh = b.h - a.h;
   double u0_cn = 0.0, u1_cn = 0.0;
 #pragma unroll
@@ -131,11 +152,11 @@ __device__ __forceinline__ void flux_compact(double F[5], const double un[3], do
   const double one_sixth_dun2 = (u1_cn * u1_cn - u0_cn * u0_cn) / 6.0;
   const double dp = b.p - a.p;
   const double fgr_avg = 0.5 * (A1.fgr + A0.fgr);
-  const double rho_avg = 0.5 * (A1.rho + A0.rho);
+  const double rho_avg = 0.5 * (b.rho + a.rho);
   const double un_avg = 0.5 * (u0_cn + u1_cn);
-  const double sos_avg = 0.5 * (A0.sos + A1.sos);
+  const double sos_avg = 0.5 * (a.sos + b.sos);
   const double Ma_n = fabs(un_avg) / sos_avg;
-  const double drho = A1.rho - A0.rho;
+  const double drho = b.rho - a.rho;
   const double lf_coeff = smin(fgr_avg * Ma_n, 0.5) * (sos_avg + fabs(un_avg)) * area;
   const double lfdrho = lf_coeff * drho;
   const double lfrhoa = lf_coeff * rho_avg;
@@ -162,6 +183,8 @@ __device__ __forceinline__ void flux_compact(double F[5], const double un[3], do
   F[4] -= lfdrho * (h_stag - pv_avg);
   F[4] -= lfrhoa * de;
   F[4] -= lfrhoa * un_avg * dun;
+  const double* g0 = A0.g;
+  const double* g1 = A1.g;
   double dundx[3];
   double one_third_dundxn = 0.0, two_third_Skk = 0.0;
 #pragma unroll
@@ -181,83 +204,29 @@ __device__ __forceinline__ void flux_compact(double F[5], const double un[3], do
   }
 }
 
-// EOS recovery of updateExtendedState (IdealGasSolver.cpp:999-1024) for one cell.
-__device__ __forceinline__ void eos_store(const State& o, const Params& P, int c, double rho, const double u[3],
-                                          double rhoE, double& T_out) {
+// EOS recovery of updateExtendedState (IdealGasSolver.cpp:999-1024).
+__device__ __forceinline__ Light eos(const Params& P, double rho, const double u[3], double rhoE, double& T) {
   double usq = 0.0;
   usq += u[0] * u[0];
   usq += u[1] * u[1];
   usq += u[2] * u[2];
-  const double p = P.gm1 * (rhoE - 0.5 * rho * usq);
-  const double T = p / (rho * P.R_gas);
-  const double sos = sqrt(P.gamma * P.R_gas * T);
-  o.p[c] = p;
-  o.sos[c] = sos;
-  o.spv[c] = 1.0 / rho;
-  o.h[c] = P.Rgogm1 * T;
-  T_out = T;
+  Light L;
+  L.p = P.gm1 * (rhoE - 0.5 * rho * usq);
+  T = L.p / (rho * P.R_gas);
+  L.sos = sqrt(P.gamma * P.R_gas * T);
+  L.u[0] = u[0]; L.u[1] = u[1]; L.u[2] = u[2];
+  L.spv = 1.0 / rho;
+  L.h = P.Rgogm1 * T;
+  L.rho = rho;
+  return L;
 }
 
-// Stage flux gather. MODE 0: write the RHS only (parity hook). MODE 1: fused
-// RK3 update (rk3Step, IdealGasSolver.cpp:852-868) and EOS recovery into `out`.
-template <int STAGE, int MODE>
-__global__ __launch_bounds__(kBlock) void k_flux(Geom g, State in, State out, Work w, Params P, double* rhs_out) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= g.ncv) return;
-  if (MODE == 1 && w.info->nan) return;
-  const int lane = c & (kSlice - 1);
-  const int base = g.ef_ptr[c >> 6] + lane;
-  const int deg = g.ef_deg[c];
-  const int ncv = g.ncv, nfa = g.nfa;
-
-  const Light self = load_light(in, c);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < deg; ++k) {
-    const int slot = base + k * kSlice;
-    const int nb = g.ef_nbr[slot];
-    const int code = g.ef_code[slot];
-    const int pidx = code & 0x7fffffff;
-    const bool self_hi = code < 0;
-    const Light other = load_light(in, nb);
-    const Light& a = self_hi ? other : self;
-    const Light& b = self_hi ? self : other;
-    const int ca = self_hi ? nb : c;
-    const int cb = self_hi ? c : nb;
-    const double n[3] = {g.pn0[pidx], g.pn1[pidx], g.pn2[pidx]};
-    const double cv[3] = {g.pc0[pidx], g.pc1[pidx], g.pc2[pidx]};
-    double F[5], u_avg[3], inv_v_avg, h_stag, p_avg;
-    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag, p_avg);
-    if (pidx < nfa) {
-      const double un[3] = {g.un0[pidx], g.un1[pidx], g.un2[pidx]};
-      const Aux A0 = load_aux(in, w, a, ca);
-      const Aux A1 = load_aux(in, w, b, cb);
-      double g0[9], g1[9];
-      load_grad(w, ncv, ca, g0);
-      load_grad(w, ncv, cb, g1);
-      flux_compact(F, un, g.area[pidx], g.aod[pidx], a, b, A0, A1, u_avg, h_stag, g0, g1);
-    }
-    if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
-      double cF[5];
-      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) F[q] += cF[q];
-    }
-    if (self_hi) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] += F[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
-    }
-  }
-
-  if (MODE == 0) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) rhs_out[(size_t)q * ncv + c] = acc[q];
-    return;
-  }
-
-  // rk3Step with RkWeights::erk_wgt3[STAGE-1]; registers of later stages are zero
+// rk3Step (IdealGasSolver.cpp:852-868) with RkWeights::erk_wgt3[STAGE-1] and
+// the EOS recovery of the updated cell; RK registers of later stages are zero.
+template <int STAGE>
+__device__ __forceinline__ void rk_update(const Geom& g, const State& in, const State& out, const Work& w,
+                                          const Params& P, int c, const Light& self, const double acc[5]) {
+  const int ncv = g.ncv;
   const double dt = w.info->dt;
   const double tmp = dt * g.inv_vol[c];
   double agg[5];
@@ -282,65 +251,230 @@ __global__ __launch_bounds__(kBlock) void k_flux(Geom g, State in, State out, Wo
       agg[q] = tmp * ((-1.0 / 12.0) * r0 + (-1.0 / 12.0) * r1 + (8.0 / 12.0) * acc[q]);
     }
   }
-  const double rho_old = in.rho[c];
+  const double rho_old = self.rho;
   const double rho = rho_old + agg[0];
   const double inv_rho = 1.0 / rho;
   double u[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) u[i] = (rho_old * self.u[i] + agg[1 + i]) * inv_rho;
   const double rhoE = in.rhoE[c] + agg[4];
-  out.rho[c] = rho;
-  out.u0[c] = u[0];
-  out.u1[c] = u[1];
-  out.u2[c] = u[2];
-  out.rhoE[c] = rhoE;
   double T;
-  eos_store(out, P, c, rho, u, rhoE, T);
-  if (STAGE == 3) {
-    w.T[c] = T;
-    w.logp[c] = log(out.p[c] * P.inv_pref);
-    w.invRT[c] = 1.0 / (P.R_gas * T);
+  const Light L = eos(P, rho, u, rhoE, T);
+  store_light(out.light, c, L);
+  out.rhoE[c] = rhoE;
+  if (STAGE == 3) w.logp[c] = log(L.p * P.inv_pref);
+}
+
+// Stage flux gather, part 1: the compact half-faces (each cell's first ef_nc
+// slots: skew + compact closure + symmetric), accumulated in pair order into
+// w.racc. The compact closure needs ~190 VGPRs, so it runs as its own kernel
+// and the bulk of the half-faces below keeps a higher occupancy.
+__global__ __launch_bounds__(kBlock) void k_flux_compact(Geom g, State in, Work w) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= g.ncv) return;
+  if (w.info->nan) return;
+  const int lane = c & (kSlice - 1);
+  const int base = g.ef_ptr[c >> 6] + lane;
+  const int cbase = g.cf_ptr[c >> 6] + lane;
+  const int nc = g.ef_nc[c];
+  const int ncv = g.ncv;
+  const Light self = load_light(in.light, c);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < nc; ++k) {
+    const int slot = base + k * kSlice;
+    const int cslot = cbase + k * kSlice;
+    const int code = ldnt(g.ef_nbr + slot);
+    const int nb = code & 0x7fffffff;
+    const bool self_hi = code < 0;
+    const Light other = load_light(in.light, nb);
+    const Aux As = load_aux(w.aux, c);
+    const Aux An = load_aux(w.aux, nb);
+    const Light& a = self_hi ? other : self;
+    const Light& b = self_hi ? self : other;
+    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
+    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
+    const double un[3] = {ldnt(g.cu0 + cslot), ldnt(g.cu1 + cslot), ldnt(g.cu2 + cslot)};
+    double F[5], u_avg[3], inv_v_avg, h_stag;
+    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
+    flux_compact(F, un, ldnt(g.carea + cslot), ldnt(g.caod + cslot), a, b, self_hi ? An : As, self_hi ? As : An, u_avg, h_stag);
+    if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
+      double cF[5];
+      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] += cF[q];
+    }
+    if (self_hi) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] += F[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+    }
   }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) w.racc[(size_t)q * ncv + c] = acc[q];
+}
+
+// Stage flux gather, part 2: the purely extended half-faces (slots ef_nc..),
+// continuing the accumulation of part 1. MODE 0: write the RHS only (parity
+// hook). MODE 1: fused RK3 update (rk3Step, IdealGasSolver.cpp:852-868) and
+// EOS recovery into `out`.
+template <int STAGE, int MODE>
+__global__ __launch_bounds__(kBlock) void k_flux(Geom g, State in, State out, Work w, Params P, double* rhs_out) {
+  const int c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= g.ncv) return;
+  if (MODE == 1 && w.info->nan) return;
+  const int lane = c & (kSlice - 1);
+  const int base = g.ef_ptr[c >> 6] + lane;
+  const int deg = g.ef_deg[c];
+  const int nc = g.ef_nc[c];
+  const int ncv = g.ncv;
+
+  const Light self = load_light(in.light, c);
+  double acc[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) acc[q] = w.racc[(size_t)q * ncv + c];
+  for (int k = nc; k < deg; ++k) {
+    const int slot = base + k * kSlice;
+    const int code = ldnt(g.ef_nbr + slot);
+    const int nb = code & 0x7fffffff;
+    const bool self_hi = code < 0;
+    const Light other = load_light(in.light, nb);
+    const Light& a = self_hi ? other : self;
+    const Light& b = self_hi ? self : other;
+    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
+    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
+    double F[5], u_avg[3], inv_v_avg, h_stag;
+    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
+    if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
+      double cF[5];
+      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] += cF[q];
+    }
+    if (self_hi) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] += F[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+    }
+  }
+
+  if (MODE == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rhs_out[(size_t)q * ncv + c] = acc[q];
+    return;
+  }
+
+  rk_update<STAGE>(g, in, out, w, P, c, self, acc);
+}
+
+// The same extended-face pass with the block's neighbour records staged in
+// LDS: the block first copies the first 48 bytes (u, sp_vol, p, h) of the
+// light record of every cell its half-faces reach (its halo, ~4x its own
+// cells) into LDS, then every gather is an LDS read. ef_loc holds the halo
+// position of each slot's neighbour (bit 15: the cell is the pair's hi side).
+template <int STAGE, int MODE>
+__global__ __launch_bounds__(kBlock) void k_flux_lds(Geom g, State in, State out, Work w, Params P, double* rhs_out) {
+  extern __shared__ double2 halo[];  // [hsize][3]
+  const int blk = xcd_block();
+  if (MODE == 1 && w.info->nan) return;
+  const int hb = g.halo_ptr[blk], hs = g.halo_ptr[blk + 1] - hb;
+  for (int i = threadIdx.x; i < hs; i += kBlock) {
+    const int gc = g.halo_v[hb + i];
+    const double2* q = reinterpret_cast<const double2*>(in.light + (size_t)gc * L_N);
+    halo[3 * i + 0] = q[0];
+    halo[3 * i + 1] = q[1];
+    halo[3 * i + 2] = q[2];
+  }
+  __syncthreads();
+  const int c = blk * kBlock + threadIdx.x;
+  if (c >= g.ncv) return;
+  const int lane = c & (kSlice - 1);
+  const int base = g.ef_ptr[c >> 6] + lane;
+  const int deg = g.ef_deg[c];
+  const int nc = g.ef_nc[c];
+  const int ncv = g.ncv;
+  const Light self = load_light(in.light, c);
+  double acc[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) acc[q] = w.racc[(size_t)q * ncv + c];
+  for (int k = nc; k < deg; ++k) {
+    const int slot = base + k * kSlice;
+    const int code = ldnt(g.ef_loc + slot);
+    const int li = code & 0x7fff;
+    const bool self_hi = (code & 0x8000) != 0;
+    const double2 r0 = halo[3 * li + 0], r1 = halo[3 * li + 1], r2 = halo[3 * li + 2];
+    Light other;
+    other.u[0] = r0.x; other.u[1] = r0.y; other.u[2] = r1.x; other.spv = r1.y; other.p = r2.x; other.h = r2.y;
+    other.rho = 0.0; other.sos = 0.0;
+    const Light& a = self_hi ? other : self;
+    const Light& b = self_hi ? self : other;
+    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
+    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
+    double F[5], u_avg[3], inv_v_avg, h_stag;
+    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
+    if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
+      double cF[5];
+      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q] += cF[q];
+    }
+    if (self_hi) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] += F[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+    }
+  }
+  if (MODE == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rhs_out[(size_t)q * ncv + c] = acc[q];
+    return;
+  }
+  rk_update<STAGE>(g, in, out, w, P, c, self, acc);
 }
 
 // _calcCvGradRange, StaticSolver.cpp:5535-5557
 __global__ __launch_bounds__(kBlock) void k_grad(Geom g, State st, Work w) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_block() * kBlock + threadIdx.x;
   if (c == 0) w.info->cfl_bits = 0ull;  // reset for the properties pass that follows
   if (c >= g.ncv) return;
   if (w.info->nan) return;
-  const int ncv = g.ncv;
   const int base = g.gr_ptr[c >> 6] + (c & (kSlice - 1));
   const int deg = g.gr_deg[c];
-  const double ui[3] = {st.u0[c], st.u1[c], st.u2[c]};
+  const double* L = st.light;
+  const double ui[3] = {L[(size_t)c * L_N + L_U0], L[(size_t)c * L_N + L_U1], L[(size_t)c * L_N + L_U2]};
   double d[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) d[k] = 0.0;
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
-    const int nb = g.gr_nbr[slot];
-    const double gc[3] = {g.gc0[slot], g.gc1[slot], g.gc2[slot]};
-    const double un[3] = {st.u0[nb], st.u1[nb], st.u2[nb]};
+    const int nb = ldnt(g.gr_nbr + slot);
+    const double gc[3] = {ldnt(g.gc0 + slot), ldnt(g.gc1 + slot), ldnt(g.gc2 + slot)};
+    const double un[3] = {L[(size_t)nb * L_N + L_U0], L[(size_t)nb * L_N + L_U1], L[(size_t)nb * L_N + L_U2]};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) d[i * 3 + j] += gc[j] * (un[i] - ui[i]);
   }
+  double* a = w.aux + (size_t)c * A_N + A_D;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) w.dudx[(size_t)k * ncv + c] = d[k];
+  for (int k = 0; k < 9; ++k) a[k] = d[k];
 }
 
-// EOS of the current state (updateExtendedState for all cells), with the
-// per-step sensor inputs.
+// EOS of the current state (updateExtendedState for all cells) from the
+// conserved fields rho, u (in the light record) and rhoE.
 __global__ __launch_bounds__(kBlock) void k_eos(Geom g, State st, Work w, Params P) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
-  const double u[3] = {st.u0[c], st.u1[c], st.u2[c]};
+  const Light in = load_light(st.light, c);
   double T;
-  eos_store(st, P, c, st.rho[c], u, st.rhoE[c], T);
-  w.T[c] = T;
-  w.logp[c] = log(st.p[c] * P.inv_pref);
-  w.invRT[c] = 1.0 / (P.R_gas * T);
+  const Light L = eos(P, in.rho, in.u, st.rhoE[c], T);
+  store_light(st.light, c, L);
+  w.logp[c] = log(L.p * P.inv_pref);
 }
 
 // MiscUtils.hpp:273-276
@@ -355,20 +489,20 @@ __device__ __forceinline__ double fast_pow_posx(double x, double p) {
 // convective CFL of the next setDt (IdealGasSolver.cpp:1048-1095, dt = 1).
 template <int FIELDS>
 __global__ __launch_bounds__(kBlock) void k_props(Geom g, State st, Work w, Params P) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_block() * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
-  const int ncv = g.ncv, nfa = g.nfa;
-  const double rho = st.rho[c];
-  const double u[3] = {st.u0[c], st.u1[c], st.u2[c]};
-  const double sos = st.sos[c];
-  const double p = st.p[c];
-  const double T = w.T[c];
+  const Light self = load_light(st.light, c);
+  const double rho = self.rho, p = self.p, sos = self.sos;
+  const double* u = self.u;
+  const double T = p / (rho * P.R_gas);
+  double* auxc = w.aux + (size_t)c * A_N;
 
   // ---- SGS ----
   double mu_sgs = 0.0, a_sgs = 0.0, loc_sgs = 0.0;
   if (P.sgs_model == 1) {
     double d[9];
-    load_grad(w, ncv, c, d);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d[k] = auxc[A_D + k];
     const double dx = g.dx13[c];
     const double dx2 = dx * dx;
     const double alpha11 = d[0], alpha11_2 = alpha11 * alpha11;
@@ -424,31 +558,34 @@ __global__ __launch_bounds__(kBlock) void k_props(Geom g, State st, Work w, Para
 
   // ---- Gibbs-remainder sensor and convective CFL over the extended half-faces ----
   const int base = g.ef_ptr[c >> 6] + (c & (kSlice - 1));
+  const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
   const int deg = g.ef_deg[c];
-  const double lp_s = w.logp[c], spv_s = st.spv[c], irt_s = w.invRT[c];
+  const int nc = g.ef_nc[c];
+  const double lp_s = w.logp[c];
+  const double irt_s = 1.0 / (P.R_gas * T);
   double vv2 = 0.0, cfl = 0.0;
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
-    const int nb = g.ef_nbr[slot];
-    const int code = g.ef_code[slot];
-    const int pidx = code & 0x7fffffff;
+    const int code = ldnt(g.ef_nbr + slot);
+    const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
-    const int c0 = self_hi ? nb : c;
-    const int c1 = self_hi ? c : nb;
-    const double p0 = self_hi ? st.p[nb] : p;
-    const double p1 = self_hi ? p : st.p[nb];
-    const double dp = p1 - p0;
-    const double v_avg = 0.5 * ((self_hi ? st.spv[nb] : spv_s) + (self_hi ? spv_s : st.spv[nb]));
-    const double beta_avg = 0.5 * ((self_hi ? w.invRT[nb] : irt_s) + (self_hi ? irt_s : w.invRT[nb]));
-    const double gr = (self_hi ? lp_s : w.logp[nb]) - (self_hi ? w.logp[nb] : lp_s) - v_avg * dp * beta_avg;
+    const Light o = load_light(st.light, nb);
+    const double lp_o = w.logp[nb];
+    const double irt_o = 1.0 / (P.R_gas * (o.p / (o.rho * P.R_gas)));
+    const Light& a = self_hi ? o : self;
+    const Light& b = self_hi ? self : o;
+    const double dp = b.p - a.p;
+    const double v_avg = 0.5 * (a.spv + b.spv);
+    const double beta_avg = 0.5 * ((self_hi ? irt_o : irt_s) + (self_hi ? irt_s : irt_o));
+    const double gr = (self_hi ? lp_s : lp_o) - (self_hi ? lp_o : lp_s) - v_avg * dp * beta_avg;
     vv2 = smax(vv2, fabs(gr));
-    if (pidx < nfa) {
-      const double n[3] = {g.nf0[pidx], g.nf1[pidx], g.nf2[pidx]};
+    if (k < nc) {
+      const int cslot = cbase + k * kSlice;
+      const double n[3] = {ldnt(g.cn0 + cslot), ldnt(g.cn1 + cslot), ldnt(g.cn2 + cslot)};
       const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      const double ua[3] = {st.u0[c0], st.u1[c0], st.u2[c0]};
-      const double ub[3] = {st.u0[c1], st.u1[c1], st.u2[c1]};
-      const double undA = 0.5 * ((ua[0] * n[0] + ua[1] * n[1] + ua[2] * n[2]) + (ub[0] * n[0] + ub[1] * n[1] + ub[2] * n[2]));
-      const double sos_avg = 0.5 * (st.sos[c0] + st.sos[c1]);
+      const double undA = 0.5 * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
+                                 (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
+      const double sos_avg = 0.5 * (a.sos + b.sos);
       cfl += fabs(undA) + area * sos_avg;
     }
   }
@@ -458,13 +595,14 @@ __global__ __launch_bounds__(kBlock) void k_props(Geom g, State st, Work w, Para
   else fgr = 0.5 / (1.0 + exp(-2.0 * (vv2 - P.gr_nrm) / 5.0e-03));
   cfl *= 0.5 * 1.0 * g.inv_vol[c];
 
-  w.mu_total[c] = mu_lam + mu_sgs;
-  w.loc_total[c] = loc_lam + loc_sgs;
-  w.a_sgs_c[c] = a_sgs;
-  w.fgr[c] = fgr;
-  w.symh[c] = symh;
-  w.symp[c] = symp;
+  auxc[A_MU] = mu_lam + mu_sgs;
+  auxc[A_LOC] = loc_lam + loc_sgs;
+  auxc[A_ASGS] = a_sgs;
+  auxc[A_FGR] = fgr;
+  auxc[A_SYMH] = symh;
+  auxc[A_SYMP] = symp;
   if (FIELDS) {
+    w.T[c] = T;
     w.mu_lam[c] = mu_lam;
     w.mu_sgs[c] = mu_sgs;
     w.loc_lam[c] = loc_lam;
@@ -502,23 +640,23 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, State st, Work w, double
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
   const int base = g.ef_ptr[c >> 6] + (c & (kSlice - 1));
-  const int deg = g.ef_deg[c];
+  const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
+  const int nc = g.ef_nc[c];
+  const Light self = load_light(st.light, c);
   double cfl = 0.0;
-  for (int k = 0; k < deg; ++k) {
-    const int slot = base + k * kSlice;
-    const int code = g.ef_code[slot];
-    const int pidx = code & 0x7fffffff;
-    if (pidx >= g.nfa) continue;
-    const int nb = g.ef_nbr[slot];
+  for (int k = 0; k < nc; ++k) {
+    const int code = ldnt(g.ef_nbr + base + k * kSlice);
+    const int cslot = cbase + k * kSlice;
+    const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
-    const int c0 = self_hi ? nb : c;
-    const int c1 = self_hi ? c : nb;
-    const double n[3] = {g.nf0[pidx], g.nf1[pidx], g.nf2[pidx]};
+    const Light o = load_light(st.light, nb);
+    const Light& a = self_hi ? o : self;
+    const Light& b = self_hi ? self : o;
+    const double n[3] = {ldnt(g.cn0 + cslot), ldnt(g.cn1 + cslot), ldnt(g.cn2 + cslot)};
     const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    const double ua[3] = {st.u0[c0], st.u1[c0], st.u2[c0]};
-    const double ub[3] = {st.u0[c1], st.u1[c1], st.u2[c1]};
-    const double undA = 0.5 * ((ua[0] * n[0] + ua[1] * n[1] + ua[2] * n[2]) + (ub[0] * n[0] + ub[1] * n[1] + ub[2] * n[2]));
-    const double sos_avg = 0.5 * (st.sos[c0] + st.sos[c1]);
+    const double undA = 0.5 * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
+                               (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
+    const double sos_avg = 0.5 * (a.sos + b.sos);
     cfl += fabs(undA) + area * sos_avg;
   }
   cfl *= 0.5 * dt * g.inv_vol[c];
@@ -532,11 +670,10 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(Geom g, State st, Work 
   const int c = blockIdx.x * kBlock + threadIdx.x;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (c < g.ncv) {
-    const int ncv = g.ncv;
-    const double vol = g.vol[c], rho = st.rho[c];
-    const double u[3] = {st.u0[c], st.u1[c], st.u2[c]};
-    double d[9];
-    load_grad(w, ncv, c, d);
+    const Light L = load_light(st.light, c);
+    const double* d = w.aux + (size_t)c * A_N + A_D;
+    const double vol = g.vol[c], rho = L.rho;
+    const double* u = L.u;
     const double w0 = d[2 * 3 + 1] - d[1 * 3 + 2];
     const double w1 = d[0 * 3 + 2] - d[2 * 3 + 0];
     const double w2 = d[1 * 3 + 0] - d[0 * 3 + 1];
@@ -584,6 +721,11 @@ __global__ void k_fill(double* p, double v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_unpack(const double* rec, int stride, int field, int64_t n, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = rec[i * stride + field];
+}
+
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
 inline int check() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
@@ -593,8 +735,27 @@ namespace cgk {
 
 int launch_flux(int stage, int mode, const Geom& g, const State& in, const State& out, const Work& w,
                 const Params& P, double* rhs_out, void* stream) {
+  if (launch_flux_compact(g, in, w, stream)) return -1;
+  return launch_flux_ext(stage, mode, g, in, out, w, P, rhs_out, stream);
+}
+
+int launch_flux_compact(const Geom& g, const State& in, const Work& w, void* stream) {
+  hipLaunchKernelGGL(k_flux_compact, dim3(blocks_for(g.ncv)), dim3(kBlock), 0, (hipStream_t)stream, g, in, w);
+  return check();
+}
+
+int launch_flux_ext(int stage, int mode, const Geom& g, const State& in, const State& out, const Work& w,
+                    const Params& P, double* rhs_out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(blocks_for(g.ncv)), block(kBlock);
+  if (g.halo_ptr) {
+    const size_t lds = (size_t)g.halo_max * 3 * sizeof(double2);
+    if (mode == 0) hipLaunchKernelGGL((k_flux_lds<1, 0>), grid, block, lds, s, g, in, out, w, P, rhs_out);
+    else if (stage == 1) hipLaunchKernelGGL((k_flux_lds<1, 1>), grid, block, lds, s, g, in, out, w, P, rhs_out);
+    else if (stage == 2) hipLaunchKernelGGL((k_flux_lds<2, 1>), grid, block, lds, s, g, in, out, w, P, rhs_out);
+    else hipLaunchKernelGGL((k_flux_lds<3, 1>), grid, block, lds, s, g, in, out, w, P, rhs_out);
+    return check();
+  }
   if (mode == 0) {
     hipLaunchKernelGGL((k_flux<1, 0>), grid, block, 0, s, g, in, out, w, P, rhs_out);
   } else if (stage == 1) {
@@ -644,6 +805,13 @@ int launch_diag(const Geom& g, const State& st, const Work& w, int nblocks, void
 int launch_fill(double* p, double v, int64_t n, void* stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, v, n);
+  return check();
+}
+
+int launch_unpack(const double* rec, int stride, int field, int64_t n, double* out, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rec, stride,
+                     field, n, out);
   return check();
 }
 

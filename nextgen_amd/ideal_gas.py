@@ -317,7 +317,7 @@ class IdealGasSolver:
 
     def time_steps(self, nsteps, per_kernel=False):
         ms = C.c_double()
-        k = (C.c_double * 4)()
+        k = (C.c_double * 5)()
         self._chk(load_library().cg_time_steps(self._h, int(nsteps), C.byref(ms), k if per_kernel else None))
         return ms.value, (list(k) if per_kernel else None)
 

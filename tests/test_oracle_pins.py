@@ -1,0 +1,182 @@
+"""Pins of the oracle (CPU restatement) against the reference's own code and
+against closed forms.
+
+* Face fluxes, fast_pow_posx, the RK3 weights: bit-identical to the
+  reference's header code compiled from its sources (oracle/_ref, built by
+  oracle/Makefile where the reference checkout exists).
+* gr_func (IdealGasSolver.cpp:736-745 lives in a .cpp and is not compiled
+  here): pinned by its closed-form properties.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+D = C.POINTER(C.c_double)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def random_state(rng, n):
+    """IdealGasState records: u[3], sp_vol, p, h."""
+    s = np.empty((n, 6))
+    s[:, 0:3] = rng.normal(scale=1.0, size=(n, 3))
+    s[:, 3] = rng.uniform(0.5, 2.0, size=n)
+    s[:, 4] = rng.uniform(50.0, 90.0, size=n)
+    s[:, 5] = rng.uniform(200.0, 300.0, size=n)
+    return s
+
+
+def random_aux(rng, n):
+    """AuxGasProp records: mu_total loc_total half_usq rho a_sgs sos fgr symh symp."""
+    a = np.empty((n, 9))
+    a[:, 0] = rng.uniform(1e-4, 1e-3, n)
+    a[:, 1] = rng.uniform(1e-4, 1e-3, n)
+    a[:, 2] = rng.uniform(0.0, 1.0, n)
+    a[:, 3] = rng.uniform(0.5, 1.5, n)
+    a[:, 4] = rng.uniform(0.0, 1e-4, n)
+    a[:, 5] = rng.uniform(8.0, 12.0, n)
+    a[:, 6] = rng.uniform(0.0, 0.5, n)
+    a[:, 7] = rng.uniform(0.0, 1e-2, n)
+    a[:, 8] = rng.uniform(0.0, 1e-2, n)
+    # exercise the Ma clip min(fgr*Ma, 0.5) and the h_fax floor max(., 0) in both branches
+    a[::7, 6] = 0.5
+    a[::5, 7] = 1e3
+    return a
+
+
+needs_ref = pytest.mark.skipif(orc.ref() is None, reason="reference pin harness oracle/_ref not built")
+
+
+@needs_ref
+def test_skew_flux_bitwise_equal_to_reference():
+    rng = np.random.default_rng(1)
+    L, R = orc.lib(), orc.ref()
+    for _ in range(2000):
+        nc = rng.normal(size=6)
+        s = random_state(rng, 2)
+        f_ref, aux_ref = np.zeros(5), np.zeros(6)
+        R.ref_flux_skew(dp(f_ref), dp(nc), dp(s[0]), dp(s[1]), dp(aux_ref))
+        f, ua = np.zeros(5), np.zeros(3)
+        iv, hs, pa = C.c_double(), C.c_double(), C.c_double()
+        L.og_flux_skew(dp(f), dp(nc), dp(s[0]), dp(s[1]), dp(ua), C.cast(C.pointer(iv), D),
+                       C.cast(C.pointer(hs), D), C.cast(C.pointer(pa), D))
+        assert np.array_equal(f, f_ref)
+        assert np.array_equal(ua, aux_ref[:3])
+        assert (iv.value, hs.value, pa.value) == tuple(aux_ref[3:])
+
+
+@needs_ref
+def test_symmetric_flux_bitwise_equal_to_reference():
+    rng = np.random.default_rng(2)
+    L, R = orc.lib(), orc.ref()
+    for _ in range(2000):
+        nc = rng.normal(size=6)
+        s = random_state(rng, 2)
+        ua = rng.normal(size=3)
+        iv, hs, pa = rng.uniform(0.5, 2), rng.uniform(200, 300), rng.uniform(50, 90)
+        f_ref, f = np.zeros(5), np.zeros(5)
+        R.ref_flux_symmetric(dp(f_ref), dp(nc), dp(ua), iv, hs, pa, dp(s[0]), dp(s[1]))
+        L.og_flux_symmetric(dp(f), dp(nc), dp(ua), iv, hs, pa, dp(s[0]), dp(s[1]))
+        assert np.array_equal(f, f_ref)
+
+
+@needs_ref
+def test_compact_flux_bitwise_equal_to_reference():
+    rng = np.random.default_rng(3)
+    L, R = orc.lib(), orc.ref()
+    for _ in range(2000):
+        fc = np.empty(5)
+        un = rng.normal(size=3)
+        fc[:3] = un / np.linalg.norm(un)
+        fc[3] = rng.uniform(0.01, 1.0)
+        fc[4] = rng.uniform(0.001, 0.1)
+        s = random_state(rng, 2)
+        a = random_aux(rng, 2)
+        ua = 0.5 * (s[0, :3] + s[1, :3])
+        iv, hs = 2.0 / (s[0, 3] + s[1, 3]), rng.uniform(200, 300)
+        g0, g1 = rng.normal(size=9), rng.normal(size=9)
+        f0 = rng.normal(size=5)
+        f_ref, f = f0.copy(), f0.copy()
+        R.ref_flux_compact(dp(f_ref), dp(fc), dp(s[0]), dp(s[1]), dp(a[0]), dp(a[1]), dp(ua), iv, hs, dp(g0), dp(g1))
+        L.og_flux_compact(dp(f), dp(fc), dp(s[0]), dp(s[1]), dp(a[0]), dp(a[1]), dp(ua), iv, hs, dp(g0), dp(g1))
+        assert np.array_equal(f, f_ref)
+
+
+@needs_ref
+def test_fast_pow_and_rk_weights_equal_to_reference():
+    rng = np.random.default_rng(4)
+    L, R = orc.lib(), orc.ref()
+    for x in rng.uniform(0.2, 5.0, 500):
+        assert L.og_fast_pow_posx(x, 0.75) == R.ref_fast_pow_posx(x, 0.75)
+    w = np.zeros(9)
+    R.ref_erk_wgt3(dp(w))
+    wl = (C.c_double * 9).in_dll(L, "og_erk_wgt3")
+    assert np.array_equal(np.array(list(wl)), w)
+
+
+def test_rk_weights_are_ssp_rk3():
+    """The weights reproduce Shu-Osher SSP-RK3 in the low-storage increment form:
+    cumulative weights 1, (1/4, 1/4), (1/6, 1/6, 2/3)."""
+    wl = np.array(list((C.c_double * 9).in_dll(orc.lib(), "og_erk_wgt3"))).reshape(3, 3)
+    cum = np.cumsum(wl, axis=0)
+    assert np.allclose(cum[0], [1, 0, 0])
+    assert np.allclose(cum[1], [0.25, 0.25, 0])
+    assert np.allclose(cum[2], [1 / 6, 1 / 6, 2 / 3])
+
+
+def test_gr_func_closed_form_properties():
+    """Logistic sensor 0.5/(1+exp(-2(x-nrm)/w)), w = 5e-3: midpoint 1/4 at x = nrm,
+    limits 0 and 1/2, slope 1/(2w)*1/2*... = 50 at the midpoint, floor at x = 0."""
+    g = orc.lib().og_gr_func
+    nrm = 0.03
+    assert g(nrm, nrm) == 0.25
+    assert g(1.0, nrm) == 0.5
+    assert g(0.0, nrm) == pytest.approx(0.5 / (1.0 + math.exp(12.0)), rel=1e-15)
+    h = 1e-7
+    slope = (g(nrm + h, nrm) - g(nrm - h, nrm)) / (2 * h)
+    assert slope == pytest.approx(0.5 * 2.0 / 5.0e-3 / 4.0, rel=1e-6)
+    xs = np.linspace(0, 0.1, 101)
+    vals = [g(x, nrm) for x in xs]
+    assert all(b >= a for a, b in zip(vals, vals[1:]))
+
+
+def test_fast_pow_matches_pow():
+    L = orc.lib()
+    for x in np.linspace(0.3, 3.0, 50):
+        assert L.og_fast_pow_posx(x, 0.75) == pytest.approx(x ** 0.75, rel=1e-14)
+
+
+@needs_ref
+def test_reference_svd_regularised_inverse():
+    """calcSvd33 (SvdNew.hpp:24) with the thresholded inverse of calcCvGradCoeff
+    (StaticSolver.cpp:5713-5756) equals the numpy SVD pseudo-inverse used by the
+    dense operator restatement (oracle/operators_dense.py)."""
+    from oracle.operators_dense import regularised_update
+
+    rng = np.random.default_rng(5)
+    R = orc.ref()
+    for trial in range(300):
+        A = np.eye(3) + 0.3 * rng.normal(size=(3, 3))
+        if trial % 3 == 0:  # force a clipped direction
+            u, s, vt = np.linalg.svd(A)
+            s[2] = 0.2
+            A = u @ np.diag(s) @ vt
+        Acm = np.ascontiguousarray(A.T.reshape(9))  # column major
+        U, V, sig, err = np.zeros(9), np.zeros(9), np.zeros(3), np.zeros(1)
+        R.ref_svd33(dp(Acm), dp(U), dp(V), dp(sig), dp(err))
+        c = rng.normal(size=3)
+        # reference arithmetic of StaticSolver.cpp:5713-5756 on the reference SVD
+        sinv = np.where(sig < 0.45, 0.0, 1.0 / np.where(sig == 0, 1, sig))
+        Um = U.reshape(3, 3).T  # U[(j)*3+i] = U_ij
+        Vm = V.reshape(3, 3).T
+        b = A @ c
+        tmp = Um.T @ (c - b)
+        ref_g = c + (Vm * sinv[None, :]) @ tmp
+        ours = regularised_update(A, c[None, :], 0.45)[0]
+        assert np.allclose(ours, ref_g, rtol=1e-12, atol=1e-14), (A, sig)
