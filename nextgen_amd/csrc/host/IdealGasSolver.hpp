@@ -1,0 +1,180 @@
+// C++ facade of the MI355X ideal-gas step with the reference solver's class surface.
+//
+// Mirrors IdealGasSolver (src/charles/ideal_gas/IdealGasSolver.hpp:16-894) and the run
+// loop of FlowSolver (src/charles/FlowSolver.hpp:2445-2678): the same method names
+// (init, initComplete, advanceSolution, calcCfl, syncPostState, report, storeData,
+// restoreData) and user hooks (initialHook, temporalHook, finalHook), the public host
+// arrays rho/u/rhoE/p/T the hooks read and write, and the input-file keys. All compute
+// runs through the C ABI of include/charles_gpu.h; host arrays are synchronised only
+// around hooks and reports.
+//
+// Errors: a failing C-ABI call throws int 0 after printing the message, like the
+// reference's CERR (src/core/common/Macros.hpp:277-281).
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../../include/charles_gpu.h"
+
+namespace charles {
+
+inline void cg_check(int rc, const cg_solver* s = nullptr) {
+  if (rc != CG_OK) {
+    std::fprintf(stderr, "ERROR: %s\n", s ? cg_last_error(s) : cg_error_string());
+    throw(0);
+  }
+}
+
+class IdealGasSolver {
+ public:
+  // conserved state and derived host mirrors (IdealGasSolver.hpp:19-32)
+  std::vector<double> rho, rhoE, p, T;
+  std::vector<double> u;  // [ncv][3]
+  std::vector<double> x_cv, vol_cv;
+  int ncv = 0;
+  int step = 0;
+  double time = 0.0, dt = 0.0;
+  int check_interval = 1;
+  int nsteps = -1;
+
+  IdealGasSolver() { cg_check(cg_params_default(&params_)); }
+  virtual ~IdealGasSolver() {
+    if (solver_) cg_destroy(solver_);
+    if (mesh_) cg_mesh_destroy(mesh_);
+  }
+  IdealGasSolver(const IdealGasSolver&) = delete;
+  IdealGasSolver& operator=(const IdealGasSolver&) = delete;
+
+  // ---- input file (Params.hpp:172-182 key names) ----
+  void parseParams(const std::string& text) {
+    cg_check(cg_params_parse(&params_, text.c_str()));
+    check_interval = params_.check_interval;
+  }
+  cg_params& params() { return params_; }
+
+  // ---- mesh (StaticSolver::init: read mesh, build operators) ----
+  void initLattice(int n, double L) { cg_check(cg_mesh_lattice(n, n, n, L, L, L, &params_, &mesh_)); }
+  void initJittered(int n, double L, double jitter, uint64_t seed) {
+    cg_check(cg_mesh_jittered(n, n, n, L, L, L, jitter, seed, &params_, &mesh_));
+  }
+
+  // FlowSolver::init + IdealGasSolver::initData: allocate the device state
+  void init() {
+    int64_t cnt[10];
+    cg_check(cg_mesh_counts(mesh_, cnt));
+    ncv = int(cnt[0]);
+    x_cv.resize(3 * (size_t)ncv);
+    vol_cv.resize(ncv);
+    cg_check(cg_mesh_get(mesh_, "x_cv", x_cv.data(), int64_t(sizeof(double) * x_cv.size())));
+    cg_check(cg_mesh_get(mesh_, "vol", vol_cv.data(), int64_t(sizeof(double) * vol_cv.size())));
+    cg_check(cg_create(mesh_, &params_, &solver_));
+    rho.assign(ncv, params_.rho_ref);
+    u.assign(3 * (size_t)ncv, 0.0);
+    rhoE.assign(ncv, params_.p_ref / (params_.gamma - 1.0));
+    p.assign(ncv, params_.p_ref);
+    T.assign(ncv, params_.T_ref);
+  }
+
+  // ---- user hooks (FlowSolver.hpp:2269-2272) ----
+  virtual void initialHook() {}
+  virtual void temporalHook() {}
+  virtual void finalHook() {}
+  // set when a hook reads or writes the host arrays every step
+  bool hooks_need_state = false;
+
+  // IdealGasSolver::initComplete (IdealGasSolver.hpp:275-301)
+  void initComplete() {
+    cg_check(cg_set_state(solver_, rho.data(), u.data(), rhoE.data()), solver_);
+    cg_check(cg_init_complete(solver_), solver_);
+    pullState();
+  }
+
+  void syncPostState() { initComplete(); }
+
+  double calcCfl(double* cfl, const double dt_) const {
+    double mx = 0.0;
+    cg_check(cg_calc_cfl(solver_, dt_, &mx), solver_);
+    if (cfl) cg_check(cg_get_field(solver_, "cfl", cfl, ncv), solver_);
+    return mx;
+  }
+
+  // one iteration of the run loop: setDt + advanceSolution; 0, or -1 after a rollback
+  int advanceSolution() {
+    const int before = step;
+    cg_check(cg_step(solver_, 1), solver_);
+    cg_diag d;
+    cg_check(cg_get_diag(solver_, &d), solver_);
+    step = int(d.step);
+    time = d.time;
+    dt = d.dt;
+    return (step == before + 1) ? 0 : -1;
+  }
+
+  void storeData() { cg_check(cg_store(solver_), solver_); }
+  void restoreData() { cg_check(cg_restore(solver_), solver_); }
+
+  // IdealGasSolver::report (IdealGasSolver.hpp:805-848)
+  cg_diag report(bool print = true) {
+    cg_diag d;
+    cg_check(cg_get_diag(solver_, &d), solver_);
+    if (print)
+      std::printf(" > step: %lld time: %.9e dt: %.6e ke: %.15e enstrophy: %.15e\n"
+                  " > time: %.9e total mass, momentum, energy: %.15e %.15e %.15e %.15e %.15e\n",
+                  (long long)d.step, d.time, d.dt, d.ke, d.enstrophy, d.time, d.sums[0], d.sums[1], d.sums[2],
+                  d.sums[3], d.sums[4]);
+    return d;
+  }
+
+  // FlowSolver::run (FlowSolver.hpp:2445-2678), without I/O
+  void run() {
+    initialHook();
+    initComplete();
+    if (nsteps < 0) return;
+    int chunk = hooks_need_state ? 1 : check_interval;
+    while (step < nsteps) {
+      const int n = std::min(chunk, nsteps - step);
+      cg_check(cg_step(solver_, n), solver_);
+      cg_diag d = report(false);
+      step = int(d.step);
+      time = d.time;
+      dt = d.dt;
+      if (hooks_need_state) {
+        pullState();
+        temporalHook();
+        pushStateIfChanged();
+      } else {
+        temporalHook();
+      }
+      if (step % check_interval == 0) report(true);
+    }
+    pullState();
+    finalHook();
+  }
+
+  cg_solver* handle() { return solver_; }
+
+ protected:
+  void pullState() {
+    cg_check(cg_get_state(solver_, rho.data(), u.data(), rhoE.data()), solver_);
+    cg_check(cg_get_field(solver_, "p", p.data(), ncv), solver_);
+    cg_check(cg_get_field(solver_, "T", T.data(), ncv), solver_);
+    snap_rho_ = rho;
+    snap_u_ = u;
+    snap_rhoE_ = rhoE;
+  }
+  // a hook that changed the conserved state gets the syncPostState treatment
+  // (primitives, gradient and properties recomputed from the new state)
+  void pushStateIfChanged() {
+    if (snap_rho_ != rho || snap_u_ != u || snap_rhoE_ != rhoE) initComplete();
+  }
+
+  cg_params params_{};
+  cg_mesh* mesh_ = nullptr;
+  cg_solver* solver_ = nullptr;
+  std::vector<double> snap_rho_, snap_u_, snap_rhoE_;
+};
+
+}  // namespace charles

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <sstream>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -579,54 +580,65 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
   rc |= upload(s, &g.dx23, dx23);
   if (rc) return bail(rc);
 
-  // extended half-faces (per-slot geometry), compact ones first in each cell
+  // half-faces (per-slot geometry): compact ones and purely extended ones in
+  // two SELL-64 regions, each starting at row 0
   {
-    std::vector<int> cnt(ncv), ccnt(ncv);
+    std::vector<int> ccnt(ncv), xcnt(ncv);
     for (int c = 0; c < ncv; ++c) {
-      cnt[c] = o.efocv_i[c + 1] - o.efocv_i[c];
+      const int cnt = o.efocv_i[c + 1] - o.efocv_i[c];
       int nc = 0;
-      while (nc < cnt[c] && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
-      ccnt[c] = nc;
-      for (int k = nc; k < cnt[c]; ++k)
+      while (nc < cnt && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
+      for (int k = nc; k < cnt; ++k)
         if (o.efocv_v[o.efocv_i[c] + k] < nfa) return bail(s->set(CG_EMESH, "compact pairs must precede extended ones"));
+      ccnt[c] = nc;
+      xcnt[c] = cnt - nc;
     }
-    Sell sl, cs;
-    sl.build(ncv, cnt);
+    Sell cs, xs;
     cs.build(ncv, ccnt);
+    xs.build(ncv, xcnt);
     const std::vector<double>& nU = o.nU();
     const std::vector<double>& cU = o.cU();
     const std::vector<double>& nf = m->vor.n_fa;
-    std::vector<int> nbr(sl.nslots, 0);
-    std::vector<double> g6[6];
-    for (auto& v : g6) v.assign(sl.nslots, 0.0);
-    std::vector<double> c8[8];
-    for (auto& v : c8) v.assign(cs.nslots, 0.0);
-    for (int c = 0; c < ncv; ++c)
-      for (int k = 0; k < cnt[c]; ++k) {
+    std::vector<int> cnbr(cs.nslots, 0), xnbr(xs.nslots, 0);
+    std::vector<double> c14[14];
+    for (auto& v : c14) v.assign(cs.nslots, 0.0);
+    std::vector<double> x6[6];
+    for (auto& v : x6) v.assign(xs.nslots, 0.0);
+    auto code_of = [&](int c, int pidx) {
+      const bool hi = (o.hi[pidx] == c);
+      return int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
+    };
+    for (int c = 0; c < ncv; ++c) {
+      for (int k = 0; k < ccnt[c]; ++k) {
         const int pidx = o.efocv_v[o.efocv_i[c] + k];
-        const bool hi = (o.hi[pidx] == c);
-        const int64_t sidx = sl.slot(c, k);
-        nbr[sidx] = int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
+        const int64_t sidx = cs.slot(c, k);
+        cnbr[sidx] = code_of(c, pidx);
         for (int d = 0; d < 3; ++d) {
-          g6[d][sidx] = nU[3 * (size_t)pidx + d];
-          g6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+          c14[d][sidx] = nU[3 * (size_t)pidx + d];
+          c14[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+          c14[6 + d][sidx] = o.unit_n_use[3 * (size_t)pidx + d];
+          c14[11 + d][sidx] = nf[3 * (size_t)pidx + d];
         }
-        if (k < ccnt[c]) {
-          const int64_t cidx = cs.slot(c, k);
-          for (int d = 0; d < 3; ++d) {
-            c8[d][cidx] = o.unit_n_use[3 * (size_t)pidx + d];
-            c8[5 + d][cidx] = nf[3 * (size_t)pidx + d];
-          }
-          c8[3][cidx] = o.area_use[pidx];
-          c8[4][cidx] = o.aod_use[pidx];
+        c14[9][sidx] = o.area_use[pidx];
+        c14[10][sidx] = o.aod_use[pidx];
+      }
+      for (int k = 0; k < xcnt[c]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
+        const int64_t sidx = xs.slot(c, k);
+        xnbr[sidx] = code_of(c, pidx);
+        for (int d = 0; d < 3; ++d) {
+          x6[d][sidx] = nU[3 * (size_t)pidx + d];
+          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
         }
       }
-    // per-block halos for LDS staging of the extended-face pass
+    }
+    // per-block halos of the extended half-faces for LDS staging
     {
       const int nblk = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
       std::vector<std::vector<int>> halos(nblk);
-      std::vector<int16_t> loc(sl.nslots, 0);
-      std::vector<std::string> herr(1);
+      std::vector<int16_t> loc(xs.nslots, 0);
+      std::string herr;
+      std::mutex hmu;
       const int nth = cgmesh::default_threads();
       std::vector<std::thread> th;
       for (int t = 0; t < nth; ++t)
@@ -637,17 +649,21 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
             v.clear();
             for (int c = c0; c < c1; ++c) {
               v.push_back(c);
-              for (int k = 0; k < cnt[c]; ++k) v.push_back(nbr[sl.slot(c, k)] & 0x7fffffff);
+              for (int k = 0; k < xcnt[c]; ++k) v.push_back(xnbr[xs.slot(c, k)] & 0x7fffffff);
             }
             std::sort(v.begin(), v.end());
             v.erase(std::unique(v.begin(), v.end()), v.end());
-            if (v.size() >= 0x8000) { herr[0] = "block halo too large"; return; }
+            if (v.size() >= 0x8000) {
+              std::lock_guard<std::mutex> lk(hmu);
+              herr = "block halo too large";
+              return;
+            }
             for (int c = c0; c < c1; ++c)
-              for (int k = 0; k < cnt[c]; ++k) {
-                const int64_t sidx = sl.slot(c, k);
-                const int nb = nbr[sidx] & 0x7fffffff;
+              for (int k = 0; k < xcnt[c]; ++k) {
+                const int64_t sidx = xs.slot(c, k);
+                const int nb = xnbr[sidx] & 0x7fffffff;
                 const int pos = int(std::lower_bound(v.begin(), v.end(), nb) - v.begin());
-                loc[sidx] = int16_t(pos | ((nbr[sidx] < 0) ? 0x8000 : 0));
+                loc[sidx] = int16_t(pos | ((xnbr[sidx] < 0) ? 0x8000 : 0));
               }
             halos[b] = v;
           }
@@ -660,28 +676,31 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
         hmax = std::max(hmax, int(halos[b].size()));
       }
       hv.reserve(hptr[nblk]);
-      for (auto& h : halos) hv.insert(hv.end(), h.begin(), h.end());
+      for (auto& hh : halos) hv.insert(hv.end(), hh.begin(), hh.end());
       int dev = 0, lds_max = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-      const bool use_lds = herr[0].empty() && (size_t)hmax * 48 <= (size_t)lds_max && !std::getenv("CG_NO_LDS");
+      const bool use_lds = herr.empty() && (size_t)hmax * 48 <= (size_t)lds_max && !std::getenv("CG_NO_LDS");
       if (use_lds) {
         rc |= upload(s, &g.halo_ptr, hptr);
         rc |= upload(s, &g.halo_v, hv);
-        rc |= upload(s, &g.ef_loc, loc);
+        rc |= upload(s, &g.xf_loc, loc);
         g.halo_max = hmax;
       }
     }
-    rc |= upload(s, &g.ef_ptr, sl.ptr);
-    rc |= upload(s, &g.ef_deg, sl.deg);
-    rc |= upload(s, &g.ef_nc, ccnt);
-    rc |= upload(s, &g.ef_nbr, nbr);
-    rc |= upload(s, &g.sn0, g6[0]); rc |= upload(s, &g.sn1, g6[1]); rc |= upload(s, &g.sn2, g6[2]);
-    rc |= upload(s, &g.sc0, g6[3]); rc |= upload(s, &g.sc1, g6[4]); rc |= upload(s, &g.sc2, g6[5]);
     rc |= upload(s, &g.cf_ptr, cs.ptr);
-    rc |= upload(s, &g.cu0, c8[0]); rc |= upload(s, &g.cu1, c8[1]); rc |= upload(s, &g.cu2, c8[2]);
-    rc |= upload(s, &g.carea, c8[3]); rc |= upload(s, &g.caod, c8[4]);
-    rc |= upload(s, &g.cn0, c8[5]); rc |= upload(s, &g.cn1, c8[6]); rc |= upload(s, &g.cn2, c8[7]);
+    rc |= upload(s, &g.ef_nc, ccnt);
+    rc |= upload(s, &g.cf_nbr, cnbr);
+    rc |= upload(s, &g.csn0, c14[0]); rc |= upload(s, &g.csn1, c14[1]); rc |= upload(s, &g.csn2, c14[2]);
+    rc |= upload(s, &g.csc0, c14[3]); rc |= upload(s, &g.csc1, c14[4]); rc |= upload(s, &g.csc2, c14[5]);
+    rc |= upload(s, &g.cu0, c14[6]); rc |= upload(s, &g.cu1, c14[7]); rc |= upload(s, &g.cu2, c14[8]);
+    rc |= upload(s, &g.carea, c14[9]); rc |= upload(s, &g.caod, c14[10]);
+    rc |= upload(s, &g.cn0, c14[11]); rc |= upload(s, &g.cn1, c14[12]); rc |= upload(s, &g.cn2, c14[13]);
+    rc |= upload(s, &g.xf_ptr, xs.ptr);
+    rc |= upload(s, &g.xf_deg, xcnt);
+    rc |= upload(s, &g.xf_nbr, xnbr);
+    rc |= upload(s, &g.xsn0, x6[0]); rc |= upload(s, &g.xsn1, x6[1]); rc |= upload(s, &g.xsn2, x6[2]);
+    rc |= upload(s, &g.xsc0, x6[3]); rc |= upload(s, &g.xsc1, x6[4]); rc |= upload(s, &g.xsc2, x6[5]);
     if (rc) return bail(rc);
   }
   // gradient stencil without the diagonal

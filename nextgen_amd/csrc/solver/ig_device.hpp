@@ -24,26 +24,30 @@ enum { A_MU = 0, A_LOC, A_ASGS, A_FGR, A_SYMH, A_SYMP, A_PAD, A_D = 7, A_N = 16 
 
 // Mesh / operator arrays, all device pointers.
 //
-// Each cell's half-faces (its extended pairs, in pair order: the compact ones
-// first) sit in SELL-64 layout: slot(k, lane) = ptr[slice] + k*64 + lane for
-// k < deg[cell], so lane-consecutive cells read consecutive slots. The pair
-// geometry is stored per slot (both cells of a pair hold a copy), which keeps
-// every geometry read coalesced.
+// Each cell's half-faces (its extended pairs, in pair order) are split in two
+// SELL-64 regions: the compact ones (cf_*, ef_nc[cell] slots) and the purely
+// extended ones (xf_*, xf_deg[cell] slots). Slot k of a cell is at
+// ptr[slice] + k*64 + lane, so at every k the 64 lanes of a wavefront read one
+// contiguous row. The pair geometry is stored per slot (both cells of a pair
+// hold a copy), which keeps every geometry read coalesced.
 struct Geom {
   int ncv, nfa, nef;
   const double *inv_vol, *vol, *sa_to_vol, *dcmag, *dx13, *dx23;
-  const int *ef_ptr, *ef_deg, *ef_nc;  // per slice / cell: slot base, half-faces, compact half-faces
-  const int* ef_nbr;                   // neighbour | (cell is the pair's hi side) << 31
-  const double *sn0, *sn1, *sn2, *sc0, *sc1, *sc2;  // pair n and c (grouping applied) per slot
-  // compact half-faces (k < ef_nc) in their own SELL-64 layout
-  const int* cf_ptr;
-  const double *cu0, *cu1, *cu2, *carea, *caod;     // unit normal, area, area/delta (grouping applied)
-  const double *cn0, *cn1, *cn2;                    // raw area vector (CFL)
+  // compact half-faces
+  const int *cf_ptr, *ef_nc;
+  const int* cf_nbr;                                   // neighbour | (cell is the pair's hi side) << 31
+  const double *csn0, *csn1, *csn2, *csc0, *csc1, *csc2;  // pair n and c (grouping applied)
+  const double *cu0, *cu1, *cu2, *carea, *caod;        // unit normal, area, area/delta (grouping applied)
+  const double *cn0, *cn1, *cn2;                       // raw area vector (CFL)
+  // purely extended half-faces
+  const int *xf_ptr, *xf_deg;
+  const int* xf_nbr;                                   // neighbour | hi-side flag << 31
+  const double *xsn0, *xsn1, *xsn2, *xsc0, *xsc1, *xsc2;
   // per-block halo for LDS staging (null when staging is off): the sorted
-  // cells block b's half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
-  // ef_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15
+  // cells block b's extended half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
+  // xf_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15
   const int *halo_ptr, *halo_v;
-  const int16_t* ef_loc;
+  const int16_t* xf_loc;
   int halo_max;
   // gradient stencil without the diagonal, SELL-64 layout
   const int *gr_ptr, *gr_deg, *gr_nbr;

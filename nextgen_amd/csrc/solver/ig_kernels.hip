@@ -274,16 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_compact(Geom g, State in, Work 
   if (c >= g.ncv) return;
   if (w.info->nan) return;
   const int lane = c & (kSlice - 1);
-  const int base = g.ef_ptr[c >> 6] + lane;
   const int cbase = g.cf_ptr[c >> 6] + lane;
   const int nc = g.ef_nc[c];
   const int ncv = g.ncv;
   const Light self = load_light(in.light, c);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < nc; ++k) {
-    const int slot = base + k * kSlice;
     const int cslot = cbase + k * kSlice;
-    const int code = ldnt(g.ef_nbr + slot);
+    const int code = ldnt(g.cf_nbr + cslot);
     const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
     const Light other = load_light(in.light, nb);
@@ -291,12 +289,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_compact(Geom g, State in, Work 
     const Aux An = load_aux(w.aux, nb);
     const Light& a = self_hi ? other : self;
     const Light& b = self_hi ? self : other;
-    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
-    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
+    const double n[3] = {ldnt(g.csn0 + cslot), ldnt(g.csn1 + cslot), ldnt(g.csn2 + cslot)};
+    const double cv[3] = {ldnt(g.csc0 + cslot), ldnt(g.csc1 + cslot), ldnt(g.csc2 + cslot)};
     const double un[3] = {ldnt(g.cu0 + cslot), ldnt(g.cu1 + cslot), ldnt(g.cu2 + cslot)};
     double F[5], u_avg[3], inv_v_avg, h_stag;
     flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
-    flux_compact(F, un, ldnt(g.carea + cslot), ldnt(g.caod + cslot), a, b, self_hi ? An : As, self_hi ? As : An, u_avg, h_stag);
+    flux_compact(F, un, ldnt(g.carea + cslot), ldnt(g.caod + cslot), a, b, self_hi ? An : As, self_hi ? As : An,
+                 u_avg, h_stag);
     if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
       double cF[5];
       flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
@@ -325,25 +324,24 @@ __global__ __launch_bounds__(kBlock) void k_flux(Geom g, State in, State out, Wo
   if (c >= g.ncv) return;
   if (MODE == 1 && w.info->nan) return;
   const int lane = c & (kSlice - 1);
-  const int base = g.ef_ptr[c >> 6] + lane;
-  const int deg = g.ef_deg[c];
-  const int nc = g.ef_nc[c];
+  const int base = g.xf_ptr[c >> 6] + lane;
+  const int deg = g.xf_deg[c];
   const int ncv = g.ncv;
 
   const Light self = load_light(in.light, c);
   double acc[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = w.racc[(size_t)q * ncv + c];
-  for (int k = nc; k < deg; ++k) {
+  for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
-    const int code = ldnt(g.ef_nbr + slot);
+    const int code = ldnt(g.xf_nbr + slot);
     const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
     const Light other = load_light(in.light, nb);
     const Light& a = self_hi ? other : self;
     const Light& b = self_hi ? self : other;
-    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
-    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
+    const double n[3] = {ldnt(g.xsn0 + slot), ldnt(g.xsn1 + slot), ldnt(g.xsn2 + slot)};
+    const double cv[3] = {ldnt(g.xsc0 + slot), ldnt(g.xsc1 + slot), ldnt(g.xsc2 + slot)};
     double F[5], u_avg[3], inv_v_avg, h_stag;
     flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
     if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
@@ -366,14 +364,13 @@ __global__ __launch_bounds__(kBlock) void k_flux(Geom g, State in, State out, Wo
     for (int q = 0; q < 5; ++q) rhs_out[(size_t)q * ncv + c] = acc[q];
     return;
   }
-
   rk_update<STAGE>(g, in, out, w, P, c, self, acc);
 }
 
 // The same extended-face pass with the block's neighbour records staged in
 // LDS: the block first copies the first 48 bytes (u, sp_vol, p, h) of the
-// light record of every cell its half-faces reach (its halo, ~4x its own
-// cells) into LDS, then every gather is an LDS read. ef_loc holds the halo
+// light record of every cell its extended half-faces reach (its halo, ~4x its own
+// cells) into LDS, then every gather is an LDS read. xf_loc holds the halo
 // position of each slot's neighbour (bit 15: the cell is the pair's hi side).
 template <int STAGE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_flux_lds(Geom g, State in, State out, Work w, Params P, double* rhs_out) {
@@ -392,41 +389,54 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(Geom g, State in, State out
   const int c = blk * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
   const int lane = c & (kSlice - 1);
-  const int base = g.ef_ptr[c >> 6] + lane;
-  const int deg = g.ef_deg[c];
-  const int nc = g.ef_nc[c];
+  const int base = g.xf_ptr[c >> 6] + lane;
+  const int deg = g.xf_deg[c];
   const int ncv = g.ncv;
   const Light self = load_light(in.light, c);
   double acc[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = w.racc[(size_t)q * ncv + c];
-  for (int k = nc; k < deg; ++k) {
-    const int slot = base + k * kSlice;
-    const int code = ldnt(g.ef_loc + slot);
-    const int li = code & 0x7fff;
-    const bool self_hi = (code & 0x8000) != 0;
-    const double2 r0 = halo[3 * li + 0], r1 = halo[3 * li + 1], r2 = halo[3 * li + 2];
-    Light other;
-    other.u[0] = r0.x; other.u[1] = r0.y; other.u[2] = r1.x; other.spv = r1.y; other.p = r2.x; other.h = r2.y;
-    other.rho = 0.0; other.sos = 0.0;
-    const Light& a = self_hi ? other : self;
-    const Light& b = self_hi ? self : other;
-    const double n[3] = {ldnt(g.sn0 + slot), ldnt(g.sn1 + slot), ldnt(g.sn2 + slot)};
-    const double cv[3] = {ldnt(g.sc0 + slot), ldnt(g.sc1 + slot), ldnt(g.sc2 + slot)};
-    double F[5], u_avg[3], inv_v_avg, h_stag;
-    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
-    if (cv[0] != 0.0 || cv[1] != 0.0 || cv[2] != 0.0) {
-      double cF[5];
-      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
+  // slots are fetched kBatch at a time so their global loads overlap; the
+  // accumulation order stays k = 0, 1, ...
+  constexpr int kBatch = 4;
+  for (int k0 = 0; k0 < deg; k0 += kBatch) {
+    int code[kBatch];
+    double n[kBatch][3], cv[kBatch][3];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) F[q] += cF[q];
+    for (int j = 0; j < kBatch; ++j) {
+      if (k0 + j < deg) {
+        const int slot = base + (k0 + j) * kSlice;
+        code[j] = ldnt(g.xf_loc + slot);
+        n[j][0] = ldnt(g.xsn0 + slot); n[j][1] = ldnt(g.xsn1 + slot); n[j][2] = ldnt(g.xsn2 + slot);
+        cv[j][0] = ldnt(g.xsc0 + slot); cv[j][1] = ldnt(g.xsc1 + slot); cv[j][2] = ldnt(g.xsc2 + slot);
+      }
     }
-    if (self_hi) {
 #pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] += F[q];
-    } else {
+    for (int j = 0; j < kBatch; ++j) {
+      if (k0 + j >= deg) break;
+      const int li = code[j] & 0x7fff;
+      const bool self_hi = (code[j] & 0x8000) != 0;
+      const double2 r0 = halo[3 * li + 0], r1 = halo[3 * li + 1], r2 = halo[3 * li + 2];
+      Light other;
+      other.u[0] = r0.x; other.u[1] = r0.y; other.u[2] = r1.x; other.spv = r1.y; other.p = r2.x; other.h = r2.y;
+      other.rho = 0.0; other.sos = 0.0;
+      const Light& a = self_hi ? other : self;
+      const Light& b = self_hi ? self : other;
+      double F[5], u_avg[3], inv_v_avg, h_stag;
+      flux_skew(F, n[j], a, b, u_avg, inv_v_avg, h_stag);
+      if (cv[j][0] != 0.0 || cv[j][1] != 0.0 || cv[j][2] != 0.0) {
+        double cF[5];
+        flux_symmetric(cF, cv[j], u_avg, inv_v_avg, h_stag, a, b);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+        for (int q = 0; q < 5; ++q) F[q] += cF[q];
+      }
+      if (self_hi) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] += F[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+      }
     }
   }
   if (MODE == 0) {
@@ -557,16 +567,19 @@ __global__ __launch_bounds__(kBlock) void k_props(Geom g, State st, Work w, Para
   const double symh = P.coeff_h * g.dcmag[c];
 
   // ---- Gibbs-remainder sensor and convective CFL over the extended half-faces ----
-  const int base = g.ef_ptr[c >> 6] + (c & (kSlice - 1));
-  const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
-  const int deg = g.ef_deg[c];
+  const int lane = c & (kSlice - 1);
+  const int cbase = g.cf_ptr[c >> 6] + lane;
+  const int xbase = g.xf_ptr[c >> 6] + lane;
   const int nc = g.ef_nc[c];
+  const int nx = g.xf_deg[c];
   const double lp_s = w.logp[c];
   const double irt_s = 1.0 / (P.R_gas * T);
   double vv2 = 0.0, cfl = 0.0;
-  for (int k = 0; k < deg; ++k) {
-    const int slot = base + k * kSlice;
-    const int code = ldnt(g.ef_nbr + slot);
+  // sensor: max over every extended pair (order-independent); CFL: compact faces in pair order
+  for (int k = 0; k < nc + nx; ++k) {
+    const bool compact = k < nc;
+    const int slot = compact ? cbase + k * kSlice : xbase + (k - nc) * kSlice;
+    const int code = compact ? ldnt(g.cf_nbr + slot) : ldnt(g.xf_nbr + slot);
     const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
     const Light o = load_light(st.light, nb);
@@ -579,9 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_props(Geom g, State st, Work w, Para
     const double beta_avg = 0.5 * ((self_hi ? irt_o : irt_s) + (self_hi ? irt_s : irt_o));
     const double gr = (self_hi ? lp_s : lp_o) - (self_hi ? lp_o : lp_s) - v_avg * dp * beta_avg;
     vv2 = smax(vv2, fabs(gr));
-    if (k < nc) {
-      const int cslot = cbase + k * kSlice;
-      const double n[3] = {ldnt(g.cn0 + cslot), ldnt(g.cn1 + cslot), ldnt(g.cn2 + cslot)};
+    if (compact) {
+      const double n[3] = {ldnt(g.cn0 + slot), ldnt(g.cn1 + slot), ldnt(g.cn2 + slot)};
       const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
       const double undA = 0.5 * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
                                  (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
@@ -639,14 +651,13 @@ __global__ void k_begin_step(Work w, Params P) {
 __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, State st, Work w, double dt) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
-  const int base = g.ef_ptr[c >> 6] + (c & (kSlice - 1));
   const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
   const int nc = g.ef_nc[c];
   const Light self = load_light(st.light, c);
   double cfl = 0.0;
   for (int k = 0; k < nc; ++k) {
-    const int code = ldnt(g.ef_nbr + base + k * kSlice);
     const int cslot = cbase + k * kSlice;
+    const int code = ldnt(g.cf_nbr + cslot);
     const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
     const Light o = load_light(st.light, nb);

@@ -20,7 +20,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libcharles_gpu.so")
+# CHARLES_GPU_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("CHARLES_GPU_LIB", os.path.join(_HERE, "lib", "libcharles_gpu.so"))
 
 CG_SGS_NONE, CG_SGS_VREMAN = 0, 1
 CG_DISS_SENSOR, CG_DISS_NONE, CG_DISS_ALL = 0, 1, 2

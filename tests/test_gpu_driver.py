@@ -1,0 +1,43 @@
+"""The C++ host side (charles_ig_gpu over include/charles_gpu.h) on a small deck,
+checked against the oracle's history of the same run."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from nextgen_amd.ic import tgv_params, tgv_state
+from nextgen_amd.ideal_gas import Mesh, parse_params
+from oracle.oracle import OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "nextgen_amd", "lib", "charles_ig_gpu")
+
+
+def test_driver_runs_deck_and_matches_oracle(tmp_path):
+    deck = tmp_path / "charles.in"
+    deck.write_text(tgv_params(sgs="VREMAN") + "MESH JITTERED 12 0.3 11\nINIT TGV 0.1\nNSTEPS 10\nCHECK_INTERVAL 5\n")
+    out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=120)
+    print(out.stdout[-2000:], out.stderr[-2000:])
+    assert out.returncode == 0
+    steps = [int(s) for s in re.findall(r"> step: (\d+)", out.stdout)]
+    kes = [float(s) for s in re.findall(r"ke: ([0-9.eE+-]+)", out.stdout)]
+    assert steps[-1] == 10
+    m = Mesh.jittered(12, 0.3, seed=11)
+    p = parse_params(tgv_params(sgs="VREMAN"))
+    o = OracleSolver(m, p)
+    o.set_state(*tgv_state(m.get("x_cv")))
+    o.init_complete()
+    o.step(10)
+    assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]
+
+
+def test_driver_reports_bad_deck(tmp_path):
+    deck = tmp_path / "bad.in"
+    deck.write_text("SGS_MODEL DSM_LOCAL\nMESH LATTICE 8\n")
+    out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1
+    assert "SGS_MODEL" in out.stderr
