@@ -50,8 +50,16 @@ extern "C" {
 #define CG_DT_CFL_DT 2    /* TIMESTEP CFL_DT <cfl> <dt_max> */
 #define CG_DT_CFL_T 3     /* TIMESTEP CFL_T <cfl> <T_end> */
 
-/* arithmetic of the flux/gradient path */
+/* Arithmetic of the step (no counterpart in the reference, which is FP64 only).
+ * CG_PREC_FP64: every operation in FP64, as the reference.
+ * CG_PREC_MIXED: face geometry, neighbour records, face fluxes and their
+ *   accumulation, gradients, lagged coefficients and RK registers in FP32; the
+ *   conserved state, the RK3 state update, the recovery of p and T from rhoE
+ *   and the conservation sums in FP64. Pressure and enthalpy are carried in
+ *   FP32 relative to the gauges P_REF and h(T_REF). Input-file key:
+ *   PRECISION FP64 | MIXED. */
 #define CG_PREC_FP64 0
+#define CG_PREC_MIXED 1
 
 typedef struct cg_mesh cg_mesh;
 typedef struct cg_solver cg_solver;
@@ -87,7 +95,7 @@ typedef struct cg_params {
   int group_member_threshold; /* GROUP_MEMBER_THRESHOLD (100) */
   int disable_grouping;    /* DISABLE_FA_GROUPING */
   int zero_c_ef;           /* ZERO_C_EF */
-  int precision;           /* CG_PREC_FP64 */
+  int precision;           /* PRECISION: CG_PREC_FP64 (default) or CG_PREC_MIXED */
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,
@@ -137,6 +145,18 @@ int cg_mesh_counts(const cg_mesh* m, int64_t counts[10]);
  * efocv_i, efocv_v, grad_flag, max_gcl). nbytes must equal the array size. */
 int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes);
 void cg_mesh_destroy(cg_mesh* m);
+
+/* Initial fields at the cell centroids, into caller arrays rho[ncv], u[ncv][3],
+ * rhoE[ncv]; what an INIT expression or initialHook sets
+ * (StaticSolver.hpp:1402-1470, charles_ig.cpp:209-222).
+ * cg_ic_tgv: Taylor-Green vortex on [0, 2 pi]^3 with U0 = rho0 = T0 = 1 and
+ *   p0 = 1/(gamma Ma^2) (the state the TGV decks initialise).
+ * cg_ic_k4: seeded solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2) over integer
+ *   wave vectors |k| <= kmax (kmax <= 0: ceil(2.5 k0); at most 64), u_rms = Mt,
+ *   rho = 1, p = 1/gamma (so c0 = 1 and Mt is the turbulent Mach number). */
+int cg_ic_tgv(const cg_mesh* m, double Ma, double gamma, double* rho, double* u, double* rhoE);
+int cg_ic_k4(const cg_mesh* m, double k0, double Mt, double gamma, uint64_t seed, int kmax, double* rho,
+             double* u, double* rhoE);
 const char* cg_error_string(void);
 
 /* ---- solver ---- */
@@ -181,6 +201,51 @@ int cg_restore(cg_solver* s);
 int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms);
 const char* cg_last_error(const cg_solver* s);
 void cg_destroy(cg_solver* s);
+
+/* ---- batches: many independent solves advanced together ----
+ * No counterpart in the reference (one solve per process). Each instance is a
+ * solve with its own mesh, parameters, state, time step (its own CFL), NaN
+ * rollback and snapshots; cg_batch_step advances every instance by the same
+ * number of steps, each kernel launch covering all instances (per-instance
+ * offsets through a block map), the step sequence replayed from a captured
+ * HIP graph. Instances may be added and removed between steps; an instance's
+ * results do not depend on the others in the batch. All instances share the
+ * batch's precision. Every call returns 0 or a CG_E* code; the message is in
+ * cg_batch_last_error. */
+typedef struct cg_batch cg_batch;
+int cg_batch_create(int precision, cg_batch** out);
+/* Add a solve (mesh, parameters with the batch's precision, initial state as
+ * for cg_set_state); initComplete is applied. *id receives its handle. */
+int cg_batch_add(cg_batch* b, const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
+                 const double* rhoE, int* id);
+int cg_batch_remove(cg_batch* b, int id);
+/* The two halves of cg_batch_add, for callers that prepare solves while a
+ * batch is stepping (a serving loop): cg_instance_create builds the device
+ * data, sets the state and applies initComplete on a stream of its own and may
+ * run on another host thread than the batch's; cg_batch_attach (between steps,
+ * on the batch's thread) hands the instance to the batch, which owns it from
+ * then on. cg_instance_bytes reports the device memory it holds. */
+typedef struct cg_instance cg_instance;
+int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
+                       const double* rhoE, cg_instance** out);
+int cg_instance_bytes(const cg_instance* x, int64_t* bytes);
+void cg_instance_destroy(cg_instance* x);
+int cg_batch_attach(cg_batch* b, cg_instance* x, int* id);
+/* nsteps of setDt + advanceSolution for every instance, NaN checks at each
+ * instance's CHECK_INTERVAL. */
+int cg_batch_step(cg_batch* b, int nsteps);
+int cg_batch_get_diag(cg_batch* b, int id, cg_diag* d);
+int cg_batch_get_state(cg_batch* b, int id, double* rho, double* u, double* rhoE);
+/* counts[0..3] = instances, cells, device bytes held by the instances,
+ * instances whose extended-face pass stages a halo in LDS */
+int cg_batch_counts(cg_batch* b, int64_t counts[4]);
+/* "graph" 0/1: replay steps from a captured HIP graph (default 1);
+ * "lds" 0/1: extended-face pass with LDS halo staging (default 1) */
+int cg_batch_set_option(cg_batch* b, const char* name, int value);
+/* time nsteps steps of the whole batch with HIP events (graph built before) */
+int cg_batch_time_steps(cg_batch* b, int nsteps, double* ms_total);
+const char* cg_batch_last_error(const cg_batch* b);
+void cg_batch_destroy(cg_batch* b);
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,7 @@ std::string build_from_arrays(Voronoi& m, int ncv, const double* x_cv, const dou
                               const double* x_fa, const double L[3]);
 std::string build_operators(const Voronoi& m, const OperatorParams& p, Operators& op, int nthreads);
 
-// Morton code of a lattice index (10 bits per axis).
+// Morton code of a lattice index (21 bits per axis).
 uint64_t morton3(uint32_t i, uint32_t j, uint32_t k);
 
 // Counter-based uniform double in [0,1) (splitmix64 of seed and counter).
@@ -87,5 +87,10 @@ double hash_uniform(uint64_t seed, uint64_t counter);
 void svd33(const double A[9], double U[9], double V[9], double s[3]);
 
 int default_threads();
+
+// Initial fields at the cell centroids (ic.cpp). ic_k4 returns "" or an error message.
+void ic_tgv(int ncv, const double* x_cv, double Ma, double gamma, double* rho, double* u, double* rhoE);
+std::string ic_k4(int ncv, const double* x_cv, double k0, double Mt, double gamma, uint64_t seed, int kmax,
+                  double* rho, double* u, double* rhoE, int nthreads);
 
 }  // namespace cgmesh

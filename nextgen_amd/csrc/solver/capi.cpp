@@ -8,6 +8,14 @@
 //   properties (SGS, material, sensor, CFL)     IdealGasSolver.hpp:580
 //
 // with the NaN rollback of checkForNans/restoreData checked at sync points.
+//
+// Every solve is an Instance: its device buffers in the working precision R
+// (double, or float for mixed precision) plus its host bookkeeping. An Engine
+// advances a set of instances of one precision together: each kernel launch
+// of a step covers all of them through a device table of instance records
+// and a block map (cgdev::Inst, ig_device.hpp). A cg_solver is an engine of
+// one instance; a cg_batch is an engine of many, whose step is replayed from
+// a captured HIP graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +29,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/charles_gpu.h"
@@ -36,6 +45,16 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// error sink of a handle: the message of the last failure
+struct Err {
+  std::string err;
+  int set(int code, const std::string& msg) {
+    err = msg;
+    g_err = msg;
+    return code;
+  }
+};
+
 }  // namespace
 
 struct cg_mesh {
@@ -44,32 +63,101 @@ struct cg_mesh {
   int max_ef_deg = 0, max_fa_deg = 0;
 };
 
-struct cg_solver {
-  cg_params prm;
-  cgdev::Params P;
-  int ncv = 0, nfa = 0, nef = 0, nblocks = 0;
-  hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
-  cgdev::Geom g{};
-  cgdev::State st[2]{};
-  int cur = 0;
-  cgdev::Work w{};
-  double* store_buf = nullptr;     // older snapshot
-  double* store_recent = nullptr;  // newer snapshot
-  size_t snap_n = 0;               // doubles per snapshot
+namespace {
+
+// device buffers of one solve in the working precision R
+template <class R>
+struct Bufs {
+  cgdev::GeomT<R> g{};
+  cgdev::StateT<R> st[2]{};
+  cgdev::WorkT<R> w{};
+};
+
+// host bookkeeping of one solve
+struct HostInst {
+  cg_params prm{};
+  cgdev::Params P{};
+  int ncv = 0, nfa = 0, nef = 0, nblocks = 0, npblocks = 0;
+  cgdev::StepInfo* info = nullptr;   // device step info
+  double* store_buf = nullptr;       // older snapshot
+  double* store_recent = nullptr;    // newer snapshot
+  size_t snap_n = 0;                 // doubles per snapshot
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
   bool have_store = false;
-  bool ready = false;   // primitives / gradient / properties are current
+  bool ready = false;                // primitives / gradient / properties are current
   int64_t host_step = 0;
-  double* rhs_tmp = nullptr;  // [5][ncv]
-  std::string err;
-  cgdev::StepInfo* info_host = nullptr;  // pinned mirror
-  int set(int code, const std::string& msg) {
-    err = msg;
-    g_err = msg;
-    return code;
+  double* rhs_tmp = nullptr;         // [5][ncv] scratch
+  size_t lds = 0;                    // LDS bytes of its extended-face halo (0: no halo)
+  std::vector<void*> allocs;         // owned device memory
+  size_t bytes = 0;
+  int id = -1;
+};
+
+template <class R>
+struct Instance : HostInst {
+  Bufs<R> b;
+  cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
+  int* solo_bmap = nullptr;          // its cell blocks in XCD order
+  int* solo_pbmap = nullptr;         // its compact-pair blocks in XCD order
+  cgk::Launch<R> solo_launch() const { return {solo, solo_bmap, nblocks, 1, lds, solo_pbmap, npblocks}; }
+  cgdev::Inst<R> record() const {
+    cgdev::Inst<R> r{};
+    r.g = b.g;
+    r.st[0] = b.st[0];
+    r.st[1] = b.st[1];
+    r.w = b.w;
+    r.P = P;
+    r.nblocks = nblocks;
+    r.npblocks = npblocks;
+    return r;
   }
+};
+
+// instances of one precision advanced together
+template <class R>
+struct Engine {
+  std::vector<Instance<R>*> inst;
+  cgdev::Inst<R>* tab = nullptr;
+  int tab_cap = 0;
+  int* bmap = nullptr;
+  int bmap_cap = 0;
+  int* pbmap = nullptr;
+  int pbmap_cap = 0;
+  cgk::Launch<R> L{};
+  int cur = 0;            // st[cur] holds the current state of every instance
+  bool dirty = true;      // table / block map need rebuilding
+  bool force_global = false;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two captured steps from parity cur (which they return to)
+};
+
+}  // namespace
+
+struct cg_solver : Err {
+  bool mixed = false;
+  hipStream_t stream = nullptr;
+  cgdev::StepInfo* info_host = nullptr;  // pinned mirror
+  Engine<double> ed;
+  Engine<float> ef;
+  Instance<double>* id = nullptr;
+  Instance<float>* ix = nullptr;
+};
+
+// a solve prepared outside any batch (its own stream while it is built)
+struct cg_instance : Err {
+  bool mixed = false;
+  Instance<double>* id = nullptr;
+  Instance<float>* ix = nullptr;
+};
+
+struct cg_batch : Err {
+  bool mixed = false;
+  hipStream_t stream = nullptr;
+  cgdev::StepInfo* info_host = nullptr;
+  Engine<double> ed;
+  Engine<float> ef;
+  int next_id = 0;
+  bool use_graph = true;
 };
 
 namespace {
@@ -81,22 +169,41 @@ namespace {
   } while (0)
 
 template <class T>
-int dev_alloc(cg_solver* s, T** p, size_t n) {
+int dev_alloc(Err* e, HostInst* h, T** p, size_t n) {
   void* q = nullptr;
   if (n == 0) n = 1;
-  HIPCHK(s, hipMalloc(&q, n * sizeof(T)));
-  s->allocs.push_back(q);
+  HIPCHK(e, hipMalloc(&q, n * sizeof(T)));
+  h->allocs.push_back(q);
+  h->bytes += n * sizeof(T);
   *p = (T*)q;
   return 0;
 }
 
 template <class T>
-int upload(cg_solver* s, const T** dst, const std::vector<T>& v) {
+int upload(Err* e, HostInst* h, const T** dst, const std::vector<T>& v) {
   T* d = nullptr;
-  if (int rc = dev_alloc(s, &d, v.size())) return rc;
-  if (!v.empty()) HIPCHK(s, hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (int rc = dev_alloc(e, h, &d, v.size())) return rc;
+  if (!v.empty()) HIPCHK(e, hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   *dst = d;
   return 0;
+}
+
+// upload FP64 host data in the working precision R
+template <class R>
+int upload_as(Err* e, HostInst* h, const R** dst, const std::vector<double>& v) {
+  if constexpr (std::is_same<R, double>::value) {
+    return upload(e, h, dst, v);
+  } else {
+    std::vector<R> t(v.begin(), v.end());
+    return upload(e, h, dst, t);
+  }
+}
+
+template <class R>
+void free_instance(Instance<R>* I) {
+  if (!I) return;
+  for (void* p : I->allocs) (void)hipFree(p);
+  delete I;
 }
 
 // SELL-64: per slice of 64 cells, slots of the k-th entry of each cell are contiguous.
@@ -125,9 +232,9 @@ std::string upper(std::string s) {
   return s;
 }
 
-void fill_dev_params(cg_solver* s) {
-  const cg_params& p = s->prm;
-  cgdev::Params& P = s->P;
+void fill_dev_params(HostInst* h) {
+  const cg_params& p = h->prm;
+  cgdev::Params& P = h->P;
   P.gamma = p.gamma;
   P.R_gas = p.p_ref / p.rho_ref / p.T_ref;
   P.gm1 = p.gamma - 1.0;
@@ -153,6 +260,10 @@ void fill_dev_params(cg_solver* s) {
   P.dt_mode = p.dt_mode;
   P.dt_data0 = p.dt_data[0];
   P.dt_data1 = p.dt_data[1];
+  // gauges of the mixed-precision records: the reference pressure and the
+  // enthalpy at the reference temperature
+  P.p_g = p.p_ref;
+  P.h_g = P.Rgogm1 * p.T_ref;
 }
 
 int check_params(const cg_params* p) {
@@ -162,7 +273,7 @@ int check_params(const cg_params* p) {
   if (!(p->Pr_lam > 0 && p->Pr_t > 0)) return fail(CG_EINVAL, "PR_LAM and PR_T must be positive");
   if (p->sgs_model != CG_SGS_NONE && p->sgs_model != CG_SGS_VREMAN) return fail(CG_EINVAL, "unsupported SGS_MODEL");
   if (p->dt_mode < 0 || p->dt_mode > 3) return fail(CG_EINVAL, "bad TIMESTEP mode");
-  if (p->precision != CG_PREC_FP64) return fail(CG_EINVAL, "unsupported precision");
+  if (p->precision != CG_PREC_FP64 && p->precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   return 0;
 }
 
@@ -201,109 +312,632 @@ int finish_mesh(cg_mesh* m, const cg_params* p, cg_mesh** out) {
   return CG_OK;
 }
 
-// initComplete / restore: primitives, gradient and properties of the current state
-int refresh(cg_solver* s) {
-  const cgdev::State& st = s->st[s->cur];
-  if (cgk::launch_eos(s->g, st, s->w, s->P, s->stream)) return s->set(CG_EHIP, "eos launch failed");
-  if (cgk::launch_grad(s->g, st, s->w, s->stream)) return s->set(CG_EHIP, "grad launch failed");
-  if (cgk::launch_props(s->g, st, s->w, s->P, 1, s->stream)) return s->set(CG_EHIP, "props launch failed");
-  s->ready = true;
-  return 0;
-}
-
-// snapshot layout: light records [ncv][8] followed by rhoE [ncv]
-int snapshot_copy(cg_solver* s, double* dst, bool to_snapshot) {
-  const cgdev::State& st = s->st[s->cur];
-  const size_t nl = (size_t)s->ncv * cgdev::L_N;
-  if (to_snapshot) {
-    HIPCHK(s, hipMemcpyAsync(dst, st.light, sizeof(double) * nl, hipMemcpyDeviceToDevice, s->stream));
-    HIPCHK(s, hipMemcpyAsync(dst + nl, st.rhoE, sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
-  } else {
-    HIPCHK(s, hipMemcpyAsync(st.light, dst, sizeof(double) * nl, hipMemcpyDeviceToDevice, s->stream));
-    HIPCHK(s, hipMemcpyAsync(st.rhoE, dst + nl, sizeof(double) * s->ncv, hipMemcpyDeviceToDevice, s->stream));
+// Block order of a launch over `nb` logical blocks: workgroups are dealt
+// round-robin over the 8 XCDs, so physical block w runs on XCD w % 8; XCD x
+// is given the contiguous logical blocks [x*q + min(x,r), ...).
+std::vector<int> xcd_order(int nb) {
+  std::vector<int> logical(nb);
+  const int q = nb / cgdev::kXcd, r = nb % cgdev::kXcd;
+  for (int w = 0; w < nb; ++w) {
+    const int x = w % cgdev::kXcd, j = w / cgdev::kXcd;
+    logical[w] = x * q + (x < r ? x : r) + j;
   }
+  return logical;
+}
+
+// the instance's record in its one-entry device table (after any change of P)
+template <class R>
+int upload_solo(Err* s, Instance<R>* I) {
+  const cgdev::Inst<R> r = I->record();
+  HIPCHK(s, hipMemcpy(I->solo, &r, sizeof(r), hipMemcpyHostToDevice));
   return 0;
 }
 
-int read_info(cg_solver* s) {
-  HIPCHK(s, hipMemcpyAsync(s->info_host, s->w.info, sizeof(cgdev::StepInfo), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s, hipStreamSynchronize(s->stream));
+// Device layout of a mesh in the working precision R: geometry, SELL-64
+// half-face regions, LDS halos, gradient stencil, state and work buffers.
+template <class R>
+int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, bool allow_lds,
+                 hipStream_t stream) {
+  constexpr bool mixed = std::is_same<R, float>::value;
+  Bufs<R>& b = I->b;
+  HostInst* h = I;
+  const cgmesh::Operators& o = m->op;
+  const int ncv = o.ncv, nfa = o.nfa;
+  h->prm = *p;
+  fill_dev_params(h);
+  h->ncv = ncv;
+  h->nfa = nfa;
+  h->nef = o.nef;
+  h->nblocks = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
+  h->npblocks = (nfa + cgdev::kBlock - 1) / cgdev::kBlock;
+  if (h->npblocks >= (1 << cgdev::kBlkBits)) return s->set(CG_EINVAL, "mesh too large for one instance");
+  cgdev::GeomT<R>& g = b.g;
+  g.ncv = ncv;
+  g.nfa = nfa;
+  g.nef = o.nef;
+  std::vector<double> dx13(ncv), dx23(ncv);
+  for (int c = 0; c < ncv; ++c) {
+    dx13[c] = std::pow(o.vol[c], 1.0 / 3.0);
+    dx23[c] = std::pow(o.vol[c], 2.0 / 3.0);
+  }
+  int rc = 0;
+  rc |= upload(s, h, &g.inv_vol, o.inv_vol);
+  rc |= upload(s, h, &g.vol, o.vol);
+  rc |= upload(s, h, &g.sa_to_vol, o.sa_to_vol);
+  rc |= upload(s, h, &g.dcmag, o.dcmag_hat);
+  rc |= upload(s, h, &g.dx13, dx13);
+  rc |= upload(s, h, &g.dx23, dx23);
+  if (rc) return rc;
+
+  // half-faces (per-slot geometry): compact ones and purely extended ones in
+  // two SELL-64 regions, each starting at row 0
+  {
+    std::vector<int> ccnt(ncv), xcnt(ncv);
+    for (int c = 0; c < ncv; ++c) {
+      const int cnt = o.efocv_i[c + 1] - o.efocv_i[c];
+      int nc = 0;
+      while (nc < cnt && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
+      for (int k = nc; k < cnt; ++k)
+        if (o.efocv_v[o.efocv_i[c] + k] < nfa) return s->set(CG_EMESH, "compact pairs must precede extended ones");
+      ccnt[c] = nc;
+      xcnt[c] = cnt - nc;
+    }
+    Sell cs, xs;
+    cs.build(ncv, ccnt);
+    xs.build(ncv, xcnt);
+    const std::vector<double>& nU = o.nU();
+    const std::vector<double>& cU = o.cU();
+    const std::vector<double>& nf = m->vor.n_fa;
+    std::vector<int> cnbr(cs.nslots, 0), cpair(cs.nslots, 0), xnbr(xs.nslots, 0);
+    std::vector<double> x6[6];
+    for (auto& v : x6) v.assign(xs.nslots, 0.0);
+    auto code_of = [&](int c, int pidx) {
+      const bool hi = (o.hi[pidx] == c);
+      return int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
+    };
+    for (int c = 0; c < ncv; ++c) {
+      for (int k = 0; k < ccnt[c]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[c] + k];
+        const int64_t sidx = cs.slot(c, k);
+        cnbr[sidx] = code_of(c, pidx);
+        cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == c) ? 0x80000000u : 0u));
+      }
+      for (int k = 0; k < xcnt[c]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
+        const int64_t sidx = xs.slot(c, k);
+        xnbr[sidx] = code_of(c, pidx);
+        for (int d = 0; d < 3; ++d) {
+          x6[d][sidx] = nU[3 * (size_t)pidx + d];
+          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+        }
+      }
+    }
+    // compact pairs (pair index = face index): per-pair SoA for the thread-per-pair kernel
+    {
+      std::vector<int> plo(nfa), phi(nfa);
+      std::vector<double> p14[14];
+      for (auto& v : p14) v.resize(nfa);
+      for (int f = 0; f < nfa; ++f) {
+        plo[f] = o.lo[f];
+        phi[f] = o.hi[f];
+        for (int d = 0; d < 3; ++d) {
+          p14[d][f] = nU[3 * (size_t)f + d];
+          p14[3 + d][f] = cU[3 * (size_t)f + d];
+          p14[6 + d][f] = o.unit_n_use[3 * (size_t)f + d];
+          p14[11 + d][f] = nf[3 * (size_t)f + d];
+        }
+        p14[9][f] = o.area_use[f];
+        p14[10][f] = o.aod_use[f];
+      }
+      rc |= upload(s, h, &g.plo, plo);
+      rc |= upload(s, h, &g.phi, phi);
+      const R** pdst[14] = {&g.pn0, &g.pn1, &g.pn2, &g.pc0, &g.pc1, &g.pc2, &g.pu0,
+                            &g.pu1, &g.pu2, &g.parea, &g.paod, &g.pf0, &g.pf1, &g.pf2};
+      for (int q = 0; q < 14; ++q) rc |= upload_as<R>(s, h, pdst[q], p14[q]);
+    }
+    // per-block halos of the extended half-faces for LDS staging
+    {
+      const int nblk = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
+      std::vector<std::vector<int>> halos(nblk);
+      std::vector<int16_t> loc(xs.nslots, 0);
+      std::string herr;
+      std::mutex hmu;
+      const int nth = cgmesh::default_threads();
+      std::vector<std::thread> th;
+      for (int t = 0; t < nth; ++t)
+        th.emplace_back([&, t] {
+          std::vector<int> v;
+          for (int bk = t; bk < nblk; bk += nth) {
+            const int c0 = bk * cgdev::kBlock, c1 = std::min(ncv, c0 + cgdev::kBlock);
+            v.clear();
+            for (int c = c0; c < c1; ++c) {
+              v.push_back(c);
+              for (int k = 0; k < xcnt[c]; ++k) v.push_back(xnbr[xs.slot(c, k)] & 0x7fffffff);
+            }
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            if (v.size() >= 0x8000) {
+              std::lock_guard<std::mutex> lk(hmu);
+              herr = "block halo too large";
+              return;
+            }
+            for (int c = c0; c < c1; ++c)
+              for (int k = 0; k < xcnt[c]; ++k) {
+                const int64_t sidx = xs.slot(c, k);
+                const int nb = xnbr[sidx] & 0x7fffffff;
+                const int pos = int(std::lower_bound(v.begin(), v.end(), nb) - v.begin());
+                loc[sidx] = int16_t(pos | ((xnbr[sidx] < 0) ? 0x8000 : 0));
+              }
+            halos[bk] = v;
+          }
+        });
+      for (auto& x : th) x.join();
+      int hmax = 0;
+      std::vector<int> hptr(nblk + 1, 0), hv;
+      for (int bk = 0; bk < nblk; ++bk) {
+        hptr[bk + 1] = hptr[bk] + int(halos[bk].size());
+        hmax = std::max(hmax, int(halos[bk].size()));
+      }
+      hv.reserve(hptr[nblk]);
+      for (auto& hh : halos) hv.insert(hv.end(), hh.begin(), hh.end());
+      int dev = 0, lds_max = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      // 6 fields of R per halo cell
+      const bool use_lds = allow_lds && herr.empty() && (size_t)hmax * 6 * sizeof(R) <= (size_t)lds_max;
+      if (use_lds) {
+        rc |= upload(s, h, &g.halo_ptr, hptr);
+        rc |= upload(s, h, &g.halo_v, hv);
+        rc |= upload(s, h, &g.xf_loc, loc);
+        g.halo_max = hmax;
+        h->lds = (size_t)hmax * 6 * sizeof(R);
+      }
+    }
+    rc |= upload(s, h, &g.cf_ptr, cs.ptr);
+    rc |= upload(s, h, &g.ef_nc, ccnt);
+    rc |= upload(s, h, &g.cf_nbr, cnbr);
+    rc |= upload(s, h, &g.cf_pair, cpair);
+    rc |= upload(s, h, &g.xf_ptr, xs.ptr);
+    rc |= upload(s, h, &g.xf_deg, xcnt);
+    rc |= upload(s, h, &g.xf_nbr, xnbr);
+    const R** xdst[6] = {&g.xsn0, &g.xsn1, &g.xsn2, &g.xsc0, &g.xsc1, &g.xsc2};
+    for (int q = 0; q < 6; ++q) rc |= upload_as<R>(s, h, xdst[q], x6[q]);
+    if (rc) return rc;
+  }
+  // gradient stencil without the diagonal
+  {
+    std::vector<int> cnt(ncv);
+    for (int c = 0; c < ncv; ++c) cnt[c] = o.cvocv_i[c + 1] - o.cvocv_i[c] - 1;
+    Sell sl;
+    sl.build(ncv, cnt);
+    std::vector<int> nbr(sl.nslots, 0);
+    std::vector<double> c0(sl.nslots, 0.0), c1(sl.nslots, 0.0), c2(sl.nslots, 0.0);
+    for (int c = 0; c < ncv; ++c)
+      for (int k = 0; k < cnt[c]; ++k) {
+        const int coc = o.cvocv_i[c] + 1 + k;
+        const int64_t sidx = sl.slot(c, k);
+        nbr[sidx] = o.cvocv_v[coc];
+        c0[sidx] = o.grad_coeff[3 * (size_t)coc];
+        c1[sidx] = o.grad_coeff[3 * (size_t)coc + 1];
+        c2[sidx] = o.grad_coeff[3 * (size_t)coc + 2];
+      }
+    rc |= upload(s, h, &g.gr_ptr, sl.ptr);
+    rc |= upload(s, h, &g.gr_deg, sl.deg);
+    rc |= upload(s, h, &g.gr_nbr, nbr);
+    rc |= upload_as<R>(s, h, &g.gc0, c0);
+    rc |= upload_as<R>(s, h, &g.gc1, c1);
+    rc |= upload_as<R>(s, h, &g.gc2, c2);
+    if (rc) return rc;
+  }
+  // state buffers
+  for (int k = 0; k < 2; ++k) {
+    rc |= dev_alloc(s, h, &b.st[k].light, (size_t)ncv * cgdev::L_N);
+    if (mixed) {
+      rc |= dev_alloc(s, h, &b.st[k].cons, 5 * (size_t)ncv);
+      if (!rc) b.st[k].rhoE = b.st[k].cons + 4 * (size_t)ncv;
+    } else {
+      rc |= dev_alloc(s, h, &b.st[k].rhoE, ncv);
+    }
+  }
+  cgdev::WorkT<R>& w = b.w;
+  rc |= dev_alloc(s, h, &w.aux, (size_t)ncv * cgdev::A_N);
+  if (!mixed) rc |= dev_alloc(s, h, &w.logp, ncv);
+  double** wf[] = {&w.T, &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
+  for (auto pp : wf) rc |= dev_alloc(s, h, pp, ncv);
+  rc |= dev_alloc(s, h, &w.rhs0, 5 * (size_t)ncv);
+  rc |= dev_alloc(s, h, &w.rhs1, 5 * (size_t)ncv);
+  rc |= dev_alloc(s, h, &w.cflux, 6 * (size_t)std::max(nfa, 1));
+  rc |= dev_alloc(s, h, &h->info, 1);
+  w.info = h->info;
+  rc |= dev_alloc(s, h, &w.partial, 8 * (size_t)h->nblocks);
+  rc |= dev_alloc(s, h, &w.diag, 8);
+  h->snap_n = mixed ? (size_t)ncv * 5 : (size_t)ncv * (cgdev::L_N + 1);
+  rc |= dev_alloc(s, h, &h->store_buf, h->snap_n);
+  rc |= dev_alloc(s, h, &h->store_recent, h->snap_n);
+  rc |= dev_alloc(s, h, &h->rhs_tmp, 5 * (size_t)ncv);
+  // one-entry table and XCD-ordered block map for launches over this instance alone
+  rc |= dev_alloc(s, h, &I->solo, 1);
+  rc |= dev_alloc(s, h, &I->solo_bmap, (size_t)h->nblocks);
+  rc |= dev_alloc(s, h, &I->solo_pbmap, (size_t)std::max(h->npblocks, 1));
+  if (rc) return rc;
+  {
+    const cgdev::StepInfo zero{};
+    HIPCHK(s, hipMemcpy(h->info, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    const std::vector<int> order = xcd_order(h->nblocks);
+    HIPCHK(s, hipMemcpy(I->solo_bmap, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    const std::vector<int> porder = xcd_order(h->npblocks);
+    if (!porder.empty())
+      HIPCHK(s, hipMemcpy(I->solo_pbmap, porder.data(), sizeof(int) * porder.size(), hipMemcpyHostToDevice));
+  }
+  // IdealGasSolver::initData reference state (IdealGasSolver.cpp:58-62)
+  for (int k = 0; k < 2; ++k) {
+    if (mixed) {
+      std::vector<double> cons(5 * (size_t)ncv, 0.0);
+      for (int c = 0; c < ncv; ++c) {
+        cons[c] = p->rho_ref;
+        cons[4 * (size_t)ncv + c] = p->p_ref / (p->gamma - 1.0);
+      }
+      HIPCHK(s, hipMemcpy(b.st[k].cons, cons.data(), sizeof(double) * cons.size(), hipMemcpyHostToDevice));
+      rc |= cgk::launch_fill(b.st[k].light, R(0), (int64_t)ncv * cgdev::L_N, stream);
+    } else {
+      std::vector<double> light((size_t)ncv * cgdev::L_N, 0.0), rhoE((size_t)ncv, p->p_ref / (p->gamma - 1.0));
+      for (int c = 0; c < ncv; ++c) light[(size_t)c * cgdev::L_N + cgdev::L_RHO] = p->rho_ref;
+      HIPCHK(s, hipMemcpy(b.st[k].light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice));
+      HIPCHK(s, hipMemcpy(b.st[k].rhoE, rhoE.data(), sizeof(double) * rhoE.size(), hipMemcpyHostToDevice));
+    }
+  }
+  rc |= cgk::launch_fill(w.aux, R(0), (int64_t)ncv * cgdev::A_N, stream);
+  rc |= cgk::launch_fill(w.rhs0, R(0), 5 * (int64_t)ncv, stream);
+  rc |= cgk::launch_fill(w.rhs1, R(0), 5 * (int64_t)ncv, stream);
+  if (rc) return s->set(CG_EHIP, "initial fill failed");
+  HIPCHK(s, hipStreamSynchronize(stream));
+  return upload_solo(s, I);
+}
+
+// ---- engine: the fused launch over all instances ----
+
+template <class R>
+void drop_graph(Engine<R>& E) {
+  for (auto& gx : E.graph) {
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+  }
+}
+
+// (re)build the device instance table and block map after a change of the set
+template <class R>
+int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
+  if (!E.dirty) return 0;
+  drop_graph(E);
+  const int n = (int)E.inst.size();
+  std::vector<cgdev::Inst<R>> tab(n);
+  std::vector<int> logical, plogical;  // logical block -> (instance, local block)
+  size_t lds = 0;
+  bool all_halo = true;
+  for (int i = 0; i < n; ++i) {
+    Instance<R>* I = E.inst[i];
+    tab[i] = I->record();
+    for (int b = 0; b < I->nblocks; ++b) logical.push_back((i << cgdev::kBlkBits) | b);
+    for (int b = 0; b < I->npblocks; ++b) plogical.push_back((i << cgdev::kBlkBits) | b);
+    lds = std::max(lds, I->lds);
+    all_halo = all_halo && I->lds > 0;
+  }
+  auto ordered = [](const std::vector<int>& lg) {
+    std::vector<int> out(lg.size());
+    const std::vector<int> order = xcd_order((int)lg.size());
+    for (size_t w = 0; w < lg.size(); ++w) out[w] = lg[order[w]];
+    return out;
+  };
+  const std::vector<int> bmap = ordered(logical), pbmap = ordered(plogical);
+  const int nb = (int)bmap.size(), npb = (int)pbmap.size();
+  HIPCHK(s, hipStreamSynchronize(stream));
+  if (n > E.tab_cap) {
+    if (E.tab) (void)hipFree(E.tab);
+    E.tab = nullptr;
+    E.tab_cap = std::max(n, 2 * E.tab_cap);
+    HIPCHK(s, hipMalloc(&E.tab, sizeof(cgdev::Inst<R>) * E.tab_cap));
+  }
+  if (nb > E.bmap_cap) {
+    if (E.bmap) (void)hipFree(E.bmap);
+    E.bmap = nullptr;
+    E.bmap_cap = std::max(nb, 2 * E.bmap_cap);
+    HIPCHK(s, hipMalloc(&E.bmap, sizeof(int) * E.bmap_cap));
+  }
+  if (npb > E.pbmap_cap) {
+    if (E.pbmap) (void)hipFree(E.pbmap);
+    E.pbmap = nullptr;
+    E.pbmap_cap = std::max(npb, 2 * E.pbmap_cap);
+    HIPCHK(s, hipMalloc(&E.pbmap, sizeof(int) * E.pbmap_cap));
+  }
+  if (n) HIPCHK(s, hipMemcpy(E.tab, tab.data(), sizeof(cgdev::Inst<R>) * n, hipMemcpyHostToDevice));
+  if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
+  // the LDS variant of the extended pass needs a halo on every instance
+  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb};
+  E.dirty = false;
   return 0;
 }
 
-int launch_one_step(cg_solver* s) {
-  if (cgk::launch_begin_step(s->w, s->P, s->stream)) return s->set(CG_EHIP, "begin_step launch failed");
+// one SSP-RK3 step of every instance (no host synchronisation)
+template <class R>
+int engine_launch_step(Err* s, Engine<R>& E, hipStream_t stream) {
+  const cgk::Launch<R>& L = E.L;
+  if (L.nblk == 0) return 0;
+  if (cgk::launch_begin_step(L, stream)) return s->set(CG_EHIP, "begin_step launch failed");
   for (int stage = 1; stage <= 3; ++stage) {
-    const int nxt = 1 - s->cur;
-    if (cgk::launch_flux(stage, 1, s->g, s->st[s->cur], s->st[nxt], s->w, s->P, nullptr, s->stream))
-      return s->set(CG_EHIP, "flux launch failed");
-    s->cur = nxt;
-    if (cgk::launch_grad(s->g, s->st[s->cur], s->w, s->stream)) return s->set(CG_EHIP, "grad launch failed");
+    if (cgk::launch_flux(stage, 1, L, E.cur, nullptr, stream)) return s->set(CG_EHIP, "flux launch failed");
+    E.cur ^= 1;
+    if (cgk::launch_grad(L, E.cur, stream)) return s->set(CG_EHIP, "grad launch failed");
   }
-  if (cgk::launch_props(s->g, s->st[s->cur], s->w, s->P, 0, s->stream)) return s->set(CG_EHIP, "props launch failed");
+  if (cgk::launch_props(L, E.cur, 0, stream)) return s->set(CG_EHIP, "props launch failed");
   return 0;
+}
+
+// nsteps steps, replayed from a captured graph of two steps when use_graph
+// (three stages flip the parity, so two steps return to it)
+template <class R>
+int engine_steps(Err* s, Engine<R>& E, hipStream_t stream, int nsteps, bool use_graph) {
+  if (int rc = engine_sync(s, E, stream)) return rc;
+  int done = 0;
+  if (use_graph && nsteps >= 2) {
+    hipGraphExec_t& gx = E.graph[E.cur];
+    if (!gx) {
+      const int cur0 = E.cur;
+      hipGraph_t gr = nullptr;
+      HIPCHK(s, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int rc = engine_launch_step(s, E, stream);
+      if (!rc) rc = engine_launch_step(s, E, stream);
+      const hipError_t ce = hipStreamEndCapture(stream, &gr);
+      E.cur = cur0;
+      if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      if (ce != hipSuccess) return s->set(CG_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (ie != hipSuccess) {
+        gx = nullptr;
+        return s->set(CG_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      }
+    }
+    for (; done + 2 <= nsteps; done += 2) HIPCHK(s, hipGraphLaunch(gx, stream));
+  }
+  for (; done < nsteps; ++done)
+    if (int rc = engine_launch_step(s, E, stream)) return rc;
+  return 0;
+}
+
+// ---- one instance ----
+
+// initComplete / restore: primitives, gradient and properties of the current state
+template <class R>
+int refresh(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
+  const cgk::Launch<R> L = I->solo_launch();
+  if (cgk::launch_eos(L, cur, stream)) return s->set(CG_EHIP, "eos launch failed");
+  if (cgk::launch_grad(L, cur, stream)) return s->set(CG_EHIP, "grad launch failed");
+  if (cgk::launch_props(L, cur, 1, stream)) return s->set(CG_EHIP, "props launch failed");
+  I->ready = true;
+  return 0;
+}
+
+// snapshot layout: FP64, light records [ncv][8] followed by rhoE [ncv]; mixed
+// precision, the FP64 state cons[5][ncv] (the records are recomputed on restore)
+template <class R>
+int snapshot_copy(Err* s, Instance<R>* I, int cur, double* dst, bool to_snapshot, hipStream_t stream) {
+  const auto dir = hipMemcpyDeviceToDevice;
+  const cgdev::StateT<R>& st = I->b.st[cur];
+  if constexpr (std::is_same<R, float>::value) {
+    const size_t n = (size_t)I->ncv * 5;
+    HIPCHK(s, hipMemcpyAsync(to_snapshot ? dst : st.cons, to_snapshot ? st.cons : dst, sizeof(double) * n, dir, stream));
+  } else {
+    const size_t nl = (size_t)I->ncv * cgdev::L_N;
+    if (to_snapshot) {
+      HIPCHK(s, hipMemcpyAsync(dst, st.light, sizeof(double) * nl, dir, stream));
+      HIPCHK(s, hipMemcpyAsync(dst + nl, st.rhoE, sizeof(double) * I->ncv, dir, stream));
+    } else {
+      HIPCHK(s, hipMemcpyAsync(st.light, dst, sizeof(double) * nl, dir, stream));
+      HIPCHK(s, hipMemcpyAsync(st.rhoE, dst + nl, sizeof(double) * I->ncv, dir, stream));
+    }
+  }
+  return 0;
+}
+
+int read_info(Err* s, HostInst* I, cgdev::StepInfo* host, hipStream_t stream) {
+  HIPCHK(s, hipMemcpyAsync(host, I->info, sizeof(cgdev::StepInfo), hipMemcpyDeviceToHost, stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
+  return 0;
+}
+
+int write_info(Err* s, HostInst* I, const cgdev::StepInfo* host, hipStream_t stream) {
+  HIPCHK(s, hipMemcpyAsync(I->info, host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice, stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
+  return 0;
+}
+
+// restoreData (IdealGasSolver.hpp:377-405): the older snapshot, dt scaled by 0.8
+template <class R>
+int restore(Err* s, Engine<R>& E, Instance<R>* I, cgdev::StepInfo* host, hipStream_t stream) {
+  if (!I->have_store) return s->set(CG_ESTATE, "nothing stored");
+  if (int rc = snapshot_copy(s, I, E.cur, I->store_buf, false, stream)) return rc;
+  if (int rc = read_info(s, I, host, stream)) return rc;
+  host->nan = 0;
+  host->step = I->store_step;
+  host->time = I->store_time;
+  if (int rc = write_info(s, I, host, stream)) return rc;
+  I->host_step = I->store_step;
+  I->prm.dt_data[0] *= 0.8;
+  fill_dev_params(I);
+  if (int rc = upload_solo(s, I)) return rc;
+  E.dirty = true;  // the fused table holds P too
+  return refresh(s, I, E.cur, stream);
 }
 
 // checkForNans + restoreData (IdealGasSolver.hpp:377-405,596-611)
-int check_nan_and_restore(cg_solver* s, bool* rolled_back) {
-  if (int rc = read_info(s)) return rc;
+template <class R>
+int check_nan_and_restore(Err* s, Engine<R>& E, Instance<R>* I, cgdev::StepInfo* host, hipStream_t stream,
+                          bool* rolled_back) {
+  if (int rc = read_info(s, I, host, stream)) return rc;
   *rolled_back = false;
-  if (!s->info_host->nan) {
-    s->host_step = s->info_host->step;
+  if (!host->nan) {
+    I->host_step = host->step;
     return 0;
   }
-  if (!s->have_store) return s->set(CG_ENAN, "NaN detected and no stored solution to restore");
-  if (int rc = snapshot_copy(s, s->store_buf, false)) return rc;
-  s->info_host->nan = 0;
-  s->info_host->step = s->store_step;
-  s->info_host->time = s->store_time;
-  HIPCHK(s, hipMemcpyAsync(s->w.info, s->info_host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice, s->stream));
-  s->prm.dt_data[0] *= 0.8;
-  fill_dev_params(s);
-  s->host_step = s->store_step;
-  if (int rc = refresh(s)) return rc;
+  if (!I->have_store) return s->set(CG_ENAN, "NaN detected and no stored solution to restore");
+  if (int rc = restore(s, E, I, host, stream)) return rc;
   *rolled_back = true;
   return 0;
 }
 
-int do_store(cg_solver* s) {
-  if (int rc = read_info(s)) return rc;
-  if (s->info_host->nan) return 0;  // never snapshot a NaN state
-  if (s->have_store) {
-    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * s->snap_n, hipMemcpyDeviceToDevice, s->stream));
-    s->store_step = s->store_step_recent;
-    s->store_time = s->store_time_recent;
+template <class R>
+int do_store(Err* s, Instance<R>* I, int cur, cgdev::StepInfo* host, hipStream_t stream) {
+  if (int rc = read_info(s, I, host, stream)) return rc;
+  if (host->nan) return 0;  // never snapshot a NaN state
+  if (I->have_store) {
+    HIPCHK(s, hipMemcpyAsync(I->store_buf, I->store_recent, sizeof(double) * I->snap_n, hipMemcpyDeviceToDevice, stream));
+    I->store_step = I->store_step_recent;
+    I->store_time = I->store_time_recent;
   }
-  if (int rc = snapshot_copy(s, s->store_recent, true)) return rc;
-  s->store_step_recent = s->info_host->step;
-  s->store_time_recent = s->info_host->time;
-  if (!s->have_store) {
-    HIPCHK(s, hipMemcpyAsync(s->store_buf, s->store_recent, sizeof(double) * s->snap_n, hipMemcpyDeviceToDevice, s->stream));
-    s->store_step = s->store_step_recent;
-    s->store_time = s->store_time_recent;
-    s->have_store = true;
+  if (int rc = snapshot_copy(s, I, cur, I->store_recent, true, stream)) return rc;
+  I->store_step_recent = host->step;
+  I->store_time_recent = host->time;
+  if (!I->have_store) {
+    HIPCHK(s, hipMemcpyAsync(I->store_buf, I->store_recent, sizeof(double) * I->snap_n, hipMemcpyDeviceToDevice, stream));
+    I->store_step = I->store_step_recent;
+    I->store_time = I->store_time_recent;
+    I->have_store = true;
   }
-  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
   return 0;
 }
 
-int get_soa(cg_solver* s, double* const dev[], int ncomp, double* out) {
-  std::vector<double> tmp((size_t)s->ncv);
+// state in: FP64 light records + rhoE, or the FP64 cons array
+template <class R>
+int set_state(Err* s, Instance<R>* I, int cur, const double* rho, const double* u, const double* rhoE,
+              hipStream_t stream) {
+  HIPCHK(s, hipStreamSynchronize(stream));
+  const size_t n = (size_t)I->ncv;
+  const cgdev::StateT<R>& st = I->b.st[cur];
+  if constexpr (std::is_same<R, float>::value) {
+    std::vector<double> cons(5 * n);
+    for (size_t c = 0; c < n; ++c) {
+      cons[c] = rho[c];
+      for (int i = 0; i < 3; ++i) cons[(1 + i) * n + c] = u[3 * c + i];
+      cons[4 * n + c] = rhoE[c];
+    }
+    HIPCHK(s, hipMemcpy(st.cons, cons.data(), sizeof(double) * cons.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> light(n * cgdev::L_N, 0.0);
+    for (size_t c = 0; c < n; ++c) {
+      double* L = &light[c * cgdev::L_N];
+      L[cgdev::L_U0] = u[3 * c];
+      L[cgdev::L_U1] = u[3 * c + 1];
+      L[cgdev::L_U2] = u[3 * c + 2];
+      L[cgdev::L_RHO] = rho[c];
+    }
+    HIPCHK(s, hipMemcpy(st.light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(st.rhoE, rhoE, sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  I->ready = false;
+  return 0;
+}
+
+// copy rows of a device SoA array (row stride ncv) to out[ncv][ncomp]
+int get_soa(Err* s, int ncv, double* const dev[], int ncomp, double* out, hipStream_t stream) {
+  std::vector<double> tmp((size_t)ncv);
   for (int q = 0; q < ncomp; ++q) {
-    HIPCHK(s, hipMemcpyAsync(tmp.data(), dev[q], sizeof(double) * s->ncv, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
-    for (int c = 0; c < s->ncv; ++c) out[(size_t)c * ncomp + q] = tmp[c];
+    HIPCHK(s, hipMemcpyAsync(tmp.data(), dev[q], sizeof(double) * ncv, hipMemcpyDeviceToHost, stream));
+    HIPCHK(s, hipStreamSynchronize(stream));
+    for (int c = 0; c < ncv; ++c) out[(size_t)c * ncomp + q] = tmp[c];
   }
   return 0;
 }
 
 // copy fields [f0, f0+nf) of a device record array [ncv][stride] to out[ncv][nf]
-int get_rec(cg_solver* s, const double* rec, int stride, int f0, int nf, double* out) {
-  std::vector<double> tmp((size_t)s->ncv * stride);
-  HIPCHK(s, hipMemcpyAsync(tmp.data(), rec, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(s, hipStreamSynchronize(s->stream));
-  for (int c = 0; c < s->ncv; ++c)
-    for (int q = 0; q < nf; ++q) out[(size_t)c * nf + q] = tmp[(size_t)c * stride + f0 + q];
+template <class T>
+int get_rec(Err* s, int ncv, const T* rec, int stride, int f0, int nf, double* out, hipStream_t stream) {
+  std::vector<T> tmp((size_t)ncv * stride);
+  HIPCHK(s, hipMemcpyAsync(tmp.data(), rec, sizeof(T) * tmp.size(), hipMemcpyDeviceToHost, stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
+  for (int c = 0; c < ncv; ++c)
+    for (int q = 0; q < nf; ++q) out[(size_t)c * nf + q] = double(tmp[(size_t)c * stride + f0 + q]);
+  return 0;
+}
+
+template <class R>
+int get_state(Err* s, Instance<R>* I, int cur, double* rho, double* u, double* rhoE, hipStream_t stream) {
+  const cgdev::StateT<R>& st = I->b.st[cur];
+  const int ncv = I->ncv;
+  if constexpr (std::is_same<R, float>::value) {
+    const size_t n = (size_t)ncv;
+    double* cons = st.cons;
+    if (rho) {
+      double* f[1] = {cons};
+      if (int rc = get_soa(s, ncv, f, 1, rho, stream)) return rc;
+    }
+    if (u) {
+      double* f[3] = {cons + n, cons + 2 * n, cons + 3 * n};
+      if (int rc = get_soa(s, ncv, f, 3, u, stream)) return rc;
+    }
+    if (rhoE) {
+      double* f[1] = {cons + 4 * n};
+      if (int rc = get_soa(s, ncv, f, 1, rhoE, stream)) return rc;
+    }
+  } else {
+    if (rho)
+      if (int rc = get_rec(s, ncv, st.light, cgdev::L_N, cgdev::L_RHO, 1, rho, stream)) return rc;
+    if (u)
+      if (int rc = get_rec(s, ncv, st.light, cgdev::L_N, cgdev::L_U0, 3, u, stream)) return rc;
+    if (rhoE) {
+      double* f[1] = {st.rhoE};
+      if (int rc = get_soa(s, ncv, f, 1, rhoE, stream)) return rc;
+    }
+  }
+  return 0;
+}
+
+template <class R>
+int calc_cfl(Err* s, Instance<R>* I, int cur, double dt, double* cfl_max, hipStream_t stream) {
+  if (cgk::launch_cfl(I->solo_launch(), cur, dt, stream)) return s->set(CG_EHIP, "cfl launch failed");
+  std::vector<double> c((size_t)I->ncv);
+  double* f[1] = {I->b.w.cfl};
+  if (int rc = get_soa(s, I->ncv, f, 1, c.data(), stream)) return rc;
+  double mx = 0.0;
+  for (double v : c) mx = (mx < v) ? v : mx;
+  *cfl_max = mx;
+  return 0;
+}
+
+template <class R>
+int get_diag(Err* s, Instance<R>* I, int cur, cgdev::StepInfo* host, cg_diag* d, hipStream_t stream) {
+  if (cgk::launch_diag(I->solo_launch(), cur, stream)) return s->set(CG_EHIP, "diag launch failed");
+  double v[8];
+  HIPCHK(s, hipMemcpyAsync(v, I->b.w.diag, sizeof(v), hipMemcpyDeviceToHost, stream));
+  if (int rc = read_info(s, I, host, stream)) return rc;
+  d->step = host->step;
+  d->time = host->time;
+  d->dt = host->dt;
+  d->ke = v[5] / v[7];
+  d->enstrophy = v[6] / v[7];
+  for (int q = 0; q < 5; ++q) d->sums[q] = v[q];
+  d->nan_flag = host->nan;
+  double cm = 0.0;
+  if (int rc = calc_cfl(s, I, cur, d->dt, &cm, stream)) return rc;
+  d->cfl_max = cm;
+  return 0;
+}
+
+// instance creation: device layout, reference initial state, then optional host state
+template <class R>
+int make_instance(Err* s, const cg_mesh* m, const cg_params* p, bool allow_lds, hipStream_t stream,
+                  Instance<R>** out) {
+  Instance<R>* I = new Instance<R>();
+  const int rc = build_device(s, I, m, p, allow_lds, stream);
+  if (rc) {
+    free_instance(I);
+    return rc;
+  }
+  *out = I;
   return 0;
 }
 
@@ -423,6 +1057,11 @@ int cg_params_parse(cg_params* p, const char* text) {
       p->disable_grouping = 1;
     } else if (key == "ZERO_C_EF") {
       p->zero_c_ef = 1;
+    } else if (key == "PRECISION") {
+      const std::string m = tok.size() > 1 ? upper(tok[1]) : "";
+      if (m == "FP64") p->precision = CG_PREC_FP64;
+      else if (m == "MIXED") p->precision = CG_PREC_MIXED;
+      else return bad("expected PRECISION FP64 or MIXED");
     } else if (key == "EF_OPS") {
       if (tok.size() < 2 || upper(tok[1]) != "DEFAULT") return bad("only EF_OPS DEFAULT is available");
     } else if (key == "USE_FULL_GR" || key == "DISS_SPONGE" || key == "USE_COMPACT_FACES" || key == "DVOL_FORM" ||
@@ -537,19 +1176,81 @@ int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes) {
 
 void cg_mesh_destroy(cg_mesh* m) { delete m; }
 
+int cg_ic_tgv(const cg_mesh* m, double Ma, double gamma, double* rho, double* u, double* rhoE) {
+  if (!m || !rho || !u || !rhoE) return fail(CG_EINVAL, "null argument");
+  if (!(Ma > 0.0) || !(gamma > 1.0)) return fail(CG_EINVAL, "cg_ic_tgv: need Ma > 0 and gamma > 1");
+  cgmesh::ic_tgv(m->op.ncv, m->vor.x_cv.data(), Ma, gamma, rho, u, rhoE);
+  return CG_OK;
+}
+
+int cg_ic_k4(const cg_mesh* m, double k0, double Mt, double gamma, uint64_t seed, int kmax, double* rho, double* u,
+             double* rhoE) {
+  if (!m || !rho || !u || !rhoE) return fail(CG_EINVAL, "null argument");
+  if (!(k0 > 0.0) || !(Mt > 0.0) || !(gamma > 1.0)) return fail(CG_EINVAL, "cg_ic_k4: need k0 > 0, Mt > 0, gamma > 1");
+  const std::string e = cgmesh::ic_k4(m->op.ncv, m->vor.x_cv.data(), k0, Mt, gamma, seed, kmax, rho, u, rhoE,
+                                      cgmesh::default_threads());
+  if (!e.empty()) return fail(CG_EINVAL, "cg_ic_k4: " + e);
+  return CG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// call fn(engine, instance) for the solver's precision
+template <class F>
+int with_inst(cg_solver* s, F&& fn) {
+  return s->mixed ? fn(s->ef, s->ix) : fn(s->ed, s->id);
+}
+
+template <class F>
+int with_engine(cg_batch* b, F&& fn) {
+  return b->mixed ? fn(b->ef) : fn(b->ed);
+}
+
+template <class R>
+Instance<R>* find_inst(Engine<R>& E, int id) {
+  for (Instance<R>* I : E.inst)
+    if (I->id == id) return I;
+  return nullptr;
+}
+
+// the step loop of one instance: steps with NaN checks at CHECK_INTERVAL and
+// snapshots every 20 intervals (FlowSolver.hpp:2500-2528, IdealGasSolver.hpp:596-611)
+template <class R>
+int solver_steps(cg_solver* s, Engine<R>& E, Instance<R>* I, int nsteps) {
+  const int ci = std::max(1, I->prm.check_interval);
+  int it = 0;
+  while (it < nsteps) {
+    // run up to the next check point as one chunk
+    const int64_t step0 = I->host_step;
+    const int64_t next_check = (step0 / ci + 1) * ci;
+    const int chunk = (int)std::min<int64_t>(nsteps - it, next_check - step0);
+    if (int rc = engine_steps(s, E, s->stream, chunk, false)) return rc;
+    it += chunk;
+    const int64_t step = step0 + chunk;
+    I->host_step = step;
+    const bool last = (it == nsteps);
+    if (step % ci == 0 || last) {
+      bool rolled = false;
+      if (int rc = check_nan_and_restore(s, E, I, s->info_host, s->stream, &rolled)) return rc;
+      if (!rolled && step % ci == 0 && step % (int64_t(ci) * 20) == 0)
+        if (int rc = do_store(s, I, E.cur, s->info_host, s->stream)) return rc;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
   if (!m || !out) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return rc;
-  cg_solver* s = new cg_solver();
   *out = nullptr;
-  s->prm = *p;
-  fill_dev_params(s);
-  const cgmesh::Operators& o = m->op;
-  const int ncv = o.ncv, nfa = o.nfa, nef = o.nef;
-  s->ncv = ncv;
-  s->nfa = nfa;
-  s->nef = nef;
-  s->nblocks = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
+  cg_solver* s = new cg_solver();
+  s->mixed = (p->precision == CG_PREC_MIXED);
   auto bail = [&](int rc) {
     std::string e = s->err;
     cg_destroy(s);
@@ -561,349 +1262,132 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(s->set(CG_EHIP, "stream creation failed"));
   if (hipHostMalloc((void**)&s->info_host, sizeof(cgdev::StepInfo)) != hipSuccess) return bail(s->set(CG_EHIP, "pinned alloc failed"));
   std::memset(s->info_host, 0, sizeof(cgdev::StepInfo));
-
-  cgdev::Geom& g = s->g;
-  g.ncv = ncv;
-  g.nfa = nfa;
-  g.nef = nef;
-  std::vector<double> dx13(ncv), dx23(ncv);
-  for (int c = 0; c < ncv; ++c) {
-    dx13[c] = std::pow(o.vol[c], 1.0 / 3.0);
-    dx23[c] = std::pow(o.vol[c], 2.0 / 3.0);
-  }
+  const bool lds = !std::getenv("CG_NO_LDS");
   int rc = 0;
-  rc |= upload(s, &g.inv_vol, o.inv_vol);
-  rc |= upload(s, &g.vol, o.vol);
-  rc |= upload(s, &g.sa_to_vol, o.sa_to_vol);
-  rc |= upload(s, &g.dcmag, o.dcmag_hat);
-  rc |= upload(s, &g.dx13, dx13);
-  rc |= upload(s, &g.dx23, dx23);
+  if (s->mixed) {
+    rc = make_instance(s, m, p, lds, s->stream, &s->ix);
+    if (!rc) s->ef.inst.push_back(s->ix);
+  } else {
+    rc = make_instance(s, m, p, lds, s->stream, &s->id);
+    if (!rc) s->ed.inst.push_back(s->id);
+  }
   if (rc) return bail(rc);
-
-  // half-faces (per-slot geometry): compact ones and purely extended ones in
-  // two SELL-64 regions, each starting at row 0
-  {
-    std::vector<int> ccnt(ncv), xcnt(ncv);
-    for (int c = 0; c < ncv; ++c) {
-      const int cnt = o.efocv_i[c + 1] - o.efocv_i[c];
-      int nc = 0;
-      while (nc < cnt && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
-      for (int k = nc; k < cnt; ++k)
-        if (o.efocv_v[o.efocv_i[c] + k] < nfa) return bail(s->set(CG_EMESH, "compact pairs must precede extended ones"));
-      ccnt[c] = nc;
-      xcnt[c] = cnt - nc;
-    }
-    Sell cs, xs;
-    cs.build(ncv, ccnt);
-    xs.build(ncv, xcnt);
-    const std::vector<double>& nU = o.nU();
-    const std::vector<double>& cU = o.cU();
-    const std::vector<double>& nf = m->vor.n_fa;
-    std::vector<int> cnbr(cs.nslots, 0), xnbr(xs.nslots, 0);
-    std::vector<double> c14[14];
-    for (auto& v : c14) v.assign(cs.nslots, 0.0);
-    std::vector<double> x6[6];
-    for (auto& v : x6) v.assign(xs.nslots, 0.0);
-    auto code_of = [&](int c, int pidx) {
-      const bool hi = (o.hi[pidx] == c);
-      return int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
-    };
-    for (int c = 0; c < ncv; ++c) {
-      for (int k = 0; k < ccnt[c]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[c] + k];
-        const int64_t sidx = cs.slot(c, k);
-        cnbr[sidx] = code_of(c, pidx);
-        for (int d = 0; d < 3; ++d) {
-          c14[d][sidx] = nU[3 * (size_t)pidx + d];
-          c14[3 + d][sidx] = cU[3 * (size_t)pidx + d];
-          c14[6 + d][sidx] = o.unit_n_use[3 * (size_t)pidx + d];
-          c14[11 + d][sidx] = nf[3 * (size_t)pidx + d];
-        }
-        c14[9][sidx] = o.area_use[pidx];
-        c14[10][sidx] = o.aod_use[pidx];
-      }
-      for (int k = 0; k < xcnt[c]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
-        const int64_t sidx = xs.slot(c, k);
-        xnbr[sidx] = code_of(c, pidx);
-        for (int d = 0; d < 3; ++d) {
-          x6[d][sidx] = nU[3 * (size_t)pidx + d];
-          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
-        }
-      }
-    }
-    // per-block halos of the extended half-faces for LDS staging
-    {
-      const int nblk = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
-      std::vector<std::vector<int>> halos(nblk);
-      std::vector<int16_t> loc(xs.nslots, 0);
-      std::string herr;
-      std::mutex hmu;
-      const int nth = cgmesh::default_threads();
-      std::vector<std::thread> th;
-      for (int t = 0; t < nth; ++t)
-        th.emplace_back([&, t] {
-          std::vector<int> v;
-          for (int b = t; b < nblk; b += nth) {
-            const int c0 = b * cgdev::kBlock, c1 = std::min(ncv, c0 + cgdev::kBlock);
-            v.clear();
-            for (int c = c0; c < c1; ++c) {
-              v.push_back(c);
-              for (int k = 0; k < xcnt[c]; ++k) v.push_back(xnbr[xs.slot(c, k)] & 0x7fffffff);
-            }
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            if (v.size() >= 0x8000) {
-              std::lock_guard<std::mutex> lk(hmu);
-              herr = "block halo too large";
-              return;
-            }
-            for (int c = c0; c < c1; ++c)
-              for (int k = 0; k < xcnt[c]; ++k) {
-                const int64_t sidx = xs.slot(c, k);
-                const int nb = xnbr[sidx] & 0x7fffffff;
-                const int pos = int(std::lower_bound(v.begin(), v.end(), nb) - v.begin());
-                loc[sidx] = int16_t(pos | ((xnbr[sidx] < 0) ? 0x8000 : 0));
-              }
-            halos[b] = v;
-          }
-        });
-      for (auto& x : th) x.join();
-      int hmax = 0;
-      std::vector<int> hptr(nblk + 1, 0), hv;
-      for (int b = 0; b < nblk; ++b) {
-        hptr[b + 1] = hptr[b] + int(halos[b].size());
-        hmax = std::max(hmax, int(halos[b].size()));
-      }
-      hv.reserve(hptr[nblk]);
-      for (auto& hh : halos) hv.insert(hv.end(), hh.begin(), hh.end());
-      int dev = 0, lds_max = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-      const bool use_lds = herr.empty() && (size_t)hmax * 48 <= (size_t)lds_max && !std::getenv("CG_NO_LDS");
-      if (use_lds) {
-        rc |= upload(s, &g.halo_ptr, hptr);
-        rc |= upload(s, &g.halo_v, hv);
-        rc |= upload(s, &g.xf_loc, loc);
-        g.halo_max = hmax;
-      }
-    }
-    rc |= upload(s, &g.cf_ptr, cs.ptr);
-    rc |= upload(s, &g.ef_nc, ccnt);
-    rc |= upload(s, &g.cf_nbr, cnbr);
-    rc |= upload(s, &g.csn0, c14[0]); rc |= upload(s, &g.csn1, c14[1]); rc |= upload(s, &g.csn2, c14[2]);
-    rc |= upload(s, &g.csc0, c14[3]); rc |= upload(s, &g.csc1, c14[4]); rc |= upload(s, &g.csc2, c14[5]);
-    rc |= upload(s, &g.cu0, c14[6]); rc |= upload(s, &g.cu1, c14[7]); rc |= upload(s, &g.cu2, c14[8]);
-    rc |= upload(s, &g.carea, c14[9]); rc |= upload(s, &g.caod, c14[10]);
-    rc |= upload(s, &g.cn0, c14[11]); rc |= upload(s, &g.cn1, c14[12]); rc |= upload(s, &g.cn2, c14[13]);
-    rc |= upload(s, &g.xf_ptr, xs.ptr);
-    rc |= upload(s, &g.xf_deg, xcnt);
-    rc |= upload(s, &g.xf_nbr, xnbr);
-    rc |= upload(s, &g.xsn0, x6[0]); rc |= upload(s, &g.xsn1, x6[1]); rc |= upload(s, &g.xsn2, x6[2]);
-    rc |= upload(s, &g.xsc0, x6[3]); rc |= upload(s, &g.xsc1, x6[4]); rc |= upload(s, &g.xsc2, x6[5]);
-    if (rc) return bail(rc);
-  }
-  // gradient stencil without the diagonal
-  {
-    std::vector<int> cnt(ncv);
-    for (int c = 0; c < ncv; ++c) cnt[c] = o.cvocv_i[c + 1] - o.cvocv_i[c] - 1;
-    Sell sl;
-    sl.build(ncv, cnt);
-    std::vector<int> nbr(sl.nslots, 0);
-    std::vector<double> c0(sl.nslots, 0.0), c1(sl.nslots, 0.0), c2(sl.nslots, 0.0);
-    for (int c = 0; c < ncv; ++c)
-      for (int k = 0; k < cnt[c]; ++k) {
-        const int coc = o.cvocv_i[c] + 1 + k;
-        const int64_t sidx = sl.slot(c, k);
-        nbr[sidx] = o.cvocv_v[coc];
-        c0[sidx] = o.grad_coeff[3 * (size_t)coc];
-        c1[sidx] = o.grad_coeff[3 * (size_t)coc + 1];
-        c2[sidx] = o.grad_coeff[3 * (size_t)coc + 2];
-      }
-    rc |= upload(s, &g.gr_ptr, sl.ptr);
-    rc |= upload(s, &g.gr_deg, sl.deg);
-    rc |= upload(s, &g.gr_nbr, nbr);
-    rc |= upload(s, &g.gc0, c0); rc |= upload(s, &g.gc1, c1); rc |= upload(s, &g.gc2, c2);
-    if (rc) return bail(rc);
-  }
-  // state buffers
-  for (int b = 0; b < 2; ++b) {
-    rc |= dev_alloc(s, &s->st[b].light, (size_t)ncv * cgdev::L_N);
-    rc |= dev_alloc(s, &s->st[b].rhoE, ncv);
-  }
-  cgdev::Work& w = s->w;
-  rc |= dev_alloc(s, &w.aux, (size_t)ncv * cgdev::A_N);
-  double** wf[] = {&w.logp, &w.T, &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
-  for (auto pp : wf) rc |= dev_alloc(s, pp, ncv);
-  rc |= dev_alloc(s, &w.rhs0, 5 * (size_t)ncv);
-  rc |= dev_alloc(s, &w.rhs1, 5 * (size_t)ncv);
-  rc |= dev_alloc(s, &w.racc, 5 * (size_t)ncv);
-  rc |= dev_alloc(s, &w.info, 1);
-  rc |= dev_alloc(s, &w.partial, 8 * (size_t)s->nblocks);
-  rc |= dev_alloc(s, &w.diag, 8);
-  s->snap_n = (size_t)ncv * (cgdev::L_N + 1);
-  rc |= dev_alloc(s, &s->store_buf, s->snap_n);
-  rc |= dev_alloc(s, &s->store_recent, s->snap_n);
-  rc |= dev_alloc(s, &s->rhs_tmp, 5 * (size_t)ncv);
+  rc = with_inst(s, [&](auto& E, auto* I) {
+    (void)I;
+    return engine_sync(s, E, s->stream);
+  });
   if (rc) return bail(rc);
-  if (hipMemcpy(w.info, s->info_host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice) != hipSuccess)
-    return bail(s->set(CG_EHIP, "info upload failed"));
-  // IdealGasSolver::initData reference state (IdealGasSolver.cpp:58-62)
-  {
-    std::vector<double> light((size_t)ncv * cgdev::L_N, 0.0), rhoE((size_t)ncv, p->p_ref / (p->gamma - 1.0));
-    for (int c = 0; c < ncv; ++c) light[(size_t)c * cgdev::L_N + cgdev::L_RHO] = p->rho_ref;
-    for (int b = 0; b < 2; ++b) {
-      if (hipMemcpy(s->st[b].light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(s->st[b].rhoE, rhoE.data(), sizeof(double) * rhoE.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return bail(s->set(CG_EHIP, "initial state upload failed"));
-    }
-  }
-  rc |= cgk::launch_fill(w.aux, 0.0, (int64_t)ncv * cgdev::A_N, s->stream);
-  rc |= cgk::launch_fill(w.rhs0, 0.0, 5 * (int64_t)ncv, s->stream);
-  rc |= cgk::launch_fill(w.rhs1, 0.0, 5 * (int64_t)ncv, s->stream);
-  if (rc || hipStreamSynchronize(s->stream) != hipSuccess) return bail(s->set(CG_EHIP, "initial fill failed"));
   *out = s;
   return CG_OK;
 }
 
 int cg_set_state(cg_solver* s, const double* rho, const double* u, const double* rhoE) {
   if (!s || !rho || !u || !rhoE) return fail(CG_EINVAL, "null argument");
-  const cgdev::State& st = s->st[s->cur];
-  HIPCHK(s, hipStreamSynchronize(s->stream));
-  std::vector<double> light((size_t)s->ncv * cgdev::L_N, 0.0);
-  for (int c = 0; c < s->ncv; ++c) {
-    double* L = &light[(size_t)c * cgdev::L_N];
-    L[cgdev::L_U0] = u[3 * (size_t)c];
-    L[cgdev::L_U1] = u[3 * (size_t)c + 1];
-    L[cgdev::L_U2] = u[3 * (size_t)c + 2];
-    L[cgdev::L_RHO] = rho[c];
-  }
-  HIPCHK(s, hipMemcpy(st.light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice));
-  HIPCHK(s, hipMemcpy(st.rhoE, rhoE, sizeof(double) * s->ncv, hipMemcpyHostToDevice));
-  s->ready = false;
-  return CG_OK;
+  const int rc = with_inst(s, [&](auto& E, auto* I) { return set_state(s, I, E.cur, rho, u, rhoE, s->stream); });
+  return rc ? rc : CG_OK;
 }
 
 int cg_get_state(cg_solver* s, double* rho, double* u, double* rhoE) {
   if (!s) return fail(CG_EINVAL, "null solver");
-  const cgdev::State& st = s->st[s->cur];
-  if (rho)
-    if (int rc = get_rec(s, st.light, cgdev::L_N, cgdev::L_RHO, 1, rho)) return rc;
-  if (u)
-    if (int rc = get_rec(s, st.light, cgdev::L_N, cgdev::L_U0, 3, u)) return rc;
-  if (rhoE) {
-    double* f[1] = {st.rhoE};
-    if (int rc = get_soa(s, f, 1, rhoE)) return rc;
-  }
-  return CG_OK;
+  const int rc = with_inst(s, [&](auto& E, auto* I) { return get_state(s, I, E.cur, rho, u, rhoE, s->stream); });
+  return rc ? rc : CG_OK;
 }
 
 int cg_init_complete(cg_solver* s) {
   if (!s) return fail(CG_EINVAL, "null solver");
-  s->info_host->cfl_bits = 0;
-  s->info_host->nan = 0;
-  HIPCHK(s, hipMemcpyAsync(s->w.info, s->info_host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice, s->stream));
-  if (int rc = refresh(s)) return rc;
-  return do_store(s);
+  return with_inst(s, [&](auto& E, auto* I) {
+    if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
+    s->info_host->cfl_bits = 0;
+    s->info_host->nan = 0;
+    if (int rc = write_info(s, I, s->info_host, s->stream)) return rc;
+    if (int rc = refresh(s, I, E.cur, s->stream)) return rc;
+    return do_store(s, I, E.cur, s->info_host, s->stream);
+  });
 }
 
 int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
   if (!s || !name || !out) return fail(CG_EINVAL, "null argument");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
   const std::string nm(name);
-  const cgdev::State& st = s->st[s->cur];
-  const cgdev::Work& w = s->w;
-  if (nm == "dudx") {
-    if (n != 9 * (int64_t)s->ncv) return s->set(CG_EINVAL, "dudx needs 9*ncv values");
-    return get_rec(s, w.aux, cgdev::A_N, cgdev::A_D, 9, out);
-  }
-  if (n != s->ncv) return s->set(CG_EINVAL, "field needs ncv values");
-  struct { const char* nm; int light; int f; } recf[] = {
-      {"p", 1, cgdev::L_P}, {"sos", 1, cgdev::L_SOS}, {"h", 1, cgdev::L_H}, {"sp_vol", 1, cgdev::L_SPV},
-      {"mu_total", 0, cgdev::A_MU}, {"loc_total", 0, cgdev::A_LOC}, {"fgr", 0, cgdev::A_FGR},
-      {"symh_fax", 0, cgdev::A_SYMH}, {"symp_fax", 0, cgdev::A_SYMP}};
-  for (auto& r : recf)
-    if (nm == r.nm) return r.light ? get_rec(s, st.light, cgdev::L_N, r.f, 1, out) : get_rec(s, w.aux, cgdev::A_N, r.f, 1, out);
-  // per-step diagnostic fields: recompute the properties of the current state
-  if (cgk::launch_props(s->g, st, s->w, s->P, 1, s->stream)) return s->set(CG_EHIP, "props launch failed");
-  double* src = nullptr;
-  if (nm == "T") src = w.T;
-  else if (nm == "mu_lam") src = w.mu_lam;
-  else if (nm == "mu_sgs") src = w.mu_sgs;
-  else if (nm == "loc_lam") src = w.loc_lam;
-  else if (nm == "loc_sgs") src = w.loc_sgs;
-  else if (nm == "a_sgs") src = w.a_sgs;
-  else if (nm == "vv2") src = w.vv2;
-  else if (nm == "cfl") {
-    if (read_info(s)) return CG_EHIP;
-    if (cgk::launch_cfl(s->g, st, s->w, s->info_host->dt, s->stream)) return s->set(CG_EHIP, "cfl launch failed");
-    src = w.cfl;
-  } else return s->set(CG_EINVAL, "unknown field " + nm);
-  double* f[1] = {src};
-  return get_soa(s, f, 1, out);
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    const auto& st = I->b.st[E.cur];
+    const auto& w = I->b.w;
+    const int ncv = I->ncv;
+    if (nm == "dudx") {
+      if (n != 9 * (int64_t)ncv) return s->set(CG_EINVAL, "dudx needs 9*ncv values");
+      return get_rec(s, ncv, w.aux, cgdev::A_N, cgdev::A_D, 9, out, s->stream);
+    }
+    if (n != ncv) return s->set(CG_EINVAL, "field needs ncv values");
+    struct { const char* nm; int light; int f; int prim; } recf[] = {
+        {"p", 1, cgdev::L_P, 0}, {"sos", 1, cgdev::L_SOS, 2}, {"h", 1, cgdev::L_H, 3}, {"sp_vol", 1, cgdev::L_SPV, 4},
+        {"mu_total", 0, cgdev::A_MU, 0}, {"loc_total", 0, cgdev::A_LOC, 0}, {"fgr", 0, cgdev::A_FGR, 0},
+        {"symh_fax", 0, cgdev::A_SYMH, 0}, {"symp_fax", 0, cgdev::A_SYMP, 0}};
+    for (auto& r : recf) {
+      if (nm != r.nm) continue;
+      if (!r.light) return get_rec(s, ncv, w.aux, cgdev::A_N, r.f, 1, out, s->stream);
+      if (!s->mixed) return get_rec(s, ncv, st.light, cgdev::L_N, r.f, 1, out, s->stream);
+      // mixed precision: FP64 primitives from the FP64 state
+      if (cgk::launch_prims(I->solo_launch(), 0, ncv, E.cur, I->rhs_tmp, s->stream))
+        return s->set(CG_EHIP, "prims launch failed");
+      double* f[1] = {I->rhs_tmp + (size_t)r.prim * ncv};
+      return get_soa(s, ncv, f, 1, out, s->stream);
+    }
+    // per-step diagnostic fields: recompute the properties of the current state
+    if (cgk::launch_props(I->solo_launch(), E.cur, 1, s->stream)) return s->set(CG_EHIP, "props launch failed");
+    double* src = nullptr;
+    if (nm == "T") src = w.T;
+    else if (nm == "mu_lam") src = w.mu_lam;
+    else if (nm == "mu_sgs") src = w.mu_sgs;
+    else if (nm == "loc_lam") src = w.loc_lam;
+    else if (nm == "loc_sgs") src = w.loc_sgs;
+    else if (nm == "a_sgs") src = w.a_sgs;
+    else if (nm == "vv2") src = w.vv2;
+    else if (nm == "cfl") {
+      if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
+      if (cgk::launch_cfl(I->solo_launch(), E.cur, s->info_host->dt, s->stream))
+        return s->set(CG_EHIP, "cfl launch failed");
+      src = w.cfl;
+    } else return s->set(CG_EINVAL, "unknown field " + nm);
+    double* f[1] = {src};
+    return get_soa(s, ncv, f, 1, out, s->stream);
+  });
 }
 
 int cg_calc_rhs(cg_solver* s, int stage, double* rhs) {
   if (!s || !rhs) return fail(CG_EINVAL, "null argument");
   if (stage < 1 || stage > 3) return s->set(CG_EINVAL, "stage must be 1, 2 or 3");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-  if (cgk::launch_flux(stage, 0, s->g, s->st[s->cur], s->st[1 - s->cur], s->w, s->P, s->rhs_tmp, s->stream))
-    return s->set(CG_EHIP, "flux launch failed");
-  double* f[5];
-  for (int q = 0; q < 5; ++q) f[q] = s->rhs_tmp + (size_t)q * s->ncv;
-  return get_soa(s, f, 5, rhs);
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    if (cgk::launch_flux(stage, 0, I->solo_launch(), E.cur, I->rhs_tmp, s->stream))
+      return s->set(CG_EHIP, "flux launch failed");
+    double* f[5];
+    for (int q = 0; q < 5; ++q) f[q] = I->rhs_tmp + (size_t)q * I->ncv;
+    return get_soa(s, I->ncv, f, 5, rhs, s->stream);
+  });
 }
 
 int cg_calc_cfl(cg_solver* s, double dt, double* cfl_max) {
   if (!s || !cfl_max) return fail(CG_EINVAL, "null argument");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-  if (cgk::launch_cfl(s->g, s->st[s->cur], s->w, dt, s->stream)) return s->set(CG_EHIP, "cfl launch failed");
-  std::vector<double> c((size_t)s->ncv);
-  double* f[1] = {s->w.cfl};
-  if (int rc = get_soa(s, f, 1, c.data())) return rc;
-  double mx = 0.0;
-  for (double v : c) mx = (mx < v) ? v : mx;
-  *cfl_max = mx;
-  return CG_OK;
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    return calc_cfl(s, I, E.cur, dt, cfl_max, s->stream);
+  });
 }
 
 int cg_step(cg_solver* s, int nsteps) {
   if (!s) return fail(CG_EINVAL, "null solver");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-  const int ci = std::max(1, s->prm.check_interval);
-  for (int it = 0; it < nsteps; ++it) {
-    if (int rc = launch_one_step(s)) return rc;
-    const int64_t step = s->host_step + 1;
-    s->host_step = step;
-    const bool last = (it == nsteps - 1);
-    if (step % ci == 0 || last) {
-      bool rolled = false;
-      if (int rc = check_nan_and_restore(s, &rolled)) return rc;
-      if (!rolled && step % ci == 0 && step % (int64_t(ci) * 20) == 0)
-        if (int rc = do_store(s)) return rc;
-    }
-  }
-  return CG_OK;
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    return solver_steps(s, E, I, nsteps);
+  });
 }
 
 int cg_get_diag(cg_solver* s, cg_diag* d) {
   if (!s || !d) return fail(CG_EINVAL, "null argument");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-  if (cgk::launch_diag(s->g, s->st[s->cur], s->w, s->nblocks, s->stream)) return s->set(CG_EHIP, "diag launch failed");
-  double v[8];
-  HIPCHK(s, hipMemcpyAsync(v, s->w.diag, sizeof(v), hipMemcpyDeviceToHost, s->stream));
-  if (int rc = read_info(s)) return rc;
-  d->step = s->info_host->step;
-  d->time = s->info_host->time;
-  d->dt = s->info_host->dt;
-  d->ke = v[5] / v[7];
-  d->enstrophy = v[6] / v[7];
-  for (int q = 0; q < 5; ++q) d->sums[q] = v[q];
-  d->nan_flag = s->info_host->nan;
-  double cm = 0.0;
-  if (int rc = cg_calc_cfl(s, d->dt, &cm)) return rc;
-  d->cfl_max = cm;
-  return CG_OK;
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    return get_diag(s, I, E.cur, s->info_host, d, s->stream);
+  });
 }
 
 int cg_step_history(cg_solver* s, int nsteps, cg_diag* diag) {
@@ -917,91 +1401,93 @@ int cg_step_history(cg_solver* s, int nsteps, cg_diag* diag) {
 
 int cg_store(cg_solver* s) {
   if (!s) return fail(CG_EINVAL, "null solver");
-  return do_store(s);
+  return with_inst(s, [&](auto& E, auto* I) { return do_store(s, I, E.cur, s->info_host, s->stream); });
 }
 
 int cg_restore(cg_solver* s) {
   if (!s) return fail(CG_EINVAL, "null solver");
-  if (!s->have_store) return s->set(CG_ESTATE, "nothing stored");
-  if (int rc = snapshot_copy(s, s->store_buf, false)) return rc;
-  if (int rc = read_info(s)) return rc;
-  s->info_host->step = s->store_step;
-  s->info_host->time = s->store_time;
-  s->info_host->nan = 0;
-  HIPCHK(s, hipMemcpyAsync(s->w.info, s->info_host, sizeof(cgdev::StepInfo), hipMemcpyHostToDevice, s->stream));
-  s->host_step = s->store_step;
-  s->prm.dt_data[0] *= 0.8;
-  fill_dev_params(s);
-  if (int rc = refresh(s)) return rc;
-  HIPCHK(s, hipStreamSynchronize(s->stream));
-  return CG_OK;
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (int rc = restore(s, E, I, s->info_host, s->stream)) return rc;
+    if (int rc = engine_sync(s, E, s->stream)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return 0;
+  });
 }
 
 int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms) {
   if (!s || !ms_total || nsteps < 1) return fail(CG_EINVAL, "bad argument");
-  if (!s->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-  hipEvent_t e0, e1;
-  HIPCHK(s, hipEventCreate(&e0));
-  HIPCHK(s, hipEventCreate(&e1));
-  HIPCHK(s, hipStreamSynchronize(s->stream));
-  HIPCHK(s, hipEventRecord(e0, s->stream));
-  for (int i = 0; i < nsteps; ++i)
-    if (int rc = launch_one_step(s)) return rc;
-  HIPCHK(s, hipEventRecord(e1, s->stream));
-  HIPCHK(s, hipEventSynchronize(e1));
-  float ms = 0.f;
-  HIPCHK(s, hipEventElapsedTime(&ms, e0, e1));
-  *ms_total = ms;
-  s->host_step += nsteps;
-  if (kernel_ms) {
-    // per-kernel durations: events bracket every launch of a second run
-    // kernel_ms: 0 compact-face flux, 1 extended-face flux + RK3 + EOS, 2 gradient,
-    // 3 properties (per launch), 4 sum of all kernels per step
-    const int nk = 10;
-    std::vector<hipEvent_t> ev(2 * nk);
-    for (auto& e : ev) HIPCHK(s, hipEventCreate(&e));
-    double acc[4] = {0, 0, 0, 0};
-    double tot = 0.0;
-    int fam[nk];
-    for (int i = 0; i < nsteps; ++i) {
-      if (cgk::launch_begin_step(s->w, s->P, s->stream)) return s->set(CG_EHIP, "launch failed");
-      int k = 0;
-      auto rec = [&](int family, auto&& launch) -> int {
-        HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
-        if (launch()) return s->set(CG_EHIP, "launch failed");
-        HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
-        fam[k++] = family;
-        return 0;
-      };
-      for (int stage = 1; stage <= 3; ++stage) {
-        const int nxt = 1 - s->cur;
-        const cgdev::State in = s->st[s->cur], out = s->st[nxt];
-        if (int rc = rec(0, [&] { return cgk::launch_flux_compact(s->g, in, s->w, s->stream); })) return rc;
-        if (int rc = rec(1, [&] { return cgk::launch_flux_ext(stage, 1, s->g, in, out, s->w, s->P, nullptr, s->stream); })) return rc;
-        s->cur = nxt;
-        if (int rc = rec(2, [&] { return cgk::launch_grad(s->g, s->st[s->cur], s->w, s->stream); })) return rc;
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    if (int rc = engine_sync(s, E, s->stream)) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(s, hipEventCreate(&e0));
+    HIPCHK(s, hipEventCreate(&e1));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipEventRecord(e0, s->stream));
+    for (int i = 0; i < nsteps; ++i)
+      if (int rc = engine_launch_step(s, E, s->stream)) return rc;
+    HIPCHK(s, hipEventRecord(e1, s->stream));
+    HIPCHK(s, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    I->host_step += nsteps;
+    if (kernel_ms) {
+      // per-kernel durations: events bracket every launch of a second run
+      // kernel_ms: 0 compact-face flux, 1 extended-face flux + RK3 + EOS, 2 gradient,
+      // 3 properties (per launch), 4 sum of all kernels per step
+      const int nk = 10;
+      std::vector<hipEvent_t> ev(2 * nk);
+      for (auto& e : ev) HIPCHK(s, hipEventCreate(&e));
+      double acc[4] = {0, 0, 0, 0};
+      double tot = 0.0;
+      int fam[nk];
+      const auto& L = E.L;
+      int rc = 0;
+      for (int i = 0; i < nsteps && !rc; ++i) {
+        if (cgk::launch_begin_step(L, s->stream)) {
+          rc = s->set(CG_EHIP, "launch failed");
+          break;
+        }
+        int k = 0;
+        auto rec = [&](int family, auto&& launch) -> int {
+          HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
+          if (launch()) return s->set(CG_EHIP, "launch failed");
+          HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
+          fam[k++] = family;
+          return 0;
+        };
+        for (int stage = 1; stage <= 3 && !rc; ++stage) {
+          const int c0 = E.cur;
+          rc = rec(0, [&] { return cgk::launch_flux_compact(L, c0, s->stream); });
+          if (!rc) rc = rec(1, [&] { return cgk::launch_flux_ext(stage, 1, L, c0, nullptr, s->stream); });
+          E.cur ^= 1;
+          if (!rc) rc = rec(2, [&] { return cgk::launch_grad(L, E.cur, s->stream); });
+        }
+        if (!rc) rc = rec(3, [&] { return cgk::launch_props(L, E.cur, 0, s->stream); });
+        if (rc) break;
+        HIPCHK(s, hipEventSynchronize(ev[2 * (k - 1) + 1]));
+        for (int q = 0; q < k; ++q) {
+          float t = 0.f;
+          HIPCHK(s, hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+          acc[fam[q]] += t;
+          tot += t;
+        }
       }
-      if (int rc = rec(3, [&] { return cgk::launch_props(s->g, s->st[s->cur], s->w, s->P, 0, s->stream); })) return rc;
-      HIPCHK(s, hipEventSynchronize(ev[2 * (k - 1) + 1]));
-      for (int q = 0; q < k; ++q) {
-        float t = 0.f;
-        HIPCHK(s, hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
-        acc[fam[q]] += t;
-        tot += t;
-      }
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      if (rc) return rc;
+      I->host_step += nsteps;
+      kernel_ms[0] = acc[0] / (3.0 * nsteps);
+      kernel_ms[1] = acc[1] / (3.0 * nsteps);
+      kernel_ms[2] = acc[2] / (3.0 * nsteps);
+      kernel_ms[3] = acc[3] / nsteps;
+      kernel_ms[4] = tot / nsteps;
     }
-    s->host_step += nsteps;
-    kernel_ms[0] = acc[0] / (3.0 * nsteps);
-    kernel_ms[1] = acc[1] / (3.0 * nsteps);
-    kernel_ms[2] = acc[2] / (3.0 * nsteps);
-    kernel_ms[3] = acc[3] / nsteps;
-    kernel_ms[4] = tot / nsteps;
-    for (auto& e : ev) (void)hipEventDestroy(e);
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  bool rolled = false;
-  return check_nan_and_restore(s, &rolled);
+    bool rolled = false;
+    return check_nan_and_restore(s, E, I, s->info_host, s->stream, &rolled);
+  });
 }
 
 const char* cg_last_error(const cg_solver* s) { return s ? s->err.c_str() : g_err.c_str(); }
@@ -1009,10 +1495,265 @@ const char* cg_last_error(const cg_solver* s) { return s ? s->err.c_str() : g_er
 void cg_destroy(cg_solver* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (void* p : s->allocs) (void)hipFree(p);
+  drop_graph(s->ed);
+  drop_graph(s->ef);
+  free_instance(s->id);
+  free_instance(s->ix);
+  for (void* p : {(void*)s->ed.tab, (void*)s->ed.bmap, (void*)s->ed.pbmap, (void*)s->ef.tab, (void*)s->ef.bmap,
+                  (void*)s->ef.pbmap})
+    if (p) (void)hipFree(p);
   if (s->info_host) (void)hipHostFree(s->info_host);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
+}
+
+// ---- batches ----
+
+int cg_batch_create(int precision, cg_batch** out) {
+  if (!out) return fail(CG_EINVAL, "null output");
+  if (precision != CG_PREC_FP64 && precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
+  *out = nullptr;
+  cg_batch* b = new cg_batch();
+  b->mixed = (precision == CG_PREC_MIXED);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1 ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&b->info_host, sizeof(cgdev::StepInfo)) != hipSuccess) {
+    cg_batch_destroy(b);
+    return fail(CG_EHIP, "no HIP device, stream or pinned memory");
+  }
+  std::memset(b->info_host, 0, sizeof(cgdev::StepInfo));
+  b->use_graph = !std::getenv("CG_NO_GRAPH");
+  *out = b;
+  return CG_OK;
+}
+
+int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
+                       const double* rhoE, cg_instance** out) {
+  if (!m || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
+  if (int rc = check_params(p)) return rc;
+  *out = nullptr;
+  cg_instance* x = new cg_instance();
+  x->mixed = (p->precision == CG_PREC_MIXED);
+  hipStream_t stream = nullptr;
+  cgdev::StepInfo* host = nullptr;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&host, sizeof(cgdev::StepInfo)) != hipSuccess) {
+    rc = x->set(CG_EHIP, "no HIP device, stream or pinned memory");
+  }
+  const bool lds = !std::getenv("CG_NO_LDS");
+  auto make = [&](auto** I) -> int {
+    if (int r = make_instance(x, m, p, lds, stream, I)) return r;
+    // the state goes to st[0]; attaching to a batch at parity 1 swaps the buffers
+    if (int r = set_state(x, *I, 0, rho, u, rhoE, stream)) return r;
+    if (int r = refresh(x, *I, 0, stream)) return r;
+    if (int r = do_store(x, *I, 0, host, stream)) return r;
+    HIPCHK(x, hipStreamSynchronize(stream));
+    return 0;
+  };
+  if (!rc) rc = x->mixed ? make(&x->ix) : make(&x->id);
+  if (host) (void)hipHostFree(host);
+  if (stream) (void)hipStreamDestroy(stream);
+  if (rc) {
+    const std::string e = x->err;
+    cg_instance_destroy(x);
+    return fail(rc, e);
+  }
+  *out = x;
+  return CG_OK;
+}
+
+int cg_instance_bytes(const cg_instance* x, int64_t* bytes) {
+  if (!x || !bytes) return fail(CG_EINVAL, "null argument");
+  *bytes = (int64_t)(x->mixed ? x->ix->bytes : x->id->bytes);
+  return CG_OK;
+}
+
+void cg_instance_destroy(cg_instance* x) {
+  if (!x) return;
+  free_instance(x->id);
+  free_instance(x->ix);
+  delete x;
+}
+
+int cg_batch_attach(cg_batch* b, cg_instance* x, int* id) {
+  if (!b || !x || !id) return fail(CG_EINVAL, "null argument");
+  if (x->mixed != b->mixed) return b->set(CG_EINVAL, "instance precision differs from the batch's");
+  auto attach = [&](auto& E, auto*& I) -> int {
+    if (E.cur == 1) {
+      std::swap(I->b.st[0], I->b.st[1]);
+      if (int rc = upload_solo(b, I)) return rc;
+    }
+    I->id = b->next_id++;
+    E.inst.push_back(I);
+    E.dirty = true;
+    *id = I->id;
+    I = nullptr;  // owned by the batch now
+    return 0;
+  };
+  const int rc = b->mixed ? attach(b->ef, x->ix) : attach(b->ed, x->id);
+  if (rc) return rc;
+  delete x;
+  return CG_OK;
+}
+
+int cg_batch_add(cg_batch* b, const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
+                 const double* rhoE, int* id) {
+  if (!b || !m || !p || !rho || !u || !rhoE || !id) return fail(CG_EINVAL, "null argument");
+  if (int rc = check_params(p)) return b->set(rc, g_err);
+  if ((p->precision == CG_PREC_MIXED) != b->mixed) return b->set(CG_EINVAL, "instance precision differs from the batch's");
+  cg_instance* x = nullptr;
+  if (int rc = cg_instance_create(m, p, rho, u, rhoE, &x)) return b->set(rc, g_err);
+  if (int rc = cg_batch_attach(b, x, id)) {
+    cg_instance_destroy(x);
+    return rc;
+  }
+  return CG_OK;
+}
+
+int cg_batch_remove(cg_batch* b, int id) {
+  if (!b) return fail(CG_EINVAL, "null batch");
+  return with_engine(b, [&](auto& E) -> int {
+    for (size_t i = 0; i < E.inst.size(); ++i) {
+      if (E.inst[i]->id != id) continue;
+      HIPCHK(b, hipStreamSynchronize(b->stream));
+      free_instance(E.inst[i]);
+      E.inst.erase(E.inst.begin() + i);
+      E.dirty = true;
+      return 0;
+    }
+    return b->set(CG_EINVAL, "no instance " + std::to_string(id));
+  });
+}
+
+int cg_batch_step(cg_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return fail(CG_EINVAL, "bad argument");
+  return with_engine(b, [&](auto& E) -> int {
+    if (E.inst.empty() || nsteps == 0) return 0;
+    // chunks end where any instance reaches a check point
+    int it = 0;
+    while (it < nsteps) {
+      int chunk = nsteps - it;
+      for (auto* I : E.inst) {
+        const int ci = std::max(1, I->prm.check_interval);
+        chunk = (int)std::min<int64_t>(chunk, (I->host_step / ci + 1) * ci - I->host_step);
+      }
+      if (int rc = engine_steps(b, E, b->stream, chunk, b->use_graph)) return rc;
+      it += chunk;
+      for (auto* I : E.inst) I->host_step += chunk;
+      for (auto* I : E.inst) {
+        const int ci = std::max(1, I->prm.check_interval);
+        if (I->host_step % ci != 0 && it != nsteps) continue;
+        bool rolled = false;
+        if (int rc = check_nan_and_restore(b, E, I, b->info_host, b->stream, &rolled)) return rc;
+        if (!rolled && I->host_step % ci == 0 && I->host_step % (int64_t(ci) * 20) == 0)
+          if (int rc = do_store(b, I, E.cur, b->info_host, b->stream)) return rc;
+      }
+    }
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    return 0;
+  });
+}
+
+int cg_batch_get_diag(cg_batch* b, int id, cg_diag* d) {
+  if (!b || !d) return fail(CG_EINVAL, "null argument");
+  return with_engine(b, [&](auto& E) -> int {
+    auto* I = find_inst(E, id);
+    if (!I) return b->set(CG_EINVAL, "no instance " + std::to_string(id));
+    return get_diag(b, I, E.cur, b->info_host, d, b->stream);
+  });
+}
+
+int cg_batch_get_state(cg_batch* b, int id, double* rho, double* u, double* rhoE) {
+  if (!b) return fail(CG_EINVAL, "null batch");
+  return with_engine(b, [&](auto& E) -> int {
+    auto* I = find_inst(E, id);
+    if (!I) return b->set(CG_EINVAL, "no instance " + std::to_string(id));
+    return get_state(b, I, E.cur, rho, u, rhoE, b->stream);
+  });
+}
+
+int cg_batch_counts(cg_batch* b, int64_t counts[4]) {
+  if (!b || !counts) return fail(CG_EINVAL, "null argument");
+  return with_engine(b, [&](auto& E) -> int {
+    int64_t cells = 0, bytes = 0, lds = 0;
+    for (auto* I : E.inst) {
+      cells += I->ncv;
+      bytes += (int64_t)I->bytes;
+      lds += I->lds > 0;
+    }
+    counts[0] = (int64_t)E.inst.size();
+    counts[1] = cells;
+    counts[2] = bytes;
+    counts[3] = lds;
+    return 0;
+  });
+}
+
+int cg_batch_set_option(cg_batch* b, const char* name, int value) {
+  if (!b || !name) return fail(CG_EINVAL, "null argument");
+  const std::string n(name);
+  if (n == "graph") {
+    b->use_graph = value != 0;
+    return 0;
+  }
+  if (n == "lds") {
+    return with_engine(b, [&](auto& E) -> int {
+      E.force_global = (value == 0);
+      E.dirty = true;
+      return 0;
+    });
+  }
+  return b->set(CG_EINVAL, "unknown option " + n);
+}
+
+int cg_batch_time_steps(cg_batch* b, int nsteps, double* ms_total) {
+  if (!b || !ms_total || nsteps < 1) return fail(CG_EINVAL, "bad argument");
+  return with_engine(b, [&](auto& E) -> int {
+    if (int rc = engine_sync(b, E, b->stream)) return rc;
+    // build the graph outside the timed region
+    if (b->use_graph && nsteps >= 2 && !E.graph[E.cur]) {
+      if (int rc = engine_steps(b, E, b->stream, 2, true)) return rc;
+      for (auto* I : E.inst) I->host_step += 2;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(b, hipEventCreate(&e0));
+    HIPCHK(b, hipEventCreate(&e1));
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    HIPCHK(b, hipEventRecord(e0, b->stream));
+    if (int rc = engine_steps(b, E, b->stream, nsteps, b->use_graph)) return rc;
+    HIPCHK(b, hipEventRecord(e1, b->stream));
+    HIPCHK(b, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(b, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    for (auto* I : E.inst) I->host_step += nsteps;
+    for (auto* I : E.inst) {
+      bool rolled = false;
+      if (int rc = check_nan_and_restore(b, E, I, b->info_host, b->stream, &rolled)) return rc;
+    }
+    return 0;
+  });
+}
+
+const char* cg_batch_last_error(const cg_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+void cg_batch_destroy(cg_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b->ed);
+  drop_graph(b->ef);
+  for (auto* I : b->ed.inst) free_instance(I);
+  for (auto* I : b->ef.inst) free_instance(I);
+  for (void* p : {(void*)b->ed.tab, (void*)b->ed.bmap, (void*)b->ed.pbmap, (void*)b->ef.tab, (void*)b->ef.bmap,
+                  (void*)b->ef.pbmap})
+    if (p) (void)hipFree(p);
+  if (b->info_host) (void)hipHostFree(b->info_host);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
 }
 
 }  // extern "C"

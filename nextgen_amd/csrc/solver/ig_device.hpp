@@ -1,5 +1,12 @@
 // Device-side data of the ideal-gas step (Morton cell order).
 //
+// The step runs in one of two working precisions R:
+//   double  FP64 throughout (the reference arithmetic);
+//   float   mixed: face geometry, neighbour records, fluxes and their
+//           accumulation, gradients, lagged coefficients and RK registers in
+//           FP32; the conserved state rho, u, rhoE, the RK3 state update, the
+//           recovery of p, T from rhoE and the conservation sums in FP64.
+//
 // Per-cell data that neighbours gather is packed in records so that one
 // neighbour costs one cache line per record instead of one line per field:
 //   light[c][8]  u0 u1 u2 sp_vol p h rho sos        (IdealGasState + rho, sos; per stage)
@@ -8,6 +15,20 @@
 //                gradient, per stage)
 // Data a cell only reads for itself (rhoE, geometry, RK registers) stays
 // structure-of-arrays so a wavefront reads it coalesced.
+//
+// In FP64 the light record is also the state (rho and u live there, rhoE
+// beside it). In mixed precision the state is the FP64 array cons[5][ncv]
+// (rho, u0, u1, u2, rhoE) and the FP32 light record is derived from it, with
+// p and h stored relative to the constant gauges p_g and h_g:
+//   light.p = p - p_g,  light.h = h - h_g.
+// Differences of p and h are unchanged by the gauge. Levels enter the flux in
+// two places: p_avg n in the momentum flux, whose constant part p_g n sums to
+// zero over a cell's half-faces (the operator's discrete GCL), so it is
+// dropped; and h in the energy flux, which is carried as
+//   F_rhoE' = F_rhoE - h_g F_rho
+// through the accumulation and restored in FP64 in the RK update
+// (rhs_rhoE = rhs_rhoE' + h_g rhs_rho). The p v terms of the compact closure
+// use the absolute pressure (see flux_compact).
 #pragma once
 #include <cstdint>
 
@@ -22,27 +43,34 @@ enum { L_U0 = 0, L_U1, L_U2, L_SPV, L_P, L_H, L_RHO, L_SOS, L_N = 8 };
 // aux record fields
 enum { A_MU = 0, A_LOC, A_ASGS, A_FGR, A_SYMH, A_SYMP, A_PAD, A_D = 7, A_N = 16 };
 
-// Mesh / operator arrays, all device pointers.
+// Mesh / operator arrays, all device pointers; geometry in the working precision R.
 //
-// Each cell's half-faces (its extended pairs, in pair order) are split in two
-// SELL-64 regions: the compact ones (cf_*, ef_nc[cell] slots) and the purely
-// extended ones (xf_*, xf_deg[cell] slots). Slot k of a cell is at
+// The compact pairs (the Voronoi faces, pair indices 0..nfa-1) are evaluated
+// once per pair by their own kernel (thread per pair, per-pair SoA geometry),
+// which writes each pair's flux to w.cflux. Each cell's half-faces (its
+// extended pairs, in pair order) are split in two SELL-64 regions: the compact
+// ones (cf_*, ef_nc[cell] slots: the pair index, to gather cflux) and the
+// purely extended ones (xf_*, xf_deg[cell] slots). Slot k of a cell is at
 // ptr[slice] + k*64 + lane, so at every k the 64 lanes of a wavefront read one
-// contiguous row. The pair geometry is stored per slot (both cells of a pair
-// hold a copy), which keeps every geometry read coalesced.
-struct Geom {
+// contiguous row. The extended pair geometry is stored per slot (both cells
+// of a pair hold a copy), which keeps every geometry read coalesced.
+template <class R>
+struct GeomT {
   int ncv, nfa, nef;
   const double *inv_vol, *vol, *sa_to_vol, *dcmag, *dx13, *dx23;
+  // compact pairs (thread per pair)
+  const int *plo, *phi;                           // the pair's cells (lo < hi in pair orientation)
+  const R *pn0, *pn1, *pn2, *pc0, *pc1, *pc2;     // pair n and c (grouping applied)
+  const R *pu0, *pu1, *pu2, *parea, *paod;        // unit normal, area, area/delta (grouping applied)
+  const R *pf0, *pf1, *pf2;                       // raw area vector n_fa (CFL)
   // compact half-faces
   const int *cf_ptr, *ef_nc;
-  const int* cf_nbr;                                   // neighbour | (cell is the pair's hi side) << 31
-  const double *csn0, *csn1, *csn2, *csc0, *csc1, *csc2;  // pair n and c (grouping applied)
-  const double *cu0, *cu1, *cu2, *carea, *caod;        // unit normal, area, area/delta (grouping applied)
-  const double *cn0, *cn1, *cn2;                       // raw area vector (CFL)
+  const int* cf_nbr;                              // neighbour | (cell is the pair's hi side) << 31
+  const int* cf_pair;                             // pair index | (cell is the pair's hi side) << 31
   // purely extended half-faces
   const int *xf_ptr, *xf_deg;
-  const int* xf_nbr;                                   // neighbour | hi-side flag << 31
-  const double *xsn0, *xsn1, *xsn2, *xsc0, *xsc1, *xsc2;
+  const int* xf_nbr;                              // neighbour | hi-side flag << 31
+  const R *xsn0, *xsn1, *xsn2, *xsc0, *xsc1, *xsc2;
   // per-block halo for LDS staging (null when staging is off): the sorted
   // cells block b's extended half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
   // xf_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15
@@ -51,13 +79,15 @@ struct Geom {
   int halo_max;
   // gradient stencil without the diagonal, SELL-64 layout
   const int *gr_ptr, *gr_deg, *gr_nbr;
-  const double *gc0, *gc1, *gc2;
+  const R *gc0, *gc1, *gc2;
 };
 
 // One buffer of the ping-pong state.
-struct State {
-  double* light;  // [ncv][8]
-  double* rhoE;   // [ncv]
+template <class R>
+struct StateT {
+  R* light;      // [ncv][8]
+  double* rhoE;  // [ncv]; in mixed precision it aliases cons + 4*ncv
+  double* cons;  // mixed precision: [5][ncv] rho u0 u1 u2 rhoE (null in FP64)
 };
 
 struct StepInfo {
@@ -68,14 +98,15 @@ struct StepInfo {
   int pad;
 };
 
-struct Work {
-  double* aux;       // [ncv][16]
-  double* logp;      // [ncv] log(p/p_ref) of the end-of-step state
+template <class R>
+struct WorkT {
+  R* aux;            // [ncv][16]
+  double* logp;      // [ncv] log(p/p_ref) of the end-of-step state (FP64 sensor)
   // per-step fields (diagnostic copies)
   double *T, *mu_lam, *mu_sgs, *loc_lam, *loc_sgs, *a_sgs, *vv2, *cfl;
-  double* rhs0;      // [5][ncv]
-  double* rhs1;      // [5][ncv]
-  double* racc;      // [5][ncv] compact-face part of the stage RHS
+  R* rhs0;           // [5][ncv]
+  R* rhs1;           // [5][ncv]
+  R* cflux;          // [nfa][6] flux of each compact pair (lo -> hi), this stage
   StepInfo* info;
   double* partial;   // reduction scratch [nblocks][8]
   double* diag;      // [8]
@@ -87,26 +118,69 @@ struct Params {
   double coeff_p, coeff_h, gr_nrm;
   int sgs_model, diss_mode, limit_asgs, dt_mode;
   double dt_data0, dt_data1;
+  double p_g, h_g;  // mixed-precision gauges (p_ref and h at T_ref)
 };
+
+using Geom = GeomT<double>;
+using State = StateT<double>;
+using Work = WorkT<double>;
+
+// One solve as the kernels see it. A launch covers any number of solves: the
+// kernels take a device table of Inst records and a block map whose entry
+// for each launched block is (instance << kBlkBits) | instance-local block.
+// The ping-pong index `cur` (which st[] holds the current state) is shared
+// by all instances of a launch, which advance in lockstep stage by stage.
+template <class R>
+struct Inst {
+  GeomT<R> g;
+  StateT<R> st[2];
+  WorkT<R> w;
+  Params P;
+  int nblocks;   // blocks of kBlock cells
+  int npblocks;  // blocks of kBlock compact pairs
+};
+
+constexpr int kBlkBits = 20;  // up to 2^20 blocks (268M cells) per instance, 2^11 instances
 
 }  // namespace cgdev
 
-// launchers (ig_kernels.hip)
+// Launchers (ig_kernels.hip), instantiated for R = double and R = float. `I`
+// is the device instance table, `bmap` / `pbmap` the device block maps of the
+// cell and compact-pair launches, `lds` the dynamic LDS bytes of the
+// extended-face pass (0: the global-gather variant; every instance then needs
+// no halo).
 namespace cgk {
-int launch_flux(int stage, int mode, const cgdev::Geom& g, const cgdev::State& in, const cgdev::State& out,
-                const cgdev::Work& w, const cgdev::Params& P, double* rhs_out, void* stream);
-int launch_flux_compact(const cgdev::Geom& g, const cgdev::State& in, const cgdev::Work& w, void* stream);
-int launch_flux_ext(int stage, int mode, const cgdev::Geom& g, const cgdev::State& in, const cgdev::State& out,
-                    const cgdev::Work& w, const cgdev::Params& P, double* rhs_out, void* stream);
-int launch_grad(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, void* stream);
-int launch_eos(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, const cgdev::Params& P,
-               void* stream);
-int launch_props(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, const cgdev::Params& P,
-                 int fields, void* stream);
-int launch_begin_step(const cgdev::Work& w, const cgdev::Params& P, void* stream);
-int launch_cfl(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, double dt, void* stream);
-int launch_diag(const cgdev::Geom& g, const cgdev::State& st, const cgdev::Work& w, int nblocks, void* stream);
+template <class R>
+struct Launch {
+  const cgdev::Inst<R>* I;
+  const int* bmap;   // cell blocks
+  int nblk;
+  int ninst;
+  size_t lds;
+  const int* pbmap;  // compact-pair blocks
+  int npblk;
+};
+template <class R>
+int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);
+template <class R>
+int launch_flux_compact(const Launch<R>& L, int cur, void* stream);
+template <class R>
+int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);
+template <class R>
+int launch_grad(const Launch<R>& L, int cur, void* stream);
+template <class R>
+int launch_eos(const Launch<R>& L, int cur, void* stream);
+template <class R>
+int launch_props(const Launch<R>& L, int cur, int fields, void* stream);
+template <class R>
+int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream);
+template <class R>
+int launch_diag(const Launch<R>& L, int cur, void* stream);
+template <class R>
+int launch_begin_step(const Launch<R>& L, void* stream);
+// FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]
+template <class R>
+int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, void* stream);
 int launch_fill(double* p, double v, int64_t n, void* stream);
-// gather a light / aux field into a SoA array
-int launch_unpack(const double* rec, int stride, int field, int64_t n, double* out, void* stream);
+int launch_fill(float* p, float v, int64_t n, void* stream);
 }  // namespace cgk

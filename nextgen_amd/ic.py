@@ -11,6 +11,10 @@ evaluated at the cell centroids the way an ``INIT`` expression or
 Nondimensionalisation: U0 = 1, rho0 = 1, T0 = T_REF = 1, c0 = U0/Ma,
 P_REF = p0 = rho0 c0^2 / gamma, RHO_REF = 1 (so R = p0), MU_REF = rho0 U0 / Re,
 MU_POWER_LAW 0.75 (the reference default).
+
+The same TGV field, and the seeded k^4 decaying-turbulence field used at
+production sizes, are computed by the library itself (``Mesh.ic_tgv`` and
+``Mesh.ic_k4`` over cg_ic_tgv / cg_ic_k4, threaded C++).
 """
 from __future__ import annotations
 
@@ -40,46 +44,4 @@ def tgv_state(x_cv, Ma=0.1, gamma=1.4):
     p = p0 + (1.0 / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
     rho = p / (R * 1.0)
     rhoE = p / (gamma - 1.0) + 0.5 * rho * (u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2])
-    return rho, u, rhoE
-
-
-def k4_state(x_cv, seed=1, k0=4.0, Mt=0.3, gamma=1.4, kmax=None):
-    """Solenoidal random field with E(k) ~ k^4 exp(-2 (k/k0)^2), random phases,
-    scaled so u_rms'/c0 = Mt with c0 = 1/Mt... (uniform rho = 1, p = p0).
-
-    Direct Fourier sum over integer wave vectors |k| <= kmax evaluated at the
-    centroids (small meshes: cost ~ ncv * nmodes).
-    """
-    rng = np.random.default_rng(seed)
-    kmax = kmax if kmax is not None else int(np.ceil(2.5 * k0))
-    ks = []
-    for a in range(-kmax, kmax + 1):
-        for b in range(-kmax, kmax + 1):
-            for c in range(0, kmax + 1):
-                if c == 0 and (b < 0 or (b == 0 and a <= 0)):
-                    continue
-                if a * a + b * b + c * c <= kmax * kmax:
-                    ks.append((a, b, c))
-    ks = np.array(ks, dtype=np.float64)
-    kmag = np.sqrt((ks ** 2).sum(1))
-    # modes per unit shell ~ 2 pi k^2 (half space): amplitude^2 ~ E(k) / (2 pi k^2)
-    E = kmag ** 4 * np.exp(-2.0 * (kmag / k0) ** 2)
-    amp = np.sqrt(E / (2.0 * np.pi * kmag ** 2))
-    # random direction perpendicular to k
-    r = rng.normal(size=ks.shape)
-    d = np.cross(ks, r)
-    d /= np.linalg.norm(d, axis=1)[:, None]
-    phase = rng.uniform(0.0, 2.0 * np.pi, size=len(ks))
-    u = np.zeros((len(x_cv), 3))
-    for i in range(len(ks)):
-        arg = x_cv @ ks[i] + phase[i]
-        u += np.outer(np.cos(arg), amp[i] * d[i])
-    u -= u.mean(axis=0)
-    urms = np.sqrt((u ** 2).sum(1).mean() / 3.0)
-    c0 = 1.0
-    u *= Mt * c0 / urms
-    p0 = c0 * c0 / gamma
-    rho = np.ones(len(x_cv))
-    p = np.full(len(x_cv), p0)
-    rhoE = p / (gamma - 1.0) + 0.5 * rho * (u ** 2).sum(1)
     return rho, u, rhoE

@@ -59,12 +59,19 @@ class cg_diag(C.Structure):
         }
 
 
+CG_PREC_FP64 = 0
+CG_PREC_MIXED = 1
+
 # symbols declared in include/charles_gpu.h
 EXPORTS = [
     "cg_params_default", "cg_params_parse", "cg_mesh_lattice", "cg_mesh_jittered", "cg_mesh_from_arrays",
     "cg_mesh_counts", "cg_mesh_get", "cg_mesh_destroy", "cg_error_string", "cg_create", "cg_set_state",
     "cg_get_state", "cg_get_field", "cg_init_complete", "cg_calc_rhs", "cg_calc_cfl", "cg_step", "cg_get_diag",
     "cg_step_history", "cg_store", "cg_restore", "cg_time_steps", "cg_last_error", "cg_destroy",
+    "cg_ic_tgv", "cg_ic_k4", "cg_batch_create", "cg_batch_add", "cg_batch_remove", "cg_batch_step",
+    "cg_batch_get_diag", "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
+    "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
+    "cg_batch_attach",
 ]
 
 _lib = None
@@ -109,6 +116,8 @@ def load_library(path: str = LIB_PATH):
         "cg_time_steps": (C.c_int, [vp, C.c_int, dp, dp]),
         "cg_last_error": (C.c_char_p, [vp]),
         "cg_destroy": (None, [vp]),
+        "cg_ic_tgv": (C.c_int, [vp, C.c_double, C.c_double, dp, dp, dp]),
+        "cg_ic_k4": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -225,6 +234,19 @@ class Mesh:
         out = np.empty(shape, dtype=dt)
         _check(load_library().cg_mesh_get(self._h, name.encode(), out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
+
+    def ic_tgv(self, Ma=0.1, gamma=1.4):
+        """Taylor-Green vortex state (rho, u, rhoE) at the centroids (cg_ic_tgv)."""
+        rho, u, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
+        _check(load_library().cg_ic_tgv(self._h, Ma, gamma, _dptr(rho), _dptr(u), _dptr(rhoE)))
+        return rho, u, rhoE
+
+    def ic_k4(self, k0=4.0, Mt=0.3, gamma=1.4, seed=1, kmax=0):
+        """Seeded k^4 exp(-2 (k/k0)^2) solenoidal field (rho, u, rhoE) (cg_ic_k4)."""
+        rho, u, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
+        _check(load_library().cg_ic_k4(self._h, k0, Mt, gamma, int(seed), int(kmax), _dptr(rho), _dptr(u),
+                                       _dptr(rhoE)))
+        return rho, u, rhoE
 
     def close(self):
         if self._h:

@@ -1,0 +1,194 @@
+"""Serving loop: continuous batching of independent solves on one GPU.
+
+Requests (a periodic jittered-Voronoi solve: mesh size, initial field, Re,
+Ma, horizon in steps) arrive over time. The server
+  * admits arrived requests first-come first-served while the device memory
+    held by admitted solves stays within an HBM budget;
+  * prepares an admitted request (device layout, state, initComplete) on a
+    worker thread through cg_instance_create while the batch keeps stepping;
+  * attaches prepared solves to the running batch between steps, so every
+    kernel launch covers all resident solves (cg_batch_step);
+  * retires a solve at its horizon (its result read back, its memory freed)
+    and records its time to solution = completion - arrival.
+A solve's result does not depend on what else is in the batch (the batch
+tests check it bit for bit against the solve run alone).
+
+`sequential` is the reference execution: the same requests one at a time in
+arrival order, each through cg_create / cg_step, with the same queueing
+arithmetic (a request starts when it has arrived and the previous one is done).
+"""
+from __future__ import annotations
+
+import collections
+import concurrent.futures as cf
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .batch import Batch, Instance
+from .ensemble import Spec
+from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, IdealGasSolver
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float   # seconds after the start of the run
+    spec: Spec       # spec.steps is the horizon
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    start: float
+    done: float
+    cells: int
+    steps: int
+    ke: float
+    enstrophy: float
+    time: float
+
+    @property
+    def tts(self):
+        return self.done - self.arrival
+
+
+def requests_poisson(rate, count, sizes, seed=7, steps_lo=200, steps_hi=2000, cells_lo=2e4, cells_hi=2e6):
+    """`count` requests with exponential inter-arrival times of mean 1/rate.
+    Cell counts are log-uniform in [cells_lo, cells_hi], mapped to the nearest
+    lattice side in `sizes` (the meshes a deployment holds); horizons uniform
+    in [steps_lo, steps_hi]; Re, Ma and the initial field as in the ensemble."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    sizes = np.asarray(sorted(sizes))
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        cells = float(np.exp(rng.uniform(np.log(cells_lo), np.log(cells_hi))))
+        n = int(sizes[np.argmin(np.abs(np.log(sizes.astype(float) ** 3) - np.log(cells)))])
+        sp = Spec(n=n, mesh_seed=1000 + n, ic="tgv" if rng.random() < 0.5 else "k4",
+                  Re=float(rng.uniform(400.0, 3200.0)), Ma=float(rng.uniform(0.05, 0.3)),
+                  ic_seed=int(rng.integers(1, 2**31)), steps=int(rng.integers(steps_lo, steps_hi + 1)))
+        out.append(Request(i, t, sp))
+    return out
+
+
+class MeshPool:
+    """Meshes by lattice side, built once (the catalogue a deployment serves)."""
+
+    def __init__(self):
+        self.meshes = {}
+
+    def get(self, sp: Spec):
+        key = (sp.n, sp.mesh_seed)
+        if key not in self.meshes:
+            self.meshes[key] = sp.mesh()
+        return self.meshes[key]
+
+
+class Server:
+    def __init__(self, pool: MeshPool, precision=CG_PREC_MIXED, hbm_budget=96e9, max_chunk=16,
+                 bytes_per_cell=None, log=None):
+        self.pool = pool
+        self.precision = precision
+        self.budget = float(hbm_budget)
+        self.max_chunk = int(max_chunk)
+        self.bpc = bytes_per_cell or (6000.0 if precision == CG_PREC_FP64 else 4000.0)
+        self.log = log or (lambda s: None)
+
+    def _prepare(self, req: Request):
+        m = self.pool.get(req.spec)
+        return Instance(m, req.spec.deck(self.precision, check_interval=100), *req.spec.state(m))
+
+    def run(self, requests):
+        reqs = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        preparing = {}          # future -> (request, reserved bytes)
+        active = {}             # batch id -> [request, steps left, start, bytes, cells]
+        results = []
+        used = 0.0
+        batch = Batch(self.precision)
+        pool = cf.ThreadPoolExecutor(max_workers=1)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        steps_run = 0
+        cell_steps = 0
+        while reqs or waiting or preparing or active:
+            t = now()
+            while reqs and reqs[0].arrival <= t:
+                waiting.append(reqs.popleft())
+            # admission by HBM budget, first come first served
+            while waiting:
+                r = waiting[0]
+                need = self.bpc * r.spec.n ** 3
+                if used + need > self.budget and (active or preparing):
+                    break
+                waiting.popleft()
+                used += need
+                preparing[pool.submit(self._prepare, r)] = (r, need)
+            # attach what is ready
+            for fut in [f for f in preparing if f.done()]:
+                r, need = preparing.pop(fut)
+                inst = fut.result()
+                nbytes = inst.bytes()
+                used += nbytes - need
+                self.bpc = max(self.bpc, nbytes / r.spec.n ** 3)
+                iid = batch.attach(inst)
+                active[iid] = [r, r.spec.steps, now(), nbytes, r.spec.n ** 3]
+            if not active:
+                nxt = [f for f in preparing]
+                if nxt:
+                    cf.wait(nxt, return_when=cf.FIRST_COMPLETED)
+                elif reqs:
+                    time.sleep(max(0.0, min(reqs[0].arrival - now(), 0.05)))
+                continue
+            k = min(self.max_chunk, min(a[1] for a in active.values()))
+            batch.step(k)
+            steps_run += k
+            cell_steps += k * sum(a[4] for a in active.values())
+            for iid in list(active):
+                a = active[iid]
+                a[1] -= k
+                if a[1] == 0:
+                    d = batch.diag(iid)
+                    r = a[0]
+                    results.append(Result(r.rid, r.arrival, a[2], now(), a[4], r.spec.steps, d.ke, d.enstrophy, d.time))
+                    batch.remove(iid)
+                    used -= a[3]
+                    del active[iid]
+        makespan = now()
+        pool.shutdown()
+        batch.close()
+        return results, {"makespan": makespan, "batch_steps": steps_run, "cell_steps": cell_steps}
+
+
+def sequential(pool: MeshPool, requests, precision=CG_PREC_FP64):
+    """One request at a time in arrival order (cg_create + cg_step per request).
+    Service times are measured; waiting follows from the arrival times."""
+    results = []
+    clock = 0.0
+    for r in sorted(requests, key=lambda q: q.arrival):
+        m = pool.get(r.spec)
+        t0 = time.perf_counter()
+        g = IdealGasSolver(m, r.spec.deck(precision, check_interval=100))
+        g.set_state(*r.spec.state(m))
+        g.init_complete()
+        g.advance_solution(r.spec.steps)
+        d = g.report()
+        g.close()
+        service = time.perf_counter() - t0
+        start = max(clock, r.arrival)
+        clock = start + service
+        results.append(Result(r.rid, r.arrival, start, clock, r.spec.n ** 3, r.spec.steps, d.ke, d.enstrophy, d.time))
+    return results
+
+
+def summarize(results, makespan=None):
+    tts = np.array([r.tts for r in results])
+    # both executions measure time from the same origin (the arrival clock's zero)
+    span = makespan if makespan is not None else max(r.done for r in results)
+    return {"solves": len(results), "makespan_s": span, "solves_per_s": len(results) / span,
+            "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
+            "tts_mean_s": float(tts.mean())}

@@ -103,3 +103,19 @@ def test_oracle_library_loads():
     L = orc.lib()
     for n in ("og_create", "og_step", "og_calc_rhs", "og_flux_skew", "og_flux_symmetric", "og_flux_compact"):
         assert hasattr(L, n)
+
+
+def test_precision_key_and_batch_errors_without_gpu():
+    assert ig.parse_params("PRECISION MIXED").precision == ig.CG_PREC_MIXED
+    assert ig.parse_params("PRECISION FP64").precision == ig.CG_PREC_FP64
+    with pytest.raises(ig.CharlesError):
+        ig.parse_params("PRECISION HALF")
+    from nextgen_amd import batch
+    lib = batch._lib()
+    h = C.c_void_p()
+    assert lib.cg_batch_create(7, C.byref(h)) == -1
+    rc = lib.cg_batch_create(ig.CG_PREC_FP64, C.byref(h))
+    if rc == 0:  # a GPU is present
+        lib.cg_batch_destroy(h)
+    else:
+        assert rc == -3 and lib.cg_error_string()
