@@ -39,8 +39,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-stage updates/s FP64 vs mixed @2.1M cells; ensemble solves/s + p99 latency"
 KERNEL_NAMES = {
-    "flux_compact": "k_flux_compact: compact-face flux gather (skew + symmetric + viscous/LF/a_sgs closure)",
-    "flux_ext_rk": "k_flux_lds: extended-face flux gather + RK3 update + EOS recovery",
+    "flux_compact": "k_cpair: compact-pair flux, thread per pair (skew + symmetric + viscous/LF/a_sgs closure)",
+    "flux_ext_rk": "k_flux_lds: compact-pair flux gather + extended-face flux + RK3 update + EOS recovery",
 }
 
 

@@ -240,8 +240,15 @@ int cg_batch_get_state(cg_batch* b, int id, double* rho, double* u, double* rhoE
  * instances whose extended-face pass stages a halo in LDS */
 int cg_batch_counts(cg_batch* b, int64_t counts[4]);
 /* "graph" 0/1: replay steps from a captured HIP graph (default 1);
- * "lds" 0/1: extended-face pass with LDS halo staging (default 1) */
+ * "lds" 0/1: extended-face pass with LDS halo staging (default 1);
+ * "tile" 2/4/8: extended half-faces fetched per batch in the LDS pass (default 4).
+ * Every setting gives bit-identical results. */
 int cg_batch_set_option(cg_batch* b, const char* name, int value);
+/* Autotuner: time nsteps steps of the batch under each variant (LDS pass with
+ * tile 2, 4, 8; global-gather pass) and keep the fastest. The tuning steps
+ * are ordinary steps (the instances advance). A text table of the timings and
+ * the choice is written to report (report_len bytes, NUL-terminated). */
+int cg_batch_tune(cg_batch* b, int nsteps, char* report, int64_t report_len);
 /* time nsteps steps of the whole batch with HIP events (graph built before) */
 int cg_batch_time_steps(cg_batch* b, int nsteps, double* ms_total);
 const char* cg_batch_last_error(const cg_batch* b);

@@ -17,7 +17,8 @@ from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, CharlesError, Diag, Mesh, cg
 BATCH_EXPORTS = [
     "cg_batch_create", "cg_batch_add", "cg_batch_remove", "cg_batch_step", "cg_batch_get_diag",
     "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
-    "cg_batch_last_error", "cg_batch_destroy",
+    "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
+    "cg_batch_attach",
 ]
 
 _ready = False
@@ -42,6 +43,10 @@ def _lib():
             "cg_batch_time_steps": (C.c_int, [vp, C.c_int, dp]),
             "cg_batch_last_error": (C.c_char_p, [vp]),
             "cg_batch_destroy": (None, [vp]),
+            "cg_instance_create": (C.c_int, [vp, P(cg_params), dp, dp, dp, P(vp)]),
+            "cg_instance_bytes": (C.c_int, [vp, P(C.c_int64)]),
+            "cg_instance_destroy": (None, [vp]),
+            "cg_batch_attach": (C.c_int, [vp, vp, P(C.c_int)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -53,6 +58,41 @@ def _lib():
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Instance:
+    """A solve prepared outside any batch (cg_instance_create): device layout,
+    state and initComplete. Safe to build on a worker thread while a batch
+    steps; hand it to a batch with Batch.attach."""
+
+    def __init__(self, mesh: Mesh, params: cg_params, rho, u, rhoE):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(mesh.ncv, 3)
+        rhoE = np.ascontiguousarray(rhoE, dtype=np.float64)
+        self.ncv = mesh.ncv
+        self.precision = params.precision
+        h = C.c_void_p()
+        rc = _lib().cg_instance_create(mesh._h, C.byref(params), _dptr(rho), _dptr(u), _dptr(rhoE), C.byref(h))
+        if rc:
+            raise CharlesError(f"error {rc}: {load_library().cg_error_string().decode()}")
+        self._h = h
+
+    def bytes(self) -> int:
+        b = C.c_int64()
+        if _lib().cg_instance_bytes(self._h, C.byref(b)):
+            raise CharlesError(load_library().cg_error_string().decode())
+        return b.value
+
+    def close(self):
+        if self._h:
+            _lib().cg_instance_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
@@ -80,6 +120,14 @@ class Batch:
         self._chk(_lib().cg_batch_add(self._h, mesh._h, C.byref(params), _dptr(rho), _dptr(u), _dptr(rhoE),
                                       C.byref(i)))
         self.ncv[i.value] = mesh.ncv
+        return i.value
+
+    def attach(self, inst: Instance) -> int:
+        """Hand a prepared instance to the batch (which owns it from then on)."""
+        i = C.c_int()
+        self._chk(_lib().cg_batch_attach(self._h, inst._h, C.byref(i)))
+        inst._h = None
+        self.ncv[i.value] = inst.ncv
         return i.value
 
     def remove(self, iid: int):
@@ -125,4 +173,4 @@ class Batch:
             pass
 
 
-__all__ = ["Batch", "BATCH_EXPORTS", "CG_PREC_FP64", "CG_PREC_MIXED"]
+__all__ = ["Batch", "Instance", "BATCH_EXPORTS", "CG_PREC_FP64", "CG_PREC_MIXED"]

@@ -100,7 +100,7 @@ struct Instance : HostInst {
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   int* solo_bmap = nullptr;          // its cell blocks in XCD order
   int* solo_pbmap = nullptr;         // its compact-pair blocks in XCD order
-  cgk::Launch<R> solo_launch() const { return {solo, solo_bmap, nblocks, 1, lds, solo_pbmap, npblocks}; }
+  cgk::Launch<R> solo_launch() const { return {solo, solo_bmap, nblocks, 1, lds, solo_pbmap, npblocks, 4}; }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
     r.g = b.g;
@@ -128,6 +128,7 @@ struct Engine {
   int cur = 0;            // st[cur] holds the current state of every instance
   bool dirty = true;      // table / block map need rebuilding
   bool force_global = false;
+  int tile = 4;           // extended slots per batch in the LDS pass
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two captured steps from parity cur (which they return to)
 };
 
@@ -653,7 +654,7 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb};
+  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile};
   E.dirty = false;
   return 0;
 }
@@ -1705,7 +1706,44 @@ int cg_batch_set_option(cg_batch* b, const char* name, int value) {
       return 0;
     });
   }
+  if (n == "tile") {
+    if (value != 2 && value != 4 && value != 8) return b->set(CG_EINVAL, "tile must be 2, 4 or 8");
+    return with_engine(b, [&](auto& E) -> int {
+      E.tile = value;
+      E.dirty = true;
+      return 0;
+    });
+  }
   return b->set(CG_EINVAL, "unknown option " + n);
+}
+
+int cg_batch_tune(cg_batch* b, int nsteps, char* report, int64_t report_len) {
+  if (!b || nsteps < 2) return fail(CG_EINVAL, "bad argument");
+  struct Variant { int lds, tile; double ms; };
+  std::vector<Variant> vs = {{1, 2, 0}, {1, 4, 0}, {1, 8, 0}, {0, 4, 0}};
+  const bool graph = b->use_graph;
+  b->use_graph = true;
+  std::string text;
+  for (auto& v : vs) {
+    if (int rc = cg_batch_set_option(b, "lds", v.lds)) return rc;
+    if (int rc = cg_batch_set_option(b, "tile", v.tile)) return rc;
+    // a variant that cannot use LDS (an instance without a halo) runs the global pass
+    if (int rc = cg_batch_time_steps(b, nsteps, &v.ms)) return rc;
+    text += "lds " + std::to_string(v.lds) + " tile " + std::to_string(v.tile) + ": " +
+            std::to_string(v.ms / nsteps) + " ms/step\n";
+  }
+  const Variant* best = &vs[0];
+  for (const auto& v : vs)
+    if (v.ms < best->ms) best = &v;
+  b->use_graph = graph;
+  if (int rc = cg_batch_set_option(b, "lds", best->lds)) return rc;
+  if (int rc = cg_batch_set_option(b, "tile", best->tile)) return rc;
+  text += "chosen: lds " + std::to_string(best->lds) + " tile " + std::to_string(best->tile) + "\n";
+  if (report && report_len > 0) {
+    std::strncpy(report, text.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = '\0';
+  }
+  return CG_OK;
 }
 
 int cg_batch_time_steps(cg_batch* b, int nsteps, double* ms_total) {

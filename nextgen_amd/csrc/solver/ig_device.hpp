@@ -159,6 +159,7 @@ struct Launch {
   size_t lds;
   const int* pbmap;  // compact-pair blocks
   int npblk;
+  int tile;          // extended slots fetched per batch in the LDS pass (2, 4 or 8)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

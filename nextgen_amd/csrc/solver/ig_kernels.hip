@@ -54,6 +54,25 @@ template <> struct V2<float> { using T = float2; };
 __device__ __forceinline__ double2 mk2(double a, double b) { return make_double2(a, b); }
 __device__ __forceinline__ float2 mk2(float a, float b) { return make_float2(a, b); }
 
+// Pointers read from the instance table are generic (flat) pointers; every
+// one of them points to device global memory. G(p) re-types a pointer into
+// the global address space at the point of access, so loads and stores are
+// global_load / global_store (flat accesses also wait on the scalar-memory
+// counter) and atomics can be combined per wavefront.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// Running maximum of non-negative doubles by their bit patterns. The atomic
+// goes through a global-address-space pointer so that the compiler combines
+// the lanes of a wavefront into one atomic (a flat atomic would stay one per
+// lane, all on the same address).
+__device__ __forceinline__ void cfl_max(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_max((__attribute__((address_space(1))) unsigned long long*)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Instance and instance-local block of this workgroup. The host orders the
 // block map so that each XCD (workgroups are dealt round-robin over the 8
 // XCDs) works on one contiguous stretch of the concatenated Morton orders.
@@ -67,41 +86,27 @@ __device__ __forceinline__ const Inst<R>& inst_of(const Inst<R>* __restrict__ I,
 
 // streamed once per launch: keep them from evicting reusable neighbour records
 template <class T>
-__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(G(p)); }
 
 template <class R>
 struct Light {  // IdealGasState (+ rho, sos); p and h gauged in mixed precision
   R u[3], spv, p, h, rho, sos;
 };
 
-__device__ __forceinline__ Light<double> load_light(const double* __restrict__ L, int c) {
-  const double2* q = reinterpret_cast<const double2*>(L + (size_t)c * L_N);
-  const double2 a = q[0], b = q[1], d = q[2], e = q[3];
-  Light<double> r;
-  r.u[0] = a.x; r.u[1] = a.y; r.u[2] = b.x; r.spv = b.y; r.p = d.x; r.h = d.y; r.rho = e.x; r.sos = e.y;
+// Record access element by element through the global address space; the
+// load/store vectorizer merges the consecutive elements into 16-byte accesses.
+template <class R>
+__device__ __forceinline__ Light<R> load_light(const R* __restrict__ L, int c) {
+  const auto q = G(L + (size_t)c * L_N);
+  Light<R> r;
+  r.u[0] = q[0]; r.u[1] = q[1]; r.u[2] = q[2]; r.spv = q[3]; r.p = q[4]; r.h = q[5]; r.rho = q[6]; r.sos = q[7];
   return r;
 }
 
-__device__ __forceinline__ Light<float> load_light(const float* __restrict__ L, int c) {
-  const float4* q = reinterpret_cast<const float4*>(L + (size_t)c * L_N);
-  const float4 a = q[0], b = q[1];
-  Light<float> r;
-  r.u[0] = a.x; r.u[1] = a.y; r.u[2] = a.z; r.spv = a.w; r.p = b.x; r.h = b.y; r.rho = b.z; r.sos = b.w;
-  return r;
-}
-
-__device__ __forceinline__ void store_light(double* __restrict__ L, int c, const Light<double>& r) {
-  double2* q = reinterpret_cast<double2*>(L + (size_t)c * L_N);
-  q[0] = make_double2(r.u[0], r.u[1]);
-  q[1] = make_double2(r.u[2], r.spv);
-  q[2] = make_double2(r.p, r.h);
-  q[3] = make_double2(r.rho, r.sos);
-}
-
-__device__ __forceinline__ void store_light(float* __restrict__ L, int c, const Light<float>& r) {
-  float4* q = reinterpret_cast<float4*>(L + (size_t)c * L_N);
-  q[0] = make_float4(r.u[0], r.u[1], r.u[2], r.spv);
-  q[1] = make_float4(r.p, r.h, r.rho, r.sos);
+template <class R>
+__device__ __forceinline__ void store_light(R* __restrict__ L, int c, const Light<R>& r) {
+  const auto q = G(L + (size_t)c * L_N);
+  q[0] = r.u[0]; q[1] = r.u[1]; q[2] = r.u[2]; q[3] = r.spv; q[4] = r.p; q[5] = r.h; q[6] = r.rho; q[7] = r.sos;
 }
 
 template <class R>
@@ -110,23 +115,14 @@ struct Aux {  // AuxGasProp lagged part + velocity gradient
   R g[9];
 };
 
-__device__ __forceinline__ Aux<double> load_aux(const double* __restrict__ A, int c) {
-  const double2* q = reinterpret_cast<const double2*>(A + (size_t)c * A_N);
-  const double2 a = q[0], b = q[1], d = q[2], e = q[3], f = q[4], h = q[5], i = q[6], j = q[7];
-  Aux<double> r;
-  r.mu_total = a.x; r.loc_total = a.y; r.a_sgs = b.x; r.fgr = b.y; r.symh = d.x; r.symp = d.y;
-  r.g[0] = e.y; r.g[1] = f.x; r.g[2] = f.y; r.g[3] = h.x; r.g[4] = h.y; r.g[5] = i.x; r.g[6] = i.y;
-  r.g[7] = j.x; r.g[8] = j.y;
-  return r;
-}
-
-__device__ __forceinline__ Aux<float> load_aux(const float* __restrict__ A, int c) {
-  const float4* q = reinterpret_cast<const float4*>(A + (size_t)c * A_N);
-  const float4 a = q[0], b = q[1], d = q[2], e = q[3];
-  Aux<float> r;
-  r.mu_total = a.x; r.loc_total = a.y; r.a_sgs = a.z; r.fgr = a.w; r.symh = b.x; r.symp = b.y;
-  r.g[0] = b.w; r.g[1] = d.x; r.g[2] = d.y; r.g[3] = d.z; r.g[4] = d.w; r.g[5] = e.x; r.g[6] = e.y;
-  r.g[7] = e.z; r.g[8] = e.w;
+template <class R>
+__device__ __forceinline__ Aux<R> load_aux(const R* __restrict__ A, int c) {
+  const auto q = G(A + (size_t)c * A_N);
+  Aux<R> r;
+  r.mu_total = q[A_MU]; r.loc_total = q[A_LOC]; r.a_sgs = q[A_ASGS]; r.fgr = q[A_FGR]; r.symh = q[A_SYMH];
+  r.symp = q[A_SYMP];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r.g[k] = q[A_D + k];
   return r;
 }
 
@@ -309,27 +305,27 @@ template <int STAGE>
 __device__ __forceinline__ void rk_update(const Geom& g, const State& in, const State& out, const Work& w,
                                           const Params& P, int c, const Light<double>& self, const double acc[5]) {
   const int ncv = g.ncv;
-  const double dt = w.info->dt;
-  const double tmp = dt * g.inv_vol[c];
+  const double dt = G(w.info)->dt;
+  const double tmp = dt * G(g.inv_vol)[c];
   double agg[5];
   if (STAGE == 1) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       agg[q] = tmp * (1.0 * acc[q] + 0.0 * 0.0 + 0.0 * 0.0);
-      w.rhs0[(size_t)q * ncv + c] = acc[q];
+      G(w.rhs0)[(size_t)q * ncv + c] = acc[q];
     }
   } else if (STAGE == 2) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      const double r0 = w.rhs0[(size_t)q * ncv + c];
+      const double r0 = G(w.rhs0)[(size_t)q * ncv + c];
       agg[q] = tmp * (-0.75 * r0 + 0.25 * acc[q] + 0.0 * 0.0);
-      w.rhs1[(size_t)q * ncv + c] = acc[q];
+      G(w.rhs1)[(size_t)q * ncv + c] = acc[q];
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      const double r0 = w.rhs0[(size_t)q * ncv + c];
-      const double r1 = w.rhs1[(size_t)q * ncv + c];
+      const double r0 = G(w.rhs0)[(size_t)q * ncv + c];
+      const double r1 = G(w.rhs1)[(size_t)q * ncv + c];
       agg[q] = tmp * ((-1.0 / 12.0) * r0 + (-1.0 / 12.0) * r1 + (8.0 / 12.0) * acc[q]);
     }
   }
@@ -339,12 +335,12 @@ __device__ __forceinline__ void rk_update(const Geom& g, const State& in, const 
   double u[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) u[i] = (rho_old * self.u[i] + agg[1 + i]) * inv_rho;
-  const double rhoE = in.rhoE[c] + agg[4];
+  const double rhoE = G(in.rhoE)[c] + agg[4];
   double T;
   const Light<double> L = eos(P, rho, u, rhoE, T);
   store_light(out.light, c, L);
-  out.rhoE[c] = rhoE;
-  if (STAGE == 3) w.logp[c] = log(L.p * P.inv_pref);
+  G(out.rhoE)[c] = rhoE;
+  if (STAGE == 3) G(w.logp)[c] = log(L.p * P.inv_pref);
 }
 
 // Mixed precision: FP32 registers (energy row gauged), FP64 combination,
@@ -355,27 +351,27 @@ __device__ __forceinline__ void rk_update(const GeomT<float>& g, const StateT<fl
                                           const WorkT<float>& w, const Params& P, int c, const Light<float>&,
                                           const float acc[5]) {
   const int ncv = g.ncv;
-  const double dt = w.info->dt;
-  const double tmp = dt * g.inv_vol[c];
+  const double dt = G(w.info)->dt;
+  const double tmp = dt * G(g.inv_vol)[c];
   double s[5];
   if (STAGE == 1) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       s[q] = double(acc[q]);
-      w.rhs0[(size_t)q * ncv + c] = acc[q];
+      G(w.rhs0)[(size_t)q * ncv + c] = acc[q];
     }
   } else if (STAGE == 2) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      const double r0 = double(w.rhs0[(size_t)q * ncv + c]);
+      const double r0 = double(G(w.rhs0)[(size_t)q * ncv + c]);
       s[q] = -0.75 * r0 + 0.25 * double(acc[q]);
-      w.rhs1[(size_t)q * ncv + c] = acc[q];
+      G(w.rhs1)[(size_t)q * ncv + c] = acc[q];
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      const double r0 = double(w.rhs0[(size_t)q * ncv + c]);
-      const double r1 = double(w.rhs1[(size_t)q * ncv + c]);
+      const double r0 = double(G(w.rhs0)[(size_t)q * ncv + c]);
+      const double r1 = double(G(w.rhs1)[(size_t)q * ncv + c]);
       s[q] = (-1.0 / 12.0) * r0 + (-1.0 / 12.0) * r1 + (8.0 / 12.0) * double(acc[q]);
     }
   }
@@ -383,17 +379,17 @@ __device__ __forceinline__ void rk_update(const GeomT<float>& g, const StateT<fl
   double agg[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) agg[q] = tmp * s[q];
-  const double rho_old = in.cons[c];
+  const double rho_old = G(in.cons)[c];
   const double rho = rho_old + agg[0];
   const double inv_rho = 1.0 / rho;
   double u[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) u[i] = (rho_old * in.cons[(size_t)(1 + i) * ncv + c] + agg[1 + i]) * inv_rho;
-  const double rhoE = in.cons[(size_t)4 * ncv + c] + agg[4];
-  out.cons[c] = rho;
+  for (int i = 0; i < 3; ++i) u[i] = (rho_old * G(in.cons)[(size_t)(1 + i) * ncv + c] + agg[1 + i]) * inv_rho;
+  const double rhoE = G(in.cons)[(size_t)4 * ncv + c] + agg[4];
+  G(out.cons)[c] = rho;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) out.cons[(size_t)(1 + i) * ncv + c] = u[i];
-  out.cons[(size_t)4 * ncv + c] = rhoE;
+  for (int i = 0; i < 3; ++i) G(out.cons)[(size_t)(1 + i) * ncv + c] = u[i];
+  G(out.cons)[(size_t)4 * ncv + c] = rhoE;
   double T;
   store_light(out.light, c, to_mixed(P, eos(P, rho, u, rhoE, T)));
 }
@@ -414,7 +410,7 @@ __device__ __forceinline__ void write_rhs(double* rhs_out, int ncv, int c, const
 
 // Compact pairs, part 1 of a stage: thread per compact pair (the Voronoi
 // faces, pair indices 0..nfa-1), skew + compact closure + symmetric flux in
-// the pair's orientation lo -> hi, written to w.cflux[pair]. Evaluated once
+// the pair's orientation lo -> hi, written to G(w.cflux)[pair]. Evaluated once
 // per pair; both cells add the same bits in part 2.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I, const int* __restrict__ pbmap,
@@ -427,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   const R p_g = kMixed<R> ? R(X.P.p_g) : R(0);
   const int f = lblk * kBlock + threadIdx.x;
   if (f >= g.nfa) return;
-  if (w.info->nan) return;
+  if (G(w.info)->nan) return;
   const int lo = ldnt(g.plo + f), hi = ldnt(g.phi + f);
   const Light<R> a = load_light(in.light, lo);
   const Light<R> b = load_light(in.light, hi);
@@ -445,27 +441,24 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
 #pragma unroll
     for (int q = 0; q < 5; ++q) F[q] += cF[q];
   }
-  using T2 = typename V2<R>::T;
-  T2* o = reinterpret_cast<T2*>(w.cflux + (size_t)f * 6);
-  o[0] = mk2(F[0], F[1]);
-  o[1] = mk2(F[2], F[3]);
-  o[2] = mk2(F[4], R(0));
+  const auto o = G(w.cflux + (size_t)f * 6);
+#pragma unroll
+  for (int q = 0; q < 5; ++q) o[q] = F[q];
+  o[5] = R(0);
 }
 
 // A cell's compact-pair fluxes in pair order (its first ef_nc pairs): the
 // start of its stage RHS accumulation.
 template <class R>
 __device__ __forceinline__ void gather_compact(const GeomT<R>& g, const WorkT<R>& w, int c, R acc[5]) {
-  using T2 = typename V2<R>::T;
-  const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
-  const int nc = g.ef_nc[c];
+  const int cbase = G(g.cf_ptr)[c >> 6] + (c & (kSlice - 1));
+  const int nc = G(g.ef_nc)[c];
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
   for (int k = 0; k < nc; ++k) {
     const int code = ldnt(g.cf_pair + cbase + k * kSlice);
-    const T2* fl = reinterpret_cast<const T2*>(w.cflux + (size_t)(code & 0x7fffffff) * 6);
-    const T2 f0 = fl[0], f1 = fl[1], f2 = fl[2];
-    const R F[5] = {f0.x, f0.y, f1.x, f1.y, f2.x};
+    const auto fl = G(w.cflux + (size_t)(code & 0x7fffffff) * 6);
+    const R F[5] = {fl[0], fl[1], fl[2], fl[3], fl[4]};
     if (code < 0) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) acc[q] += F[q];
@@ -491,10 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
   const Params& P = X.P;
   const int c = lblk * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
-  if (MODE == 1 && w.info->nan) return;
+  if (MODE == 1 && G(w.info)->nan) return;
   const int lane = c & (kSlice - 1);
-  const int base = g.xf_ptr[c >> 6] + lane;
-  const int deg = g.xf_deg[c];
+  const int base = G(g.xf_ptr)[c >> 6] + lane;
+  const int deg = G(g.xf_deg)[c];
   const int ncv = g.ncv;
 
   const Light<R> self = load_light(in.light, c);
@@ -539,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
 // light record of every cell its extended half-faces reach (its halo, ~4x its own
 // cells) into LDS, then every gather is an LDS read. xf_loc holds the halo
 // position of each slot's neighbour (bit 15: the cell is the pair's hi side).
-template <class R, int STAGE, int MODE>
+template <class R, int STAGE, int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                      int cur, double* rhs_out) {
   using T2 = typename V2<R>::T;
@@ -551,28 +544,28 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
   const Params& P = X.P;
-  if (MODE == 1 && w.info->nan) return;
-  const int hb = g.halo_ptr[blk], hs = g.halo_ptr[blk + 1] - hb;
+  if (MODE == 1 && G(w.info)->nan) return;
+  const int hb = G(g.halo_ptr)[blk], hs = G(g.halo_ptr)[blk + 1] - hb;
   for (int i = threadIdx.x; i < hs; i += kBlock) {
-    const int gc = g.halo_v[hb + i];
-    const T2* q = reinterpret_cast<const T2*>(in.light + (size_t)gc * L_N);
-    halo[3 * i + 0] = q[0];
-    halo[3 * i + 1] = q[1];
-    halo[3 * i + 2] = q[2];
+    const int gc = G(g.halo_v)[hb + i];
+    const auto q = G(in.light + (size_t)gc * L_N);
+    halo[3 * i + 0] = mk2(q[0], q[1]);
+    halo[3 * i + 1] = mk2(q[2], q[3]);
+    halo[3 * i + 2] = mk2(q[4], q[5]);
   }
   __syncthreads();
   const int c = blk * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
   const int lane = c & (kSlice - 1);
-  const int base = g.xf_ptr[c >> 6] + lane;
-  const int deg = g.xf_deg[c];
+  const int base = G(g.xf_ptr)[c >> 6] + lane;
+  const int deg = G(g.xf_deg)[c];
   const int ncv = g.ncv;
   const Light<R> self = load_light(in.light, c);
   R acc[5];
   gather_compact(g, w, c, acc);
-  // slots are fetched kBatch at a time so their global loads overlap; the
-  // accumulation order stays k = 0, 1, ...
-  constexpr int kBatch = 4;
+  // slots are fetched NB at a time so their global loads overlap (NB is a
+  // tuning choice); the accumulation order stays k = 0, 1, ...
+  constexpr int kBatch = NB;
   for (int k0 = 0; k0 < deg; k0 += kBatch) {
     int code[kBatch];
     R n[kBatch][3], cv[kBatch][3];
@@ -629,12 +622,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
   const int c = lblk * kBlock + threadIdx.x;
-  if (c == 0) w.info->cfl_bits = 0ull;  // reset for the properties pass that follows
+  if (c == 0) G(w.info)->cfl_bits = 0ull;  // reset for the properties pass that follows
   if (c >= g.ncv) return;
-  if (w.info->nan) return;
-  const int base = g.gr_ptr[c >> 6] + (c & (kSlice - 1));
-  const int deg = g.gr_deg[c];
-  const R* L = st.light;
+  if (G(w.info)->nan) return;
+  const int base = G(g.gr_ptr)[c >> 6] + (c & (kSlice - 1));
+  const int deg = G(g.gr_deg)[c];
+  const auto L = G(st.light);
   const R ui[3] = {L[(size_t)c * L_N + L_U0], L[(size_t)c * L_N + L_U1], L[(size_t)c * L_N + L_U2]};
   R d[9];
 #pragma unroll
@@ -649,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
 #pragma unroll
       for (int j = 0; j < 3; ++j) d[i * 3 + j] += gc[j] * (un[i] - ui[i]);
   }
-  R* a = w.aux + (size_t)c * A_N + A_D;
+  const auto a = G(w.aux + (size_t)c * A_N + A_D);
 #pragma unroll
   for (int k = 0; k < 9; ++k) a[k] = d[k];
 }
@@ -670,13 +663,13 @@ __global__ __launch_bounds__(kBlock) void k_eos(const Inst<R>* __restrict__ I, c
   double T;
   if constexpr (kMixed<R>) {
     const int ncv = g.ncv;
-    const double u[3] = {st.cons[(size_t)ncv + c], st.cons[(size_t)2 * ncv + c], st.cons[(size_t)3 * ncv + c]};
-    store_light(st.light, c, to_mixed(P, eos(P, st.cons[c], u, st.cons[(size_t)4 * ncv + c], T)));
+    const double u[3] = {G(st.cons)[(size_t)ncv + c], G(st.cons)[(size_t)2 * ncv + c], G(st.cons)[(size_t)3 * ncv + c]};
+    store_light(st.light, c, to_mixed(P, eos(P, G(st.cons)[c], u, G(st.cons)[(size_t)4 * ncv + c], T)));
   } else {
     const Light<double> in = load_light(st.light, c);
-    const Light<double> L = eos(P, in.rho, in.u, st.rhoE[c], T);
+    const Light<double> L = eos(P, in.rho, in.u, G(st.rhoE)[c], T);
     store_light(st.light, c, L);
-    w.logp[c] = log(L.p * P.inv_pref);
+    G(w.logp)[c] = log(L.p * P.inv_pref);
   }
 }
 
@@ -692,8 +685,8 @@ __global__ __launch_bounds__(kBlock) void k_prims(const Inst<R>* __restrict__ I,
   Light<double> L;
   double T;
   if constexpr (kMixed<R>) {
-    const double u[3] = {st.cons[(size_t)ncv + c], st.cons[(size_t)2 * ncv + c], st.cons[(size_t)3 * ncv + c]};
-    L = eos(P, st.cons[c], u, st.cons[(size_t)4 * ncv + c], T);
+    const double u[3] = {G(st.cons)[(size_t)ncv + c], G(st.cons)[(size_t)2 * ncv + c], G(st.cons)[(size_t)3 * ncv + c]};
+    L = eos(P, G(st.cons)[c], u, G(st.cons)[(size_t)4 * ncv + c], T);
   } else {
     L = load_light(st.light, c);
     T = L.p / (L.rho * P.R_gas);
@@ -719,7 +712,13 @@ __device__ __forceinline__ R fast_pow_posx(R x, R p) {
 // Mixed precision evaluates it in FP32 on the gauged records; its Gibbs
 // remainder takes ln p1 - ln p0 as log1p((p1' - p0') / p0), which keeps the
 // pressure difference exact instead of subtracting two rounded logarithms.
-template <class R, int FIELDS>
+//
+// LDS = true: the block first stages p, sp_vol, rho (and logp in FP64) of
+// its extended-face halo in LDS (the halo of the extended flux pass), and the
+// sensor reads its extended pairs from there; compact pairs (sensor and CFL)
+// gather the full neighbour record from memory. The values are the same bits
+// either way, and the sensor is a maximum, so both variants agree exactly.
+template <class R, int FIELDS, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                   int cur) {
   int lblk;
@@ -728,6 +727,20 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
   const Params& P = X.P;
+  extern __shared__ __align__(16) unsigned char smem[];
+  R* hrec = reinterpret_cast<R*>(smem);  // [hsize][4]: p, sp_vol, rho, logp
+  if constexpr (LDS) {
+    const int hb = G(g.halo_ptr)[lblk], hs = G(g.halo_ptr)[lblk + 1] - hb;
+    for (int i = threadIdx.x; i < hs; i += kBlock) {
+      const int gc = G(g.halo_v)[hb + i];
+      const auto q = G(st.light + (size_t)gc * L_N);
+      hrec[4 * i + 0] = q[L_P];
+      hrec[4 * i + 1] = q[L_SPV];
+      hrec[4 * i + 2] = q[L_RHO];
+      if constexpr (!kMixed<R>) hrec[4 * i + 3] = G(w.logp)[gc];
+    }
+    __syncthreads();
+  }
   const int c = lblk * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
   const R Rg = R(P.R_gas);
@@ -737,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   const R p = kMixed<R> ? self.p + pg : self.p;  // absolute pressure
   const R* u = self.u;
   const R T = p / (rho * Rg);
-  R* auxc = w.aux + (size_t)c * A_N;
+  const auto auxc = G(w.aux + (size_t)c * A_N);
 
   // ---- SGS ----
   R mu_sgs = R(0), a_sgs = R(0), loc_sgs = R(0);
@@ -745,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
     R d[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) d[k] = auxc[A_D + k];
-    const R dx = R(g.dx13[c]);
+    const R dx = R(G(g.dx13)[c]);
     const R dx2 = dx * dx;
     const R alpha11 = d[0], alpha11_2 = alpha11 * alpha11;
     const R alpha22 = d[4], alpha22_2 = alpha22 * alpha22;
@@ -775,11 +788,11 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
     const R tmp_ = mabs(dil_) * dx / sos;
     if (tmp_ > R(0.01)) {
       const R Masq = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * inv_sos2;
-      a_sgs += R(P.a_sgs_coeff2) * R(g.sa_to_vol[c]) * mfmin(Masq, R(10)) * sos;
+      a_sgs += R(P.a_sgs_coeff2) * R(G(g.sa_to_vol)[c]) * mfmin(Masq, R(10)) * sos;
     }
     a_sgs *= inv_sos2;
     loc_sgs = R(P.lambda_dil_coeff) * rho * mabs(dil_) * dx2;
-    const double dt = w.info->dt;
+    const double dt = G(w.info)->dt;
     if (P.limit_asgs && dt > 0.0) {
       const R cfl_visc = R(1);
       const R usq = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
@@ -795,56 +808,73 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   const R loc_lam = mu_lam * R(P.invPr_lam);
   const R sos2 = R(P.gamma) * p / rho;
   const R lambda = msqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + msqrt(sos2);
-  const R symp = smax(R(P.coeff_p) * R(g.dcmag[c]) * lambda / sos2 - R(g.dx23[c]) * a_sgs, R(0));
-  const R symh = R(P.coeff_h) * R(g.dcmag[c]);
+  const R symp = smax(R(P.coeff_p) * R(G(g.dcmag)[c]) * lambda / sos2 - R(G(g.dx23)[c]) * a_sgs, R(0));
+  const R symh = R(P.coeff_h) * R(G(g.dcmag)[c]);
 
   // ---- Gibbs-remainder sensor and convective CFL over the extended half-faces ----
   const int lane = c & (kSlice - 1);
-  const int cbase = g.cf_ptr[c >> 6] + lane;
-  const int xbase = g.xf_ptr[c >> 6] + lane;
-  const int nc = g.ef_nc[c];
-  const int nx = g.xf_deg[c];
+  const int cbase = G(g.cf_ptr)[c >> 6] + lane;
+  const int xbase = G(g.xf_ptr)[c >> 6] + lane;
+  const int nc = G(g.ef_nc)[c];
+  const int nx = G(g.xf_deg)[c];
   R lp_s = R(0);
-  if constexpr (!kMixed<R>) lp_s = w.logp[c];
+  if constexpr (!kMixed<R>) lp_s = G(w.logp)[c];
   const R irt_s = R(1) / (Rg * T);
   R vv2 = R(0), cfl = R(0);
-  // sensor: max over every extended pair (order-independent); CFL: compact faces in pair order
-  for (int k = 0; k < nc + nx; ++k) {
-    const bool compact = k < nc;
-    const int slot = compact ? cbase + k * kSlice : xbase + (k - nc) * kSlice;
-    const int code = compact ? ldnt(g.cf_nbr + slot) : ldnt(g.xf_nbr + slot);
-    const int nb = code & 0x7fffffff;
-    const bool self_hi = code < 0;
-    const Light<R> o = load_light(st.light, nb);
-    const R irt_o = R(1) / (Rg * ((kMixed<R> ? o.p + pg : o.p) / (o.rho * Rg)));
-    const Light<R>& a = self_hi ? o : self;
-    const Light<R>& b = self_hi ? self : o;
-    const R dp = b.p - a.p;
-    const R v_avg = R(0.5) * (a.spv + b.spv);
+  // Gibbs remainder of one pair from the neighbour's p, sp_vol, rho (and logp)
+  auto gibbs = [&](R op, R ospv, R orho, R olp, bool self_hi) {
+    const R irt_o = R(1) / (Rg * ((kMixed<R> ? op + pg : op) / (orho * Rg)));
+    const R ap = self_hi ? op : self.p, bp = self_hi ? self.p : op;
+    const R aspv = self_hi ? ospv : self.spv, bspv = self_hi ? self.spv : ospv;
+    const R dp = bp - ap;
+    const R v_avg = R(0.5) * (aspv + bspv);
     const R beta_avg = R(0.5) * ((self_hi ? irt_o : irt_s) + (self_hi ? irt_s : irt_o));
     R gr;
     if constexpr (kMixed<R>) {
-      gr = log1pf(dp / (a.p + pg)) - v_avg * dp * beta_avg;
+      (void)olp;
+      gr = log1pf(dp / (ap + pg)) - v_avg * dp * beta_avg;
     } else {
-      const double lp_o = w.logp[nb];
-      gr = (self_hi ? lp_s : lp_o) - (self_hi ? lp_o : lp_s) - v_avg * dp * beta_avg;
+      gr = (self_hi ? lp_s : olp) - (self_hi ? olp : lp_s) - v_avg * dp * beta_avg;
     }
     vv2 = smax(vv2, mabs(gr));
-    if (compact) {
-      const int f = ldnt(g.cf_pair + slot) & 0x7fffffff;
-      const R n[3] = {g.pf0[f], g.pf1[f], g.pf2[f]};
-      const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      const R undA = R(0.5) * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
-                               (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
-      const R sos_avg = R(0.5) * (a.sos + b.sos);
-      cfl += mabs(undA) + area * sos_avg;
+  };
+  // compact pairs: sensor and CFL (pair order)
+  for (int k = 0; k < nc; ++k) {
+    const int slot = cbase + k * kSlice;
+    const int code = ldnt(g.cf_nbr + slot);
+    const int nb = code & 0x7fffffff;
+    const bool self_hi = code < 0;
+    const Light<R> o = load_light(st.light, nb);
+    gibbs(o.p, o.spv, o.rho, kMixed<R> ? R(0) : R(G(w.logp)[nb]), self_hi);
+    const Light<R>& a = self_hi ? o : self;
+    const Light<R>& b = self_hi ? self : o;
+    const int f = ldnt(g.cf_pair + slot) & 0x7fffffff;
+    const R n[3] = {G(g.pf0)[f], G(g.pf1)[f], G(g.pf2)[f]};
+    const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const R undA = R(0.5) * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
+                             (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
+    const R sos_avg = R(0.5) * (a.sos + b.sos);
+    cfl += mabs(undA) + area * sos_avg;
+  }
+  // purely extended pairs: sensor only (a maximum: order-independent)
+  for (int k = 0; k < nx; ++k) {
+    const int slot = xbase + k * kSlice;
+    if constexpr (LDS) {
+      const int code = ldnt(g.xf_loc + slot);
+      const R* h = hrec + 4 * (code & 0x7fff);
+      gibbs(h[0], h[1], h[2], h[3], (code & 0x8000) != 0);
+    } else {
+      const int code = ldnt(g.xf_nbr + slot);
+      const int nb = code & 0x7fffffff;
+      const auto q = G(st.light + (size_t)nb * L_N);
+      gibbs(q[L_P], q[L_SPV], q[L_RHO], kMixed<R> ? R(0) : R(G(w.logp)[nb]), code < 0);
     }
   }
   R fgr;
   if (P.diss_mode == 1) fgr = R(0);
   else if (P.diss_mode == 2) fgr = R(0.5);
   else fgr = R(0.5) / (R(1) + mexp(R(-2) * (vv2 - R(P.gr_nrm)) / R(5.0e-03)));
-  cfl *= R(0.5) * R(1) * R(g.inv_vol[c]);
+  cfl *= R(0.5) * R(1) * R(G(g.inv_vol)[c]);
 
   auxc[A_MU] = mu_lam + mu_sgs;
   auxc[A_LOC] = loc_lam + loc_sgs;
@@ -853,16 +883,16 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   auxc[A_SYMH] = symh;
   auxc[A_SYMP] = symp;
   if (FIELDS) {
-    w.T[c] = T;
-    w.mu_lam[c] = mu_lam;
-    w.mu_sgs[c] = mu_sgs;
-    w.loc_lam[c] = loc_lam;
-    w.loc_sgs[c] = loc_sgs;
-    w.a_sgs[c] = a_sgs;
-    w.vv2[c] = vv2;
+    G(w.T)[c] = T;
+    G(w.mu_lam)[c] = mu_lam;
+    G(w.mu_sgs)[c] = mu_sgs;
+    G(w.loc_lam)[c] = loc_lam;
+    G(w.loc_sgs)[c] = loc_sgs;
+    G(w.a_sgs)[c] = a_sgs;
+    G(w.vv2)[c] = vv2;
   }
-  if (cfl == cfl) atomicMax(&w.info->cfl_bits, (unsigned long long)__double_as_longlong(double(cfl)));
-  if (rho != rho || p != p || T != T) w.info->nan = 1;
+  if (cfl == cfl) cfl_max(&w.info->cfl_bits, (unsigned long long)__double_as_longlong(double(cfl)));
+  if (rho != rho || p != p || T != T) G(w.info)->nan = 1;
 }
 
 // setDt (FlowSolver.hpp:2274-2328) and the step/time bookkeeping of
@@ -901,8 +931,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
   const WorkT<R> w = X.w;
   const int c = lblk * kBlock + threadIdx.x;
   if (c >= g.ncv) return;
-  const int cbase = g.cf_ptr[c >> 6] + (c & (kSlice - 1));
-  const int nc = g.ef_nc[c];
+  const int cbase = G(g.cf_ptr)[c >> 6] + (c & (kSlice - 1));
+  const int nc = G(g.ef_nc)[c];
   const Light<R> self = load_light(st.light, c);
   R cfl = R(0);
   for (int k = 0; k < nc; ++k) {
@@ -914,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
     const Light<R>& a = self_hi ? o : self;
     const Light<R>& b = self_hi ? self : o;
     const int f = ldnt(g.cf_pair + cslot) & 0x7fffffff;
-    const R n[3] = {g.pf0[f], g.pf1[f], g.pf2[f]};
+    const R n[3] = {G(g.pf0)[f], G(g.pf1)[f], G(g.pf2)[f]};
     const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const R undA = R(0.5) * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
                              (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
@@ -922,8 +952,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
     cfl += mabs(undA) + area * sos_avg;
   }
   double cd = double(cfl);
-  cd *= 0.5 * dt * g.inv_vol[c];
-  w.cfl[c] = cd;
+  cd *= 0.5 * dt * G(g.inv_vol)[c];
+  G(w.cfl)[c] = cd;
 }
 
 // Diagnostics: fixed-shape block reduction of 8 sums (deterministic order), FP64.
@@ -943,20 +973,20 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
     const int ncv = g.ncv;
     double rho, u[3];
     if constexpr (kMixed<R>) {
-      rho = st.cons[c];
-      u[0] = st.cons[(size_t)ncv + c];
-      u[1] = st.cons[(size_t)2 * ncv + c];
-      u[2] = st.cons[(size_t)3 * ncv + c];
+      rho = G(st.cons)[c];
+      u[0] = G(st.cons)[(size_t)ncv + c];
+      u[1] = G(st.cons)[(size_t)2 * ncv + c];
+      u[2] = G(st.cons)[(size_t)3 * ncv + c];
     } else {
       const Light<double> L = load_light(st.light, c);
       rho = L.rho;
       u[0] = L.u[0]; u[1] = L.u[1]; u[2] = L.u[2];
     }
-    const R* dr = w.aux + (size_t)c * A_N + A_D;
+    const auto dr = G(w.aux + (size_t)c * A_N + A_D);
     double d[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) d[k] = double(dr[k]);
-    const double vol = g.vol[c];
+    const double vol = G(g.vol)[c];
     const double w0 = d[2 * 3 + 1] - d[1 * 3 + 2];
     const double w1 = d[0 * 3 + 2] - d[2 * 3 + 0];
     const double w2 = d[1 * 3 + 0] - d[0 * 3 + 1];
@@ -964,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
     v[1] = rho * u[0] * vol;
     v[2] = rho * u[1] * vol;
     v[3] = rho * u[2] * vol;
-    v[4] = st.rhoE[c] * vol;
+    v[4] = G(st.rhoE)[c] * vol;
     v[5] = 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * vol;
     v[6] = 0.5 * rho * (w0 * w0 + w1 * w1 + w2 * w2) * vol;
     v[7] = vol;
@@ -978,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
       for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 8) w.partial[(size_t)lblk * 8 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 8) G(w.partial)[(size_t)lblk * 8 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // one workgroup per instance: reduce its block partials in block order
@@ -986,8 +1016,8 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict__ I) {
   __shared__ double sh[8][kBlock];
   const Inst<R>& X = I[blockIdx.x];
-  const double* partial = X.w.partial;
-  double* diag = X.w.diag;
+  const auto partial = G(X.w.partial);
+  const auto diag = G(X.w.diag);
   const int nblocks = X.nblocks;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += kBlock)
@@ -1036,10 +1066,16 @@ int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rh
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(L.nblk), block(kBlock);
   if (L.lds) {
-    if (mode == 0) hipLaunchKernelGGL((k_flux_lds<R, 1, 0>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
-    else if (stage == 1) hipLaunchKernelGGL((k_flux_lds<R, 1, 1>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
-    else if (stage == 2) hipLaunchKernelGGL((k_flux_lds<R, 2, 1>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
-    else hipLaunchKernelGGL((k_flux_lds<R, 3, 1>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
+    auto go = [&](auto nb) {
+      constexpr int NB = decltype(nb)::value;
+      if (mode == 0) hipLaunchKernelGGL((k_flux_lds<R, 1, 0, NB>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
+      else if (stage == 1) hipLaunchKernelGGL((k_flux_lds<R, 1, 1, NB>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
+      else if (stage == 2) hipLaunchKernelGGL((k_flux_lds<R, 2, 1, NB>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
+      else hipLaunchKernelGGL((k_flux_lds<R, 3, 1, NB>), grid, block, L.lds, s, L.I, L.bmap, cur, rhs_out);
+    };
+    if (L.tile == 2) go(std::integral_constant<int, 2>());
+    else if (L.tile == 8) go(std::integral_constant<int, 8>());
+    else go(std::integral_constant<int, 4>());
     return check();
   }
   if (mode == 0) hipLaunchKernelGGL((k_flux<R, 1, 0>), grid, block, 0, s, L.I, L.bmap, cur, rhs_out);
@@ -1069,10 +1105,16 @@ int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, vo
 
 template <class R>
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
-  if (fields)
-    hipLaunchKernelGGL((k_props<R, 1>), dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur);
-  else
-    hipLaunchKernelGGL((k_props<R, 0>), dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur);
+  hipStream_t s = (hipStream_t)stream;
+  // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS
+  const size_t lds = L.lds / 6 * 4;
+  if (lds) {
+    if (fields) hipLaunchKernelGGL((k_props<R, 1, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_props<R, 0, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);
+  } else {
+    if (fields) hipLaunchKernelGGL((k_props<R, 1, false>), dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_props<R, 0, false>), dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+  }
   return check();
 }
 

@@ -22,7 +22,7 @@ from __future__ import annotations
 import collections
 import concurrent.futures as cf
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -89,14 +89,21 @@ class MeshPool:
 
 
 class Server:
+    """`batch_factory(precision)` and `prepare(request)` default to the GPU
+    Batch and cg_instance_create; they are parameters so the scheduling logic
+    can be exercised without a GPU."""
+
     def __init__(self, pool: MeshPool, precision=CG_PREC_MIXED, hbm_budget=96e9, max_chunk=16,
-                 bytes_per_cell=None, log=None):
+                 bytes_per_cell=None, log=None, batch_factory=None, prepare=None):
         self.pool = pool
         self.precision = precision
         self.budget = float(hbm_budget)
         self.max_chunk = int(max_chunk)
         self.bpc = bytes_per_cell or (6000.0 if precision == CG_PREC_FP64 else 4000.0)
         self.log = log or (lambda s: None)
+        self.batch_factory = batch_factory or Batch
+        self.prepare = prepare or self._prepare
+        self.peak_bytes = 0.0
 
     def _prepare(self, req: Request):
         m = self.pool.get(req.spec)
@@ -109,7 +116,7 @@ class Server:
         active = {}             # batch id -> [request, steps left, start, bytes, cells]
         results = []
         used = 0.0
-        batch = Batch(self.precision)
+        batch = self.batch_factory(self.precision)
         pool = cf.ThreadPoolExecutor(max_workers=1)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
@@ -127,13 +134,15 @@ class Server:
                     break
                 waiting.popleft()
                 used += need
-                preparing[pool.submit(self._prepare, r)] = (r, need)
+                self.peak_bytes = max(self.peak_bytes, used)
+                preparing[pool.submit(self.prepare, r)] = (r, need)
             # attach what is ready
             for fut in [f for f in preparing if f.done()]:
                 r, need = preparing.pop(fut)
                 inst = fut.result()
                 nbytes = inst.bytes()
                 used += nbytes - need
+                self.peak_bytes = max(self.peak_bytes, used)
                 self.bpc = max(self.bpc, nbytes / r.spec.n ** 3)
                 iid = batch.attach(inst)
                 active[iid] = [r, r.spec.steps, now(), nbytes, r.spec.n ** 3]

@@ -109,3 +109,53 @@ def test_batch_variants_bitwise(option):
     for s1, s0 in zip(*runs):
         for x, y in zip(s1, s0):
             assert_same(x, y)
+
+
+def test_instance_prepared_on_worker_thread_then_attached():
+    import threading
+
+    prec = CG_PREC_MIXED
+    from nextgen_amd.batch import Instance
+
+    b = Batch(prec)
+    a = add_case(b, 0, prec)
+    box = {}
+
+    def prepare():
+        m, p, state = case(1, prec)
+        box["inst"] = Instance(m, p, *state)
+
+    t = threading.Thread(target=prepare)
+    t.start()
+    b.step(3)  # the batch keeps stepping while the worker builds
+    t.join()
+    c = b.attach(box["inst"])
+    b.step(4)
+    assert_same(b.state(a), alone(0, prec, 7)[0])
+    assert_same(b.state(c), alone(1, prec, 4)[0])
+    b.close()
+
+
+def test_served_results_equal_solo_runs():
+    from nextgen_amd.ensemble import Spec
+    from nextgen_amd.serve import MeshPool, Request, Server
+
+    specs = [Spec(n=8, mesh_seed=3, ic="tgv", Re=500.0, Ma=0.1, ic_seed=1, steps=12),
+             Spec(n=10, mesh_seed=4, ic="k4", Re=900.0, Ma=0.2, ic_seed=2, steps=7),
+             Spec(n=9, mesh_seed=5, ic="tgv", Re=1500.0, Ma=0.05, ic_seed=3, steps=20)]
+    reqs = [Request(i, 0.01 * i, sp) for i, sp in enumerate(specs)]
+    pool = MeshPool()
+    for r in reqs:
+        pool.get(r.spec)
+    res, info = Server(pool, CG_PREC_MIXED, max_chunk=4).run(reqs)
+    assert len(res) == 3
+    for r in res:
+        sp = specs[r.rid]
+        m = pool.get(sp)
+        g = IdealGasSolver(m, sp.deck(CG_PREC_MIXED, check_interval=100))
+        g.set_state(*sp.state(m))
+        g.init_complete()
+        g.advance_solution(sp.steps)
+        d = g.report()
+        g.close()
+        assert d.step == sp.steps and r.ke == d.ke and r.time == d.time and r.enstrophy == d.enstrophy
