@@ -11,6 +11,8 @@ same mesh and initial field, timed the same way.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--ic tgv|k4]
                   [--precision both|fp64|mixed] [--no-cpu]
   python bench.py --accuracy N [--ic k4] [--every E]   (mixed vs FP64 over N steps)
+  python bench.py --ensemble 256 [--ens-steps 300]       (batched ensemble vs one at a time)
+  python bench.py --serve 24 [--loads 0.5 1 2]            (serving loop vs sequential FP64)
 
 Timing: W untimed steps, then exactly K steps bracketed by stream
 synchronisation and HIP events on the solver's stream (state resident in HBM).
@@ -192,6 +194,101 @@ def accuracy(args):
     return 0
 
 
+def ensemble_bench(args):
+    """Batched ensemble: `args.ensemble` solves of uneven size in one batch (one
+    fused launch per kernel, graph replay), FP64 and mixed precision, against
+    the same solves run one at a time (a uniform sample, scaled to the set)."""
+    from nextgen_amd.batch import Batch
+    from nextgen_amd.ensemble import ensemble_specs, run_alone
+    from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
+
+    specs = ensemble_specs(args.ensemble, seed=args.seed)
+    t0 = time.time()
+    meshes = [sp.mesh() for sp in specs]
+    cells = sum(m.ncv for m in meshes)
+    log(f"ensemble: {len(specs)} meshes, {cells} cells, built in {time.time() - t0:.1f}s")
+    states = [sp.state(m) for sp, m in zip(specs, meshes)]
+    out = {"mode": "ensemble", "instances": len(specs), "cells": cells, "steps": args.ens_steps,
+           "spec": f"n ~ U{{16..48}} (jitter 0.3h), Re ~ U[400,3200], Ma ~ U[0.05,0.3], TGV or k^4 by coin flip, "
+                   f"seed {args.seed}"}
+    kes = {}
+    for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
+        t0 = time.time()
+        b = Batch(prec)
+        ids = [b.add(m, sp.deck(prec, check_interval=100), *st) for sp, m, st in zip(specs, meshes, states)]
+        setup = time.time() - t0
+        cnt = b.counts()
+        log(f"{name}: {cnt['instances']} instances on the device ({cnt['device_bytes'] / 1e9:.1f} GB) in {setup:.1f}s")
+        ms = b.time_steps(args.ens_steps)
+        kes[name] = [b.diag(i) for i in ids]
+        b.close()
+        sec = ms * 1e-3
+        out[name] = {"seconds": sec, "solves_per_s": len(specs) / sec,
+                     "cell_stage_per_s": 3.0 * cells * args.ens_steps / sec,
+                     "latency_p50_s": sec, "latency_p99_s": sec, "setup_s": setup,
+                     "device_bytes": cnt["device_bytes"]}
+        log(f"{name}: {args.ens_steps} steps of all instances in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s")
+    errs = [abs(a.ke - b.ke) / b.ke for a, b in zip(kes["mixed"], kes["fp64"])]
+    out["mixed"]["ke_rel_err_vs_fp64_max"] = max(errs)
+    # one at a time, FP64 (every 16th solve, scaled to the whole set)
+    sample = list(range(0, len(specs), max(1, len(specs) // 16)))
+    t_alone = 0.0
+    for i in sample:
+        sec, _ = run_alone(specs[i], CG_PREC_FP64, args.ens_steps)
+        t_alone += sec
+    seq = t_alone * len(specs) / len(sample)
+    out["sequential_fp64"] = {"seconds_estimate": seq, "solves_per_s": len(specs) / seq,
+                              "sample": f"{len(sample)} of {len(specs)} solves timed alone (GPU stepping only), scaled"}
+    out["speedup_vs_sequential_fp64"] = {k: out[k]["solves_per_s"] / out["sequential_fp64"]["solves_per_s"]
+                                         for k in ("fp64", "mixed")}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def serve_bench(args):
+    """Serving loop (continuous batching, mixed precision) against sequential
+    FP64 execution of the same requests, over a sweep of arrival rates."""
+    from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
+    from nextgen_amd.serve import MeshPool, Server, requests_poisson, sequential, summarize
+
+    sizes = [27, 33, 41, 50, 62, 76, 94, 115, 126]
+    pool = MeshPool()
+    base = requests_poisson(1.0, args.serve, sizes, seed=args.seed)
+    t0 = time.time()
+    for r in base:
+        pool.get(r.spec)
+    log(f"serving: {len(pool.meshes)} meshes built in {time.time() - t0:.1f}s")
+    # sequential FP64 service times (independent of the arrival rate)
+    seq = sequential(pool, base, CG_PREC_FP64)
+    service = {r.rid: r.done - r.start for r in seq}
+    mean_service = sum(service.values()) / len(service)
+    ke64 = {r.rid: r.ke for r in seq}
+    out = {"mode": "serving", "requests": len(base), "mesh_sides": sizes,
+           "spec": "cells log-uniform [2e4, 2e6] (nearest mesh side), horizon U{200..2000} steps, Re U[400,3200], "
+                   "Ma U[0.05,0.3], TGV or k^4; Poisson arrivals",
+           "sequential_fp64_mean_service_s": mean_service, "sweep": []}
+    for load in args.loads:
+        rate = load / mean_service
+        reqs = [type(r)(r.rid, r.arrival / rate, r.spec) for r in base]
+        # sequential queueing with the measured service times
+        clock, seq_res = 0.0, []
+        for r in sorted(reqs, key=lambda q: q.arrival):
+            start = max(clock, r.arrival)
+            clock = start + service[r.rid]
+            seq_res.append(type(seq[0])(r.rid, r.arrival, start, clock, 0, 0, 0.0, 0.0, 0.0))
+        res, info = Server(pool, CG_PREC_MIXED, hbm_budget=args.hbm_budget * 1e9).run(reqs)
+        errs = [abs(r.ke - ke64[r.rid]) / ke64[r.rid] for r in res]
+        row = {"load": load, "rate_per_s": rate, "served_mixed": summarize(res, info["makespan"]),
+               "sequential_fp64": summarize(seq_res), "ke_rel_err_vs_fp64_max": max(errs),
+               "batch_steps": info["batch_steps"]}
+        row["solves_per_s_ratio"] = row["served_mixed"]["solves_per_s"] / row["sequential_fp64"]["solves_per_s"]
+        out["sweep"].append(row)
+        log(f"load {load}: served {row['served_mixed']['solves_per_s']:.3f}/s p99 {row['served_mixed']['tts_p99_s']:.1f}s"
+            f" vs sequential {row['sequential_fp64']['solves_per_s']:.3f}/s p99 {row['sequential_fp64']['tts_p99_s']:.1f}s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +302,12 @@ def main():
     ap.add_argument("--accuracy", type=int, default=0, help="compare mixed with FP64 over this many steps")
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--copy-calibration", type=int, default=0, help="run only a device copy of this many bytes")
+    ap.add_argument("--ensemble", type=int, default=0, help="batched ensemble of this many solves")
+    ap.add_argument("--ens-steps", type=int, default=300)
+    ap.add_argument("--serve", type=int, default=0, help="serving loop over this many requests")
+    ap.add_argument("--loads", type=float, nargs="+", default=[0.5, 1.0, 2.0],
+                    help="arrival rates as multiples of the sequential FP64 service rate")
+    ap.add_argument("--hbm-budget", type=float, default=96.0, help="GB of device memory for admitted solves")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -216,6 +319,10 @@ def main():
         return 2
     if args.accuracy:
         return accuracy(args)
+    if args.ensemble:
+        return ensemble_bench(args)
+    if args.serve:
+        return serve_bench(args)
 
     mesh, text, state, desc = workload(args)
     ncv = mesh.ncv

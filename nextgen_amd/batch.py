@@ -18,7 +18,7 @@ BATCH_EXPORTS = [
     "cg_batch_create", "cg_batch_add", "cg_batch_remove", "cg_batch_step", "cg_batch_get_diag",
     "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
-    "cg_batch_attach",
+    "cg_batch_attach", "cg_batch_tune",
 ]
 
 _ready = False
@@ -47,6 +47,7 @@ def _lib():
             "cg_instance_bytes": (C.c_int, [vp, P(C.c_int64)]),
             "cg_instance_destroy": (None, [vp]),
             "cg_batch_attach": (C.c_int, [vp, vp, P(C.c_int)]),
+            "cg_batch_tune": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -155,6 +156,13 @@ class Batch:
 
     def set_option(self, name: str, value: int):
         self._chk(_lib().cg_batch_set_option(self._h, name.encode(), int(value)))
+
+    def tune(self, nsteps=4) -> str:
+        """Time each launch variant for nsteps steps (the instances advance) and
+        keep the fastest; returns the timing table."""
+        buf = C.create_string_buffer(1024)
+        self._chk(_lib().cg_batch_tune(self._h, int(nsteps), buf, len(buf)))
+        return buf.value.decode()
 
     def time_steps(self, nsteps) -> float:
         ms = C.c_double()

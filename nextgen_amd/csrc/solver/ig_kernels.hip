@@ -455,16 +455,33 @@ __device__ __forceinline__ void gather_compact(const GeomT<R>& g, const WorkT<R>
   const int nc = G(g.ef_nc)[c];
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
-  for (int k = 0; k < nc; ++k) {
-    const int code = ldnt(g.cf_pair + cbase + k * kSlice);
-    const auto fl = G(w.cflux + (size_t)(code & 0x7fffffff) * 6);
-    const R F[5] = {fl[0], fl[1], fl[2], fl[3], fl[4]};
-    if (code < 0) {
+  // pair codes and fluxes are fetched 4 slots at a time so the gathers
+  // overlap; the accumulation order stays k = 0, 1, ...
+  constexpr int kC = 4;
+  for (int k0 = 0; k0 < nc; k0 += kC) {
+    int code[kC];
+    R F[kC][5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] += F[q];
-    } else {
+    for (int j = 0; j < kC; ++j)
+      if (k0 + j < nc) code[j] = ldnt(g.cf_pair + cbase + (k0 + j) * kSlice);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
+    for (int j = 0; j < kC; ++j) {
+      if (k0 + j < nc) {
+        const auto fl = G(w.cflux + (size_t)(code[j] & 0x7fffffff) * 6);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) F[j][q] = fl[q];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kC; ++j) {
+      if (k0 + j >= nc) break;
+      if (code[j] < 0) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] += F[j][q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) acc[q] -= F[j][q];
+      }
     }
   }
 }
@@ -632,15 +649,35 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
   R d[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) d[k] = R(0);
-  for (int k = 0; k < deg; ++k) {
-    const int slot = base + k * kSlice;
-    const int nb = ldnt(g.gr_nbr + slot);
-    const R gc[3] = {ldnt(g.gc0 + slot), ldnt(g.gc1 + slot), ldnt(g.gc2 + slot)};
-    const R un[3] = {L[(size_t)nb * L_N + L_U0], L[(size_t)nb * L_N + L_U1], L[(size_t)nb * L_N + L_U2]};
+  // entries are fetched 4 at a time (codes and coefficients, then the four
+  // neighbour gathers) so their loads overlap; the sums keep the order k = 0, 1, ...
+  constexpr int kB = 4;
+  for (int k0 = 0; k0 < deg; k0 += kB) {
+    int nb[kB];
+    R gc[kB][3], un[kB][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j < deg) {
+        const int slot = base + (k0 + j) * kSlice;
+        nb[j] = ldnt(g.gr_nbr + slot);
+        gc[j][0] = ldnt(g.gc0 + slot); gc[j][1] = ldnt(g.gc1 + slot); gc[j][2] = ldnt(g.gc2 + slot);
+      }
+    }
 #pragma unroll
-      for (int j = 0; j < 3; ++j) d[i * 3 + j] += gc[j] * (un[i] - ui[i]);
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j < deg) {
+        const size_t o = (size_t)nb[j] * L_N;
+        un[j][0] = L[o + L_U0]; un[j][1] = L[o + L_U1]; un[j][2] = L[o + L_U2];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j >= deg) break;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) d[i * 3 + jj] += gc[j][jj] * (un[j][i] - ui[i]);
+    }
   }
   const auto a = G(w.aux + (size_t)c * A_N + A_D);
 #pragma unroll
@@ -856,18 +893,25 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
     const R sos_avg = R(0.5) * (a.sos + b.sos);
     cfl += mabs(undA) + area * sos_avg;
   }
-  // purely extended pairs: sensor only (a maximum: order-independent)
-  for (int k = 0; k < nx; ++k) {
-    const int slot = xbase + k * kSlice;
-    if constexpr (LDS) {
-      const int code = ldnt(g.xf_loc + slot);
-      const R* h = hrec + 4 * (code & 0x7fff);
-      gibbs(h[0], h[1], h[2], h[3], (code & 0x8000) != 0);
-    } else {
-      const int code = ldnt(g.xf_nbr + slot);
-      const int nb = code & 0x7fffffff;
-      const auto q = G(st.light + (size_t)nb * L_N);
-      gibbs(q[L_P], q[L_SPV], q[L_RHO], kMixed<R> ? R(0) : R(G(w.logp)[nb]), code < 0);
+  // purely extended pairs: sensor only (a maximum: order-independent); their
+  // codes are fetched kP at a time so the loads overlap
+  constexpr int kP = 8;
+  for (int k0 = 0; k0 < nx; k0 += kP) {
+    int code[kP];
+#pragma unroll
+    for (int j = 0; j < kP; ++j)
+      if (k0 + j < nx) code[j] = LDS ? int(ldnt(g.xf_loc + xbase + (k0 + j) * kSlice)) : ldnt(g.xf_nbr + xbase + (k0 + j) * kSlice);
+#pragma unroll
+    for (int j = 0; j < kP; ++j) {
+      if (k0 + j >= nx) break;
+      if constexpr (LDS) {
+        const R* h = hrec + 4 * (code[j] & 0x7fff);
+        gibbs(h[0], h[1], h[2], h[3], (code[j] & 0x8000) != 0);
+      } else {
+        const int nb = code[j] & 0x7fffffff;
+        const auto q = G(st.light + (size_t)nb * L_N);
+        gibbs(q[L_P], q[L_SPV], q[L_RHO], kMixed<R> ? R(0) : R(G(w.logp)[nb]), code[j] < 0);
+      }
     }
   }
   R fgr;

@@ -91,11 +91,11 @@ def test_batch_join_and_leave():
     b.close()
 
 
-@pytest.mark.parametrize("option", ["graph", "lds"])
-def test_batch_variants_bitwise(option):
+@pytest.mark.parametrize("option,values", [("graph", (1, 0)), ("lds", (1, 0)), ("tile", (4, 2, 8))])
+def test_batch_variants_bitwise(option, values):
     prec = CG_PREC_FP64
     runs = []
-    for value in (1, 0):
+    for value in values:
         b = Batch(prec)
         b.set_option(option, value)
         ids = [add_case(b, i, prec) for i in range(3)]
@@ -106,9 +106,24 @@ def test_batch_variants_bitwise(option):
             snaps.append([b.state(i) for i in ids])
         runs.append(snaps)
         b.close()
-    for s1, s0 in zip(*runs):
-        for x, y in zip(s1, s0):
-            assert_same(x, y)
+    for other in runs[1:]:
+        for s1, s0 in zip(runs[0], other):
+            for x, y in zip(s1, s0):
+                assert_same(x, y)
+
+
+def test_tuner_keeps_results_exact():
+    prec = CG_PREC_MIXED
+    b = Batch(prec)
+    ids = [add_case(b, i, prec) for i in range(3)]
+    report = b.tune(2)  # every variant advances the instances (graph capture + timed steps)
+    assert "chosen" in report
+    b.step(3)
+    for i, iid in enumerate(ids):
+        n = b.diag(iid).step
+        assert n > 3
+        assert_same(b.state(iid), alone(i, prec, n)[0])
+    b.close()
 
 
 def test_instance_prepared_on_worker_thread_then_attached():
