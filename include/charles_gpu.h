@@ -224,13 +224,24 @@ int cg_batch_remove(cg_batch* b, int id);
  * data, sets the state and applies initComplete on a stream of its own and may
  * run on another host thread than the batch's; cg_batch_attach (between steps,
  * on the batch's thread) hands the instance to the batch, which owns it from
- * then on. cg_instance_bytes reports the device memory it holds. */
+ * then on. bytes[0] = device memory of the solve itself (state, work,
+ * snapshots), bytes[1] = of its mesh layout (possibly shared, below). */
 typedef struct cg_instance cg_instance;
 int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
                        const double* rhoE, cg_instance** out);
-int cg_instance_bytes(const cg_instance* x, int64_t* bytes);
+int cg_instance_bytes(const cg_instance* x, int64_t bytes[2]);
 void cg_instance_destroy(cg_instance* x);
 int cg_batch_attach(cg_batch* b, cg_instance* x, int* id);
+/* A mesh's device layout (geometry, half-face lists, halos; read-only) built
+ * once and shared by every solve created on it with cg_instance_create_on;
+ * the memory is released when the layout handle and the last solve using it
+ * are gone. Thread-safe like cg_instance_create. */
+typedef struct cg_layout cg_layout;
+int cg_layout_create(const cg_mesh* m, int precision, cg_layout** out);
+int cg_layout_bytes(const cg_layout* L, int64_t* bytes);
+void cg_layout_destroy(cg_layout* L);
+int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* rho, const double* u,
+                          const double* rhoE, cg_instance** out);
 /* nsteps of setDt + advanceSolution for every instance, NaN checks at each
  * instance's CHECK_INTERVAL. */
 int cg_batch_step(cg_batch* b, int nsteps);

@@ -18,7 +18,8 @@ BATCH_EXPORTS = [
     "cg_batch_create", "cg_batch_add", "cg_batch_remove", "cg_batch_step", "cg_batch_get_diag",
     "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
-    "cg_batch_attach", "cg_batch_tune",
+    "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
+    "cg_instance_create_on",
 ]
 
 _ready = False
@@ -45,6 +46,10 @@ def _lib():
             "cg_batch_destroy": (None, [vp]),
             "cg_instance_create": (C.c_int, [vp, P(cg_params), dp, dp, dp, P(vp)]),
             "cg_instance_bytes": (C.c_int, [vp, P(C.c_int64)]),
+            "cg_layout_create": (C.c_int, [vp, C.c_int, P(vp)]),
+            "cg_layout_bytes": (C.c_int, [vp, P(C.c_int64)]),
+            "cg_layout_destroy": (None, [vp]),
+            "cg_instance_create_on": (C.c_int, [vp, P(cg_params), dp, dp, dp, P(vp)]),
             "cg_instance_destroy": (None, [vp]),
             "cg_batch_attach": (C.c_int, [vp, vp, P(C.c_int)]),
             "cg_batch_tune": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64]),
@@ -61,28 +66,65 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-class Instance:
-    """A solve prepared outside any batch (cg_instance_create): device layout,
-    state and initComplete. Safe to build on a worker thread while a batch
-    steps; hand it to a batch with Batch.attach."""
+class Layout:
+    """A mesh's device layout (cg_layout_create), shared by every solve created
+    on it with Instance(..., layout=...)."""
 
-    def __init__(self, mesh: Mesh, params: cg_params, rho, u, rhoE):
-        rho = np.ascontiguousarray(rho, dtype=np.float64)
-        u = np.ascontiguousarray(u, dtype=np.float64).reshape(mesh.ncv, 3)
-        rhoE = np.ascontiguousarray(rhoE, dtype=np.float64)
+    def __init__(self, mesh: Mesh, precision=CG_PREC_FP64):
         self.ncv = mesh.ncv
-        self.precision = params.precision
+        self.precision = precision
         h = C.c_void_p()
-        rc = _lib().cg_instance_create(mesh._h, C.byref(params), _dptr(rho), _dptr(u), _dptr(rhoE), C.byref(h))
+        rc = _lib().cg_layout_create(mesh._h, int(precision), C.byref(h))
         if rc:
             raise CharlesError(f"error {rc}: {load_library().cg_error_string().decode()}")
         self._h = h
 
     def bytes(self) -> int:
         b = C.c_int64()
-        if _lib().cg_instance_bytes(self._h, C.byref(b)):
+        if _lib().cg_layout_bytes(self._h, C.byref(b)):
             raise CharlesError(load_library().cg_error_string().decode())
         return b.value
+
+    def close(self):
+        if self._h:
+            _lib().cg_layout_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Instance:
+    """A solve prepared outside any batch (cg_instance_create, or
+    cg_instance_create_on a shared Layout): device buffers, state and
+    initComplete. Safe to build on a worker thread while a batch steps; hand
+    it to a batch with Batch.attach."""
+
+    def __init__(self, mesh: Mesh, params: cg_params, rho, u, rhoE, layout: Layout | None = None):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(mesh.ncv, 3)
+        rhoE = np.ascontiguousarray(rhoE, dtype=np.float64)
+        self.ncv = mesh.ncv
+        self.precision = params.precision
+        h = C.c_void_p()
+        if layout is None:
+            rc = _lib().cg_instance_create(mesh._h, C.byref(params), _dptr(rho), _dptr(u), _dptr(rhoE), C.byref(h))
+        else:
+            rc = _lib().cg_instance_create_on(layout._h, C.byref(params), _dptr(rho), _dptr(u), _dptr(rhoE),
+                                              C.byref(h))
+        if rc:
+            raise CharlesError(f"error {rc}: {load_library().cg_error_string().decode()}")
+        self._h = h
+
+    def bytes(self):
+        """(device bytes of the solve itself, of its mesh layout)"""
+        b = (C.c_int64 * 2)()
+        if _lib().cg_instance_bytes(self._h, b):
+            raise CharlesError(load_library().cg_error_string().decode())
+        return b[0], b[1]
 
     def close(self):
         if self._h:
@@ -181,4 +223,4 @@ class Batch:
             pass
 
 
-__all__ = ["Batch", "Instance", "BATCH_EXPORTS", "CG_PREC_FP64", "CG_PREC_MIXED"]
+__all__ = ["Batch", "Instance", "Layout", "BATCH_EXPORTS", "CG_PREC_FP64", "CG_PREC_MIXED"]

@@ -78,6 +78,12 @@ class IdealGasSolver {
     T.assign(ncv, params_.T_ref);
   }
 
+  // ---- library initial fields into rho, u, rhoE (for initialHook) ----
+  void setTgv(double Ma) { cg_check(cg_ic_tgv(mesh_, Ma, params_.gamma, rho.data(), u.data(), rhoE.data())); }
+  void setK4(double k0, double Mt, uint64_t seed, int kmax = 0) {
+    cg_check(cg_ic_k4(mesh_, k0, Mt, params_.gamma, seed, kmax, rho.data(), u.data(), rhoE.data()));
+  }
+
   // ---- user hooks (FlowSolver.hpp:2269-2272) ----
   virtual void initialHook() {}
   virtual void temporalHook() {}

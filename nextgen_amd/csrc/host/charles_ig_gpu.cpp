@@ -7,7 +7,8 @@
 //   MESH LATTICE <n> [L]
 //   MESH JITTERED <n> <jitter> <seed> [L]
 // and INIT TGV [Ma] sets the Taylor-Green vortex at the cell centroids, as an
-// initialHook would (charles_ig.cpp:209-222).
+// initialHook would (charles_ig.cpp:209-222); INIT K4 <k0> <Mt> <seed> the seeded
+// k^4-spectrum decaying-turbulence field (cg_ic_k4).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +29,9 @@ struct Deck {
   unsigned long long seed = 1;
   bool tgv = false;
   double tgv_ma = 0.1;
+  bool k4 = false;
+  double k0 = 4.0, mt = 0.3;
+  unsigned long long k4_seed = 1;
   int nsteps = -1;
 };
 
@@ -60,6 +64,12 @@ Deck read_deck(const char* path) {
         d.tgv = true;
         double ma;
         if (ls >> ma) d.tgv_ma = ma;
+      } else if (what == "K4") {
+        d.k4 = true;
+        ls >> d.k0 >> d.mt >> d.k4_seed;
+      } else {
+        std::fprintf(stderr, "ERROR: INIT %s: expected TGV [Ma] or K4 <k0> <Mt> <seed>\n", what.c_str());
+        throw(0);
       }
       continue;
     }
@@ -73,7 +83,14 @@ class MyIdealGasSolver : public charles::IdealGasSolver {
  public:
   bool tgv = false;
   double ma = 0.1;
+  bool k4 = false;
+  double k0 = 4.0, mt = 0.3;
+  unsigned long long k4_seed = 1;
   void initialHook() override {
+    if (k4) {
+      setK4(k0, mt, k4_seed);
+      return;
+    }
     if (!tgv) return;
     const double gamma = params().gamma;
     const double c0 = 1.0 / ma, p0 = c0 * c0 / gamma, R = params().p_ref / params().rho_ref / params().T_ref;
@@ -102,6 +119,10 @@ int main(int argc, char** argv) {
     solver.parseParams(d.text);
     solver.tgv = d.tgv;
     solver.ma = d.tgv_ma;
+    solver.k4 = d.k4;
+    solver.k0 = d.k0;
+    solver.mt = d.mt;
+    solver.k4_seed = d.k4_seed;
     solver.nsteps = d.nsteps;
     if (d.mesh_kind == "JITTERED") solver.initJittered(d.n, d.L, d.jitter, d.seed);
     else solver.initLattice(d.n, d.L);

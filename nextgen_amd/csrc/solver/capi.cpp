@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <sstream>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -73,6 +74,30 @@ struct Bufs {
   cgdev::WorkT<R> w{};
 };
 
+// owned device allocations, freed with the owner
+struct Alloc {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  Alloc() = default;
+  Alloc(const Alloc&) = delete;
+  Alloc& operator=(const Alloc&) = delete;
+  ~Alloc() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+// device layout of one mesh in the working precision R: read-only once built,
+// shared (reference-counted) by every solve on the mesh
+template <class R>
+struct Layout {
+  Alloc mem;
+  cgdev::GeomT<R> g{};
+  int ncv = 0, nfa = 0, nef = 0, nblocks = 0, npblocks = 0;
+  size_t lds = 0;             // LDS bytes of its extended-face halo (0: no halo)
+  int* solo_bmap = nullptr;   // cell blocks in XCD order
+  int* solo_pbmap = nullptr;  // compact-pair blocks in XCD order
+};
+
 // host bookkeeping of one solve
 struct HostInst {
   cg_params prm{};
@@ -89,18 +114,18 @@ struct HostInst {
   int64_t host_step = 0;
   double* rhs_tmp = nullptr;         // [5][ncv] scratch
   size_t lds = 0;                    // LDS bytes of its extended-face halo (0: no halo)
-  std::vector<void*> allocs;         // owned device memory
-  size_t bytes = 0;
+  Alloc mem;                         // its own device memory (state, work, snapshots)
   int id = -1;
 };
 
 template <class R>
 struct Instance : HostInst {
   Bufs<R> b;
+  std::shared_ptr<Layout<R>> lay;    // its mesh's device layout
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
-  int* solo_bmap = nullptr;          // its cell blocks in XCD order
-  int* solo_pbmap = nullptr;         // its compact-pair blocks in XCD order
-  cgk::Launch<R> solo_launch() const { return {solo, solo_bmap, nblocks, 1, lds, solo_pbmap, npblocks, 4}; }
+  cgk::Launch<R> solo_launch() const {
+    return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4};
+  }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
     r.g = b.g;
@@ -144,6 +169,13 @@ struct cg_solver : Err {
   Instance<float>* ix = nullptr;
 };
 
+// a mesh's device layout, for solves that share it
+struct cg_layout : Err {
+  bool mixed = false;
+  std::shared_ptr<Layout<double>> d;
+  std::shared_ptr<Layout<float>> f;
+};
+
 // a solve prepared outside any batch (its own stream while it is built)
 struct cg_instance : Err {
   bool mixed = false;
@@ -170,18 +202,18 @@ namespace {
   } while (0)
 
 template <class T>
-int dev_alloc(Err* e, HostInst* h, T** p, size_t n) {
+int dev_alloc(Err* e, Alloc* h, T** p, size_t n) {
   void* q = nullptr;
   if (n == 0) n = 1;
   HIPCHK(e, hipMalloc(&q, n * sizeof(T)));
-  h->allocs.push_back(q);
+  h->ptrs.push_back(q);
   h->bytes += n * sizeof(T);
   *p = (T*)q;
   return 0;
 }
 
 template <class T>
-int upload(Err* e, HostInst* h, const T** dst, const std::vector<T>& v) {
+int upload(Err* e, Alloc* h, const T** dst, const std::vector<T>& v) {
   T* d = nullptr;
   if (int rc = dev_alloc(e, h, &d, v.size())) return rc;
   if (!v.empty()) HIPCHK(e, hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -191,7 +223,7 @@ int upload(Err* e, HostInst* h, const T** dst, const std::vector<T>& v) {
 
 // upload FP64 host data in the working precision R
 template <class R>
-int upload_as(Err* e, HostInst* h, const R** dst, const std::vector<double>& v) {
+int upload_as(Err* e, Alloc* h, const R** dst, const std::vector<double>& v) {
   if constexpr (std::is_same<R, double>::value) {
     return upload(e, h, dst, v);
   } else {
@@ -200,10 +232,9 @@ int upload_as(Err* e, HostInst* h, const R** dst, const std::vector<double>& v) 
   }
 }
 
+// frees its own memory; the layout goes with the last solve that uses it
 template <class R>
 void free_instance(Instance<R>* I) {
-  if (!I) return;
-  for (void* p : I->allocs) (void)hipFree(p);
   delete I;
 }
 
@@ -334,25 +365,21 @@ int upload_solo(Err* s, Instance<R>* I) {
   return 0;
 }
 
-// Device layout of a mesh in the working precision R: geometry, SELL-64
-// half-face regions, LDS halos, gradient stencil, state and work buffers.
+// Device layout of a mesh in the working precision R (read-only once built,
+// shared by every solve on the mesh): per-cell geometry, compact pairs, the
+// SELL-64 half-face regions, LDS halos, gradient stencil, block maps.
 template <class R>
-int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, bool allow_lds,
-                 hipStream_t stream) {
-  constexpr bool mixed = std::is_same<R, float>::value;
-  Bufs<R>& b = I->b;
-  HostInst* h = I;
+int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
+  Alloc* h = &Lay->mem;
   const cgmesh::Operators& o = m->op;
   const int ncv = o.ncv, nfa = o.nfa;
-  h->prm = *p;
-  fill_dev_params(h);
-  h->ncv = ncv;
-  h->nfa = nfa;
-  h->nef = o.nef;
-  h->nblocks = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
-  h->npblocks = (nfa + cgdev::kBlock - 1) / cgdev::kBlock;
-  if (h->npblocks >= (1 << cgdev::kBlkBits)) return s->set(CG_EINVAL, "mesh too large for one instance");
-  cgdev::GeomT<R>& g = b.g;
+  Lay->ncv = ncv;
+  Lay->nfa = nfa;
+  Lay->nef = o.nef;
+  Lay->nblocks = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
+  Lay->npblocks = (nfa + cgdev::kBlock - 1) / cgdev::kBlock;
+  if (Lay->npblocks >= (1 << cgdev::kBlkBits)) return s->set(CG_EINVAL, "mesh too large for one instance");
+  cgdev::GeomT<R>& g = Lay->g;
   g.ncv = ncv;
   g.nfa = nfa;
   g.nef = o.nef;
@@ -390,8 +417,11 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
     const std::vector<double>& cU = o.cU();
     const std::vector<double>& nf = m->vor.n_fa;
     std::vector<int> cnbr(cs.nslots, 0), cpair(cs.nslots, 0), xnbr(xs.nslots, 0);
-    std::vector<double> x6[6];
-    for (auto& v : x6) v.assign(xs.nslots, 0.0);
+    // extended pair geometry (AoS, 6 values per slot) at the lo cell's slot;
+    // the hi cell's slot records where that copy is
+    std::vector<double> xgeo((size_t)xs.nslots * 6, 0.0);
+    std::vector<int> xsrc(xs.nslots, 0);
+    std::vector<int> lo_slot((size_t)std::max<int64_t>(o.nef - nfa, 0), -1);
     auto code_of = [&](int c, int pidx) {
       const bool hi = (o.hi[pidx] == c);
       return int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
@@ -407,12 +437,24 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
         const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
         const int64_t sidx = xs.slot(c, k);
         xnbr[sidx] = code_of(c, pidx);
-        for (int d = 0; d < 3; ++d) {
-          x6[d][sidx] = nU[3 * (size_t)pidx + d];
-          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+        if (o.lo[pidx] == c) {
+          lo_slot[pidx - nfa] = int(sidx);
+          for (int d = 0; d < 3; ++d) {
+            xgeo[6 * (size_t)sidx + d] = nU[3 * (size_t)pidx + d];
+            xgeo[6 * (size_t)sidx + 3 + d] = cU[3 * (size_t)pidx + d];
+          }
         }
       }
     }
+    if (xs.nslots >= (int64_t(1) << 31)) return s->set(CG_EMESH, "too many extended half-faces for one instance");
+    for (int c = 0; c < ncv; ++c)
+      for (int k = 0; k < xcnt[c]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
+        if (o.hi[pidx] != c) continue;
+        const int src = lo_slot[pidx - nfa];
+        if (src < 0) return s->set(CG_EMESH, "extended pair without a lo-side half-face");
+        xsrc[xs.slot(c, k)] = src;
+      }
     // compact pairs (pair index = face index): per-pair SoA for the thread-per-pair kernel
     {
       std::vector<int> plo(nfa), phi(nfa);
@@ -491,7 +533,7 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
         rc |= upload(s, h, &g.halo_v, hv);
         rc |= upload(s, h, &g.xf_loc, loc);
         g.halo_max = hmax;
-        h->lds = (size_t)hmax * 6 * sizeof(R);
+        Lay->lds = (size_t)hmax * 6 * sizeof(R);
       }
     }
     rc |= upload(s, h, &g.cf_ptr, cs.ptr);
@@ -501,8 +543,8 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
     rc |= upload(s, h, &g.xf_ptr, xs.ptr);
     rc |= upload(s, h, &g.xf_deg, xcnt);
     rc |= upload(s, h, &g.xf_nbr, xnbr);
-    const R** xdst[6] = {&g.xsn0, &g.xsn1, &g.xsn2, &g.xsc0, &g.xsc1, &g.xsc2};
-    for (int q = 0; q < 6; ++q) rc |= upload_as<R>(s, h, xdst[q], x6[q]);
+    rc |= upload_as<R>(s, h, &g.xgeo, xgeo);
+    rc |= upload(s, h, &g.xsrc, xsrc);
     if (rc) return rc;
   }
   // gradient stencil without the diagonal
@@ -530,6 +572,38 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
     rc |= upload_as<R>(s, h, &g.gc2, c2);
     if (rc) return rc;
   }
+  // XCD-ordered block maps for launches over one instance
+  rc |= dev_alloc(s, h, &Lay->solo_bmap, (size_t)Lay->nblocks);
+  rc |= dev_alloc(s, h, &Lay->solo_pbmap, (size_t)std::max(Lay->npblocks, 1));
+  if (rc) return rc;
+  const std::vector<int> order = xcd_order(Lay->nblocks);
+  HIPCHK(s, hipMemcpy(Lay->solo_bmap, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  const std::vector<int> porder = xcd_order(Lay->npblocks);
+  if (!porder.empty())
+    HIPCHK(s, hipMemcpy(Lay->solo_pbmap, porder.data(), sizeof(int) * porder.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// State, work buffers and snapshots of one solve on a layout, in its
+// reference initial state (IdealGasSolver::initData, IdealGasSolver.cpp:58-62).
+template <class R>
+int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const cg_params* p, hipStream_t stream) {
+  constexpr bool mixed = std::is_same<R, float>::value;
+  Bufs<R>& b = I->b;
+  HostInst* hi = I;
+  Alloc* h = &I->mem;
+  I->lay = lay;
+  b.g = lay->g;
+  hi->prm = *p;
+  fill_dev_params(hi);
+  const int ncv = lay->ncv, nfa = lay->nfa;
+  hi->ncv = ncv;
+  hi->nfa = nfa;
+  hi->nef = lay->nef;
+  hi->nblocks = lay->nblocks;
+  hi->npblocks = lay->npblocks;
+  hi->lds = lay->lds;
+  int rc = 0;
   // state buffers
   for (int k = 0; k < 2; ++k) {
     rc |= dev_alloc(s, h, &b.st[k].light, (size_t)ncv * cgdev::L_N);
@@ -548,27 +622,20 @@ int build_device(Err* s, Instance<R>* I, const cg_mesh* m, const cg_params* p, b
   rc |= dev_alloc(s, h, &w.rhs0, 5 * (size_t)ncv);
   rc |= dev_alloc(s, h, &w.rhs1, 5 * (size_t)ncv);
   rc |= dev_alloc(s, h, &w.cflux, 6 * (size_t)std::max(nfa, 1));
-  rc |= dev_alloc(s, h, &h->info, 1);
-  w.info = h->info;
-  rc |= dev_alloc(s, h, &w.partial, 8 * (size_t)h->nblocks);
+  rc |= dev_alloc(s, h, &hi->info, 1);
+  w.info = hi->info;
+  rc |= dev_alloc(s, h, &w.partial, 8 * (size_t)hi->nblocks);
   rc |= dev_alloc(s, h, &w.diag, 8);
-  h->snap_n = mixed ? (size_t)ncv * 5 : (size_t)ncv * (cgdev::L_N + 1);
-  rc |= dev_alloc(s, h, &h->store_buf, h->snap_n);
-  rc |= dev_alloc(s, h, &h->store_recent, h->snap_n);
-  rc |= dev_alloc(s, h, &h->rhs_tmp, 5 * (size_t)ncv);
-  // one-entry table and XCD-ordered block map for launches over this instance alone
+  hi->snap_n = mixed ? (size_t)ncv * 5 : (size_t)ncv * (cgdev::L_N + 1);
+  rc |= dev_alloc(s, h, &hi->store_buf, hi->snap_n);
+  rc |= dev_alloc(s, h, &hi->store_recent, hi->snap_n);
+  rc |= dev_alloc(s, h, &hi->rhs_tmp, 5 * (size_t)ncv);
+  // one-entry table for launches over this instance alone
   rc |= dev_alloc(s, h, &I->solo, 1);
-  rc |= dev_alloc(s, h, &I->solo_bmap, (size_t)h->nblocks);
-  rc |= dev_alloc(s, h, &I->solo_pbmap, (size_t)std::max(h->npblocks, 1));
   if (rc) return rc;
   {
     const cgdev::StepInfo zero{};
-    HIPCHK(s, hipMemcpy(h->info, &zero, sizeof(zero), hipMemcpyHostToDevice));
-    const std::vector<int> order = xcd_order(h->nblocks);
-    HIPCHK(s, hipMemcpy(I->solo_bmap, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
-    const std::vector<int> porder = xcd_order(h->npblocks);
-    if (!porder.empty())
-      HIPCHK(s, hipMemcpy(I->solo_pbmap, porder.data(), sizeof(int) * porder.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(hi->info, &zero, sizeof(zero), hipMemcpyHostToDevice));
   }
   // IdealGasSolver::initData reference state (IdealGasSolver.cpp:58-62)
   for (int k = 0; k < 2; ++k) {
@@ -928,18 +995,35 @@ int get_diag(Err* s, Instance<R>* I, int cur, cgdev::StepInfo* host, cg_diag* d,
   return 0;
 }
 
-// instance creation: device layout, reference initial state, then optional host state
 template <class R>
-int make_instance(Err* s, const cg_mesh* m, const cg_params* p, bool allow_lds, hipStream_t stream,
-                  Instance<R>** out) {
+int make_layout(Err* s, const cg_mesh* m, bool allow_lds, std::shared_ptr<Layout<R>>* out) {
+  auto lay = std::make_shared<Layout<R>>();
+  if (int rc = build_layout(s, lay.get(), m, allow_lds)) return rc;
+  *out = std::move(lay);
+  return 0;
+}
+
+// a solve on a layout, in its reference initial state
+template <class R>
+int make_instance_on(Err* s, std::shared_ptr<Layout<R>> lay, const cg_params* p, hipStream_t stream,
+                     Instance<R>** out) {
   Instance<R>* I = new Instance<R>();
-  const int rc = build_device(s, I, m, p, allow_lds, stream);
+  const int rc = build_instance(s, I, std::move(lay), p, stream);
   if (rc) {
     free_instance(I);
     return rc;
   }
   *out = I;
   return 0;
+}
+
+// a solve with a layout of its own
+template <class R>
+int make_instance(Err* s, const cg_mesh* m, const cg_params* p, bool allow_lds, hipStream_t stream,
+                  Instance<R>** out) {
+  std::shared_ptr<Layout<R>> lay;
+  if (int rc = make_layout(s, m, allow_lds, &lay)) return rc;
+  return make_instance_on(s, std::move(lay), p, stream, out);
 }
 
 }  // namespace
@@ -1529,33 +1613,72 @@ int cg_batch_create(int precision, cg_batch** out) {
   return CG_OK;
 }
 
-int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
-                       const double* rhoE, cg_instance** out) {
-  if (!m || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
-  if (int rc = check_params(p)) return rc;
+int cg_layout_create(const cg_mesh* m, int precision, cg_layout** out) {
+  if (!m || !out) return fail(CG_EINVAL, "null argument");
+  if (precision != CG_PREC_FP64 && precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   *out = nullptr;
-  cg_instance* x = new cg_instance();
-  x->mixed = (p->precision == CG_PREC_MIXED);
+  cg_layout* L = new cg_layout();
+  L->mixed = (precision == CG_PREC_MIXED);
+  const bool lds = !std::getenv("CG_NO_LDS");
+  const int rc = L->mixed ? make_layout(L, m, lds, &L->f) : make_layout(L, m, lds, &L->d);
+  if (rc) {
+    const std::string e = L->err;
+    delete L;
+    return fail(rc, e);
+  }
+  *out = L;
+  return CG_OK;
+}
+
+int cg_layout_bytes(const cg_layout* L, int64_t* bytes) {
+  if (!L || !bytes) return fail(CG_EINVAL, "null argument");
+  *bytes = (int64_t)(L->mixed ? L->f->mem.bytes : L->d->mem.bytes);
+  return CG_OK;
+}
+
+void cg_layout_destroy(cg_layout* L) { delete L; }
+
+}  // extern "C"
+
+namespace {
+
+// cg_instance_create / cg_instance_create_on: the solve's buffers on `lay`
+// (built from `m` when lay is null), its state, initComplete, first snapshot
+template <class R>
+int create_instance(cg_instance* x, std::shared_ptr<Layout<R>> lay, const cg_mesh* m, const cg_params* p,
+                    const double* rho, const double* u, const double* rhoE, Instance<R>** out) {
   hipStream_t stream = nullptr;
   cgdev::StepInfo* host = nullptr;
   int rc = 0;
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&host, sizeof(cgdev::StepInfo)) != hipSuccess) {
+      hipHostMalloc((void**)&host, sizeof(cgdev::StepInfo)) != hipSuccess)
     rc = x->set(CG_EHIP, "no HIP device, stream or pinned memory");
-  }
-  const bool lds = !std::getenv("CG_NO_LDS");
-  auto make = [&](auto** I) -> int {
-    if (int r = make_instance(x, m, p, lds, stream, I)) return r;
-    // the state goes to st[0]; attaching to a batch at parity 1 swaps the buffers
-    if (int r = set_state(x, *I, 0, rho, u, rhoE, stream)) return r;
-    if (int r = refresh(x, *I, 0, stream)) return r;
-    if (int r = do_store(x, *I, 0, host, stream)) return r;
-    HIPCHK(x, hipStreamSynchronize(stream));
-    return 0;
-  };
-  if (!rc) rc = x->mixed ? make(&x->ix) : make(&x->id);
+  if (!rc && !lay) rc = make_layout(x, m, !std::getenv("CG_NO_LDS"), &lay);
+  if (!rc) rc = make_instance_on(x, lay, p, stream, out);
+  // the state goes to st[0]; attaching to a batch at parity 1 swaps the buffers
+  if (!rc) rc = set_state(x, *out, 0, rho, u, rhoE, stream);
+  if (!rc) rc = refresh(x, *out, 0, stream);
+  if (!rc) rc = do_store(x, *out, 0, host, stream);
+  if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = x->set(CG_EHIP, "stream synchronize failed");
   if (host) (void)hipHostFree(host);
   if (stream) (void)hipStreamDestroy(stream);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* rho, const double* u,
+                          const double* rhoE, cg_instance** out) {
+  if (!L || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
+  if (int rc = check_params(p)) return rc;
+  if ((p->precision == CG_PREC_MIXED) != L->mixed) return fail(CG_EINVAL, "parameters' precision differs from the layout's");
+  *out = nullptr;
+  cg_instance* x = new cg_instance();
+  x->mixed = L->mixed;
+  const int rc = x->mixed ? create_instance(x, L->f, nullptr, p, rho, u, rhoE, &x->ix)
+                          : create_instance(x, L->d, nullptr, p, rho, u, rhoE, &x->id);
   if (rc) {
     const std::string e = x->err;
     cg_instance_destroy(x);
@@ -1565,9 +1688,28 @@ int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, 
   return CG_OK;
 }
 
-int cg_instance_bytes(const cg_instance* x, int64_t* bytes) {
+int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
+                       const double* rhoE, cg_instance** out) {
+  if (!m || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
+  if (int rc = check_params(p)) return rc;
+  *out = nullptr;
+  cg_instance* x = new cg_instance();
+  x->mixed = (p->precision == CG_PREC_MIXED);
+  const int rc = x->mixed ? create_instance(x, std::shared_ptr<Layout<float>>(), m, p, rho, u, rhoE, &x->ix)
+                          : create_instance(x, std::shared_ptr<Layout<double>>(), m, p, rho, u, rhoE, &x->id);
+  if (rc) {
+    const std::string e = x->err;
+    cg_instance_destroy(x);
+    return fail(rc, e);
+  }
+  *out = x;
+  return CG_OK;
+}
+
+int cg_instance_bytes(const cg_instance* x, int64_t bytes[2]) {
   if (!x || !bytes) return fail(CG_EINVAL, "null argument");
-  *bytes = (int64_t)(x->mixed ? x->ix->bytes : x->id->bytes);
+  bytes[0] = (int64_t)(x->mixed ? x->ix->mem.bytes : x->id->mem.bytes);
+  bytes[1] = (int64_t)(x->mixed ? x->ix->lay->mem.bytes : x->id->lay->mem.bytes);
   return CG_OK;
 }
 
@@ -1679,10 +1821,15 @@ int cg_batch_counts(cg_batch* b, int64_t counts[4]) {
   if (!b || !counts) return fail(CG_EINVAL, "null argument");
   return with_engine(b, [&](auto& E) -> int {
     int64_t cells = 0, bytes = 0, lds = 0;
+    std::vector<const void*> layouts;
     for (auto* I : E.inst) {
       cells += I->ncv;
-      bytes += (int64_t)I->bytes;
+      bytes += (int64_t)I->mem.bytes;
       lds += I->lds > 0;
+      if (std::find(layouts.begin(), layouts.end(), (const void*)I->lay.get()) == layouts.end()) {
+        layouts.push_back(I->lay.get());
+        bytes += (int64_t)I->lay->mem.bytes;
+      }
     }
     counts[0] = (int64_t)E.inst.size();
     counts[1] = cells;

@@ -52,8 +52,7 @@ enum { A_MU = 0, A_LOC, A_ASGS, A_FGR, A_SYMH, A_SYMP, A_PAD, A_D = 7, A_N = 16 
 // ones (cf_*, ef_nc[cell] slots: the pair index, to gather cflux) and the
 // purely extended ones (xf_*, xf_deg[cell] slots). Slot k of a cell is at
 // ptr[slice] + k*64 + lane, so at every k the 64 lanes of a wavefront read one
-// contiguous row. The extended pair geometry is stored per slot (both cells
-// of a pair hold a copy), which keeps every geometry read coalesced.
+// contiguous row.
 template <class R>
 struct GeomT {
   int ncv, nfa, nef;
@@ -67,10 +66,13 @@ struct GeomT {
   const int *cf_ptr, *ef_nc;
   const int* cf_nbr;                              // neighbour | (cell is the pair's hi side) << 31
   const int* cf_pair;                             // pair index | (cell is the pair's hi side) << 31
-  // purely extended half-faces
+  // purely extended half-faces. The pair geometry n, c is stored once, at the
+  // slot of the pair's lo cell (xgeo[slot][6]); the hi cell's slot holds the
+  // index of that slot (xsrc) and gathers it.
   const int *xf_ptr, *xf_deg;
   const int* xf_nbr;                              // neighbour | hi-side flag << 31
-  const R *xsn0, *xsn1, *xsn2, *xsc0, *xsc1, *xsc2;
+  const R* xgeo;                                  // [slots][6]: n0 n1 n2 c0 c1 c2 (lo-side slots)
+  const int* xsrc;                                // hi-side slot -> slot of the lo-side copy
   // per-block halo for LDS staging (null when staging is off): the sorted
   // cells block b's extended half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
   // xf_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15

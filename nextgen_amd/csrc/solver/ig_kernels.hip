@@ -447,6 +447,16 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   o[5] = R(0);
 }
 
+// Geometry n, c of an extended half-face: from the slot itself on the pair's
+// lo side, through xsrc from the lo cell's slot on the hi side.
+template <class R>
+__device__ __forceinline__ void ext_geom(const GeomT<R>& g, int slot, bool self_hi, R n[3], R cv[3]) {
+  const int gi = self_hi ? ldnt(g.xsrc + slot) : slot;
+  const auto q = G(g.xgeo + (size_t)gi * 6);
+  n[0] = q[0]; n[1] = q[1]; n[2] = q[2];
+  cv[0] = q[3]; cv[1] = q[4]; cv[2] = q[5];
+}
+
 // A cell's compact-pair fluxes in pair order (its first ef_nc pairs): the
 // start of its stage RHS accumulation.
 template <class R>
@@ -518,8 +528,8 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
     const Light<R> other = load_light(in.light, nb);
     const Light<R>& a = self_hi ? other : self;
     const Light<R>& b = self_hi ? self : other;
-    const R n[3] = {ldnt(g.xsn0 + slot), ldnt(g.xsn1 + slot), ldnt(g.xsn2 + slot)};
-    const R cv[3] = {ldnt(g.xsc0 + slot), ldnt(g.xsc1 + slot), ldnt(g.xsc2 + slot)};
+    R n[3], cv[3];
+    ext_geom(g, slot, self_hi, n, cv);
     R F[5], u_avg[3], inv_v_avg, h_stag;
     flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
     if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
@@ -591,8 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
       if (k0 + j < deg) {
         const int slot = base + (k0 + j) * kSlice;
         code[j] = ldnt(g.xf_loc + slot);
-        n[j][0] = ldnt(g.xsn0 + slot); n[j][1] = ldnt(g.xsn1 + slot); n[j][2] = ldnt(g.xsn2 + slot);
-        cv[j][0] = ldnt(g.xsc0 + slot); cv[j][1] = ldnt(g.xsc1 + slot); cv[j][2] = ldnt(g.xsc2 + slot);
+        ext_geom(g, slot, (code[j] & 0x8000) != 0, n[j], cv[j]);
       }
     }
 #pragma unroll

@@ -71,7 +71,8 @@ EXPORTS = [
     "cg_ic_tgv", "cg_ic_k4", "cg_batch_create", "cg_batch_add", "cg_batch_remove", "cg_batch_step",
     "cg_batch_get_diag", "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
-    "cg_batch_attach", "cg_batch_tune",
+    "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
+    "cg_instance_create_on",
 ]
 
 _lib = None

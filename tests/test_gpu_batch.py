@@ -174,3 +174,33 @@ def test_served_results_equal_solo_runs():
         d = g.report()
         g.close()
         assert d.step == sp.steps and r.ke == d.ke and r.time == d.time and r.enstrophy == d.enstrophy
+
+
+def test_solves_sharing_a_layout_match_solo_runs():
+    from nextgen_amd.batch import Instance, Layout
+
+    prec = CG_PREC_MIXED
+    m, p, state = case(1, prec)
+    lay = Layout(m, prec)
+    b = Batch(prec)
+    a = b.attach(Instance(m, p, *state, layout=lay))
+    # the same mesh with another deck and field
+    n, seed, ic, Re, Ma = CASES[1]
+    p2 = parse_params(tgv_params(Re=2000.0, Ma=0.15, sgs="VREMAN") + "CHECK_INTERVAL 3\nPRECISION MIXED\n")
+    s2 = m.ic_tgv(0.15, 1.4)
+    inst2 = Instance(m, p2, *s2, layout=lay)
+    own, shared = inst2.bytes()
+    assert shared == lay.bytes() and own < shared
+    c = b.attach(inst2)
+    lay.close()  # the solves keep the layout alive
+    b.step(6)
+    assert_same(b.state(a), alone(1, prec, 6)[0])
+    g = IdealGasSolver(m, p2)
+    g.set_state(*s2)
+    g.init_complete()
+    g.advance_solution(6)
+    assert_same(b.state(c), g.get_state())
+    g.close()
+    cnt = b.counts()
+    assert cnt["device_bytes"] == 2 * own + shared  # the layout is counted once
+    b.close()

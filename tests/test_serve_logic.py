@@ -17,7 +17,7 @@ class FakeInstance:
         self.req = req
 
     def bytes(self):
-        return int(BPC * self.req.spec.n ** 3)
+        return int(BPC * self.req.spec.n ** 3), 0
 
 
 class FakeBatch:
