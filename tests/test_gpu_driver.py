@@ -41,3 +41,19 @@ def test_driver_reports_bad_deck(tmp_path):
     out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1
     assert "SGS_MODEL" in out.stderr
+
+
+def test_driver_k4_deck_matches_oracle(tmp_path):
+    k4 = ("GAMMA 1.4\nP_REF 0.7142857142857143\nRHO_REF 1.0\nT_REF 1.0\nMU_REF 0.0003\n"
+          "MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL VREMAN\nTIMESTEP CFL 0.8\n")
+    deck = tmp_path / "k4.in"
+    deck.write_text(k4 + "MESH JITTERED 10 0.3 4\nINIT K4 2.0 0.2 9\nNSTEPS 6\nCHECK_INTERVAL 3\n")
+    out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    kes = [float(s) for s in re.findall(r"ke: ([0-9.eE+-]+)", out.stdout)]
+    m = Mesh.jittered(10, 0.3, seed=4)
+    o = OracleSolver(m, parse_params(k4))
+    o.set_state(*m.ic_k4(2.0, 0.2, 1.4, seed=9))
+    o.init_complete()
+    o.step(6)
+    assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]
