@@ -41,6 +41,12 @@ namespace {
 
 thread_local std::string g_err;
 
+// Work that allocates, frees or synchronises the device, and graph capture, are
+// serialised across threads: a device-wide wait issued by one thread while
+// another thread's stream is being captured would invalidate that capture.
+std::recursive_mutex g_dev_mu;
+using DevLock = std::lock_guard<std::recursive_mutex>;
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -55,6 +61,12 @@ struct Err {
     return code;
   }
 };
+
+// a launcher's nonzero return is the hipError_t the launch left
+template <class S>
+int launch_fail(S* s, const char* what, int code) {
+  return s->set(CG_EHIP, std::string(what) + " launch failed: " + hipGetErrorString((hipError_t)code));
+}
 
 }  // namespace
 
@@ -96,6 +108,9 @@ struct Layout {
   size_t lds = 0;             // LDS bytes of its extended-face halo (0: no halo)
   int* solo_bmap = nullptr;   // cell blocks in XCD order
   int* solo_pbmap = nullptr;  // compact-pair blocks in XCD order
+  // device cell i holds mesh cell ext_of[i]: every 256-cell block keeps its
+  // Morton cells, sorted by their half-face counts
+  std::vector<int> ext_of;
 };
 
 // host bookkeeping of one solve
@@ -114,6 +129,7 @@ struct HostInst {
   int64_t host_step = 0;
   double* rhs_tmp = nullptr;         // [5][ncv] scratch
   size_t lds = 0;                    // LDS bytes of its extended-face halo (0: no halo)
+  const int* ext_of = nullptr;       // device cell -> mesh cell (its layout's)
   Alloc mem;                         // its own device memory (state, work, snapshots)
   int id = -1;
 };
@@ -365,6 +381,28 @@ int upload_solo(Err* s, Instance<R>* I) {
   return 0;
 }
 
+// cells sorted within groups of this many (0: mesh order)
+constexpr int kDefaultSortGroup = 0;
+
+// Lane order of a SELL-64 region: within each 64-cell slice the cells by
+// decreasing key (stable); perm[pos] = offset in its slice of the cell at lane
+// position pos, cell[pos] = that cell. The identity when `sorted` is false.
+void lane_order(int ncv, const std::vector<int64_t>& key, bool sorted, std::vector<uint8_t>& perm,
+                std::vector<int>& cell) {
+  perm.resize(ncv);
+  cell.resize(ncv);
+  for (int s0 = 0; s0 < ncv; s0 += cgdev::kSlice) {
+    const int n = std::min(cgdev::kSlice, ncv - s0);
+    int idx[cgdev::kSlice];
+    for (int j = 0; j < n; ++j) idx[j] = j;
+    if (sorted) std::stable_sort(idx, idx + n, [&](int a, int b) { return key[s0 + a] > key[s0 + b]; });
+    for (int j = 0; j < n; ++j) {
+      perm[s0 + j] = uint8_t(idx[j]);
+      cell[s0 + j] = s0 + idx[j];
+    }
+  }
+}
+
 // Device layout of a mesh in the working precision R (read-only once built,
 // shared by every solve on the mesh): per-cell geometry, compact pairs, the
 // SELL-64 half-face regions, LDS halos, gradient stencil, block maps.
@@ -383,32 +421,77 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   g.ncv = ncv;
   g.nfa = nfa;
   g.nef = o.nef;
-  std::vector<double> dx13(ncv), dx23(ncv);
+  // compact half-faces (pair index < nfa) precede the purely extended ones in
+  // every cell's list (mesh numbering)
+  std::vector<int> ccnt_m(ncv), xcnt_m(ncv);
   for (int c = 0; c < ncv; ++c) {
-    dx13[c] = std::pow(o.vol[c], 1.0 / 3.0);
-    dx23[c] = std::pow(o.vol[c], 2.0 / 3.0);
+    const int cnt = o.efocv_i[c + 1] - o.efocv_i[c];
+    int nc = 0;
+    while (nc < cnt && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
+    for (int k = nc; k < cnt; ++k)
+      if (o.efocv_v[o.efocv_i[c] + k] < nfa) return s->set(CG_EMESH, "compact pairs must precede extended ones");
+    ccnt_m[c] = nc;
+    xcnt_m[c] = cnt - nc;
+  }
+  // Device cell order: the mesh's (Morton) by default. CG_CELL_ORDER =
+  // sorted64 | sorted256 sorts each group of 64 (one SELL slice) or 256 (one
+  // workgroup) consecutive cells by purely extended then compact half-face
+  // count, descending: the padded tail rows of a slice then fetch fewer cache
+  // lines (64) or are shorter (256), and the extended pass gains 4-7%, but the
+  // compact-pair pass, whose consecutive threads share lo-cell records in the
+  // mesh order, loses more (measured on the 2.1M-cell mesh). A cell's slot
+  // order, its pairs' orientation and every summation order are those of the
+  // mesh numbering whatever the order.
+  std::vector<int>& ext_of = Lay->ext_of;
+  ext_of.resize(ncv);
+  for (int c = 0; c < ncv; ++c) ext_of[c] = c;
+  int sort_group = kDefaultSortGroup;
+  if (const char* order_env = std::getenv("CG_CELL_ORDER")) {
+    const std::string ov(order_env);
+    sort_group = ov == "sorted256" ? cgdev::kBlock : ov == "sorted64" ? cgdev::kSlice : 0;
+  }
+  if (sort_group > 0)
+    for (int b0 = 0; b0 < ncv; b0 += sort_group)
+      std::stable_sort(ext_of.begin() + b0, ext_of.begin() + std::min(ncv, b0 + sort_group), [&](int a, int b) {
+        return xcnt_m[a] != xcnt_m[b] ? xcnt_m[a] > xcnt_m[b] : ccnt_m[a] > ccnt_m[b];
+      });
+  std::vector<int> int_of(ncv);
+  for (int c = 0; c < ncv; ++c) int_of[ext_of[c]] = c;
+  std::vector<double> inv_vol(ncv), vol(ncv), sa_to_vol(ncv), dcmag(ncv), dx13(ncv), dx23(ncv);
+  for (int c = 0; c < ncv; ++c) {
+    const int e = ext_of[c];
+    inv_vol[c] = o.inv_vol[e];
+    vol[c] = o.vol[e];
+    sa_to_vol[c] = o.sa_to_vol[e];
+    dcmag[c] = o.dcmag_hat[e];
+    dx13[c] = std::pow(o.vol[e], 1.0 / 3.0);
+    dx23[c] = std::pow(o.vol[e], 2.0 / 3.0);
   }
   int rc = 0;
-  rc |= upload(s, h, &g.inv_vol, o.inv_vol);
-  rc |= upload(s, h, &g.vol, o.vol);
-  rc |= upload(s, h, &g.sa_to_vol, o.sa_to_vol);
-  rc |= upload(s, h, &g.dcmag, o.dcmag_hat);
+  rc |= upload(s, h, &g.inv_vol, inv_vol);
+  rc |= upload(s, h, &g.vol, vol);
+  rc |= upload(s, h, &g.sa_to_vol, sa_to_vol);
+  rc |= upload(s, h, &g.dcmag, dcmag);
   rc |= upload(s, h, &g.dx13, dx13);
   rc |= upload(s, h, &g.dx23, dx23);
   if (rc) return rc;
 
+  // SELL lane orders (CG_LANE_ORDER=plain: lane = cell, for comparison)
+  const char* lane_env = std::getenv("CG_LANE_ORDER");
+  const bool sort_lanes = !(lane_env && std::string(lane_env) == "plain");
   // half-faces (per-slot geometry): compact ones and purely extended ones in
-  // two SELL-64 regions, each starting at row 0
+  // two SELL-64 regions, each starting at row 0, indexed by lane position
   {
-    std::vector<int> ccnt(ncv), xcnt(ncv);
-    for (int c = 0; c < ncv; ++c) {
-      const int cnt = o.efocv_i[c + 1] - o.efocv_i[c];
-      int nc = 0;
-      while (nc < cnt && o.efocv_v[o.efocv_i[c] + nc] < nfa) ++nc;
-      for (int k = nc; k < cnt; ++k)
-        if (o.efocv_v[o.efocv_i[c] + k] < nfa) return s->set(CG_EMESH, "compact pairs must precede extended ones");
-      ccnt[c] = nc;
-      xcnt[c] = cnt - nc;
+    std::vector<int64_t> key(ncv);
+    for (int c = 0; c < ncv; ++c) key[c] = int64_t(xcnt_m[ext_of[c]]) * 4096 + ccnt_m[ext_of[c]];
+    std::vector<uint8_t> xperm;
+    std::vector<int> xcell;
+    lane_order(ncv, key, sort_lanes, xperm, xcell);
+    rc |= upload(s, h, &g.xperm, xperm);
+    std::vector<int> ccnt(ncv), xcnt(ncv);  // by lane position
+    for (int q = 0; q < ncv; ++q) {
+      ccnt[q] = ccnt_m[ext_of[xcell[q]]];
+      xcnt[q] = xcnt_m[ext_of[xcell[q]]];
     }
     Sell cs, xs;
     cs.build(ncv, ccnt);
@@ -417,66 +500,59 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     const std::vector<double>& cU = o.cU();
     const std::vector<double>& nf = m->vor.n_fa;
     std::vector<int> cnbr(cs.nslots, 0), cpair(cs.nslots, 0), xnbr(xs.nslots, 0);
-    // extended pair geometry (AoS, 6 values per slot) at the lo cell's slot;
-    // the hi cell's slot records where that copy is
-    std::vector<double> xgeo((size_t)xs.nslots * 6, 0.0);
-    std::vector<int> xsrc(xs.nslots, 0);
-    std::vector<int> lo_slot((size_t)std::max<int64_t>(o.nef - nfa, 0), -1);
-    auto code_of = [&](int c, int pidx) {
-      const bool hi = (o.hi[pidx] == c);
-      return int(uint32_t(hi ? o.lo[pidx] : o.hi[pidx]) | (hi ? 0x80000000u : 0u));
+    // raw face area vector per compact slot (the CFL sum streams it in rows
+    // instead of gathering it by pair index)
+    std::vector<double> cfn[3];
+    for (auto& v : cfn) v.assign(cs.nslots, 0.0);
+    // extended pair geometry per slot (SoA; both cells of a pair hold a copy)
+    std::vector<double> x6[6];
+    for (auto& v : x6) v.assign(xs.nslots, 0.0);
+    // neighbour (device numbering) | the cell is the pair's hi side << 31;
+    // e is the cell's mesh number
+    auto code_of = [&](int e, int pidx) {
+      const bool hi = (o.hi[pidx] == e);
+      return int(uint32_t(int_of[hi ? o.lo[pidx] : o.hi[pidx]]) | (hi ? 0x80000000u : 0u));
     };
-    for (int c = 0; c < ncv; ++c) {
-      for (int k = 0; k < ccnt[c]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[c] + k];
-        const int64_t sidx = cs.slot(c, k);
-        cnbr[sidx] = code_of(c, pidx);
-        cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == c) ? 0x80000000u : 0u));
+    for (int q = 0; q < ncv; ++q) {
+      const int e = ext_of[xcell[q]];
+      for (int k = 0; k < ccnt[q]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[e] + k];
+        const int64_t sidx = cs.slot(q, k);
+        cnbr[sidx] = code_of(e, pidx);
+        cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == e) ? 0x80000000u : 0u));
+        for (int d = 0; d < 3; ++d) cfn[d][sidx] = nf[3 * (size_t)pidx + d];
       }
-      for (int k = 0; k < xcnt[c]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
-        const int64_t sidx = xs.slot(c, k);
-        xnbr[sidx] = code_of(c, pidx);
-        if (o.lo[pidx] == c) {
-          lo_slot[pidx - nfa] = int(sidx);
-          for (int d = 0; d < 3; ++d) {
-            xgeo[6 * (size_t)sidx + d] = nU[3 * (size_t)pidx + d];
-            xgeo[6 * (size_t)sidx + 3 + d] = cU[3 * (size_t)pidx + d];
-          }
+      for (int k = 0; k < xcnt[q]; ++k) {
+        const int pidx = o.efocv_v[o.efocv_i[e] + ccnt[q] + k];
+        const int64_t sidx = xs.slot(q, k);
+        xnbr[sidx] = code_of(e, pidx);
+        for (int d = 0; d < 3; ++d) {
+          x6[d][sidx] = nU[3 * (size_t)pidx + d];
+          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
         }
       }
     }
-    if (xs.nslots >= (int64_t(1) << 31)) return s->set(CG_EMESH, "too many extended half-faces for one instance");
-    for (int c = 0; c < ncv; ++c)
-      for (int k = 0; k < xcnt[c]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[c] + ccnt[c] + k];
-        if (o.hi[pidx] != c) continue;
-        const int src = lo_slot[pidx - nfa];
-        if (src < 0) return s->set(CG_EMESH, "extended pair without a lo-side half-face");
-        xsrc[xs.slot(c, k)] = src;
-      }
     // compact pairs (pair index = face index): per-pair SoA for the thread-per-pair kernel
     {
       std::vector<int> plo(nfa), phi(nfa);
-      std::vector<double> p14[14];
-      for (auto& v : p14) v.resize(nfa);
+      std::vector<double> p11[11];
+      for (auto& v : p11) v.resize(nfa);
       for (int f = 0; f < nfa; ++f) {
-        plo[f] = o.lo[f];
-        phi[f] = o.hi[f];
+        plo[f] = int_of[o.lo[f]];
+        phi[f] = int_of[o.hi[f]];
         for (int d = 0; d < 3; ++d) {
-          p14[d][f] = nU[3 * (size_t)f + d];
-          p14[3 + d][f] = cU[3 * (size_t)f + d];
-          p14[6 + d][f] = o.unit_n_use[3 * (size_t)f + d];
-          p14[11 + d][f] = nf[3 * (size_t)f + d];
+          p11[d][f] = nU[3 * (size_t)f + d];
+          p11[3 + d][f] = cU[3 * (size_t)f + d];
+          p11[6 + d][f] = o.unit_n_use[3 * (size_t)f + d];
         }
-        p14[9][f] = o.area_use[f];
-        p14[10][f] = o.aod_use[f];
+        p11[9][f] = o.area_use[f];
+        p11[10][f] = o.aod_use[f];
       }
       rc |= upload(s, h, &g.plo, plo);
       rc |= upload(s, h, &g.phi, phi);
-      const R** pdst[14] = {&g.pn0, &g.pn1, &g.pn2, &g.pc0, &g.pc1, &g.pc2, &g.pu0,
-                            &g.pu1, &g.pu2, &g.parea, &g.paod, &g.pf0, &g.pf1, &g.pf2};
-      for (int q = 0; q < 14; ++q) rc |= upload_as<R>(s, h, pdst[q], p14[q]);
+      const R** pdst[11] = {&g.pn0, &g.pn1, &g.pn2, &g.pc0, &g.pc1, &g.pc2,
+                            &g.pu0, &g.pu1, &g.pu2, &g.parea, &g.paod};
+      for (int q = 0; q < 11; ++q) rc |= upload_as<R>(s, h, pdst[q], p11[q]);
     }
     // per-block halos of the extended half-faces for LDS staging
     {
@@ -491,11 +567,12 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
         th.emplace_back([&, t] {
           std::vector<int> v;
           for (int bk = t; bk < nblk; bk += nth) {
-            const int c0 = bk * cgdev::kBlock, c1 = std::min(ncv, c0 + cgdev::kBlock);
+            // lane positions q0..q1 of the block hold the block's own cells q0..q1
+            const int q0 = bk * cgdev::kBlock, q1 = std::min(ncv, q0 + cgdev::kBlock);
             v.clear();
-            for (int c = c0; c < c1; ++c) {
-              v.push_back(c);
-              for (int k = 0; k < xcnt[c]; ++k) v.push_back(xnbr[xs.slot(c, k)] & 0x7fffffff);
+            for (int q = q0; q < q1; ++q) {
+              v.push_back(q);
+              for (int k = 0; k < xcnt[q]; ++k) v.push_back(xnbr[xs.slot(q, k)] & 0x7fffffff);
             }
             std::sort(v.begin(), v.end());
             v.erase(std::unique(v.begin(), v.end()), v.end());
@@ -504,9 +581,9 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
               herr = "block halo too large";
               return;
             }
-            for (int c = c0; c < c1; ++c)
-              for (int k = 0; k < xcnt[c]; ++k) {
-                const int64_t sidx = xs.slot(c, k);
+            for (int q = q0; q < q1; ++q)
+              for (int k = 0; k < xcnt[q]; ++k) {
+                const int64_t sidx = xs.slot(q, k);
                 const int nb = xnbr[sidx] & 0x7fffffff;
                 const int pos = int(std::lower_bound(v.begin(), v.end(), nb) - v.begin());
                 loc[sidx] = int16_t(pos | ((xnbr[sidx] < 0) ? 0x8000 : 0));
@@ -540,26 +617,35 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     rc |= upload(s, h, &g.ef_nc, ccnt);
     rc |= upload(s, h, &g.cf_nbr, cnbr);
     rc |= upload(s, h, &g.cf_pair, cpair);
+    rc |= upload_as<R>(s, h, &g.cfn0, cfn[0]);
+    rc |= upload_as<R>(s, h, &g.cfn1, cfn[1]);
+    rc |= upload_as<R>(s, h, &g.cfn2, cfn[2]);
     rc |= upload(s, h, &g.xf_ptr, xs.ptr);
     rc |= upload(s, h, &g.xf_deg, xcnt);
     rc |= upload(s, h, &g.xf_nbr, xnbr);
-    rc |= upload_as<R>(s, h, &g.xgeo, xgeo);
-    rc |= upload(s, h, &g.xsrc, xsrc);
+    const R** xdst[6] = {&g.xsn0, &g.xsn1, &g.xsn2, &g.xsc0, &g.xsc1, &g.xsc2};
+    for (int q = 0; q < 6; ++q) rc |= upload_as<R>(s, h, xdst[q], x6[q]);
     if (rc) return rc;
   }
   // gradient stencil without the diagonal
   {
-    std::vector<int> cnt(ncv);
-    for (int c = 0; c < ncv; ++c) cnt[c] = o.cvocv_i[c + 1] - o.cvocv_i[c] - 1;
+    std::vector<int64_t> key(ncv);
+    for (int c = 0; c < ncv; ++c) key[c] = o.cvocv_i[ext_of[c] + 1] - o.cvocv_i[ext_of[c]] - 1;
+    std::vector<uint8_t> gperm;
+    std::vector<int> gcell;
+    lane_order(ncv, key, sort_lanes, gperm, gcell);
+    rc |= upload(s, h, &g.gperm, gperm);
+    std::vector<int> cnt(ncv);  // by lane position
+    for (int q = 0; q < ncv; ++q) cnt[q] = int(key[gcell[q]]);
     Sell sl;
     sl.build(ncv, cnt);
     std::vector<int> nbr(sl.nslots, 0);
     std::vector<double> c0(sl.nslots, 0.0), c1(sl.nslots, 0.0), c2(sl.nslots, 0.0);
-    for (int c = 0; c < ncv; ++c)
-      for (int k = 0; k < cnt[c]; ++k) {
-        const int coc = o.cvocv_i[c] + 1 + k;
-        const int64_t sidx = sl.slot(c, k);
-        nbr[sidx] = o.cvocv_v[coc];
+    for (int q = 0; q < ncv; ++q)
+      for (int k = 0; k < cnt[q]; ++k) {
+        const int coc = o.cvocv_i[ext_of[gcell[q]]] + 1 + k;
+        const int64_t sidx = sl.slot(q, k);
+        nbr[sidx] = int_of[o.cvocv_v[coc]];
         c0[sidx] = o.grad_coeff[3 * (size_t)coc];
         c1[sidx] = o.grad_coeff[3 * (size_t)coc + 1];
         c2[sidx] = o.grad_coeff[3 * (size_t)coc + 2];
@@ -603,6 +689,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   hi->nblocks = lay->nblocks;
   hi->npblocks = lay->npblocks;
   hi->lds = lay->lds;
+  hi->ext_of = lay->ext_of.data();
   int rc = 0;
   // state buffers
   for (int k = 0; k < 2; ++k) {
@@ -731,13 +818,13 @@ template <class R>
 int engine_launch_step(Err* s, Engine<R>& E, hipStream_t stream) {
   const cgk::Launch<R>& L = E.L;
   if (L.nblk == 0) return 0;
-  if (cgk::launch_begin_step(L, stream)) return s->set(CG_EHIP, "begin_step launch failed");
+  if (const int le = cgk::launch_begin_step(L, stream)) return launch_fail(s, "begin_step", le);
   for (int stage = 1; stage <= 3; ++stage) {
-    if (cgk::launch_flux(stage, 1, L, E.cur, nullptr, stream)) return s->set(CG_EHIP, "flux launch failed");
+    if (const int le = cgk::launch_flux(stage, 1, L, E.cur, nullptr, stream)) return launch_fail(s, "flux", le);
     E.cur ^= 1;
-    if (cgk::launch_grad(L, E.cur, stream)) return s->set(CG_EHIP, "grad launch failed");
+    if (const int le = cgk::launch_grad(L, E.cur, stream)) return launch_fail(s, "grad", le);
   }
-  if (cgk::launch_props(L, E.cur, 0, stream)) return s->set(CG_EHIP, "props launch failed");
+  if (const int le = cgk::launch_props(L, E.cur, 0, stream)) return launch_fail(s, "props", le);
   return 0;
 }
 
@@ -745,6 +832,9 @@ int engine_launch_step(Err* s, Engine<R>& E, hipStream_t stream) {
 // (three stages flip the parity, so two steps return to it)
 template <class R>
 int engine_steps(Err* s, Engine<R>& E, hipStream_t stream, int nsteps, bool use_graph) {
+  // the lock is taken only to rebuild the table or to capture
+  std::unique_lock<std::recursive_mutex> lk(g_dev_mu, std::defer_lock);
+  if (E.dirty || (use_graph && nsteps >= 2 && !E.graph[E.cur])) lk.lock();
   if (int rc = engine_sync(s, E, stream)) return rc;
   int done = 0;
   if (use_graph && nsteps >= 2) {
@@ -769,6 +859,7 @@ int engine_steps(Err* s, Engine<R>& E, hipStream_t stream, int nsteps, bool use_
         return s->set(CG_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
       }
     }
+    if (lk.owns_lock()) lk.unlock();
     for (; done + 2 <= nsteps; done += 2) HIPCHK(s, hipGraphLaunch(gx, stream));
   }
   for (; done < nsteps; ++done)
@@ -782,9 +873,9 @@ int engine_steps(Err* s, Engine<R>& E, hipStream_t stream, int nsteps, bool use_
 template <class R>
 int refresh(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   const cgk::Launch<R> L = I->solo_launch();
-  if (cgk::launch_eos(L, cur, stream)) return s->set(CG_EHIP, "eos launch failed");
-  if (cgk::launch_grad(L, cur, stream)) return s->set(CG_EHIP, "grad launch failed");
-  if (cgk::launch_props(L, cur, 1, stream)) return s->set(CG_EHIP, "props launch failed");
+  if (const int le = cgk::launch_eos(L, cur, stream)) return launch_fail(s, "eos", le);
+  if (const int le = cgk::launch_grad(L, cur, stream)) return launch_fail(s, "grad", le);
+  if (const int le = cgk::launch_props(L, cur, 1, stream)) return launch_fail(s, "props", le);
   I->ready = true;
   return 0;
 }
@@ -889,46 +980,50 @@ int set_state(Err* s, Instance<R>* I, int cur, const double* rho, const double* 
   if constexpr (std::is_same<R, float>::value) {
     std::vector<double> cons(5 * n);
     for (size_t c = 0; c < n; ++c) {
-      cons[c] = rho[c];
-      for (int i = 0; i < 3; ++i) cons[(1 + i) * n + c] = u[3 * c + i];
-      cons[4 * n + c] = rhoE[c];
+      const size_t e = (size_t)I->ext_of[c];
+      cons[c] = rho[e];
+      for (int i = 0; i < 3; ++i) cons[(1 + i) * n + c] = u[3 * e + i];
+      cons[4 * n + c] = rhoE[e];
     }
     HIPCHK(s, hipMemcpy(st.cons, cons.data(), sizeof(double) * cons.size(), hipMemcpyHostToDevice));
   } else {
-    std::vector<double> light(n * cgdev::L_N, 0.0);
+    std::vector<double> light(n * cgdev::L_N, 0.0), e_tot(n);
     for (size_t c = 0; c < n; ++c) {
+      const size_t e = (size_t)I->ext_of[c];
       double* L = &light[c * cgdev::L_N];
-      L[cgdev::L_U0] = u[3 * c];
-      L[cgdev::L_U1] = u[3 * c + 1];
-      L[cgdev::L_U2] = u[3 * c + 2];
-      L[cgdev::L_RHO] = rho[c];
+      L[cgdev::L_U0] = u[3 * e];
+      L[cgdev::L_U1] = u[3 * e + 1];
+      L[cgdev::L_U2] = u[3 * e + 2];
+      L[cgdev::L_RHO] = rho[e];
+      e_tot[c] = rhoE[e];
     }
     HIPCHK(s, hipMemcpy(st.light, light.data(), sizeof(double) * light.size(), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(st.rhoE, rhoE, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(st.rhoE, e_tot.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   }
   I->ready = false;
   return 0;
 }
 
 // copy rows of a device SoA array (row stride ncv) to out[ncv][ncomp]
-int get_soa(Err* s, int ncv, double* const dev[], int ncomp, double* out, hipStream_t stream) {
+int get_soa(Err* s, int ncv, const int* ext, double* const dev[], int ncomp, double* out, hipStream_t stream) {
   std::vector<double> tmp((size_t)ncv);
   for (int q = 0; q < ncomp; ++q) {
     HIPCHK(s, hipMemcpyAsync(tmp.data(), dev[q], sizeof(double) * ncv, hipMemcpyDeviceToHost, stream));
     HIPCHK(s, hipStreamSynchronize(stream));
-    for (int c = 0; c < ncv; ++c) out[(size_t)c * ncomp + q] = tmp[c];
+    for (int c = 0; c < ncv; ++c) out[(size_t)ext[c] * ncomp + q] = tmp[c];
   }
   return 0;
 }
 
 // copy fields [f0, f0+nf) of a device record array [ncv][stride] to out[ncv][nf]
 template <class T>
-int get_rec(Err* s, int ncv, const T* rec, int stride, int f0, int nf, double* out, hipStream_t stream) {
+int get_rec(Err* s, int ncv, const int* ext, const T* rec, int stride, int f0, int nf, double* out,
+            hipStream_t stream) {
   std::vector<T> tmp((size_t)ncv * stride);
   HIPCHK(s, hipMemcpyAsync(tmp.data(), rec, sizeof(T) * tmp.size(), hipMemcpyDeviceToHost, stream));
   HIPCHK(s, hipStreamSynchronize(stream));
   for (int c = 0; c < ncv; ++c)
-    for (int q = 0; q < nf; ++q) out[(size_t)c * nf + q] = double(tmp[(size_t)c * stride + f0 + q]);
+    for (int q = 0; q < nf; ++q) out[(size_t)ext[c] * nf + q] = double(tmp[(size_t)c * stride + f0 + q]);
   return 0;
 }
 
@@ -941,24 +1036,24 @@ int get_state(Err* s, Instance<R>* I, int cur, double* rho, double* u, double* r
     double* cons = st.cons;
     if (rho) {
       double* f[1] = {cons};
-      if (int rc = get_soa(s, ncv, f, 1, rho, stream)) return rc;
+      if (int rc = get_soa(s, ncv, I->ext_of, f, 1, rho, stream)) return rc;
     }
     if (u) {
       double* f[3] = {cons + n, cons + 2 * n, cons + 3 * n};
-      if (int rc = get_soa(s, ncv, f, 3, u, stream)) return rc;
+      if (int rc = get_soa(s, ncv, I->ext_of, f, 3, u, stream)) return rc;
     }
     if (rhoE) {
       double* f[1] = {cons + 4 * n};
-      if (int rc = get_soa(s, ncv, f, 1, rhoE, stream)) return rc;
+      if (int rc = get_soa(s, ncv, I->ext_of, f, 1, rhoE, stream)) return rc;
     }
   } else {
     if (rho)
-      if (int rc = get_rec(s, ncv, st.light, cgdev::L_N, cgdev::L_RHO, 1, rho, stream)) return rc;
+      if (int rc = get_rec(s, ncv, I->ext_of, st.light, cgdev::L_N, cgdev::L_RHO, 1, rho, stream)) return rc;
     if (u)
-      if (int rc = get_rec(s, ncv, st.light, cgdev::L_N, cgdev::L_U0, 3, u, stream)) return rc;
+      if (int rc = get_rec(s, ncv, I->ext_of, st.light, cgdev::L_N, cgdev::L_U0, 3, u, stream)) return rc;
     if (rhoE) {
       double* f[1] = {st.rhoE};
-      if (int rc = get_soa(s, ncv, f, 1, rhoE, stream)) return rc;
+      if (int rc = get_soa(s, ncv, I->ext_of, f, 1, rhoE, stream)) return rc;
     }
   }
   return 0;
@@ -966,10 +1061,10 @@ int get_state(Err* s, Instance<R>* I, int cur, double* rho, double* u, double* r
 
 template <class R>
 int calc_cfl(Err* s, Instance<R>* I, int cur, double dt, double* cfl_max, hipStream_t stream) {
-  if (cgk::launch_cfl(I->solo_launch(), cur, dt, stream)) return s->set(CG_EHIP, "cfl launch failed");
+  if (const int le = cgk::launch_cfl(I->solo_launch(), cur, dt, stream)) return launch_fail(s, "cfl", le);
   std::vector<double> c((size_t)I->ncv);
   double* f[1] = {I->b.w.cfl};
-  if (int rc = get_soa(s, I->ncv, f, 1, c.data(), stream)) return rc;
+  if (int rc = get_soa(s, I->ncv, I->ext_of, f, 1, c.data(), stream)) return rc;
   double mx = 0.0;
   for (double v : c) mx = (mx < v) ? v : mx;
   *cfl_max = mx;
@@ -978,7 +1073,7 @@ int calc_cfl(Err* s, Instance<R>* I, int cur, double dt, double* cfl_max, hipStr
 
 template <class R>
 int get_diag(Err* s, Instance<R>* I, int cur, cgdev::StepInfo* host, cg_diag* d, hipStream_t stream) {
-  if (cgk::launch_diag(I->solo_launch(), cur, stream)) return s->set(CG_EHIP, "diag launch failed");
+  if (const int le = cgk::launch_diag(I->solo_launch(), cur, stream)) return launch_fail(s, "diag", le);
   double v[8];
   HIPCHK(s, hipMemcpyAsync(v, I->b.w.diag, sizeof(v), hipMemcpyDeviceToHost, stream));
   if (int rc = read_info(s, I, host, stream)) return rc;
@@ -1331,6 +1426,7 @@ int solver_steps(cg_solver* s, Engine<R>& E, Instance<R>* I, int nsteps) {
 extern "C" {
 
 int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
+  DevLock lk(g_dev_mu);
   if (!m || !out) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return rc;
   *out = nullptr;
@@ -1357,8 +1453,14 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out) {
     if (!rc) s->ed.inst.push_back(s->id);
   }
   if (rc) return bail(rc);
+  // CG_TILE: extended slots per batch of the LDS pass (2, 4 or 8; as the
+  // batch option "tile")
+  int tile = 0;
+  if (const char* t = std::getenv("CG_TILE")) tile = std::atoi(t);
+  if (tile && tile != 2 && tile != 4 && tile != 8) return bail(s->set(CG_EINVAL, "CG_TILE must be 2, 4 or 8"));
   rc = with_inst(s, [&](auto& E, auto* I) {
     (void)I;
+    if (tile) E.tile = tile;
     return engine_sync(s, E, s->stream);
   });
   if (rc) return bail(rc);
@@ -1400,7 +1502,7 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
     const int ncv = I->ncv;
     if (nm == "dudx") {
       if (n != 9 * (int64_t)ncv) return s->set(CG_EINVAL, "dudx needs 9*ncv values");
-      return get_rec(s, ncv, w.aux, cgdev::A_N, cgdev::A_D, 9, out, s->stream);
+      return get_rec(s, ncv, I->ext_of, w.aux, cgdev::A_N, cgdev::A_D, 9, out, s->stream);
     }
     if (n != ncv) return s->set(CG_EINVAL, "field needs ncv values");
     struct { const char* nm; int light; int f; int prim; } recf[] = {
@@ -1409,16 +1511,16 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
         {"symh_fax", 0, cgdev::A_SYMH, 0}, {"symp_fax", 0, cgdev::A_SYMP, 0}};
     for (auto& r : recf) {
       if (nm != r.nm) continue;
-      if (!r.light) return get_rec(s, ncv, w.aux, cgdev::A_N, r.f, 1, out, s->stream);
-      if (!s->mixed) return get_rec(s, ncv, st.light, cgdev::L_N, r.f, 1, out, s->stream);
+      if (!r.light) return get_rec(s, ncv, I->ext_of, w.aux, cgdev::A_N, r.f, 1, out, s->stream);
+      if (!s->mixed) return get_rec(s, ncv, I->ext_of, st.light, cgdev::L_N, r.f, 1, out, s->stream);
       // mixed precision: FP64 primitives from the FP64 state
-      if (cgk::launch_prims(I->solo_launch(), 0, ncv, E.cur, I->rhs_tmp, s->stream))
-        return s->set(CG_EHIP, "prims launch failed");
+      if (const int le = cgk::launch_prims(I->solo_launch(), 0, ncv, E.cur, I->rhs_tmp, s->stream))
+        return launch_fail(s, "prims", le);
       double* f[1] = {I->rhs_tmp + (size_t)r.prim * ncv};
-      return get_soa(s, ncv, f, 1, out, s->stream);
+      return get_soa(s, ncv, I->ext_of, f, 1, out, s->stream);
     }
     // per-step diagnostic fields: recompute the properties of the current state
-    if (cgk::launch_props(I->solo_launch(), E.cur, 1, s->stream)) return s->set(CG_EHIP, "props launch failed");
+    if (const int le = cgk::launch_props(I->solo_launch(), E.cur, 1, s->stream)) return launch_fail(s, "props", le);
     double* src = nullptr;
     if (nm == "T") src = w.T;
     else if (nm == "mu_lam") src = w.mu_lam;
@@ -1429,12 +1531,12 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
     else if (nm == "vv2") src = w.vv2;
     else if (nm == "cfl") {
       if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
-      if (cgk::launch_cfl(I->solo_launch(), E.cur, s->info_host->dt, s->stream))
-        return s->set(CG_EHIP, "cfl launch failed");
+      if (const int le = cgk::launch_cfl(I->solo_launch(), E.cur, s->info_host->dt, s->stream))
+        return launch_fail(s, "cfl", le);
       src = w.cfl;
     } else return s->set(CG_EINVAL, "unknown field " + nm);
     double* f[1] = {src};
-    return get_soa(s, ncv, f, 1, out, s->stream);
+    return get_soa(s, ncv, I->ext_of, f, 1, out, s->stream);
   });
 }
 
@@ -1443,11 +1545,11 @@ int cg_calc_rhs(cg_solver* s, int stage, double* rhs) {
   if (stage < 1 || stage > 3) return s->set(CG_EINVAL, "stage must be 1, 2 or 3");
   return with_inst(s, [&](auto& E, auto* I) -> int {
     if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
-    if (cgk::launch_flux(stage, 0, I->solo_launch(), E.cur, I->rhs_tmp, s->stream))
-      return s->set(CG_EHIP, "flux launch failed");
+    if (const int le = cgk::launch_flux(stage, 0, I->solo_launch(), E.cur, I->rhs_tmp, s->stream))
+      return launch_fail(s, "flux", le);
     double* f[5];
     for (int q = 0; q < 5; ++q) f[q] = I->rhs_tmp + (size_t)q * I->ncv;
-    return get_soa(s, I->ncv, f, 5, rhs, s->stream);
+    return get_soa(s, I->ncv, I->ext_of, f, 5, rhs, s->stream);
   });
 }
 
@@ -1532,14 +1634,14 @@ int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms)
       const auto& L = E.L;
       int rc = 0;
       for (int i = 0; i < nsteps && !rc; ++i) {
-        if (cgk::launch_begin_step(L, s->stream)) {
-          rc = s->set(CG_EHIP, "launch failed");
+        if (const int le = cgk::launch_begin_step(L, s->stream)) {
+          rc = launch_fail(s, "begin_step", le);
           break;
         }
         int k = 0;
         auto rec = [&](int family, auto&& launch) -> int {
           HIPCHK(s, hipEventRecord(ev[2 * k], s->stream));
-          if (launch()) return s->set(CG_EHIP, "launch failed");
+          if (const int le = launch()) return launch_fail(s, "timed", le);
           HIPCHK(s, hipEventRecord(ev[2 * k + 1], s->stream));
           fam[k++] = family;
           return 0;
@@ -1579,6 +1681,7 @@ const char* cg_last_error(const cg_solver* s) { return s ? s->err.c_str() : g_er
 
 void cg_destroy(cg_solver* s) {
   if (!s) return;
+  DevLock lk(g_dev_mu);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   drop_graph(s->ed);
   drop_graph(s->ef);
@@ -1595,6 +1698,7 @@ void cg_destroy(cg_solver* s) {
 // ---- batches ----
 
 int cg_batch_create(int precision, cg_batch** out) {
+  DevLock lk(g_dev_mu);
   if (!out) return fail(CG_EINVAL, "null output");
   if (precision != CG_PREC_FP64 && precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   *out = nullptr;
@@ -1614,6 +1718,7 @@ int cg_batch_create(int precision, cg_batch** out) {
 }
 
 int cg_layout_create(const cg_mesh* m, int precision, cg_layout** out) {
+  DevLock lk(g_dev_mu);
   if (!m || !out) return fail(CG_EINVAL, "null argument");
   if (precision != CG_PREC_FP64 && precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   *out = nullptr;
@@ -1636,7 +1741,10 @@ int cg_layout_bytes(const cg_layout* L, int64_t* bytes) {
   return CG_OK;
 }
 
-void cg_layout_destroy(cg_layout* L) { delete L; }
+void cg_layout_destroy(cg_layout* L) {
+  DevLock lk(g_dev_mu);
+  delete L;
+}
 
 }  // extern "C"
 
@@ -1671,6 +1779,7 @@ extern "C" {
 
 int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* rho, const double* u,
                           const double* rhoE, cg_instance** out) {
+  DevLock lk(g_dev_mu);
   if (!L || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return rc;
   if ((p->precision == CG_PREC_MIXED) != L->mixed) return fail(CG_EINVAL, "parameters' precision differs from the layout's");
@@ -1690,6 +1799,7 @@ int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* 
 
 int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
                        const double* rhoE, cg_instance** out) {
+  DevLock lk(g_dev_mu);
   if (!m || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return rc;
   *out = nullptr;
@@ -1715,12 +1825,14 @@ int cg_instance_bytes(const cg_instance* x, int64_t bytes[2]) {
 
 void cg_instance_destroy(cg_instance* x) {
   if (!x) return;
+  DevLock lk(g_dev_mu);
   free_instance(x->id);
   free_instance(x->ix);
   delete x;
 }
 
 int cg_batch_attach(cg_batch* b, cg_instance* x, int* id) {
+  DevLock lk(g_dev_mu);
   if (!b || !x || !id) return fail(CG_EINVAL, "null argument");
   if (x->mixed != b->mixed) return b->set(CG_EINVAL, "instance precision differs from the batch's");
   auto attach = [&](auto& E, auto*& I) -> int {
@@ -1743,6 +1855,7 @@ int cg_batch_attach(cg_batch* b, cg_instance* x, int* id) {
 
 int cg_batch_add(cg_batch* b, const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
                  const double* rhoE, int* id) {
+  DevLock lk(g_dev_mu);
   if (!b || !m || !p || !rho || !u || !rhoE || !id) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return b->set(rc, g_err);
   if ((p->precision == CG_PREC_MIXED) != b->mixed) return b->set(CG_EINVAL, "instance precision differs from the batch's");
@@ -1756,6 +1869,7 @@ int cg_batch_add(cg_batch* b, const cg_mesh* m, const cg_params* p, const double
 }
 
 int cg_batch_remove(cg_batch* b, int id) {
+  DevLock lk(g_dev_mu);
   if (!b) return fail(CG_EINVAL, "null batch");
   return with_engine(b, [&](auto& E) -> int {
     for (size_t i = 0; i < E.inst.size(); ++i) {
@@ -1928,6 +2042,7 @@ const char* cg_batch_last_error(const cg_batch* b) { return b ? b->err.c_str() :
 
 void cg_batch_destroy(cg_batch* b) {
   if (!b) return;
+  DevLock lk(g_dev_mu);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b->ed);
   drop_graph(b->ef);

@@ -49,10 +49,17 @@ enum { A_MU = 0, A_LOC, A_ASGS, A_FGR, A_SYMH, A_SYMP, A_PAD, A_D = 7, A_N = 16 
 // once per pair by their own kernel (thread per pair, per-pair SoA geometry),
 // which writes each pair's flux to w.cflux. Each cell's half-faces (its
 // extended pairs, in pair order) are split in two SELL-64 regions: the compact
-// ones (cf_*, ef_nc[cell] slots: the pair index, to gather cflux) and the
-// purely extended ones (xf_*, xf_deg[cell] slots). Slot k of a cell is at
-// ptr[slice] + k*64 + lane, so at every k the 64 lanes of a wavefront read one
-// contiguous row.
+// ones (cf_*, ef_nc[pos] slots: the pair index, to gather cflux) and the
+// purely extended ones (xf_*, xf_deg[pos] slots). Slot k at lane position pos
+// is at ptr[pos / 64] + k*64 + pos % 64, so at every k the 64 lanes of a
+// wavefront read one contiguous row. Position pos holds cell
+// (pos & ~63) | xperm[pos]: within each 64-cell slice the lanes take the cells
+// in decreasing order of their row counts (SELL-C-sigma with C = sigma = 64),
+// so the lanes still active in a slice's tail rows are contiguous and cache
+// lines of finished lanes are not fetched; a permutation within the slice
+// touches the same cache lines for every per-cell array. The gradient region
+// has its own order (gperm). The extended pair geometry is stored per slot
+// (both cells of a pair hold a copy), which keeps every geometry read coalesced.
 template <class R>
 struct GeomT {
   int ncv, nfa, nef;
@@ -61,25 +68,26 @@ struct GeomT {
   const int *plo, *phi;                           // the pair's cells (lo < hi in pair orientation)
   const R *pn0, *pn1, *pn2, *pc0, *pc1, *pc2;     // pair n and c (grouping applied)
   const R *pu0, *pu1, *pu2, *parea, *paod;        // unit normal, area, area/delta (grouping applied)
-  const R *pf0, *pf1, *pf2;                       // raw area vector n_fa (CFL)
+  // lane order of the half-face regions: position -> cell offset in its slice
+  const uint8_t* xperm;
   // compact half-faces
   const int *cf_ptr, *ef_nc;
   const int* cf_nbr;                              // neighbour | (cell is the pair's hi side) << 31
   const int* cf_pair;                             // pair index | (cell is the pair's hi side) << 31
-  // purely extended half-faces. The pair geometry n, c is stored once, at the
-  // slot of the pair's lo cell (xgeo[slot][6]); the hi cell's slot holds the
-  // index of that slot (xsrc) and gathers it.
+  const R *cfn0, *cfn1, *cfn2;                    // the pair's raw area vector n_fa (CFL), per slot
+  // purely extended half-faces
   const int *xf_ptr, *xf_deg;
   const int* xf_nbr;                              // neighbour | hi-side flag << 31
-  const R* xgeo;                                  // [slots][6]: n0 n1 n2 c0 c1 c2 (lo-side slots)
-  const int* xsrc;                                // hi-side slot -> slot of the lo-side copy
+  const R *xsn0, *xsn1, *xsn2, *xsc0, *xsc1, *xsc2;
   // per-block halo for LDS staging (null when staging is off): the sorted
   // cells block b's extended half-faces reach are halo_v[halo_ptr[b] .. halo_ptr[b+1]);
   // xf_loc[slot] = halo position of the slot's neighbour | hi-side flag << 15
   const int *halo_ptr, *halo_v;
   const int16_t* xf_loc;
   int halo_max;
-  // gradient stencil without the diagonal, SELL-64 layout
+  // gradient stencil without the diagonal, SELL-64 layout in its own lane
+  // order (gperm, as xperm)
+  const uint8_t* gperm;
   const int *gr_ptr, *gr_deg, *gr_nbr;
   const R *gc0, *gc1, *gc2;
 };
@@ -146,11 +154,11 @@ constexpr int kBlkBits = 20;  // up to 2^20 blocks (268M cells) per instance, 2^
 
 }  // namespace cgdev
 
-// Launchers (ig_kernels.hip), instantiated for R = double and R = float. `I`
-// is the device instance table, `bmap` / `pbmap` the device block maps of the
-// cell and compact-pair launches, `lds` the dynamic LDS bytes of the
-// extended-face pass (0: the global-gather variant; every instance then needs
-// no halo).
+// Launchers (ig_kernels.inl), instantiated for R = double (ig_kernels_f64.hip)
+// and R = float (ig_kernels_f32.hip). `I` is the device instance table,
+// `bmap` / `pbmap` the device block maps of the cell and compact-pair
+// launches, `lds` the dynamic LDS bytes of the extended-face pass (0: the
+// global-gather variant; every instance then needs no halo).
 namespace cgk {
 template <class R>
 struct Launch {

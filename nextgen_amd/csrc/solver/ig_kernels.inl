@@ -14,10 +14,14 @@
 // the 8 XCDs) works on one contiguous stretch of the Morton curve: a spatial
 // sub-box whose neighbour records stay in that XCD's L2.
 //
-// This file is compiled with -ffp-contract=off: every product and sum is
-// rounded on its own, in the order of the reference expressions. Constants are
-// written R(c) so the FP64 instantiation evaluates exactly the reference
-// expressions and the FP32 one stays in FP32.
+// Included by ig_kernels_f64.hip (R = double, compiled with -ffp-contract=off:
+// every product and sum rounded on its own, in the order of the reference
+// expressions) and ig_kernels_f32.hip (R = float, -ffp-contract=on: FP32
+// products and sums within one expression fuse into FMAs; the FP64 code the
+// mixed kernels share, EOS recovery, RK combination, sums and dt, keeps
+// contraction off by pragma).
+// Constants are written R(c) so the FP64 instantiation evaluates exactly the
+// reference expressions and the FP32 one stays in FP32.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -47,6 +51,28 @@ __device__ __forceinline__ double mexpm1(double x) { return expm1(x); }
 __device__ __forceinline__ float mexpm1(float x) { return expm1f(x); }
 __device__ __forceinline__ double mfmin(double a, double b) { return fmin(a, b); }
 __device__ __forceinline__ float mfmin(float a, float b) { return fminf(a, b); }
+// Quotients of the flux functions: correctly rounded in FP64 (the reference's
+// arithmetic); in FP32 the hardware reciprocal (1 ulp) times the numerator,
+// 2 instructions instead of the ~10 of an IEEE division
+__device__ __forceinline__ double mdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float mdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// log(1 + x) in FP32 for the mixed-precision Gibbs remainder: for |x| < 1/16,
+// 2 artanh(s) with s = x / (2 + x) (|s| < 1/31), the odd series through s^5
+// (the first omitted term is below 2^-50 relative; measured within 2 ulp of
+// log1p over the range); log1pf beyond
+__device__ __forceinline__ float mlog1p(float x) {
+  if (fabsf(x) < 0.0625f) {
+    const float s = x * __builtin_amdgcn_rcpf(2.0f + x);
+    const float s2 = s * s;
+    return 2.0f * (s + s * s2 * ((1.0f / 3.0f) + s2 * (1.0f / 5.0f)));
+  }
+  return log1pf(x);
+}
+// a / 3 and a / 6: a division in FP64, a product with the rounded reciprocal in FP32
+__device__ __forceinline__ double mdiv3(double a) { return a / 3.0; }
+__device__ __forceinline__ float mdiv3(float a) { return a * (1.0f / 3.0f); }
+__device__ __forceinline__ double mdiv6(double a) { return a / 6.0; }
+__device__ __forceinline__ float mdiv6(float a) { return a * (1.0f / 6.0f); }
 
 template <class R> struct V2;
 template <> struct V2<double> { using T = double2; };
@@ -139,7 +165,7 @@ __device__ __forceinline__ R half_usq(const Light<R>& L) {
 template <class R>
 __device__ __forceinline__ void flux_skew(R F[5], const R n[3], const Light<R>& a, const Light<R>& b, R u_avg[3],
                                           R& inv_v_avg, R& h_stag) {
-  inv_v_avg = R(2) / (a.spv + b.spv);
+  inv_v_avg = mdiv(R(2), a.spv + b.spv);
   const R p_avg = R(0.5) * (b.p + a.p);
   R un_avg = R(0), u0u1 = R(0);
 #pragma unroll
@@ -168,7 +194,7 @@ __device__ __forceinline__ void flux_symmetric(R cF[5], const R c[3], const R u_
   R duc = R(0), uavg_c = R(0);
   const R dp = b.p - a.p;
   R dH = b.h - a.h;
-  const R drho = (a.spv - b.spv) / (a.spv * b.spv);
+  const R drho = mdiv(a.spv - b.spv, a.spv * b.spv);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const R du_ = b.u[i] - a.u[i];
@@ -194,7 +220,7 @@ __device__ __forceinline__ void flux_compact(R F[5], const R un[3], R area, R ao
                                              const Light<R>& b, const Aux<R>& A0, const Aux<R>& A1, const R u_avg[3],
                                              R h_stag, R p_g) {
   const R aod_half = R(0.5) * aod;
-  const R delta = area / aod;
+  const R delta = mdiv(area, aod);
   const R mu_tmp = R(0.5) * (A0.mu_total + A1.mu_total) * area;
   const R mu_total_c = (A0.mu_total + A1.mu_total) * aod_half;
   const R loc_coeff = (A0.loc_total + A1.loc_total) * aod_half;
@@ -207,14 +233,14 @@ __device__ __forceinline__ void flux_compact(R F[5], const R un[3], R area, R ao
     u1_cn += b.u[i] * un[i];
   }
   const R dun = u1_cn - u0_cn;
-  const R one_third_dun = dun / R(3);
-  const R one_sixth_dun2 = (u1_cn * u1_cn - u0_cn * u0_cn) / R(6);
+  const R one_third_dun = mdiv3(dun);
+  const R one_sixth_dun2 = mdiv6(u1_cn * u1_cn - u0_cn * u0_cn);
   const R dp = b.p - a.p;
   const R fgr_avg = R(0.5) * (A1.fgr + A0.fgr);
   const R rho_avg = R(0.5) * (b.rho + a.rho);
   const R un_avg = R(0.5) * (u0_cn + u1_cn);
   const R sos_avg = R(0.5) * (a.sos + b.sos);
-  const R Ma_n = mabs(un_avg) / sos_avg;
+  const R Ma_n = mdiv(mabs(un_avg), sos_avg);
   const R drho = b.rho - a.rho;
   const R lf_coeff = smin(fgr_avg * Ma_n, R(0.5)) * (sos_avg + mabs(un_avg)) * area;
   const R lfdrho = lf_coeff * drho;
@@ -260,8 +286,8 @@ __device__ __forceinline__ void flux_compact(R F[5], const R un[3], R area, R ao
     one_third_dundxn += dundx[k] * un[k];
     two_third_Skk += g0[k * 3 + k] + g1[k * 3 + k];
   }
-  two_third_Skk /= R(3);
-  one_third_dundxn /= R(3);
+  two_third_Skk = mdiv3(two_third_Skk);
+  one_third_dundxn = mdiv3(one_third_dundxn);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const R tmp = (dundx[i] - un[i] * (one_third_dundxn + two_third_Skk)) * mu_tmp;
@@ -272,6 +298,7 @@ __device__ __forceinline__ void flux_compact(R F[5], const R un[3], R area, R ao
 
 // EOS recovery of updateExtendedState (IdealGasSolver.cpp:999-1024), FP64.
 __device__ __forceinline__ Light<double> eos(const Params& P, double rho, const double u[3], double rhoE, double& T) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   double usq = 0.0;
   usq += u[0] * u[0];
   usq += u[1] * u[1];
@@ -289,6 +316,7 @@ __device__ __forceinline__ Light<double> eos(const Params& P, double rho, const 
 
 // FP32 neighbour record of an FP64 primitive state, with the p and h gauges
 __device__ __forceinline__ Light<float> to_mixed(const Params& P, const Light<double>& L) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   Light<float> r;
   r.u[0] = float(L.u[0]); r.u[1] = float(L.u[1]); r.u[2] = float(L.u[2]);
   r.spv = float(L.spv);
@@ -350,6 +378,7 @@ template <int STAGE>
 __device__ __forceinline__ void rk_update(const GeomT<float>& g, const StateT<float>& in, const StateT<float>& out,
                                           const WorkT<float>& w, const Params& P, int c, const Light<float>&,
                                           const float acc[5]) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   const int ncv = g.ncv;
   const double dt = G(w.info)->dt;
   const double tmp = dt * G(g.inv_vol)[c];
@@ -398,6 +427,7 @@ __device__ __forceinline__ void rk_update(const GeomT<float>& g, const StateT<fl
 // energy row from its gauged form
 template <class R>
 __device__ __forceinline__ void write_rhs(double* rhs_out, int ncv, int c, const R acc[5], const Params& P) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   if constexpr (kMixed<R>) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) rhs_out[(size_t)q * ncv + c] = double(acc[q]);
@@ -447,22 +477,17 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   o[5] = R(0);
 }
 
-// Geometry n, c of an extended half-face: from the slot itself on the pair's
-// lo side, through xsrc from the lo cell's slot on the hi side.
-template <class R>
-__device__ __forceinline__ void ext_geom(const GeomT<R>& g, int slot, bool self_hi, R n[3], R cv[3]) {
-  const int gi = self_hi ? ldnt(g.xsrc + slot) : slot;
-  const auto q = G(g.xgeo + (size_t)gi * 6);
-  n[0] = q[0]; n[1] = q[1]; n[2] = q[2];
-  cv[0] = q[3]; cv[1] = q[4]; cv[2] = q[5];
+// The cell at lane position pos of a SELL region with lane order perm.
+__device__ __forceinline__ int cell_at(const uint8_t* perm, int pos) {
+  return (pos & ~(kSlice - 1)) | int(G(perm)[pos]);
 }
 
-// A cell's compact-pair fluxes in pair order (its first ef_nc pairs): the
-// start of its stage RHS accumulation.
+// The compact-pair fluxes, in pair order, of the cell at lane position pos
+// (its first ef_nc pairs): the start of its stage RHS accumulation.
 template <class R>
-__device__ __forceinline__ void gather_compact(const GeomT<R>& g, const WorkT<R>& w, int c, R acc[5]) {
-  const int cbase = G(g.cf_ptr)[c >> 6] + (c & (kSlice - 1));
-  const int nc = G(g.ef_nc)[c];
+__device__ __forceinline__ void gather_compact(const GeomT<R>& g, const WorkT<R>& w, int pos, R acc[5]) {
+  const int cbase = G(g.cf_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int nc = G(g.ef_nc)[pos];
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
   // pair codes and fluxes are fetched 4 slots at a time so the gathers
@@ -509,17 +534,17 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
   const Params& P = X.P;
-  const int c = lblk * kBlock + threadIdx.x;
-  if (c >= g.ncv) return;
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
   if (MODE == 1 && G(w.info)->nan) return;
-  const int lane = c & (kSlice - 1);
-  const int base = G(g.xf_ptr)[c >> 6] + lane;
-  const int deg = G(g.xf_deg)[c];
+  const int c = cell_at(g.xperm, pos);
+  const int base = G(g.xf_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int deg = G(g.xf_deg)[pos];
   const int ncv = g.ncv;
 
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  gather_compact(g, w, c, acc);
+  gather_compact(g, w, pos, acc);
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
     const int code = ldnt(g.xf_nbr + slot);
@@ -528,8 +553,8 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
     const Light<R> other = load_light(in.light, nb);
     const Light<R>& a = self_hi ? other : self;
     const Light<R>& b = self_hi ? self : other;
-    R n[3], cv[3];
-    ext_geom(g, slot, self_hi, n, cv);
+    const R n[3] = {ldnt(g.xsn0 + slot), ldnt(g.xsn1 + slot), ldnt(g.xsn2 + slot)};
+    const R cv[3] = {ldnt(g.xsc0 + slot), ldnt(g.xsc1 + slot), ldnt(g.xsc2 + slot)};
     R F[5], u_avg[3], inv_v_avg, h_stag;
     flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
     if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
@@ -581,15 +606,15 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
     halo[3 * i + 2] = mk2(q[4], q[5]);
   }
   __syncthreads();
-  const int c = blk * kBlock + threadIdx.x;
-  if (c >= g.ncv) return;
-  const int lane = c & (kSlice - 1);
-  const int base = G(g.xf_ptr)[c >> 6] + lane;
-  const int deg = G(g.xf_deg)[c];
+  const int pos = blk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.xperm, pos);
+  const int base = G(g.xf_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int deg = G(g.xf_deg)[pos];
   const int ncv = g.ncv;
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  gather_compact(g, w, c, acc);
+  gather_compact(g, w, pos, acc);
   // slots are fetched NB at a time so their global loads overlap (NB is a
   // tuning choice); the accumulation order stays k = 0, 1, ...
   constexpr int kBatch = NB;
@@ -601,7 +626,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
       if (k0 + j < deg) {
         const int slot = base + (k0 + j) * kSlice;
         code[j] = ldnt(g.xf_loc + slot);
-        ext_geom(g, slot, (code[j] & 0x8000) != 0, n[j], cv[j]);
+        n[j][0] = ldnt(g.xsn0 + slot); n[j][1] = ldnt(g.xsn1 + slot); n[j][2] = ldnt(g.xsn2 + slot);
+        cv[j][0] = ldnt(g.xsc0 + slot); cv[j][1] = ldnt(g.xsc1 + slot); cv[j][2] = ldnt(g.xsc2 + slot);
       }
     }
 #pragma unroll
@@ -647,12 +673,13 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
-  const int c = lblk * kBlock + threadIdx.x;
-  if (c == 0) G(w.info)->cfl_bits = 0ull;  // reset for the properties pass that follows
-  if (c >= g.ncv) return;
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos == 0) G(w.info)->cfl_bits = 0ull;  // reset for the properties pass that follows
+  if (pos >= g.ncv) return;
   if (G(w.info)->nan) return;
-  const int base = G(g.gr_ptr)[c >> 6] + (c & (kSlice - 1));
-  const int deg = G(g.gr_deg)[c];
+  const int c = cell_at(g.gperm, pos);
+  const int base = G(g.gr_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int deg = G(g.gr_deg)[pos];
   const auto L = G(st.light);
   const R ui[3] = {L[(size_t)c * L_N + L_U0], L[(size_t)c * L_N + L_U1], L[(size_t)c * L_N + L_U2]};
   R d[9];
@@ -773,24 +800,31 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
   const Params& P = X.P;
+  const R Rg = R(P.R_gas);
+  const R pg = kMixed<R> ? R(P.p_g) : R(0);
+  // per-cell terms of the Gibbs remainder, evaluated once per cell instead of
+  // once per half-face: 1/(R T) = 1/(R (p / (rho R))), and the last one
+  // ln p (FP64) or 1/p with p absolute (mixed precision)
+  auto irt_of = [&](R pp, R rr) { return mdiv(R(1), Rg * mdiv(kMixed<R> ? pp + pg : pp, rr * Rg)); };
   extern __shared__ __align__(16) unsigned char smem[];
-  R* hrec = reinterpret_cast<R*>(smem);  // [hsize][4]: p, sp_vol, rho, logp
+  R* hrec = reinterpret_cast<R*>(smem);  // [hsize][4]: p, sp_vol, 1/(R T), ln p | 1/p
   if constexpr (LDS) {
     const int hb = G(g.halo_ptr)[lblk], hs = G(g.halo_ptr)[lblk + 1] - hb;
     for (int i = threadIdx.x; i < hs; i += kBlock) {
       const int gc = G(g.halo_v)[hb + i];
       const auto q = G(st.light + (size_t)gc * L_N);
-      hrec[4 * i + 0] = q[L_P];
+      const R qp = q[L_P];
+      hrec[4 * i + 0] = qp;
       hrec[4 * i + 1] = q[L_SPV];
-      hrec[4 * i + 2] = q[L_RHO];
-      if constexpr (!kMixed<R>) hrec[4 * i + 3] = G(w.logp)[gc];
+      hrec[4 * i + 2] = irt_of(qp, q[L_RHO]);
+      if constexpr (kMixed<R>) hrec[4 * i + 3] = mdiv(R(1), qp + pg);
+      else hrec[4 * i + 3] = G(w.logp)[gc];
     }
     __syncthreads();
   }
-  const int c = lblk * kBlock + threadIdx.x;
-  if (c >= g.ncv) return;
-  const R Rg = R(P.R_gas);
-  const R pg = kMixed<R> ? R(P.p_g) : R(0);
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.xperm, pos);
   const Light<R> self = load_light(st.light, c);
   const R rho = self.rho, sos = self.sos;
   const R p = kMixed<R> ? self.p + pg : self.p;  // absolute pressure
@@ -858,49 +892,68 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   const R symh = R(P.coeff_h) * R(G(g.dcmag)[c]);
 
   // ---- Gibbs-remainder sensor and convective CFL over the extended half-faces ----
-  const int lane = c & (kSlice - 1);
-  const int cbase = G(g.cf_ptr)[c >> 6] + lane;
-  const int xbase = G(g.xf_ptr)[c >> 6] + lane;
-  const int nc = G(g.ef_nc)[c];
-  const int nx = G(g.xf_deg)[c];
-  R lp_s = R(0);
-  if constexpr (!kMixed<R>) lp_s = G(w.logp)[c];
-  const R irt_s = R(1) / (Rg * T);
+  const int lane = pos & (kSlice - 1);
+  const int cbase = G(g.cf_ptr)[pos >> 6] + lane;
+  const int xbase = G(g.xf_ptr)[pos >> 6] + lane;
+  const int nc = G(g.ef_nc)[pos];
+  const int nx = G(g.xf_deg)[pos];
+  // own ln p (FP64) or 1/p (mixed), as staged for the halo
+  const R l4_s = kMixed<R> ? mdiv(R(1), self.p + pg) : R(G(w.logp)[c]);
+  const R irt_s = irt_of(self.p, rho);  // 1 / (R T), as for the halo
   R vv2 = R(0), cfl = R(0);
-  // Gibbs remainder of one pair from the neighbour's p, sp_vol, rho (and logp)
-  auto gibbs = [&](R op, R ospv, R orho, R olp, bool self_hi) {
-    const R irt_o = R(1) / (Rg * ((kMixed<R> ? op + pg : op) / (orho * Rg)));
+  // Gibbs remainder of one pair from the neighbour's p, sp_vol, 1/(R T) and
+  // ln p (FP64: ln p1 - ln p0) or 1/p (mixed: log1p((p1 - p0) / p0))
+  auto gibbs = [&](R op, R ospv, R irt_o, R o4, bool self_hi) {
     const R ap = self_hi ? op : self.p, bp = self_hi ? self.p : op;
     const R aspv = self_hi ? ospv : self.spv, bspv = self_hi ? self.spv : ospv;
     const R dp = bp - ap;
     const R v_avg = R(0.5) * (aspv + bspv);
     const R beta_avg = R(0.5) * ((self_hi ? irt_o : irt_s) + (self_hi ? irt_s : irt_o));
     R gr;
-    if constexpr (kMixed<R>) {
-      (void)olp;
-      gr = log1pf(dp / (ap + pg)) - v_avg * dp * beta_avg;
-    } else {
-      gr = (self_hi ? lp_s : olp) - (self_hi ? olp : lp_s) - v_avg * dp * beta_avg;
-    }
+    if constexpr (kMixed<R>) gr = mlog1p(dp * (self_hi ? o4 : l4_s)) - v_avg * dp * beta_avg;
+    else gr = (self_hi ? l4_s : o4) - (self_hi ? o4 : l4_s) - v_avg * dp * beta_avg;
     vv2 = smax(vv2, mabs(gr));
   };
-  // compact pairs: sensor and CFL (pair order)
-  for (int k = 0; k < nc; ++k) {
-    const int slot = cbase + k * kSlice;
-    const int code = ldnt(g.cf_nbr + slot);
-    const int nb = code & 0x7fffffff;
-    const bool self_hi = code < 0;
-    const Light<R> o = load_light(st.light, nb);
-    gibbs(o.p, o.spv, o.rho, kMixed<R> ? R(0) : R(G(w.logp)[nb]), self_hi);
-    const Light<R>& a = self_hi ? o : self;
-    const Light<R>& b = self_hi ? self : o;
-    const int f = ldnt(g.cf_pair + slot) & 0x7fffffff;
-    const R n[3] = {G(g.pf0)[f], G(g.pf1)[f], G(g.pf2)[f]};
-    const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    const R undA = R(0.5) * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
-                             (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
-    const R sos_avg = R(0.5) * (a.sos + b.sos);
-    cfl += mabs(undA) + area * sos_avg;
+  // compact pairs: sensor and CFL (the CFL sum in pair order). Slots are
+  // taken kQ at a time: their codes, then the neighbours' records and the face
+  // normals of all kQ, so the dependent gathers of different slots overlap.
+  constexpr int kQ = 2;
+  for (int k0 = 0; k0 < nc; k0 += kQ) {
+    int code[kQ];
+    R n[kQ][3];
+#pragma unroll
+    for (int j = 0; j < kQ; ++j)
+      if (k0 + j < nc) {
+        const int slot = cbase + (k0 + j) * kSlice;
+        code[j] = ldnt(g.cf_nbr + slot);
+        n[j][0] = ldnt(g.cfn0 + slot);
+        n[j][1] = ldnt(g.cfn1 + slot);
+        n[j][2] = ldnt(g.cfn2 + slot);
+      }
+    Light<R> o[kQ];
+    R l4[kQ];
+#pragma unroll
+    for (int j = 0; j < kQ; ++j)
+      if (k0 + j < nc) {
+        const int nb = code[j] & 0x7fffffff;
+        o[j] = load_light(st.light, nb);
+        if constexpr (!kMixed<R>) l4[j] = G(w.logp)[nb];
+      }
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) {
+      if (k0 + j >= nc) break;
+      const bool self_hi = code[j] < 0;
+      const Light<R>& oj = o[j];
+      gibbs(oj.p, oj.spv, irt_of(oj.p, oj.rho), kMixed<R> ? mdiv(R(1), oj.p + pg) : l4[j], self_hi);
+      const Light<R>& a = self_hi ? oj : self;
+      const Light<R>& b = self_hi ? self : oj;
+      const R* nj = n[j];
+      const R area = msqrt(nj[0] * nj[0] + nj[1] * nj[1] + nj[2] * nj[2]);
+      const R undA = R(0.5) * ((a.u[0] * nj[0] + a.u[1] * nj[1] + a.u[2] * nj[2]) +
+                               (b.u[0] * nj[0] + b.u[1] * nj[1] + b.u[2] * nj[2]));
+      const R sos_avg = R(0.5) * (a.sos + b.sos);
+      cfl += mabs(undA) + area * sos_avg;
+    }
   }
   // purely extended pairs: sensor only (a maximum: order-independent); their
   // codes are fetched kP at a time so the loads overlap
@@ -919,7 +972,8 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
       } else {
         const int nb = code[j] & 0x7fffffff;
         const auto q = G(st.light + (size_t)nb * L_N);
-        gibbs(q[L_P], q[L_SPV], q[L_RHO], kMixed<R> ? R(0) : R(G(w.logp)[nb]), code[j] < 0);
+        const R qp = q[L_P];
+        gibbs(qp, q[L_SPV], irt_of(qp, q[L_RHO]), kMixed<R> ? mdiv(R(1), qp + pg) : R(G(w.logp)[nb]), code[j] < 0);
       }
     }
   }
@@ -951,6 +1005,7 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
 // setDt (FlowSolver.hpp:2274-2328) and the step/time bookkeeping of
 // advanceSolution (time += dt, -= dt/2, += dt/2).
 __device__ __forceinline__ void begin_step(StepInfo* s, const Params& P) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   if (s->nan) return;
   const double cfl_max = __longlong_as_double((long long)s->cfl_bits);
   double dt;
@@ -982,10 +1037,11 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
-  const int c = lblk * kBlock + threadIdx.x;
-  if (c >= g.ncv) return;
-  const int cbase = G(g.cf_ptr)[c >> 6] + (c & (kSlice - 1));
-  const int nc = G(g.ef_nc)[c];
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.xperm, pos);
+  const int cbase = G(g.cf_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int nc = G(g.ef_nc)[pos];
   const Light<R> self = load_light(st.light, c);
   R cfl = R(0);
   for (int k = 0; k < nc; ++k) {
@@ -996,8 +1052,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
     const Light<R> o = load_light(st.light, nb);
     const Light<R>& a = self_hi ? o : self;
     const Light<R>& b = self_hi ? self : o;
-    const int f = ldnt(g.cf_pair + cslot) & 0x7fffffff;
-    const R n[3] = {G(g.pf0)[f], G(g.pf1)[f], G(g.pf2)[f]};
+    const R n[3] = {ldnt(g.cfn0 + cslot), ldnt(g.cfn1 + cslot), ldnt(g.cfn2 + cslot)};
     const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const R undA = R(0.5) * ((a.u[0] * n[0] + a.u[1] * n[1] + a.u[2] * n[2]) +
                              (b.u[0] * n[0] + b.u[1] * n[1] + b.u[2] * n[2]));
@@ -1014,6 +1069,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                          int cur) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   int lblk;
   const Inst<R>& X = inst_of(I, bmap, lblk);
@@ -1067,6 +1123,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
 // one workgroup per instance: reduce its block partials in block order
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict__ I) {
+  #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   const Inst<R>& X = I[blockIdx.x];
   const auto partial = G(X.w.partial);
@@ -1095,7 +1152,7 @@ __global__ void k_fill(T* p, T v, int64_t n) {
 }
 
 inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
-inline int check() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+inline int check() { return (int)hipGetLastError(); }
 
 }  // namespace
 
@@ -1190,6 +1247,7 @@ int launch_diag(const Launch<R>& L, int cur, void* stream) {
   return check();
 }
 
+#ifdef CGK_COMMON
 int launch_fill(double* p, double v, int64_t n, void* stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, v, n);
@@ -1201,6 +1259,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, v, n);
   return check();
 }
+#endif
 
 #define CGK_INSTANTIATE(R)                                                                         \
   template int launch_flux<R>(int, int, const Launch<R>&, int, double*, void*);                   \
@@ -1213,8 +1272,5 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
   template int launch_cfl<R>(const Launch<R>&, int, double, void*);                               \
   template int launch_diag<R>(const Launch<R>&, int, void*);
-
-CGK_INSTANTIATE(double)
-CGK_INSTANTIATE(float)
 
 }  // namespace cgk
