@@ -76,17 +76,53 @@ def algorithmic_bytes(Ge, Gc, Gg, mixed=False):
     return step, compact, ext
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """HBM bytes/s (read + write) of a plain device-to-device copy of nbytes
+    (hipMemcpy through the HIP runtime, timed with HIP events): the practical
+    bandwidth the flux kernels are also compared with."""
+    import ctypes as C
+
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    vp = C.c_void_p
+    src, dst = vp(), vp()
+    e0, e1 = vp(), vp()
+    ms = C.c_float()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    ok(hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)), "hipMalloc")
+        ok(hip.hipMemset(src, 1, C.c_size_t(nbytes)), "hipMemset")
+        ok(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        kind = 3  # hipMemcpyDeviceToDevice
+        for _ in range(3):
+            ok(hip.hipMemcpy(dst, src, C.c_size_t(nbytes), kind), "hipMemcpy")
+        ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), kind, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ok(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        if dst:
+            hip.hipFree(dst)
+        hip.hipFree(src)
+    return 2.0 * nbytes / (ms.value * 1e-3 / reps)
+
+
 def copy_calibration(nbytes):
     """Plain device copy of known size, for scaling rocprof FETCH_SIZE/WRITE_SIZE."""
-    import torch
-
-    n = nbytes // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_each_way": n * 4}))
+    bw = copy_bandwidth(nbytes, reps=3)
+    print(json.dumps({"copy_bytes_each_way": nbytes, "copy_GBps": bw / 1e9}))
 
 
 def k4_params(Mt=0.3, Re=1600.0, gamma=1.4, sgs="VREMAN", cfl=0.8):
@@ -373,6 +409,8 @@ def main():
     dom = max(r["kr"], key=lambda k: r["kr"][k]["ms_per_step"])
     kd = r["kr"][dom]
     traffic = recorded_traffic(dom, main_prec, args.n, args.ic)
+    copy_gbs = copy_bandwidth() / 1e9
+    log(f"device copy: {copy_gbs:.0f} GB/s")
     out = {
         "metric": METRIC,
         "value": r["value"],
@@ -403,6 +441,8 @@ def main():
             "kernel": kd["kernel"],
             "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
             "avg_launch_ms": kd["avg_launch_ms"],
+            "measured_GBps": traffic / (kd["avg_launch_ms"] * 1e-3) / 1e9 if traffic else None,
+            "copy_GBps": copy_gbs,
         },
         "kernel_rooflines": r["kr"],
         "step_roofline": {"achieved": r["b_step"] * r["value"] / 1e9, "frac": r["b_step"] * r["value"] / 1e9 / HBM_PEAK_GBS,

@@ -453,18 +453,23 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   const R p_g = kMixed<R> ? R(X.P.p_g) : R(0);
   const int f = lblk * kBlock + threadIdx.x;
   if (f >= g.nfa) return;
-  if (G(w.info)->nan) return;
+  // The pair's streams (its cells and all its geometry) are requested
+  // together, ahead of the record gathers that depend on the cells, so the
+  // geometry rides the first memory round trip instead of the second.
   const int lo = ldnt(g.plo + f), hi = ldnt(g.phi + f);
+  const R n[3] = {ldnt(g.pn0 + f), ldnt(g.pn1 + f), ldnt(g.pn2 + f)};
+  const R cv[3] = {ldnt(g.pc0 + f), ldnt(g.pc1 + f), ldnt(g.pc2 + f)};
+  const R un[3] = {ldnt(g.pu0 + f), ldnt(g.pu1 + f), ldnt(g.pu2 + f)};
+  const R area = ldnt(g.parea + f), aod = ldnt(g.paod + f);
+  __builtin_amdgcn_sched_barrier(0);
+  if (G(w.info)->nan) return;
   const Light<R> a = load_light(in.light, lo);
   const Light<R> b = load_light(in.light, hi);
   const Aux<R> A0 = load_aux(w.aux, lo);
   const Aux<R> A1 = load_aux(w.aux, hi);
-  const R n[3] = {ldnt(g.pn0 + f), ldnt(g.pn1 + f), ldnt(g.pn2 + f)};
-  const R cv[3] = {ldnt(g.pc0 + f), ldnt(g.pc1 + f), ldnt(g.pc2 + f)};
-  const R un[3] = {ldnt(g.pu0 + f), ldnt(g.pu1 + f), ldnt(g.pu2 + f)};
   R F[5], u_avg[3], inv_v_avg, h_stag;
   flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
-  flux_compact(F, un, ldnt(g.parea + f), ldnt(g.paod + f), a, b, A0, A1, u_avg, h_stag, p_g);
+  flux_compact(F, un, area, aod, a, b, A0, A1, u_avg, h_stag, p_g);
   if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
     R cF[5];
     flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
@@ -685,20 +690,26 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
   R d[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) d[k] = R(0);
-  // entries are fetched 4 at a time (codes and coefficients, then the four
-  // neighbour gathers) so their loads overlap; the sums keep the order k = 0, 1, ...
+  // entries are taken 4 at a time: the batch's neighbour gathers are issued,
+  // then the next batch's codes and coefficients (a stream independent of the
+  // gathers, so both share one memory round trip), then the batch is summed;
+  // the sums keep the order k = 0, 1, ...
   constexpr int kB = 4;
-  for (int k0 = 0; k0 < deg; k0 += kB) {
-    int nb[kB];
-    R gc[kB][3], un[kB][3];
+  int nb[kB];
+  R gc[kB][3];
+  auto fetch = [&](int k0, int (&nbv)[kB], R (&gcv)[kB][3]) {
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
       if (k0 + j < deg) {
         const int slot = base + (k0 + j) * kSlice;
-        nb[j] = ldnt(g.gr_nbr + slot);
-        gc[j][0] = ldnt(g.gc0 + slot); gc[j][1] = ldnt(g.gc1 + slot); gc[j][2] = ldnt(g.gc2 + slot);
+        nbv[j] = ldnt(g.gr_nbr + slot);
+        gcv[j][0] = ldnt(g.gc0 + slot); gcv[j][1] = ldnt(g.gc1 + slot); gcv[j][2] = ldnt(g.gc2 + slot);
       }
     }
+  };
+  fetch(0, nb, gc);
+  for (int k0 = 0; k0 < deg; k0 += kB) {
+    R un[kB][3];
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
       if (k0 + j < deg) {
@@ -706,6 +717,9 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
         un[j][0] = L[o + L_U0]; un[j][1] = L[o + L_U1]; un[j][2] = L[o + L_U2];
       }
     }
+    int nb_next[kB];
+    R gc_next[kB][3];
+    fetch(k0 + kB, nb_next, gc_next);
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
       if (k0 + j >= deg) break;
@@ -713,6 +727,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) d[i * 3 + jj] += gc[j][jj] * (un[j][i] - ui[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      nb[j] = nb_next[j];
+      gc[j][0] = gc_next[j][0]; gc[j][1] = gc_next[j][1]; gc[j][2] = gc_next[j][2];
     }
   }
   const auto a = G(w.aux + (size_t)c * A_N + A_D);
