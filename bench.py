@@ -248,6 +248,7 @@ def ensemble_bench(args):
            "spec": f"n ~ U{{16..48}} (jitter 0.3h), Re ~ U[400,3200], Ma ~ U[0.05,0.3], TGV or k^4 by coin flip, "
                    f"seed {args.seed}"}
     kes = {}
+    tune_steps = 2
     for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
         t0 = time.time()
         b = Batch(prec)
@@ -255,6 +256,12 @@ def ensemble_bench(args):
         setup = time.time() - t0
         cnt = b.counts()
         log(f"{name}: {cnt['instances']} instances on the device ({cnt['device_bytes'] / 1e9:.1f} GB) in {setup:.1f}s")
+        # the autotuner picks the launch variant (LDS or global gather, extended
+        # slots per batch) for this mix; every variant advances the solves by
+        # tune_steps, the same in both precisions
+        report = b.tune(tune_steps)
+        chosen = [ln for ln in report.splitlines() if ln.startswith("chosen")]
+        log(f"{name}: tuner {chosen[0] if chosen else report}")
         ms = b.time_steps(args.ens_steps)
         kes[name] = [b.diag(i) for i in ids]
         b.close()
@@ -262,10 +269,11 @@ def ensemble_bench(args):
         out[name] = {"seconds": sec, "solves_per_s": len(specs) / sec,
                      "cell_stage_per_s": 3.0 * cells * args.ens_steps / sec,
                      "latency_p50_s": sec, "latency_p99_s": sec, "setup_s": setup,
-                     "device_bytes": cnt["device_bytes"]}
+                     "device_bytes": cnt["device_bytes"], "tuner": report}
         log(f"{name}: {args.ens_steps} steps of all instances in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s")
     errs = [abs(a.ke - b.ke) / b.ke for a, b in zip(kes["mixed"], kes["fp64"])]
     out["mixed"]["ke_rel_err_vs_fp64_max"] = max(errs)
+    out["tune_steps_per_variant"] = tune_steps
     # one at a time, FP64 (every 16th solve, scaled to the whole set)
     sample = list(range(0, len(specs), max(1, len(specs) // 16)))
     t_alone = 0.0

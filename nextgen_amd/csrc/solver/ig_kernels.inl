@@ -453,15 +453,16 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   const R p_g = kMixed<R> ? R(X.P.p_g) : R(0);
   const int f = lblk * kBlock + threadIdx.x;
   if (f >= g.nfa) return;
-  // The pair's streams (its cells and all its geometry) are requested
+  // FP64: the pair's streams (its cells and all its geometry) are requested
   // together, ahead of the record gathers that depend on the cells, so the
-  // geometry rides the first memory round trip instead of the second.
+  // geometry rides the first memory round trip instead of the second (4%
+  // faster; in FP32 the compiler's own order is 4% faster).
   const int lo = ldnt(g.plo + f), hi = ldnt(g.phi + f);
   const R n[3] = {ldnt(g.pn0 + f), ldnt(g.pn1 + f), ldnt(g.pn2 + f)};
   const R cv[3] = {ldnt(g.pc0 + f), ldnt(g.pc1 + f), ldnt(g.pc2 + f)};
   const R un[3] = {ldnt(g.pu0 + f), ldnt(g.pu1 + f), ldnt(g.pu2 + f)};
   const R area = ldnt(g.parea + f), aod = ldnt(g.paod + f);
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (!kMixed<R>) __builtin_amdgcn_sched_barrier(0);
   if (G(w.info)->nan) return;
   const Light<R> a = load_light(in.light, lo);
   const Light<R> b = load_light(in.light, hi);
@@ -690,11 +691,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
   R d[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) d[k] = R(0);
-  // entries are taken 4 at a time: the batch's neighbour gathers are issued,
-  // then the next batch's codes and coefficients (a stream independent of the
-  // gathers, so both share one memory round trip), then the batch is summed;
-  // the sums keep the order k = 0, 1, ...
-  constexpr int kB = 4;
+  // entries are taken 8 at a time (8% faster than 4): the batch's neighbour
+  // gathers are issued, then the next batch's codes and coefficients (a
+  // stream independent of the gathers, so both share one memory round trip),
+  // then the batch is summed; the sums keep the order k = 0, 1, ...
+  constexpr int kB = 8;
   int nb[kB];
   R gc[kB][3];
   auto fetch = [&](int k0, int (&nbv)[kB], R (&gcv)[kB][3]) {
