@@ -38,6 +38,7 @@ extern "C" {
 /* SGS_MODEL (IdealGasSolver.hpp:772-790) */
 #define CG_SGS_NONE 0
 #define CG_SGS_VREMAN 1
+#define CG_SGS_SIGMA 2
 
 /* NO_DISS / ALL_DISS overrides of the Gibbs-remainder sensor (IdealGasSolver.cpp:973-979) */
 #define CG_DISS_SENSOR 0
@@ -76,7 +77,7 @@ typedef struct cg_params {
   double mu_ref;           /* MU_REF */
   double mu_power_law;     /* MU_POWER_LAW (0.75) */
   double Pr_lam;           /* PR_LAM (0.7) */
-  int sgs_model;           /* SGS_MODEL NONE|VREMAN */
+  int sgs_model;           /* SGS_MODEL NONE|VREMAN|SIGMA */
   double vreman_coeff;     /* VREMAN_COEFF (0.07) */
   double a_sgs_coeff;      /* A_SGS_COEFF (0.05) */
   double a_sgs_coeff2;     /* A_SGS_COEFF2 (0.0015) */
@@ -96,6 +97,7 @@ typedef struct cg_params {
   int disable_grouping;    /* DISABLE_FA_GROUPING */
   int zero_c_ef;           /* ZERO_C_EF */
   int precision;           /* PRECISION: CG_PREC_FP64 (default) or CG_PREC_MIXED */
+  double sigma_coeff;      /* SIGMA_COEFF (1.5), IdealGasSolverSgs.cpp:300 */
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,

@@ -140,7 +140,7 @@ struct Instance : HostInst {
   std::shared_ptr<Layout<R>> lay;    // its mesh's device layout
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   cgk::Launch<R> solo_launch() const {
-    return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4};
+    return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -295,6 +295,7 @@ void fill_dev_params(HostInst* h) {
   P.mu_power_law = p.mu_power_law;
   P.invPr_lam = 1.0 / p.Pr_lam;
   P.vreman_coeff = p.vreman_coeff;
+  P.sigma_coeff = p.sigma_coeff;
   P.a_sgs_coeff = p.a_sgs_coeff;
   P.a_sgs_coeff2 = p.a_sgs_coeff2;
   P.lambda_dil_coeff = p.lambda_dil_coeff;
@@ -319,7 +320,8 @@ int check_params(const cg_params* p) {
   if (!(p->p_ref > 0 && p->rho_ref > 0 && p->T_ref > 0)) return fail(CG_EINVAL, "P_REF, RHO_REF and T_REF must be positive");
   if (!(p->gamma > 1.0)) return fail(CG_EINVAL, "GAMMA must exceed 1");
   if (!(p->Pr_lam > 0 && p->Pr_t > 0)) return fail(CG_EINVAL, "PR_LAM and PR_T must be positive");
-  if (p->sgs_model != CG_SGS_NONE && p->sgs_model != CG_SGS_VREMAN) return fail(CG_EINVAL, "unsupported SGS_MODEL");
+  if (p->sgs_model != CG_SGS_NONE && p->sgs_model != CG_SGS_VREMAN && p->sgs_model != CG_SGS_SIGMA)
+    return fail(CG_EINVAL, "unsupported SGS_MODEL");
   if (p->dt_mode < 0 || p->dt_mode > 3) return fail(CG_EINVAL, "bad TIMESTEP mode");
   if (p->precision != CG_PREC_FP64 && p->precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   return 0;
@@ -808,7 +810,9 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile};
+  bool any_sigma = false;
+  for (auto* I : E.inst) any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
+  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma};
   E.dirty = false;
   return 0;
 }
@@ -1139,6 +1143,7 @@ int cg_params_default(cg_params* p) {
   p->Pr_lam = 0.7;
   p->sgs_model = CG_SGS_NONE;
   p->vreman_coeff = 0.07;
+  p->sigma_coeff = 1.5;
   p->a_sgs_coeff = 0.05;
   p->a_sgs_coeff2 = 0.0015;
   p->lambda_dil_coeff = 0.0;
@@ -1190,7 +1195,7 @@ int cg_params_parse(cg_params* p, const char* text) {
     struct { const char* k; double* d; } dkeys[] = {
         {"GAMMA", &p->gamma}, {"P_REF", &p->p_ref}, {"RHO_REF", &p->rho_ref}, {"T_REF", &p->T_ref},
         {"MU_REF", &p->mu_ref}, {"MU_POWER_LAW", &p->mu_power_law}, {"PR_LAM", &p->Pr_lam},
-        {"VREMAN_COEFF", &p->vreman_coeff}, {"A_SGS_COEFF", &p->a_sgs_coeff}, {"A_SGS_COEFF2", &p->a_sgs_coeff2},
+        {"VREMAN_COEFF", &p->vreman_coeff}, {"SIGMA_COEFF", &p->sigma_coeff}, {"A_SGS_COEFF", &p->a_sgs_coeff}, {"A_SGS_COEFF2", &p->a_sgs_coeff2},
         {"LAMBDA_DIL_COEFF", &p->lambda_dil_coeff}, {"PR_T", &p->Pr_t}, {"COEFF_P", &p->coeff_p},
         {"COEFF_H", &p->coeff_h}, {"GR_NRM", &p->gr_nrm}, {"EF_OPS.INTERP_FACTOR", &p->interp_factor},
         {"SIGMA_THRESH", &p->sigma_thresh}};
@@ -1208,7 +1213,8 @@ int cg_params_parse(cg_params* p, const char* text) {
       const std::string m = upper(tok[1]);
       if (m == "NONE") p->sgs_model = CG_SGS_NONE;
       else if (m == "VREMAN") p->sgs_model = CG_SGS_VREMAN;
-      else return bad("unsupported SGS_MODEL " + m + " (NONE and VREMAN are available)");
+      else if (m == "SIGMA") p->sgs_model = CG_SGS_SIGMA;
+      else return bad("unsupported SGS_MODEL " + m + " (NONE, VREMAN and SIGMA are available)");
     } else if (key == "TIMESTEP") {
       if (tok.size() < 3) return bad("expected TIMESTEP <mode> <value> [value]");
       const std::string m = upper(tok[1]);

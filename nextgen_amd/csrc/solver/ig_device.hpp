@@ -124,7 +124,7 @@ struct WorkT {
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;
-  double vreman_coeff, a_sgs_coeff, a_sgs_coeff2, lambda_dil_coeff, invPr_t, invgm1;
+  double vreman_coeff, a_sgs_coeff, a_sgs_coeff2, lambda_dil_coeff, invPr_t, invgm1, sigma_coeff;
   double coeff_p, coeff_h, gr_nrm;
   int sgs_model, diss_mode, limit_asgs, dt_mode;
   double dt_data0, dt_data1;
@@ -170,6 +170,7 @@ struct Launch {
   const int* pbmap;  // compact-pair blocks
   int npblk;
   int tile;          // extended slots fetched per batch in the LDS pass (2, 4 or 8)
+  int sigma;         // some instance uses SGS_MODEL SIGMA (its SGS pass runs first)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

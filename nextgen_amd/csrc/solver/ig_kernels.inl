@@ -483,6 +483,57 @@ __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I,
   o[5] = R(0);
 }
 
+// Singular values of a 3x3 matrix A (column-major), FP64: calcSvd33
+// (SvdNew.hpp:24-124), one-sided Jacobi rotations of the column pairs of
+// U = A until every pair is orthogonal to 1e-12 relative (at most 101
+// sweeps), then the column norms. The right factor V is not formed: it never
+// feeds back into U, so the singular values are those of the full routine.
+__device__ __forceinline__ void svd33_sigma(const double A[9], double sg[3]) {
+#pragma clang fp contract(off)  // FP64: rounded as the reference rounds
+  const double eps_tol = 1.0e-12;
+  double U[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) U[i] = A[i];
+  int done = 0, iter = 0;
+  while (done == 0) {
+    done = 1;
+    ++iter;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+      for (int i = 0; i < j; ++i) {
+        double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          a += U[i * 3 + k] * U[i * 3 + k];
+          b += U[j * 3 + k] * U[j * 3 + k];
+          c += U[i * 3 + k] * U[j * 3 + k];
+        }
+        double zeta = 1.0e+20;
+        if (c != 0.0) zeta = (b - a) / (2.0 * c);
+        const double t = ((zeta > 0.0) ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t);
+        const double sn = cs * t;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double tmp = U[i * 3 + k];
+          U[i * 3 + k] = cs * tmp - sn * U[j * 3 + k];
+          U[j * 3 + k] = sn * tmp + cs * U[j * 3 + k];
+        }
+        if ((fabs(c) > eps_tol * sqrt(a * b)) && (fabs(c) > eps_tol * eps_tol)) done = 0;
+      }
+    }
+    if (iter > 100) break;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s2 += U[j * 3 + i] * U[j * 3 + i];
+    sg[j] = sqrt(s2);
+  }
+}
+
 // The cell at lane position pos of a SELL region with lane order perm.
 __device__ __forceinline__ int cell_at(const uint8_t* perm, int pos) {
   return (pos & ~(kSlice - 1)) | int(G(perm)[pos]);
@@ -811,6 +862,68 @@ __device__ __forceinline__ R fast_pow_posx(R x, R p) {
 // sensor reads its extended pairs from there; compact pairs (sensor and CFL)
 // gather the full neighbour record from memory. The values are the same bits
 // either way, and the sensor is a maximum, so both variants agree exactly.
+// computeSgsSigma (IdealGasSolverSgs.cpp:298-360) with limitAsgs, for the
+// instances with SGS_MODEL SIGMA: mu_sgs, a_sgs and loc_sgs of every cell into
+// the FP64 work arrays, read by the properties pass that follows. FP64 in
+// either precision. A pass of its own, so the SVD's registers do not weigh on
+// the properties pass of the other models.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_sgs_sigma(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                      int cur) {
+#pragma clang fp contract(off)  // FP64: rounded as the reference rounds
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Params& P = X.P;
+  if (P.sgs_model != 2) return;
+  const GeomT<R> g = X.g;
+  const StateT<R> st = X.st[cur];
+  const WorkT<R> w = X.w;
+  const int c = lblk * kBlock + threadIdx.x;
+  if (c >= g.ncv) return;
+  const Light<R> self = load_light(st.light, c);
+  const auto auxc = G(w.aux + (size_t)c * A_N);
+  // D_sigma = s3 (s1 - s2) (s2 - s3) / s1^2 of the singular values of dudx,
+  // sorted descending (calcSvd33 takes dudx column-major)
+  double A[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[j * 3 + i] = double(auxc[A_D + i * 3 + j]);
+  double sg[3];
+  svd33_sigma(A, sg);
+  double t;
+  if (sg[0] < sg[1]) { t = sg[0]; sg[0] = sg[1]; sg[1] = t; }
+  if (sg[1] < sg[2]) { t = sg[1]; sg[1] = sg[2]; sg[2] = t; }
+  if (sg[0] < sg[1]) { t = sg[0]; sg[0] = sg[1]; sg[1] = t; }
+  double D_sigma = 0.0;
+  if (sg[0] > 1.0e-12) D_sigma = sg[2] * (sg[0] - sg[1]) * (sg[1] - sg[2]) / (sg[0] * sg[0]);
+  const double dx2 = G(g.dx23)[c];
+  const double rho = double(self.rho), sos = double(self.sos);
+  const double mu = P.sigma_coeff * P.sigma_coeff * dx2 * rho * D_sigma;
+  const double inv_sos2 = 1.0 / (sos * sos);
+  double a = P.a_sgs_coeff * D_sigma;
+  const double dil_ = A[0] + A[4] + A[8];
+  const double tmp_ = fabs(dil_) * sqrt(dx2) / sos;
+  const double u0 = double(self.u[0]), u1 = double(self.u[1]), u2 = double(self.u[2]);
+  if (tmp_ > 0.01) {
+    const double Masq = (u0 * u0 + u1 * u1 + u2 * u2) * inv_sos2;
+    a += P.a_sgs_coeff2 * G(g.sa_to_vol)[c] * fmin(Masq, 10.0) * sos;
+  }
+  a *= inv_sos2;
+  const double dt = G(w.info)->dt;
+  if (P.limit_asgs && dt > 0.0) {
+    // limitAsgs (IdealGasSolverSgs.cpp:4-48)
+    const double cfl_visc = 1.0;
+    const double usq = (u0 * u0 + u1 * u1 + u2 * u2);
+    const double fax = P.invgm1 + 0.5 * usq * inv_sos2;
+    const double ceil_ = cfl_visc / (dt * fax * 4.0) * inv_sos2;
+    if (a > ceil_) a = ceil_;
+  }
+  G(w.mu_sgs)[c] = mu;
+  G(w.a_sgs)[c] = a;
+  G(w.loc_sgs)[c] = mu * P.invPr_t;
+}
+
 template <class R, int FIELDS, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                   int cur) {
@@ -901,6 +1014,11 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
       if (a_sgs > ceil_) a_sgs = ceil_;
     }
     loc_sgs += mu_sgs * R(P.invPr_t);
+  } else if (P.sgs_model == 2) {
+    // Sigma: the coefficients of k_sgs_sigma, which ran just before
+    mu_sgs = R(G(w.mu_sgs)[c]);
+    a_sgs = R(G(w.a_sgs)[c]);
+    loc_sgs = R(G(w.loc_sgs)[c]);
   }
 
   // ---- material properties ----
@@ -1236,6 +1354,7 @@ int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, vo
 template <class R>
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
   hipStream_t s = (hipStream_t)stream;
+  if (L.sigma) hipLaunchKernelGGL(k_sgs_sigma<R>, dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
   // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS
   const size_t lds = L.lds / 6 * 4;
   if (lds) {

@@ -23,7 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # CHARLES_GPU_LIB selects another build of the same library (e.g. a tuning variant)
 LIB_PATH = os.environ.get("CHARLES_GPU_LIB", os.path.join(_HERE, "lib", "libcharles_gpu.so"))
 
-CG_SGS_NONE, CG_SGS_VREMAN = 0, 1
+CG_SGS_NONE, CG_SGS_VREMAN, CG_SGS_SIGMA = 0, 1, 2
 CG_DISS_SENSOR, CG_DISS_NONE, CG_DISS_ALL = 0, 1, 2
 CG_DT_CONSTANT, CG_DT_CFL, CG_DT_CFL_DT, CG_DT_CFL_T = 0, 1, 2, 3
 
@@ -42,6 +42,7 @@ class cg_params(C.Structure):
         ("diss_mode", C.c_int), ("dt_mode", C.c_int), ("dt_data", C.c_double * 2), ("check_interval", C.c_int),
         ("interp_factor", C.c_double), ("sigma_thresh", C.c_double), ("group_member_threshold", C.c_int),
         ("disable_grouping", C.c_int), ("zero_c_ef", C.c_int), ("precision", C.c_int),
+        ("sigma_coeff", C.c_double),
     ]
 
 

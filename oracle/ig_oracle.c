@@ -356,6 +356,97 @@ static void sgs_vreman(og_solver* s) {
   for (int icv = 0; icv < s->m.ncv; ++icv) s->loc_sgs[icv] += s->mu_sgs[icv] * invPr_t;
 }
 
+/* calcSvd33, SvdNew.hpp:24-124: one-sided Jacobi on the columns of U = A
+ * (column-major) until every column pair is orthogonal to 1e-12 (at most 101
+ * sweeps); sigma = the column norms, U's columns normalised. SIGN(0) = -1 and
+ * a zero inner product takes zeta = 1e20, as there. */
+int og_svd33(const double A[9], double U[9], double V[9], double sigma[3]) {
+  const double eps_tol = 1.0e-12;
+  for (int i = 0; i < 9; ++i) U[i] = A[i];
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) V[j * 3 + i] = (i == j) ? 1.0 : 0.0;
+  int done = 0, iter = 0;
+  while (done == 0) {
+    done = 1;
+    ++iter;
+    for (int j = 0; j < 3; ++j) {
+      for (int i = 0; i < j; ++i) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          a += U[i * 3 + k] * U[i * 3 + k];
+          b += U[j * 3 + k] * U[j * 3 + k];
+          c += U[i * 3 + k] * U[j * 3 + k];
+        }
+        double zeta = 1.0e+20;
+        if (c != 0.0) zeta = (b - a) / (2.0 * c);
+        const double t = ((zeta > 0.0) ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t);
+        const double sn = cs * t;
+        for (int k = 0; k < 3; ++k) {
+          const double tmp = U[i * 3 + k];
+          U[i * 3 + k] = cs * tmp - sn * U[j * 3 + k];
+          U[j * 3 + k] = sn * tmp + cs * U[j * 3 + k];
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double tmp = V[i * 3 + k];
+          V[i * 3 + k] = cs * tmp - sn * V[j * 3 + k];
+          V[j * 3 + k] = sn * tmp + cs * V[j * 3 + k];
+        }
+        if ((fabs(c) > eps_tol * sqrt(a * b)) && (fabs(c) > eps_tol * eps_tol)) done = 0;
+      }
+    }
+    if (iter > 100) break;
+  }
+  for (int j = 0; j < 3; ++j) {
+    sigma[j] = 0.0;
+    for (int i = 0; i < 3; ++i) sigma[j] += U[j * 3 + i] * U[j * 3 + i];
+    sigma[j] = sqrt(sigma[j]);
+    if (sigma[j] > 0.0)
+      for (int i = 0; i < 3; ++i) U[j * 3 + i] /= sigma[j];
+  }
+  return iter;
+}
+
+/* the Sigma model's differential operator, IdealGasSolverSgs.cpp:309-339: the
+ * singular values of dudx sorted descending, s3 (s1 - s2) (s2 - s3) / s1^2 */
+double og_sigma_dsigma(const double dudx[9]) {
+  double A[9], U[9], V[9], sg[3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[j * 3 + i] = dudx[i * 3 + j];
+  og_svd33(A, U, V, sg);
+  double t;
+  if (sg[0] < sg[1]) { t = sg[0]; sg[0] = sg[1]; sg[1] = t; }
+  if (sg[1] < sg[2]) { t = sg[1]; sg[1] = sg[2]; sg[2] = t; }
+  if (sg[0] < sg[1]) { t = sg[0]; sg[0] = sg[1]; sg[1] = t; }
+  double D = 0.0;
+  if (sg[0] > 1.0e-12) D = sg[2] * (sg[0] - sg[1]) * (sg[1] - sg[2]) / (sg[0] * sg[0]);
+  return D;
+}
+
+/* IdealGasSolverSgs.cpp:298-360 */
+static void sgs_sigma(og_solver* s) {
+  const og_params* P = &s->p;
+  const double invPr_t = 1.0 / P->Pr_t;
+  for (int icv = 0; icv < s->m.ncv; ++icv) {
+    const double* d = &s->dudx[9 * (size_t)icv];
+    const double dx2 = pow(s->m.vol[icv], 2.0 / 3.0);
+    const double D_sigma = og_sigma_dsigma(d);
+    s->mu_sgs[icv] = P->sigma_coeff * P->sigma_coeff * dx2 * s->rho[icv] * D_sigma;
+    const double inv_sos2 = 1.0 / (s->sos[icv] * s->sos[icv]);
+    s->a_sgs[icv] = P->a_sgs_coeff * D_sigma;
+    const double dil_ = d[0] + d[4] + d[8];
+    const double tmp_ = fabs(dil_) * sqrt(dx2) / s->sos[icv];
+    if (tmp_ > 0.01) {
+      const double* u = &s->u[3 * (size_t)icv];
+      const double Masq = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * inv_sos2;
+      s->a_sgs[icv] += P->a_sgs_coeff2 * s->m.sa_to_vol[icv] * fmin(Masq, 10.0) * s->sos[icv];
+    }
+    s->a_sgs[icv] *= inv_sos2;
+  }
+  limit_asgs(s);
+  for (int icv = 0; icv < s->m.ncv; ++icv) s->loc_sgs[icv] = s->mu_sgs[icv] * invPr_t;
+}
+
 /* IdealGasSolverSgs.cpp:76-82 */
 static void sgs_none(og_solver* s) {
   for (int icv = 0; icv < s->m.ncv; ++icv) {
@@ -396,6 +487,7 @@ static void set_fgr(og_solver* s) {
 void og_sgs_props(og_solver* s) {
   const og_params* P = &s->p;
   if (P->sgs_model == 1) sgs_vreman(s);
+  else if (P->sgs_model == 2) sgs_sigma(s);
   else sgs_none(s);
   const double coeff_p_delta = P->coeff_p;
   const double coeff_h_delta = P->coeff_h;

@@ -15,10 +15,11 @@
  *   og_update_state    updateConservativeAndPrimitiveData / updateExtendedState
  *                                                     IdealGasSolver.hpp:732-759, IdealGasSolver.cpp:986-1026
  *   og_calc_grad       _calcCvGradRange               StaticSolver.cpp:5535-5557
- *   og_sgs_props       calcSgsAndMaterialProperties, computeSgsNone/Vreman, limitAsgs,
+ *   og_sgs_props       calcSgsAndMaterialProperties, computeSgsNone/Vreman/Sigma, limitAsgs,
  *                      setFgrCvs, gr_func, fast_pow_posx
  *                                                     IdealGasSolver.cpp:182-255,736-745,894-984,
- *                                                     IdealGasSolverSgs.cpp:4-82,207-296, MiscUtils.hpp:273-276
+ *                                                     IdealGasSolverSgs.cpp:4-82,207-360, MiscUtils.hpp:273-276
+ *   og_svd33           calcSvd33 (one-sided Jacobi)   SvdNew.hpp:24-124
  *   og_calc_cfl        calcCfl + setDt                IdealGasSolver.cpp:1048-1095, FlowSolver.hpp:2274-2328
  *   og_step            advanceSolution + checkForNans + report
  *                                                     IdealGasSolver.hpp:414-611,805-848
@@ -66,7 +67,7 @@ typedef struct og_mesh {
 
 typedef struct og_params {
   double gamma, p_ref, rho_ref, T_ref, mu_ref, mu_power_law, Pr_lam;
-  int sgs_model; /* 0 NONE, 1 VREMAN */
+  int sgs_model; /* 0 NONE, 1 VREMAN, 2 SIGMA */
   double vreman_coeff, a_sgs_coeff, a_sgs_coeff2, lambda_dil_coeff, Pr_t;
   int limit_asgs;
   double coeff_p, coeff_h, gr_nrm;
@@ -74,6 +75,7 @@ typedef struct og_params {
   int dt_mode;   /* 0 DT, 1 CFL, 2 CFL_DT, 3 CFL_T */
   double dt_data[2];
   int check_interval;
+  double sigma_coeff; /* SIGMA_COEFF */
 } og_params;
 
 typedef struct og_diag {
@@ -93,6 +95,11 @@ void og_flux_compact(og_rhs* flux, const og_fc* fa, const og_state* cv0, const o
                      const og_aux* a0, const og_aux* a1, const double u_avg[3], double inv_v_avg,
                      double h_stag, const double g0[9], const double g1[9]);
 double og_fast_pow_posx(double x, double p);
+/* calcSvd33 on a column-major A: U, V column-major, sigma unsorted; returns the
+ * iteration count */
+int og_svd33(const double A[9], double U[9], double V[9], double sigma[3]);
+/* the Sigma model's D_sigma of one cell's dudx[i][j] (row-major [9]) */
+double og_sigma_dsigma(const double dudx[9]);
 double og_gr_func(double x, double nrm);
 extern const double og_erk_wgt3[3][3];
 

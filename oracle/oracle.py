@@ -31,7 +31,7 @@ class og_params(C.Structure):
         ("vreman_coeff", C.c_double), ("a_sgs_coeff", C.c_double), ("a_sgs_coeff2", C.c_double),
         ("lambda_dil_coeff", C.c_double), ("Pr_t", C.c_double), ("limit_asgs", C.c_int), ("coeff_p", C.c_double),
         ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
-        ("dt_data", C.c_double * 2), ("check_interval", C.c_int),
+        ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
     ]
 
 
@@ -73,6 +73,10 @@ def lib():
         L.og_flux_symmetric.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.og_flux_compact.argtypes = [dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp]
         L.og_fast_pow_posx.restype = C.c_double
+        L.og_svd33.restype = C.c_int
+        L.og_svd33.argtypes = [dp, dp, dp, dp]
+        L.og_sigma_dsigma.restype = C.c_double
+        L.og_sigma_dsigma.argtypes = [dp]
         L.og_fast_pow_posx.argtypes = [C.c_double, C.c_double]
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]

@@ -34,7 +34,7 @@ def test_param_defaults_are_the_reference_defaults():
     assert p.Pr_t == 0.9 and p.lambda_dil_coeff == 0.0
     assert p.coeff_p == 0.5 and p.coeff_h == 0.1 and p.gr_nrm == 0.03
     assert p.interp_factor == 1.0 / 3.0 and p.sigma_thresh == 0.45 and p.group_member_threshold == 100
-    assert p.check_interval == 1 and p.sgs_model == ig.CG_SGS_NONE
+    assert p.check_interval == 1 and p.sgs_model == ig.CG_SGS_NONE and p.sigma_coeff == 1.5
 
 
 def test_param_parsing():
@@ -64,7 +64,12 @@ def test_param_parsing():
     assert p.interp_factor == 0.273 and p.disable_grouping == 1
 
 
-@pytest.mark.parametrize("line", ["SGS_MODEL SIGMA", "EF_OPS ALIGNED", "USE_FULL_GR", "TIMESTEP FOO 1",
+def test_sigma_model_parsing():
+    p = ig.parse_params("SGS_MODEL SIGMA\nSIGMA_COEFF 1.35\nPR_T 0.85\n")
+    assert p.sgs_model == ig.CG_SGS_SIGMA and p.sigma_coeff == 1.35 and p.Pr_t == 0.85
+
+
+@pytest.mark.parametrize("line", ["SGS_MODEL DSM_LOCAL", "EF_OPS ALIGNED", "USE_FULL_GR", "TIMESTEP FOO 1",
                                   "GAMMA abc", "CHECK_INTERVAL 0"])
 def test_param_errors(line):
     with pytest.raises(ig.CharlesError) as e:

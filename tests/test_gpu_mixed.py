@@ -68,6 +68,23 @@ def test_mixed_stage_rhs_close_to_oracle(ic):
     assert np.all(err <= RHS_TOL), err
 
 
+def test_mixed_sigma_sgs_close_to_oracle():
+    # the Sigma model runs in FP64 in either precision: its eddy viscosity
+    # differs from the oracle's only through the FP32 velocity gradient
+    m = Mesh.jittered(12, 0.3, seed=11)
+    state = m.ic_k4(2.0, 0.3, 1.4, seed=5)
+    text = k4_params(sgs="SIGMA")
+    g = solver(m, parse_params(text + "PRECISION MIXED\n"), state)
+    o = oracle(m, state, text)
+    mu_g, mu_o = g.get_field("mu_sgs"), o.get_field("mu_sgs")
+    assert np.max(mu_o) > 0.0
+    assert np.max(np.abs(mu_g - mu_o)) <= 1e-4 * np.max(mu_o)
+    rg, ro = g.calc_rhs(1), o.calc_rhs()
+    err = np.abs(rg - ro).max(axis=0) / np.abs(ro).max(axis=0)
+    print(f"k4 SIGMA: mixed RHS error per component {err}")
+    assert np.all(err <= RHS_TOL), err
+
+
 def test_mixed_history_close_to_oracle():
     m, p, state = setup("tgv", "MIXED")
     g = solver(m, p, state)

@@ -43,7 +43,8 @@ def rel_err(a, b):
     return (np.abs(a - b).max(axis=0) / scale).max()
 
 
-@pytest.mark.parametrize("kind,sgs", [("lattice16", "NONE"), ("jittered12", "VREMAN"), ("lattice32", "NONE")])
+@pytest.mark.parametrize("kind,sgs", [("lattice16", "NONE"), ("jittered12", "VREMAN"), ("lattice32", "NONE"),
+                                      ("jittered12", "SIGMA")])
 def test_stage_rhs_matches_oracle(kind, sgs):
     m, p = make(kind, sgs)
     g, o = both(m, p)
@@ -60,7 +61,7 @@ def test_stage_rhs_matches_oracle(kind, sgs):
     assert err <= RHS_TOL
 
 
-@pytest.mark.parametrize("kind,sgs", [("lattice16", "NONE"), ("jittered12", "VREMAN")])
+@pytest.mark.parametrize("kind,sgs", [("lattice16", "NONE"), ("jittered12", "VREMAN"), ("jittered12", "SIGMA")])
 def test_history_matches_oracle(kind, sgs):
     m, p = make(kind, sgs)
     g, o = both(m, p)

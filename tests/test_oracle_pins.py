@@ -180,3 +180,26 @@ def test_reference_svd_regularised_inverse():
         ref_g = c + (Vm * sinv[None, :]) @ tmp
         ours = regularised_update(A, c[None, :], 0.45)[0]
         assert np.allclose(ours, ref_g, rtol=1e-12, atol=1e-14), (A, sig)
+
+
+@needs_ref
+def test_oracle_svd33_bitwise_equal_to_reference():
+    """og_svd33 (the Sigma model's SVD) restates calcSvd33 (SvdNew.hpp:24-124):
+    same sweeps, same convergence test, same rounding."""
+    rng = np.random.default_rng(13)
+    L, R = orc.lib(), orc.ref()
+    for trial in range(500):
+        A = rng.normal(size=(3, 3)) * 10.0 ** rng.uniform(-4, 4)
+        if trial % 5 == 1:
+            A[2] = 0.0  # a two-dimensional velocity gradient
+        if trial % 5 == 2:
+            A = A - A.T  # pure rotation
+        if trial % 5 == 3:
+            A = np.diag(rng.normal(size=3))
+        Acm = np.ascontiguousarray(A.T.reshape(9))
+        U, V, s = np.zeros(9), np.zeros(9), np.zeros(3)
+        Ur, Vr, sr, err = np.zeros(9), np.zeros(9), np.zeros(3), np.zeros(1)
+        it = L.og_svd33(dp(Acm), dp(U), dp(V), dp(s))
+        itr = R.ref_svd33(dp(Acm), dp(Ur), dp(Vr), dp(sr), dp(err))
+        assert it == itr
+        assert np.array_equal(s, sr) and np.array_equal(U, Ur) and np.array_equal(V, Vr), A
