@@ -98,6 +98,7 @@ typedef struct cg_params {
   int zero_c_ef;           /* ZERO_C_EF */
   int precision;           /* PRECISION: CG_PREC_FP64 (default) or CG_PREC_MIXED */
   double sigma_coeff;      /* SIGMA_COEFF (1.5), IdealGasSolverSgs.cpp:300 */
+  int use_full_gr;         /* USE_FULL_GR: the full Gibbs-remainder sensor, IdealGasSolver.cpp:244-247 */
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,

@@ -140,7 +140,8 @@ struct Instance : HostInst {
   std::shared_ptr<Layout<R>> lay;    // its mesh's device layout
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   cgk::Launch<R> solo_launch() const {
-    return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA};
+    return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
+            prm.use_full_gr != 0};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -309,6 +310,11 @@ void fill_dev_params(HostInst* h) {
   P.dt_mode = p.dt_mode;
   P.dt_data0 = p.dt_data[0];
   P.dt_data1 = p.dt_data[1];
+  // full Gibbs-remainder sensor (setFgrCvsFull, IdealGasSolver.cpp:755-761)
+  P.full_gr = p.use_full_gr;
+  P.cp = P.R_gas * p.gamma / (p.gamma - 1.0);
+  P.rho_ref = p.rho_ref;
+  P.inv_rhoref = 1.0 / p.rho_ref;
   // gauges of the mixed-precision records: the reference pressure and the
   // enthalpy at the reference temperature
   P.p_g = p.p_ref;
@@ -810,9 +816,13 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false;
-  for (auto* I : E.inst) any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
-  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma};
+  bool any_sigma = false, any_full_gr = false;
+  for (auto* I : E.inst) {
+    any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
+    any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
+  }
+  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma,
+         any_full_gr};
   E.dirty = false;
   return 0;
 }
@@ -1239,6 +1249,8 @@ int cg_params_parse(cg_params* p, const char* text) {
       p->diss_mode = CG_DISS_NONE;
     } else if (key == "ALL_DISS") {
       p->diss_mode = CG_DISS_ALL;
+    } else if (key == "USE_FULL_GR") {
+      p->use_full_gr = 1;
     } else if (key == "DISABLE_FA_GROUPING") {
       p->disable_grouping = 1;
     } else if (key == "ZERO_C_EF") {
@@ -1250,7 +1262,7 @@ int cg_params_parse(cg_params* p, const char* text) {
       else return bad("expected PRECISION FP64 or MIXED");
     } else if (key == "EF_OPS") {
       if (tok.size() < 2 || upper(tok[1]) != "DEFAULT") return bad("only EF_OPS DEFAULT is available");
-    } else if (key == "USE_FULL_GR" || key == "DISS_SPONGE" || key == "USE_COMPACT_FACES" || key == "DVOL_FORM" ||
+    } else if (key == "DISS_SPONGE" || key == "USE_COMPACT_FACES" || key == "DVOL_FORM" ||
                key == "ROTATING_REFERENCE_FRAME") {
       return bad("not available on this path");
     }

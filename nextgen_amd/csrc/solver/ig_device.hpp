@@ -129,6 +129,8 @@ struct Params {
   int sgs_model, diss_mode, limit_asgs, dt_mode;
   double dt_data0, dt_data1;
   double p_g, h_g;  // mixed-precision gauges (p_ref and h at T_ref)
+  double cp, rho_ref, inv_rhoref;  // full Gibbs-remainder sensor
+  int full_gr;                     // USE_FULL_GR
 };
 
 using Geom = GeomT<double>;
@@ -171,6 +173,7 @@ struct Launch {
   int npblk;
   int tile;          // extended slots fetched per batch in the LDS pass (2, 4 or 8)
   int sigma;         // some instance uses SGS_MODEL SIGMA (its SGS pass runs first)
+  int full_gr;       // some instance uses USE_FULL_GR (its sensor pass runs first)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

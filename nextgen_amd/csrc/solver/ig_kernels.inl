@@ -924,6 +924,103 @@ __global__ __launch_bounds__(kBlock) void k_sgs_sigma(const Inst<R>* __restrict_
   G(w.loc_sgs)[c] = mu * P.invPr_t;
 }
 
+// setFgrCvsFull (IdealGasSolver.cpp:747-838), for the instances with
+// USE_FULL_GR: vv2 = the maximum over all extended pairs of |gr| with
+//   gr = (-(s1 - s0) + beta_avg (cp (T1 - T0) - v_avg (p1 - p0))) / Ts_avg,
+// beta_avg, v_avg, Ts_avg the pair means of 1/(R T), 1/rho and the
+// stagnation temperature ratio Ts = 1 + (gamma-1)/2 |u|^2 / c^2, and the
+// entropy s = (log1p((p - p_ref)/p_ref) - gamma log1p((rho - rho_ref)/rho_ref))
+// / (gamma - 1), into the FP64 work array vv2 (the properties pass turns it
+// into fgr). The extended neighbours' six values are staged in LDS from the
+// block halo, the compact neighbours' are formed from their records.
+template <class R>
+struct FullGr {
+  R p, T, spv, irt, ts, ent;
+};
+
+template <class R>
+__device__ __forceinline__ FullGr<R> full_gr_of(const Params& P, const Light<R>& L) {
+  // FP64: the reference's expressions; mixed: p is stored as p - P_REF
+  const R Rg = R(P.R_gas);
+  const R pg = kMixed<R> ? R(P.p_g) : R(0);
+  FullGr<R> f;
+  f.p = L.p;
+  f.T = (kMixed<R> ? L.p + pg : L.p) / (L.rho * Rg);
+  f.spv = L.spv;
+  f.irt = R(1) / (Rg * f.T);
+  f.ts = R(1) + R(0.5) * R(P.gm1) * (L.u[0] * L.u[0] + L.u[1] * L.u[1] + L.u[2] * L.u[2]) / (L.sos * L.sos);
+  const R tmp1 = kMixed<R> ? R(log1pf(float(L.p) * float(P.inv_pref)))
+                           : R(log1p((double(L.p) - P.p_ref) * P.inv_pref));
+  const R tmp2 = kMixed<R> ? R(log1pf((float(L.rho) - float(P.rho_ref)) * float(P.inv_rhoref)))
+                           : R(log1p((double(L.rho) - P.rho_ref) * P.inv_rhoref));
+  f.ent = (tmp1 - R(P.gamma) * tmp2) * R(P.invgm1);
+  return f;
+}
+
+template <class R>
+__device__ __forceinline__ R full_gr_pair(const Params& P, const FullGr<R>& a, const FullGr<R>& b) {
+  // a: the pair's lo cell, b: its hi cell
+  const R Ts_avg = R(0.5) * (a.ts + b.ts);
+  const R dp = b.p - a.p;
+  const R dh = R(P.cp) * (b.T - a.T);
+  const R v_avg = R(0.5) * (a.spv + b.spv);
+  const R beta_avg = R(0.5) * (a.irt + b.irt);
+  const R dsor = b.ent - a.ent;
+  return (-dsor + beta_avg * (dh - v_avg * dp)) / Ts_avg;
+}
+
+template <class R, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_sensor_full(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                        int cur) {
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Params& P = X.P;
+  if (!P.full_gr) return;  // the whole block belongs to one instance
+  const GeomT<R> g = X.g;
+  const StateT<R> st = X.st[cur];
+  const WorkT<R> w = X.w;
+  extern __shared__ __align__(16) unsigned char smem[];
+  FullGr<R>* hrec = reinterpret_cast<FullGr<R>*>(smem);  // [hsize]
+  if constexpr (LDS) {
+    const int hb = G(g.halo_ptr)[lblk], hs = G(g.halo_ptr)[lblk + 1] - hb;
+    for (int i = threadIdx.x; i < hs; i += kBlock) hrec[i] = full_gr_of(P, load_light(st.light, G(g.halo_v)[hb + i]));
+    __syncthreads();
+  }
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.xperm, pos);
+  const FullGr<R> self = full_gr_of(P, load_light(st.light, c));
+  const int lane = pos & (kSlice - 1);
+  const int cbase = G(g.cf_ptr)[pos >> 6] + lane;
+  const int xbase = G(g.xf_ptr)[pos >> 6] + lane;
+  const int nc = G(g.ef_nc)[pos];
+  const int nx = G(g.xf_deg)[pos];
+  R vv2 = R(0);
+  for (int k = 0; k < nc; ++k) {
+    const int code = ldnt(g.cf_nbr + cbase + k * kSlice);
+    const FullGr<R> o = full_gr_of(P, load_light(st.light, code & 0x7fffffff));
+    const R gr = code < 0 ? full_gr_pair(P, o, self) : full_gr_pair(P, self, o);
+    vv2 = smax(vv2, mabs(gr));
+  }
+  for (int k = 0; k < nx; ++k) {
+    const int slot = xbase + k * kSlice;
+    FullGr<R> o;
+    bool self_hi;
+    if constexpr (LDS) {
+      const int code = int(ldnt(g.xf_loc + slot));
+      o = hrec[code & 0x7fff];
+      self_hi = (code & 0x8000) != 0;
+    } else {
+      const int code = ldnt(g.xf_nbr + slot);
+      o = full_gr_of(P, load_light(st.light, code & 0x7fffffff));
+      self_hi = code < 0;
+    }
+    const R gr = self_hi ? full_gr_pair(P, o, self) : full_gr_pair(P, self, o);
+    vv2 = smax(vv2, mabs(gr));
+  }
+  G(w.vv2)[c] = double(vv2);
+}
+
 template <class R, int FIELDS, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                   int cur) {
@@ -1082,7 +1179,7 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
       if (k0 + j >= nc) break;
       const bool self_hi = code[j] < 0;
       const Light<R>& oj = o[j];
-      gibbs(oj.p, oj.spv, irt_of(oj.p, oj.rho), kMixed<R> ? mdiv(R(1), oj.p + pg) : l4[j], self_hi);
+      if (!P.full_gr) gibbs(oj.p, oj.spv, irt_of(oj.p, oj.rho), kMixed<R> ? mdiv(R(1), oj.p + pg) : l4[j], self_hi);
       const Light<R>& a = self_hi ? oj : self;
       const Light<R>& b = self_hi ? self : oj;
       const R* nj = n[j];
@@ -1096,7 +1193,9 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   // purely extended pairs: sensor only (a maximum: order-independent); their
   // codes are fetched kP at a time so the loads overlap
   constexpr int kP = 8;
-  for (int k0 = 0; k0 < nx; k0 += kP) {
+  // (with USE_FULL_GR the sensor maximum comes from k_sensor_full instead)
+  if (P.full_gr) vv2 = R(G(w.vv2)[c]);
+  for (int k0 = 0; k0 < (P.full_gr ? 0 : nx); k0 += kP) {
     int code[kP];
 #pragma unroll
     for (int j = 0; j < kP; ++j)
@@ -1116,8 +1215,8 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
     }
   }
   R fgr;
-  if (P.diss_mode == 1) fgr = R(0);
-  else if (P.diss_mode == 2) fgr = R(0.5);
+  if (P.diss_mode == 1 && !P.full_gr) fgr = R(0);  // setFgrCvsFull has no NO_DISS / ALL_DISS
+  else if (P.diss_mode == 2 && !P.full_gr) fgr = R(0.5);
   else fgr = R(0.5) / (R(1) + mexp(R(-2) * (vv2 - R(P.gr_nrm)) / R(5.0e-03)));
   cfl *= R(0.5) * R(1) * R(G(g.inv_vol)[c]);
 
@@ -1355,6 +1454,11 @@ template <class R>
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   if (L.sigma) hipLaunchKernelGGL(k_sgs_sigma<R>, dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+  if (L.full_gr) {
+    // the flux pass's LDS (6 values of R per halo cell) holds FullGr<R>
+    if (L.lds) hipLaunchKernelGGL((k_sensor_full<R, true>), dim3(L.nblk), dim3(kBlock), L.lds, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_sensor_full<R, false>), dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+  }
   // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS
   const size_t lds = L.lds / 6 * 4;
   if (lds) {

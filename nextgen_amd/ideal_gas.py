@@ -42,7 +42,7 @@ class cg_params(C.Structure):
         ("diss_mode", C.c_int), ("dt_mode", C.c_int), ("dt_data", C.c_double * 2), ("check_interval", C.c_int),
         ("interp_factor", C.c_double), ("sigma_thresh", C.c_double), ("group_member_threshold", C.c_int),
         ("disable_grouping", C.c_int), ("zero_c_ef", C.c_int), ("precision", C.c_int),
-        ("sigma_coeff", C.c_double),
+        ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int),
     ]
 
 

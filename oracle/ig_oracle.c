@@ -483,6 +483,46 @@ static void set_fgr(og_solver* s) {
   free(logp);
 }
 
+/* setFgrCvsFull, IdealGasSolver.cpp:747-838: the Gibbs remainder with the
+ * entropy and the stagnation temperature, made dimensionless per pair with
+ * the mean stagnation temperature; no NO_DISS / ALL_DISS override there */
+static void set_fgr_full(og_solver* s) {
+  const og_mesh* m = &s->m;
+  const int n = m->ncv;
+  const double gr_nrm = s->p.gr_nrm;
+  const double gamma = s->p.gamma, R_gas = s->R_gas;
+  double* T_stag = (double*)malloc(sizeof(double) * n);
+  double* ent = (double*)malloc(sizeof(double) * n);
+  const double cp = R_gas * gamma / (gamma - 1.0);
+  const double gm1 = gamma - 1.0;
+  const double invgm1 = 1.0 / gm1;
+  const double inv_pref = 1.0 / s->p.p_ref;
+  const double inv_rhoref = 1.0 / s->p.rho_ref;
+  for (int icv = 0; icv < n; ++icv) {
+    const double* u = &s->u[3 * (size_t)icv];
+    s->vv2[icv] = 0.0;
+    T_stag[icv] = 1.0 + 0.5 * gm1 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / (s->sos[icv] * s->sos[icv]);
+    const double tmp1 = log1p((s->pr[icv] - s->p.p_ref) * inv_pref);
+    const double tmp2 = log1p((s->rho[icv] - s->p.rho_ref) * inv_rhoref);
+    ent[icv] = (tmp1 - gamma * tmp2) * invgm1;
+  }
+  for (int ief = 0; ief < m->nef; ++ief) {
+    const int icv0 = m->lo[ief], icv1 = m->hi[ief];
+    const double Ts_avg = 0.5 * (T_stag[icv0] + T_stag[icv1]);
+    const double dp = s->pr[icv1] - s->pr[icv0];
+    const double dh = cp * (s->T[icv1] - s->T[icv0]);
+    const double v_avg = 0.5 * (1.0 / s->rho[icv0] + 1.0 / s->rho[icv1]);
+    const double beta_avg = 0.5 * (1.0 / (R_gas * s->T[icv0]) + 1.0 / (R_gas * s->T[icv1]));
+    const double dsor = ent[icv1] - ent[icv0];
+    const double gr = (-dsor + beta_avg * (dh - v_avg * dp)) / Ts_avg;
+    s->vv2[icv0] = smax(s->vv2[icv0], fabs(gr));
+    s->vv2[icv1] = smax(s->vv2[icv1], fabs(gr));
+  }
+  for (int icv = 0; icv < n; ++icv) s->cv_compact[icv].fgr = og_gr_func(s->vv2[icv], gr_nrm);
+  free(T_stag);
+  free(ent);
+}
+
 /* IdealGasSolver.cpp:182-255 */
 void og_sgs_props(og_solver* s) {
   const og_params* P = &s->p;
@@ -507,7 +547,8 @@ void og_sgs_props(og_solver* s) {
     a->symp_fax = smax(coeff_p_delta * s->m.dcmag_hat[icv] * lambda / sos2 - dx2_approx * s->a_sgs[icv], 0.0);
     a->symh_fax = coeff_h_delta * s->m.dcmag_hat[icv];
   }
-  set_fgr(s);
+  if (s->p.full_gr) set_fgr_full(s);
+  else set_fgr(s);
 }
 
 /* IdealGasSolver.cpp:257-535 over the uniform pair list */

@@ -20,6 +20,7 @@
  *                                                     IdealGasSolver.cpp:182-255,736-745,894-984,
  *                                                     IdealGasSolverSgs.cpp:4-82,207-360, MiscUtils.hpp:273-276
  *   og_svd33           calcSvd33 (one-sided Jacobi)   SvdNew.hpp:24-124
+ *   (USE_FULL_GR)      setFgrCvsFull                  IdealGasSolver.cpp:747-838
  *   og_calc_cfl        calcCfl + setDt                IdealGasSolver.cpp:1048-1095, FlowSolver.hpp:2274-2328
  *   og_step            advanceSolution + checkForNans + report
  *                                                     IdealGasSolver.hpp:414-611,805-848
@@ -76,6 +77,7 @@ typedef struct og_params {
   double dt_data[2];
   int check_interval;
   double sigma_coeff; /* SIGMA_COEFF */
+  int full_gr;        /* USE_FULL_GR */
 } og_params;
 
 typedef struct og_diag {

@@ -32,6 +32,7 @@ class og_params(C.Structure):
         ("lambda_dil_coeff", C.c_double), ("Pr_t", C.c_double), ("limit_asgs", C.c_int), ("coeff_p", C.c_double),
         ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
         ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
+        ("full_gr", C.c_int),
     ]
 
 
@@ -109,7 +110,9 @@ def dptr(a):
 def params_from(cgp) -> og_params:
     p = og_params()
     for name, _ in og_params._fields_:
-        if name == "dt_data":
+        if name == "full_gr":
+            p.full_gr = cgp.use_full_gr
+        elif name == "dt_data":
             p.dt_data[0] = cgp.dt_data[0]
             p.dt_data[1] = cgp.dt_data[1]
         else:

@@ -67,9 +67,11 @@ def test_param_parsing():
 def test_sigma_model_parsing():
     p = ig.parse_params("SGS_MODEL SIGMA\nSIGMA_COEFF 1.35\nPR_T 0.85\n")
     assert p.sgs_model == ig.CG_SGS_SIGMA and p.sigma_coeff == 1.35 and p.Pr_t == 0.85
+    assert p.use_full_gr == 0
+    assert ig.parse_params("USE_FULL_GR\n").use_full_gr == 1
 
 
-@pytest.mark.parametrize("line", ["SGS_MODEL DSM_LOCAL", "EF_OPS ALIGNED", "USE_FULL_GR", "TIMESTEP FOO 1",
+@pytest.mark.parametrize("line", ["SGS_MODEL DSM_LOCAL", "EF_OPS ALIGNED", "DISS_SPONGE", "TIMESTEP FOO 1",
                                   "GAMMA abc", "CHECK_INTERVAL 0"])
 def test_param_errors(line):
     with pytest.raises(ig.CharlesError) as e:

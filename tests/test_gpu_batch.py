@@ -204,3 +204,34 @@ def test_solves_sharing_a_layout_match_solo_runs():
     cnt = b.counts()
     assert cnt["device_bytes"] == 2 * own + shared  # the layout is counted once
     b.close()
+
+
+@pytest.mark.parametrize("precision", [CG_PREC_FP64, CG_PREC_MIXED])
+def test_batch_with_mixed_sgs_models_matches_solo_runs(precision):
+    # NONE, VREMAN and SIGMA solves in one batch: the Sigma pass runs for the
+    # batch and leaves the other instances alone
+    prec = "MIXED" if precision == CG_PREC_MIXED else "FP64"
+    models = ["SIGMA", "VREMAN", "NONE", "SIGMA"]
+
+    def make(i):
+        n, seed, ic, Re, Ma = CASES[i]
+        m = Mesh.jittered(n, 0.3, seed=seed)
+        text = tgv_params(Re=Re, Ma=Ma, sgs=models[i])
+        return m, parse_params(text + f"CHECK_INTERVAL 3\nPRECISION {prec}\n"), m.ic_tgv(Ma, 1.4)
+
+    b = Batch(precision)
+    ids = []
+    for i in range(len(models)):
+        m, p, st = make(i)
+        ids.append(b.add(m, p, *st))
+    b.step(5)
+    for i, iid in enumerate(ids):
+        m, p, st = make(i)
+        g = IdealGasSolver(m, p)
+        g.set_state(*st)
+        g.init_complete()
+        g.advance_solution(5)
+        assert_same(b.state(iid), g.get_state())
+        assert b.diag(iid).ke == g.report().ke
+        g.close()
+    b.close()

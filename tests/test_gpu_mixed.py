@@ -85,6 +85,17 @@ def test_mixed_sigma_sgs_close_to_oracle():
     assert np.all(err <= RHS_TOL), err
 
 
+def test_mixed_full_gibbs_sensor_close_to_oracle():
+    m, p, state = setup("k4", "MIXED")
+    text = k4_params() + "USE_FULL_GR\n"
+    g = solver(m, parse_params(text + "PRECISION MIXED\n"), state)
+    o = oracle(m, state, text)
+    rg, ro = g.calc_rhs(1), o.calc_rhs()
+    err = np.abs(rg - ro).max(axis=0) / np.abs(ro).max(axis=0)
+    print(f"k4 USE_FULL_GR: mixed RHS error per component {err}")
+    assert np.all(err <= RHS_TOL), err
+
+
 def test_mixed_history_close_to_oracle():
     m, p, state = setup("tgv", "MIXED")
     g = solver(m, p, state)
