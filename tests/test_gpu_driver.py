@@ -57,3 +57,19 @@ def test_driver_k4_deck_matches_oracle(tmp_path):
     o.init_complete()
     o.step(6)
     assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]
+
+
+def test_driver_sigma_and_full_sensor_deck_matches_oracle(tmp_path):
+    # SGS_MODEL SIGMA and USE_FULL_GR through the C++ host side
+    text = tgv_params(sgs="SIGMA") + "SIGMA_COEFF 1.4\nUSE_FULL_GR\n"
+    deck = tmp_path / "sigma.in"
+    deck.write_text(text + "MESH JITTERED 12 0.3 11\nINIT TGV 0.1\nNSTEPS 8\nCHECK_INTERVAL 4\n")
+    out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    kes = [float(s) for s in re.findall(r"ke: ([0-9.eE+-]+)", out.stdout)]
+    m = Mesh.jittered(12, 0.3, seed=11)
+    o = OracleSolver(m, parse_params(text))
+    o.set_state(*tgv_state(m.get("x_cv")))
+    o.init_complete()
+    o.step(8)
+    assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]
