@@ -34,6 +34,7 @@ extern "C" {
 #define CG_EHIP -3     /* HIP runtime error */
 #define CG_ENAN -4     /* NaN detected and no snapshot to restore */
 #define CG_ESTATE -5   /* call out of order (e.g. step before init_complete) */
+#define CG_EIO -6      /* a mesh or data file could not be read or written */
 
 /* SGS_MODEL (IdealGasSolver.hpp:772-790) */
 #define CG_SGS_NONE 0
@@ -137,6 +138,33 @@ int cg_mesh_jittered(int nx, int ny, int nz, double Lx, double Ly, double Lz, do
 int cg_mesh_from_arrays(int ncv, const double* x_cv, const double* vol, int nfa, const int* cvofa,
                         const int* shift, const double* n_fa, const double* x_fa,
                         const double L[3], const cg_params* p, cg_mesh** out);
+/* .mles mesh files (StripedMesh::read_mles, StripedMesh.hpp:622,1336-2400;
+ * records Defs.hpp:27-249), io version 5, either byte order.
+ * cg_mles_counts: counts[0..7] = nno, nbf, nfa, nfa_p (periodic faces), ncv,
+ *   nno_p, nno_pb, nef -- the globals read_mles sets (the fields
+ *   src/core/common/tests/testStripedMesh.cpp checks); nef is 0 (extended
+ *   faces are built by the solver, not stored).
+ * cg_mesh_read_mles: the mesh of a fully periodic file (three axis-aligned
+ *   Cartesian periodic transforms, no boundary faces; face zones 0..25 give the
+ *   periodic image of cvofa[f][1], StripedMesh.hpp:3043-3105), with cells in
+ *   the file's order and operators built as in cg_mesh_from_arrays. A file
+ *   with boundary faces is rejected (CG_EMESH): this solver has no boundary
+ *   conditions.
+ * cg_mesh_write_mles: a periodic mesh as a geometry-only file (counts, periodic
+ *   transforms, cvofa, one fluid cell zone, vol_cv, x_vv, x_cv, n_fa, x_fa;
+ *   faces in zone order as StitchNS::writeMles, StitchNS.cpp:8122, but no node
+ *   records) that cg_mesh_read_mles reads back to the same mesh. */
+int cg_mles_counts(const char* path, int64_t counts[8]);
+/* Copy a record array of a .mles file out, as stored (the StripedMesh arrays
+ * read_mles fills): int64 cvofa [nfa][2], cvobf [nbf], noofa_i [nfa+1] (CSR
+ * offsets), noofa_v, noobf_i [nbf+1], noobf_v, nfa_zone [27], bf_zone_nbf
+ * [zones]; double x_no [nno][3], x_vv, x_cv [ncv][3], vol_cv, r_vv [ncv], n_fa,
+ * x_fa [nfa][3], n_bf, x_bf [nbf][3], area_bf, area_over_delta_bf [nbf], Gij_bf
+ * [nbf][3][3], bf_zone_area [zones]. nbytes must equal the array's size (0 for
+ * a record the file does not hold). */
+int cg_mles_get(const char* path, const char* name, void* out, int64_t nbytes);
+int cg_mesh_read_mles(const char* path, const cg_params* p, cg_mesh** out);
+int cg_mesh_write_mles(const cg_mesh* m, const char* path);
 /* counts[0..9] = ncv, nfa, nef, nnz (gradient stencil incl. diagonal),
  * grouped extended pairs, grouped compact faces, thresholded gradient cells,
  * dropped sliver faces, max extended pairs per cell, max compact faces per cell */
@@ -202,6 +230,16 @@ int cg_restore(cg_solver* s);
  * [1] extended-face flux + RK3 + EOS, [2] gradient, [3] properties, and
  * [4] the sum of all kernel durations per step. */
 int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms);
+/* .sles data files (StaticSolver::writeSles / writeData, StaticSolver.hpp:2311-2321,
+ * StaticSolver.cpp:7560; CtiRegister.cpp:412-610 for the record of each value):
+ * cg_write_sles writes step (I0), time and dt (D0), rho, rhoE, p, T (CV_D1) and
+ * u (CV_D2) in the mesh's cell order, after cg_init_complete. cg_read_sles sets
+ * the state (rho, u, rhoE) and step, time, dt from such a file, as the restart
+ * read of the registered READWRITE_DATA (FlowSolver.hpp:362-364,
+ * IdealGasSolver.hpp:96-100); call cg_init_complete next, which recomputes p
+ * and T from the state. The cell count must match the solver's mesh. */
+int cg_write_sles(cg_solver* s, const char* path);
+int cg_read_sles(cg_solver* s, const char* path);
 const char* cg_last_error(const cg_solver* s);
 void cg_destroy(cg_solver* s);
 

@@ -60,6 +60,16 @@ class IdealGasSolver {
   void initJittered(int n, double L, double jitter, uint64_t seed) {
     cg_check(cg_mesh_jittered(n, n, n, L, L, L, jitter, seed, &params_, &mesh_));
   }
+  // RESTART <mesh.mles> (StaticSolver::init, StaticSolver.hpp:1168-1230): a periodic mesh file
+  void initMles(const std::string& path) { cg_check(cg_mesh_read_mles(path.c_str(), &params_, &mesh_)); }
+
+  // RESTART <mesh.mles> <data.sles>: run() starts from the state, step, time and dt
+  // of this data file instead of initialHook
+  std::string restart_sles;
+  // WRITE_SLES <interval> | WRITE_SLES INTERVAL <interval> [NAME <prefix>]
+  // (StaticSolver.hpp:3378-3455): a data file <prefix>.<step %08d>.sles every interval steps
+  int write_sles_interval = 0;
+  std::string write_sles_prefix = "result";
 
   // FlowSolver::init + IdealGasSolver::initData: allocate the device state
   void init() {
@@ -119,6 +129,14 @@ class IdealGasSolver {
     return (step == before + 1) ? 0 : -1;
   }
 
+  // StaticSolver::writeSles (StaticSolver.hpp:2311-2321)
+  void writeSles(int step_) {
+    char fn[512];
+    std::snprintf(fn, sizeof(fn), "%s.%08d.sles", write_sles_prefix.c_str(), step_);
+    std::printf(" Writing sles file \"%s\"...\n", fn);
+    cg_check(cg_write_sles(solver_, fn), solver_);
+  }
+
   void storeData() { cg_check(cg_store(solver_), solver_); }
   void restoreData() { cg_check(cg_restore(solver_), solver_); }
 
@@ -134,14 +152,25 @@ class IdealGasSolver {
     return d;
   }
 
-  // FlowSolver::run (FlowSolver.hpp:2445-2678), without I/O
+  // FlowSolver::run (FlowSolver.hpp:2445-2678)
   void run() {
-    initialHook();
-    initComplete();
+    if (!restart_sles.empty()) {
+      cg_check(cg_read_sles(solver_, restart_sles.c_str()), solver_);
+      cg_check(cg_init_complete(solver_), solver_);
+      pullState();
+      cg_diag d = report(false);
+      step = int(d.step);
+      time = d.time;
+      dt = d.dt;
+    } else {
+      initialHook();
+      initComplete();
+    }
     if (nsteps < 0) return;
     int chunk = hooks_need_state ? 1 : check_interval;
     while (step < nsteps) {
-      const int n = std::min(chunk, nsteps - step);
+      int n = std::min(chunk - step % chunk, nsteps - step);
+      if (write_sles_interval > 0) n = std::min(n, write_sles_interval - step % write_sles_interval);
       cg_check(cg_step(solver_, n), solver_);
       cg_diag d = report(false);
       step = int(d.step);
@@ -155,6 +184,7 @@ class IdealGasSolver {
         temporalHook();
       }
       if (step % check_interval == 0) report(true);
+      if (write_sles_interval > 0 && step % write_sles_interval == 0) writeSles(step);
     }
     pullState();
     finalHook();

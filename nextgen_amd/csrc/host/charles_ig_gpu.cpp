@@ -6,9 +6,12 @@
 // CHECK_INTERVAL); the mesh comes from this driver's MESH key (periodic box meshes):
 //   MESH LATTICE <n> [L]
 //   MESH JITTERED <n> <jitter> <seed> [L]
-// and INIT TGV [Ma] sets the Taylor-Green vortex at the cell centroids, as an
-// initialHook would (charles_ig.cpp:209-222); INIT K4 <k0> <Mt> <seed> the seeded
-// k^4-spectrum decaying-turbulence field (cg_ic_k4).
+// or, as the reference, RESTART <mesh.mles> [<data.sles>] (StaticSolver.hpp:1168-1230;
+// periodic mesh files). INIT TGV [Ma] sets the Taylor-Green vortex at the cell
+// centroids, as an initialHook would (charles_ig.cpp:209-222); INIT K4 <k0> <Mt>
+// <seed> the seeded k^4-spectrum decaying-turbulence field (cg_ic_k4). WRITE_SLES
+// <interval> or WRITE_SLES INTERVAL <interval> [NAME <prefix>] writes data files
+// (StaticSolver.hpp:3378-3455).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +36,9 @@ struct Deck {
   double k0 = 4.0, mt = 0.3;
   unsigned long long k4_seed = 1;
   int nsteps = -1;
+  std::string restart_mles, restart_sles;
+  int write_sles_interval = 0;
+  std::string write_sles_prefix = "result";
 };
 
 Deck read_deck(const char* path) {
@@ -69,6 +75,24 @@ Deck read_deck(const char* path) {
         ls >> d.k0 >> d.mt >> d.k4_seed;
       } else {
         std::fprintf(stderr, "ERROR: INIT %s: expected TGV [Ma] or K4 <k0> <Mt> <seed>\n", what.c_str());
+        throw(0);
+      }
+      continue;
+    }
+    if (key == "RESTART") {
+      ls >> d.restart_mles;
+      ls >> d.restart_sles;
+      continue;
+    }
+    if (key == "WRITE_SLES") {
+      std::string tok;
+      while (ls >> tok) {
+        if (tok == "INTERVAL") ls >> d.write_sles_interval;
+        else if (tok == "NAME") ls >> d.write_sles_prefix;
+        else d.write_sles_interval = std::atoi(tok.c_str());
+      }
+      if (d.write_sles_interval <= 0) {
+        std::fprintf(stderr, "ERROR: WRITE_SLES: expected <interval> or INTERVAL <interval> [NAME <prefix>]\n");
         throw(0);
       }
       continue;
@@ -124,7 +148,11 @@ int main(int argc, char** argv) {
     solver.mt = d.mt;
     solver.k4_seed = d.k4_seed;
     solver.nsteps = d.nsteps;
-    if (d.mesh_kind == "JITTERED") solver.initJittered(d.n, d.L, d.jitter, d.seed);
+    solver.restart_sles = d.restart_sles;
+    solver.write_sles_interval = d.write_sles_interval;
+    solver.write_sles_prefix = d.write_sles_prefix;
+    if (!d.restart_mles.empty()) solver.initMles(d.restart_mles);
+    else if (d.mesh_kind == "JITTERED") solver.initJittered(d.n, d.L, d.jitter, d.seed);
     else solver.initLattice(d.n, d.L);
     solver.init();
     std::printf(" > mesh: %d cells\n", solver.ncv);

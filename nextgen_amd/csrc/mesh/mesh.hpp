@@ -77,6 +77,60 @@ std::string build_from_arrays(Voronoi& m, int ncv, const double* x_cv, const dou
                               const double* x_fa, const double L[3]);
 std::string build_operators(const Voronoi& m, const OperatorParams& p, Operators& op, int nthreads);
 
+// ---- .mles mesh files (mles.cpp) ----
+struct MlesZone {
+  std::string name;
+  int64_t nbf = 0, noobf = 0, ibf_f = 0;
+  double area = 0.0, area_over_delta = 0.0, n[3] = {0, 0, 0}, x[3] = {0, 0, 0};
+};
+
+// The records of a Stitch mesh file that the solver uses, in file order.
+struct Mles {
+  int version = 0;
+  bool swapped = false;          // written with the other byte order
+  bool surface = false;          // the counts record announces a surface record
+  int64_t nno = 0, nbf = 0, nfa = 0, ncv = 0, nno_p = 0, nno_pb = 0;
+  int64_t nfa_zone[27] = {0}, noofa_zone[27] = {0};  // zones 0..25 periodic, 26 interior
+  int npt = 0, pt_kind[3] = {0, 0, 0};
+  double pt_data[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  std::vector<MlesZone> bf_zones;
+  std::vector<std::string> cv_zones;
+  std::vector<int64_t> cvofa, cvobf;        // [nfa][2], [nbf]
+  std::vector<int64_t> noofa_i, noofa_v;    // CSR [nfa+1], node indices
+  std::vector<int64_t> noobf_i, noobf_v;    // CSR [nbf+1], node indices
+  std::vector<uint64_t> pbi_pno;            // [nno_p]
+  std::vector<double> x_no;                 // [nno][3]
+  std::vector<double> x_vv, x_cv, vol_cv, r_vv;                     // per cell
+  std::vector<double> n_fa, x_fa;                                   // [nfa][3]
+  std::vector<double> n_bf, x_bf, area_bf, area_over_delta_bf, Gij_bf;  // per boundary face
+  int64_t nfa_p() const;                    // periodic faces (zones 0..25)
+};
+
+// Read a .mles file (io version 5). Returns "" or an error message.
+std::string read_mles(const std::string& path, Mles& out);
+// The periodic-box mesh of a .mles file: three axis-aligned Cartesian transforms
+// and no boundary faces; face zones give the periodic image of cvofa[f][1].
+std::string voronoi_from_mles(const Mles& f, Voronoi& m);
+// Write a periodic-box mesh as a geometry-only .mles file (counts, periodic
+// transforms, cvofa, cell zone, vol_cv, x_vv, x_cv, n_fa, x_fa; no node records).
+std::string write_mles(const Voronoi& m, const std::string& path);
+
+// ---- .sles data files of the ideal-gas state (mles.cpp) ----
+struct SlesData {
+  int64_t ncv = 0;
+  int step = 0;
+  double time = 0.0, dt = 0.0;
+  const double *rho = nullptr, *u = nullptr, *rhoE = nullptr, *p = nullptr, *T = nullptr;  // p, T optional
+};
+struct SlesRead {
+  int step = 0;
+  double time = 0.0, dt = 0.0;
+  bool have_step = false, have_time = false, have_dt = false;
+  std::vector<double> rho, u, rhoE, p, T;
+};
+std::string write_sles(const std::string& path, const SlesData& s);
+std::string read_sles(const std::string& path, int64_t ncv, SlesRead& out);
+
 // Morton code of a lattice index (21 bits per axis).
 uint64_t morton3(uint32_t i, uint32_t j, uint32_t k);
 

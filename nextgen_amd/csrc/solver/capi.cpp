@@ -1312,6 +1312,79 @@ int cg_mesh_from_arrays(int ncv, const double* x_cv, const double* vol, int nfa,
   return finish_mesh(m, p, out);
 }
 
+int cg_mles_counts(const char* path, int64_t counts[8]) {
+  if (!path || !counts) return fail(CG_EINVAL, "null argument");
+  cgmesh::Mles f;
+  const std::string e = cgmesh::read_mles(path, f);
+  if (!e.empty()) return fail(CG_EIO, e);
+  counts[0] = f.nno;
+  counts[1] = f.nbf;
+  counts[2] = f.nfa;
+  counts[3] = f.nfa_p();
+  counts[4] = f.ncv;
+  counts[5] = f.nno_p;
+  counts[6] = f.nno_pb;
+  counts[7] = 0;  // extended faces are built by the solver, never stored in the mesh file
+  return CG_OK;
+}
+
+int cg_mles_get(const char* path, const char* name, void* out, int64_t nbytes) {
+  if (!path || !name || !out) return fail(CG_EINVAL, "null argument");
+  cgmesh::Mles f;
+  const std::string e = cgmesh::read_mles(path, f);
+  if (!e.empty()) return fail(CG_EIO, e);
+  std::vector<int64_t> zone_nbf;
+  std::vector<double> zone_area;
+  for (const auto& z : f.bf_zones) {
+    zone_nbf.push_back(z.nbf);
+    zone_area.push_back(z.area);
+  }
+  struct Item {
+    const char* key;
+    const void* p;
+    size_t bytes;
+  };
+  auto item = [](const char* key, const auto& v) { return Item{key, v.data(), sizeof(v[0]) * v.size()}; };
+  const Item items[] = {
+      item("cvofa", f.cvofa),     item("cvobf", f.cvobf),     item("noofa_i", f.noofa_i), item("noofa_v", f.noofa_v),
+      item("noobf_i", f.noobf_i), item("noobf_v", f.noobf_v), item("x_no", f.x_no),       item("x_vv", f.x_vv),
+      item("x_cv", f.x_cv),       item("vol_cv", f.vol_cv),   item("r_vv", f.r_vv),       item("n_fa", f.n_fa),
+      item("x_fa", f.x_fa),       item("n_bf", f.n_bf),       item("x_bf", f.x_bf),       item("area_bf", f.area_bf),
+      item("area_over_delta_bf", f.area_over_delta_bf),      item("Gij_bf", f.Gij_bf),
+      item("bf_zone_nbf", zone_nbf), item("bf_zone_area", zone_area),
+      Item{"nfa_zone", f.nfa_zone, sizeof(f.nfa_zone)}};
+  const std::string n(name);
+  for (const Item& it : items) {
+    if (n != it.key) continue;
+    if ((int64_t)it.bytes != nbytes)
+      return fail(CG_EINVAL, n + ": " + std::to_string(it.bytes) + " bytes in the file, " + std::to_string(nbytes) +
+                                 " asked");
+    if (it.bytes) std::memcpy(out, it.p, it.bytes);
+    return CG_OK;
+  }
+  return fail(CG_EINVAL, "unknown mesh file array " + n);
+}
+
+int cg_mesh_read_mles(const char* path, const cg_params* p, cg_mesh** out) {
+  if (!path || !out) return fail(CG_EINVAL, "null argument");
+  cgmesh::Mles f;
+  std::string e = cgmesh::read_mles(path, f);
+  if (!e.empty()) return fail(CG_EIO, e);
+  cg_mesh* m = new cg_mesh();
+  e = cgmesh::voronoi_from_mles(f, m->vor);
+  if (!e.empty()) {
+    delete m;
+    return fail(CG_EMESH, std::string(path) + ": " + e);
+  }
+  return finish_mesh(m, p, out);
+}
+
+int cg_mesh_write_mles(const cg_mesh* m, const char* path) {
+  if (!m || !path) return fail(CG_EINVAL, "null argument");
+  const std::string e = cgmesh::write_mles(m->vor, path);
+  return e.empty() ? CG_OK : fail(CG_EIO, e);
+}
+
 int cg_mesh_counts(const cg_mesh* m, int64_t counts[10]) {
   if (!m || !counts) return fail(CG_EINVAL, "null argument");
   counts[0] = m->op.ncv;
@@ -1616,6 +1689,46 @@ int cg_restore(cg_solver* s) {
     if (int rc = engine_sync(s, E, s->stream)) return rc;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return 0;
+  });
+}
+
+int cg_write_sles(cg_solver* s, const char* path) {
+  if (!s || !path) return fail(CG_EINVAL, "null argument");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    const size_t n = (size_t)I->ncv;
+    std::vector<double> rho(n), u(3 * n), rhoE(n), p(n), T(n);
+    if (int rc = get_state(s, I, E.cur, rho.data(), u.data(), rhoE.data(), s->stream)) return rc;
+    if (int rc = cg_get_field(s, "p", p.data(), (int64_t)n)) return rc;
+    if (int rc = cg_get_field(s, "T", T.data(), (int64_t)n)) return rc;
+    if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
+    cgmesh::SlesData d;
+    d.ncv = (int64_t)n;
+    d.step = (int)s->info_host->step;
+    d.time = s->info_host->time;
+    d.dt = s->info_host->dt;
+    d.rho = rho.data();
+    d.u = u.data();
+    d.rhoE = rhoE.data();
+    d.p = p.data();
+    d.T = T.data();
+    const std::string e = cgmesh::write_sles(path, d);
+    return e.empty() ? 0 : s->set(CG_EIO, e);
+  });
+}
+
+int cg_read_sles(cg_solver* s, const char* path) {
+  if (!s || !path) return fail(CG_EINVAL, "null argument");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    cgmesh::SlesRead r;
+    const std::string e = cgmesh::read_sles(path, I->ncv, r);
+    if (!e.empty()) return s->set(CG_EIO, e);
+    if (int rc = set_state(s, I, E.cur, r.rho.data(), r.u.data(), r.rhoE.data(), s->stream)) return rc;
+    if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
+    if (r.have_step) s->info_host->step = r.step;
+    if (r.have_time) s->info_host->time = r.time;
+    if (r.have_dt) s->info_host->dt = r.dt;
+    return write_info(s, I, s->info_host, s->stream);
   });
 }
 

@@ -73,7 +73,8 @@ EXPORTS = [
     "cg_batch_get_diag", "cg_batch_get_state", "cg_batch_counts", "cg_batch_set_option", "cg_batch_time_steps",
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
     "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
-    "cg_instance_create_on",
+    "cg_instance_create_on", "cg_mles_counts", "cg_mles_get", "cg_mesh_read_mles", "cg_mesh_write_mles",
+    "cg_write_sles", "cg_read_sles",
 ]
 
 _lib = None
@@ -100,6 +101,12 @@ def load_library(path: str = LIB_PATH):
                                        C.c_double, C.c_uint64, P(cg_params), P(vp)]),
         "cg_mesh_from_arrays": (C.c_int, [C.c_int, dp, dp, C.c_int, ip, ip, dp, dp, dp, P(cg_params), P(vp)]),
         "cg_mesh_counts": (C.c_int, [vp, P(C.c_int64)]),
+        "cg_mles_counts": (C.c_int, [C.c_char_p, P(C.c_int64)]),
+        "cg_mles_get": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.c_int64]),
+        "cg_mesh_read_mles": (C.c_int, [C.c_char_p, P(cg_params), P(vp)]),
+        "cg_mesh_write_mles": (C.c_int, [vp, C.c_char_p]),
+        "cg_write_sles": (C.c_int, [vp, C.c_char_p]),
+        "cg_read_sles": (C.c_int, [vp, C.c_char_p]),
         "cg_mesh_get": (C.c_int, [vp, C.c_char_p, vp, C.c_int64]),
         "cg_mesh_destroy": (None, [vp]),
         "cg_error_string": (C.c_char_p, []),
@@ -127,6 +134,27 @@ def load_library(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def mles_counts(path):
+    """The globals read_mles sets: nno, nbf, nfa, nfa_p, ncv, nno_p, nno_pb, nef
+    (the fields src/core/common/tests/testStripedMesh.cpp checks)."""
+    cnt = (C.c_int64 * 8)()
+    _check(load_library().cg_mles_counts(os.fsencode(path), cnt))
+    keys = ("nno", "nbf", "nfa", "nfa_p", "ncv", "nno_p", "nno_pb", "nef")
+    return dict(zip(keys, (int(x) for x in cnt)))
+
+
+MLES_INT_ARRAYS = {"cvofa", "cvobf", "noofa_i", "noofa_v", "noobf_i", "noobf_v", "nfa_zone", "bf_zone_nbf"}
+
+
+def mles_get(path, name, shape):
+    """A record array of a .mles file as stored (int64 index arrays, float64
+    geometry); `shape` is the caller's expected shape."""
+    dt = np.int64 if name in MLES_INT_ARRAYS else np.float64
+    out = np.empty(shape, dtype=dt)
+    _check(load_library().cg_mles_get(os.fsencode(path), name.encode(), out.ctypes.data_as(C.c_void_p), out.nbytes))
+    return out
 
 
 def _dptr(a):
@@ -216,6 +244,19 @@ class Mesh:
                                        _dptr(n_fa), _dptr(x_fa), _dptr(Lv), C.byref(params or default_params()),
                                        C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def read_mles(cls, path, params=None):
+        """The periodic mesh of a .mles file (StripedMesh::read_mles,
+        src/core/common/StripedMesh.hpp:622), cells in the file's order."""
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.cg_mesh_read_mles(os.fsencode(path), C.byref(params or default_params()), C.byref(h)))
+        return cls(h)
+
+    def write_mles(self, path):
+        """Write the mesh as a geometry-only .mles file (no node records)."""
+        _check(load_library().cg_mesh_write_mles(self._h, os.fsencode(path)))
 
     def _size(self, name):
         n = {"x_cv": (self.ncv, 3), "x_vv": (self.ncv, 3), "vol": (self.ncv,), "cvofa": (self.nfa, 2),
@@ -333,6 +374,15 @@ class IdealGasSolver:
         arr = (cg_diag * nsteps)()
         self._chk(load_library().cg_step_history(self._h, nsteps, arr))
         return [Diag(**d.as_dict()) for d in arr]
+
+    def write_sles(self, path):
+        """StaticSolver::writeSles / writeData (src/core/common/StaticSolver.hpp:2311-2321):
+        step, time, dt, rho, u, rhoE, p, T."""
+        self._chk(load_library().cg_write_sles(self._h, os.fsencode(path)))
+
+    def read_sles(self, path):
+        """Restart read of rho, u, rhoE, step, time, dt; call init_complete() next."""
+        self._chk(load_library().cg_read_sles(self._h, os.fsencode(path)))
 
     def store_data(self):
         self._chk(load_library().cg_store(self._h))

@@ -73,3 +73,31 @@ def test_driver_sigma_and_full_sensor_deck_matches_oracle(tmp_path):
     o.init_complete()
     o.step(8)
     assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]
+
+
+def test_driver_restart_from_mesh_and_data_files(tmp_path):
+    # RESTART <mesh.mles> and WRITE_SLES, then RESTART <mesh.mles> <data.sles>:
+    # the restarted run ends where the uninterrupted run ends
+    m = Mesh.jittered(12, 0.3, seed=11)
+    mles = str(tmp_path / "box.mles")
+    m.write_mles(mles)
+    prefix = str(tmp_path / "res")
+    full = tmp_path / "full.in"
+    full.write_text(tgv_params(sgs="VREMAN") + f"RESTART {mles}\nINIT TGV 0.1\nNSTEPS 10\nCHECK_INTERVAL 5\n"
+                    f"WRITE_SLES INTERVAL 4 NAME {prefix}\n")
+    a = subprocess.run([EXE, "-i", str(full)], capture_output=True, text=True, timeout=120)
+    assert a.returncode == 0, a.stderr
+    assert os.path.exists(prefix + ".00000004.sles") and os.path.exists(prefix + ".00000008.sles")
+    rest = tmp_path / "rest.in"
+    rest.write_text(tgv_params(sgs="VREMAN") + f"RESTART {mles} {prefix}.00000004.sles\nNSTEPS 10\nCHECK_INTERVAL 5\n")
+    b = subprocess.run([EXE, "-i", str(rest)], capture_output=True, text=True, timeout=120)
+    assert b.returncode == 0, b.stderr
+    last = [re.findall(r"> step: 10 .*", o.stdout) for o in (a, b)]
+    assert last[0] and last[0] == last[1]
+    # the uninterrupted run also matches the oracle on the same mesh
+    kes = [float(s) for s in re.findall(r"ke: ([0-9.eE+-]+)", a.stdout)]
+    o = OracleSolver(m, parse_params(tgv_params(sgs="VREMAN")))
+    o.set_state(*tgv_state(m.get("x_cv")))
+    o.init_complete()
+    o.step(10)
+    assert abs(kes[-1] - o.diag()["ke"]) <= 1e-10 * o.diag()["ke"]

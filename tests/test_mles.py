@@ -1,0 +1,185 @@
+""".mles mesh files through the C ABI (host code, no GPU).
+
+The reference's own mesh-reader test (src/core/common/tests/testStripedMesh.cpp:12-23)
+reads its fixture orifice.mles and checks the counts read_mles sets; the same
+counts are checked here, then the arrays as read are checked against what the
+geometry fixes: every cell closes (the area vectors of its faces, oriented from
+cvofa[f][0] to cvofa[f][1], and of its boundary faces sum to zero), cells with
+no boundary face have the divergence-theorem volume sum (x_fa - x_cv).n_fa / 3
+of vol_cv, and the node loops of the faces span their area vectors. The fixture
+is read where the reference checkout is present (it is not copied into this
+repository: 3.3 MB); periodic files are written by this library and read back.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from nextgen_amd.ideal_gas import CharlesError, Mesh, mles_counts, mles_get
+
+ORIFICE = "/root/reference/src/core/common/tests/assets/orifice.mles"
+need_orifice = pytest.mark.skipif(not os.path.exists(ORIFICE), reason="reference mesh fixture not present")
+
+MESH_ARRAYS = ["x_cv", "vol", "cvofa", "fa_shift", "n_fa", "x_fa", "lo", "hi", "pshift", "n_ef", "c_ef",
+               "area", "area_over_delta", "unit_n", "cvocv_i", "cvocv_v", "grad_coeff", "sa_to_vol",
+               "dcmag_hat", "efocv_i", "efocv_v"]
+
+
+@need_orifice
+def test_orifice_counts_match_the_reference_reader_test():
+    # testStripedMesh.cpp:16-23
+    assert mles_counts(ORIFICE) == {"nno": 46348, "nbf": 693, "nfa": 5654, "nfa_p": 0, "ncv": 1056,
+                                    "nno_p": 0, "nno_pb": 42383, "nef": 0}
+
+
+@need_orifice
+def test_orifice_arrays_close_every_cell():
+    c = mles_counts(ORIFICE)
+    ncv, nfa, nbf = c["ncv"], c["nfa"], c["nbf"]
+    cvofa = mles_get(ORIFICE, "cvofa", (nfa, 2))
+    cvobf = mles_get(ORIFICE, "cvobf", (nbf,))
+    n_fa = mles_get(ORIFICE, "n_fa", (nfa, 3))
+    x_fa = mles_get(ORIFICE, "x_fa", (nfa, 3))
+    n_bf = mles_get(ORIFICE, "n_bf", (nbf, 3))
+    x_cv = mles_get(ORIFICE, "x_cv", (ncv, 3))
+    vol = mles_get(ORIFICE, "vol_cv", (ncv,))
+    assert cvofa.min() >= 0 and cvofa.max() < ncv and np.all(cvofa[:, 0] != cvofa[:, 1])
+    s = np.zeros((ncv, 3))
+    np.add.at(s, cvofa[:, 0], n_fa)
+    np.add.at(s, cvofa[:, 1], -n_fa)
+    np.add.at(s, cvobf, n_bf)
+    assert np.max(np.linalg.norm(s, axis=1) / vol ** (2.0 / 3.0)) < 1e-7
+    # a flipped orientation would not close
+    s2 = np.zeros((ncv, 3))
+    np.add.at(s2, cvofa[:, 0], -n_fa)
+    np.add.at(s2, cvofa[:, 1], n_fa)
+    np.add.at(s2, cvobf, n_bf)
+    assert np.max(np.linalg.norm(s2, axis=1) / vol ** (2.0 / 3.0)) > 1e-2
+    # divergence-theorem volume of the cells with planar faces only
+    vd = np.zeros(ncv)
+    np.add.at(vd, cvofa[:, 0], np.einsum("ij,ij->i", x_fa - x_cv[cvofa[:, 0]], n_fa) / 3.0)
+    np.add.at(vd, cvofa[:, 1], -np.einsum("ij,ij->i", x_fa - x_cv[cvofa[:, 1]], n_fa) / 3.0)
+    interior = np.ones(ncv, bool)
+    interior[cvobf] = False
+    assert interior.sum() > 400
+    assert np.max(np.abs(vd[interior] / vol[interior] - 1.0)) < 1e-12
+    assert vol.sum() > 0 and np.all(vol > 0)
+
+
+@need_orifice
+def test_orifice_node_loops_span_the_face_area_vectors():
+    c = mles_counts(ORIFICE)
+    nno, nfa, nbf = c["nno"], c["nfa"], c["nbf"]
+    x_no = mles_get(ORIFICE, "x_no", (nno, 3))
+
+    def loop_area(ptr, v, i):
+        X = x_no[v[ptr[i]:ptr[i + 1]]]
+        return 0.5 * np.cross(X, np.roll(X, -1, axis=0)).sum(0)
+
+    for kind, n, normals in (("fa", nfa, "n_fa"), ("bf", nbf, "n_bf")):
+        ptr = mles_get(ORIFICE, f"noo{kind}_i", (n + 1,))
+        v = mles_get(ORIFICE, f"noo{kind}_v", (int(ptr[-1]),))
+        assert ptr[0] == 0 and np.all(np.diff(ptr) >= 3) and v.min() >= 0 and v.max() < nno
+        nrm = mles_get(ORIFICE, normals, (n, 3))
+        err = np.array([np.linalg.norm(loop_area(ptr, v, i) - nrm[i]) / np.linalg.norm(nrm[i]) for i in range(n)])
+        if kind == "bf":
+            assert err.max() < 1e-7
+        else:
+            # a handful of tiny interior faces carry an averaged area vector
+            assert np.mean(err < 1e-6) > 0.995
+    # boundary zones: sizes add up to nbf, and each zone's area is the sum of its faces' areas
+    zn = mles_get(ORIFICE, "bf_zone_nbf", (6,))
+    za = mles_get(ORIFICE, "bf_zone_area", (6,))
+    area_bf = mles_get(ORIFICE, "area_bf", (nbf,))
+    assert zn.sum() == nbf
+    off = np.concatenate([[0], np.cumsum(zn)])
+    for z in range(6):
+        assert abs(area_bf[off[z]:off[z + 1]].sum() - za[z]) <= 1e-12 * za[z]
+
+
+@need_orifice
+def test_mesh_with_boundary_faces_is_rejected():
+    with pytest.raises(CharlesError, match="boundary faces"):
+        Mesh.read_mles(ORIFICE)
+
+
+def _same_mesh(a, b):
+    assert (a.ncv, a.nfa, a.nef, a.nnz) == (b.ncv, b.nfa, b.nef, b.nnz)
+    for name in MESH_ARRAYS:
+        np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
+
+
+@pytest.mark.parametrize("kind", ["lattice", "jittered"])
+def test_periodic_mesh_round_trip_is_exact(tmp_path, kind):
+    m = Mesh.lattice(8) if kind == "lattice" else Mesh.jittered(10, 0.3, seed=5)
+    path = str(tmp_path / "box.mles")
+    m.write_mles(path)
+    c = mles_counts(path)
+    assert (c["ncv"], c["nfa"], c["nbf"], c["nno"]) == (m.ncv, m.nfa, 0, 0)
+    # periodic faces sit in zones 0..25, the interior faces in zone 26
+    zones = mles_get(path, "nfa_zone", (27,))
+    shift = m.get("fa_shift")
+    assert zones[26] == np.sum(np.all(shift == 0, axis=1)) and c["nfa_p"] == m.nfa - zones[26]
+    # a face across the +x boundary only: zone 0 (transform 0) or 1 (its inverse)
+    only_x = np.sum((shift[:, 0] != 0) & (shift[:, 1] == 0) & (shift[:, 2] == 0))
+    assert zones[0] + zones[1] == only_x
+    r = Mesh.read_mles(path)
+    _same_mesh(m, r)
+    np.testing.assert_array_equal(m.get("x_vv"), r.get("x_vv"))
+
+
+def _byte_swap_file(src, dst):
+    """Rewrite a file written by this library in the other byte order (all of
+    its record data are 8-byte words)."""
+    b = bytearray(open(src, "rb").read())
+    out = bytearray()
+    out += struct.pack(">2i", *struct.unpack("<2i", b[:8]))
+    off = 8
+    while True:
+        h = b[off:off + 256]
+        rid, = struct.unpack("<i", h[52:56])
+        skip, = struct.unpack("<q", h[56:64])
+        out += h[:52] + struct.pack(">i", rid) + struct.pack(">q", skip)
+        out += struct.pack(">16i", *struct.unpack("<16i", h[64:128]))
+        out += struct.pack(">12d", *struct.unpack("<12d", h[128:224]))
+        out += struct.pack(">4Q", *struct.unpack("<4Q", h[224:256]))
+        nwords = (skip - 256) // 8
+        out += struct.pack(f">{nwords}Q", *struct.unpack(f"<{nwords}Q", b[off + 256:off + skip]))
+        if rid == 51:
+            break
+        off += skip
+    open(dst, "wb").write(bytes(out))
+
+
+def test_other_byte_order_reads_the_same(tmp_path):
+    m = Mesh.jittered(8, 0.25, seed=2)
+    a, b = str(tmp_path / "le.mles"), str(tmp_path / "be.mles")
+    m.write_mles(a)
+    _byte_swap_file(a, b)
+    assert open(a, "rb").read() != open(b, "rb").read()
+    assert mles_counts(a) == mles_counts(b)
+    _same_mesh(m, Mesh.read_mles(b))
+
+
+def test_bad_files_raise(tmp_path):
+    with pytest.raises(CharlesError, match="cannot open"):
+        mles_counts(str(tmp_path / "missing.mles"))
+    v3 = tmp_path / "v3.mles"
+    v3.write_bytes(struct.pack("<2i", 123581321, 3) + bytes(256))
+    with pytest.raises(CharlesError, match="version 3"):
+        mles_counts(str(v3))
+    junk = tmp_path / "junk.mles"
+    junk.write_bytes(b"not a mesh file at all")
+    with pytest.raises(CharlesError, match="magic"):
+        mles_counts(str(junk))
+    m = Mesh.lattice(6)
+    good = str(tmp_path / "good.mles")
+    m.write_mles(good)
+    data = open(good, "rb").read()
+    cut = tmp_path / "cut.mles"
+    cut.write_bytes(data[:len(data) // 2])
+    with pytest.raises(CharlesError, match="past the end|truncated"):
+        Mesh.read_mles(str(cut))
+    with pytest.raises(CharlesError, match="unknown mesh file array"):
+        mles_get(good, "no_such_array", (1,))
