@@ -36,10 +36,13 @@ extern "C" {
 #define CG_ESTATE -5   /* call out of order (e.g. step before init_complete) */
 #define CG_EIO -6      /* a mesh or data file could not be read or written */
 
-/* SGS_MODEL (IdealGasSolver.hpp:772-790) */
+/* SGS_MODEL (IdealGasSolver.hpp:772-790). DSM_LOCAL needs the cells' Voronoi
+ * radii: meshes from cg_mesh_lattice, cg_mesh_jittered or a .mles file with an
+ * r_vv record carry them, cg_mesh_from_arrays meshes do not (cg_create fails). */
 #define CG_SGS_NONE 0
 #define CG_SGS_VREMAN 1
 #define CG_SGS_SIGMA 2
+#define CG_SGS_DSM_LOCAL 3
 
 /* NO_DISS / ALL_DISS overrides of the Gibbs-remainder sensor (IdealGasSolver.cpp:973-979) */
 #define CG_DISS_SENSOR 0
@@ -100,6 +103,7 @@ typedef struct cg_params {
   int precision;           /* PRECISION: CG_PREC_FP64 (default) or CG_PREC_MIXED */
   double sigma_coeff;      /* SIGMA_COEFF (1.5), IdealGasSolverSgs.cpp:300 */
   int use_full_gr;         /* USE_FULL_GR: the full Gibbs-remainder sensor, IdealGasSolver.cpp:244-247 */
+  double delta_r_dsm;      /* DELTA_R_DSM (2.0): test-to-grid filter width ratio of DSM_LOCAL, Sgs.hpp:20 */
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,
@@ -200,7 +204,9 @@ int cg_set_state(cg_solver* s, const double* rho, const double* u, const double*
 int cg_get_state(cg_solver* s, double* rho, double* u, double* rhoE);
 /* Derived per-cell field by name: p, T, sos, mu_lam, mu_sgs, loc_lam, loc_sgs,
  * a_sgs, vv2, fgr, mu_total, loc_total, symp_fax, symh_fax, dudx ([ncv][3][3]),
- * cfl (per-cell convective CFL at the current dt). */
+ * cfl (per-cell convective CFL at the current dt); with SGS_MODEL DSM_LOCAL
+ * also lijmij, mijmij (the time-relaxed L:M and M:M, Sgs.hpp:189-232). Reading a
+ * field never advances that relaxation. */
 int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n);
 /* IdealGasSolver::initComplete (IdealGasSolver.hpp:275-301):
  * updateConservativeAndPrimitiveData + calcSgsAndMaterialProperties, and the

@@ -22,6 +22,8 @@ struct Voronoi {
   std::vector<double> x_vv;     // generators [ncv][3]
   std::vector<double> x_cv;     // centroids  [ncv][3]
   std::vector<double> vol;      // [ncv]
+  std::vector<double> r_vv;     // [ncv] Voronoi radius: largest generator-to-vertex distance
+                                // (StitchNS.cpp:8262-8266,8414); empty when not known
   int nfa = 0;
   std::vector<int> cvofa;       // [nfa][2]
   std::vector<int8_t> shift;    // [nfa][3]
@@ -121,12 +123,13 @@ struct SlesData {
   int step = 0;
   double time = 0.0, dt = 0.0;
   const double *rho = nullptr, *u = nullptr, *rhoE = nullptr, *p = nullptr, *T = nullptr;  // p, T optional
+  const double *lijmij = nullptr, *mijmij = nullptr;  // DSM_LOCAL relaxation state (optional)
 };
 struct SlesRead {
   int step = 0;
   double time = 0.0, dt = 0.0;
   bool have_step = false, have_time = false, have_dt = false;
-  std::vector<double> rho, u, rhoE, p, T;
+  std::vector<double> rho, u, rhoE, p, T, lijmij, mijmij;
 };
 std::string write_sles(const std::string& path, const SlesData& s);
 std::string read_sles(const std::string& path, int64_t ncv, SlesRead& out);

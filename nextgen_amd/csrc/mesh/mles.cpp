@@ -379,6 +379,7 @@ std::string voronoi_from_mles(const Mles& f, Voronoi& m) {
                                     f.n_fa.data(), f.x_fa.data(), L);
   if (!e.empty()) return e;
   if (f.x_vv.size() == size_t(3 * ncv)) m.x_vv = f.x_vv;
+  if (f.r_vv.size() == size_t(ncv)) m.r_vv = f.r_vv;
   return "";
 }
 
@@ -508,6 +509,7 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
   }
   w.put_array("vol_cv", kCvD1, m.vol.data(), ncv, 1);
   w.put_array("x_vv", kCvD2, m.x_vv.data(), ncv, 3);
+  if (m.r_vv.size() == size_t(ncv)) w.put_array("r_vv", kCvD1, m.r_vv.data(), ncv, 1);
   w.put_array("x_cv", kCvD2, m.x_cv.data(), ncv, 3);
   w.put_array("n_fa", kFaD2, n_fa.data(), nfa, 3);
   w.put_array("x_fa", kFaD2, x_fa.data(), nfa, 3);
@@ -543,6 +545,8 @@ std::string write_sles(const std::string& path, const SlesData& s) {
   w.put_array("rhoE", kCvD1, s.rhoE, ncv, 1);
   if (s.p) w.put_array("p", kCvD1, s.p, ncv, 1);
   if (s.T) w.put_array("T", kCvD1, s.T, ncv, 1);
+  if (s.lijmij) w.put_array("lijmij", kCvD1, s.lijmij, ncv, 1);
+  if (s.mijmij) w.put_array("mijmij", kCvD1, s.mijmij, ncv, 1);
   w.put_header(w.header("EOF", kEof, 0));
   if (!w.close()) return "write failed: " + path;
   return "";
@@ -587,6 +591,8 @@ std::string read_sles(const std::string& path, int64_t ncv, SlesRead& s) {
       if (h.id == kCvD1 && name == "rhoE") dst = &s.rhoE;
       if (h.id == kCvD1 && name == "p") dst = &s.p;
       if (h.id == kCvD1 && name == "T") dst = &s.T;
+      if (h.id == kCvD1 && name == "lijmij") dst = &s.lijmij;
+      if (h.id == kCvD1 && name == "mijmij") dst = &s.mijmij;
       if (dst) {
         const int width = h.id == kCvD1 ? 1 : 3;
         int64_t n = 0;

@@ -216,7 +216,7 @@ struct HalfFace {
 };
 
 struct CellOut {
-  double vol, xc[3];
+  double vol, xc[3], r2;
   std::vector<HalfFace> hf;
 };
 
@@ -278,6 +278,8 @@ std::string build_lattice(Voronoi& m, int nx, int ny, int nz, double Lx, double 
   for (int c = 0; c < m.ncv; ++c)
     for (int d = 0; d < 3; ++d) m.x_vv[3 * c + d] = (ml.ijk[3 * c + d] + 0.5) * h[d];
   m.x_cv = m.x_vv;
+  // every vertex of a box cell is half its diagonal from the generator
+  m.r_vv.assign(m.ncv, std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]) / 2.0);
   m.nfa = 3 * m.ncv;
   m.cvofa.resize(2 * m.nfa);
   m.shift.assign(3 * m.nfa, 0);
@@ -402,6 +404,7 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
           co.hf.push_back(hf);
         }
         co.vol = vol;
+        co.r2 = P.rmax2();
         for (int d = 0; d < 3; ++d) co.xc[d] = xc[d] / vol;
         std::sort(co.hf.begin(), co.hf.end(), [](const HalfFace& a, const HalfFace& b) {
           return std::make_tuple(a.nbr, a.s[0], a.s[1], a.s[2]) < std::make_tuple(b.nbr, b.s[0], b.s[1], b.s[2]);
@@ -421,8 +424,10 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
 
   m.x_cv.resize(3 * ncv);
   m.vol.resize(ncv);
+  m.r_vv.resize(ncv);
   for (int c = 0; c < ncv; ++c) {
     m.vol[c] = cells[c].vol;
+    m.r_vv[c] = std::sqrt(cells[c].r2);
     for (int d = 0; d < 3; ++d) m.x_cv[3 * c + d] = m.x_vv[3 * c + d] + cells[c].xc[d];
   }
   // merge the two sides of every face: the lower cell index owns the geometry

@@ -111,6 +111,7 @@ struct Layout {
   // device cell i holds mesh cell ext_of[i]: every 256-cell block keeps its
   // Morton cells, sorted by their half-face counts
   std::vector<int> ext_of;
+  bool has_rvv = false;       // the mesh knows its cells' Voronoi radii (DSM_LOCAL needs them)
 };
 
 // host bookkeeping of one solve
@@ -132,6 +133,8 @@ struct HostInst {
   const int* ext_of = nullptr;       // device cell -> mesh cell (its layout's)
   Alloc mem;                         // its own device memory (state, work, snapshots)
   int id = -1;
+  bool dsm_kept = false;             // DSM_LOCAL L:M, M:M set from a data file: the next
+                                     // cg_init_complete keeps them
 };
 
 template <class R>
@@ -141,7 +144,7 @@ struct Instance : HostInst {
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
-            prm.use_full_gr != 0};
+            prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -312,6 +315,7 @@ void fill_dev_params(HostInst* h) {
   P.dt_data1 = p.dt_data[1];
   // full Gibbs-remainder sensor (setFgrCvsFull, IdealGasSolver.cpp:755-761)
   P.full_gr = p.use_full_gr;
+  P.dsm_alpha = p.delta_r_dsm * p.delta_r_dsm;
   P.cp = P.R_gas * p.gamma / (p.gamma - 1.0);
   P.rho_ref = p.rho_ref;
   P.inv_rhoref = 1.0 / p.rho_ref;
@@ -326,7 +330,8 @@ int check_params(const cg_params* p) {
   if (!(p->p_ref > 0 && p->rho_ref > 0 && p->T_ref > 0)) return fail(CG_EINVAL, "P_REF, RHO_REF and T_REF must be positive");
   if (!(p->gamma > 1.0)) return fail(CG_EINVAL, "GAMMA must exceed 1");
   if (!(p->Pr_lam > 0 && p->Pr_t > 0)) return fail(CG_EINVAL, "PR_LAM and PR_T must be positive");
-  if (p->sgs_model != CG_SGS_NONE && p->sgs_model != CG_SGS_VREMAN && p->sgs_model != CG_SGS_SIGMA)
+  if (p->sgs_model != CG_SGS_NONE && p->sgs_model != CG_SGS_VREMAN && p->sgs_model != CG_SGS_SIGMA &&
+      p->sgs_model != CG_SGS_DSM_LOCAL)
     return fail(CG_EINVAL, "unsupported SGS_MODEL");
   if (p->dt_mode < 0 || p->dt_mode > 3) return fail(CG_EINVAL, "bad TIMESTEP mode");
   if (p->precision != CG_PREC_FP64 && p->precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
@@ -482,6 +487,14 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   rc |= upload(s, h, &g.dcmag, dcmag);
   rc |= upload(s, h, &g.dx13, dx13);
   rc |= upload(s, h, &g.dx23, dx23);
+  {
+    const std::vector<double>& rv = m->vor.r_vv;
+    Lay->has_rvv = rv.size() == (size_t)ncv;
+    std::vector<double> r_vv(ncv, 0.0);
+    if (Lay->has_rvv)
+      for (int c = 0; c < ncv; ++c) r_vv[c] = rv[ext_of[c]];
+    rc |= upload(s, h, &g.r_vv, r_vv);
+  }
   if (rc) return rc;
 
   // SELL lane orders (CG_LANE_ORDER=plain: lane = cell, for comparison)
@@ -512,6 +525,24 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     // instead of gathering it by pair index)
     std::vector<double> cfn[3];
     for (auto& v : cfn) v.assign(cs.nslots, 0.0);
+    // test-filter weight of each compact face (filterCvR1/R2, StaticSolver.cpp:9849-9975):
+    // |n_fa| |x_vv1 - x_vv0| / 6, x_vv1 the periodic image, per compact slot
+    std::vector<double> fw(nfa), cfw(cs.nslots, 0.0);
+    {
+      const cgmesh::Voronoi& v = m->vor;
+      for (int f = 0; f < nfa; ++f) {
+        const int a = v.cvofa[2 * (size_t)f], b = v.cvofa[2 * (size_t)f + 1];
+        double d[3], nn[3];
+        for (int k = 0; k < 3; ++k) {
+          const double x1 = v.x_vv[3 * (size_t)b + k] + double(v.shift[3 * (size_t)f + k]) * v.L[k];
+          d[k] = x1 - v.x_vv[3 * (size_t)a + k];
+          nn[k] = v.n_fa[3 * (size_t)f + k];
+        }
+        const double area = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+        const double delta = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        fw[f] = area * delta / 6.0;
+      }
+    }
     // extended pair geometry per slot (SoA; both cells of a pair hold a copy)
     std::vector<double> x6[6];
     for (auto& v : x6) v.assign(xs.nslots, 0.0);
@@ -529,6 +560,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
         cnbr[sidx] = code_of(e, pidx);
         cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == e) ? 0x80000000u : 0u));
         for (int d = 0; d < 3; ++d) cfn[d][sidx] = nf[3 * (size_t)pidx + d];
+        cfw[sidx] = fw[pidx];
       }
       for (int k = 0; k < xcnt[q]; ++k) {
         const int pidx = o.efocv_v[o.efocv_i[e] + ccnt[q] + k];
@@ -628,6 +660,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     rc |= upload_as<R>(s, h, &g.cfn0, cfn[0]);
     rc |= upload_as<R>(s, h, &g.cfn1, cfn[1]);
     rc |= upload_as<R>(s, h, &g.cfn2, cfn[2]);
+    rc |= upload(s, h, &g.cfw, cfw);
     rc |= upload(s, h, &g.xf_ptr, xs.ptr);
     rc |= upload(s, h, &g.xf_deg, xcnt);
     rc |= upload(s, h, &g.xf_nbr, xnbr);
@@ -714,6 +747,14 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   if (!mixed) rc |= dev_alloc(s, h, &w.logp, ncv);
   double** wf[] = {&w.T, &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
   for (auto pp : wf) rc |= dev_alloc(s, h, pp, ncv);
+  if (p->sgs_model == CG_SGS_DSM_LOCAL) {
+    if (!lay->has_rvv)
+      return s->set(CG_EINVAL, "SGS_MODEL DSM_LOCAL needs the cells' Voronoi radii r_vv, which a mesh from raw "
+                               "arrays does not carry");
+    rc |= dev_alloc(s, h, &w.dsm, 16 * (size_t)ncv);
+    rc |= dev_alloc(s, h, &w.lijmij, ncv);
+    rc |= dev_alloc(s, h, &w.mijmij, ncv);
+  }
   rc |= dev_alloc(s, h, &w.rhs0, 5 * (size_t)ncv);
   rc |= dev_alloc(s, h, &w.rhs1, 5 * (size_t)ncv);
   rc |= dev_alloc(s, h, &w.cflux, 6 * (size_t)std::max(nfa, 1));
@@ -816,13 +857,14 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false;
   for (auto* I : E.inst) {
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
+    any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
   }
   E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr};
+         any_full_gr, any_dsm};
   E.dirty = false;
   return 0;
 }
@@ -884,6 +926,23 @@ int engine_steps(Err* s, Engine<R>& E, hipStream_t stream, int nsteps, bool use_
 // ---- one instance ----
 
 // initComplete / restore: primitives, gradient and properties of the current state
+// completeSgsStuff (IdealGasSolverSgs.cpp:50-74): a DSM_LOCAL solve starts its
+// relaxation from L:M = 0, M:M = 1e-12, unless a data file has just set them
+template <class R>
+int dsm_reset(Err* s, Instance<R>* I, hipStream_t stream) {
+  if (I->prm.sgs_model != CG_SGS_DSM_LOCAL) return 0;
+  if (I->dsm_kept) {
+    I->dsm_kept = false;
+    return 0;
+  }
+  const size_t n = (size_t)I->ncv;
+  const std::vector<double> lm(n, 0.0), mm(n, 1.0e-12);
+  HIPCHK(s, hipMemcpyAsync(I->b.w.lijmij, lm.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(s, hipMemcpyAsync(I->b.w.mijmij, mm.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
+  return 0;
+}
+
 template <class R>
 int refresh(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   const cgk::Launch<R> L = I->solo_launch();
@@ -1154,6 +1213,7 @@ int cg_params_default(cg_params* p) {
   p->sgs_model = CG_SGS_NONE;
   p->vreman_coeff = 0.07;
   p->sigma_coeff = 1.5;
+  p->delta_r_dsm = 2.0;
   p->a_sgs_coeff = 0.05;
   p->a_sgs_coeff2 = 0.0015;
   p->lambda_dil_coeff = 0.0;
@@ -1205,7 +1265,7 @@ int cg_params_parse(cg_params* p, const char* text) {
     struct { const char* k; double* d; } dkeys[] = {
         {"GAMMA", &p->gamma}, {"P_REF", &p->p_ref}, {"RHO_REF", &p->rho_ref}, {"T_REF", &p->T_ref},
         {"MU_REF", &p->mu_ref}, {"MU_POWER_LAW", &p->mu_power_law}, {"PR_LAM", &p->Pr_lam},
-        {"VREMAN_COEFF", &p->vreman_coeff}, {"SIGMA_COEFF", &p->sigma_coeff}, {"A_SGS_COEFF", &p->a_sgs_coeff}, {"A_SGS_COEFF2", &p->a_sgs_coeff2},
+        {"VREMAN_COEFF", &p->vreman_coeff}, {"SIGMA_COEFF", &p->sigma_coeff}, {"DELTA_R_DSM", &p->delta_r_dsm}, {"A_SGS_COEFF", &p->a_sgs_coeff}, {"A_SGS_COEFF2", &p->a_sgs_coeff2},
         {"LAMBDA_DIL_COEFF", &p->lambda_dil_coeff}, {"PR_T", &p->Pr_t}, {"COEFF_P", &p->coeff_p},
         {"COEFF_H", &p->coeff_h}, {"GR_NRM", &p->gr_nrm}, {"EF_OPS.INTERP_FACTOR", &p->interp_factor},
         {"SIGMA_THRESH", &p->sigma_thresh}};
@@ -1224,7 +1284,8 @@ int cg_params_parse(cg_params* p, const char* text) {
       if (m == "NONE") p->sgs_model = CG_SGS_NONE;
       else if (m == "VREMAN") p->sgs_model = CG_SGS_VREMAN;
       else if (m == "SIGMA") p->sgs_model = CG_SGS_SIGMA;
-      else return bad("unsupported SGS_MODEL " + m + " (NONE, VREMAN and SIGMA are available)");
+      else if (m == "DSM_LOCAL") p->sgs_model = CG_SGS_DSM_LOCAL;
+      else return bad("unsupported SGS_MODEL " + m + " (NONE, VREMAN, SIGMA and DSM_LOCAL are available)");
     } else if (key == "TIMESTEP") {
       if (tok.size() < 3) return bad("expected TIMESTEP <mode> <value> [value]");
       const std::string m = upper(tok[1]);
@@ -1428,6 +1489,11 @@ int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes) {
     itmp.assign(s8.begin(), s8.end());
     src = itmp.data();
     sz = itmp.size() * sizeof(int);
+  } else if (n == "r_vv") {
+    if (v.r_vv.size() == (size_t)v.ncv) dtmp = v.r_vv;
+    else dtmp.assign(v.ncv, 0.0);
+    src = dtmp.data();
+    sz = dtmp.size() * sizeof(double);
   } else if (n == "L") {
     dtmp.assign(v.L, v.L + 3);
     src = dtmp.data();
@@ -1578,6 +1644,7 @@ int cg_init_complete(cg_solver* s) {
     s->info_host->cfl_bits = 0;
     s->info_host->nan = 0;
     if (int rc = write_info(s, I, s->info_host, s->stream)) return rc;
+    if (int rc = dsm_reset(s, I, s->stream)) return rc;
     if (int rc = refresh(s, I, E.cur, s->stream)) return rc;
     return do_store(s, I, E.cur, s->info_host, s->stream);
   });
@@ -1611,7 +1678,7 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
       return get_soa(s, ncv, I->ext_of, f, 1, out, s->stream);
     }
     // per-step diagnostic fields: recompute the properties of the current state
-    if (const int le = cgk::launch_props(I->solo_launch(), E.cur, 1, s->stream)) return launch_fail(s, "props", le);
+    if (const int le = cgk::launch_props(I->solo_launch(), E.cur, 2, s->stream)) return launch_fail(s, "props", le);
     double* src = nullptr;
     if (nm == "T") src = w.T;
     else if (nm == "mu_lam") src = w.mu_lam;
@@ -1620,7 +1687,10 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
     else if (nm == "loc_sgs") src = w.loc_sgs;
     else if (nm == "a_sgs") src = w.a_sgs;
     else if (nm == "vv2") src = w.vv2;
-    else if (nm == "cfl") {
+    else if (nm == "lijmij" || nm == "mijmij") {
+      if (!w.lijmij) return s->set(CG_EINVAL, nm + " exists for SGS_MODEL DSM_LOCAL only");
+      src = nm == "lijmij" ? w.lijmij : w.mijmij;
+    } else if (nm == "cfl") {
       if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
       if (const int le = cgk::launch_cfl(I->solo_launch(), E.cur, s->info_host->dt, s->stream))
         return launch_fail(s, "cfl", le);
@@ -1712,6 +1782,17 @@ int cg_write_sles(cg_solver* s, const char* path) {
     d.rhoE = rhoE.data();
     d.p = p.data();
     d.T = T.data();
+    std::vector<double> lm, mm;
+    if (I->b.w.lijmij) {
+      lm.resize(n);
+      mm.resize(n);
+      double* f[1] = {I->b.w.lijmij};
+      if (int rc = get_soa(s, (int)n, I->ext_of, f, 1, lm.data(), s->stream)) return rc;
+      f[0] = I->b.w.mijmij;
+      if (int rc = get_soa(s, (int)n, I->ext_of, f, 1, mm.data(), s->stream)) return rc;
+      d.lijmij = lm.data();
+      d.mijmij = mm.data();
+    }
     const std::string e = cgmesh::write_sles(path, d);
     return e.empty() ? 0 : s->set(CG_EIO, e);
   });
@@ -1728,6 +1809,18 @@ int cg_read_sles(cg_solver* s, const char* path) {
     if (r.have_step) s->info_host->step = r.step;
     if (r.have_time) s->info_host->time = r.time;
     if (r.have_dt) s->info_host->dt = r.dt;
+    if (I->b.w.lijmij && !r.lijmij.empty() && !r.mijmij.empty()) {
+      // the registered DSM_LOCAL data (Sgs.hpp:7-14) in device cell order
+      const size_t n = (size_t)I->ncv;
+      std::vector<double> lm(n), mm(n);
+      for (size_t c = 0; c < n; ++c) {
+        lm[c] = r.lijmij[(size_t)I->ext_of[c]];
+        mm[c] = r.mijmij[(size_t)I->ext_of[c]];
+      }
+      HIPCHK(s, hipMemcpy(I->b.w.lijmij, lm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      HIPCHK(s, hipMemcpy(I->b.w.mijmij, mm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      I->dsm_kept = true;
+    }
     return write_info(s, I, s->info_host, s->stream);
   });
 }
@@ -1896,6 +1989,7 @@ int create_instance(cg_instance* x, std::shared_ptr<Layout<R>> lay, const cg_mes
   if (!rc) rc = make_instance_on(x, lay, p, stream, out);
   // the state goes to st[0]; attaching to a batch at parity 1 swaps the buffers
   if (!rc) rc = set_state(x, *out, 0, rho, u, rhoE, stream);
+  if (!rc) rc = dsm_reset(x, *out, stream);
   if (!rc) rc = refresh(x, *out, 0, stream);
   if (!rc) rc = do_store(x, *out, 0, host, stream);
   if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = x->set(CG_EHIP, "stream synchronize failed");

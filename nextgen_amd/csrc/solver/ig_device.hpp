@@ -90,6 +90,10 @@ struct GeomT {
   const uint8_t* gperm;
   const int *gr_ptr, *gr_deg, *gr_nbr;
   const R *gc0, *gc1, *gc2;
+  // DSM_LOCAL: the test-filter weight of each compact slot's face (FP64,
+  // |n_fa| |x_vv1 - x_vv0| / 6) and each cell's Voronoi radius r_vv
+  const double* cfw;
+  const double* r_vv;
 };
 
 // One buffer of the ping-pong state.
@@ -120,6 +124,11 @@ struct WorkT {
   StepInfo* info;
   double* partial;   // reduction scratch [nblocks][8]
   double* diag;      // [8]
+  // DSM_LOCAL only (null otherwise): the filtered fields [ncv][16] (rho, u, the
+  // model tensor smag rho S and rho u u, deviatoric-symmetric storage) and the
+  // time-relaxed L:M and M:M [ncv] carried from step to step
+  double* dsm;
+  double *lijmij, *mijmij;
 };
 
 struct Params {
@@ -131,6 +140,7 @@ struct Params {
   double p_g, h_g;  // mixed-precision gauges (p_ref and h at T_ref)
   double cp, rho_ref, inv_rhoref;  // full Gibbs-remainder sensor
   int full_gr;                     // USE_FULL_GR
+  double dsm_alpha;                // DELTA_R_DSM^2
 };
 
 using Geom = GeomT<double>;
@@ -174,6 +184,7 @@ struct Launch {
   int tile;          // extended slots fetched per batch in the LDS pass (2, 4 or 8)
   int sigma;         // some instance uses SGS_MODEL SIGMA (its SGS pass runs first)
   int full_gr;       // some instance uses USE_FULL_GR (its sensor pass runs first)
+  int dsm;           // some instance uses SGS_MODEL DSM_LOCAL (its two passes run first)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);
@@ -186,6 +197,9 @@ int launch_grad(const Launch<R>& L, int cur, void* stream);
 template <class R>
 int launch_eos(const Launch<R>& L, int cur, void* stream);
 template <class R>
+// fields: 0 the step's pass; 1 the same and the per-cell diagnostic copies (T,
+// mu_lam, ...); 2 the copies again for a read of the current state (DSM_LOCAL's
+// relaxation state is then left as it is)
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream);
 template <class R>
 int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream);

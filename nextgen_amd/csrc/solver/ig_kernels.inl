@@ -924,6 +924,242 @@ __global__ __launch_bounds__(kBlock) void k_sgs_sigma(const Inst<R>* __restrict_
   G(w.loc_sgs)[c] = mu * P.invPr_t;
 }
 
+// DSM_LOCAL (DynamicSmagorinsky::computeLijMij, Sgs.hpp:16-244, and
+// IdealGasSolver::computeSgsDsm, IdealGasSolverSgs.cpp:84-205), FP64 in either
+// precision, in two passes before the properties pass.
+//
+// The 16 per-cell quantities the test filter is applied to: rho, u, the model
+// tensor smag rho S_dev (diagonal, then the 12, 02, 01 entries; smag = sqrt(2 S:S)
+// of the symmetric part S of dudx before its trace is removed) and rho u u
+// (diagonal, then u1 u2, u0 u2, u0 u1).
+__device__ inline void dsm_q(double rho, const double u[3], const double d[9], double q[16]) {
+#pragma clang fp contract(off)
+  double S[3][3], smag = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      S[i][j] = 0.5 * (d[i * 3 + j] + d[j * 3 + i]);
+      smag += 2.0 * S[i][j] * S[i][j];
+    }
+  smag = sqrt(smag);
+  const double Skk = S[0][0] + S[1][1] + S[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) S[i][i] -= Skk / 3.0;
+  smag *= rho;
+  q[0] = rho;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    q[1 + i] = u[i];
+    q[4 + i] = smag * S[i][i];
+    q[10 + i] = rho * u[i] * u[i];
+  }
+  q[7] = smag * S[1][2];
+  q[8] = smag * S[0][2];
+  q[9] = smag * S[0][1];
+  q[13] = rho * u[1] * u[2];
+  q[14] = rho * u[0] * u[2];
+  q[15] = rho * u[0] * u[1];
+}
+
+// Pass 1: the test filter (filterCvR1 / filterCvR2, StaticSolver.cpp:9849-9975)
+// of the 16 quantities over the compact faces: each face adds (q0 + q1)/2 w and w
+// to both of its cells, here gathered per cell in pair order; then / sum w.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_dsm_filter(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                       int cur) {
+#pragma clang fp contract(off)
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Params& P = X.P;
+  if (P.sgs_model != 3) return;
+  const GeomT<R> g = X.g;
+  const StateT<R> st = X.st[cur];
+  const WorkT<R> w = X.w;
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.xperm, pos);
+  auto quantities = [&](int cc, double q[16]) {
+    const Light<R> L = load_light(st.light, cc);
+    const auto a = G(w.aux + (size_t)cc * A_N + A_D);
+    const double u[3] = {double(L.u[0]), double(L.u[1]), double(L.u[2])};
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d[k] = double(a[k]);
+    dsm_q(double(L.rho), u, d, q);
+  };
+  double qc[16], f[16];
+  quantities(c, qc);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) f[i] = 0.0;
+  double wgt = 0.0;
+  const int nc = G(g.ef_nc)[pos];
+  const int cbase = G(g.cf_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  for (int k = 0; k < nc; ++k) {
+    const int slot = cbase + k * kSlice;
+    const int nb = G(g.cf_nbr)[slot] & 0x7fffffff;
+    const double fax = G(g.cfw)[slot];
+    double qn[16];
+    quantities(nb, qn);
+    wgt += fax;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) f[i] += (0.5 * (qc[i] + qn[i])) * fax;
+  }
+  const auto out = G(w.dsm + (size_t)c * 16);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) out[i] = f[i] / wgt;
+}
+
+// Pass 2: the gradient of the filtered velocity (calcCvGrad over the gradient
+// stencil), L and M, the time relaxation of L:M and M:M (time scale 1/|S|; only
+// when dt > 0 and `relax`: a recompute of the diagnostic fields leaves them), then
+// c_delta^2 = -L:M / (2 M:M) clipped to [0, 0.5 dx^2] (the ceiling sets
+// c_delta^2 = 0.5 dx^2, as the reference does) and mu_sgs, a_sgs (with the
+// dilatational term on r_vv and limitAsgs) and loc_sgs from it.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_dsm_finish(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                       int cur, int relax) {
+#pragma clang fp contract(off)
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Params& P = X.P;
+  if (P.sgs_model != 3) return;
+  const GeomT<R> g = X.g;
+  const StateT<R> st = X.st[cur];
+  const WorkT<R> w = X.w;
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  const int c = cell_at(g.gperm, pos);
+  double f[16];
+  {
+    const auto fc = G(w.dsm + (size_t)c * 16);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) f[i] = fc[i];
+  }
+  double dh[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dh[k] = 0.0;
+  const int base = G(g.gr_ptr)[pos >> 6] + (pos & (kSlice - 1));
+  const int deg = G(g.gr_deg)[pos];
+  for (int k = 0; k < deg; ++k) {
+    const int slot = base + k * kSlice;
+    const int nb = G(g.gr_nbr)[slot];
+    const double gc[3] = {double(G(g.gc0)[slot]), double(G(g.gc1)[slot]), double(G(g.gc2)[slot])};
+    const auto un = G(w.dsm + (size_t)nb * 16 + 1);
+    const double u0 = un[0], u1 = un[1], u2 = un[2];
+    const double du[3] = {u0 - f[1], u1 - f[2], u2 - f[3]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dh[i * 3 + j] += gc[j] * du[i];
+  }
+  // L and M (Sgs.hpp:145-187)
+  double Sh[3][3], smag_hat = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Sh[i][j] = 0.5 * (dh[i * 3 + j] + dh[j * 3 + i]);
+      smag_hat += 2.0 * Sh[i][j] * Sh[i][j];
+    }
+  smag_hat = sqrt(smag_hat);
+  {
+    const double Skk = Sh[0][0] + Sh[1][1] + Sh[2][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Sh[i][i] -= Skk / 3.0;
+  }
+  const double rho_hat = f[0];
+  smag_hat *= rho_hat;
+  const double alpha = P.dsm_alpha;
+  double md[3], mo[3], ld[3], lo[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    md[i] = alpha * smag_hat * Sh[i][i] - f[4 + i];
+    ld[i] = f[10 + i] - rho_hat * f[1 + i] * f[1 + i];
+  }
+  mo[0] = alpha * smag_hat * Sh[1][2] - f[7];
+  mo[1] = alpha * smag_hat * Sh[0][2] - f[8];
+  mo[2] = alpha * smag_hat * Sh[0][1] - f[9];
+  lo[0] = f[13] - rho_hat * f[2] * f[3];
+  lo[1] = f[14] - rho_hat * f[1] * f[3];
+  lo[2] = f[15] - rho_hat * f[1] * f[2];
+  {
+    const double Lkk = ld[0] + ld[1] + ld[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ld[i] -= Lkk / 3.0;
+  }
+  // the cell's own strain rate
+  const auto a = G(w.aux + (size_t)c * A_N + A_D);
+  double d[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) d[k] = double(a[k]);
+  double smag = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double Sij = 0.5 * (d[i * 3 + j] + d[j * 3 + i]);
+      smag += 2.0 * Sij * Sij;
+    }
+  smag = sqrt(smag);
+  const double dt = G(w.info)->dt;
+  double lm_c = G(w.lijmij)[c], mm_c = G(w.mijmij)[c];
+  if (relax && dt > 0.0) {
+    // Sgs.hpp:191-232
+    const double T_relax = 1.0 / (smag + 1.0e-15);
+    const double epsln = dt / T_relax;
+    double lm = 0.0, mm = 1.0e-12;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      lm += ld[i] * md[i];
+      lm += 2.0 * lo[i] * mo[i];
+      mm += md[i] * md[i];
+      mm += 2.0 * mo[i] * mo[i];
+    }
+    const double x_lm = lm_c - 0.0 + epsln * lm;
+    const double rhs_lm = (x_lm < 0.0) ? x_lm : 0.0;
+    const double rhs_mm = mm_c - 0.0 + epsln * mm;
+    const double lhs_fax = 1.0 / (1.0 + epsln);
+    lm_c = rhs_lm * lhs_fax;
+    mm_c = rhs_mm * lhs_fax;
+    G(w.lijmij)[c] = lm_c;
+    G(w.mijmij)[c] = mm_c;
+  }
+  // IdealGasSolverSgs.cpp:121-190
+  double cdel2 = -0.5 * lm_c / (mm_c + 1.0e-12);
+  if (cdel2 < 0.0) cdel2 = 0.0;
+  const double dx2 = G(g.dx23)[c];
+  double c_smag = sqrt(cdel2 / dx2);
+  if (c_smag > 0.5) {
+    c_smag = 0.5;
+    cdel2 = c_smag * dx2;
+  }
+  const Light<R> self = load_light(st.light, c);
+  const double rho = double(self.rho), sos = double(self.sos);
+  const double mu = 1.0 * cdel2 * rho * smag;
+  const double inv_sos2 = 1.0 / (sos * sos);
+  double as = P.a_sgs_coeff * cdel2 * smag / (0.19 * 0.19 * dx2);
+  const double dil_ = d[0] + d[4] + d[8];
+  const double tmp_ = fabs(dil_) * G(g.r_vv)[c] / sos;
+  const double u0 = double(self.u[0]), u1 = double(self.u[1]), u2 = double(self.u[2]);
+  if (tmp_ > 0.01) {
+    const double Masq = (u0 * u0 + u1 * u1 + u2 * u2) * inv_sos2;
+    as += P.a_sgs_coeff2 * G(g.sa_to_vol)[c] * fmin(Masq, 10.0) * sos;
+  }
+  as *= inv_sos2;
+  if (P.limit_asgs && dt > 0.0) {
+    // limitAsgs (IdealGasSolverSgs.cpp:4-48)
+    const double cfl_visc = 1.0;
+    const double usq = (u0 * u0 + u1 * u1 + u2 * u2);
+    const double fax = P.invgm1 + 0.5 * usq * inv_sos2;
+    const double ceil_ = cfl_visc / (dt * fax * 4.0) * inv_sos2;
+    if (as > ceil_) as = ceil_;
+  }
+  G(w.mu_sgs)[c] = mu;
+  G(w.a_sgs)[c] = as;
+  G(w.loc_sgs)[c] = mu * P.invPr_t;
+}
+
 // setFgrCvsFull (IdealGasSolver.cpp:747-838), for the instances with
 // USE_FULL_GR: vv2 = the maximum over all extended pairs of |gr| with
 //   gr = (-(s1 - s0) + beta_avg (cp (T1 - T0) - v_avg (p1 - p0))) / Ts_avg,
@@ -1111,8 +1347,8 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
       if (a_sgs > ceil_) a_sgs = ceil_;
     }
     loc_sgs += mu_sgs * R(P.invPr_t);
-  } else if (P.sgs_model == 2) {
-    // Sigma: the coefficients of k_sgs_sigma, which ran just before
+  } else if (P.sgs_model >= 2) {
+    // Sigma, DSM_LOCAL: the coefficients of their own passes, which ran just before
     mu_sgs = R(G(w.mu_sgs)[c]);
     a_sgs = R(G(w.a_sgs)[c]);
     loc_sgs = R(G(w.loc_sgs)[c]);
@@ -1454,6 +1690,10 @@ template <class R>
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   if (L.sigma) hipLaunchKernelGGL(k_sgs_sigma<R>, dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+  if (L.dsm) {
+    hipLaunchKernelGGL(k_dsm_filter<R>, dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
+    hipLaunchKernelGGL(k_dsm_finish<R>, dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur, int(fields != 2));
+  }
   if (L.full_gr) {
     // the flux pass's LDS (6 values of R per halo cell) holds FullGr<R>
     if (L.lds) hipLaunchKernelGGL((k_sensor_full<R, true>), dim3(L.nblk), dim3(kBlock), L.lds, s, L.I, L.bmap, cur);

@@ -23,7 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # CHARLES_GPU_LIB selects another build of the same library (e.g. a tuning variant)
 LIB_PATH = os.environ.get("CHARLES_GPU_LIB", os.path.join(_HERE, "lib", "libcharles_gpu.so"))
 
-CG_SGS_NONE, CG_SGS_VREMAN, CG_SGS_SIGMA = 0, 1, 2
+CG_SGS_NONE, CG_SGS_VREMAN, CG_SGS_SIGMA, CG_SGS_DSM_LOCAL = 0, 1, 2, 3
 CG_DISS_SENSOR, CG_DISS_NONE, CG_DISS_ALL = 0, 1, 2
 CG_DT_CONSTANT, CG_DT_CFL, CG_DT_CFL_DT, CG_DT_CFL_T = 0, 1, 2, 3
 
@@ -42,7 +42,7 @@ class cg_params(C.Structure):
         ("diss_mode", C.c_int), ("dt_mode", C.c_int), ("dt_data", C.c_double * 2), ("check_interval", C.c_int),
         ("interp_factor", C.c_double), ("sigma_thresh", C.c_double), ("group_member_threshold", C.c_int),
         ("disable_grouping", C.c_int), ("zero_c_ef", C.c_int), ("precision", C.c_int),
-        ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int),
+        ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int), ("delta_r_dsm", C.c_double),
     ]
 
 
@@ -268,7 +268,8 @@ class Mesh:
              "cvocv_i": (self.ncv + 1,), "cvocv_v": (self.nnz,), "grad_coeff": (self.nnz, 3),
              "inv_vol": (self.ncv,), "sa_to_vol": (self.ncv,), "dcmag_hat": (self.ncv,), "cmag": (self.ncv,),
              "faocv_i": (self.ncv + 1,), "faocv_v": (2 * self.nfa,), "efocv_i": (self.ncv + 1,),
-             "efocv_v": (2 * self.nef,), "grad_flag": (self.ncv,), "L": (3,), "max_gcl": (1,)}
+             "efocv_v": (2 * self.nef,), "grad_flag": (self.ncv,), "L": (3,), "max_gcl": (1,),
+             "r_vv": (self.ncv,)}
         return n[name]
 
     def get(self, name):

@@ -152,6 +152,8 @@ struct og_solver {
   og_params p;
   double R_gas;
   double *rho, *u, *rhoE, *pr, *T, *sos, *mu_lam, *mu_sgs, *loc_lam, *loc_sgs, *a_sgs, *vv2, *dudx, *cfl;
+  double *lijmij, *mijmij; /* DSM_LOCAL relaxation state */
+  int dsm_kept;            /* set from a restart: the next og_init_complete keeps it */
   og_state* cv_light;
   og_aux* cv_compact;
   og_rhs *rhs0, *rhs1, *rhs2;
@@ -183,6 +185,8 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   s->vv2 = (double*)calloc(n, sizeof(double));
   s->cfl = (double*)calloc(n, sizeof(double));
   s->dudx = (double*)calloc(9 * (size_t)n, sizeof(double));
+  s->lijmij = (double*)calloc(n, sizeof(double));
+  s->mijmij = (double*)calloc(n, sizeof(double));
   s->cv_light = (og_state*)calloc(n, sizeof(og_state));
   s->cv_compact = (og_aux*)calloc(n, sizeof(og_aux));
   s->rhs0 = (og_rhs*)calloc(n, sizeof(og_rhs));
@@ -204,6 +208,7 @@ void og_destroy(og_solver* s) {
   free(s->mu_lam); free(s->mu_sgs); free(s->loc_lam); free(s->loc_sgs); free(s->a_sgs);
   free(s->vv2); free(s->cfl); free(s->dudx); free(s->cv_light); free(s->cv_compact);
   free(s->rhs0); free(s->rhs1); free(s->rhs2); free(s->store_buf); free(s->store_recent);
+  free(s->lijmij); free(s->mijmij);
   free(s);
 }
 
@@ -236,6 +241,8 @@ int og_get_field(const og_solver* s, const char* name, double* out) {
   else if (!strcmp(name, "a_sgs")) src = s->a_sgs;
   else if (!strcmp(name, "vv2")) src = s->vv2;
   else if (!strcmp(name, "cfl")) src = s->cfl;
+  else if (!strcmp(name, "lijmij")) src = s->lijmij;
+  else if (!strcmp(name, "mijmij")) src = s->mijmij;
   else if (!strcmp(name, "dudx")) { memcpy(out, s->dudx, sizeof(double) * 9 * (size_t)n); return 0; }
   if (src) { memcpy(out, src, sizeof(double) * n); return 0; }
   for (int i = 0; i < n; ++i) {
@@ -447,6 +454,184 @@ static void sgs_sigma(og_solver* s) {
   for (int icv = 0; icv < s->m.ncv; ++icv) s->loc_sgs[icv] = s->mu_sgs[icv] * invPr_t;
 }
 
+/* filterCvR1 / filterCvR2 (StaticSolver.cpp:9849-9975) on a periodic mesh: each
+ * compact face adds (phi0 + phi1)/2 w and w to both of its cells, in face order,
+ * then phif /= sum w; `phi` and `phif` are [ncv][ncomp] */
+static void filter_cv(const og_mesh* m, const double* phi, double* phif, int ncomp, double* wgt) {
+  for (int icv = 0; icv < m->ncv; ++icv) {
+    wgt[icv] = 0.0;
+    for (int k = 0; k < ncomp; ++k) phif[(size_t)icv * ncomp + k] = 0.0;
+  }
+  for (int ifa = 0; ifa < m->nfa; ++ifa) {
+    const int icv0 = m->lo[ifa], icv1 = m->hi[ifa];
+    const double fax = m->filt_w[ifa];
+    wgt[icv0] += fax;
+    wgt[icv1] += fax;
+    for (int k = 0; k < ncomp; ++k) {
+      const double c_avg = 0.5 * (phi[(size_t)icv0 * ncomp + k] + phi[(size_t)icv1 * ncomp + k]);
+      phif[(size_t)icv0 * ncomp + k] += c_avg * fax;
+      phif[(size_t)icv1 * ncomp + k] += c_avg * fax;
+    }
+  }
+  for (int icv = 0; icv < m->ncv; ++icv)
+    for (int k = 0; k < ncomp; ++k) phif[(size_t)icv * ncomp + k] /= wgt[icv];
+}
+
+void og_filter_cv(const og_mesh* m, const double* phi, double* phif, int ncomp) {
+  double* wgt = (double*)malloc(sizeof(double) * m->ncv);
+  filter_cv(m, phi, phif, ncomp, wgt);
+  free(wgt);
+}
+
+void og_set_dsm_state(og_solver* s, const double* lijmij, const double* mijmij) {
+  memcpy(s->lijmij, lijmij, sizeof(double) * s->m.ncv);
+  memcpy(s->mijmij, mijmij, sizeof(double) * s->m.ncv);
+  s->dsm_kept = 1;
+}
+
+void og_set_time(og_solver* s, int64_t step, double time, double dt) {
+  s->step = step;
+  s->time = time;
+  s->dt = dt;
+}
+
+/* strain-rate magnitude sqrt(2 S:S) of one cell's dudx (row-major) and the
+ * deviatoric S (the order of DynamicSmagorinsky::computeLijMij, Sgs.hpp:71-85) */
+static double strain_dev(const double* d, double S[3][3]) {
+  double smag = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      S[i][j] = 0.5 * (d[i * 3 + j] + d[j * 3 + i]);
+      smag += 2.0 * S[i][j] * S[i][j];
+    }
+  smag = sqrt(smag);
+  const double Skk = S[0][0] + S[1][1] + S[2][2];
+  for (int i = 0; i < 3; ++i) S[i][i] -= Skk / 3.0;
+  return smag;
+}
+
+/* DSM_LOCAL: DynamicSmagorinsky::computeLijMij (Sgs.hpp:16-244) and
+ * IdealGasSolver::computeSgsDsm (IdealGasSolverSgs.cpp:84-205). The test filter
+ * of rho, u, the model tensor smag rho S (mij) and rho u u (lij); the gradient of
+ * the filtered u; L and M; the time relaxation of L:M and M:M with the inverse
+ * strain rate as time scale (only when dt > 0); then c_delta^2 = -L:M / (2 M:M)
+ * clipped to [0, 0.5 dx^2] and the coefficients from it. */
+static void sgs_dsm(og_solver* s) {
+  const og_mesh* m = &s->m;
+  const og_params* P = &s->p;
+  const int n = m->ncv;
+  const double alpha = P->delta_r_dsm * P->delta_r_dsm;
+  double* wgt = (double*)malloc(sizeof(double) * n);
+  double* in = (double*)calloc(16 * (size_t)n, sizeof(double));
+  double* out = (double*)malloc(sizeof(double) * 16 * (size_t)n);
+  /* per cell: rho, u[3], lij_d[3] = smag rho S_ii, lij_od[3] = smag rho (S12, S02, S01),
+   * tmp_d[3] = rho u_i u_i, tmp_od[3] = rho (u1 u2, u0 u2, u0 u1) */
+  for (int icv = 0; icv < n; ++icv) {
+    double* q = &in[16 * (size_t)icv];
+    const double rho = s->rho[icv];
+    const double* u = &s->u[3 * (size_t)icv];
+    double S[3][3];
+    const double smag = strain_dev(&s->dudx[9 * (size_t)icv], S) * rho;
+    q[0] = rho;
+    for (int i = 0; i < 3; ++i) q[1 + i] = u[i];
+    for (int i = 0; i < 3; ++i) q[4 + i] = smag * S[i][i];
+    q[7] = smag * S[1][2];
+    q[8] = smag * S[0][2];
+    q[9] = smag * S[0][1];
+    for (int i = 0; i < 3; ++i) q[10 + i] = rho * u[i] * u[i];
+    q[13] = rho * u[1] * u[2];
+    q[14] = rho * u[0] * u[2];
+    q[15] = rho * u[0] * u[1];
+  }
+  filter_cv(m, in, out, 16, wgt); /* each component filtered alone, as the reference's separate calls */
+  /* gradient of the filtered velocity (calcCvGrad, StaticSolver.cpp:5535-5557) */
+  double* dudx_hat = (double*)calloc(9 * (size_t)n, sizeof(double));
+  for (int icv = 0; icv < n; ++icv) {
+    double* g = &dudx_hat[9 * (size_t)icv];
+    const double* ui = &out[16 * (size_t)icv + 1];
+    for (int coc = m->cvocv_i[icv] + 1; coc != m->cvocv_i[icv + 1]; ++coc) {
+      const double* un = &out[16 * (size_t)m->cvocv_v[coc] + 1];
+      const double* gc = &m->grad_coeff[3 * (size_t)coc];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) g[i * 3 + j] += gc[j] * (un[i] - ui[i]);
+    }
+  }
+  for (int icv = 0; icv < n; ++icv) {
+    const double* f = &out[16 * (size_t)icv];
+    const double rho_hat = f[0];
+    const double* u_hat = &f[1];
+    double Sh[3][3];
+    const double smag_hat = strain_dev(&dudx_hat[9 * (size_t)icv], Sh) * rho_hat;
+    double mij_d[3], mij_od[3], lij_d[3], lij_od[3];
+    for (int i = 0; i < 3; ++i) mij_d[i] = alpha * smag_hat * Sh[i][i] - f[4 + i];
+    mij_od[0] = alpha * smag_hat * Sh[1][2] - f[7];
+    mij_od[1] = alpha * smag_hat * Sh[0][2] - f[8];
+    mij_od[2] = alpha * smag_hat * Sh[0][1] - f[9];
+    for (int i = 0; i < 3; ++i) lij_d[i] = f[10 + i] - rho_hat * u_hat[i] * u_hat[i];
+    lij_od[0] = f[13] - rho_hat * u_hat[1] * u_hat[2];
+    lij_od[1] = f[14] - rho_hat * u_hat[0] * u_hat[2];
+    lij_od[2] = f[15] - rho_hat * u_hat[0] * u_hat[1];
+    const double Lkk = lij_d[0] + lij_d[1] + lij_d[2];
+    for (int i = 0; i < 3; ++i) lij_d[i] -= Lkk / 3.0;
+    if (s->dt > 0.0) {
+      double S[3][3], smag = 0.0;
+      const double* d = &s->dudx[9 * (size_t)icv];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          S[i][j] = 0.5 * (d[i * 3 + j] + d[j * 3 + i]);
+          smag += 2.0 * S[i][j] * S[i][j];
+        }
+      smag = sqrt(smag) + 1.0e-15;
+      const double T_relax = 1.0 / smag;
+      const double epsln = s->dt / T_relax;
+      double lm = 0.0, mm = 1.0e-12;
+      for (int i = 0; i < 3; ++i) {
+        lm += lij_d[i] * mij_d[i];
+        lm += 2.0 * lij_od[i] * mij_od[i];
+        mm += mij_d[i] * mij_d[i];
+        mm += 2.0 * mij_od[i] * mij_od[i];
+      }
+      const double x_lm = s->lijmij[icv] - 0.0 + epsln * lm;
+      const double rhs_lm = (x_lm < 0.0) ? x_lm : 0.0; /* std::min(0.0, x) */
+      const double rhs_mm = s->mijmij[icv] - 0.0 + epsln * mm;
+      const double lhs_fax = 1.0 / (1.0 + epsln);
+      s->lijmij[icv] = rhs_lm * lhs_fax;
+      s->mijmij[icv] = rhs_mm * lhs_fax;
+    }
+  }
+  const double invPr_t = 1.0 / P->Pr_t;
+  for (int icv = 0; icv < n; ++icv) {
+    const double* d = &s->dudx[9 * (size_t)icv];
+    double S[3][3];
+    const double smag = strain_dev(d, S);
+    double cdel2 = -0.5 * s->lijmij[icv] / (s->mijmij[icv] + 1.0e-12);
+    if (cdel2 < 0.0) cdel2 = 0.0;
+    const double dx2 = pow(m->vol[icv], 2.0 / 3.0);
+    double c_smag = sqrt(cdel2 / dx2);
+    if (c_smag > 0.5) {
+      c_smag = 0.5;
+      cdel2 = c_smag * dx2;
+    }
+    s->mu_sgs[icv] = 1.0 * cdel2 * s->rho[icv] * smag;
+    const double inv_sos2 = 1.0 / (s->sos[icv] * s->sos[icv]);
+    s->a_sgs[icv] = P->a_sgs_coeff * cdel2 * smag / (0.19 * 0.19 * dx2);
+    const double dil_ = d[0] + d[4] + d[8];
+    const double tmp_ = fabs(dil_) * m->r_vv[icv] / s->sos[icv];
+    if (tmp_ > 0.01) {
+      const double* u = &s->u[3 * (size_t)icv];
+      const double Masq = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * inv_sos2;
+      s->a_sgs[icv] += P->a_sgs_coeff2 * m->sa_to_vol[icv] * fmin(Masq, 10.0) * s->sos[icv];
+    }
+    s->a_sgs[icv] *= inv_sos2;
+  }
+  limit_asgs(s);
+  for (int icv = 0; icv < n; ++icv) s->loc_sgs[icv] = s->mu_sgs[icv] * invPr_t;
+  free(wgt);
+  free(in);
+  free(out);
+  free(dudx_hat);
+}
+
 /* IdealGasSolverSgs.cpp:76-82 */
 static void sgs_none(og_solver* s) {
   for (int icv = 0; icv < s->m.ncv; ++icv) {
@@ -528,6 +713,7 @@ void og_sgs_props(og_solver* s) {
   const og_params* P = &s->p;
   if (P->sgs_model == 1) sgs_vreman(s);
   else if (P->sgs_model == 2) sgs_sigma(s);
+  else if (P->sgs_model == 3) sgs_dsm(s);
   else sgs_none(s);
   const double coeff_p_delta = P->coeff_p;
   const double coeff_h_delta = P->coeff_h;
@@ -670,6 +856,15 @@ static void set_dt(og_solver* s) {
 }
 
 void og_init_complete(og_solver* s) {
+  /* completeSgsStuff (IdealGasSolverSgs.cpp:50-74): a fresh DSM_LOCAL relaxation
+   * state unless a restart has set it */
+  if (!s->dsm_kept) {
+    for (int icv = 0; icv < s->m.ncv; ++icv) {
+      s->lijmij[icv] = 0.0;
+      s->mijmij[icv] = 1.0e-12;
+    }
+  }
+  s->dsm_kept = 0;
   og_update_state(s);
   og_sgs_props(s);
   og_store(s);

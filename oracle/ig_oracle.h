@@ -64,11 +64,16 @@ typedef struct og_mesh {
   const int* cvocv_i;       /* [ncv+1] */
   const int* cvocv_v;       /* [nnz] */
   const double* grad_coeff; /* [nnz][3] */
+  /* DSM_LOCAL only (may be NULL otherwise): the test-filter weight of each
+   * compact face, |n_fa| |x_vv1 - x_vv0| / 6 (x_vv1 the periodic image), and
+   * the Voronoi radius of each cell */
+  const double* filt_w;     /* [nfa] */
+  const double* r_vv;       /* [ncv] */
 } og_mesh;
 
 typedef struct og_params {
   double gamma, p_ref, rho_ref, T_ref, mu_ref, mu_power_law, Pr_lam;
-  int sgs_model; /* 0 NONE, 1 VREMAN, 2 SIGMA */
+  int sgs_model; /* 0 NONE, 1 VREMAN, 2 SIGMA, 3 DSM_LOCAL */
   double vreman_coeff, a_sgs_coeff, a_sgs_coeff2, lambda_dil_coeff, Pr_t;
   int limit_asgs;
   double coeff_p, coeff_h, gr_nrm;
@@ -78,6 +83,7 @@ typedef struct og_params {
   int check_interval;
   double sigma_coeff; /* SIGMA_COEFF */
   int full_gr;        /* USE_FULL_GR */
+  double delta_r_dsm; /* DELTA_R_DSM */
 } og_params;
 
 typedef struct og_diag {
@@ -122,6 +128,12 @@ int og_step(og_solver* s);
 void og_diag_get(og_solver* s, og_diag* d);
 void og_store(og_solver* s);
 void og_restore(og_solver* s);
+/* the DSM_LOCAL test filter alone (phi, phif [ncv][ncomp]; needs m->filt_w) */
+void og_filter_cv(const og_mesh* m, const double* phi, double* phif, int ncomp);
+/* DSM_LOCAL L:M and M:M as a restart read sets them (kept by the next og_init_complete) */
+void og_set_dsm_state(og_solver* s, const double* lijmij, const double* mijmij);
+/* step counter, time and dt, as a restart read sets them */
+void og_set_time(og_solver* s, int64_t step, double time, double dt);
 
 #ifdef __cplusplus
 }

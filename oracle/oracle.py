@@ -20,7 +20,8 @@ REF_SO = os.path.join(HERE, "_ref", "libref_pin.so")
 class og_mesh(C.Structure):
     _fields_ = [("ncv", C.c_int), ("nfa", C.c_int), ("nef", C.c_int)] + [
         (n, C.c_void_p) for n in ("vol", "inv_vol", "sa_to_vol", "dcmag_hat", "lo", "hi", "n_use", "c_use",
-                                  "unit_n_use", "area_use", "aod_use", "n_fa", "cvocv_i", "cvocv_v", "grad_coeff")
+                                  "unit_n_use", "area_use", "aod_use", "n_fa", "cvocv_i", "cvocv_v", "grad_coeff",
+                                  "filt_w", "r_vv")
     ]
 
 
@@ -32,7 +33,7 @@ class og_params(C.Structure):
         ("lambda_dil_coeff", C.c_double), ("Pr_t", C.c_double), ("limit_asgs", C.c_int), ("coeff_p", C.c_double),
         ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
         ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
-        ("full_gr", C.c_int),
+        ("full_gr", C.c_int), ("delta_r_dsm", C.c_double),
     ]
 
 
@@ -79,6 +80,12 @@ def lib():
         L.og_sigma_dsigma.restype = C.c_double
         L.og_sigma_dsigma.argtypes = [dp]
         L.og_fast_pow_posx.argtypes = [C.c_double, C.c_double]
+        L.og_filter_cv.argtypes = [C.POINTER(og_mesh), dp, dp, C.c_int]
+        L.og_filter_cv.restype = None
+        L.og_set_time.argtypes = [vp, C.c_int64, C.c_double, C.c_double]
+        L.og_set_time.restype = None
+        L.og_set_dsm_state.argtypes = [vp, dp, dp]
+        L.og_set_dsm_state.restype = None
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
         _lib = L
@@ -105,6 +112,24 @@ def ref():
 
 def dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def filter_weights(mesh):
+    """Test-filter weight of each compact face, filterCvR1/R2
+    (StaticSolver.cpp:9849-9975): |n_fa| |x_vv1 - x_vv0| / 6 with x_vv1 the
+    periodic image of the face's second cell (its translation, x + s L, rounded
+    once as PeriodicData::periodicTranslate does); MAG and DIST summed left to
+    right (Macros.hpp:139,253; pow(x, 2) is taken as x * x)."""
+    x_vv = mesh.get("x_vv")
+    cvofa = mesh.get("cvofa")
+    shift = mesh.get("fa_shift").astype(np.float64)
+    L = mesh.get("L")
+    n = mesh.get("n_fa")
+    x1 = x_vv[cvofa[:, 1]] + shift * L
+    d = x1 - x_vv[cvofa[:, 0]]
+    dist = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    area = np.sqrt((n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1]) + n[:, 2] * n[:, 2])
+    return np.ascontiguousarray(area * dist / 6.0)
 
 
 def params_from(cgp) -> og_params:
@@ -135,6 +160,9 @@ class OracleSolver:
             a = np.ascontiguousarray(mesh.get(arr))
             self._keep[field] = a
             setattr(m, field, a.ctypes.data)
+        for field, a in (("filt_w", filter_weights(mesh)), ("r_vv", np.ascontiguousarray(mesh.get("r_vv")))):
+            self._keep[field] = a
+            setattr(m, field, a.ctypes.data)
         self._m = m
         self._p = params_from(cgparams)
         self.ncv = mesh.ncv
@@ -159,6 +187,24 @@ class OracleSolver:
 
     def init_complete(self):
         lib().og_init_complete(self._h)
+
+    def set_time(self, step, time, dt):
+        lib().og_set_time(self._h, int(step), float(time), float(dt))
+
+    def set_dsm_state(self, lijmij, mijmij):
+        lm = np.ascontiguousarray(lijmij, dtype=np.float64)
+        mm = np.ascontiguousarray(mijmij, dtype=np.float64)
+        lib().og_set_dsm_state(self._h, dptr(lm), dptr(mm))
+
+    def sgs_props(self):
+        lib().og_sgs_props(self._h)
+
+    def filter_cv(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        ncomp = 1 if phi.ndim == 1 else phi.shape[1]
+        out = np.empty_like(phi)
+        lib().og_filter_cv(C.byref(self._m), dptr(phi), dptr(out), ncomp)
+        return out
 
     def calc_rhs(self):
         out = np.empty((self.ncv, 5))

@@ -71,7 +71,22 @@ def test_sigma_model_parsing():
     assert ig.parse_params("USE_FULL_GR\n").use_full_gr == 1
 
 
-@pytest.mark.parametrize("line", ["SGS_MODEL DSM_LOCAL", "EF_OPS ALIGNED", "DISS_SPONGE", "TIMESTEP FOO 1",
+def test_dsm_model_parsing():
+    p = ig.parse_params("SGS_MODEL DSM_LOCAL\nDELTA_R_DSM 1.8\n")
+    assert p.sgs_model == ig.CG_SGS_DSM_LOCAL and p.delta_r_dsm == 1.8
+    assert ig.default_params().delta_r_dsm == 2.0  # Sgs.hpp:20
+
+
+def test_dsm_needs_voronoi_radii_of_the_mesh():
+    # a mesh from raw arrays carries no Voronoi radii: r_vv reads as zeros (a
+    # DSM_LOCAL solver on it is refused, tests/test_gpu_parity.py)
+    m = ig.Mesh.lattice(6)
+    r = ig.Mesh.from_arrays(m.get("x_cv"), m.get("vol"), m.get("cvofa"), m.get("fa_shift"), m.get("n_fa"),
+                            m.get("x_fa"), m.get("L"))
+    assert not r.get("r_vv").any() and m.get("r_vv").all()
+
+
+@pytest.mark.parametrize("line", ["SGS_MODEL DSM_GLOBAL", "EF_OPS ALIGNED", "DISS_SPONGE", "TIMESTEP FOO 1",
                                   "GAMMA abc", "CHECK_INTERVAL 0"])
 def test_param_errors(line):
     with pytest.raises(ig.CharlesError) as e:

@@ -208,10 +208,10 @@ def test_solves_sharing_a_layout_match_solo_runs():
 
 @pytest.mark.parametrize("precision", [CG_PREC_FP64, CG_PREC_MIXED])
 def test_batch_with_mixed_sgs_models_matches_solo_runs(precision):
-    # NONE, VREMAN and SIGMA solves in one batch: the Sigma pass runs for the
-    # batch and leaves the other instances alone
+    # NONE, VREMAN, SIGMA and DSM_LOCAL solves in one batch: the Sigma and DSM
+    # passes run for the batch and leave the other instances alone
     prec = "MIXED" if precision == CG_PREC_MIXED else "FP64"
-    models = ["SIGMA", "VREMAN", "NONE", "SIGMA"]
+    models = ["SIGMA", "VREMAN", "NONE", "DSM_LOCAL"]
 
     def make(i):
         n, seed, ic, Re, Ma = CASES[i]

@@ -37,7 +37,7 @@ def test_driver_runs_deck_and_matches_oracle(tmp_path):
 
 def test_driver_reports_bad_deck(tmp_path):
     deck = tmp_path / "bad.in"
-    deck.write_text("SGS_MODEL DSM_LOCAL\nMESH LATTICE 8\n")
+    deck.write_text("SGS_MODEL DSM_GLOBAL\nMESH LATTICE 8\n")
     out = subprocess.run([EXE, "-i", str(deck)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1
     assert "SGS_MODEL" in out.stderr

@@ -85,6 +85,24 @@ def test_mixed_sigma_sgs_close_to_oracle():
     assert np.all(err <= RHS_TOL), err
 
 
+def test_mixed_dsm_local_close_to_oracle():
+    # DSM_LOCAL runs in FP64 from the FP32 records and gradient: after ten steps
+    # its relaxed L:M and eddy viscosity follow the oracle's to the FP32 level
+    m = Mesh.jittered(12, 0.3, seed=11)
+    state = m.ic_k4(2.0, 0.3, 1.4, seed=5)
+    text = k4_params(sgs="DSM_LOCAL")
+    g = solver(m, parse_params(text + "PRECISION MIXED\n"), state)
+    o = oracle(m, state, text)
+    g.advance_solution(10)
+    o.step(10)
+    for name in ("lijmij", "mu_sgs"):
+        a, b = g.get_field(name), o.get_field(name)
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"k4 DSM_LOCAL mixed {name}: {err:.3e}")
+        assert err <= 1e-3, name
+    assert np.max(o.get_field("mu_sgs")) > 0.0
+
+
 def test_mixed_full_gibbs_sensor_close_to_oracle():
     m, p, state = setup("k4", "MIXED")
     text = k4_params() + "USE_FULL_GR\n"

@@ -36,11 +36,11 @@ def test_solve_on_mesh_read_from_file_is_identical(tmp_path, prec):
     assert (d0.step, d0.time, d0.dt, d0.ke) == (d1.step, d1.time, d1.dt, d1.ke)
 
 
-@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
-def test_restart_from_data_file_continues_bitwise(tmp_path, prec):
+@pytest.mark.parametrize("prec,sgs", [("FP64", "VREMAN"), ("MIXED", "VREMAN"), ("FP64", "SIGMA")])
+def test_restart_from_data_file_continues_bitwise(tmp_path, prec, sgs):
     m = Mesh.jittered(12, 0.3, seed=6)
     path = str(tmp_path / "result.00000005.sles")
-    a = _solver(m, prec)
+    a = _solver(m, prec, sgs)
     a.set_state(*m.ic_tgv(0.2, 1.4))
     a.init_complete()
     a.advance_solution(5)
@@ -50,7 +50,7 @@ def test_restart_from_data_file_continues_bitwise(tmp_path, prec):
     ref_state, ref = a.get_state(), a.report()
     a.close()
 
-    b = _solver(m, prec)
+    b = _solver(m, prec, sgs)
     b.read_sles(path)
     b.init_complete()
     d = b.report()
@@ -63,6 +63,43 @@ def test_restart_from_data_file_continues_bitwise(tmp_path, prec):
     for x, y in zip(got_state, ref_state):
         np.testing.assert_array_equal(x, y)
     assert got.ke == ref.ke and got.enstrophy == ref.enstrophy
+
+
+def test_dsm_restart_carries_its_relaxation_state(tmp_path):
+    # DSM_LOCAL's L:M and M:M are registered data (Sgs.hpp:7-14): the data file
+    # carries them and the restart's initComplete relaxes them once more with the
+    # file's dt, as the reference's does (completeSgsStuff keeps data that was
+    # read, calcSgsAndMaterialProperties then relaxes); the oracle given the same
+    # state, counters and L:M, M:M agrees
+    from oracle.oracle import OracleSolver
+    m = Mesh.jittered(12, 0.3, seed=6)
+    path = str(tmp_path / "dsm.sles")
+    a = _solver(m, "FP64", "DSM_LOCAL")
+    a.set_state(*m.ic_tgv(0.2, 1.4))
+    a.init_complete()
+    a.advance_solution(5)
+    a.write_sles(path)
+    lm5, mm5, st5, d5 = a.get_field("lijmij"), a.get_field("mijmij"), a.get_state(), a.report()
+    a.close()
+    b = _solver(m, "FP64", "DSM_LOCAL")
+    b.read_sles(path)
+    b.init_complete()
+    assert np.min(lm5) < 0.0
+    p = parse_params(tgv_params(Re=800.0, Ma=0.2, sgs="DSM_LOCAL"))
+    o = OracleSolver(m, p)
+    o.set_state(*st5)
+    o.set_time(d5.step, d5.time, d5.dt)
+    o.set_dsm_state(lm5, mm5)
+    o.init_complete()
+    for name in ("lijmij", "mijmij", "mu_sgs"):
+        x, y = b.get_field(name), o.get_field(name)
+        assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), name
+    assert not np.array_equal(b.get_field("lijmij"), lm5)  # relaxed once more
+    b.advance_solution(4)
+    o.step(4)
+    for x, y in zip(b.get_state(), o.get_state()):
+        assert np.max(np.abs(x - y)) <= 1e-11 * np.max(np.abs(y))
+    b.close()
 
 
 def test_data_file_of_another_mesh_is_rejected(tmp_path):
