@@ -155,7 +155,7 @@ int cg_mesh_from_arrays(int ncv, const double* x_cv, const double* vol, int nfa,
  *   with boundary faces is rejected (CG_EMESH): this solver has no boundary
  *   conditions.
  * cg_mesh_write_mles: a periodic mesh as a geometry-only file (counts, periodic
- *   transforms, cvofa, one fluid cell zone, vol_cv, x_vv, x_cv, n_fa, x_fa;
+ *   transforms, cvofa, one fluid cell zone, vol_cv, x_vv, r_vv, x_cv, n_fa, x_fa;
  *   faces in zone order as StitchNS::writeMles, StitchNS.cpp:8122, but no node
  *   records) that cg_mesh_read_mles reads back to the same mesh. */
 int cg_mles_counts(const char* path, int64_t counts[8]);

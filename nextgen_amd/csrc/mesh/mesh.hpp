@@ -114,7 +114,7 @@ std::string read_mles(const std::string& path, Mles& out);
 // and no boundary faces; face zones give the periodic image of cvofa[f][1].
 std::string voronoi_from_mles(const Mles& f, Voronoi& m);
 // Write a periodic-box mesh as a geometry-only .mles file (counts, periodic
-// transforms, cvofa, cell zone, vol_cv, x_vv, x_cv, n_fa, x_fa; no node records).
+// transforms, cvofa, cell zone, vol_cv, x_vv, r_vv, x_cv, n_fa, x_fa; no node records).
 std::string write_mles(const Voronoi& m, const std::string& path);
 
 // ---- .sles data files of the ideal-gas state (mles.cpp) ----
