@@ -44,6 +44,8 @@ extern "C" {
 #define CG_SGS_SIGMA 2
 #define CG_SGS_DSM_LOCAL 3
 
+#define CG_MAX_SCALARS 8
+
 /* NO_DISS / ALL_DISS overrides of the Gibbs-remainder sensor (IdealGasSolver.cpp:973-979) */
 #define CG_DISS_SENSOR 0
 #define CG_DISS_NONE 1
@@ -104,6 +106,12 @@ typedef struct cg_params {
   double sigma_coeff;      /* SIGMA_COEFF (1.5), IdealGasSolverSgs.cpp:300 */
   int use_full_gr;         /* USE_FULL_GR: the full Gibbs-remainder sensor, IdealGasSolver.cpp:244-247 */
   double delta_r_dsm;      /* DELTA_R_DSM (2.0): test-to-grid filter width ratio of DSM_LOCAL, Sgs.hpp:20 */
+  /* passive scalars (FlowSolver.hpp:531-549, 1465-1496): REGISTER_SCALAR names and,
+   * with the key RES_TIME, the residence-time scalar (source rho, IdealGasSolver.cpp:675-688),
+   * in name order as the reference's scalar map; at most CG_MAX_SCALARS */
+  int nsc;
+  int res_time;            /* index of RES_TIME in sc_names, or -1 */
+  char sc_names[8][32];
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,
@@ -245,6 +253,12 @@ int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms)
  * IdealGasSolver.hpp:96-100); call cg_init_complete next, which recomputes p
  * and T from the state. The cell count must match the solver's mesh. */
 int cg_write_sles(cg_solver* s, const char* path);
+/* A passive scalar by its registered name (FlowSolver::_registerScalar), [ncv]
+ * in the mesh's cell order: the transported phi (rho phi is conserved). Scalars
+ * start at zero (FlowSolver.hpp:590); set them before cg_init_complete or between
+ * steps. */
+int cg_set_scalar(cg_solver* s, const char* name, const double* phi);
+int cg_get_scalar(cg_solver* s, const char* name, double* phi);
 int cg_read_sles(cg_solver* s, const char* path);
 const char* cg_last_error(const cg_solver* s);
 void cg_destroy(cg_solver* s);

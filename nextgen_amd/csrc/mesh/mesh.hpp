@@ -11,6 +11,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace cgmesh {
@@ -124,12 +125,15 @@ struct SlesData {
   double time = 0.0, dt = 0.0;
   const double *rho = nullptr, *u = nullptr, *rhoE = nullptr, *p = nullptr, *T = nullptr;  // p, T optional
   const double *lijmij = nullptr, *mijmij = nullptr;  // DSM_LOCAL relaxation state (optional)
+  std::vector<std::string> sc_names;                  // passive scalars by name (optional)
+  std::vector<const double*> sc;
 };
 struct SlesRead {
   int step = 0;
   double time = 0.0, dt = 0.0;
   bool have_step = false, have_time = false, have_dt = false;
   std::vector<double> rho, u, rhoE, p, T, lijmij, mijmij;
+  std::vector<std::pair<std::string, std::vector<double>>> other;  // other per-cell records by name
 };
 std::string write_sles(const std::string& path, const SlesData& s);
 std::string read_sles(const std::string& path, int64_t ncv, SlesRead& out);

@@ -547,6 +547,8 @@ std::string write_sles(const std::string& path, const SlesData& s) {
   if (s.T) w.put_array("T", kCvD1, s.T, ncv, 1);
   if (s.lijmij) w.put_array("lijmij", kCvD1, s.lijmij, ncv, 1);
   if (s.mijmij) w.put_array("mijmij", kCvD1, s.mijmij, ncv, 1);
+  for (size_t i = 0; i < s.sc_names.size() && i < s.sc.size(); ++i)
+    w.put_array(s.sc_names[i].c_str(), kCvD1, s.sc[i], ncv, 1);
   w.put_header(w.header("EOF", kEof, 0));
   if (!w.close()) return "write failed: " + path;
   return "";
@@ -593,6 +595,10 @@ std::string read_sles(const std::string& path, int64_t ncv, SlesRead& s) {
       if (h.id == kCvD1 && name == "T") dst = &s.T;
       if (h.id == kCvD1 && name == "lijmij") dst = &s.lijmij;
       if (h.id == kCvD1 && name == "mijmij") dst = &s.mijmij;
+      if (h.id == kCvD1 && !dst) {
+        s.other.emplace_back(name, std::vector<double>());
+        dst = &s.other.back().second;
+      }
       if (dst) {
         const int width = h.id == kCvD1 ? 1 : 3;
         int64_t n = 0;

@@ -144,7 +144,7 @@ struct Instance : HostInst {
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
-            prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL};
+            prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc > 0};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -316,6 +316,8 @@ void fill_dev_params(HostInst* h) {
   // full Gibbs-remainder sensor (setFgrCvsFull, IdealGasSolver.cpp:755-761)
   P.full_gr = p.use_full_gr;
   P.dsm_alpha = p.delta_r_dsm * p.delta_r_dsm;
+  P.nsc = p.nsc;
+  P.res_time = p.res_time;
   P.cp = P.R_gas * p.gamma / (p.gamma - 1.0);
   P.rho_ref = p.rho_ref;
   P.inv_rhoref = 1.0 / p.rho_ref;
@@ -334,6 +336,8 @@ int check_params(const cg_params* p) {
       p->sgs_model != CG_SGS_DSM_LOCAL)
     return fail(CG_EINVAL, "unsupported SGS_MODEL");
   if (p->dt_mode < 0 || p->dt_mode > 3) return fail(CG_EINVAL, "bad TIMESTEP mode");
+  if (p->nsc < 0 || p->nsc > CG_MAX_SCALARS || p->res_time < -1 || p->res_time >= std::max(p->nsc, 1))
+    return fail(CG_EINVAL, "bad passive scalar count or RES_TIME index");
   if (p->precision != CG_PREC_FP64 && p->precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   return 0;
 }
@@ -743,6 +747,11 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     }
   }
   cgdev::WorkT<R>& w = b.w;
+  if (p->nsc > 0) {
+    for (int k = 0; k < 2; ++k) rc |= dev_alloc(s, h, &b.st[k].sc, (size_t)p->nsc * ncv);
+    rc |= dev_alloc(s, h, &w.rhs_sc0, (size_t)p->nsc * ncv);
+    rc |= dev_alloc(s, h, &w.rhs_sc1, (size_t)p->nsc * ncv);
+  }
   rc |= dev_alloc(s, h, &w.aux, (size_t)ncv * cgdev::A_N);
   if (!mixed) rc |= dev_alloc(s, h, &w.logp, ncv);
   double** wf[] = {&w.T, &w.mu_lam, &w.mu_sgs, &w.loc_lam, &w.loc_sgs, &w.a_sgs, &w.vv2, &w.cfl};
@@ -791,6 +800,14 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     }
   }
   rc |= cgk::launch_fill(w.aux, R(0), (int64_t)ncv * cgdev::A_N, stream);
+  if (p->nsc > 0) {
+    // FlowSolver.hpp:590: scalars start at zero
+    const std::vector<double> zero((size_t)p->nsc * ncv, 0.0);
+    for (int k = 0; k < 2; ++k)
+      HIPCHK(s, hipMemcpy(b.st[k].sc, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(w.rhs_sc0, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(w.rhs_sc1, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
+  }
   rc |= cgk::launch_fill(w.rhs0, R(0), 5 * (int64_t)ncv, stream);
   rc |= cgk::launch_fill(w.rhs1, R(0), 5 * (int64_t)ncv, stream);
   if (rc) return s->set(CG_EHIP, "initial fill failed");
@@ -857,14 +874,15 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false, any_dsm = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_sc = false;
   for (auto* I : E.inst) {
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
+    any_sc = any_sc || I->prm.nsc > 0;
   }
   E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm};
+         any_full_gr, any_dsm, any_sc};
   E.dirty = false;
   return 0;
 }
@@ -1214,6 +1232,7 @@ int cg_params_default(cg_params* p) {
   p->vreman_coeff = 0.07;
   p->sigma_coeff = 1.5;
   p->delta_r_dsm = 2.0;
+  p->res_time = -1;
   p->a_sgs_coeff = 0.05;
   p->a_sgs_coeff2 = 0.0015;
   p->lambda_dil_coeff = 0.0;
@@ -1304,6 +1323,30 @@ int cg_params_parse(cg_params* p, const char* text) {
     } else if (key == "GROUP_MEMBER_THRESHOLD") {
       if (!num(1, &v)) return bad("expected an integer");
       p->group_member_threshold = int(v);
+    } else if (key == "REGISTER_SCALAR" || key == "RES_TIME") {
+      // FlowSolver::registerScalars (FlowSolver.hpp:531-549); RES_TIME
+      // (IdealGasSolver.hpp:121-122) registers the residence-time scalar
+      std::vector<std::string> names;
+      if (key == "RES_TIME") names.push_back("RES_TIME");
+      else
+        for (size_t i = 1; i < tok.size(); ++i) names.push_back(tok[i]);
+      if (names.empty()) return bad("expected scalar names");
+      for (const std::string& nm : names) {
+        if (nm.size() >= sizeof(p->sc_names[0])) return bad("scalar name too long: " + nm);
+        for (int i = 0; i < p->nsc; ++i)
+          if (nm == p->sc_names[i]) return bad("scalar registered twice: " + nm);
+        if (p->nsc >= CG_MAX_SCALARS) return bad("more than 8 passive scalars");
+        std::snprintf(p->sc_names[p->nsc], sizeof(p->sc_names[0]), "%s", nm.c_str());
+        ++p->nsc;
+      }
+      // name order (the reference's scalar map), and RES_TIME's index
+      std::vector<std::string> all(p->sc_names, p->sc_names + p->nsc);
+      std::sort(all.begin(), all.end());
+      p->res_time = -1;
+      for (int i = 0; i < p->nsc; ++i) {
+        std::snprintf(p->sc_names[i], sizeof(p->sc_names[0]), "%s", all[(size_t)i].c_str());
+        if (all[(size_t)i] == "RES_TIME") p->res_time = i;
+      }
     } else if (key == "LIMIT_ASGS") {
       p->limit_asgs = 1;
     } else if (key == "NO_DISS") {
@@ -1782,6 +1825,13 @@ int cg_write_sles(cg_solver* s, const char* path) {
     d.rhoE = rhoE.data();
     d.p = p.data();
     d.T = T.data();
+    std::vector<std::vector<double>> scv((size_t)I->prm.nsc, std::vector<double>(n));
+    for (int isc = 0; isc < I->prm.nsc; ++isc) {
+      double* f[1] = {I->b.st[E.cur].sc + (size_t)isc * n};
+      if (int rc = get_soa(s, (int)n, I->ext_of, f, 1, scv[(size_t)isc].data(), s->stream)) return rc;
+      d.sc_names.push_back(I->prm.sc_names[isc]);
+      d.sc.push_back(scv[(size_t)isc].data());
+    }
     std::vector<double> lm, mm;
     if (I->b.w.lijmij) {
       lm.resize(n);
@@ -1809,6 +1859,17 @@ int cg_read_sles(cg_solver* s, const char* path) {
     if (r.have_step) s->info_host->step = r.step;
     if (r.have_time) s->info_host->time = r.time;
     if (r.have_dt) s->info_host->dt = r.dt;
+    for (const auto& rec : r.other) {
+      // a passive scalar of this solve, by name
+      for (int isc = 0; isc < I->prm.nsc; ++isc) {
+        if (rec.first != I->prm.sc_names[isc]) continue;
+        const size_t n = (size_t)I->ncv;
+        std::vector<double> dev(n);
+        for (size_t c = 0; c < n; ++c) dev[c] = rec.second[(size_t)I->ext_of[c]];
+        HIPCHK(s, hipMemcpy(I->b.st[E.cur].sc + (size_t)isc * n, dev.data(), sizeof(double) * n,
+                            hipMemcpyHostToDevice));
+      }
+    }
     if (I->b.w.lijmij && !r.lijmij.empty() && !r.mijmij.empty()) {
       // the registered DSM_LOCAL data (Sgs.hpp:7-14) in device cell order
       const size_t n = (size_t)I->ncv;
@@ -1822,6 +1883,38 @@ int cg_read_sles(cg_solver* s, const char* path) {
       I->dsm_kept = true;
     }
     return write_info(s, I, s->info_host, s->stream);
+  });
+}
+
+namespace {
+int scalar_index(const cg_params& p, const std::string& name) {
+  for (int i = 0; i < p.nsc; ++i)
+    if (name == p.sc_names[i]) return i;
+  return -1;
+}
+}  // namespace
+
+int cg_set_scalar(cg_solver* s, const char* name, const double* phi) {
+  if (!s || !name || !phi) return fail(CG_EINVAL, "null argument");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    const int isc = scalar_index(I->prm, name);
+    if (isc < 0) return s->set(CG_EINVAL, std::string("no passive scalar named ") + name);
+    const size_t n = (size_t)I->ncv;
+    std::vector<double> dev(n);
+    for (size_t c = 0; c < n; ++c) dev[c] = phi[(size_t)I->ext_of[c]];
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(I->b.st[E.cur].sc + (size_t)isc * n, dev.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  });
+}
+
+int cg_get_scalar(cg_solver* s, const char* name, double* phi) {
+  if (!s || !name || !phi) return fail(CG_EINVAL, "null argument");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    const int isc = scalar_index(I->prm, name);
+    if (isc < 0) return s->set(CG_EINVAL, std::string("no passive scalar named ") + name);
+    double* f[1] = {I->b.st[E.cur].sc + (size_t)isc * I->ncv};
+    return get_soa(s, I->ncv, I->ext_of, f, 1, phi, s->stream);
   });
 }
 

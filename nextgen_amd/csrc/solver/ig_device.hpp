@@ -102,6 +102,7 @@ struct StateT {
   R* light;      // [ncv][8]
   double* rhoE;  // [ncv]; in mixed precision it aliases cons + 4*ncv
   double* cons;  // mixed precision: [5][ncv] rho u0 u1 u2 rhoE (null in FP64)
+  double* sc;    // passive scalars [nsc][ncv] (FP64; null without scalars)
 };
 
 struct StepInfo {
@@ -129,6 +130,8 @@ struct WorkT {
   // time-relaxed L:M and M:M [ncv] carried from step to step
   double* dsm;
   double *lijmij, *mijmij;
+  // passive scalars: RK3 registers of the scalar RHS [nsc][ncv] (null without scalars)
+  double *rhs_sc0, *rhs_sc1;
 };
 
 struct Params {
@@ -141,6 +144,7 @@ struct Params {
   double cp, rho_ref, inv_rhoref;  // full Gibbs-remainder sensor
   int full_gr;                     // USE_FULL_GR
   double dsm_alpha;                // DELTA_R_DSM^2
+  int nsc, res_time;               // passive scalars; index of RES_TIME or -1
 };
 
 using Geom = GeomT<double>;
@@ -162,7 +166,8 @@ struct Inst {
   int npblocks;  // blocks of kBlock compact pairs
 };
 
-constexpr int kBlkBits = 20;  // up to 2^20 blocks (268M cells) per instance, 2^11 instances
+constexpr int kBlkBits = 20;
+constexpr int kMaxScalars = 8;  // passive scalars per solve  // up to 2^20 blocks (268M cells) per instance, 2^11 instances
 
 }  // namespace cgdev
 
@@ -185,6 +190,7 @@ struct Launch {
   int sigma;         // some instance uses SGS_MODEL SIGMA (its SGS pass runs first)
   int full_gr;       // some instance uses USE_FULL_GR (its sensor pass runs first)
   int dsm;           // some instance uses SGS_MODEL DSM_LOCAL (its two passes run first)
+  int scalars;       // some instance transports passive scalars (a pass after each stage's flux)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

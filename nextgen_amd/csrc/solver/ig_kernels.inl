@@ -722,6 +722,114 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   rk_update<STAGE>(g, in, out, w, P, c, self, acc);
 }
 
+// Passive scalars (calcRhsScalars and advanceScalars, IdealGasSolver.cpp:537-693,
+// 874-892), after the stage's flux pass, FP64: the scalar flux of a pair is the
+// pair mean of phi times the pair's mass flux F_rho (compact pairs: the stored pair
+// flux; extended pairs: the inviscid and symmetric flux recomputed from the stage
+// state, as the flux pass computes it), less on compact pairs the diffusion
+// 0.5 (loc_lam0 + loc_sgs0 + loc_lam1 + loc_sgs1) area/delta (phi1 - phi0) (unit
+// Lewis number; 0 the pair's lo cell); summed per cell in pair order, then the
+// RES_TIME source vol rho, then phi = (rho_old phi + dt/V (w . r)) / rho_new with
+// the RK3 registers of the flow.
+template <class R, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                    int cur) {
+#pragma clang fp contract(off)
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Params& P = X.P;
+  const int nsc = P.nsc;
+  if (nsc <= 0) return;
+  const GeomT<R> g = X.g;
+  const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
+  const WorkT<R> w = X.w;
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= g.ncv) return;
+  if (G(w.info)->nan) return;  // the flux pass has not advanced the state either
+  const int c = cell_at(g.xperm, pos);
+  const int ncv = g.ncv;
+  const int lane = pos & (kSlice - 1);
+  const auto phi = G(in.sc);
+  const auto loc_lam = G(w.loc_lam);
+  const auto loc_sgs = G(w.loc_sgs);
+  double acc[kMaxScalars];
+#pragma unroll
+  for (int i = 0; i < kMaxScalars; ++i) acc[i] = 0.0;
+  auto add = [&](int lo, int hi, double mf, double d_coeff, bool compact, bool self_hi) {
+    for (int isc = 0; isc < nsc; ++isc) {
+      const double phi0 = phi[(size_t)isc * ncv + lo], phi1 = phi[(size_t)isc * ncv + hi];
+      double fsc = 0.5 * (phi0 + phi1) * mf;
+      if (compact) fsc -= d_coeff * (phi1 - phi0);
+      if (self_hi) acc[isc] += fsc;
+      else acc[isc] -= fsc;
+    }
+  };
+  // compact pairs, pair order
+  const int nc = G(g.ef_nc)[pos];
+  const int cbase = G(g.cf_ptr)[pos >> 6] + lane;
+  for (int k = 0; k < nc; ++k) {
+    const int slot = cbase + k * kSlice;
+    const int code = G(g.cf_nbr)[slot];
+    const int nb = code & 0x7fffffff;
+    const bool self_hi = code < 0;
+    const int pair = G(g.cf_pair)[slot] & 0x7fffffff;
+    const int lo = self_hi ? nb : c, hi = self_hi ? c : nb;
+    const double mf = double(G(w.cflux)[(size_t)pair * 6]);
+    const double loc_total = 0.5 * (loc_lam[lo] + loc_sgs[lo] + loc_lam[hi] + loc_sgs[hi]);
+    const double d_coeff = loc_total * double(G(g.paod)[pair]);
+    add(lo, hi, mf, d_coeff, true, self_hi);
+  }
+  // purely extended pairs, pair order
+  const Light<R> self = load_light(in.light, c);
+  const int deg = G(g.xf_deg)[pos];
+  const int xbase = G(g.xf_ptr)[pos >> 6] + lane;
+  for (int k = 0; k < deg; ++k) {
+    const int slot = xbase + k * kSlice;
+    const int code = G(g.xf_nbr)[slot];
+    const int nb = code & 0x7fffffff;
+    const bool self_hi = code < 0;
+    const Light<R> other = load_light(in.light, nb);
+    const Light<R>& a = self_hi ? other : self;
+    const Light<R>& b = self_hi ? self : other;
+    const R n[3] = {G(g.xsn0)[slot], G(g.xsn1)[slot], G(g.xsn2)[slot]};
+    const R cv[3] = {G(g.xsc0)[slot], G(g.xsc1)[slot], G(g.xsc2)[slot]};
+    R F[5], u_avg[3], inv_v_avg, h_stag;
+    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
+    if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
+      R cF[5];
+      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
+      F[0] += cF[0];
+    }
+    add(self_hi ? nb : c, self_hi ? c : nb, double(F[0]), 0.0, false, self_hi);
+  }
+  double rho_old, rho_new;
+  if constexpr (kMixed<R>) {
+    rho_old = G(in.cons)[c];
+    rho_new = G(out.cons)[c];
+  } else {
+    rho_old = G(in.light)[(size_t)c * L_N + L_RHO];
+    rho_new = G(out.light)[(size_t)c * L_N + L_RHO];
+  }
+  if (P.res_time >= 0) acc[P.res_time] += G(g.vol)[c] * rho_old;
+  const double dt = G(w.info)->dt;
+  const double tmp = dt * G(g.inv_vol)[c];
+  const double inv_rho = 1.0 / rho_new;
+  for (int isc = 0; isc < nsc; ++isc) {
+    const size_t ii = (size_t)isc * ncv + c;
+    double agg;
+    if (STAGE == 1) {
+      agg = tmp * (1.0 * acc[isc] + 0.0 * 0.0 + 0.0 * 0.0);
+      G(w.rhs_sc0)[ii] = acc[isc];
+    } else if (STAGE == 2) {
+      agg = tmp * (-0.75 * G(w.rhs_sc0)[ii] + 0.25 * acc[isc] + 0.0 * 0.0);
+      G(w.rhs_sc1)[ii] = acc[isc];
+    } else {
+      agg = tmp * ((-1.0 / 12.0) * G(w.rhs_sc0)[ii] + (-1.0 / 12.0) * G(w.rhs_sc1)[ii] + (8.0 / 12.0) * acc[isc]);
+    }
+    G(out.sc)[ii] = (rho_old * phi[ii] + agg) * inv_rho;
+  }
+}
+
 // _calcCvGradRange, StaticSolver.cpp:5535-5557
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur) {
@@ -1648,6 +1756,17 @@ template <class R>
 int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid(L.nblk), block(kBlock);
+  // the scalars of a step follow the stage's flux pass (they need its new density)
+  if (mode == 1 && L.scalars) {
+    if (int rc = launch_flux_ext(stage, mode, Launch<R>{L.I, L.bmap, L.nblk, L.ninst, L.lds, L.pbmap, L.npblk, L.tile,
+                                                      L.sigma, L.full_gr, L.dsm, 0},
+                                 cur, rhs_out, stream))
+      return rc;
+    if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1>), grid, block, 0, s, L.I, L.bmap, cur);
+    else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2>), grid, block, 0, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_scalars<R, 3>), grid, block, 0, s, L.I, L.bmap, cur);
+    return check();
+  }
   if (L.lds) {
     auto go = [&](auto nb) {
       constexpr int NB = decltype(nb)::value;
@@ -1699,8 +1818,10 @@ int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
     if (L.lds) hipLaunchKernelGGL((k_sensor_full<R, true>), dim3(L.nblk), dim3(kBlock), L.lds, s, L.I, L.bmap, cur);
     else hipLaunchKernelGGL((k_sensor_full<R, false>), dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
   }
-  // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS
+  // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS;
+  // passive scalars need its FP64 loc_lam and loc_sgs every step
   const size_t lds = L.lds / 6 * 4;
+  fields = (fields || L.scalars) ? 1 : 0;
   if (lds) {
     if (fields) hipLaunchKernelGGL((k_props<R, 1, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);
     else hipLaunchKernelGGL((k_props<R, 0, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);

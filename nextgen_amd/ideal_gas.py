@@ -43,6 +43,7 @@ class cg_params(C.Structure):
         ("interp_factor", C.c_double), ("sigma_thresh", C.c_double), ("group_member_threshold", C.c_int),
         ("disable_grouping", C.c_int), ("zero_c_ef", C.c_int), ("precision", C.c_int),
         ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int), ("delta_r_dsm", C.c_double),
+        ("nsc", C.c_int), ("res_time", C.c_int), ("sc_names", (C.c_char * 32) * 8),
     ]
 
 
@@ -74,7 +75,7 @@ EXPORTS = [
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
     "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
     "cg_instance_create_on", "cg_mles_counts", "cg_mles_get", "cg_mesh_read_mles", "cg_mesh_write_mles",
-    "cg_write_sles", "cg_read_sles",
+    "cg_write_sles", "cg_read_sles", "cg_set_scalar", "cg_get_scalar",
 ]
 
 _lib = None
@@ -106,6 +107,8 @@ def load_library(path: str = LIB_PATH):
         "cg_mesh_read_mles": (C.c_int, [C.c_char_p, P(cg_params), P(vp)]),
         "cg_mesh_write_mles": (C.c_int, [vp, C.c_char_p]),
         "cg_write_sles": (C.c_int, [vp, C.c_char_p]),
+        "cg_set_scalar": (C.c_int, [vp, C.c_char_p, dp]),
+        "cg_get_scalar": (C.c_int, [vp, C.c_char_p, dp]),
         "cg_read_sles": (C.c_int, [vp, C.c_char_p]),
         "cg_mesh_get": (C.c_int, [vp, C.c_char_p, vp, C.c_int64]),
         "cg_mesh_destroy": (None, [vp]),
@@ -380,6 +383,19 @@ class IdealGasSolver:
         """StaticSolver::writeSles / writeData (src/core/common/StaticSolver.hpp:2311-2321):
         step, time, dt, rho, u, rhoE, p, T."""
         self._chk(load_library().cg_write_sles(self._h, os.fsencode(path)))
+
+    def scalar_names(self):
+        return [self.params.sc_names[i].value.decode() for i in range(self.params.nsc)]
+
+    def set_scalar(self, name, phi):
+        """A passive scalar (REGISTER_SCALAR / RES_TIME) by name, [ncv]."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self._chk(load_library().cg_set_scalar(self._h, name.encode(), _dptr(phi)))
+
+    def get_scalar(self, name):
+        out = np.empty(self.ncv)
+        self._chk(load_library().cg_get_scalar(self._h, name.encode(), _dptr(out)))
+        return out
 
     def read_sles(self, path):
         """Restart read of rho, u, rhoE, step, time, dt; call init_complete() next."""

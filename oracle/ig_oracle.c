@@ -154,6 +154,8 @@ struct og_solver {
   double *rho, *u, *rhoE, *pr, *T, *sos, *mu_lam, *mu_sgs, *loc_lam, *loc_sgs, *a_sgs, *vv2, *dudx, *cfl;
   double *lijmij, *mijmij; /* DSM_LOCAL relaxation state */
   int dsm_kept;            /* set from a restart: the next og_init_complete keeps it */
+  double *sc;              /* passive scalars [ncv][nsc] */
+  double *rhs0_sc, *rhs1_sc, *rhs2_sc, *rhs_sc_tmp; /* [ncv][nsc] */
   og_state* cv_light;
   og_aux* cv_compact;
   og_rhs *rhs0, *rhs1, *rhs2;
@@ -186,6 +188,14 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   s->cfl = (double*)calloc(n, sizeof(double));
   s->dudx = (double*)calloc(9 * (size_t)n, sizeof(double));
   s->lijmij = (double*)calloc(n, sizeof(double));
+  {
+    const size_t nsc = (size_t)(p->nsc > 0 ? p->nsc : 0) * n;
+    s->sc = (double*)calloc(nsc + 1, sizeof(double)); /* FlowSolver.hpp:590: zero */
+    s->rhs0_sc = (double*)calloc(nsc + 1, sizeof(double));
+    s->rhs1_sc = (double*)calloc(nsc + 1, sizeof(double));
+    s->rhs2_sc = (double*)calloc(nsc + 1, sizeof(double));
+    s->rhs_sc_tmp = (double*)calloc(nsc + 1, sizeof(double));
+  }
   s->mijmij = (double*)calloc(n, sizeof(double));
   s->cv_light = (og_state*)calloc(n, sizeof(og_state));
   s->cv_compact = (og_aux*)calloc(n, sizeof(og_aux));
@@ -209,6 +219,7 @@ void og_destroy(og_solver* s) {
   free(s->vv2); free(s->cfl); free(s->dudx); free(s->cv_light); free(s->cv_compact);
   free(s->rhs0); free(s->rhs1); free(s->rhs2); free(s->store_buf); free(s->store_recent);
   free(s->lijmij); free(s->mijmij);
+  free(s->sc); free(s->rhs0_sc); free(s->rhs1_sc); free(s->rhs2_sc); free(s->rhs_sc_tmp);
   free(s);
 }
 
@@ -738,10 +749,18 @@ void og_sgs_props(og_solver* s) {
 }
 
 /* IdealGasSolver.cpp:257-535 over the uniform pair list */
-void og_calc_rhs(og_solver* s, double* rhs_out) {
+/* calcRhsBase with calcRhsScalars (IdealGasSolver.cpp:257-693): each pair's
+ * scalar flux is the pair mean of phi times the pair's mass flux mf = F_rho (after
+ * the compact closure and the symmetric flux), less, on compact pairs, the
+ * diffusion 0.5 (loc_lam0 + loc_sgs0 + loc_lam1 + loc_sgs1) area/delta (phi1 -
+ * phi0) (unit Lewis number); accumulated in pair order like the flow RHS, then
+ * the RES_TIME source vol rho */
+static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
   const og_mesh* m = &s->m;
   const int n = m->ncv;
+  const int nsc = s->p.nsc;
   og_rhs* rhs = (og_rhs*)calloc(n, sizeof(og_rhs));
+  if (nsc > 0) memset(rhs_sc, 0, sizeof(double) * (size_t)nsc * n);
   for (int ief = 0; ief < m->nef; ++ief) {
     const int icv0 = m->lo[ief], icv1 = m->hi[ief];
     og_nc nc;
@@ -774,7 +793,25 @@ void og_calc_rhs(og_solver* s, double* rhs_out) {
     rhs[icv1].rho += flux.rho;
     for (int i = 0; i < 3; ++i) rhs[icv1].rhou[i] += flux.rhou[i];
     rhs[icv1].rhoE += flux.rhoE;
+    if (nsc > 0) {
+      const double mf = flux.rho;
+      double d_coeff = 0.0;
+      if (ief < m->nfa) {
+        const double loc_total =
+            0.5 * (s->loc_lam[icv0] + s->loc_sgs[icv0] + s->loc_lam[icv1] + s->loc_sgs[icv1]);
+        d_coeff = loc_total * m->aod_use[ief];
+      }
+      for (int isc = 0; isc < nsc; ++isc) {
+        const double phi0 = s->sc[(size_t)icv0 * nsc + isc], phi1 = s->sc[(size_t)icv1 * nsc + isc];
+        double fsc = 0.5 * (phi0 + phi1) * mf;
+        if (ief < m->nfa) fsc -= d_coeff * (phi1 - phi0);
+        rhs_sc[(size_t)icv0 * nsc + isc] -= fsc;
+        rhs_sc[(size_t)icv1 * nsc + isc] += fsc;
+      }
+    }
   }
+  if (nsc > 0 && s->p.res_time >= 0)
+    for (int icv = 0; icv < n; ++icv) rhs_sc[(size_t)icv * nsc + s->p.res_time] += m->vol[icv] * s->rho[icv];
   for (int icv = 0; icv < n; ++icv) {
     double* o = &rhs_out[5 * (size_t)icv];
     o[0] = rhs[icv].rho;
@@ -786,10 +823,27 @@ void og_calc_rhs(og_solver* s, double* rhs_out) {
   free(rhs);
 }
 
-static void calc_rhs_into(og_solver* s, og_rhs* rhs) {
+void og_calc_rhs(og_solver* s, double* rhs_out) { calc_rhs_full(s, rhs_out, s->rhs_sc_tmp); }
+
+void og_calc_rhs_scalars(og_solver* s, double* rhs_sc) {
   const int n = s->m.ncv;
   double* tmp = (double*)malloc(sizeof(double) * 5 * (size_t)n);
-  og_calc_rhs(s, tmp);
+  calc_rhs_full(s, tmp, rhs_sc);
+  free(tmp);
+}
+
+void og_set_scalar(og_solver* s, int isc, const double* phi) {
+  for (int icv = 0; icv < s->m.ncv; ++icv) s->sc[(size_t)icv * s->p.nsc + isc] = phi[icv];
+}
+
+void og_get_scalar(const og_solver* s, int isc, double* phi) {
+  for (int icv = 0; icv < s->m.ncv; ++icv) phi[icv] = s->sc[(size_t)icv * s->p.nsc + isc];
+}
+
+static void calc_rhs_into(og_solver* s, og_rhs* rhs, double* rhs_sc) {
+  const int n = s->m.ncv;
+  double* tmp = (double*)malloc(sizeof(double) * 5 * (size_t)n);
+  calc_rhs_full(s, tmp, rhs_sc);
   for (int icv = 0; icv < n; ++icv) {
     rhs[icv].rho = tmp[5 * (size_t)icv];
     for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] = tmp[5 * (size_t)icv + 1 + i];
@@ -814,6 +868,14 @@ static void rk3_step(og_solver* s, const double rk_wgt[3]) {
     double* u = &s->u[3 * (size_t)icv];
     for (int i = 0; i < 3; ++i) u[i] = (rho_old * u[i] + agg.rhou[i]) * inv_rho;
     s->rhoE[icv] += agg.rhoE;
+    /* advanceScalars (IdealGasSolver.cpp:874-892) */
+    const int nsc = s->p.nsc;
+    for (int isc = 0; isc < nsc; ++isc) {
+      const size_t ii = (size_t)icv * nsc + isc;
+      const double rhs_agg =
+          tmp * (rk_wgt[0] * s->rhs0_sc[ii] + rk_wgt[1] * s->rhs1_sc[ii] + rk_wgt[2] * s->rhs2_sc[ii]);
+      s->sc[ii] = (rho_old * s->sc[ii] + rhs_agg) * inv_rho;
+    }
   }
 }
 
@@ -918,15 +980,21 @@ int og_step(og_solver* s) {
   memset(s->rhs0, 0, sizeof(og_rhs) * n);
   memset(s->rhs1, 0, sizeof(og_rhs) * n);
   memset(s->rhs2, 0, sizeof(og_rhs) * n);
-  calc_rhs_into(s, s->rhs0);
+  if (s->p.nsc > 0) {
+    const size_t nn = sizeof(double) * (size_t)s->p.nsc * n;
+    memset(s->rhs0_sc, 0, nn);
+    memset(s->rhs1_sc, 0, nn);
+    memset(s->rhs2_sc, 0, nn);
+  }
+  calc_rhs_into(s, s->rhs0, s->rhs0_sc);
   rk3_step(s, og_erk_wgt3[0]);
   s->time += s->dt;
   og_update_state(s);
-  calc_rhs_into(s, s->rhs1);
+  calc_rhs_into(s, s->rhs1, s->rhs1_sc);
   rk3_step(s, og_erk_wgt3[1]);
   s->time -= 0.5 * s->dt;
   og_update_state(s);
-  calc_rhs_into(s, s->rhs2);
+  calc_rhs_into(s, s->rhs2, s->rhs2_sc);
   rk3_step(s, og_erk_wgt3[2]);
   s->time += 0.5 * s->dt;
   og_update_state(s);

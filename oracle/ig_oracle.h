@@ -84,6 +84,8 @@ typedef struct og_params {
   double sigma_coeff; /* SIGMA_COEFF */
   int full_gr;        /* USE_FULL_GR */
   double delta_r_dsm; /* DELTA_R_DSM */
+  int nsc;            /* transported passive scalars (REGISTER_SCALAR names, RES_TIME) */
+  int res_time;       /* index of the RES_TIME scalar, or -1 */
 } og_params;
 
 typedef struct og_diag {
@@ -128,6 +130,11 @@ int og_step(og_solver* s);
 void og_diag_get(og_solver* s, og_diag* d);
 void og_store(og_solver* s);
 void og_restore(og_solver* s);
+/* passive scalar isc, [ncv] */
+void og_set_scalar(og_solver* s, int isc, const double* phi);
+void og_get_scalar(const og_solver* s, int isc, double* phi);
+/* the scalar RHS [ncv][nsc] of the current state (the RHS of og_calc_rhs' pass) */
+void og_calc_rhs_scalars(og_solver* s, double* rhs_sc);
 /* the DSM_LOCAL test filter alone (phi, phif [ncv][ncomp]; needs m->filt_w) */
 void og_filter_cv(const og_mesh* m, const double* phi, double* phif, int ncomp);
 /* DSM_LOCAL L:M and M:M as a restart read sets them (kept by the next og_init_complete) */

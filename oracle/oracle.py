@@ -33,7 +33,7 @@ class og_params(C.Structure):
         ("lambda_dil_coeff", C.c_double), ("Pr_t", C.c_double), ("limit_asgs", C.c_int), ("coeff_p", C.c_double),
         ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
         ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
-        ("full_gr", C.c_int), ("delta_r_dsm", C.c_double),
+        ("full_gr", C.c_int), ("delta_r_dsm", C.c_double), ("nsc", C.c_int), ("res_time", C.c_int),
     ]
 
 
@@ -86,6 +86,11 @@ def lib():
         L.og_set_time.restype = None
         L.og_set_dsm_state.argtypes = [vp, dp, dp]
         L.og_set_dsm_state.restype = None
+        for f in ("og_set_scalar", "og_get_scalar"):
+            getattr(L, f).argtypes = [vp, C.c_int, dp]
+            getattr(L, f).restype = None
+        L.og_calc_rhs_scalars.argtypes = [vp, dp]
+        L.og_calc_rhs_scalars.restype = None
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
         _lib = L
@@ -195,6 +200,20 @@ class OracleSolver:
         lm = np.ascontiguousarray(lijmij, dtype=np.float64)
         mm = np.ascontiguousarray(mijmij, dtype=np.float64)
         lib().og_set_dsm_state(self._h, dptr(lm), dptr(mm))
+
+    def set_scalar(self, isc, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        lib().og_set_scalar(self._h, int(isc), dptr(phi))
+
+    def get_scalar(self, isc):
+        out = np.empty(self.ncv)
+        lib().og_get_scalar(self._h, int(isc), dptr(out))
+        return out
+
+    def calc_rhs_scalars(self):
+        out = np.empty((self.ncv, max(self._p.nsc, 1)))
+        lib().og_calc_rhs_scalars(self._h, dptr(out))
+        return out
 
     def sgs_props(self):
         lib().og_sgs_props(self._h)
