@@ -132,6 +132,9 @@ def k4_params(Mt=0.3, Re=1600.0, gamma=1.4, sgs="VREMAN", cfl=0.8):
             f"MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL {sgs}\nTIMESTEP CFL {cfl!r}\n")
 
 
+SGS_NAMES = {"NONE": "no", "VREMAN": "Vreman", "SIGMA": "Sigma", "DSM_LOCAL": "DSM_LOCAL"}
+
+
 def workload(args):
     from nextgen_amd.ic import tgv_params
     from nextgen_amd.ideal_gas import Mesh
@@ -140,13 +143,16 @@ def workload(args):
     mesh = Mesh.jittered(args.n, jitter=0.3, seed=args.seed)
     log(f"mesh: ncv={mesh.ncv} nfa={mesh.nfa} nef={mesh.nef} nnz={mesh.nnz} built in {time.time() - t0:.1f}s")
     if args.ic == "tgv":
-        text = tgv_params(Re=1600.0, Ma=0.1, sgs="VREMAN", cfl=0.8)
+        text = tgv_params(Re=1600.0, Ma=0.1, sgs=args.sgs, cfl=0.8)
         state = mesh.ic_tgv(0.1, 1.4)
         desc = "TGV Re 1600 Ma 0.1"
     else:
-        text = k4_params()
+        text = k4_params(sgs=args.sgs)
         state = mesh.ic_k4(4.0, 0.3, 1.4, seed=args.seed)
         desc = f"k^4 spectrum k0 4 Mt 0.3 (seed {args.seed}), Re 1600"
+    if args.scalars:
+        text += "REGISTER_SCALAR " + " ".join(args.scalars) + "\n"
+        desc += f", {len(args.scalars)} passive scalars"
     return mesh, text + "CHECK_INTERVAL 1000000\n", state, desc
 
 
@@ -349,6 +355,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--ic", choices=["tgv", "k4"], default="tgv")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--sgs", choices=["NONE", "VREMAN", "SIGMA", "DSM_LOCAL"], default="VREMAN",
+                    help="SGS_MODEL of the step workload")
+    ap.add_argument("--scalars", nargs="*", default=[], help="passive scalars to register (names)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--accuracy", type=int, default=0, help="compare mixed with FP64 over this many steps")
     ap.add_argument("--every", type=int, default=50)
@@ -435,7 +444,7 @@ def main():
         "config": {
             "workload": f"{'FP64 drop-in' if main_prec == 'fp64' else 'mixed precision'}: periodic Voronoi mesh "
                         f"of a {args.n}^3 lattice jittered +-0.3h (seed {args.seed}), {ncv} cells, {desc}, "
-                        f"Vreman SGS, RK3 CFL 0.8",
+                        f"{SGS_NAMES[args.sgs]} SGS, RK3 CFL 0.8",
             "ncv": ncv, "nfa": mesh.nfa, "nef": mesh.nef, "nnz": mesh.nnz,
             "G_e": Ge, "G_c": Gc, "G_g": Gg, "precision": main_prec,
         },

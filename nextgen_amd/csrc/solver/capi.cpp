@@ -112,7 +112,18 @@ struct Layout {
   // Morton cells, sorted by their half-face counts
   std::vector<int> ext_of;
   bool has_rvv = false;       // the mesh knows its cells' Voronoi radii (DSM_LOCAL needs them)
+  int lds_max = 0;            // the device's LDS bytes per workgroup
 };
+
+// LDS bytes of the scalar pass's halo variant: the flux pass's records of a
+// halo of `lds` bytes (6 values of R per cell), 16-aligned, then nsc FP64
+// scalars per halo cell; 0 (global gathers) without a halo or beyond lds_max
+template <class R>
+size_t scalar_lds(size_t lds, int nsc, int lds_max) {
+  if (!lds || nsc <= 0) return 0;
+  const size_t need = ((lds + 15) & ~size_t(15)) + lds / (6 * sizeof(R)) * (size_t)nsc * sizeof(double);
+  return need <= (size_t)lds_max ? need : 0;
+}
 
 // host bookkeeping of one solve
 struct HostInst {
@@ -144,7 +155,8 @@ struct Instance : HostInst {
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
-            prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc > 0};
+            prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
+            scalar_lds<R>(lds, prm.nsc, lay->lds_max)};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -649,6 +661,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
       (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
       // 6 fields of R per halo cell
       const bool use_lds = allow_lds && herr.empty() && (size_t)hmax * 6 * sizeof(R) <= (size_t)lds_max;
+      Lay->lds_max = lds_max;
       if (use_lds) {
         rc |= upload(s, h, &g.halo_ptr, hptr);
         rc |= upload(s, h, &g.halo_v, hv);
@@ -874,15 +887,18 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_sc = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false;
+  int max_nsc = 0, lds_max = 0;
   for (auto* I : E.inst) {
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
-    any_sc = any_sc || I->prm.nsc > 0;
+    max_nsc = std::max(max_nsc, I->prm.nsc);
+    lds_max = I->lay->lds_max;
   }
-  E.L = {E.tab, E.bmap, nb, n, (all_halo && !E.force_global) ? lds : 0, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm, any_sc};
+  const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
+  E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
+         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max)};
   E.dirty = false;
   return 0;
 }

@@ -190,7 +190,8 @@ struct Launch {
   int sigma;         // some instance uses SGS_MODEL SIGMA (its SGS pass runs first)
   int full_gr;       // some instance uses USE_FULL_GR (its sensor pass runs first)
   int dsm;           // some instance uses SGS_MODEL DSM_LOCAL (its two passes run first)
-  int scalars;       // some instance transports passive scalars (a pass after each stage's flux)
+  int scalars;       // most passive scalars of an instance (0: no scalar pass after each stage's flux)
+  size_t sc_lds;     // dynamic LDS bytes of the scalar pass's halo variant (0: global gathers)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

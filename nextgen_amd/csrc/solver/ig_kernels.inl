@@ -731,10 +731,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
 // Lewis number; 0 the pair's lo cell); summed per cell in pair order, then the
 // RES_TIME source vol rho, then phi = (rho_old phi + dt/V (w . r)) / rho_new with
 // the RK3 registers of the flow.
-template <class R, int STAGE>
+// LDS: the block's halo (the cells its extended half-faces reach, as the flux
+// pass stages them) holds u, sp_vol, p, h and the scalars of each halo cell.
+template <class R, int STAGE, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                     int cur) {
 #pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char smem[];
   int lblk;
   const Inst<R>& X = inst_of(I, bmap, lblk);
   const Params& P = X.P;
@@ -743,21 +746,41 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
   const GeomT<R> g = X.g;
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
-  const int pos = lblk * kBlock + threadIdx.x;
-  if (pos >= g.ncv) return;
   if (G(w.info)->nan) return;  // the flux pass has not advanced the state either
-  const int c = cell_at(g.xperm, pos);
   const int ncv = g.ncv;
-  const int lane = pos & (kSlice - 1);
   const auto phi = G(in.sc);
+  R* hrec = reinterpret_cast<R*>(smem);  // [hs][6]
+  double* hphi = nullptr;                // [hs][nsc]
+  if constexpr (LDS) {
+    const int hb = G(g.halo_ptr)[lblk], hs = G(g.halo_ptr)[lblk + 1] - hb;
+    hphi = reinterpret_cast<double*>(smem + (((size_t)g.halo_max * 6 * sizeof(R) + 15) & ~size_t(15)));
+    for (int i = threadIdx.x; i < hs; i += kBlock) {
+      const int gc = G(g.halo_v)[hb + i];
+      const auto q = G(in.light + (size_t)gc * L_N);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) hrec[6 * i + k] = q[k];
+      for (int isc = 0; isc < nsc; ++isc) hphi[i * nsc + isc] = phi[(size_t)isc * ncv + gc];
+    }
+    __syncthreads();
+  }
+  const int pos = lblk * kBlock + threadIdx.x;
+  if (pos >= ncv) return;
+  const int c = cell_at(g.xperm, pos);
+  const int lane = pos & (kSlice - 1);
   const auto loc_lam = G(w.loc_lam);
   const auto loc_sgs = G(w.loc_sgs);
-  double acc[kMaxScalars];
+  double acc[kMaxScalars], phic[kMaxScalars], phin[kMaxScalars];
 #pragma unroll
-  for (int i = 0; i < kMaxScalars; ++i) acc[i] = 0.0;
-  auto add = [&](int lo, int hi, double mf, double d_coeff, bool compact, bool self_hi) {
-    for (int isc = 0; isc < nsc; ++isc) {
-      const double phi0 = phi[(size_t)isc * ncv + lo], phi1 = phi[(size_t)isc * ncv + hi];
+  for (int i = 0; i < kMaxScalars; ++i) {
+    acc[i] = 0.0;
+    phic[i] = i < nsc ? phi[(size_t)i * ncv + c] : 0.0;
+  }
+  // one pair: lo/hi phi in its orientation, flux subtracted at lo, added at hi
+  auto add = [&](double mf, double d_coeff, bool compact, bool self_hi) {
+#pragma unroll
+    for (int isc = 0; isc < kMaxScalars; ++isc) {
+      if (isc >= nsc) break;
+      const double phi0 = self_hi ? phin[isc] : phic[isc], phi1 = self_hi ? phic[isc] : phin[isc];
       double fsc = 0.5 * (phi0 + phi1) * mf;
       if (compact) fsc -= d_coeff * (phi1 - phi0);
       if (self_hi) acc[isc] += fsc;
@@ -777,30 +800,65 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
     const double mf = double(G(w.cflux)[(size_t)pair * 6]);
     const double loc_total = 0.5 * (loc_lam[lo] + loc_sgs[lo] + loc_lam[hi] + loc_sgs[hi]);
     const double d_coeff = loc_total * double(G(g.paod)[pair]);
-    add(lo, hi, mf, d_coeff, true, self_hi);
+#pragma unroll
+    for (int isc = 0; isc < kMaxScalars; ++isc)
+      if (isc < nsc) phin[isc] = phi[(size_t)isc * ncv + nb];
+    add(mf, d_coeff, true, self_hi);
   }
-  // purely extended pairs, pair order
+  // purely extended pairs, pair order: F_rho as the flux pass computes it
   const Light<R> self = load_light(in.light, c);
   const int deg = G(g.xf_deg)[pos];
   const int xbase = G(g.xf_ptr)[pos >> 6] + lane;
-  for (int k = 0; k < deg; ++k) {
-    const int slot = xbase + k * kSlice;
-    const int code = G(g.xf_nbr)[slot];
-    const int nb = code & 0x7fffffff;
-    const bool self_hi = code < 0;
-    const Light<R> other = load_light(in.light, nb);
-    const Light<R>& a = self_hi ? other : self;
-    const Light<R>& b = self_hi ? self : other;
-    const R n[3] = {G(g.xsn0)[slot], G(g.xsn1)[slot], G(g.xsn2)[slot]};
-    const R cv[3] = {G(g.xsc0)[slot], G(g.xsc1)[slot], G(g.xsc2)[slot]};
-    R F[5], u_avg[3], inv_v_avg, h_stag;
-    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
-    if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
-      R cF[5];
-      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
-      F[0] += cF[0];
+  // slots are taken 4 at a time: their codes and geometry are loaded first so
+  // the loads overlap; the sums keep the order k = 0, 1, ...
+  constexpr int kB = 4;
+  for (int k0 = 0; k0 < deg; k0 += kB) {
+    int codes[kB];
+    R gn[kB][3], gcv[kB][3];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j < deg) {
+        const int slot = xbase + (k0 + j) * kSlice;
+        if constexpr (LDS) codes[j] = G(g.xf_loc)[slot];
+        else codes[j] = G(g.xf_nbr)[slot];
+        gn[j][0] = G(g.xsn0)[slot]; gn[j][1] = G(g.xsn1)[slot]; gn[j][2] = G(g.xsn2)[slot];
+        gcv[j][0] = G(g.xsc0)[slot]; gcv[j][1] = G(g.xsc1)[slot]; gcv[j][2] = G(g.xsc2)[slot];
+      }
     }
-    add(self_hi ? nb : c, self_hi ? c : nb, double(F[0]), 0.0, false, self_hi);
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (k0 + j >= deg) break;
+      Light<R> other;
+      bool self_hi;
+      if constexpr (LDS) {
+        const int li = codes[j] & 0x7fff;
+        self_hi = (codes[j] & 0x8000) != 0;
+        const R* q = hrec + 6 * li;
+        other.u[0] = q[0]; other.u[1] = q[1]; other.u[2] = q[2];
+        other.spv = q[3]; other.p = q[4]; other.h = q[5];
+        other.rho = R(0); other.sos = R(0);
+#pragma unroll
+        for (int isc = 0; isc < kMaxScalars; ++isc)
+          if (isc < nsc) phin[isc] = hphi[li * nsc + isc];
+      } else {
+        const int nb = codes[j] & 0x7fffffff;
+        self_hi = codes[j] < 0;
+        other = load_light(in.light, nb);
+#pragma unroll
+        for (int isc = 0; isc < kMaxScalars; ++isc)
+          if (isc < nsc) phin[isc] = phi[(size_t)isc * ncv + nb];
+      }
+      const Light<R>& a = self_hi ? other : self;
+      const Light<R>& b = self_hi ? self : other;
+      R F[5], u_avg[3], inv_v_avg, h_stag;
+      flux_skew(F, gn[j], a, b, u_avg, inv_v_avg, h_stag);
+      if (gcv[j][0] != R(0) || gcv[j][1] != R(0) || gcv[j][2] != R(0)) {
+        R cF[5];
+        flux_symmetric(cF, gcv[j], u_avg, inv_v_avg, h_stag, a, b);
+        F[0] += cF[0];
+      }
+      add(double(F[0]), 0.0, false, self_hi);
+    }
   }
   double rho_old, rho_new;
   if constexpr (kMixed<R>) {
@@ -814,7 +872,9 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
   const double dt = G(w.info)->dt;
   const double tmp = dt * G(g.inv_vol)[c];
   const double inv_rho = 1.0 / rho_new;
-  for (int isc = 0; isc < nsc; ++isc) {
+#pragma unroll
+  for (int isc = 0; isc < kMaxScalars; ++isc) {
+    if (isc >= nsc) break;
     const size_t ii = (size_t)isc * ncv + c;
     double agg;
     if (STAGE == 1) {
@@ -826,7 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
     } else {
       agg = tmp * ((-1.0 / 12.0) * G(w.rhs_sc0)[ii] + (-1.0 / 12.0) * G(w.rhs_sc1)[ii] + (8.0 / 12.0) * acc[isc]);
     }
-    G(out.sc)[ii] = (rho_old * phi[ii] + agg) * inv_rho;
+    G(out.sc)[ii] = (rho_old * phic[isc] + agg) * inv_rho;
   }
 }
 
@@ -1758,13 +1818,21 @@ int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rh
   const dim3 grid(L.nblk), block(kBlock);
   // the scalars of a step follow the stage's flux pass (they need its new density)
   if (mode == 1 && L.scalars) {
-    if (int rc = launch_flux_ext(stage, mode, Launch<R>{L.I, L.bmap, L.nblk, L.ninst, L.lds, L.pbmap, L.npblk, L.tile,
-                                                      L.sigma, L.full_gr, L.dsm, 0},
-                                 cur, rhs_out, stream))
-      return rc;
-    if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1>), grid, block, 0, s, L.I, L.bmap, cur);
-    else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2>), grid, block, 0, s, L.I, L.bmap, cur);
-    else hipLaunchKernelGGL((k_scalars<R, 3>), grid, block, 0, s, L.I, L.bmap, cur);
+    Launch<R> flow = L;
+    flow.scalars = 0;
+    if (int rc = launch_flux_ext(stage, mode, flow, cur, rhs_out, stream)) return rc;
+    // with halos: the flux pass's records (6 values of R per halo cell) and the
+    // scalars (FP64) of each halo cell in LDS, sc_lds bytes (host-sized)
+    const size_t lds = L.sc_lds;
+    if (lds) {
+      if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1, true>), grid, block, lds, s, L.I, L.bmap, cur);
+      else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2, true>), grid, block, lds, s, L.I, L.bmap, cur);
+      else hipLaunchKernelGGL((k_scalars<R, 3, true>), grid, block, lds, s, L.I, L.bmap, cur);
+    } else {
+      if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1, false>), grid, block, 0, s, L.I, L.bmap, cur);
+      else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2, false>), grid, block, 0, s, L.I, L.bmap, cur);
+      else hipLaunchKernelGGL((k_scalars<R, 3, false>), grid, block, 0, s, L.I, L.bmap, cur);
+    }
     return check();
   }
   if (L.lds) {
