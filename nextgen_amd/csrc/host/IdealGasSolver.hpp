@@ -137,6 +137,20 @@ class IdealGasSolver {
     cg_check(cg_write_sles(solver_, fn), solver_);
   }
 
+  // passive scalars by registered name (REGISTER_SCALAR / RES_TIME), [ncv]
+  std::vector<double> getScalar(const std::string& name) const {
+    std::vector<double> v((size_t)ncv);
+    cg_check(cg_get_scalar(solver_, name.c_str(), v.data()), solver_);
+    return v;
+  }
+  void setScalar(const std::string& name, const std::vector<double>& v) {
+    if (v.size() != (size_t)ncv) {
+      std::fprintf(stderr, "ERROR: setScalar %s: %zu values for %d cells\n", name.c_str(), v.size(), ncv);
+      throw(0);
+    }
+    cg_check(cg_set_scalar(solver_, name.c_str(), v.data()), solver_);
+  }
+
   void storeData() { cg_check(cg_store(solver_), solver_); }
   void restoreData() { cg_check(cg_restore(solver_), solver_); }
 
