@@ -106,7 +106,7 @@ constexpr int kZoneBits[26] = {
 
 class Reader {
  public:
-  explicit Reader(const std::string& path) : path_(path) { fp_ = std::fopen(path.c_str(), "rb"); }
+  explicit Reader(const std::string& path) { fp_ = std::fopen(path.c_str(), "rb"); }
   ~Reader() {
     if (fp_) std::fclose(fp_);
   }
@@ -125,7 +125,6 @@ class Reader {
   }
 
  private:
-  std::string path_;
   FILE* fp_ = nullptr;
 };
 
