@@ -153,3 +153,23 @@ def test_dsm_local_refused_on_a_mesh_without_voronoi_radii():
                          m.get("x_fa"), m.get("L"))
     with pytest.raises(CharlesError, match="r_vv"):
         IdealGasSolver(r, parse_params(tgv_params(sgs="DSM_LOCAL")))
+
+
+@pytest.mark.parametrize("mode", ["DT 0.002", "CFL_DT 0.8 0.3", "CFL_DT 0.8 0.005", "CFL_T 0.8 0.1", "CFL 0.5"])
+def test_time_step_modes_match_oracle(mode):
+    # setDt (FlowSolver.hpp:2274-2328) in each TIMESTEP mode: a fixed dt, the CFL dt
+    # capped by a maximum (the CFL binding, then the cap binding), the CFL dt landing
+    # on a target time (reached within the run), and another CFL number
+    m = Mesh.jittered(10, 0.3, seed=5)
+    p = parse_params(tgv_params(sgs="VREMAN") + f"TIMESTEP {mode}\n")
+    g, o = both(m, p)
+    hist = g.run_history(8)
+    for d in hist:
+        o.step()
+        od = o.diag()
+        assert d.step == od["step"]
+        assert abs(d.dt - od["dt"]) <= HIST_TOL * od["dt"], (mode, d.dt, od["dt"])
+        assert abs(d.time - od["time"]) <= HIST_TOL * max(od["time"], 1e-300)
+        assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"]
+    if mode.startswith("DT"):
+        assert all(d.dt == 0.002 for d in hist)
