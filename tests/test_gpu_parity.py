@@ -173,3 +173,33 @@ def test_time_step_modes_match_oracle(mode):
         assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"]
     if mode.startswith("DT"):
         assert all(d.dt == 0.002 for d in hist)
+
+
+@pytest.mark.parametrize("extra", ["NO_DISS\n", "ALL_DISS\n", "LIMIT_ASGS\n", "LAMBDA_DIL_COEFF 0.2\n",
+                                   "DISABLE_FA_GROUPING\n", "ZERO_C_EF\n",
+                                   "PR_T 0.7\nA_SGS_COEFF 0.1\nA_SGS_COEFF2 0.003\nCOEFF_P 0.4\nCOEFF_H 0.2\nGR_NRM 0.05\n"])
+def test_parameter_variants_match_oracle(extra):
+    # the sensor overrides (IdealGasSolver.cpp:973-979), limitAsgs, the dilatational
+    # conductivity of Vreman, the operator options and the coefficient keys, each
+    # through 8 steps of the TGV deck at Ma 0.3 (compressible enough that the
+    # sensor and the a_sgs ceiling act)
+    m = Mesh.jittered(10, 0.3, seed=7)
+    p = parse_params(tgv_params(Ma=0.3, sgs="VREMAN") + extra)
+    x = m.get("x_cv")
+    rho, u, rhoE = tgv_state(x, Ma=0.3)
+    g = IdealGasSolver(m, p)
+    g.set_state(rho, u, rhoE)
+    g.init_complete()
+    o = OracleSolver(m, p)
+    o.set_state(rho, u, rhoE)
+    o.init_complete()
+    for name in ("fgr", "a_sgs", "loc_sgs", "mu_total"):
+        a, b = g.get_field(name), o.get_field(name)
+        assert np.max(np.abs(a - b)) <= 1e-13 * max(np.max(np.abs(b)), 1e-300), (extra, name)
+    hist = g.run_history(8)
+    for d in hist:
+        o.step()
+        od = o.diag()
+        assert abs(d.dt - od["dt"]) <= HIST_TOL * od["dt"]
+        assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"], extra
+        assert abs(d.enstrophy - od["enstrophy"]) <= HIST_TOL * od["enstrophy"], extra
