@@ -193,6 +193,12 @@ class OracleSolver:
     def init_complete(self):
         lib().og_init_complete(self._h)
 
+    def store(self):
+        lib().og_store(self._h)
+
+    def restore(self):
+        lib().og_restore(self._h)
+
     def set_time(self, step, time, dt):
         lib().og_set_time(self._h, int(step), float(time), float(dt))
 

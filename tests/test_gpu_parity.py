@@ -203,3 +203,34 @@ def test_parameter_variants_match_oracle(extra):
         assert abs(d.dt - od["dt"]) <= HIST_TOL * od["dt"]
         assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"], extra
         assert abs(d.enstrophy - od["enstrophy"]) <= HIST_TOL * od["enstrophy"], extra
+
+
+def test_store_and_restore_match_oracle():
+    # storeData / restoreData (IdealGasSolver.hpp:333-405): two snapshots, the
+    # restore brings back the older one, its step and time, and scales the
+    # TIMESTEP value by 0.8; the steps after it follow the oracle's
+    m, p = make("jittered12", "VREMAN")
+    g, o = both(m, p)
+    g.advance_solution(4)
+    o.step(4)
+    g.store_data()
+    o.store()
+    g.advance_solution(3)
+    o.step(3)
+    g.store_data()
+    o.store()
+    g.advance_solution(2)
+    o.step(2)
+    g.restore_data()
+    o.restore()
+    d, od = g.report(), o.diag()
+    assert d.step == od["step"] == 4 and d.time == od["time"]
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    hist = g.run_history(4)
+    for dd in hist:
+        o.step()
+        od = o.diag()
+        assert dd.step == od["step"]
+        assert abs(dd.dt - od["dt"]) <= HIST_TOL * od["dt"]
+        assert abs(dd.ke - od["ke"]) <= HIST_TOL * od["ke"]
