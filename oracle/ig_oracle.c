@@ -165,6 +165,8 @@ struct og_solver {
   int64_t store_step, store_step_recent;
   double store_time, store_time_recent;
   double cfl_last;
+  og_source_fn hook; /* addSourceHook (IdealGasSolver.hpp:713), or NULL */
+  void* hook_user;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -498,6 +500,11 @@ void og_set_dsm_state(og_solver* s, const double* lijmij, const double* mijmij) 
   memcpy(s->lijmij, lijmij, sizeof(double) * s->m.ncv);
   memcpy(s->mijmij, mijmij, sizeof(double) * s->m.ncv);
   s->dsm_kept = 1;
+}
+
+void og_set_source_hook(og_solver* s, og_source_fn fn, void* user) {
+  s->hook = fn;
+  s->hook_user = user;
 }
 
 void og_set_time(og_solver* s, int64_t step, double time, double dt) {
@@ -986,15 +993,20 @@ int og_step(og_solver* s) {
     memset(s->rhs1_sc, 0, nn);
     memset(s->rhs2_sc, 0, nn);
   }
+  /* calcRhs = calcRhsBase + calcRhsScalars + addSourceHook(rhs, time, rk_stage)
+   * (IdealGasSolver.hpp:697-705), at the stage times t, t + dt, t + dt/2 */
   calc_rhs_into(s, s->rhs0, s->rhs0_sc);
+  if (s->hook) s->hook(s->hook_user, (double*)s->rhs0, n, s->time, 1);
   rk3_step(s, og_erk_wgt3[0]);
   s->time += s->dt;
   og_update_state(s);
   calc_rhs_into(s, s->rhs1, s->rhs1_sc);
+  if (s->hook) s->hook(s->hook_user, (double*)s->rhs1, n, s->time, 2);
   rk3_step(s, og_erk_wgt3[1]);
   s->time -= 0.5 * s->dt;
   og_update_state(s);
   calc_rhs_into(s, s->rhs2, s->rhs2_sc);
+  if (s->hook) s->hook(s->hook_user, (double*)s->rhs2, n, s->time, 3);
   rk3_step(s, og_erk_wgt3[2]);
   s->time += 0.5 * s->dt;
   og_update_state(s);

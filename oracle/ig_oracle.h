@@ -95,6 +95,9 @@ typedef struct og_diag {
 } og_diag;
 
 typedef struct og_solver og_solver;
+/* addSourceHook(IdealGasRhs* rhs, time, rk_stage) (IdealGasSolver.hpp:702,713):
+ * called after each stage's RHS, rhs [ncv][5] = (rho, rhou[3], rhoE) */
+typedef void (*og_source_fn)(void* user, double* rhs, int ncv, double time, int rk_stage);
 
 /* per-face flux functions (exported for the pin against the reference's own header) */
 void og_flux_skew(og_rhs* flux, const og_nc* fa, const og_state* cv0, const og_state* cv1,
@@ -140,6 +143,7 @@ void og_filter_cv(const og_mesh* m, const double* phi, double* phif, int ncomp);
 /* DSM_LOCAL L:M and M:M as a restart read sets them (kept by the next og_init_complete) */
 void og_set_dsm_state(og_solver* s, const double* lijmij, const double* mijmij);
 /* step counter, time and dt, as a restart read sets them */
+void og_set_source_hook(og_solver* s, og_source_fn fn, void* user);
 void og_set_time(og_solver* s, int64_t step, double time, double dt);
 
 #ifdef __cplusplus

@@ -43,6 +43,9 @@ class og_diag(C.Structure):
                 ("nan_flag", C.c_int)]
 
 
+# addSourceHook(IdealGasRhs* rhs, time, rk_stage): (user, rhs [ncv][5], ncv, time, stage)
+OG_SOURCE_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int)
+
 _lib = None
 _ref = None
 
@@ -91,6 +94,8 @@ def lib():
             getattr(L, f).restype = None
         L.og_calc_rhs_scalars.argtypes = [vp, dp]
         L.og_calc_rhs_scalars.restype = None
+        L.og_set_source_hook.argtypes = [vp, OG_SOURCE_FN, vp]
+        L.og_set_source_hook.restype = None
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
         _lib = L
@@ -223,6 +228,27 @@ class OracleSolver:
 
     def sgs_props(self):
         lib().og_sgs_props(self._h)
+
+    def sync_post_state(self):
+        """syncPostState (IdealGasSolver.hpp:323-330): primitives, gradient,
+        SGS and material properties of the current state."""
+        lib().og_update_state(self._h)
+        lib().og_sgs_props(self._h)
+
+    def set_source_hook(self, fn):
+        """fn(rhs [ncv][5] numpy view, time, rk_stage) adds to the stage RHS in place
+        (addSourceHook, IdealGasSolver.hpp:702,713); None removes it."""
+        if fn is None:
+            self._hook = None
+            lib().og_set_source_hook(self._h, OG_SOURCE_FN(), None)
+            return
+        ncv = self.ncv
+
+        def tramp(user, rhs, n, time, stage):
+            fn(np.ctypeslib.as_array(rhs, shape=(ncv, 5)), time, stage)
+
+        self._hook = OG_SOURCE_FN(tramp)
+        lib().og_set_source_hook(self._h, self._hook, None)
 
     def filter_cv(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)

@@ -9,6 +9,7 @@ against closed forms.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -50,7 +51,8 @@ def random_aux(rng, n):
     return a
 
 
-needs_ref = pytest.mark.skipif(orc.ref() is None, reason="reference pin harness oracle/_ref not built")
+# decided from the file's presence alone: collecting this module loads no library
+needs_ref = pytest.mark.skipif(not os.path.exists(orc.REF_SO), reason="reference pin harness oracle/_ref not built")
 
 
 @needs_ref
