@@ -112,6 +112,12 @@ typedef struct cg_params {
   int nsc;
   int res_time;            /* index of RES_TIME in sc_names, or -1 */
   char sc_names[8][32];
+  /* operator-build variants (FlowSolver::init, FlowSolver.hpp:680-700,762-852) */
+  int use_compact_faces;   /* USE_COMPACT_FACES: the flux runs over the compact faces alone
+                            * (n = n_fa, c = 0, compact grouping); the sensor and dcmag_hat
+                            * keep the extended operator */
+  int dvol_form;           /* DVOL_FORM: dcmag_hat from volume jumps over the extended pairs,
+                            * sum |n_ef| |V1 - V0| / (V1 + V0) / sum |n_ef| * V^(1/3) */
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,
@@ -126,6 +132,13 @@ typedef struct cg_diag {
   double cfl_max;     /* convective CFL of the current state at dt */
   int nan_flag;
 } cg_diag;
+
+/* A per-stage source term (IdealGasSolver::addSourceHook(IdealGasRhs* rhs, time,
+ * rk_stage), IdealGasSolver.hpp:713, called from calcRhs at :702): rhs[ncv][5] =
+ * (rho, rhou_x, rhou_y, rhou_z, rhoE) of the stage, in the mesh's cell order, to
+ * which the hook adds in place; time is the stage time (t, t + dt, t + dt/2) and
+ * rk_stage 1, 2 or 3. Return 0, or nonzero to stop the step with an error. */
+typedef int (*cg_source_fn)(void* user, double* rhs, int ncv, double time, int rk_stage);
 
 /* ---- parameters ---- */
 /* Reference defaults (IdealGasSolver() constructor, IdealGasSolver.hpp:147-155). */
@@ -210,6 +223,18 @@ int cg_create(const cg_mesh* m, const cg_params* p, cg_solver** out);
  * IdealGasSolver.hpp:96-98. u is [ncv][3]. */
 int cg_set_state(cg_solver* s, const double* rho, const double* u, const double* rhoE);
 int cg_get_state(cg_solver* s, double* rho, double* u, double* rhoE);
+/* syncPostState (IdealGasSolver.hpp:323-330): after the state was changed with
+ * cg_set_state between steps, recompute the primitives, the gradient and the SGS
+ * and material properties (DSM_LOCAL's L:M relaxation advances, as a call of
+ * calcSgsAndMaterialProperties does). Unlike cg_init_complete it takes no
+ * snapshot and leaves the relaxation state and the NaN flag alone. */
+int cg_sync_post_state(cg_solver* s);
+/* addSourceHook (IdealGasSolver.hpp:702,713): fn is called on the calling thread
+ * after the flux pass of every RK stage of cg_step with that stage's RHS (host
+ * memory, mesh order); while a hook is set the stages run unfused (flux pass into
+ * a stage RHS buffer, the hook, then the RK3 update), with a device-host round
+ * trip of the RHS per stage. fn = NULL removes it. Solves in batches have no hook. */
+int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user);
 /* Derived per-cell field by name: p, T, sos, mu_lam, mu_sgs, loc_lam, loc_sgs,
  * a_sgs, vv2, fgr, mu_total, loc_total, symp_fax, symh_fax, dudx ([ncv][3][3]),
  * cfl (per-cell convective CFL at the current dt); with SGS_MODEL DSM_LOCAL
@@ -226,9 +251,13 @@ int cg_init_complete(cg_solver* s);
 int cg_calc_rhs(cg_solver* s, int stage, double* rhs);
 /* IdealGasSolver::calcCfl (IdealGasSolver.cpp:1048-1095): max convective CFL at dt. */
 int cg_calc_cfl(cg_solver* s, double dt, double* cfl_max);
-/* nsteps iterations of FlowSolver::run's loop body (FlowSolver.hpp:2500-2528):
- * setDt then advanceSolution (IdealGasSolver.hpp:414-594), with the NaN
- * rollback of checkForNans/restoreData (:596-611, :377-405). */
+/* FlowSolver::run's loop body (FlowSolver.hpp:2490-2528) until the step counter
+ * has advanced by nsteps: setDt then advanceSolution (IdealGasSolver.hpp:414-594),
+ * with the NaN rollback of checkForNans/restoreData (:596-611, :377-405). NaNs
+ * are flagged on the device every step (later steps then do nothing) and acted on
+ * at the next CHECK_INTERVAL boundary or the last step: the older snapshot is
+ * restored with dt * 0.8 and the loop steps on, as the reference's loop does
+ * after a rollback; CG_ENAN after 64 rollbacks in one call. */
 int cg_step(cg_solver* s, int nsteps);
 /* report() (IdealGasSolver.hpp:805-848) of the current state. */
 int cg_get_diag(cg_solver* s, cg_diag* d);

@@ -81,6 +81,7 @@ class IdealGasSolver {
     cg_check(cg_mesh_get(mesh_, "x_cv", x_cv.data(), int64_t(sizeof(double) * x_cv.size())));
     cg_check(cg_mesh_get(mesh_, "vol", vol_cv.data(), int64_t(sizeof(double) * vol_cv.size())));
     cg_check(cg_create(mesh_, &params_, &solver_));
+    if (source_hook) cg_check(cg_set_source_hook(solver_, &IdealGasSolver::sourceTrampoline, this), solver_);
     rho.assign(ncv, params_.rho_ref);
     u.assign(3 * (size_t)ncv, 0.0);
     rhoE.assign(ncv, params_.p_ref / (params_.gamma - 1.0));
@@ -108,7 +109,26 @@ class IdealGasSolver {
     pullState();
   }
 
-  void syncPostState() { initComplete(); }
+  // IdealGasSolver::syncPostState (IdealGasSolver.hpp:323-330): primitives, gradient,
+  // SGS and material properties of the host state; no snapshot, no relaxation reset
+  void syncPostState() {
+    cg_check(cg_set_state(solver_, rho.data(), u.data(), rhoE.data()), solver_);
+    cg_check(cg_sync_post_state(solver_), solver_);
+    pullState();
+  }
+
+  // addSourceHook (IdealGasSolver.hpp:702,713): a subclass that overrides it and sets
+  // source_hook = true gets it called after every stage's RHS; rhs is [ncv]
+  // IdealGasRhs records in the mesh's cell order
+  struct IdealGasRhs {
+    double rho, rhou[3], rhoE;
+  };
+  virtual void addSourceHook(IdealGasRhs* rhs, const double time, const int rk_stage) {
+    (void)rhs;
+    (void)time;
+    (void)rk_stage;
+  }
+  bool source_hook = false;
 
   double calcCfl(double* cfl, const double dt_) const {
     double mx = 0.0;
@@ -218,7 +238,17 @@ class IdealGasSolver {
   // a hook that changed the conserved state gets the syncPostState treatment
   // (primitives, gradient and properties recomputed from the new state)
   void pushStateIfChanged() {
-    if (snap_rho_ != rho || snap_u_ != u || snap_rhoE_ != rhoE) initComplete();
+    if (snap_rho_ != rho || snap_u_ != u || snap_rhoE_ != rhoE) syncPostState();
+  }
+  static int sourceTrampoline(void* user, double* rhs, int n, double time, int rk_stage) {
+    static_assert(sizeof(IdealGasRhs) == 5 * sizeof(double), "IdealGasRhs is 5 doubles");
+    (void)n;
+    try {
+      static_cast<IdealGasSolver*>(user)->addSourceHook(reinterpret_cast<IdealGasRhs*>(rhs), time, rk_stage);
+    } catch (...) {
+      return 1;
+    }
+    return 0;
   }
 
   cg_params params_{};

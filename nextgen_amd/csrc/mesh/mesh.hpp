@@ -39,6 +39,8 @@ struct OperatorParams {
   int group_member_threshold = 100;   // GROUP_MEMBER_THRESHOLD
   bool disable_grouping = false;      // DISABLE_FA_GROUPING
   bool zero_c_ef = false;             // ZERO_C_EF
+  bool use_compact_faces = false;     // USE_COMPACT_FACES (FlowSolver.hpp:680-700)
+  bool dvol_form = false;             // DVOL_FORM (FlowSolver.hpp:762-852)
 };
 
 // Everything the time step consumes.

@@ -575,6 +575,26 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
     }
   }
 
+  // USE_COMPACT_FACES (FlowSolver.hpp:680-700): initFaStructures over the compact
+  // faces alone, with n = n_fa, c = 0 and the compact grouping (a grouped face takes
+  // its group representative's n_fa); the purely extended pairs carry no flux
+  // (n = c = 0) but stay in the pair lists for the sensor
+  if (P.use_compact_faces) {
+    op.n_use.assign(3 * size_t(nef), 0.0);
+    op.c_use.assign(3 * size_t(nef), 0.0);
+    std::vector<std::pair<int, int>> gl;
+    for (int f = 0; f < nfa; ++f) {
+      for (int d = 0; d < 3; ++d) op.n_use[3 * size_t(f) + d] = m.n_fa[3 * size_t(f) + d];
+      if (op.group_fa[f] >= 0) gl.push_back({op.group_fa[f], f});
+    }
+    std::stable_sort(gl.begin(), gl.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+    int cur_rep = 0;
+    for (size_t i = 0; i < gl.size(); ++i) {
+      if (i == 0 || gl[i].first != gl[i - 1].first) cur_rep = gl[i].second;
+      for (int d = 0; d < 3; ++d) op.n_use[3 * size_t(gl[i].second) + d] = m.n_fa[3 * size_t(cur_rep) + d];
+    }
+  }
+
   // ---- extended pairs of each cell, in pair order ----
   op.efocv_i.assign(ncv + 1, 0);
   for (int p = 0; p < nef; ++p) {
@@ -621,6 +641,23 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
     for (int foc = op.faocv_i[c]; foc < op.faocv_i[c + 1]; ++foc) area_sum += mag3(&m.n_fa[3 * op.faocv_v[foc]]);
     if (!(area_sum > 0.0)) return "cell without faces";
     op.dcmag_hat[c] /= area_sum;
+  }
+  // DVOL_FORM (FlowSolver.hpp:762-852): dcmag_hat from the volume jumps over the
+  // extended pairs, sum |n_ef| |V1 - V0| / (V1 + V0) / sum |n_ef|, times V^(1/3)
+  if (P.dvol_form) {
+    parallel_blocks(ncv, nthreads, [&](int, int b, int e) {
+      for (int c = b; c < e; ++c) {
+        double num = 0.0, wsum = 0.0;
+        for (int k = op.efocv_i[c]; k < op.efocv_i[c + 1]; ++k) {
+          const int p = op.efocv_v[k];
+          const double area = mag3(&op.n_ef[3 * size_t(p)]);
+          const double v0 = m.vol[op.lo[p]], v1 = m.vol[op.hi[p]];
+          num += area * std::fabs(v1 - v0) / (v1 + v0);
+          wsum += area;
+        }
+        op.dcmag_hat[c] = num / wsum * std::pow(m.vol[c], 1.0 / 3.0);
+      }
+    });
   }
   for (int f = 0; f < nfa; ++f) {
     const double nm = mag3(&m.n_fa[3 * f]);

@@ -146,6 +146,9 @@ struct HostInst {
   int id = -1;
   bool dsm_kept = false;             // DSM_LOCAL L:M, M:M set from a data file: the next
                                      // cg_init_complete keeps them
+  cg_source_fn hook = nullptr;       // addSourceHook: the stages run unfused while it is set
+  void* hook_user = nullptr;
+  std::vector<double> hook_buf;      // [ncv][5] host RHS in mesh order for the hook
 };
 
 template <class R>
@@ -362,6 +365,8 @@ cgmesh::OperatorParams op_params(const cg_params* p) {
     o.group_member_threshold = p->group_member_threshold;
     o.disable_grouping = p->disable_grouping != 0;
     o.zero_c_ef = p->zero_c_ef != 0;
+    o.use_compact_faces = p->use_compact_faces != 0;
+    o.dvol_form = p->dvol_form != 0;
   }
   return o;
 }
@@ -1375,6 +1380,10 @@ int cg_params_parse(cg_params* p, const char* text) {
       p->disable_grouping = 1;
     } else if (key == "ZERO_C_EF") {
       p->zero_c_ef = 1;
+    } else if (key == "USE_COMPACT_FACES") {
+      p->use_compact_faces = 1;
+    } else if (key == "DVOL_FORM") {
+      p->dvol_form = 1;
     } else if (key == "PRECISION") {
       const std::string m = tok.size() > 1 ? upper(tok[1]) : "";
       if (m == "FP64") p->precision = CG_PREC_FP64;
@@ -1382,8 +1391,10 @@ int cg_params_parse(cg_params* p, const char* text) {
       else return bad("expected PRECISION FP64 or MIXED");
     } else if (key == "EF_OPS") {
       if (tok.size() < 2 || upper(tok[1]) != "DEFAULT") return bad("only EF_OPS DEFAULT is available");
-    } else if (key == "DISS_SPONGE" || key == "USE_COMPACT_FACES" || key == "DVOL_FORM" ||
-               key == "ROTATING_REFERENCE_FRAME") {
+    } else if (key == "DISS_SPONGE" || key == "ROTATING_REFERENCE_FRAME" || key == "ADD_BODY_FORCE") {
+      // body forces (addBodyForceSourceTerms, IdealGasSolver.cpp:2159-2202) and the
+      // rotating frame are not on this path; a per-stage source can be added with
+      // cg_set_source_hook instead
       return bad("not available on this path");
     }
     // other keys (RESTART, NSTEPS, INIT, EOS, ...) belong to the driver
@@ -1611,26 +1622,86 @@ Instance<R>* find_inst(Engine<R>& E, int id) {
   return nullptr;
 }
 
-// the step loop of one instance: steps with NaN checks at CHECK_INTERVAL and
-// snapshots every 20 intervals (FlowSolver.hpp:2500-2528, IdealGasSolver.hpp:596-611)
+// One step with the stages unfused around the source hook: the flux pass writes
+// the stage RHS, the host hook adds its source in mesh order (addSourceHook,
+// IdealGasSolver.hpp:697-713, at the stage times t, t + dt, (t + dt) - dt/2 of
+// advanceSolution), then the RK3 update. Synchronises the host once per stage.
+template <class R>
+int hook_step(cg_solver* s, Engine<R>& E, Instance<R>* I) {
+  const cgk::Launch<R>& L = E.L;
+  hipStream_t st = s->stream;
+  if (int rc = read_info(s, I, s->info_host, st)) return rc;
+  const double t0 = s->info_host->time;
+  if (const int le = cgk::launch_begin_step(L, st)) return launch_fail(s, "begin_step", le);
+  if (int rc = read_info(s, I, s->info_host, st)) return rc;
+  const bool nan = s->info_host->nan != 0;
+  const double dt = s->info_host->dt;
+  double tt = t0;
+  const size_t n = (size_t)I->ncv;
+  I->hook_buf.resize(5 * n);
+  std::vector<double> rows(5 * n);
+  for (int stage = 1; stage <= 3; ++stage) {
+    if (!nan) {
+      if (const int le = cgk::launch_flux(stage, 0, L, E.cur, I->rhs_tmp, st)) return launch_fail(s, "flux", le);
+      HIPCHK(s, hipMemcpyAsync(rows.data(), I->rhs_tmp, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, st));
+      HIPCHK(s, hipStreamSynchronize(st));
+      for (size_t c = 0; c < n; ++c)
+        for (int q = 0; q < 5; ++q) I->hook_buf[(size_t)I->ext_of[c] * 5 + q] = rows[(size_t)q * n + c];
+      if (I->hook(I->hook_user, I->hook_buf.data(), (int)n, tt, stage) != 0)
+        return s->set(CG_EINVAL, "source hook returned an error at stage " + std::to_string(stage));
+      for (size_t c = 0; c < n; ++c)
+        for (int q = 0; q < 5; ++q) rows[(size_t)q * n + c] = I->hook_buf[(size_t)I->ext_of[c] * 5 + q];
+      HIPCHK(s, hipMemcpyAsync(I->rhs_tmp, rows.data(), sizeof(double) * 5 * n, hipMemcpyHostToDevice, st));
+    }
+    if (const int le = cgk::launch_rk_apply(stage, L, E.cur, I->rhs_tmp, st)) return launch_fail(s, "rk", le);
+    E.cur ^= 1;
+    if (const int le = cgk::launch_grad(L, E.cur, st)) return launch_fail(s, "grad", le);
+    if (stage == 1) tt += dt;
+    else if (stage == 2) tt -= 0.5 * dt;
+  }
+  if (const int le = cgk::launch_props(L, E.cur, 0, st)) return launch_fail(s, "props", le);
+  // the staging copies read host memory: finish before it goes
+  HIPCHK(s, hipStreamSynchronize(st));
+  return 0;
+}
+
+// rollbacks one call tolerates before it gives up (each scales dt by 0.8)
+constexpr int kMaxRollbacks = 64;
+
+// The run loop of one instance (FlowSolver.hpp:2490-2528): nsteps net steps.
+// NaNs are flagged on the device by every step's properties pass; once the
+// flag is set the following steps of the chunk do nothing, and at the chunk's
+// end (the next CHECK_INTERVAL boundary, or the last step) the host restores
+// the older snapshot with dt * 0.8 (checkForNans + restoreData,
+// IdealGasSolver.hpp:377-405,582-611) and steps on until the step counter has
+// advanced by nsteps -- the reference's loop, which continues after a rollback
+// until step reaches NSTEPS. Snapshots every 20 check intervals (report,
+// IdealGasSolver.hpp:805-812).
 template <class R>
 int solver_steps(cg_solver* s, Engine<R>& E, Instance<R>* I, int nsteps) {
   const int ci = std::max(1, I->prm.check_interval);
-  int it = 0;
-  while (it < nsteps) {
-    // run up to the next check point as one chunk
+  const int64_t target = I->host_step + nsteps;
+  int rollbacks = 0;
+  while (I->host_step < target) {
     const int64_t step0 = I->host_step;
     const int64_t next_check = (step0 / ci + 1) * ci;
-    const int chunk = (int)std::min<int64_t>(nsteps - it, next_check - step0);
-    if (int rc = engine_steps(s, E, s->stream, chunk, false)) return rc;
-    it += chunk;
+    const int chunk = (int)std::min<int64_t>(target - step0, next_check - step0);
+    if (I->hook) {
+      for (int k = 0; k < chunk; ++k)
+        if (int rc = hook_step(s, E, I)) return rc;
+    } else if (int rc = engine_steps(s, E, s->stream, chunk, false)) {
+      return rc;
+    }
     const int64_t step = step0 + chunk;
     I->host_step = step;
-    const bool last = (it == nsteps);
-    if (step % ci == 0 || last) {
+    if (step % ci == 0 || step == target) {
       bool rolled = false;
       if (int rc = check_nan_and_restore(s, E, I, s->info_host, s->stream, &rolled)) return rc;
-      if (!rolled && step % ci == 0 && step % (int64_t(ci) * 20) == 0)
+      if (rolled) {
+        if (++rollbacks > kMaxRollbacks) return s->set(CG_ENAN, "NaN after repeated rollbacks");
+        continue;
+      }
+      if (step % ci == 0 && step % (int64_t(ci) * 20) == 0)
         if (int rc = do_store(s, I, E.cur, s->info_host, s->stream)) return rc;
     }
   }
@@ -1706,6 +1777,26 @@ int cg_init_complete(cg_solver* s) {
     if (int rc = dsm_reset(s, I, s->stream)) return rc;
     if (int rc = refresh(s, I, E.cur, s->stream)) return rc;
     return do_store(s, I, E.cur, s->info_host, s->stream);
+  });
+}
+
+int cg_sync_post_state(cg_solver* s) {
+  if (!s) return fail(CG_EINVAL, "null solver");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (int rc = refresh(s, I, E.cur, s->stream)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return 0;
+  });
+}
+
+int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
+  if (!s) return fail(CG_EINVAL, "null solver");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    (void)E;
+    I->hook = fn;
+    I->hook_user = fn ? user : nullptr;
+    if (!fn) std::vector<double>().swap(I->hook_buf);
+    return 0;
   });
 }
 
@@ -1938,6 +2029,7 @@ int cg_time_steps(cg_solver* s, int nsteps, double* ms_total, double* kernel_ms)
   if (!s || !ms_total || nsteps < 1) return fail(CG_EINVAL, "bad argument");
   return with_inst(s, [&](auto& E, auto* I) -> int {
     if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    if (I->hook) return s->set(CG_ESTATE, "cg_time_steps times the fused step: remove the source hook first");
     if (int rc = engine_sync(s, E, s->stream)) return rc;
     hipEvent_t e0, e1;
     HIPCHK(s, hipEventCreate(&e0));

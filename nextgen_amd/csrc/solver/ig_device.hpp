@@ -210,8 +210,14 @@ template <class R>
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream);
 template <class R>
 int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream);
+// the 8 diagnostic sums of every instance; partial / out: device buffers for the
+// block partials and the sums (one instance; null: each instance's own buffers)
 template <class R>
-int launch_diag(const Launch<R>& L, int cur, void* stream);
+int launch_diag(const Launch<R>& L, int cur, void* stream, double* partial = nullptr, double* out = nullptr);
+// rk3Step of a stage from an FP64 stage RHS rhs[5][ncv] (device cell order), then
+// the stage's scalar pass (the unfused stage of a solve with a source hook)
+template <class R>
+int launch_rk_apply(int stage, const Launch<R>& L, int cur, const double* rhs, void* stream);
 template <class R>
 int launch_begin_step(const Launch<R>& L, void* stream);
 // FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]

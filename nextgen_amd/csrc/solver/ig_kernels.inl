@@ -722,6 +722,38 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   rk_update<STAGE>(g, in, out, w, P, c, self, acc);
 }
 
+// rk3Step of one stage from a stage RHS held in device memory, FP64 [5][ncv] in
+// device cell order: the unfused stage of a solve with a source hook (flux pass
+// into rhs, addSourceHook on that rhs, then this update, IdealGasSolver.hpp:697-705).
+// In FP64 the update is the fused epilogue's on the same values; in mixed
+// precision the energy row is gauged back into the FP32 register form.
+template <class R, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_rk_apply(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                     int cur, const double* __restrict__ rhs) {
+  int lblk;
+  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const GeomT<R> g = X.g;
+  const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
+  const WorkT<R> w = X.w;
+  const Params& P = X.P;
+  const int c = lblk * kBlock + threadIdx.x;
+  if (c >= g.ncv) return;
+  if (G(w.info)->nan) return;
+  const int ncv = g.ncv;
+  R acc[5];
+  if constexpr (kMixed<R>) {
+    #pragma clang fp contract(off)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = R(G(rhs)[(size_t)q * ncv + c]);
+    acc[4] = R(G(rhs)[(size_t)4 * ncv + c] - P.h_g * G(rhs)[c]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[q] = G(rhs)[(size_t)q * ncv + c];
+  }
+  const Light<R> self = load_light(in.light, c);
+  rk_update<STAGE>(g, in, out, w, P, c, self, acc);
+}
+
 // Passive scalars (calcRhsScalars and advanceScalars, IdealGasSolver.cpp:537-693,
 // 874-892), after the stage's flux pass, FP64: the scalar flux of a pair is the
 // pair mean of phi times the pair's mass flux F_rho (compact pairs: the stored pair
@@ -1707,9 +1739,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
 
 // Diagnostics: fixed-shape block reduction of 8 sums (deterministic order), FP64.
 // 0: rho V, 1-3: rho u V, 4: rhoE V, 5: 0.5 rho |u|^2 V, 6: 0.5 rho |w|^2 V, 7: V
+// partial: the block partials (null: the instance's own work buffer)
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
-                                                         int cur) {
+                                                         int cur, double* partial) {
   #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   int lblk;
@@ -1758,17 +1791,19 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
       for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 8) G(w.partial)[(size_t)lblk * 8 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 8) G(partial ? partial : w.partial)[(size_t)lblk * 8 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // one workgroup per instance: reduce its block partials in block order
+// (partial_in / out: the partials and the 8 sums, null: the instance's own buffers)
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict__ I) {
+__global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict__ I, const double* partial_in,
+                                                       double* out) {
   #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   const Inst<R>& X = I[blockIdx.x];
-  const auto partial = G(X.w.partial);
-  const auto diag = G(X.w.diag);
+  const auto partial = G(partial_in ? partial_in : X.w.partial);
+  const auto diag = G(out ? out : X.w.diag);
   const int nblocks = X.nblocks;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += kBlock)
@@ -1913,9 +1948,34 @@ int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream) {
 }
 
 template <class R>
-int launch_diag(const Launch<R>& L, int cur, void* stream) {
-  hipLaunchKernelGGL(k_diag_partial<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur);
-  hipLaunchKernelGGL(k_diag_final<R>, dim3(L.ninst), dim3(kBlock), 0, (hipStream_t)stream, L.I);
+int launch_diag(const Launch<R>& L, int cur, void* stream, double* partial, double* out) {
+  hipLaunchKernelGGL(k_diag_partial<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur,
+                     partial);
+  hipLaunchKernelGGL(k_diag_final<R>, dim3(L.ninst), dim3(kBlock), 0, (hipStream_t)stream, L.I,
+                     (const double*)partial, out);
+  return check();
+}
+
+template <class R>
+int launch_rk_apply(int stage, const Launch<R>& L, int cur, const double* rhs, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid(L.nblk), block(kBlock);
+  if (stage == 1) hipLaunchKernelGGL((k_rk_apply<R, 1>), grid, block, 0, s, L.I, L.bmap, cur, rhs);
+  else if (stage == 2) hipLaunchKernelGGL((k_rk_apply<R, 2>), grid, block, 0, s, L.I, L.bmap, cur, rhs);
+  else hipLaunchKernelGGL((k_rk_apply<R, 3>), grid, block, 0, s, L.I, L.bmap, cur, rhs);
+  if (int rc = check()) return rc;
+  if (!L.scalars) return 0;
+  // the scalar pass of the stage follows the flow update (it needs the new density)
+  const size_t lds = L.sc_lds;
+  if (lds) {
+    if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1, true>), grid, block, lds, s, L.I, L.bmap, cur);
+    else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2, true>), grid, block, lds, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_scalars<R, 3, true>), grid, block, lds, s, L.I, L.bmap, cur);
+  } else {
+    if (stage == 1) hipLaunchKernelGGL((k_scalars<R, 1, false>), grid, block, 0, s, L.I, L.bmap, cur);
+    else if (stage == 2) hipLaunchKernelGGL((k_scalars<R, 2, false>), grid, block, 0, s, L.I, L.bmap, cur);
+    else hipLaunchKernelGGL((k_scalars<R, 3, false>), grid, block, 0, s, L.I, L.bmap, cur);
+  }
   return check();
 }
 
@@ -1943,6 +2003,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_props<R>(const Launch<R>&, int, int, void*);                                \
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
   template int launch_cfl<R>(const Launch<R>&, int, double, void*);                               \
-  template int launch_diag<R>(const Launch<R>&, int, void*);
+  template int launch_diag<R>(const Launch<R>&, int, void*, double*, double*);                   \
+  template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);
 
 }  // namespace cgk

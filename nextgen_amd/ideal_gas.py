@@ -44,6 +44,7 @@ class cg_params(C.Structure):
         ("disable_grouping", C.c_int), ("zero_c_ef", C.c_int), ("precision", C.c_int),
         ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int), ("delta_r_dsm", C.c_double),
         ("nsc", C.c_int), ("res_time", C.c_int), ("sc_names", (C.c_char * 32) * 8),
+        ("use_compact_faces", C.c_int), ("dvol_form", C.c_int),
     ]
 
 
@@ -75,8 +76,12 @@ EXPORTS = [
     "cg_batch_last_error", "cg_batch_destroy", "cg_instance_create", "cg_instance_bytes", "cg_instance_destroy",
     "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
     "cg_instance_create_on", "cg_mles_counts", "cg_mles_get", "cg_mesh_read_mles", "cg_mesh_write_mles",
-    "cg_write_sles", "cg_read_sles", "cg_set_scalar", "cg_get_scalar",
+    "cg_write_sles", "cg_read_sles", "cg_set_scalar", "cg_get_scalar", "cg_sync_post_state",
+    "cg_set_source_hook",
 ]
+
+# addSourceHook(IdealGasRhs* rhs, time, rk_stage): (user, rhs [ncv][5], ncv, time, stage) -> 0
+CG_SOURCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int)
 
 _lib = None
 
@@ -118,6 +123,8 @@ def load_library(path: str = LIB_PATH):
         "cg_get_state": (C.c_int, [vp, dp, dp, dp]),
         "cg_get_field": (C.c_int, [vp, C.c_char_p, dp, C.c_int64]),
         "cg_init_complete": (C.c_int, [vp]),
+        "cg_sync_post_state": (C.c_int, [vp]),
+        "cg_set_source_hook": (C.c_int, [vp, CG_SOURCE_FN, vp]),
         "cg_calc_rhs": (C.c_int, [vp, C.c_int, dp]),
         "cg_calc_cfl": (C.c_int, [vp, C.c_double, dp]),
         "cg_step": (C.c_int, [vp, C.c_int]),
@@ -355,6 +362,32 @@ class IdealGasSolver:
 
     def init_complete(self):
         self._chk(load_library().cg_init_complete(self._h))
+
+    def sync_post_state(self):
+        """syncPostState (IdealGasSolver.hpp:323-330) after set_state between steps:
+        primitives, gradient, SGS and material properties; no snapshot."""
+        self._chk(load_library().cg_sync_post_state(self._h))
+
+    def set_source_hook(self, fn):
+        """addSourceHook (IdealGasSolver.hpp:702,713): fn(rhs, time, rk_stage) adds to
+        the stage RHS rhs [ncv][5] (mesh order) in place; None removes it. An
+        exception in fn stops the step with an error."""
+        lib = load_library()
+        if fn is None:
+            self._hook = None
+            self._chk(lib.cg_set_source_hook(self._h, CG_SOURCE_FN(), None))
+            return
+        ncv = self.ncv
+
+        def tramp(user, rhs, n, time, stage):
+            try:
+                fn(np.ctypeslib.as_array(rhs, shape=(ncv, 5)), time, stage)
+                return 0
+            except Exception:  # no exception crosses the C ABI
+                return 1
+
+        self._hook = CG_SOURCE_FN(tramp)
+        self._chk(lib.cg_set_source_hook(self._h, self._hook, None))
 
     def calc_rhs(self, stage=1):
         out = np.empty((self.ncv, 5))
