@@ -125,18 +125,11 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes_each_way": nbytes, "copy_GBps": bw / 1e9}))
 
 
-def k4_params(Mt=0.3, Re=1600.0, gamma=1.4, sgs="VREMAN", cfl=0.8):
-    """Decaying-turbulence deck: uniform rho = 1, p = 1/gamma (c0 = 1), u_rms = Mt,
-    MU_REF = Mt / Re (unit length)."""
-    return (f"GAMMA {gamma!r}\nP_REF {1.0 / gamma!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF {Mt / Re!r}\n"
-            f"MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL {sgs}\nTIMESTEP CFL {cfl!r}\n")
-
-
 SGS_NAMES = {"NONE": "no", "VREMAN": "Vreman", "SIGMA": "Sigma", "DSM_LOCAL": "DSM_LOCAL"}
 
 
 def workload(args):
-    from nextgen_amd.ic import tgv_params
+    from nextgen_amd.ic import k4_params, tgv_params
     from nextgen_amd.ideal_gas import Mesh
 
     t0 = time.time()

@@ -1964,6 +1964,7 @@ int cg_read_sles(cg_solver* s, const char* path) {
     if (int rc = set_state(s, I, E.cur, r.rho.data(), r.u.data(), r.rhoE.data(), s->stream)) return rc;
     if (int rc = read_info(s, I, s->info_host, s->stream)) return rc;
     if (r.have_step) s->info_host->step = r.step;
+    I->host_step = s->info_host->step;  // the run loop counts net steps from here
     if (r.have_time) s->info_host->time = r.time;
     if (r.have_dt) s->info_host->dt = r.dt;
     for (const auto& rec : r.other) {

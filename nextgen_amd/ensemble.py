@@ -13,15 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .ic import tgv_params
+from .ic import k4_params, tgv_params
 from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, IdealGasSolver, Mesh, parse_params
-
-
-def k4_params(Mt=0.3, Re=1600.0, gamma=1.4, sgs="VREMAN", cfl=0.8):
-    """Decaying-turbulence deck: uniform rho = 1, p = 1/gamma (c0 = 1), u_rms = Mt,
-    MU_REF = Mt / Re (unit length)."""
-    return (f"GAMMA {gamma!r}\nP_REF {1.0 / gamma!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF {Mt / Re!r}\n"
-            f"MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL {sgs}\nTIMESTEP CFL {cfl!r}\n")
 
 
 @dataclass

@@ -32,6 +32,13 @@ def tgv_params(Re=1600.0, Ma=0.1, gamma=1.4, sgs="NONE", cfl=0.8):
     )
 
 
+def k4_params(Mt=0.3, Re=1600.0, gamma=1.4, sgs="VREMAN", cfl=0.8):
+    """Decaying-turbulence deck: uniform rho = 1, p = 1/gamma (c0 = 1), u_rms = Mt,
+    MU_REF = Mt / Re (unit length)."""
+    return (f"GAMMA {gamma!r}\nP_REF {1.0 / gamma!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF {Mt / Re!r}\n"
+            f"MU_POWER_LAW 0.75\nPR_LAM 0.7\nSGS_MODEL {sgs}\nTIMESTEP CFL {cfl!r}\n")
+
+
 def tgv_state(x_cv, Ma=0.1, gamma=1.4):
     x, y, z = x_cv[:, 0], x_cv[:, 1], x_cv[:, 2]
     c0 = 1.0 / Ma

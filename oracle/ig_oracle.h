@@ -118,6 +118,9 @@ extern const double og_erk_wgt3[3][3];
 
 og_solver* og_create(const og_mesh* m, const og_params* p);
 void og_destroy(og_solver* s);
+/* threads of the oracle's loops (default 1); the results are bit-identical for
+ * every count */
+void og_set_threads(og_solver* s, int n);
 int og_ncv(const og_solver* s);
 void og_set_state(og_solver* s, const double* rho, const double* u, const double* rhoE);
 void og_get_state(const og_solver* s, double* rho, double* u, double* rhoE);

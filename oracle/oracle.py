@@ -60,6 +60,8 @@ def lib():
         L.og_create.restype = vp
         L.og_create.argtypes = [C.POINTER(og_mesh), C.POINTER(og_params)]
         L.og_destroy.argtypes = [vp]
+        L.og_set_threads.argtypes = [vp, C.c_int]
+        L.og_set_threads.restype = None
         L.og_set_state.argtypes = [vp, dp, dp, dp]
         L.og_get_state.argtypes = [vp, dp, dp, dp]
         L.og_get_field.argtypes = [vp, C.c_char_p, dp]
@@ -156,9 +158,10 @@ def params_from(cgp) -> og_params:
 
 
 class OracleSolver:
-    """The oracle step on the operator arrays of a product mesh."""
+    """The oracle step on the operator arrays of a product mesh. `threads` > 1
+    runs its loops on that many threads, bit-identical to one thread."""
 
-    def __init__(self, mesh, cgparams):
+    def __init__(self, mesh, cgparams, threads=1):
         self._keep = {}
         m = og_mesh()
         m.ncv, m.nfa, m.nef = mesh.ncv, mesh.nfa, mesh.nef
@@ -177,6 +180,7 @@ class OracleSolver:
         self._p = params_from(cgparams)
         self.ncv = mesh.ncv
         self._h = lib().og_create(C.byref(self._m), C.byref(self._p))
+        lib().og_set_threads(self._h, int(threads))
 
     def set_state(self, rho, u, rhoE):
         rho = np.ascontiguousarray(rho, dtype=np.float64)

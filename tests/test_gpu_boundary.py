@@ -108,7 +108,7 @@ def test_sync_post_state_keeps_dsm_relaxation():
 def make_source(A, log):
     def src(rhs, time, stage, x=None, vol=None):
         log.append((time, stage))
-        f = A * vol * np.sin(x[:, 1]) * (1.0 + 3.0 * time)
+        f = A * vol * (1.0 + np.sin(x[:, 1])) * (1.0 + 3.0 * time)
         rhs[:, 1] += f
         rhs[:, 4] += 0.5 * f
     return src
@@ -128,7 +128,8 @@ def test_source_hook_matches_oracle():
         close(g.report(), o.diag(), "hook")
     assert [s for _, s in lg] == [1, 2, 3] * 6
     assert np.allclose([t for t, _ in lg], [t for t, _ in lo], rtol=1e-12, atol=0)
-    # the source moved the flow: momentum x grew against a run without it
+    # the source moved the flow: the TGV carries no net x momentum, the source's
+    # mean A (1 + 3t) per unit volume adds it
     assert o.diag()["sums"][1] > 1e-4
 
 
