@@ -84,7 +84,7 @@ def test_ensemble_solves_equal_solo_runs_and_oracle(ensemble, precision):
             close_to(d, od, tol, cons)
         print(f"batched solve {i} (n {specs[i].n}, {specs[i].ic}) matches the oracle over 10 steps", flush=True)
     # the autotuner's variants (each advances the batch), then graph replays
-    report = b.tune(1)
+    report = b.tune(2)
     print(report.splitlines()[-1] if report else "", flush=True)
     b.step(6)
     steps = b.diag(ids[0]).step
