@@ -261,6 +261,15 @@ int cg_calc_cfl(cg_solver* s, double dt, double* cfl_max);
 int cg_step(cg_solver* s, int nsteps);
 /* report() (IdealGasSolver.hpp:805-848) of the current state. */
 int cg_get_diag(cg_solver* s, cg_diag* d);
+/* cg_get_diag without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device or host memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it). The same kernels and
+ * summation order, so the same bits as cg_get_diag; the block partials, the
+ * sums and the CFL maximum live in a 4 MiB device array of the library (one
+ * per GPU, held until the process ends), which limits the solve to 16,776,704
+ * cells (CG_EINVAL beyond). Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+int cg_get_diag_static(cg_solver* s, cg_diag* d);
 /* Run nsteps steps and record the diagnostics after every step (diag[nsteps]). */
 int cg_step_history(cg_solver* s, int nsteps, cg_diag* diag);
 /* storeData / restoreData (IdealGasSolver.hpp:333-405). */

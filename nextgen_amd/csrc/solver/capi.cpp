@@ -1888,6 +1888,78 @@ int cg_get_diag(cg_solver* s, cg_diag* d) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// Device scratch of cg_get_diag_static, one copy per GPU for the life of the
+// process: [0, 8) the eight sums, [8] the CFL maximum (bit pattern of a
+// non-negative double), [16, 16 + 8 * nblocks) the block partials. The name is
+// a host-side stand-in in host code: kernels get the per-device address from
+// hipGetSymbolAddress, never the name.
+constexpr size_t kScratchDoubles = size_t(1) << 19;  // 4 MiB
+constexpr int kScratchMaxBlocks = (int)((kScratchDoubles - 16) / 8);
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// the calling thread's current device's copy of `scratch` (looked up once per device)
+int scratch_on_current_device(Err* e, double** out) {
+  int dev = -1;
+  HIPCHK(e, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchMaxDevices) return e->set(CG_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(e, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return e->set(CG_EHIP, "no device address for the diagnostics scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// get_diag with the reduction partials, the sums and the CFL maximum in the
+// device scratch: the same kernels and summation order, so the same bits
+template <class R>
+int get_diag_static(Err* s, Instance<R>* I, int cur, cgdev::StepInfo* host, cg_diag* d, hipStream_t stream) {
+  if (I->nblocks > kScratchMaxBlocks)
+    return s->set(CG_EINVAL, "cg_get_diag_static: " + std::to_string(I->ncv) + " cells exceed the scratch limit of " +
+                                 std::to_string((int64_t)kScratchMaxBlocks * cgdev::kBlock) + " cells");
+  double* sc = nullptr;
+  if (int rc = scratch_on_current_device(s, &sc)) return rc;
+  // the CFL maximum starts at zero (+0.0 has the all-zero bit pattern); a fill
+  // kernel rather than a memset, so the call can sit in a captured graph
+  if (const int le = cgk::launch_fill(sc + 8, 0.0, 1, stream)) return launch_fail(s, "fill", le);
+  if (const int le = cgk::launch_diag(I->solo_launch(), cur, stream, sc + 16, sc)) return launch_fail(s, "diag", le);
+  double v[9];
+  if (int rc = read_info(s, I, host, stream)) return rc;
+  if (const int le = cgk::launch_cfl(I->solo_launch(), cur, host->dt, stream, reinterpret_cast<unsigned long long*>(sc + 8)))
+    return launch_fail(s, "cfl", le);
+  HIPCHK(s, hipMemcpyAsync(v, sc, sizeof(v), hipMemcpyDeviceToHost, stream));
+  HIPCHK(s, hipStreamSynchronize(stream));
+  d->step = host->step;
+  d->time = host->time;
+  d->dt = host->dt;
+  d->ke = v[5] / v[7];
+  d->enstrophy = v[6] / v[7];
+  for (int q = 0; q < 5; ++q) d->sums[q] = v[q];
+  d->nan_flag = host->nan;
+  d->cfl_max = v[8];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int cg_get_diag_static(cg_solver* s, cg_diag* d) {
+  if (!s || !d) return fail(CG_EINVAL, "null argument");
+  return with_inst(s, [&](auto& E, auto* I) -> int {
+    if (!I->ready) return s->set(CG_ESTATE, "call cg_init_complete first");
+    return get_diag_static(s, I, E.cur, s->info_host, d, s->stream);
+  });
+}
+
 int cg_step_history(cg_solver* s, int nsteps, cg_diag* diag) {
   if (!s || !diag) return fail(CG_EINVAL, "null argument");
   for (int i = 0; i < nsteps; ++i) {

@@ -208,8 +208,11 @@ template <class R>
 // mu_lam, ...); 2 the copies again for a read of the current state (DSM_LOCAL's
 // relaxation state is then left as it is)
 int launch_props(const Launch<R>& L, int cur, int fields, void* stream);
+// per-cell convective CFL into the work field cfl, or (max_bits not null) only
+// its maximum into *max_bits as the bit pattern of a non-negative double (the
+// caller zeroes it first)
 template <class R>
-int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream);
+int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream, unsigned long long* max_bits = nullptr);
 // the 8 diagnostic sums of every instance; partial / out: device buffers for the
 // block partials and the sums (one instance; null: each instance's own buffers)
 template <class R>

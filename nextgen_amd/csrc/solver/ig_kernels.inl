@@ -1704,7 +1704,7 @@ __global__ void k_begin_step(const Inst<R>* __restrict__ I, int ninst) {
 // per-cell convective CFL at a given dt
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur,
-                                                double dt) {
+                                                double dt, unsigned long long* max_bits) {
   int lblk;
   const Inst<R>& X = inst_of(I, bmap, lblk);
   const GeomT<R> g = X.g;
@@ -1734,7 +1734,13 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
   }
   double cd = double(cfl);
   cd *= 0.5 * dt * G(g.inv_vol)[c];
-  G(w.cfl)[c] = cd;
+  if (!max_bits) {
+    G(w.cfl)[c] = cd;
+    return;
+  }
+  // the maximum alone, by the bit pattern of the non-negative value (a NaN
+  // is skipped, as a running max with std::max semantics skips it)
+  if (cd == cd) cfl_max(max_bits, (unsigned long long)__double_as_longlong(cd));
 }
 
 // Diagnostics: fixed-shape block reduction of 8 sums (deterministic order), FP64.
@@ -1942,8 +1948,9 @@ int launch_begin_step(const Launch<R>& L, void* stream) {
 }
 
 template <class R>
-int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream) {
-  hipLaunchKernelGGL(k_cfl<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur, dt);
+int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream, unsigned long long* max_bits) {
+  hipLaunchKernelGGL(k_cfl<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur, dt,
+                     max_bits);
   return check();
 }
 
@@ -2002,7 +2009,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_prims<R>(const Launch<R>&, int, int, int, double*, void*);                  \
   template int launch_props<R>(const Launch<R>&, int, int, void*);                                \
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
-  template int launch_cfl<R>(const Launch<R>&, int, double, void*);                               \
+  template int launch_cfl<R>(const Launch<R>&, int, double, void*, unsigned long long*);          \
   template int launch_diag<R>(const Launch<R>&, int, void*, double*, double*);                   \
   template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);
 

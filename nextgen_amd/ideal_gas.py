@@ -77,7 +77,7 @@ EXPORTS = [
     "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
     "cg_instance_create_on", "cg_mles_counts", "cg_mles_get", "cg_mesh_read_mles", "cg_mesh_write_mles",
     "cg_write_sles", "cg_read_sles", "cg_set_scalar", "cg_get_scalar", "cg_sync_post_state",
-    "cg_set_source_hook",
+    "cg_set_source_hook", "cg_get_diag_static",
 ]
 
 # addSourceHook(IdealGasRhs* rhs, time, rk_stage): (user, rhs [ncv][5], ncv, time, stage) -> 0
@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
         "cg_calc_cfl": (C.c_int, [vp, C.c_double, dp]),
         "cg_step": (C.c_int, [vp, C.c_int]),
         "cg_get_diag": (C.c_int, [vp, P(cg_diag)]),
+        "cg_get_diag_static": (C.c_int, [vp, P(cg_diag)]),
         "cg_step_history": (C.c_int, [vp, C.c_int, P(cg_diag)]),
         "cg_store": (C.c_int, [vp]),
         "cg_restore": (C.c_int, [vp]),
