@@ -14,6 +14,10 @@ same mesh and initial field, timed the same way.
   python bench.py --ensemble 256 [--ens-steps 300]       (batched ensemble vs one at a time)
   python bench.py --serve 24 [--loads 0.5 1 2]            (serving loop vs sequential FP64)
 
+The default run also appends bounded `ensemble` (48 solves, 100 steps) and
+`serving` (10 requests at 4x the sequential FP64 service rate) objects
+(--no-extras skips them).
+
 Timing: W untimed steps, then exactly K steps bracketed by stream
 synchronisation and HIP events on the solver's stream (state resident in HBM).
 A second pass of K steps records HIP events around every kernel launch for the
@@ -25,9 +29,12 @@ launch of that kernel measured with rocprofv3 FETCH_SIZE and WRITE_SIZE passes
 (scaled by the plain-copy calibration) and recorded in profiles/traffic.json
 for this workload; null when no measurement of this workload is recorded.
 
-`cpu_baseline`: the oracle (the FP64 C restatement of the reference step,
-single thread) timed on a bounded sample of the same workload: two RK3 steps
-of the same 2.1M-cell mesh and initial field.
+`cpu_baseline`: the oracle (the FP64 C restatement of the reference step) on
+the host's cores (OpenMP threads, at most 16), timed on a bounded sample of
+the same workload (one RK3 step of the same 2.1M-cell mesh and field after a
+warm-up step) in two builds: the oracle's (-O3 -ffp-contract=off) and one for
+the host CPU (-O3 -march=native); also in the reference's own unit,
+core-s/Mcv/step.
 """
 import argparse
 import json
@@ -194,6 +201,93 @@ def recorded_traffic(key, precision, n, ic):
     return rec.get(f"{key}/{precision}/n{n}/{ic}", {}).get("bytes_per_launch")
 
 
+def host_cores():
+    """Host threads this process may use (the GPU box grants 16 per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle_build():
+    """ig_oracle.c compiled for this host CPU (-O3 -march=native, the
+    production-like build; contraction left at gcc's default), or None."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        return None, "no C compiler on this host"
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    cmd = [cc, "-O3", "-march=native", "-std=c11", "-fPIC", "-fopenmp", "-shared", "-o", out,
+           os.path.join(ROOT, "oracle", "ig_oracle.c"), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return None, r.stderr[-300:]
+    return out, " ".join(cmd[1:6])
+
+
+def cpu_baseline(mesh, text, state, steps=1):
+    """The oracle (FP64 C restatement of the reference step) on the host's
+    cores: one warm-up step, then `steps` timed RK3 steps of the same mesh and
+    field, in the oracle build (-O3 -ffp-contract=off) and in a build for this
+    host CPU (-O3 -march=native). Reports cell-stage updates/s of the faster
+    build and core-s/Mcv/step (FlowSolver.hpp:2596-2605) of both."""
+    from nextgen_amd.ideal_gas import parse_params
+    from oracle.oracle import ORACLE_SO, OracleSolver
+
+    cores = host_cores()
+    ncv = mesh.ncv
+    builds = [("oracle", ORACLE_SO, "gcc -O3 -ffp-contract=off -fopenmp")]
+    native, flags = native_oracle_build()
+    if native:
+        builds.append(("native", native, "gcc " + flags + " -fopenmp"))
+    else:
+        log(f"cpu baseline: no native build ({flags})")
+    res = {}
+    for name, path, how in builds:
+        t0 = time.time()
+        o = OracleSolver(mesh, parse_params(text), threads=cores, libpath=path)
+        o.set_state(*state)
+        o.init_complete()
+        o.step(1)
+        t1 = time.time()
+        o.step(steps)
+        sec = (time.time() - t1) / steps
+        o.close()
+        res[name] = {"s_per_step": sec, "cell_stage_per_s": 3.0 * ncv / sec,
+                     "core_s_per_Mcv_per_step": cores * sec / (ncv * 1e-6), "compiler": how}
+        log(f"cpu baseline ({name}, {cores} threads): {sec:.2f} s/step, "
+            f"{cores * sec / (ncv * 1e-6):.2f} core-s/Mcv/step (ready in {t1 - t0:.1f}s)")
+    best = min(res, key=lambda k: res[k]["s_per_step"])
+    b = res[best]
+    return {
+        "value": b["cell_stage_per_s"],
+        "unit": "cell-stage updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} RK3 step(s) after one warm-up step, the same {ncv}-cell mesh and initial field, "
+                  f"FP64 C restatement of the reference step on {cores} OpenMP threads ({cpu_model()}); "
+                  f"value from the {best} build ({b['compiler']})",
+        "cpu_model": cpu_model(),
+        "builds": res,
+        "core_s_per_Mcv_per_step": b["core_s_per_Mcv_per_step"],
+    }
+
+
 def accuracy(args):
     """Mixed precision against FP64 on the same mesh and field over N steps."""
     mesh, text, state, desc = workload(args)
@@ -229,25 +323,29 @@ def accuracy(args):
     return 0
 
 
-def ensemble_bench(args):
-    """Batched ensemble: `args.ensemble` solves of uneven size in one batch (one
-    fused launch per kernel, graph replay), FP64 and mixed precision, against
-    the same solves run one at a time (a uniform sample, scaled to the set)."""
+def ensemble_leg(count, steps, seed, tune_steps=2, sample_count=16):
+    """Batched ensemble: `count` solves of uneven size (n ~ U{16..48}, jitter
+    0.3h, Re, Ma, TGV or k^4 as the ensemble draws them) in one batch (one fused
+    launch per kernel, graph replay, autotuned variant), FP64 and mixed, against
+    the same solves run one at a time in FP64 (a uniform sample, scaled). All
+    solves are resident and advance together, so each completes at the batch's
+    makespan: the per-solve completion times are recorded and their p50 / p99
+    reported."""
     from nextgen_amd.batch import Batch
     from nextgen_amd.ensemble import ensemble_specs, run_alone
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
+    import numpy as np
 
-    specs = ensemble_specs(args.ensemble, seed=args.seed)
+    specs = ensemble_specs(count, seed=seed)
     t0 = time.time()
     meshes = [sp.mesh() for sp in specs]
     cells = sum(m.ncv for m in meshes)
     log(f"ensemble: {len(specs)} meshes, {cells} cells, built in {time.time() - t0:.1f}s")
     states = [sp.state(m) for sp, m in zip(specs, meshes)]
-    out = {"mode": "ensemble", "instances": len(specs), "cells": cells, "steps": args.ens_steps,
+    out = {"instances": len(specs), "cells": cells, "steps": steps,
            "spec": f"n ~ U{{16..48}} (jitter 0.3h), Re ~ U[400,3200], Ma ~ U[0.05,0.3], TGV or k^4 by coin flip, "
-                   f"seed {args.seed}"}
+                   f"seed {seed}"}
     kes = {}
-    tune_steps = 2
     for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
         t0 = time.time()
         b = Batch(prec)
@@ -261,80 +359,102 @@ def ensemble_bench(args):
         report = b.tune(tune_steps)
         chosen = [ln for ln in report.splitlines() if ln.startswith("chosen")]
         log(f"{name}: tuner {chosen[0] if chosen else report}")
-        ms = b.time_steps(args.ens_steps)
+        ms = b.time_steps(steps)
+        done = [ms * 1e-3 for _ in ids]  # lockstep: every solve's last step ends with the batch's
         kes[name] = [b.diag(i) for i in ids]
         b.close()
         sec = ms * 1e-3
         out[name] = {"seconds": sec, "solves_per_s": len(specs) / sec,
-                     "cell_stage_per_s": 3.0 * cells * args.ens_steps / sec,
-                     "latency_p50_s": sec, "latency_p99_s": sec, "setup_s": setup,
-                     "device_bytes": cnt["device_bytes"], "tuner": report}
-        log(f"{name}: {args.ens_steps} steps of all instances in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s")
+                     "cell_stage_per_s": 3.0 * cells * steps / sec,
+                     "latency_p50_s": float(np.percentile(done, 50)), "latency_p99_s": float(np.percentile(done, 99)),
+                     "setup_s": setup, "device_bytes": cnt["device_bytes"],
+                     "tuner": chosen[0] if chosen else report}
+        log(f"{name}: {steps} steps of all instances in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s")
     errs = [abs(a.ke - b.ke) / b.ke for a, b in zip(kes["mixed"], kes["fp64"])]
     out["mixed"]["ke_rel_err_vs_fp64_max"] = max(errs)
     out["tune_steps_per_variant"] = tune_steps
-    # one at a time, FP64 (every 16th solve, scaled to the whole set)
-    sample = list(range(0, len(specs), max(1, len(specs) // 16)))
+    # one at a time, FP64 (a uniform sample, scaled to the whole set)
+    sample = list(range(0, len(specs), max(1, len(specs) // sample_count)))
     t_alone = 0.0
     for i in sample:
-        sec, _ = run_alone(specs[i], CG_PREC_FP64, args.ens_steps)
+        sec, _ = run_alone(specs[i], CG_PREC_FP64, steps)
         t_alone += sec
     seq = t_alone * len(specs) / len(sample)
     out["sequential_fp64"] = {"seconds_estimate": seq, "solves_per_s": len(specs) / seq,
                               "sample": f"{len(sample)} of {len(specs)} solves timed alone (GPU stepping only), scaled"}
     out["speedup_vs_sequential_fp64"] = {k: out[k]["solves_per_s"] / out["sequential_fp64"]["solves_per_s"]
                                          for k in ("fp64", "mixed")}
+    return out
+
+
+def ensemble_bench(args):
+    out = {"mode": "ensemble", **ensemble_leg(args.ensemble, args.ens_steps, args.seed)}
     print(json.dumps(out), flush=True)
     return 0
 
 
-def serve_bench(args):
-    """Serving loop (continuous batching, mixed precision) against sequential
-    FP64 execution of the same requests, over a sweep of arrival rates."""
+def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True):
+    """Serving loop (continuous batching, mixed precision) against the same
+    requests one at a time (FP64, and in the server's own precision), over
+    arrival rates given as multiples of the sequential FP64 service rate. Each
+    request has its own mesh (built before the clock); its setup (device
+    layout, state, initComplete) counts in both executions."""
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
-    from nextgen_amd.serve import MeshPool, Server, build_layouts, requests_poisson, sequential, summarize
+    from nextgen_amd.serve import MeshPool, Server, requests_poisson, sequential, summarize
 
-    sizes = [27, 33, 41, 50, 62, 76, 94, 115, 126]
     pool = MeshPool()
-    base = requests_poisson(1.0, args.serve, sizes, seed=args.seed)
+    base = requests_poisson(1.0, count, None, seed=seed)
     t0 = time.time()
-    for r in base:
-        pool.get(r.spec)
-    log(f"serving: {len(pool.meshes)} meshes built in {time.time() - t0:.1f}s")
-    # the mesh catalogue's device layouts are resident for both executions
-    t0 = time.time()
-    lay64 = build_layouts(pool, CG_PREC_FP64)
-    laymx = build_layouts(pool, CG_PREC_MIXED)
-    log(f"serving: layouts resident in {time.time() - t0:.1f}s "
-        f"({sum(v.bytes() for v in lay64.values()) / 1e9:.1f} GB FP64, {sum(v.bytes() for v in laymx.values()) / 1e9:.1f} GB mixed)")
-    # sequential FP64 service times (independent of the arrival rate)
-    seq = sequential(pool, base, CG_PREC_FP64, layouts=lay64)
-    service = {r.rid: r.done - r.start for r in seq}
-    mean_service = sum(service.values()) / len(service)
-    ke64 = {r.rid: r.ke for r in seq}
-    out = {"mode": "serving", "requests": len(base), "mesh_sides": sizes,
-           "spec": "cells log-uniform [2e4, 2e6] (nearest mesh side), horizon U{200..2000} steps, Re U[400,3200], "
-                   "Ma U[0.05,0.3], TGV or k^4; Poisson arrivals",
-           "sequential_fp64_mean_service_s": mean_service, "sweep": []}
-    for load in args.loads:
-        rate = load / mean_service
-        reqs = [type(r)(r.rid, r.arrival / rate, r.spec) for r in base]
-        # sequential queueing with the measured service times
-        clock, seq_res = 0.0, []
+    pool.build([r.spec for r in base])
+    cells = [r.spec.n ** 3 for r in base]
+    log(f"serving: {len(base)} request meshes ({min(cells)}..{max(cells)} cells) built in {time.time() - t0:.1f}s")
+    seq = {}
+    for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
+        if name == "mixed" and not seq_mixed:
+            continue
+        t0 = time.time()
+        seq[name] = sequential(pool, base, prec)
+        log(f"serving: sequential {name} over all requests in {time.time() - t0:.1f}s")
+    service = {k: {r.rid: r.done - r.start for r in v} for k, v in seq.items()}
+    mean_service = sum(service["fp64"].values()) / len(base)
+    ke64 = {r.rid: r.ke for r in seq["fp64"]}
+    out = {"requests": len(base), "cells": cells, "horizons": [r.spec.steps for r in base],
+           "spec": "cells log-uniform [2e4, 2e6] (own mesh per request), horizon U{200..2000} steps, "
+                   "Re U[400,3200], Ma U[0.05,0.3], TGV or k^4; Poisson arrivals; setup counted in both",
+           "sequential_fp64_mean_service_s": mean_service, "workers": workers, "sweep": []}
+
+    def queue(reqs, svc):
+        clock, res = 0.0, []
         for r in sorted(reqs, key=lambda q: q.arrival):
             start = max(clock, r.arrival)
-            clock = start + service[r.rid]
-            seq_res.append(type(seq[0])(r.rid, r.arrival, start, clock, 0, 0, 0.0, 0.0, 0.0))
-        srv = Server(pool, CG_PREC_MIXED, hbm_budget=args.hbm_budget * 1e9, layouts=laymx)
+            clock = start + svc[r.rid]
+            res.append(type(seq["fp64"][0])(r.rid, r.arrival, start, clock, 0, 0, 0.0, 0.0, 0.0))
+        return res
+
+    for load in loads:
+        rate = load / mean_service
+        reqs = [type(r)(r.rid, r.arrival / rate, r.spec) for r in base]
+        srv = Server(pool, CG_PREC_MIXED, hbm_budget=hbm_budget, workers=workers)
         res, info = srv.run(reqs)
         errs = [abs(r.ke - ke64[r.rid]) / ke64[r.rid] for r in res]
         row = {"load": load, "rate_per_s": rate, "served_mixed": summarize(res, info["makespan"]),
-               "sequential_fp64": summarize(seq_res), "ke_rel_err_vs_fp64_max": max(errs),
-               "batch_steps": info["batch_steps"], "peak_device_bytes": srv.peak_bytes}
+               "sequential_fp64": summarize(queue(reqs, service["fp64"])),
+               "ke_rel_err_vs_fp64_max": max(errs), "batch_steps": info["batch_steps"],
+               "peak_device_bytes": srv.peak_bytes}
+        if "mixed" in service:
+            row["sequential_mixed"] = summarize(queue(reqs, service["mixed"]))
+            row["solves_per_s_ratio_vs_sequential_mixed"] = (row["served_mixed"]["solves_per_s"] /
+                                                             row["sequential_mixed"]["solves_per_s"])
         row["solves_per_s_ratio"] = row["served_mixed"]["solves_per_s"] / row["sequential_fp64"]["solves_per_s"]
         out["sweep"].append(row)
-        log(f"load {load}: served {row['served_mixed']['solves_per_s']:.3f}/s p99 {row['served_mixed']['tts_p99_s']:.1f}s"
-            f" vs sequential {row['sequential_fp64']['solves_per_s']:.3f}/s p99 {row['sequential_fp64']['tts_p99_s']:.1f}s")
+        log(f"load {load}: served {row['served_mixed']['solves_per_s']:.3f}/s p50 {row['served_mixed']['tts_p50_s']:.1f}s "
+            f"p99 {row['served_mixed']['tts_p99_s']:.1f}s vs sequential FP64 {row['sequential_fp64']['solves_per_s']:.3f}/s "
+            f"p99 {row['sequential_fp64']['tts_p99_s']:.1f}s ({row['solves_per_s_ratio']:.2f}x)")
+    return out
+
+
+def serve_bench(args):
+    out = {"mode": "serving", **serve_leg(args.serve, args.loads, args.seed, args.hbm_budget * 1e9)}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -361,6 +481,11 @@ def main():
     ap.add_argument("--loads", type=float, nargs="+", default=[0.5, 1.0, 2.0],
                     help="arrival rates as multiples of the sequential FP64 service rate")
     ap.add_argument("--hbm-budget", type=float, default=96.0, help="GB of device memory for admitted solves")
+    ap.add_argument("--no-extras", action="store_true", help="skip the bounded ensemble and serving legs")
+    ap.add_argument("--extra-ensemble", type=int, default=48, help="solves of the default run's ensemble leg")
+    ap.add_argument("--extra-ens-steps", type=int, default=100)
+    ap.add_argument("--extra-serve", type=int, default=10, help="requests of the default run's serving leg")
+    ap.add_argument("--extra-load", type=float, default=4.0)
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -391,30 +516,7 @@ def main():
     main_prec = "fp64" if "fp64" in res else "mixed"
     r = res[main_prec]
 
-    cpu = None
-    if not args.no_cpu:
-        from nextgen_amd.ideal_gas import parse_params
-        from oracle.oracle import OracleSolver
-
-        t0 = time.time()
-        o = OracleSolver(mesh, parse_params(text))
-        o.set_state(*state)
-        o.init_complete()
-        log(f"oracle ready in {time.time() - t0:.1f}s")
-        nsteps = 2
-        t0 = time.time()
-        o.step(nsteps)
-        dt_cpu = time.time() - t0
-        cpu = {
-            "value": 3.0 * ncv * nsteps / dt_cpu,
-            "unit": "cell-stage updates/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{nsteps} RK3 steps of the same {ncv}-cell mesh and initial field, FP64 C restatement "
-                      f"(gcc -O2 -ffp-contract=off), single thread: {dt_cpu:.2f} s",
-        }
-        log(f"cpu baseline: {dt_cpu / nsteps:.2f}s per step")
-        o.close()
+    cpu = None if args.no_cpu else cpu_baseline(mesh, text, state)
 
     dom = max(r["kr"], key=lambda k: r["kr"][k]["ms_per_step"])
     kd = r["kr"][dom]
@@ -474,6 +576,12 @@ def main():
             "diag": {"step": mx["diag"].step, "ke": mx["diag"].ke, "enstrophy": mx["diag"].enstrophy,
                      "ke_rel_err_vs_fp64": abs(mx["diag"].ke - r["diag"].ke) / r["diag"].ke},
         }
+    if not args.no_extras:
+        # bounded forms of the ensemble and serving workloads (the metric's
+        # second half); full runs: --ensemble 256, --serve N
+        out["ensemble"] = ensemble_leg(args.extra_ensemble, args.extra_ens_steps, args.seed, sample_count=8)
+        out["serving"] = serve_leg(args.extra_serve, [args.extra_load], args.seed, args.hbm_budget * 1e9,
+                                   seq_mixed=True)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -94,6 +94,8 @@ struct Alloc {
   Alloc(const Alloc&) = delete;
   Alloc& operator=(const Alloc&) = delete;
   ~Alloc() {
+    if (ptrs.empty()) return;
+    DevLock lk(g_dev_mu);  // hipFree may wait on the device: never during another thread's capture
     for (void* p : ptrs) (void)hipFree(p);
   }
 };
@@ -236,8 +238,12 @@ namespace {
     if (e_ != hipSuccess) return (s)->set(CG_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device allocations and uploads take the library lock one call at a time, so a
+// layout built on a worker thread (host-side work of seconds for large meshes)
+// waits only for the short device calls while another thread captures a graph.
 template <class T>
 int dev_alloc(Err* e, Alloc* h, T** p, size_t n) {
+  DevLock lk(g_dev_mu);
   void* q = nullptr;
   if (n == 0) n = 1;
   HIPCHK(e, hipMalloc(&q, n * sizeof(T)));
@@ -249,6 +255,7 @@ int dev_alloc(Err* e, Alloc* h, T** p, size_t n) {
 
 template <class T>
 int upload(Err* e, Alloc* h, const T** dst, const std::vector<T>& v) {
+  DevLock lk(g_dev_mu);
   T* d = nullptr;
   if (int rc = dev_alloc(e, h, &d, v.size())) return rc;
   if (!v.empty()) HIPCHK(e, hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -256,15 +263,37 @@ int upload(Err* e, Alloc* h, const T** dst, const std::vector<T>& v) {
   return 0;
 }
 
-// upload FP64 host data in the working precision R
-template <class R>
-int upload_as(Err* e, Alloc* h, const R** dst, const std::vector<double>& v) {
-  if constexpr (std::is_same<R, double>::value) {
-    return upload(e, h, dst, v);
-  } else {
-    std::vector<R> t(v.begin(), v.end());
-    return upload(e, h, dst, t);
+// a host array of n elements left uninitialised (filled by the caller's
+// threads: a zero-filling std::vector would first-touch it on one thread)
+template <class T>
+struct HostBuf {
+  std::unique_ptr<T[]> p;
+  size_t n = 0;
+  explicit HostBuf(size_t n_) : p(new T[n_ ? n_ : 1]), n(n_) {}
+  T& operator[](size_t i) { return p[i]; }
+};
+
+template <class T>
+int upload(Err* e, Alloc* h, const T** dst, const HostBuf<T>& v) {
+  DevLock lk(g_dev_mu);
+  T* d = nullptr;
+  if (int rc = dev_alloc(e, h, &d, v.n)) return rc;
+  if (v.n) HIPCHK(e, hipMemcpy(d, v.p.get(), v.n * sizeof(T), hipMemcpyHostToDevice));
+  *dst = d;
+  return 0;
+}
+
+// f(lo, hi) over [0, n) in contiguous chunks on the host's threads
+template <class F>
+void par_range(int64_t n, F&& f) {
+  const int64_t nth = std::max<int64_t>(1, std::min<int64_t>(cgmesh::default_threads(), (n + 32767) / 32768));
+  if (nth <= 1) {
+    f(int64_t(0), n);
+    return;
   }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nth; ++t) th.emplace_back([&, t] { f(n * t / nth, n * (t + 1) / nth); });
+  for (auto& x : th) x.join();
 }
 
 // frees its own memory; the layout goes with the last solve that uses it
@@ -492,15 +521,17 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   std::vector<int> int_of(ncv);
   for (int c = 0; c < ncv; ++c) int_of[ext_of[c]] = c;
   std::vector<double> inv_vol(ncv), vol(ncv), sa_to_vol(ncv), dcmag(ncv), dx13(ncv), dx23(ncv);
-  for (int c = 0; c < ncv; ++c) {
-    const int e = ext_of[c];
-    inv_vol[c] = o.inv_vol[e];
-    vol[c] = o.vol[e];
-    sa_to_vol[c] = o.sa_to_vol[e];
-    dcmag[c] = o.dcmag_hat[e];
-    dx13[c] = std::pow(o.vol[e], 1.0 / 3.0);
-    dx23[c] = std::pow(o.vol[e], 2.0 / 3.0);
-  }
+  par_range(ncv, [&](int64_t c0, int64_t c1) {
+    for (int64_t c = c0; c < c1; ++c) {
+      const int e = ext_of[c];
+      inv_vol[c] = o.inv_vol[e];
+      vol[c] = o.vol[e];
+      sa_to_vol[c] = o.sa_to_vol[e];
+      dcmag[c] = o.dcmag_hat[e];
+      dx13[c] = std::pow(o.vol[e], 1.0 / 3.0);
+      dx23[c] = std::pow(o.vol[e], 2.0 / 3.0);
+    }
+  });
   int rc = 0;
   rc |= upload(s, h, &g.inv_vol, inv_vol);
   rc |= upload(s, h, &g.vol, vol);
@@ -541,79 +572,89 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     const std::vector<double>& nU = o.nU();
     const std::vector<double>& cU = o.cU();
     const std::vector<double>& nf = m->vor.n_fa;
-    std::vector<int> cnbr(cs.nslots, 0), cpair(cs.nslots, 0), xnbr(xs.nslots, 0);
-    // raw face area vector per compact slot (the CFL sum streams it in rows
-    // instead of gathering it by pair index)
-    std::vector<double> cfn[3];
-    for (auto& v : cfn) v.assign(cs.nslots, 0.0);
+    // slot arrays: every slot written below (padding slots zero, or -1 in the
+    // pair columns), by the host's threads; the per-slot geometry is gathered
+    // from the per-pair arrays on the device (launch_layout_geom)
+    HostBuf<int> cnbr(cs.nslots), cpair(cs.nslots), xnbr(xs.nslots), xpair(xs.nslots);
     // test-filter weight of each compact face (filterCvR1/R2, StaticSolver.cpp:9849-9975):
-    // |n_fa| |x_vv1 - x_vv0| / 6, x_vv1 the periodic image, per compact slot
-    std::vector<double> fw(nfa), cfw(cs.nslots, 0.0);
+    // |n_fa| |x_vv1 - x_vv0| / 6, x_vv1 the periodic image
+    std::vector<double> fw(nfa);
     {
       const cgmesh::Voronoi& v = m->vor;
-      for (int f = 0; f < nfa; ++f) {
-        const int a = v.cvofa[2 * (size_t)f], b = v.cvofa[2 * (size_t)f + 1];
-        double d[3], nn[3];
-        for (int k = 0; k < 3; ++k) {
-          const double x1 = v.x_vv[3 * (size_t)b + k] + double(v.shift[3 * (size_t)f + k]) * v.L[k];
-          d[k] = x1 - v.x_vv[3 * (size_t)a + k];
-          nn[k] = v.n_fa[3 * (size_t)f + k];
+      par_range(nfa, [&](int64_t f0, int64_t f1) {
+        for (int64_t f = f0; f < f1; ++f) {
+          const int a = v.cvofa[2 * (size_t)f], b = v.cvofa[2 * (size_t)f + 1];
+          double d[3], nn[3];
+          for (int k = 0; k < 3; ++k) {
+            const double x1 = v.x_vv[3 * (size_t)b + k] + double(v.shift[3 * (size_t)f + k]) * v.L[k];
+            d[k] = x1 - v.x_vv[3 * (size_t)a + k];
+            nn[k] = v.n_fa[3 * (size_t)f + k];
+          }
+          const double area = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+          const double delta = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          fw[f] = area * delta / 6.0;
         }
-        const double area = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
-        const double delta = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        fw[f] = area * delta / 6.0;
-      }
+      });
     }
-    // extended pair geometry per slot (SoA; both cells of a pair hold a copy)
-    std::vector<double> x6[6];
-    for (auto& v : x6) v.assign(xs.nslots, 0.0);
     // neighbour (device numbering) | the cell is the pair's hi side << 31;
     // e is the cell's mesh number
     auto code_of = [&](int e, int pidx) {
       const bool hi = (o.hi[pidx] == e);
       return int(uint32_t(int_of[hi ? o.lo[pidx] : o.hi[pidx]]) | (hi ? 0x80000000u : 0u));
     };
-    for (int q = 0; q < ncv; ++q) {
-      const int e = ext_of[xcell[q]];
-      for (int k = 0; k < ccnt[q]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[e] + k];
-        const int64_t sidx = cs.slot(q, k);
-        cnbr[sidx] = code_of(e, pidx);
-        cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == e) ? 0x80000000u : 0u));
-        for (int d = 0; d < 3; ++d) cfn[d][sidx] = nf[3 * (size_t)pidx + d];
-        cfw[sidx] = fw[pidx];
-      }
-      for (int k = 0; k < xcnt[q]; ++k) {
-        const int pidx = o.efocv_v[o.efocv_i[e] + ccnt[q] + k];
-        const int64_t sidx = xs.slot(q, k);
-        xnbr[sidx] = code_of(e, pidx);
-        for (int d = 0; d < 3; ++d) {
-          x6[d][sidx] = nU[3 * (size_t)pidx + d];
-          x6[3 + d][sidx] = cU[3 * (size_t)pidx + d];
+    // by slices of 64 lane positions (a slice owns its rows ptr[sl] .. ptr[sl+1])
+    const int64_t nslice = (ncv + cgdev::kSlice - 1) / cgdev::kSlice;
+    par_range(nslice, [&](int64_t sl0, int64_t sl1) {
+      for (int64_t sl = sl0; sl < sl1; ++sl) {
+        for (int64_t i = cs.ptr[sl]; i < cs.ptr[sl + 1]; ++i) {
+          cnbr[i] = 0;
+          cpair[i] = -1;
+        }
+        for (int64_t i = xs.ptr[sl]; i < xs.ptr[sl + 1]; ++i) {
+          xnbr[i] = 0;
+          xpair[i] = -1;
+        }
+        const int q1 = (int)std::min<int64_t>(ncv, (sl + 1) * cgdev::kSlice);
+        for (int q = (int)(sl * cgdev::kSlice); q < q1; ++q) {
+          const int e = ext_of[xcell[q]];
+          for (int k = 0; k < ccnt[q]; ++k) {
+            const int pidx = o.efocv_v[o.efocv_i[e] + k];
+            const int64_t sidx = cs.slot(q, k);
+            cnbr[sidx] = code_of(e, pidx);
+            cpair[sidx] = int(uint32_t(pidx) | ((o.hi[pidx] == e) ? 0x80000000u : 0u));
+          }
+          for (int k = 0; k < xcnt[q]; ++k) {
+            const int pidx = o.efocv_v[o.efocv_i[e] + ccnt[q] + k];
+            const int64_t sidx = xs.slot(q, k);
+            xnbr[sidx] = code_of(e, pidx);
+            xpair[sidx] = pidx;
+          }
         }
       }
-    }
+    });
     // compact pairs (pair index = face index): per-pair SoA for the thread-per-pair kernel
     {
       std::vector<int> plo(nfa), phi(nfa);
-      std::vector<double> p11[11];
+      std::vector<R> p11[11];
       for (auto& v : p11) v.resize(nfa);
-      for (int f = 0; f < nfa; ++f) {
-        plo[f] = int_of[o.lo[f]];
-        phi[f] = int_of[o.hi[f]];
-        for (int d = 0; d < 3; ++d) {
-          p11[d][f] = nU[3 * (size_t)f + d];
-          p11[3 + d][f] = cU[3 * (size_t)f + d];
-          p11[6 + d][f] = o.unit_n_use[3 * (size_t)f + d];
+      par_range(nfa, [&](int64_t f0, int64_t f1) {
+        for (int64_t f = f0; f < f1; ++f) {
+          plo[f] = int_of[o.lo[f]];
+          phi[f] = int_of[o.hi[f]];
+          for (int d = 0; d < 3; ++d) {
+            p11[d][f] = R(nU[3 * (size_t)f + d]);
+            p11[3 + d][f] = R(cU[3 * (size_t)f + d]);
+            p11[6 + d][f] = R(o.unit_n_use[3 * (size_t)f + d]);
+          }
+          p11[9][f] = R(o.area_use[f]);
+          p11[10][f] = R(o.aod_use[f]);
         }
-        p11[9][f] = o.area_use[f];
-        p11[10][f] = o.aod_use[f];
-      }
+      });
       rc |= upload(s, h, &g.plo, plo);
       rc |= upload(s, h, &g.phi, phi);
       const R** pdst[11] = {&g.pn0, &g.pn1, &g.pn2, &g.pc0, &g.pc1, &g.pc2,
                             &g.pu0, &g.pu1, &g.pu2, &g.parea, &g.paod};
-      for (int q = 0; q < 11; ++q) rc |= upload_as<R>(s, h, pdst[q], p11[q]);
+      for (int q = 0; q < 11; ++q) rc |= upload(s, h, pdst[q], p11[q]);
     }
     // per-block halos of the extended half-faces for LDS staging
     {
@@ -679,16 +720,43 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     rc |= upload(s, h, &g.ef_nc, ccnt);
     rc |= upload(s, h, &g.cf_nbr, cnbr);
     rc |= upload(s, h, &g.cf_pair, cpair);
-    rc |= upload_as<R>(s, h, &g.cfn0, cfn[0]);
-    rc |= upload_as<R>(s, h, &g.cfn1, cfn[1]);
-    rc |= upload_as<R>(s, h, &g.cfn2, cfn[2]);
-    rc |= upload(s, h, &g.cfw, cfw);
     rc |= upload(s, h, &g.xf_ptr, xs.ptr);
     rc |= upload(s, h, &g.xf_deg, xcnt);
     rc |= upload(s, h, &g.xf_nbr, xnbr);
-    const R** xdst[6] = {&g.xsn0, &g.xsn1, &g.xsn2, &g.xsc0, &g.xsc1, &g.xsc2};
-    for (int q = 0; q < 6; ++q) rc |= upload_as<R>(s, h, xdst[q], x6[q]);
     if (rc) return rc;
+    // per-slot geometry, gathered on the device from the per-pair arrays
+    // (uploaded once, freed below): extended n, c in R; compact n_fa in R and
+    // the filter weight
+    {
+      Alloc tmp;
+      const double *d_nU = nullptr, *d_cU = nullptr, *d_nf = nullptr, *d_fw = nullptr;
+      const int* d_xp = nullptr;
+      rc |= upload(s, &tmp, &d_nU, nU);
+      rc |= upload(s, &tmp, &d_cU, cU);
+      rc |= upload(s, &tmp, &d_nf, nf);
+      rc |= upload(s, &tmp, &d_fw, fw);
+      rc |= upload(s, &tmp, &d_xp, xpair);
+      R* x6[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+      R* cn[3] = {nullptr, nullptr, nullptr};
+      double* cw = nullptr;
+      for (auto& p : x6) rc |= dev_alloc(s, h, &p, (size_t)xs.nslots);
+      for (auto& p : cn) rc |= dev_alloc(s, h, &p, (size_t)cs.nslots);
+      rc |= dev_alloc(s, h, &cw, (size_t)cs.nslots);
+      if (rc) return rc;
+      DevLock lk(g_dev_mu);
+      hipStream_t st = nullptr;
+      HIPCHK(s, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      const int le = cgk::launch_layout_geom<R>(xs.nslots, d_xp, d_nU, d_cU, x6, cs.nslots, g.cf_pair, d_nf, d_fw, cn,
+                                                cw, st);
+      const hipError_t se = hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+      if (le) return launch_fail(s, "layout geometry", le);
+      if (se != hipSuccess) return s->set(CG_EHIP, std::string("layout geometry: ") + hipGetErrorString(se));
+      g.xsn0 = x6[0]; g.xsn1 = x6[1]; g.xsn2 = x6[2];
+      g.xsc0 = x6[3]; g.xsc1 = x6[4]; g.xsc2 = x6[5];
+      g.cfn0 = cn[0]; g.cfn1 = cn[1]; g.cfn2 = cn[2];
+      g.cfw = cw;
+    }
   }
   // gradient stencil without the diagonal
   {
@@ -702,23 +770,31 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     for (int q = 0; q < ncv; ++q) cnt[q] = int(key[gcell[q]]);
     Sell sl;
     sl.build(ncv, cnt);
-    std::vector<int> nbr(sl.nslots, 0);
-    std::vector<double> c0(sl.nslots, 0.0), c1(sl.nslots, 0.0), c2(sl.nslots, 0.0);
-    for (int q = 0; q < ncv; ++q)
-      for (int k = 0; k < cnt[q]; ++k) {
-        const int coc = o.cvocv_i[ext_of[gcell[q]]] + 1 + k;
-        const int64_t sidx = sl.slot(q, k);
-        nbr[sidx] = int_of[o.cvocv_v[coc]];
-        c0[sidx] = o.grad_coeff[3 * (size_t)coc];
-        c1[sidx] = o.grad_coeff[3 * (size_t)coc + 1];
-        c2[sidx] = o.grad_coeff[3 * (size_t)coc + 2];
+    HostBuf<int> nbr(sl.nslots);
+    HostBuf<R> c0(sl.nslots), c1(sl.nslots), c2(sl.nslots);
+    const int64_t nslice = (ncv + cgdev::kSlice - 1) / cgdev::kSlice;
+    par_range(nslice, [&](int64_t s0, int64_t s1) {
+      for (int64_t i = sl.ptr[s0]; i < sl.ptr[s1]; ++i) {
+        nbr[i] = 0;
+        c0[i] = c1[i] = c2[i] = R(0);
       }
+      const int q1 = (int)std::min<int64_t>(ncv, s1 * cgdev::kSlice);
+      for (int q = (int)(s0 * cgdev::kSlice); q < q1; ++q)
+        for (int k = 0; k < cnt[q]; ++k) {
+          const int coc = o.cvocv_i[ext_of[gcell[q]]] + 1 + k;
+          const int64_t sidx = sl.slot(q, k);
+          nbr[sidx] = int_of[o.cvocv_v[coc]];
+          c0[sidx] = R(o.grad_coeff[3 * (size_t)coc]);
+          c1[sidx] = R(o.grad_coeff[3 * (size_t)coc + 1]);
+          c2[sidx] = R(o.grad_coeff[3 * (size_t)coc + 2]);
+        }
+    });
     rc |= upload(s, h, &g.gr_ptr, sl.ptr);
     rc |= upload(s, h, &g.gr_deg, sl.deg);
     rc |= upload(s, h, &g.gr_nbr, nbr);
-    rc |= upload_as<R>(s, h, &g.gc0, c0);
-    rc |= upload_as<R>(s, h, &g.gc1, c1);
-    rc |= upload_as<R>(s, h, &g.gc2, c2);
+    rc |= upload(s, h, &g.gc0, c0);
+    rc |= upload(s, h, &g.gc1, c1);
+    rc |= upload(s, h, &g.gc2, c2);
     if (rc) return rc;
   }
   // XCD-ordered block maps for launches over one instance
@@ -726,6 +802,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   rc |= dev_alloc(s, h, &Lay->solo_pbmap, (size_t)std::max(Lay->npblocks, 1));
   if (rc) return rc;
   const std::vector<int> order = xcd_order(Lay->nblocks);
+  DevLock lk(g_dev_mu);
   HIPCHK(s, hipMemcpy(Lay->solo_bmap, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
   const std::vector<int> porder = xcd_order(Lay->npblocks);
   if (!porder.empty())
@@ -2216,7 +2293,7 @@ int cg_batch_create(int precision, cg_batch** out) {
 }
 
 int cg_layout_create(const cg_mesh* m, int precision, cg_layout** out) {
-  DevLock lk(g_dev_mu);
+  // no library lock around the build: its device calls take it one by one
   if (!m || !out) return fail(CG_EINVAL, "null argument");
   if (precision != CG_PREC_FP64 && precision != CG_PREC_MIXED) return fail(CG_EINVAL, "unsupported precision");
   *out = nullptr;
@@ -2256,10 +2333,13 @@ int create_instance(cg_instance* x, std::shared_ptr<Layout<R>> lay, const cg_mes
   hipStream_t stream = nullptr;
   cgdev::StepInfo* host = nullptr;
   int rc = 0;
-  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&host, sizeof(cgdev::StepInfo)) != hipSuccess)
+  // the layout's host-side build runs without the library lock (its device
+  // calls take it one by one); the solve's buffers and initComplete under it
+  if (!lay) rc = make_layout(x, m, !std::getenv("CG_NO_LDS"), &lay);
+  DevLock lk(g_dev_mu);
+  if (!rc && (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+              hipHostMalloc((void**)&host, sizeof(cgdev::StepInfo)) != hipSuccess))
     rc = x->set(CG_EHIP, "no HIP device, stream or pinned memory");
-  if (!rc && !lay) rc = make_layout(x, m, !std::getenv("CG_NO_LDS"), &lay);
   if (!rc) rc = make_instance_on(x, lay, p, stream, out);
   // the state goes to st[0]; attaching to a batch at parity 1 swaps the buffers
   if (!rc) rc = set_state(x, *out, 0, rho, u, rhoE, stream);
@@ -2298,7 +2378,6 @@ int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* 
 
 int cg_instance_create(const cg_mesh* m, const cg_params* p, const double* rho, const double* u,
                        const double* rhoE, cg_instance** out) {
-  DevLock lk(g_dev_mu);
   if (!m || !p || !rho || !u || !rhoE || !out) return fail(CG_EINVAL, "null argument");
   if (int rc = check_params(p)) return rc;
   *out = nullptr;

@@ -226,6 +226,12 @@ int launch_begin_step(const Launch<R>& L, void* stream);
 // FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]
 template <class R>
 int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, void* stream);
+// layout setup: the per-slot pair geometry (extended n, c; compact n_fa and
+// filter weight) gathered from the per-pair FP64 arrays on the device
+template <class R>
+int launch_layout_geom(int64_t nx, const int* xpair, const double* nU, const double* cU, R* const x6[6], int64_t nc,
+                       const int* cpair, const double* nfa, const double* fw, R* const cn[3], double* cw,
+                       void* stream);
 int launch_fill(double* p, double v, int64_t n, void* stream);
 int launch_fill(float* p, float v, int64_t n, void* stream);
 }  // namespace cgk

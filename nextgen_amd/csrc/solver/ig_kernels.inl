@@ -1986,6 +1986,52 @@ int launch_rk_apply(int stage, const Launch<R>& L, int cur, const double* rhs, v
   return check();
 }
 
+// Layout setup: per-slot pair geometry from the per-pair arrays (one thread
+// per slot). Extended slots: n, c of pair xpair[i] in the working precision
+// (xpair < 0: a padding slot, zeros). Compact slots: the raw area vector n_fa
+// and the test-filter weight of pair cpair[i] & 0x7fffffff (cpair == -1: padding).
+template <class R>
+__global__ void k_layout_geom(int64_t nx, const int* __restrict__ xpair, const double* __restrict__ nU,
+                              const double* __restrict__ cU, R* xn0, R* xn1, R* xn2, R* xc0, R* xc1, R* xc2,
+                              int64_t nc, const int* __restrict__ cpair, const double* __restrict__ nfa,
+                              const double* __restrict__ fw, R* cn0, R* cn1, R* cn2, double* cw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nx) {
+    const int p = xpair[i];
+    if (p >= 0) {
+      const size_t b = 3 * (size_t)p;
+      xn0[i] = R(nU[b]); xn1[i] = R(nU[b + 1]); xn2[i] = R(nU[b + 2]);
+      xc0[i] = R(cU[b]); xc1[i] = R(cU[b + 1]); xc2[i] = R(cU[b + 2]);
+    } else {
+      xn0[i] = R(0); xn1[i] = R(0); xn2[i] = R(0);
+      xc0[i] = R(0); xc1[i] = R(0); xc2[i] = R(0);
+    }
+  }
+  if (i < nc) {
+    const int code = cpair[i];
+    if (code != -1) {
+      const size_t f = (size_t)(code & 0x7fffffff);
+      cn0[i] = R(nfa[3 * f]); cn1[i] = R(nfa[3 * f + 1]); cn2[i] = R(nfa[3 * f + 2]);
+      cw[i] = fw[f];
+    } else {
+      cn0[i] = R(0); cn1[i] = R(0); cn2[i] = R(0);
+      cw[i] = 0.0;
+    }
+  }
+}
+
+template <class R>
+int launch_layout_geom(int64_t nx, const int* xpair, const double* nU, const double* cU, R* const x6[6], int64_t nc,
+                       const int* cpair, const double* nfa, const double* fw, R* const cn[3], double* cw,
+                       void* stream) {
+  const int64_t n = nx > nc ? nx : nc;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_layout_geom<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nx,
+                     xpair, nU, cU, x6[0], x6[1], x6[2], x6[3], x6[4], x6[5], nc, cpair, nfa, fw, cn[0], cn[1], cn[2],
+                     cw);
+  return check();
+}
+
 #ifdef CGK_COMMON
 int launch_fill(double* p, double v, int64_t n, void* stream) {
   if (n <= 0) return 0;
@@ -2011,6 +2057,8 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
   template int launch_cfl<R>(const Launch<R>&, int, double, void*, unsigned long long*);          \
   template int launch_diag<R>(const Launch<R>&, int, void*, double*, double*);                   \
-  template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);
+  template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);               \
+  template int launch_layout_geom<R>(int64_t, const int*, const double*, const double*, R* const[6], int64_t, \
+                                     const int*, const double*, const double*, R* const[3], double*, void*);
 
 }  // namespace cgk

@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .ic import k4_params, tgv_params
-from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, IdealGasSolver, Mesh, parse_params
+from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, Mesh, parse_params
 
 
 @dataclass
@@ -73,13 +73,15 @@ def build_batch(specs, precision, log=None):
 
 
 def run_alone(sp: Spec, precision, steps):
-    """One solve through cg_create/cg_step; returns (seconds of GPU stepping, diag)."""
+    """One solve alone on the GPU (a batch of one: the same graph replays as a
+    batch); returns (seconds of GPU stepping, diag)."""
+    from .batch import Batch
+
     m = sp.mesh()
-    g = IdealGasSolver(m, sp.deck(precision))
-    g.set_state(*sp.state(m))
-    g.init_complete()
+    b = Batch(precision)
+    iid = b.add(m, sp.deck(precision), *sp.state(m))
     m.close()
-    ms, _ = g.time_steps(steps)
-    d = g.report()
-    g.close()
+    ms = b.time_steps(steps)
+    d = b.diag(iid)
+    b.close()
     return ms * 1e-3, d

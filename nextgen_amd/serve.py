@@ -4,22 +4,27 @@ Requests (a periodic jittered-Voronoi solve: mesh size, initial field, Re,
 Ma, horizon in steps) arrive over time. The server
   * admits arrived requests first-come first-served while the device memory
     held by admitted solves stays within an HBM budget;
-  * prepares an admitted request (device layout, state, initComplete) on a
-    worker thread through cg_instance_create while the batch keeps stepping;
+  * prepares admitted requests (device layout of the request's own mesh,
+    state, initComplete) on worker threads through cg_instance_create while
+    the batch keeps stepping (the layout's host-side build holds no library
+    lock, so it overlaps the batch's stepping and graph captures);
   * attaches prepared solves to the running batch between steps, so every
     kernel launch covers all resident solves (cg_batch_step);
   * retires a solve at its horizon (its result read back, its memory freed)
     and records its time to solution = completion - arrival.
-Each mesh's device layout (geometry, half-face lists, halos) is built once and
-stays resident, shared by every request on that mesh (cg_layout_create /
-cg_instance_create_on); a request brings only its own state and work buffers.
+Each request brings its own mesh (built before the clock, as a request's mesh
+file would be read); its device layout is built with the request and freed
+when it retires. Layouts given to the server up front (`layouts`) stay
+resident and are shared by every request on that mesh (cg_layout_create /
+cg_instance_create_on).
 A solve's result does not depend on what else is in the batch (the batch
 tests check it bit for bit against the solve run alone).
 
 `sequential` is the reference execution: the same requests one at a time in
-arrival order (FP64, the same resident layouts, one solve on the GPU at a
-time), with the same queueing arithmetic (a request starts when it has
-arrived and the previous one is done).
+arrival order, each set up and run alone through cg_create / cg_step (FP64,
+or the server's own precision for comparison), with the same queueing
+arithmetic (a request starts when it has arrived and the previous one is
+done).
 """
 from __future__ import annotations
 
@@ -32,7 +37,7 @@ import numpy as np
 
 from .batch import Batch, Instance, Layout
 from .ensemble import Spec
-from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED, IdealGasSolver
+from .ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
 
 
 @dataclass
@@ -59,28 +64,35 @@ class Result:
         return self.done - self.arrival
 
 
-def requests_poisson(rate, count, sizes, seed=7, steps_lo=200, steps_hi=2000, cells_lo=2e4, cells_hi=2e6):
+def requests_poisson(rate, count, sizes=None, seed=7, steps_lo=200, steps_hi=2000, cells_lo=2e4, cells_hi=2e6):
     """`count` requests with exponential inter-arrival times of mean 1/rate.
-    Cell counts are log-uniform in [cells_lo, cells_hi], mapped to the nearest
-    lattice side in `sizes` (the meshes a deployment holds); horizons uniform
-    in [steps_lo, steps_hi]; Re, Ma and the initial field as in the ensemble."""
+    Cell counts are log-uniform in [cells_lo, cells_hi]; each request has its
+    own jittered mesh (lattice side the cube root of its cell count, its own
+    seed), or with `sizes` the nearest side of that catalogue (shared meshes).
+    Horizons uniform in [steps_lo, steps_hi]; Re, Ma and the initial field as
+    in the ensemble."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
-    sizes = np.asarray(sorted(sizes))
+    cat = None if sizes is None else np.asarray(sorted(sizes))
     for i in range(count):
         t += rng.exponential(1.0 / rate)
         cells = float(np.exp(rng.uniform(np.log(cells_lo), np.log(cells_hi))))
-        n = int(sizes[np.argmin(np.abs(np.log(sizes.astype(float) ** 3) - np.log(cells)))])
+        if cat is None:
+            n = int(round(cells ** (1.0 / 3.0)))
+        else:
+            n = int(cat[np.argmin(np.abs(np.log(cat.astype(float) ** 3) - np.log(cells)))])
         sp = Spec(n=n, mesh_seed=1000 + n, ic="tgv" if rng.random() < 0.5 else "k4",
                   Re=float(rng.uniform(400.0, 3200.0)), Ma=float(rng.uniform(0.05, 0.3)),
                   ic_seed=int(rng.integers(1, 2**31)), steps=int(rng.integers(steps_lo, steps_hi + 1)))
+        if cat is None:
+            sp.mesh_seed = 100000 + i
         out.append(Request(i, t, sp))
     return out
 
 
 class MeshPool:
-    """Meshes by lattice side, built once (the catalogue a deployment serves)."""
+    """The requests' meshes by (lattice side, seed), each built once."""
 
     def __init__(self):
         self.meshes = {}
@@ -91,6 +103,17 @@ class MeshPool:
             self.meshes[key] = sp.mesh()
         return self.meshes[key]
 
+    def build(self, specs, threads=4):
+        """Build the meshes of `specs` (several at a time: the builder releases
+        the interpreter while it works)."""
+        todo = {}
+        for sp in specs:
+            todo.setdefault((sp.n, sp.mesh_seed), sp)
+        todo = {k: sp for k, sp in todo.items() if k not in self.meshes}
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            for key, m in zip(todo, ex.map(lambda sp: sp.mesh(), todo.values())):
+                self.meshes[key] = m
+
 
 def build_layouts(pool: MeshPool, precision):
     """Resident device layouts of every mesh in the pool."""
@@ -99,18 +122,22 @@ def build_layouts(pool: MeshPool, precision):
 
 class Server:
     """`batch_factory(precision)` and `prepare(request)` default to the GPU
-    Batch and cg_instance_create_on the request's resident layout; they are
-    parameters so the scheduling logic can be exercised without a GPU. The
-    HBM budget covers the admitted solves' own memory plus the resident
-    layouts."""
+    Batch and cg_instance_create (the request's own layout) or
+    cg_instance_create_on a resident layout; they are parameters so the
+    scheduling logic can be exercised without a GPU. `workers` requests are
+    prepared at a time. The HBM budget covers the admitted solves' memory,
+    their own layouts and the resident layouts."""
 
     def __init__(self, pool: MeshPool, precision=CG_PREC_MIXED, hbm_budget=96e9, max_chunk=16,
-                 bytes_per_cell=None, log=None, batch_factory=None, prepare=None, layouts=None):
+                 bytes_per_cell=None, log=None, batch_factory=None, prepare=None, layouts=None, workers=3):
         self.pool = pool
         self.precision = precision
         self.budget = float(hbm_budget)
         self.max_chunk = int(max_chunk)
-        self.bpc = bytes_per_cell or (1500.0 if precision == CG_PREC_FP64 else 800.0)
+        self.workers = max(1, int(workers))
+        # admission estimate of a request's device bytes per cell (its state
+        # and work buffers, and its own layout), raised to what requests hold
+        self.bpc = bytes_per_cell or (5000.0 if precision == CG_PREC_FP64 else 3000.0)
         self.log = log or (lambda s: None)
         self.batch_factory = batch_factory or Batch
         self.prepare = prepare or self._prepare
@@ -120,10 +147,12 @@ class Server:
     def _prepare(self, req: Request):
         m = self.pool.get(req.spec)
         key = (req.spec.n, req.spec.mesh_seed)
-        if key not in self.layouts:
-            self.layouts[key] = Layout(m, self.precision)
-        return Instance(m, req.spec.deck(self.precision, check_interval=100), *req.spec.state(m),
-                        layout=self.layouts[key])
+        deck = req.spec.deck(self.precision, check_interval=100)
+        if key in self.layouts:
+            return Instance(m, deck, *req.spec.state(m), layout=self.layouts[key])
+        inst = Instance(m, deck, *req.spec.state(m))  # its own layout, freed with it
+        inst.private_layout = True
+        return inst
 
     def run(self, requests):
         reqs = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -134,7 +163,7 @@ class Server:
         layout_bytes = {k: float(v.bytes()) for k, v in self.layouts.items()} if self.layouts else {}
         used = sum(layout_bytes.values())
         batch = self.batch_factory(self.precision)
-        pool = cf.ThreadPoolExecutor(max_workers=1)
+        pool = cf.ThreadPoolExecutor(max_workers=self.workers)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         steps_run = 0
@@ -159,14 +188,17 @@ class Server:
                 inst = fut.result()
                 own, lay = inst.bytes()
                 key = (r.spec.n, r.spec.mesh_seed)
-                used += own - need
-                if key not in layout_bytes:  # a layout built for this request stays resident
+                hold = float(own)
+                if getattr(inst, "private_layout", False):
+                    hold += lay  # the request's own layout, freed when it retires
+                elif key not in layout_bytes:  # a shared layout stays resident
                     layout_bytes[key] = float(lay)
                     used += lay
+                used += hold - need
                 self.peak_bytes = max(self.peak_bytes, used)
-                self.bpc = max(self.bpc, own / r.spec.n ** 3)
+                self.bpc = max(self.bpc, hold / r.spec.n ** 3)
                 iid = batch.attach(inst)
-                active[iid] = [r, r.spec.steps, now(), own, r.spec.n ** 3]
+                active[iid] = [r, r.spec.steps, now(), hold, r.spec.n ** 3]
             if not active:
                 nxt = [f for f in preparing]
                 if nxt:
@@ -195,8 +227,9 @@ class Server:
 
 
 def sequential(pool: MeshPool, requests, precision=CG_PREC_FP64, layouts=None):
-    """One request at a time in arrival order, each alone on the GPU (on the
-    mesh's resident layout when `layouts` has it, else through cg_create).
+    """One request at a time in arrival order, each alone on the GPU: set up
+    (cg_instance_create: its own layout, or the mesh's resident layout when
+    `layouts` has it), then stepped alone with graph replays (a batch of one).
     Service times are measured; waiting follows from the arrival times."""
     results = []
     clock = 0.0
@@ -204,20 +237,12 @@ def sequential(pool: MeshPool, requests, precision=CG_PREC_FP64, layouts=None):
         m = pool.get(r.spec)
         key = (r.spec.n, r.spec.mesh_seed)
         t0 = time.perf_counter()
-        if layouts and key in layouts:
-            b = Batch(precision)
-            iid = b.attach(Instance(m, r.spec.deck(precision, check_interval=100), *r.spec.state(m),
-                                    layout=layouts[key]))
-            b.step(r.spec.steps)
-            d = b.diag(iid)
-            b.close()
-        else:
-            g = IdealGasSolver(m, r.spec.deck(precision, check_interval=100))
-            g.set_state(*r.spec.state(m))
-            g.init_complete()
-            g.advance_solution(r.spec.steps)
-            d = g.report()
-            g.close()
+        lay = layouts.get(key) if layouts else None
+        b = Batch(precision)
+        iid = b.attach(Instance(m, r.spec.deck(precision, check_interval=100), *r.spec.state(m), layout=lay))
+        b.step(r.spec.steps)
+        d = b.diag(iid)
+        b.close()
         service = time.perf_counter() - t0
         start = max(clock, r.arrival)
         clock = start + service

@@ -46,16 +46,18 @@ class og_diag(C.Structure):
 # addSourceHook(IdealGasRhs* rhs, time, rk_stage): (user, rhs [ncv][5], ncv, time, stage)
 OG_SOURCE_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int)
 
-_lib = None
+_libs = {}
 _ref = None
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError("oracle/liboracle.so missing: run __graft_entry__.build()")
-        L = C.CDLL(ORACLE_SO)
+def lib(path=None):
+    """The oracle library (default oracle/liboracle.so; another build of
+    ig_oracle.c by path, e.g. one compiled for the host CPU)."""
+    path = path or ORACLE_SO
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run __graft_entry__.build()")
+        L = C.CDLL(path)
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
         L.og_create.restype = vp
         L.og_create.argtypes = [C.POINTER(og_mesh), C.POINTER(og_params)]
@@ -100,8 +102,8 @@ def lib():
         L.og_set_source_hook.restype = None
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def ref():
@@ -161,7 +163,8 @@ class OracleSolver:
     """The oracle step on the operator arrays of a product mesh. `threads` > 1
     runs its loops on that many threads, bit-identical to one thread."""
 
-    def __init__(self, mesh, cgparams, threads=1):
+    def __init__(self, mesh, cgparams, threads=1, libpath=None):
+        self._L = lib(libpath)
         self._keep = {}
         m = og_mesh()
         m.ncv, m.nfa, m.nef = mesh.ncv, mesh.nfa, mesh.nef
@@ -179,72 +182,72 @@ class OracleSolver:
         self._m = m
         self._p = params_from(cgparams)
         self.ncv = mesh.ncv
-        self._h = lib().og_create(C.byref(self._m), C.byref(self._p))
-        lib().og_set_threads(self._h, int(threads))
+        self._h = self._L.og_create(C.byref(self._m), C.byref(self._p))
+        self._L.og_set_threads(self._h, int(threads))
 
     def set_state(self, rho, u, rhoE):
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.ncv, 3)
         rhoE = np.ascontiguousarray(rhoE, dtype=np.float64)
-        lib().og_set_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
+        self._L.og_set_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
 
     def get_state(self):
         rho, u, rhoE = np.empty(self.ncv), np.empty((self.ncv, 3)), np.empty(self.ncv)
-        lib().og_get_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
+        self._L.og_get_state(self._h, dptr(rho), dptr(u), dptr(rhoE))
         return rho, u, rhoE
 
     def get_field(self, name):
         out = np.empty((self.ncv, 3, 3) if name == "dudx" else self.ncv)
-        if lib().og_get_field(self._h, name.encode(), dptr(out)) != 0:
+        if self._L.og_get_field(self._h, name.encode(), dptr(out)) != 0:
             raise KeyError(name)
         return out
 
     def init_complete(self):
-        lib().og_init_complete(self._h)
+        self._L.og_init_complete(self._h)
 
     def store(self):
-        lib().og_store(self._h)
+        self._L.og_store(self._h)
 
     def restore(self):
-        lib().og_restore(self._h)
+        self._L.og_restore(self._h)
 
     def set_time(self, step, time, dt):
-        lib().og_set_time(self._h, int(step), float(time), float(dt))
+        self._L.og_set_time(self._h, int(step), float(time), float(dt))
 
     def set_dsm_state(self, lijmij, mijmij):
         lm = np.ascontiguousarray(lijmij, dtype=np.float64)
         mm = np.ascontiguousarray(mijmij, dtype=np.float64)
-        lib().og_set_dsm_state(self._h, dptr(lm), dptr(mm))
+        self._L.og_set_dsm_state(self._h, dptr(lm), dptr(mm))
 
     def set_scalar(self, isc, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
-        lib().og_set_scalar(self._h, int(isc), dptr(phi))
+        self._L.og_set_scalar(self._h, int(isc), dptr(phi))
 
     def get_scalar(self, isc):
         out = np.empty(self.ncv)
-        lib().og_get_scalar(self._h, int(isc), dptr(out))
+        self._L.og_get_scalar(self._h, int(isc), dptr(out))
         return out
 
     def calc_rhs_scalars(self):
         out = np.empty((self.ncv, max(self._p.nsc, 1)))
-        lib().og_calc_rhs_scalars(self._h, dptr(out))
+        self._L.og_calc_rhs_scalars(self._h, dptr(out))
         return out
 
     def sgs_props(self):
-        lib().og_sgs_props(self._h)
+        self._L.og_sgs_props(self._h)
 
     def sync_post_state(self):
         """syncPostState (IdealGasSolver.hpp:323-330): primitives, gradient,
         SGS and material properties of the current state."""
-        lib().og_update_state(self._h)
-        lib().og_sgs_props(self._h)
+        self._L.og_update_state(self._h)
+        self._L.og_sgs_props(self._h)
 
     def set_source_hook(self, fn):
         """fn(rhs [ncv][5] numpy view, time, rk_stage) adds to the stage RHS in place
         (addSourceHook, IdealGasSolver.hpp:702,713); None removes it."""
         if fn is None:
             self._hook = None
-            lib().og_set_source_hook(self._h, OG_SOURCE_FN(), None)
+            self._L.og_set_source_hook(self._h, OG_SOURCE_FN(), None)
             return
         ncv = self.ncv
 
@@ -252,38 +255,38 @@ class OracleSolver:
             fn(np.ctypeslib.as_array(rhs, shape=(ncv, 5)), time, stage)
 
         self._hook = OG_SOURCE_FN(tramp)
-        lib().og_set_source_hook(self._h, self._hook, None)
+        self._L.og_set_source_hook(self._h, self._hook, None)
 
     def filter_cv(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         ncomp = 1 if phi.ndim == 1 else phi.shape[1]
         out = np.empty_like(phi)
-        lib().og_filter_cv(C.byref(self._m), dptr(phi), dptr(out), ncomp)
+        self._L.og_filter_cv(C.byref(self._m), dptr(phi), dptr(out), ncomp)
         return out
 
     def calc_rhs(self):
         out = np.empty((self.ncv, 5))
-        lib().og_calc_rhs(self._h, dptr(out))
+        self._L.og_calc_rhs(self._h, dptr(out))
         return out
 
     def calc_cfl(self, dt):
-        return lib().og_calc_cfl(self._h, dt)
+        return self._L.og_calc_cfl(self._h, dt)
 
     def step(self, n=1):
         rc = 0
         for _ in range(n):
-            rc = lib().og_step(self._h)
+            rc = self._L.og_step(self._h)
         return rc
 
     def diag(self):
         d = og_diag()
-        lib().og_diag_get(self._h, C.byref(d))
+        self._L.og_diag_get(self._h, C.byref(d))
         return {"step": d.step, "time": d.time, "dt": d.dt, "ke": d.ke, "enstrophy": d.enstrophy,
                 "sums": list(d.sums), "cfl_max": d.cfl_max, "nan_flag": d.nan_flag}
 
     def close(self):
         if self._h:
-            lib().og_destroy(self._h)
+            self._L.og_destroy(self._h)
             self._h = None
 
     def __del__(self):
