@@ -71,6 +71,23 @@ extern "C" {
 typedef struct cg_mesh cg_mesh;
 typedef struct cg_solver cg_solver;
 
+/* Boundary conditions by zone (IdealGasSolver::registerBoundaryConditions,
+ * IdealGasSolver.hpp:162-238); deck lines "<zone> <type> ...":
+ *   SLIP or SYMMETRY                 SlipWallBc (IdealGasSolverBcs.cpp:14-23)
+ *   WALL_ADIABATIC [TRANSLATE u v w]  WallAdiabatic (:25-47, options :450-497)
+ *   WALL_ISOTHERMAL <T>              WallIsothermal (:1480-1510)
+ * The other types of the reference are refused by cg_params_parse. */
+#define CG_MAX_BC_ZONES 16
+#define CG_BC_SLIP 1
+#define CG_BC_WALL_ADIABATIC 2
+#define CG_BC_WALL_ISOTHERMAL 3
+typedef struct cg_bc {
+  char zone[48];  /* boundary zone name of the mesh */
+  int type;       /* CG_BC_* */
+  double T;       /* WALL_ISOTHERMAL wall temperature */
+  double u[3];    /* WALL_ADIABATIC TRANSLATE wall velocity (0 otherwise) */
+} cg_bc;
+
 /* Input-file parameters of the path; the names in comments are the reference's
  * input-file keys (IdealGasSolver.hpp:147-155, IdealGasSolverSgs.cpp:209-213,
  * IdealGasSolver.cpp:202-203,753,904, StripedMesh.hpp:6013-6016,
@@ -118,6 +135,9 @@ typedef struct cg_params {
                             * keep the extended operator */
   int dvol_form;           /* DVOL_FORM: dcmag_hat from volume jumps over the extended pairs,
                             * sum |n_ef| |V1 - V0| / (V1 + V0) / sum |n_ef| * V^(1/3) */
+  /* boundary conditions of the mesh's boundary zones (every zone needs one) */
+  int nbc;
+  cg_bc bc[CG_MAX_BC_ZONES];
 } cg_params;
 
 /* Per-step diagnostics (IdealGasSolver::report, IdealGasSolver.hpp:805-848,
@@ -147,6 +167,7 @@ int cg_params_default(cg_params* p);
  * subset of Params.hpp:172-182 keys listed in cg_params. */
 int cg_params_parse(cg_params* p, const char* text);
 
+
 /* ---- meshes (StaticSolver::init: read_mles + buildExtendedFaces +
  *      calcCvGradCoeff + groupFaces, StaticSolver.hpp:1168-1330;
  *      FlowSolver::init + initFaStructures, FlowSolver.hpp:659-878,1748-2100) ---- */
@@ -157,6 +178,18 @@ int cg_mesh_lattice(int nx, int ny, int nz, double Lx, double Ly, double Lz,
  * uniformly by +-jitter*h per component (seeded). */
 int cg_mesh_jittered(int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter,
                      uint64_t seed, const cg_params* p, cg_mesh** out);
+/* A channel mesh (no counterpart in the reference's solver: a Stitch box with
+ * PERIODIC_X PERIODIC_Z and walls): periodic in x and z, walls at y = 0
+ * (boundary zone "y0") and y = Ly ("y1"); the Voronoi diagram of an nx*ny*nz
+ * lattice jittered by +-jitter*h (jitter 0: the uniform lattice of box cells).
+ * Boundary faces carry the geometry of StripedMesh.hpp:2760-2794 (area,
+ * area_over_delta_bf, Gij_bf into the gradient, StaticSolver.cpp:5643-5653). */
+int cg_mesh_channel(int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter, uint64_t seed,
+                    const cg_params* p, cg_mesh** out);
+/* counts[0] = boundary faces, counts[1] = boundary zones of a mesh */
+int cg_mesh_boundary(const cg_mesh* m, int64_t counts[2]);
+/* name of boundary zone `zone` (the name its condition is given under) */
+int cg_mesh_zone_name(const cg_mesh* m, int zone, char* name, int64_t len);
 /* Periodic Voronoi mesh from raw arrays (the .mles records x_cv, vol_cv,
  * cvofa, n_fa, x_fa of StripedMesh::read_mles, StripedMesh.hpp:622):
  * shift[nfa][3] in {-1,0,1} gives the periodic image of cvofa[f][1]. */
@@ -198,7 +231,8 @@ int cg_mesh_counts(const cg_mesh* m, int64_t counts[10]);
  * n_fa, x_fa, lo, hi, pshift, n_ef, c_ef, n_use, c_use, group_ef, group_fa,
  * area, area_over_delta, unit_n, area_use, aod_use, unit_n_use, cvocv_i,
  * cvocv_v, grad_coeff, inv_vol, sa_to_vol, dcmag_hat, cmag, faocv_i, faocv_v,
- * efocv_i, efocv_v, grad_flag, max_gcl). nbytes must equal the array size. */
+ * efocv_i, efocv_v, grad_flag, max_gcl; boundary faces: cvobf, zone_bf, n_bf, x_bf,
+ * area_bf, aod_bf, bfocv_i, bfocv_v). nbytes must equal the array size. */
 int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes);
 void cg_mesh_destroy(cg_mesh* m);
 

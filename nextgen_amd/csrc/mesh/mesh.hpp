@@ -31,6 +31,16 @@ struct Voronoi {
   std::vector<double> n_fa;     // [nfa][3]
   std::vector<double> x_fa;     // [nfa][3]
   int ndropped = 0;             // one-sided slivers removed during the merge
+  // boundary faces (meshes that are not periodic in every direction), zone by
+  // zone: zone z holds bf_zone_i[z] .. bf_zone_i[z+1], each zone's faces by cell
+  int nbf = 0;
+  std::vector<std::string> bf_zone_name;
+  std::vector<int> bf_zone_i;         // [nzone+1]
+  std::vector<int> cvobf;             // [nbf]
+  std::vector<double> n_bf, x_bf;     // [nbf][3] outward area vector, centroid
+  // per boundary face as a mesh file carries them (empty: computed from the
+  // geometry, StripedMesh.hpp:2760-2794)
+  std::vector<double> aod_bf_file, gij_bf_file;  // [nbf], [nbf][3][3]
 };
 
 struct OperatorParams {
@@ -69,6 +79,13 @@ struct Operators {
   // per-cell lists of compact faces (face order) and of extended pairs (pair order)
   std::vector<int> faocv_i, faocv_v;   // CSR over compact faces
   std::vector<int> efocv_i, efocv_v;   // CSR over extended pairs
+  // boundary faces in zone-major order (Voronoi::cvobf ...): zone, outward area
+  // vector, area and area_over_delta_bf (StripedMesh.hpp:2760-2794); per cell
+  // its boundary faces in that order
+  int nbf = 0;
+  std::vector<int> cvobf, zone_bf;     // [nbf]
+  std::vector<double> n_bf, area_bf, aod_bf;  // [nbf][3], [nbf], [nbf]
+  std::vector<int> bfocv_i, bfocv_v;   // CSR over boundary faces
   int ngrouped_ef = 0, ngrouped_fa = 0, nthresholded = 0;
   double max_gcl = 0.0;                // max |sum_f n_fa| / vol^(2/3)
 };
@@ -77,6 +94,10 @@ struct Operators {
 std::string build_lattice(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz);
 std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz,
                            double jitter, uint64_t seed, int nthreads);
+// Channel: periodic in x and z, walls at y = 0 (boundary zone "y0") and y = Ly
+// ("y1"); jitter 0 gives the uniform lattice of box cells.
+std::string build_channel(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter,
+                          uint64_t seed, int nthreads);
 std::string build_from_arrays(Voronoi& m, int ncv, const double* x_cv, const double* vol,
                               int nfa, const int* cvofa, const int* shift, const double* n_fa,
                               const double* x_fa, const double L[3]);

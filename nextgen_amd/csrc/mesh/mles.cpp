@@ -330,18 +330,16 @@ std::string read_mles(const std::string& path, Mles& f) {
 }
 
 std::string voronoi_from_mles(const Mles& f, Voronoi& m) {
-  if (f.nbf != 0)
-    return "the mesh has " + std::to_string(f.nbf) +
-           " boundary faces: only fully periodic meshes run on this solver (no boundary conditions)";
   if (f.ncv <= 0 || f.nfa <= 0) return "empty mesh";
   if (f.ncv > 2147483647LL / 2 || f.nfa > 2147483647LL / 3) return "mesh too large for 32-bit cell indices";
   if (f.x_cv.empty() || f.vol_cv.empty() || f.cvofa.empty() || f.n_fa.empty() || f.x_fa.empty())
     return "mesh file lacks one of x_cv, vol_cv, cvofa, n_fa, x_fa";
-  // three Cartesian transforms, one along each axis: a periodic box
-  if (f.npt != 3) return "need three periodic transforms (a box periodic in x, y and z)";
-  int axis[3], sgn[3];
+  // Cartesian transforms along the coordinate axes: three for a periodic box,
+  // fewer (or none) when boundary faces close the other directions
+  if (f.nbf == 0 && f.npt != 3) return "need three periodic transforms (a box periodic in x, y and z) or boundary faces";
+  int axis[3] = {0, 1, 2}, sgn[3] = {1, 1, 1};
   double L[3] = {0, 0, 0};
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < f.npt; ++i) {
     if (f.pt_kind[i] != kTransformCart) return "only Cartesian periodic transforms are supported";
     int nz = 0;
     for (int d = 0; d < 3; ++d)
@@ -379,6 +377,32 @@ std::string voronoi_from_mles(const Mles& f, Voronoi& m) {
   if (!e.empty()) return e;
   if (f.x_vv.size() == size_t(3 * ncv)) m.x_vv = f.x_vv;
   if (f.r_vv.size() == size_t(ncv)) m.r_vv = f.r_vv;
+  if (f.nbf > 0) {
+    // boundary zones in file order, each a contiguous range of boundary faces
+    const int nbf = int(f.nbf);
+    if (f.cvobf.size() != size_t(nbf) || f.n_bf.size() != size_t(3 * nbf) || f.x_bf.size() != size_t(3 * nbf))
+      return "mesh file lacks one of cvobf, n_bf, x_bf";
+    if (f.x_vv.size() != size_t(3 * ncv) &&
+        (f.area_over_delta_bf.size() != size_t(nbf) || f.Gij_bf.size() != size_t(9 * nbf)))
+      return "mesh file lacks x_vv to build its boundary-face operators";
+    m.nbf = nbf;
+    m.bf_zone_i.assign(1, 0);
+    for (const auto& z : f.bf_zones) {
+      if (z.ibf_f != m.bf_zone_i.back()) return "boundary zones are not contiguous in face order";
+      m.bf_zone_name.push_back(z.name);
+      m.bf_zone_i.push_back(int(z.ibf_f + z.nbf));
+    }
+    if (m.bf_zone_i.back() != nbf) return "boundary zone counts do not add up to nbf";
+    m.cvobf.resize(size_t(nbf));
+    for (int b = 0; b < nbf; ++b) {
+      m.cvobf[size_t(b)] = int(f.cvobf[size_t(b)]);
+      if (m.cvobf[size_t(b)] < 0 || m.cvobf[size_t(b)] >= ncv) return "cvobf index out of range";
+    }
+    m.n_bf = f.n_bf;
+    m.x_bf = f.x_bf;
+    if (f.area_over_delta_bf.size() == size_t(nbf)) m.aod_bf_file = f.area_over_delta_bf;
+    if (f.Gij_bf.size() == size_t(9 * nbf)) m.gij_bf_file = f.Gij_bf;
+  }
   return "";
 }
 

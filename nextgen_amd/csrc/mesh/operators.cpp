@@ -10,6 +10,8 @@
 //                                     (StaticSolver.cpp:4669-5060), consumed by
 //                                     FlowSolver::initFaStructures (FlowSolver.hpp:1748-1850)
 //  * dcmag_hat_cv, sa_to_vol_ratio    FlowSolver::init (FlowSolver.hpp:702-758, 859-878)
+//  * boundary faces: area, area_over_delta_bf and Gij_bf (StripedMesh.hpp:2760-2794),
+//    which enters the gradient's A (StaticSolver.cpp:5643-5653); D has no boundary terms
 //
 // The reference assembles D with a global map over (row, column) pairs filled
 // by every cell for its own row and its neighbours' rows. Here each row is
@@ -200,6 +202,45 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
     for (int d = 0; d < 3; ++d) x[d] = m.x_cv[3 * c + d] + s[d] * L[d];
   };
 
+  // ---- boundary faces (zone-major), StripedMesh.hpp:2760-2794 ----
+  const int nbf = m.nbf;
+  op.nbf = nbf;
+  op.cvobf = m.cvobf;
+  op.n_bf = m.n_bf;
+  op.zone_bf.assign(nbf, 0);
+  for (int z = 0; z + 1 < int(m.bf_zone_i.size()); ++z)
+    for (int b = m.bf_zone_i[z]; b < m.bf_zone_i[z + 1]; ++b) op.zone_bf[b] = z;
+  op.area_bf.resize(nbf);
+  op.aod_bf.resize(nbf);
+  std::vector<double> gij(9 * size_t(nbf));  // Gij_bf[b][i][j] = -dx[j] n_bf[i], dx = x_bf - x_vv
+  for (int b = 0; b < nbf; ++b) {
+    const int c = m.cvobf[b];
+    const double* n = &m.n_bf[3 * size_t(b)];
+    const double area_eps = 1.0E-20;
+    op.area_bf[b] = mag3(n);
+    double dx[3];
+    for (int d = 0; d < 3; ++d) dx[d] = m.x_bf[3 * size_t(b) + d] - m.x_vv[3 * size_t(c) + d];
+    const double amax = std::max(op.area_bf[b], area_eps);
+    const double un[3] = {n[0] / amax, n[1] / amax, n[2] / amax};
+    const double dp = dot3(dx, un);
+    const double dx_vol = std::pow(m.vol[c], 1.0 / 3.0);
+    const double dx_vol_area = m.vol[c] / amax;
+    const double dx_clip = std::min(0.25 * dx_vol, 0.25 * dx_vol_area);
+    op.aod_bf[b] = dp < dx_clip ? op.area_bf[b] / dx_clip : op.area_bf[b] / dp;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gij[9 * size_t(b) + 3 * i + j] = -dx[j] * n[i];
+  }
+  if (!m.aod_bf_file.empty()) op.aod_bf = m.aod_bf_file;
+  if (!m.gij_bf_file.empty()) gij = m.gij_bf_file;
+  op.bfocv_i.assign(ncv + 1, 0);
+  for (int b = 0; b < nbf; ++b) ++op.bfocv_i[m.cvobf[b] + 1];
+  for (int c = 0; c < ncv; ++c) op.bfocv_i[c + 1] += op.bfocv_i[c];
+  op.bfocv_v.resize(nbf);
+  {
+    std::vector<int> pos(op.bfocv_i.begin(), op.bfocv_i.end() - 1);
+    for (int b = 0; b < nbf; ++b) op.bfocv_v[pos[m.cvobf[b]]++] = b;
+  }
+
   // ---- gradient stencil: diagonal first, then neighbours in face order ----
   op.cvocv_i.assign(ncv + 1, 0);
   for (int c = 0; c < ncv; ++c) op.cvocv_i[c + 1] = op.cvocv_i[c] + 1 + (op.faocv_i[c + 1] - op.faocv_i[c]);
@@ -228,6 +269,13 @@ std::string build_operators(const Voronoi& m, const OperatorParams& P, Operators
       double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       for (int i = 0; i < 3; ++i) A[i * 3 + i] = m.vol[c];
       double gcl[3] = {0, 0, 0};
+      // boundary faces first (StaticSolver.cpp:5643-5653)
+      for (int boc = op.bfocv_i[c]; boc < op.bfocv_i[c + 1]; ++boc) {
+        const int b = op.bfocv_v[boc];
+        for (int j = 0; j < 3; ++j)
+          for (int i = 0; i < 3; ++i) A[j * 3 + i] += gij[9 * size_t(b) + 3 * i + j];
+        for (int i = 0; i < 3; ++i) gcl[i] += m.n_bf[3 * size_t(b) + i];
+      }
       for (int foc = op.faocv_i[c]; foc < op.faocv_i[c + 1]; ++foc) {
         const FaceView v = view(c, op.faocv_v[foc]);
         const double* n = &m.n_fa[3 * v.f];

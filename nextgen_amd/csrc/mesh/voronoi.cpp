@@ -215,10 +215,21 @@ struct HalfFace {
   double x[3];  // centroid, generator-relative
 };
 
+struct BoundaryFace {
+  int zone;
+  double n[3];  // outward area vector
+  double x[3];  // centroid, generator-relative
+};
+
 struct CellOut {
   double vol, xc[3], r2;
   std::vector<HalfFace> hf;
+  std::vector<BoundaryFace> bf;
 };
+
+// clipping-plane ids of the channel walls y = 0 and y = Ly (candidate ids are
+// their index in the candidate list, the initial box's faces are negative)
+constexpr int kWallId = 1 << 28;
 
 std::string sort_and_check(Voronoi& m) {
   // orient faces cv0 < cv1 and sort by (cv0, cv1, shift)
@@ -306,8 +317,9 @@ std::string build_lattice(Voronoi& m, int nx, int ny, int nz, double Lx, double 
   return sort_and_check(m);
 }
 
-std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz,
-                           double jitter, uint64_t seed, int nthreads) {
+namespace {
+std::string build_clipped(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter,
+                          uint64_t seed, int nthreads, bool walls_y) {
   if (nx < 8 || ny < 8 || nz < 8) return "jittered mesh needs at least 8 cells per direction";
   if (!(jitter >= 0.0 && jitter < 0.5)) return "jitter must be in [0, 0.5)";
   MortonLattice ml;
@@ -345,6 +357,7 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
           for (int dj = -R; dj <= R; ++dj)
             for (int dk = -R; dk <= R; ++dk) {
               if (!di && !dj && !dk) continue;
+              if (walls_y && (q[1] + dj < 0 || q[1] + dj >= ny)) continue;  // no images across the walls
               Cand cc;
               cc.nbr = ml.wrap(q[0] + di, q[1] + dj, q[2] + dk, cc.s);
               const V3 x = {m.x_vv[3 * cc.nbr] + cc.s[0] * Lx, m.x_vv[3 * cc.nbr + 1] + cc.s[1] * Ly,
@@ -355,9 +368,15 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
             }
         std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.d2 < b.d2; });
         P.box(3.0 * hmax, 3.0 * hmax, 3.0 * hmax);
-        double r2 = P.rmax2();
         const double eps = 1e-13 * hmax * hmax;
         bool exhausted = true, bad = false;
+        if (walls_y) {
+          // keep y >= 0 and y <= Ly (generator-relative half-spaces)
+          if (P.clip({0.0, -1.0, 0.0}, p.y, 1e-13 * hmax, kWallId) < 0 ||
+              P.clip({0.0, 1.0, 0.0}, Ly - p.y, 1e-13 * hmax, kWallId + 1) < 0)
+            bad = true;
+        }
+        double r2 = P.rmax2();
         for (size_t ic = 0; ic < cand.size(); ++ic) {
           if (0.25 * cand[ic].d2 > r2) { exhausted = false; break; }
           const int rc = P.clip(cand[ic].r, 0.5 * cand[ic].d2, eps, int(ic));
@@ -394,6 +413,14 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
           else {
             xf = {0, 0, 0};
             for (int a : l) xf = xf + (1.0 / l.size()) * P.v[a];
+          }
+          if (P.id[f] >= kWallId) {
+            BoundaryFace b;
+            b.zone = P.id[f] - kWallId;
+            b.n[0] = A.x; b.n[1] = A.y; b.n[2] = A.z;
+            b.x[0] = xf.x; b.x[1] = xf.y; b.x[2] = xf.z;
+            co.bf.push_back(b);
+            continue;
           }
           HalfFace hf;
           const Cand& cc = cand[P.id[f]];
@@ -462,7 +489,71 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
   }
   m.nfa = int(m.cvofa.size() / 2);
   m.ndropped = ndrop;
+  if (walls_y) {
+    // boundary faces zone by zone, each zone's faces by cell
+    m.bf_zone_name = {"y0", "y1"};
+    m.bf_zone_i.assign(3, 0);
+    for (int z = 0; z < 2; ++z) {
+      for (int c = 0; c < ncv; ++c)
+        for (const auto& b : cells[c].bf) {
+          if (b.zone != z) continue;
+          m.cvobf.push_back(c);
+          for (int d = 0; d < 3; ++d) {
+            m.n_bf.push_back(b.n[d]);
+            m.x_bf.push_back(m.x_vv[3 * c + d] + b.x[d]);
+          }
+        }
+      m.bf_zone_i[z + 1] = int(m.cvobf.size());
+    }
+    m.nbf = int(m.cvobf.size());
+  }
   return sort_and_check(m);
+}
+}  // namespace
+
+std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                           double jitter, uint64_t seed, int nthreads) {
+  return build_clipped(m, nx, ny, nz, Lx, Ly, Lz, jitter, seed, nthreads, false);
+}
+
+std::string build_channel(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter,
+                          uint64_t seed, int nthreads) {
+  if (jitter > 0.0) return build_clipped(m, nx, ny, nz, Lx, Ly, Lz, jitter, seed, nthreads, true);
+  // the uniform lattice of box cells: periodic faces in x and z, interior faces
+  // in y, a wall face on each cell of the first and last layer
+  std::string e = build_lattice(m, nx, ny, nz, Lx, Ly, Lz);
+  if (!e.empty()) return e;
+  const double h[3] = {Lx / nx, Ly / ny, Lz / nz};
+  Voronoi t = m;
+  m.cvofa.clear(); m.shift.clear(); m.n_fa.clear(); m.x_fa.clear();
+  for (int f = 0; f < t.nfa; ++f) {
+    if (t.shift[3 * f + 1] != 0) continue;  // the wrap across y becomes two walls
+    for (int d = 0; d < 2; ++d) m.cvofa.push_back(t.cvofa[2 * f + d]);
+    for (int d = 0; d < 3; ++d) {
+      m.shift.push_back(t.shift[3 * f + d]);
+      m.n_fa.push_back(t.n_fa[3 * f + d]);
+      m.x_fa.push_back(t.x_fa[3 * f + d]);
+    }
+  }
+  m.nfa = int(m.cvofa.size() / 2);
+  m.bf_zone_name = {"y0", "y1"};
+  m.bf_zone_i.assign(3, 0);
+  for (int z = 0; z < 2; ++z) {
+    for (int c = 0; c < m.ncv; ++c) {
+      const double y = m.x_cv[3 * c + 1];
+      const bool first = y < h[1], last = y > Ly - h[1];
+      if ((z == 0 && !first) || (z == 1 && !last)) continue;
+      m.cvobf.push_back(c);
+      const double sgn = z == 0 ? -1.0 : 1.0;
+      m.n_bf.push_back(0.0); m.n_bf.push_back(sgn * h[0] * h[2]); m.n_bf.push_back(0.0);
+      m.x_bf.push_back(m.x_cv[3 * c]);
+      m.x_bf.push_back(z == 0 ? 0.0 : Ly);
+      m.x_bf.push_back(m.x_cv[3 * c + 2]);
+    }
+    m.bf_zone_i[z + 1] = int(m.cvobf.size());
+  }
+  m.nbf = int(m.cvobf.size());
+  return "";
 }
 
 std::string build_from_arrays(Voronoi& m, int ncv, const double* x_cv, const double* vol, int nfa,

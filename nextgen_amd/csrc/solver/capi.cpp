@@ -115,6 +115,7 @@ struct Layout {
   std::vector<int> ext_of;
   bool has_rvv = false;       // the mesh knows its cells' Voronoi radii (DSM_LOCAL needs them)
   int lds_max = 0;            // the device's LDS bytes per workgroup
+  std::vector<std::string> bf_zone_names;  // the mesh's boundary zones (index = zone)
 };
 
 // LDS bytes of the scalar pass's halo variant: the flux pass's records of a
@@ -161,7 +162,7 @@ struct Instance : HostInst {
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
             prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
-            scalar_lds<R>(lds, prm.nsc, lay->lds_max)};
+            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -547,6 +548,31 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
       for (int c = 0; c < ncv; ++c) r_vv[c] = rv[ext_of[c]];
     rc |= upload(s, h, &g.r_vv, r_vv);
   }
+  // boundary faces in device cell order (each cell's in zone-major order)
+  g.nbf = o.nbf;
+  Lay->bf_zone_names = m->vor.bf_zone_name;
+  if (o.nbf > 0) {
+    std::vector<int> bptr(ncv + 1, 0), bzone(o.nbf);
+    std::vector<double> bn[3], baod(o.nbf);
+    for (auto& v : bn) v.resize(o.nbf);
+    int k = 0;
+    for (int c = 0; c < ncv; ++c) {
+      const int e = ext_of[c];
+      for (int boc = o.bfocv_i[e]; boc < o.bfocv_i[e + 1]; ++boc, ++k) {
+        const int b = o.bfocv_v[boc];
+        bzone[k] = o.zone_bf[b];
+        for (int d = 0; d < 3; ++d) bn[d][k] = o.n_bf[3 * size_t(b) + d];
+        baod[k] = o.aod_bf[b];
+      }
+      bptr[c + 1] = k;
+    }
+    rc |= upload(s, h, &g.bf_ptr, bptr);
+    rc |= upload(s, h, &g.bf_zone, bzone);
+    rc |= upload(s, h, &g.bfn0, bn[0]);
+    rc |= upload(s, h, &g.bfn1, bn[1]);
+    rc |= upload(s, h, &g.bfn2, bn[2]);
+    rc |= upload(s, h, &g.bf_aod, baod);
+  }
   if (rc) return rc;
 
   // SELL lane orders (CG_LANE_ORDER=plain: lane = cell, for comparison)
@@ -822,6 +848,29 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   b.g = lay->g;
   hi->prm = *p;
   fill_dev_params(hi);
+  // every boundary zone of the mesh needs its condition (registerBoundaryConditions,
+  // IdealGasSolver.hpp:162-238); conditions for zones the mesh lacks are unused
+  {
+    const auto& zones = lay->bf_zone_names;
+    if ((int)zones.size() > cgdev::kMaxBcZones) return s->set(CG_EINVAL, "more than 16 boundary zones");
+    std::string missing;
+    for (size_t z = 0; z < zones.size(); ++z) {
+      int k = -1;
+      for (int i = 0; i < p->nbc; ++i)
+        if (zones[z] == p->bc[i].zone) k = i;
+      if (k < 0) {
+        missing += (missing.empty() ? "" : ", ") + zones[z];
+        continue;
+      }
+      hi->P.bc_type[z] = p->bc[k].type;
+      hi->P.bc_T[z] = p->bc[k].T;
+      for (int d = 0; d < 3; ++d) hi->P.bc_u[z][d] = p->bc[k].u[d];
+    }
+    if (!missing.empty()) return s->set(CG_EINVAL, "no boundary condition for zone(s) " + missing);
+    if (lay->g.nbf > 0 && (p->sgs_model == CG_SGS_DSM_LOCAL || p->nsc > 0))
+      return s->set(CG_EINVAL, "SGS_MODEL DSM_LOCAL and passive scalars need a periodic mesh (their boundary "
+                               "terms are not on this path)");
+  }
   const int ncv = lay->ncv, nfa = lay->nfa;
   hi->ncv = ncv;
   hi->nfa = nfa;
@@ -969,9 +1018,10 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false, any_dsm = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false;
   int max_nsc = 0, lds_max = 0;
   for (auto* I : E.inst) {
+    any_bf = any_bf || I->lay->g.nbf > 0;
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
@@ -980,7 +1030,7 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   }
   const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
   E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max)};
+         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf};
   E.dirty = false;
   return 0;
 }
@@ -1353,6 +1403,20 @@ int cg_params_default(cg_params* p) {
   return CG_OK;
 }
 
+// the boundary-condition type names of IdealGasSolver::registerBoundaryConditions
+// (IdealGasSolver.hpp:169-231)
+static bool is_bc_type(const std::string& t) {
+  static const char* names[] = {"SLIP", "SYMMETRY", "WALL_ADIABATIC", "WALL_ISOTHERMAL", "WALL_CHT",
+                                "WALL_CONDUCTIVE", "CBC_RUP", "CBC_RUNP", "CBC_MPT", "CBC_PROFILE", "CBC_UPT",
+                                "CBC_EXTRAPOLATE", "NSCBC_MT", "NSCBC_OUTLET_P", "NSCBC_OUTLET_PRESSURE",
+                                "NSCBC_OUTLET_MDOT", "CBC_TOTAL_PT", "WM_ALG_ADIABATIC", "WM_EQ_ADIABATIC",
+                                "WM_ALG_ISOTHERMAL", "WM_EQ_ISOTHERMAL", "WM_ALG_CHT", "WM_EXCHANGE_ADIABATIC",
+                                "WM_EXCHANGE_ISOTHERMAL", "INFLOW_TURB", "SPONGE", "CBC_PROFILE_SIMPLE", "HOOK"};
+  for (const char* n : names)
+    if (t == n) return true;
+  return false;
+}
+
 int cg_params_parse(cg_params* p, const char* text) {
   if (!p || !text) return fail(CG_EINVAL, "null argument");
   std::istringstream in(text);
@@ -1468,6 +1532,44 @@ int cg_params_parse(cg_params* p, const char* text) {
       else return bad("expected PRECISION FP64 or MIXED");
     } else if (key == "EF_OPS") {
       if (tok.size() < 2 || upper(tok[1]) != "DEFAULT") return bad("only EF_OPS DEFAULT is available");
+    } else if (tok.size() >= 2 && is_bc_type(upper(tok[1]))) {
+      // "<zone> <type> ...": a boundary zone's condition (IdealGasSolver.hpp:162-238)
+      const std::string type = upper(tok[1]);
+      cg_bc b;
+      std::memset(&b, 0, sizeof(b));
+      if (tok[0].size() >= sizeof(b.zone)) return bad("zone name too long");
+      std::snprintf(b.zone, sizeof(b.zone), "%s", tok[0].c_str());
+      if (type == "SLIP" || type == "SYMMETRY") {
+        b.type = CG_BC_SLIP;
+      } else if (type == "WALL_ADIABATIC") {
+        b.type = CG_BC_WALL_ADIABATIC;
+        for (size_t i = 2; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          if (opt == "TRANSLATE") {
+            for (int d = 0; d < 3; ++d)
+              if (!num(i + 1 + d, &b.u[d])) return bad("expected TRANSLATE <ux> <uy> <uz>");
+            i += 3;
+          } else if (opt == "STATIONARY" || opt == "STATIONARY_FRAME") {
+            // the wall velocity of a rotating frame; without one the wall is at rest
+          } else {
+            return bad("unknown WALL_ADIABATIC option " + opt);
+          }
+        }
+      } else if (type == "WALL_ISOTHERMAL") {
+        b.type = CG_BC_WALL_ISOTHERMAL;
+        if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WALL_ISOTHERMAL <T>");
+      } else {
+        return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC and "
+                   "WALL_ISOTHERMAL are)");
+      }
+      int slot = -1;
+      for (int i = 0; i < p->nbc; ++i)
+        if (std::string(p->bc[i].zone) == b.zone) slot = i;
+      if (slot < 0) {
+        if (p->nbc >= CG_MAX_BC_ZONES) return bad("more than 16 boundary zones");
+        slot = p->nbc++;
+      }
+      p->bc[slot] = b;
     } else if (key == "DISS_SPONGE" || key == "ROTATING_REFERENCE_FRAME" || key == "ADD_BODY_FORCE") {
       // body forces (addBodyForceSourceTerms, IdealGasSolver.cpp:2159-2202) and the
       // rotating frame are not on this path; a per-stage source can be added with
@@ -1506,6 +1608,34 @@ int cg_mesh_jittered(int nx, int ny, int nz, double Lx, double Ly, double Lz, do
     return fail(CG_EMESH, e);
   }
   return finish_mesh(m, p, out);
+}
+
+int cg_mesh_channel(int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter, uint64_t seed,
+                    const cg_params* p, cg_mesh** out) {
+  if (!out) return fail(CG_EINVAL, "null output");
+  if (!(Lx > 0 && Ly > 0 && Lz > 0)) return fail(CG_EINVAL, "box lengths must be positive");
+  if (!(jitter >= 0.0 && jitter < 0.5)) return fail(CG_EINVAL, "jitter must be in [0, 0.5)");
+  cg_mesh* m = new cg_mesh();
+  std::string e = cgmesh::build_channel(m->vor, nx, ny, nz, Lx, Ly, Lz, jitter, seed, cgmesh::default_threads());
+  if (!e.empty()) {
+    delete m;
+    return fail(CG_EMESH, e);
+  }
+  return finish_mesh(m, p, out);
+}
+
+int cg_mesh_boundary(const cg_mesh* m, int64_t counts[2]) {
+  if (!m || !counts) return fail(CG_EINVAL, "null argument");
+  counts[0] = m->op.nbf;
+  counts[1] = (int64_t)m->vor.bf_zone_name.size();
+  return CG_OK;
+}
+
+int cg_mesh_zone_name(const cg_mesh* m, int zone, char* name, int64_t len) {
+  if (!m || !name || len <= 0) return fail(CG_EINVAL, "null argument");
+  if (zone < 0 || zone >= (int)m->vor.bf_zone_name.size()) return fail(CG_EINVAL, "no such boundary zone");
+  std::snprintf(name, (size_t)len, "%s", m->vor.bf_zone_name[(size_t)zone].c_str());
+  return CG_OK;
 }
 
 int cg_mesh_from_arrays(int ncv, const double* x_cv, const double* vol, int nfa, const int* cvofa, const int* shift,
@@ -1631,6 +1761,8 @@ int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes) {
   ARR("cvocv_i", o.cvocv_i) ARR("cvocv_v", o.cvocv_v) ARR("grad_coeff", o.grad_coeff) ARR("inv_vol", o.inv_vol)
   ARR("sa_to_vol", o.sa_to_vol) ARR("dcmag_hat", o.dcmag_hat) ARR("cmag", o.cmag) ARR("faocv_i", o.faocv_i)
   ARR("faocv_v", o.faocv_v) ARR("efocv_i", o.efocv_i) ARR("efocv_v", o.efocv_v) ARR("grad_flag", o.grad_flag)
+  ARR("cvobf", o.cvobf) ARR("zone_bf", o.zone_bf) ARR("n_bf", o.n_bf) ARR("x_bf", v.x_bf) ARR("area_bf", o.area_bf)
+  ARR("aod_bf", o.aod_bf) ARR("bfocv_i", o.bfocv_i) ARR("bfocv_v", o.bfocv_v)
   else if (n == "fa_shift" || n == "pshift") {
     const std::vector<int8_t>& s8 = (n == "fa_shift") ? v.shift : o.pshift;
     itmp.assign(s8.begin(), s8.end());

@@ -94,6 +94,12 @@ struct GeomT {
   // |n_fa| |x_vv1 - x_vv0| / 6) and each cell's Voronoi radius r_vv
   const double* cfw;
   const double* r_vv;
+  // boundary faces (null / 0 on a periodic mesh): cell c's faces are
+  // bf_ptr[c] .. bf_ptr[c+1] in zone-major order; per face its zone, outward
+  // area vector and area_over_delta_bf (FP64 in either precision)
+  int nbf;
+  const int *bf_ptr, *bf_zone;
+  const double *bfn0, *bfn1, *bfn2, *bf_aod;
 };
 
 // One buffer of the ping-pong state.
@@ -134,6 +140,9 @@ struct WorkT {
   double *rhs_sc0, *rhs_sc1;
 };
 
+constexpr int kMaxBcZones = 16;
+enum { BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3 };
+
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;
   double vreman_coeff, a_sgs_coeff, a_sgs_coeff2, lambda_dil_coeff, invPr_t, invgm1, sigma_coeff;
@@ -145,6 +154,10 @@ struct Params {
   int full_gr;                     // USE_FULL_GR
   double dsm_alpha;                // DELTA_R_DSM^2
   int nsc, res_time;               // passive scalars; index of RES_TIME or -1
+  // boundary conditions of the mesh's boundary zones (by zone index)
+  int bc_type[kMaxBcZones];
+  double bc_T[kMaxBcZones];
+  double bc_u[kMaxBcZones][3];
 };
 
 using Geom = GeomT<double>;
@@ -192,6 +205,7 @@ struct Launch {
   int dsm;           // some instance uses SGS_MODEL DSM_LOCAL (its two passes run first)
   int scalars;       // most passive scalars of an instance (0: no scalar pass after each stage's flux)
   size_t sc_lds;     // dynamic LDS bytes of the scalar pass's halo variant (0: global gathers)
+  int boundary;      // some instance has boundary faces (the properties pass keeps mu_lam, loc_lam)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

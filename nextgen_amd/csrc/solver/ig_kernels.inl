@@ -539,14 +539,72 @@ __device__ __forceinline__ int cell_at(const uint8_t* perm, int pos) {
   return (pos & ~(kSlice - 1)) | int(G(perm)[pos]);
 }
 
+// The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
+// IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
+// face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
+// p is the stage pressure (gauged in mixed precision, as in the pair fluxes:
+// the gauge's p_g n sums to zero over a cell's pair and boundary faces), mu_lam
+// and loc_lam the lagged laminar coefficients, T = p / (rho R) of the stage.
+template <class R>
+__device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>& w, const Params& P, int c,
+                                              const Light<R>& self, R acc[5]) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) acc[q] = R(0);
+  if (g.nbf == 0) return;
+  const int b1 = G(g.bf_ptr)[c + 1];
+  for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
+    const int z = G(g.bf_zone)[b];
+    const int type = P.bc_type[z];
+    const R n[3] = {R(G(g.bfn0)[b]), R(G(g.bfn1)[b]), R(G(g.bfn2)[b])};
+    const R p = self.p;
+    if (type == BC_SLIP) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[1 + i] -= p * n[i];
+    } else if (type == BC_WALL_ADIABATIC) {
+      const R visc_coeff = R(G(w.mu_lam)[c]) * R(G(g.bf_aod)[b]);
+      R visc_work = R(0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const R ubc = R(P.bc_u[z][i]);
+        const R tauijnj = visc_coeff * (self.u[i] - ubc);
+        acc[1 + i] -= p * n[i] + tauijnj;
+        visc_work += ubc * tauijnj;
+      }
+      acc[4] -= visc_work;
+    } else {
+      const R aod = R(G(g.bf_aod)[b]);
+      const R visc_coeff = R(G(w.mu_lam)[c]) * aod;
+      const R k_coeff = R(G(w.loc_lam)[c]) * aod * R(P.cp);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[1 + i] -= p * n[i] + visc_coeff * (self.u[i] - R(0));
+      const R T = (kMixed<R> ? self.p + R(P.p_g) : self.p) / (self.rho * R(P.R_gas));
+      acc[4] -= k_coeff * (T - R(P.bc_T[z]));
+    }
+  }
+}
+
+// The boundary faces' convective CFL terms of cell c (calcCfl,
+// IdealGasSolver.cpp:1069-1074), summed before its faces'.
+template <class R>
+__device__ __forceinline__ R boundary_cfl(const GeomT<R>& g, int c, const Light<R>& self) {
+  R cfl = R(0);
+  if (g.nbf == 0) return cfl;
+  const int b1 = G(g.bf_ptr)[c + 1];
+  for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
+    const R n[3] = {R(G(g.bfn0)[b]), R(G(g.bfn1)[b]), R(G(g.bfn2)[b])};
+    const R area = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const R undA = self.u[0] * n[0] + self.u[1] * n[1] + self.u[2] * n[2];
+    cfl += mabs(undA) + area * self.sos;
+  }
+  return cfl;
+}
+
 // The compact-pair fluxes, in pair order, of the cell at lane position pos
-// (its first ef_nc pairs): the start of its stage RHS accumulation.
+// (its first ef_nc pairs), added to its stage RHS accumulation.
 template <class R>
 __device__ __forceinline__ void gather_compact(const GeomT<R>& g, const WorkT<R>& w, int pos, R acc[5]) {
   const int cbase = G(g.cf_ptr)[pos >> 6] + (pos & (kSlice - 1));
   const int nc = G(g.ef_nc)[pos];
-#pragma unroll
-  for (int q = 0; q < 5; ++q) acc[q] = R(0);
   // pair codes and fluxes are fetched 4 slots at a time so the gathers
   // overlap; the accumulation order stays k = 0, 1, ...
   constexpr int kC = 4;
@@ -601,6 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
 
   const Light<R> self = load_light(in.light, c);
   R acc[5];
+  boundary_flux(g, w, P, c, self, acc);
   gather_compact(g, w, pos, acc);
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
@@ -671,6 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   const int ncv = g.ncv;
   const Light<R> self = load_light(in.light, c);
   R acc[5];
+  boundary_flux(g, w, P, c, self, acc);
   gather_compact(g, w, pos, acc);
   // slots are fetched NB at a time so their global loads overlap (NB is a
   // tuning choice); the accumulation order stays k = 0, 1, ...
@@ -1571,7 +1631,7 @@ __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I,
   // own ln p (FP64) or 1/p (mixed), as staged for the halo
   const R l4_s = kMixed<R> ? mdiv(R(1), self.p + pg) : R(G(w.logp)[c]);
   const R irt_s = irt_of(self.p, rho);  // 1 / (R T), as for the halo
-  R vv2 = R(0), cfl = R(0);
+  R vv2 = R(0), cfl = boundary_cfl(g, c, self);
   // Gibbs remainder of one pair from the neighbour's p, sp_vol, 1/(R T) and
   // ln p (FP64: ln p1 - ln p0) or 1/p (mixed: log1p((p1 - p0) / p0))
   auto gibbs = [&](R op, R ospv, R irt_o, R o4, bool self_hi) {
@@ -1716,7 +1776,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, c
   const int cbase = G(g.cf_ptr)[pos >> 6] + (pos & (kSlice - 1));
   const int nc = G(g.ef_nc)[pos];
   const Light<R> self = load_light(st.light, c);
-  R cfl = R(0);
+  R cfl = boundary_cfl(g, c, self);
   for (int k = 0; k < nc; ++k) {
     const int cslot = cbase + k * kSlice;
     const int code = ldnt(g.cf_nbr + cslot);
@@ -1928,9 +1988,10 @@ int launch_props(const Launch<R>& L, int cur, int fields, void* stream) {
     else hipLaunchKernelGGL((k_sensor_full<R, false>), dim3(L.nblk), dim3(kBlock), 0, s, L.I, L.bmap, cur);
   }
   // the properties pass stages 4 values per halo cell: 4/6 of the flux pass's LDS;
-  // passive scalars need its FP64 loc_lam and loc_sgs every step
+  // passive scalars and boundary faces need its FP64 loc_lam, loc_sgs (and
+  // mu_lam) every step
   const size_t lds = L.lds / 6 * 4;
-  fields = (fields || L.scalars) ? 1 : 0;
+  fields = (fields || L.scalars || L.boundary) ? 1 : 0;
   if (lds) {
     if (fields) hipLaunchKernelGGL((k_props<R, 1, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);
     else hipLaunchKernelGGL((k_props<R, 0, true>), dim3(L.nblk), dim3(kBlock), lds, s, L.I, L.bmap, cur);

@@ -32,6 +32,14 @@ class CharlesError(RuntimeError):
     pass
 
 
+CG_MAX_BC_ZONES = 16
+CG_BC_SLIP, CG_BC_WALL_ADIABATIC, CG_BC_WALL_ISOTHERMAL = 1, 2, 3
+
+
+class cg_bc(C.Structure):
+    _fields_ = [("zone", C.c_char * 48), ("type", C.c_int), ("T", C.c_double), ("u", C.c_double * 3)]
+
+
 class cg_params(C.Structure):
     _fields_ = [
         ("gamma", C.c_double), ("p_ref", C.c_double), ("rho_ref", C.c_double), ("T_ref", C.c_double),
@@ -45,6 +53,7 @@ class cg_params(C.Structure):
         ("sigma_coeff", C.c_double), ("use_full_gr", C.c_int), ("delta_r_dsm", C.c_double),
         ("nsc", C.c_int), ("res_time", C.c_int), ("sc_names", (C.c_char * 32) * 8),
         ("use_compact_faces", C.c_int), ("dvol_form", C.c_int),
+        ("nbc", C.c_int), ("bc", cg_bc * CG_MAX_BC_ZONES),
     ]
 
 
@@ -77,7 +86,7 @@ EXPORTS = [
     "cg_batch_attach", "cg_batch_tune", "cg_layout_create", "cg_layout_bytes", "cg_layout_destroy",
     "cg_instance_create_on", "cg_mles_counts", "cg_mles_get", "cg_mesh_read_mles", "cg_mesh_write_mles",
     "cg_write_sles", "cg_read_sles", "cg_set_scalar", "cg_get_scalar", "cg_sync_post_state",
-    "cg_set_source_hook", "cg_get_diag_static",
+    "cg_set_source_hook", "cg_get_diag_static", "cg_mesh_channel", "cg_mesh_boundary", "cg_mesh_zone_name",
 ]
 
 # addSourceHook(IdealGasRhs* rhs, time, rk_stage): (user, rhs [ncv][5], ncv, time, stage) -> 0
@@ -105,6 +114,10 @@ def load_library(path: str = LIB_PATH):
                                       P(cg_params), P(vp)]),
         "cg_mesh_jittered": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_uint64, P(cg_params), P(vp)]),
+        "cg_mesh_channel": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_uint64, P(cg_params), P(vp)]),
+        "cg_mesh_boundary": (C.c_int, [vp, P(C.c_int64)]),
+        "cg_mesh_zone_name": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int64]),
         "cg_mesh_from_arrays": (C.c_int, [C.c_int, dp, dp, C.c_int, ip, ip, dp, dp, dp, P(cg_params), P(vp)]),
         "cg_mesh_counts": (C.c_int, [vp, P(C.c_int64)]),
         "cg_mles_counts": (C.c_int, [C.c_char_p, P(C.c_int64)]),
@@ -207,7 +220,7 @@ def parse_params(text: str, base: cg_params | None = None) -> cg_params:
 
 MESH_INT_ARRAYS = {"cvofa": 2, "fa_shift": 3, "lo": 1, "hi": 1, "pshift": 3, "group_ef": 1, "group_fa": 1,
                    "cvocv_i": None, "cvocv_v": None, "faocv_i": None, "faocv_v": None, "efocv_i": None,
-                   "efocv_v": None, "grad_flag": 1}
+                   "efocv_v": None, "grad_flag": 1, "cvobf": 1, "zone_bf": 1, "bfocv_i": None, "bfocv_v": None}
 
 
 class Mesh:
@@ -220,6 +233,14 @@ class Mesh:
         _check(lib.cg_mesh_counts(self._h, cnt))
         (self.ncv, self.nfa, self.nef, self.nnz, self.ngrouped_ef, self.ngrouped_fa, self.nthresholded,
          self.ndropped, self.max_ef_deg, self.max_fa_deg) = [int(x) for x in cnt]
+        bnd = (C.c_int64 * 2)()
+        _check(lib.cg_mesh_boundary(self._h, bnd))
+        self.nbf = int(bnd[0])
+        self.zones = []
+        for z in range(int(bnd[1])):
+            buf = C.create_string_buffer(64)
+            _check(lib.cg_mesh_zone_name(self._h, z, buf, len(buf)))
+            self.zones.append(buf.value.decode())
 
     @classmethod
     def lattice(cls, n, L=2 * np.pi, params=None):
@@ -238,6 +259,18 @@ class Mesh:
         h = C.c_void_p()
         _check(lib.cg_mesh_jittered(nx, ny, nz, Lx, Ly, Lz, float(jitter), int(seed),
                                     C.byref(params or default_params()), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def channel(cls, n, jitter=0.3, seed=1, L=2 * np.pi, params=None):
+        """Periodic in x and z, walls at y = 0 (zone "y0") and y = L_y ("y1");
+        jitter 0: the uniform lattice (cg_mesh_channel)."""
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.cg_mesh_channel(nx, ny, nz, Lx, Ly, Lz, float(jitter), int(seed),
+                                   C.byref(params or default_params()), C.byref(h)))
         return cls(h)
 
     @classmethod
@@ -280,7 +313,9 @@ class Mesh:
              "inv_vol": (self.ncv,), "sa_to_vol": (self.ncv,), "dcmag_hat": (self.ncv,), "cmag": (self.ncv,),
              "faocv_i": (self.ncv + 1,), "faocv_v": (2 * self.nfa,), "efocv_i": (self.ncv + 1,),
              "efocv_v": (2 * self.nef,), "grad_flag": (self.ncv,), "L": (3,), "max_gcl": (1,),
-             "r_vv": (self.ncv,)}
+             "r_vv": (self.ncv,), "cvobf": (self.nbf,), "zone_bf": (self.nbf,), "n_bf": (self.nbf, 3),
+             "x_bf": (self.nbf, 3), "area_bf": (self.nbf,), "aod_bf": (self.nbf,), "bfocv_i": (self.ncv + 1,),
+             "bfocv_v": (self.nbf,)}
         return n[name]
 
     def get(self, name):

@@ -865,6 +865,39 @@ static void pair_flux(const og_solver* s, int ief, og_rhs* out) {
   *out = flux;
 }
 
+/* addBoundaryFlux of every zone in zone order (IdealGasSolverBcs.cpp:14-47 SlipWallBc /
+ * WallAdiabatic, :1485-1510 WallIsothermal), first in calcRhsBase
+ * (IdealGasSolver.cpp:261-262): stage p, u, T; lagged mu_lam, loc_lam */
+static void boundary_flux(const og_solver* s, og_rhs* rhs) {
+  const og_mesh* m = &s->m;
+  const og_params* P = &s->p;
+  const double cp = s->R_gas * P->gamma / (P->gamma - 1.0);
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int icv = m->cvobf[ibf];
+    const int z = m->zone_bf[ibf];
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double* u = &s->u[3 * (size_t)icv];
+    const double p = s->pr[icv];
+    if (P->bc_type[z] == 1) {
+      for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i];
+    } else if (P->bc_type[z] == 2) {
+      const double visc_coeff = s->mu_lam[icv] * m->aod_bf[ibf];
+      double visc_work = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        const double tauijnj = visc_coeff * (u[i] - P->bc_u[z][i]);
+        rhs[icv].rhou[i] -= p * n[i] + tauijnj;
+        visc_work += P->bc_u[z][i] * tauijnj;
+      }
+      rhs[icv].rhoE -= visc_work;
+    } else {
+      const double visc_coeff = (s->mu_lam[icv]) * m->aod_bf[ibf];
+      const double k_coeff = (s->loc_lam[icv]) * m->aod_bf[ibf] * cp;
+      for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i] + visc_coeff * (u[i] - 0.0);
+      rhs[icv].rhoE -= k_coeff * (s->T[icv] - P->bc_T[z]);
+    }
+  }
+}
+
 static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
   const og_mesh* m = &s->m;
   const int n = m->ncv;
@@ -872,6 +905,7 @@ static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
   const int nt_ = s->nthreads;
   og_rhs* rhs = (og_rhs*)calloc(n, sizeof(og_rhs));
   if (nsc > 0) memset(rhs_sc, 0, sizeof(double) * (size_t)nsc * n);
+  boundary_flux(s, rhs);
   if (nt_ > 1 && nsc == 0) {
     /* every pair's flux, then each cell adds its pairs' fluxes in pair order */
     build_cell_pairs(s);
@@ -1002,6 +1036,15 @@ double og_calc_cfl(og_solver* s, double dt_) {
   if (nt_ > 1) build_cell_pairs(s);
   double* vb = nt_ > 1 ? (double*)s->pair_buf : NULL;
   for (int icv = 0; icv < m->ncv; ++icv) cfl[icv] = 0.0;
+  /* boundary faces first (IdealGasSolver.cpp:1069-1074) */
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int icv = m->cvobf[ibf];
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double* u = &s->u[3 * (size_t)icv];
+    const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double undA = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    cfl[icv] += fabs(undA) + area * s->sos[icv];
+  }
   OG_PARFOR
   for (int ifa = 0; ifa < m->nfa; ++ifa) {
     const int icv0 = m->lo[ifa], icv1 = m->hi[ifa];

@@ -69,6 +69,13 @@ typedef struct og_mesh {
    * the Voronoi radius of each cell */
   const double* filt_w;     /* [nfa] */
   const double* r_vv;       /* [ncv] */
+  /* boundary faces (zone-major; nbf 0 on a periodic mesh): cell, zone, outward
+   * area vector, area_over_delta_bf */
+  int nbf;
+  const int* cvobf;         /* [nbf] */
+  const int* zone_bf;       /* [nbf] */
+  const double* n_bf;       /* [nbf][3] */
+  const double* aod_bf;     /* [nbf] */
 } og_mesh;
 
 typedef struct og_params {
@@ -86,6 +93,11 @@ typedef struct og_params {
   double delta_r_dsm; /* DELTA_R_DSM */
   int nsc;            /* transported passive scalars (REGISTER_SCALAR names, RES_TIME) */
   int res_time;       /* index of the RES_TIME scalar, or -1 */
+  /* boundary condition of each boundary zone (by zone index): 1 SLIP/SYMMETRY,
+   * 2 WALL_ADIABATIC (wall velocity bc_u), 3 WALL_ISOTHERMAL (wall temperature bc_T) */
+  int bc_type[16];
+  double bc_T[16];
+  double bc_u[16][3];
 } og_params;
 
 typedef struct og_diag {

@@ -22,7 +22,7 @@ class og_mesh(C.Structure):
         (n, C.c_void_p) for n in ("vol", "inv_vol", "sa_to_vol", "dcmag_hat", "lo", "hi", "n_use", "c_use",
                                   "unit_n_use", "area_use", "aod_use", "n_fa", "cvocv_i", "cvocv_v", "grad_coeff",
                                   "filt_w", "r_vv")
-    ]
+    ] + [("nbf", C.c_int)] + [(n, C.c_void_p) for n in ("cvobf", "zone_bf", "n_bf", "aod_bf")]
 
 
 class og_params(C.Structure):
@@ -34,6 +34,7 @@ class og_params(C.Structure):
         ("coeff_h", C.c_double), ("gr_nrm", C.c_double), ("diss_mode", C.c_int), ("dt_mode", C.c_int),
         ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
         ("full_gr", C.c_int), ("delta_r_dsm", C.c_double), ("nsc", C.c_int), ("res_time", C.c_int),
+        ("bc_type", C.c_int * 16), ("bc_T", C.c_double * 16), ("bc_u", (C.c_double * 3) * 16),
     ]
 
 
@@ -146,9 +147,20 @@ def filter_weights(mesh):
     return np.ascontiguousarray(area * dist / 6.0)
 
 
-def params_from(cgp) -> og_params:
+def params_from(cgp, zones=()) -> og_params:
+    """The oracle's parameters; `zones` are the mesh's boundary zone names, each
+    given the condition cgp holds under that name."""
     p = og_params()
+    bcs = {cgp.bc[i].zone.decode(): cgp.bc[i] for i in range(cgp.nbc)}
+    for z, name in enumerate(zones):
+        b = bcs[name]
+        p.bc_type[z] = b.type
+        p.bc_T[z] = b.T
+        for d in range(3):
+            p.bc_u[z][d] = b.u[d]
     for name, _ in og_params._fields_:
+        if name in ("bc_type", "bc_T", "bc_u"):
+            continue
         if name == "full_gr":
             p.full_gr = cgp.use_full_gr
         elif name == "dt_data":
@@ -179,8 +191,14 @@ class OracleSolver:
         for field, a in (("filt_w", filter_weights(mesh)), ("r_vv", np.ascontiguousarray(mesh.get("r_vv")))):
             self._keep[field] = a
             setattr(m, field, a.ctypes.data)
+        m.nbf = getattr(mesh, "nbf", 0)
+        if m.nbf:
+            for field in ("cvobf", "zone_bf", "n_bf", "aod_bf"):
+                a = np.ascontiguousarray(mesh.get(field))
+                self._keep[field] = a
+                setattr(m, field, a.ctypes.data)
         self._m = m
-        self._p = params_from(cgparams)
+        self._p = params_from(cgparams, getattr(mesh, "zones", ()))
         self.ncv = mesh.ncv
         self._h = self._L.og_create(C.byref(self._m), C.byref(self._p))
         self._L.og_set_threads(self._h, int(threads))

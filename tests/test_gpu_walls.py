@@ -1,0 +1,187 @@
+"""GPU parity of wall boundary conditions against the oracle (FP64 CPU
+restatement; its wall terms are pinned by closed forms in
+test_boundary_conditions.py): SLIP / SYMMETRY, WALL_ADIABATIC (with a
+TRANSLATE wall velocity) and WALL_ISOTHERMAL on jittered channel meshes and on
+the reference's own Stitch mesh orifice.mles (six boundary zones).
+
+FP64: p, T, sos and dudx bit-identical, the lagged coefficients within 1e-14,
+the stage RHS within 1e-12 of each component's largest magnitude, the CFL
+maximum within 1e-14 and 20-step histories of dt, time, KE, enstrophy and the
+five sums within 1e-10. Mixed precision: the stage RHS within 2e-5 and KE after
+20 steps within 1e-5 of the FP64 oracle. A channel solve batched with a
+periodic one equals both solves run alone, bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from nextgen_amd.ic import tgv_params
+from nextgen_amd.ideal_gas import IdealGasSolver, Mesh, parse_params
+from oracle.oracle import OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+ORIFICE = "/root/reference/src/core/common/tests/assets/orifice.mles"
+RHS_TOL = 1e-12
+HIST_TOL = 1e-10
+
+WALLS = {
+    "slip+translate": {"y0": "SLIP", "y1": "WALL_ADIABATIC TRANSLATE 0.3 0 0.1"},
+    "isothermal+adiabatic": {"y0": "WALL_ISOTHERMAL 1.05", "y1": "WALL_ADIABATIC"},
+}
+
+
+def deck(walls, sgs="VREMAN", prec="FP64"):
+    return tgv_params(sgs=sgs) + "".join(f"{z} {w}\n" for z, w in walls.items()) + f"PRECISION {prec}\n"
+
+
+def channel_state(m, Ma=0.1, gamma=1.4):
+    """A wall-bounded shear layer with TGV-like cells: u = 1 - ((y - pi)/pi)^2 + a
+    cellular perturbation, uniform T, pressure ripples."""
+    x = m.get("x_cv")
+    c0 = 1.0 / Ma
+    p0 = c0 * c0 / gamma
+    y = (x[:, 1] - np.pi) / np.pi
+    u = np.empty((m.ncv, 3))
+    u[:, 0] = 1.0 - y * y + 0.2 * np.sin(x[:, 0]) * np.cos(x[:, 2])
+    u[:, 1] = 0.1 * np.sin(x[:, 2]) * np.sin(x[:, 1])
+    u[:, 2] = -0.2 * np.cos(x[:, 0]) * np.sin(x[:, 2])
+    p = p0 + 0.05 * np.cos(x[:, 0]) * np.cos(2 * x[:, 1])
+    rho = p / p0
+    rhoE = p / (gamma - 1.0) + 0.5 * rho * (u * u).sum(1)
+    return rho, u, rhoE
+
+
+def pair(m, text, state):
+    p = parse_params(text)
+    g = IdealGasSolver(m, p)
+    g.set_state(*state)
+    g.init_complete()
+    o = OracleSolver(m, p)
+    o.set_state(*state)
+    o.init_complete()
+    return g, o
+
+
+def rel(a, b):
+    return (np.abs(a - b).max(axis=0) / np.maximum(np.abs(b).max(axis=0), 1e-300)).max()
+
+
+def check_state_and_rhs(g, o, what):
+    for name in ("p", "T", "sos", "dudx"):
+        np.testing.assert_array_equal(g.get_field(name), o.get_field(name), err_msg=f"{what} {name}")
+    for name in ("mu_lam", "loc_lam", "mu_sgs", "a_sgs", "fgr", "mu_total", "loc_total"):
+        a, b = g.get_field(name), o.get_field(name)
+        assert np.max(np.abs(a - b)) <= 1e-14 * max(np.max(np.abs(b)), 1e-300), (what, name)
+    rg, ro = g.calc_rhs(1), o.calc_rhs()
+    err = rel(rg, ro)
+    print(f"{what}: RHS rel err {err:.3e}, bit-identical cells {np.mean(np.all(rg == ro, axis=1)):.4f}", flush=True)
+    assert err <= RHS_TOL, what
+    dt = 2e-3
+    cg = g.calc_cfl(dt)
+    co = o.calc_cfl(dt)
+    assert abs(cg - co) <= 1e-14 * co, (what, cg, co)
+
+
+def check_history(g, o, n, what):
+    for d in g.run_history(n):
+        assert o.step() == 0
+        od = o.diag()
+        assert d.step == od["step"]
+        assert abs(d.dt - od["dt"]) <= HIST_TOL * od["dt"], what
+        assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"], (what, d.ke, od["ke"])
+        assert abs(d.enstrophy - od["enstrophy"]) <= HIST_TOL * od["enstrophy"], what
+        for q in range(5):
+            assert abs(d.sums[q] - od["sums"][q]) <= HIST_TOL * max(abs(od["sums"][q]), od["sums"][0]), (what, q)
+
+
+@pytest.mark.parametrize("walls", sorted(WALLS))
+@pytest.mark.parametrize("sgs", ["NONE", "VREMAN"])
+def test_channel_walls_match_oracle(walls, sgs):
+    m = Mesh.channel(12, jitter=0.3, seed=7)
+    g, o = pair(m, deck(WALLS[walls], sgs), channel_state(m))
+    check_state_and_rhs(g, o, f"channel {walls} {sgs}")
+    check_history(g, o, 20, f"channel {walls} {sgs}")
+
+
+def test_lattice_channel_walls_match_oracle():
+    m = Mesh.channel(10, jitter=0.0)
+    g, o = pair(m, deck({"y0": "WALL_ISOTHERMAL 0.97", "y1": "SYMMETRY"}, "NONE"), channel_state(m))
+    check_state_and_rhs(g, o, "lattice channel")
+    check_history(g, o, 20, "lattice channel")
+
+
+@pytest.mark.skipif(not os.path.exists(ORIFICE), reason="reference mesh fixture not present")
+def test_orifice_mesh_with_walls_matches_oracle():
+    # the reference's own Stitch mesh (an orifice plate in a pipe), every zone a
+    # wall: the pipe and plate no-slip, inlet and outlet closed by an isothermal
+    # wall and a slip wall; a pressure pulse and a swirl in a gas at rest
+    m = Mesh.read_mles(ORIFICE)
+    walls = {"inlet": "WALL_ISOTHERMAL 1.1", "outlet": "SLIP", "oriface-pipe": "WALL_ADIABATIC",
+             "wall": "WALL_ADIABATIC", "oriface-wall-0": "WALL_ADIABATIC", "oriface-wall-1": "WALL_ADIABATIC"}
+    x = m.get("x_cv")
+    xc = x.mean(axis=0)
+    r2 = ((x - xc) ** 2).sum(1) / ((x - xc) ** 2).sum(1).max()
+    gamma, p0 = 1.4, 1.0 / 1.4
+    p = p0 * (1.0 + 0.05 * np.exp(-10 * r2))
+    rho = np.ones(m.ncv) * (1.0 + 0.02 * np.exp(-10 * r2))
+    u = np.zeros((m.ncv, 3))
+    u[:, 0] = 0.05 * (x[:, 1] - xc[1]) / np.abs(x[:, 1] - xc[1]).max()
+    u[:, 1] = -0.05 * (x[:, 0] - xc[0]) / np.abs(x[:, 0] - xc[0]).max()
+    rhoE = p / (gamma - 1.0) + 0.5 * rho * (u * u).sum(1)
+    text = (f"GAMMA 1.4\nP_REF {p0!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF 1e-5\nSGS_MODEL VREMAN\nTIMESTEP CFL 0.8\n" +
+            "".join(f"{z} {w}\n" for z, w in walls.items()))
+    g, o = pair(m, text, (rho, u, rhoE))
+    check_state_and_rhs(g, o, "orifice")
+    check_history(g, o, 10, "orifice")
+
+
+def test_mixed_precision_channel_close_to_oracle():
+    m = Mesh.channel(12, jitter=0.3, seed=8)
+    walls = {"y0": "WALL_ISOTHERMAL 1.05", "y1": "WALL_ADIABATIC TRANSLATE 0.3 0 0"}
+    st = channel_state(m)
+    g = IdealGasSolver(m, parse_params(deck(walls, prec="MIXED")))
+    g.set_state(*st)
+    g.init_complete()
+    o = OracleSolver(m, parse_params(deck(walls)))
+    o.set_state(*st)
+    o.init_complete()
+    err = rel(g.calc_rhs(1), o.calc_rhs())
+    print(f"mixed channel RHS rel err {err:.2e}", flush=True)
+    assert err <= 2e-5
+    g.advance_solution(20)
+    o.step(20)
+    d, od = g.report(), o.diag()
+    assert abs(d.ke - od["ke"]) <= 1e-5 * od["ke"]
+    assert abs(d.sums[0] - od["sums"][0]) <= 1e-9 * od["sums"][0]
+
+
+@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
+def test_channel_batched_with_periodic_solve_equals_solo(prec):
+    from nextgen_amd.batch import Batch
+    from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
+    precision = CG_PREC_MIXED if prec == "MIXED" else CG_PREC_FP64
+    mc = Mesh.channel(12, jitter=0.3, seed=9)
+    mp = Mesh.jittered(10, 0.3, seed=10)
+    cases = [(mc, deck(WALLS["slip+translate"], prec=prec), channel_state(mc)),
+             (mp, tgv_params(sgs="VREMAN") + f"PRECISION {prec}\n", mp.ic_tgv(0.1, 1.4))]
+    b = Batch(precision)
+    ids = [b.add(m, parse_params(t), *st) for m, t, st in cases]
+    b.step(7)
+    for iid, (m, t, st) in zip(ids, cases):
+        g = IdealGasSolver(m, parse_params(t))
+        g.set_state(*st)
+        g.init_complete()
+        g.advance_solution(7)
+        for x, y in zip(b.state(iid), g.get_state()):
+            np.testing.assert_array_equal(x, y)
+        g.close()
+    b.close()
+
+
+def test_zone_without_condition_is_refused_on_the_device():
+    from nextgen_amd.ideal_gas import CharlesError
+    m = Mesh.channel(10, jitter=0.0)
+    with pytest.raises(CharlesError, match="y1"):
+        IdealGasSolver(m, parse_params(deck({"y0": "SLIP"})))
