@@ -76,16 +76,28 @@ typedef struct cg_solver cg_solver;
  *   SLIP or SYMMETRY                 SlipWallBc (IdealGasSolverBcs.cpp:14-23)
  *   WALL_ADIABATIC [TRANSLATE u v w]  WallAdiabatic (:25-47, options :450-497)
  *   WALL_ISOTHERMAL <T>              WallIsothermal (:1480-1510)
+ *   CBC_UPT <u> <v> <w> <p> <T>       constant boundary states with the
+ *   CBC_MPT <mdot> <p> <T>            Riemann flux calcRiemannFlux
+ *   CBC_RUNP <rho> <un> <p>           (IdealGasSolverFlux.hpp:218-306):
+ *   CBC_RUP <rho> <u> <v> <w> <p>     Cbc::addBoundaryFlux (:2960-2976),
+ *                                    CbcUpt/Mpt/Runp/Rup::initData (:3182-3300)
  * The other types of the reference are refused by cg_params_parse. */
 #define CG_MAX_BC_ZONES 16
 #define CG_BC_SLIP 1
 #define CG_BC_WALL_ADIABATIC 2
 #define CG_BC_WALL_ISOTHERMAL 3
+#define CG_BC_CBC_UPT 4
+#define CG_BC_CBC_MPT 5
+#define CG_BC_CBC_RUNP 6
+#define CG_BC_CBC_RUP 7
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
-  double T;       /* WALL_ISOTHERMAL wall temperature */
-  double u[3];    /* WALL_ADIABATIC TRANSLATE wall velocity (0 otherwise) */
+  double T;       /* wall temperature (WALL_ISOTHERMAL), T of CBC_UPT / CBC_MPT */
+  double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
+  double p;       /* CBC pressure */
+  double rho;     /* CBC_RUNP / CBC_RUP density */
+  double q;       /* CBC_MPT mass flow mdot, CBC_RUNP inflow normal velocity un */
 } cg_bc;
 
 /* Input-file parameters of the path; the names in comments are the reference's

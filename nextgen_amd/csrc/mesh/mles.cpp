@@ -443,6 +443,7 @@ class Writer {
     RecHeader h = header(name, id, 8 * rows * width);
     set_count(h.idata, rows);
     if (width == 3) h.idata[2] = 3;
+    if (width == 9) h.idata[2] = h.idata[3] = 3;
     put_header(h);
     put(p, size_t(rows * width));
   }
@@ -491,13 +492,20 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
     }
   }
 
+  bool periodic = false;
+  for (int8_t sh : m.shift) periodic = periodic || sh != 0;
+  const int nbf = m.nbf;
+  if (nbf > 0 && (m.cvobf.size() != size_t(nbf) || m.n_bf.size() != size_t(3 * nbf) ||
+                  m.x_bf.size() != size_t(3 * nbf) || m.bf_zone_i.size() != m.bf_zone_name.size() + 1))
+    return "inconsistent boundary faces";
+
   Writer w(path);
   if (!w.ok()) return "cannot create " + path;
   w.start(kMagic, 5);
   {
     RecHeader h = w.header("no_fa_bf_cv_counts", kCounts, 2 * 27 * 8);
     set_count(h.idata + 0, 0);  // nno: geometry-only file, no node records
-    set_count(h.idata + 2, 0);  // nbf
+    set_count(h.idata + 2, nbf);
     set_count(h.idata + 4, nfa);
     set_count(h.idata + 6, ncv);
     set_count(h.idata + 8, 0);   // nno_p
@@ -506,7 +514,7 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
     w.put(nfa_zone, 27);
     w.put(noofa_zone, 27);
   }
-  {
+  if (periodic) {
     RecHeader h = w.header("periodic_transforms", kPeriodic, 0);
     h.idata[0] = 3;
     for (int d = 0; d < 3; ++d) {
@@ -514,6 +522,27 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
       h.rdata[3 * d + d] = m.L[d];
     }
     w.put_header(h);
+  }
+  // boundary zones (a contiguous range of boundary faces each) and their faces
+  for (size_t z = 0; z < m.bf_zone_name.size(); ++z) {
+    RecHeader h = w.header(m.bf_zone_name[z].c_str(), kBfZoneHeader, 0);
+    h.idata[1] = int32_t(z);
+    set_count(h.idata + 2, m.bf_zone_i[z + 1] - m.bf_zone_i[z]);
+    set_count(h.idata + 4, 0);  // no node records
+    set_count(h.idata + 6, m.bf_zone_i[z]);
+    w.put_header(h);
+  }
+  if (nbf > 0) {
+    std::vector<int64_t> cvobf(m.cvobf.begin(), m.cvobf.end());
+    RecHeader h = w.header("cvobf_global", kCvobf, 8 * int64_t(nbf));
+    set_count(h.idata, nbf);
+    w.put_header(h);
+    w.put(cvobf.data(), cvobf.size());
+    w.put_array("n_bf", kBfD2, m.n_bf.data(), nbf, 3);
+    w.put_array("x_bf", kBfD2, m.x_bf.data(), nbf, 3);
+    if (m.aod_bf_file.size() == size_t(nbf))
+      w.put_array("area_over_delta_bf", kBfD1, m.aod_bf_file.data(), nbf, 1);
+    if (m.gij_bf_file.size() == size_t(9 * nbf)) w.put_array("Gij_bf", kBfDN33, m.gij_bf_file.data(), nbf, 9);
   }
   {
     RecHeader h = w.header("cvofa_global", kCvofa, 16 * int64_t(nfa));

@@ -116,6 +116,7 @@ struct Layout {
   bool has_rvv = false;       // the mesh knows its cells' Voronoi radii (DSM_LOCAL needs them)
   int lds_max = 0;            // the device's LDS bytes per workgroup
   std::vector<std::string> bf_zone_names;  // the mesh's boundary zones (index = zone)
+  std::vector<double> zone_area;           // sum of each zone's |n_bf| in face order
 };
 
 // LDS bytes of the scalar pass's halo variant: the flux pass's records of a
@@ -551,6 +552,9 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   // boundary faces in device cell order (each cell's in zone-major order)
   g.nbf = o.nbf;
   Lay->bf_zone_names = m->vor.bf_zone_name;
+  // each zone's area_global (StripedMesh.hpp:2818-2831): its faces' areas in face order
+  Lay->zone_area.assign(Lay->bf_zone_names.size(), 0.0);
+  for (int b = 0; b < o.nbf; ++b) Lay->zone_area[(size_t)o.zone_bf[b]] += o.area_bf[b];
   if (o.nbf > 0) {
     std::vector<int> bptr(ncv + 1, 0), bzone(o.nbf);
     std::vector<double> bn[3], baod(o.nbf);
@@ -865,6 +869,10 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_type[z] = p->bc[k].type;
       hi->P.bc_T[z] = p->bc[k].T;
       for (int d = 0; d < 3; ++d) hi->P.bc_u[z][d] = p->bc[k].u[d];
+      hi->P.bc_p[z] = p->bc[k].p;
+      hi->P.bc_rho[z] = p->bc[k].rho;
+      hi->P.bc_q[z] = p->bc[k].q;
+      hi->P.bc_area[z] = lay->zone_area[z];
     }
     if (!missing.empty()) return s->set(CG_EINVAL, "no boundary condition for zone(s) " + missing);
     if (lay->g.nbf > 0 && (p->sgs_model == CG_SGS_DSM_LOCAL || p->nsc > 0))
@@ -1558,9 +1566,29 @@ int cg_params_parse(cg_params* p, const char* text) {
       } else if (type == "WALL_ISOTHERMAL") {
         b.type = CG_BC_WALL_ISOTHERMAL;
         if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WALL_ISOTHERMAL <T>");
+      } else if (type == "CBC_UPT") {
+        b.type = CG_BC_CBC_UPT;
+        if (tok.size() != 7 || !num(2, &b.u[0]) || !num(3, &b.u[1]) || !num(4, &b.u[2]) || !num(5, &b.p) ||
+            !num(6, &b.T) || !(b.p > 0.0 && b.T > 0.0))
+          return bad("expected CBC_UPT <u> <v> <w> <p> <T>");
+      } else if (type == "CBC_MPT") {
+        b.type = CG_BC_CBC_MPT;
+        if (tok.size() != 5 || !num(2, &b.q) || !num(3, &b.p) || !num(4, &b.T) || !(b.p > 0.0 && b.T > 0.0))
+          return bad("expected CBC_MPT <mdot> <p> <T>");
+      } else if (type == "CBC_RUNP") {
+        b.type = CG_BC_CBC_RUNP;
+        if (tok.size() != 5 || !num(2, &b.rho) || !num(3, &b.q) || !num(4, &b.p) || !(b.p > 0.0 && b.rho > 0.0))
+          return bad("expected CBC_RUNP <rho> <un> <p>");
+      } else if (type == "CBC_RUP") {
+        b.type = CG_BC_CBC_RUP;
+        const bool frame = tok.size() == 8 && upper(tok[7]) == "STATIONARY_FRAME";
+        if ((tok.size() != 7 && !frame) || !num(2, &b.rho) || !num(3, &b.u[0]) || !num(4, &b.u[1]) ||
+            !num(5, &b.u[2]) || !num(6, &b.p) || !(b.p > 0.0 && b.rho > 0.0))
+          return bad("expected CBC_RUP <rho> <u> <v> <w> <p> [STATIONARY_FRAME]");
+        // STATIONARY_FRAME matters only in a rotating frame, which this path does not have
       } else {
-        return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC and "
-                   "WALL_ISOTHERMAL are)");
+        return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
+                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP and CBC_RUP are)");
       }
       int slot = -1;
       for (int i = 0; i < p->nbc; ++i)

@@ -141,7 +141,10 @@ struct WorkT {
 };
 
 constexpr int kMaxBcZones = 16;
-enum { BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3 };
+enum {
+  BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
+  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7
+};
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;
@@ -158,6 +161,8 @@ struct Params {
   int bc_type[kMaxBcZones];
   double bc_T[kMaxBcZones];
   double bc_u[kMaxBcZones][3];
+  double bc_p[kMaxBcZones], bc_rho[kMaxBcZones], bc_q[kMaxBcZones];
+  double bc_area[kMaxBcZones];  // the zone's area (sum of its faces' |n_bf|), CBC_MPT
 };
 
 using Geom = GeomT<double>;

@@ -539,6 +539,115 @@ __device__ __forceinline__ int cell_at(const uint8_t* perm, int pos) {
   return (pos & ~(kSlice - 1)) | int(G(perm)[pos]);
 }
 
+// calcRiemannFlux (IdealGasSolverFlux.hpp:218-306), FP64 in either precision:
+// the HLLC flux through the boundary face n_bf between the cell's state
+// (u, sp_vol, p, h) = a and a boundary state b (gammaL = gammaR = gamma).
+__device__ __forceinline__ void riemann_flux(double F[5], const double n_bf[3], const double ua[3], double va,
+                                             double pa, double ha, const double ub[3], double vb, double pb,
+                                             double hb, double gamma) {
+#pragma clang fp contract(off)  // FP64: rounded as the reference rounds
+  const double area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+  double unit_n[3];
+  for (int i = 0; i < 3; ++i) unit_n[i] = n_bf[i] / area;
+  const double unL = ua[0] * unit_n[0] + ua[1] * unit_n[1] + ua[2] * unit_n[2];
+  const double uLuL = ua[0] * ua[0] + ua[1] * ua[1] + ua[2] * ua[2];
+  const double cL = sqrt(gamma * pa * va);
+  const double pL = pa;
+  const double HL = ha + 0.5 * uLuL;
+  const double rhoL = 1.0 / va;
+  const double EL = rhoL * HL - pa;
+  const double unR = ub[0] * unit_n[0] + ub[1] * unit_n[1] + ub[2] * unit_n[2];
+  const double uRuR = ub[0] * ub[0] + ub[1] * ub[1] + ub[2] * ub[2];
+  const double cR = sqrt(gamma * pb * vb);
+  const double pR = pb;
+  const double HR = hb + 0.5 * uRuR;
+  const double rhoR = 1.0 / vb;
+  const double ER = rhoR * HR - pb;
+  const double Rrho = sqrt(rhoR / rhoL);
+  const double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; ++i) velRoe[i] = tmp * (ua[i] + ub[i] * Rrho);
+  const double uRoe = velRoe[0] * unit_n[0] + velRoe[1] * unit_n[1] + velRoe[2] * unit_n[2];
+  const double gamPdivRho =
+      tmp * ((gamma * pL / rhoL + 0.5 * (gamma - 1.0) * uLuL) + (gamma * pR / rhoR + 0.5 * (gamma - 1.0) * uRuR) * Rrho);
+  const double cRoe = sqrt(gamPdivRho - ((gamma + gamma) * 0.5 - 1.0) * 0.5 *
+                                            (velRoe[0] * velRoe[0] + velRoe[1] * velRoe[1] + velRoe[2] * velRoe[2]));
+  const double sL = smin(uRoe - cRoe, unL - cL);
+  const double sR = smax(uRoe + cRoe, unR + cR);
+  const double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  const double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      F[0] = rhoL * unL;
+      for (int i = 0; i < 3; ++i) F[1 + i] = rhoL * ua[i] * unL + pL * unit_n[i];
+      F[4] = (EL + pL) * unL;
+    } else {
+      const double invSLmSs = 1.0 / (sL - sM);
+      const double sLmuL = sL - unL;
+      const double rhoSL = rhoL * sLmuL * invSLmSs;
+      double rhouSL[3];
+      for (int i = 0; i < 3; ++i) rhouSL[i] = (rhoL * ua[i] * sLmuL + (pStar - pL) * unit_n[i]) * invSLmSs;
+      const double eSL = (sLmuL * EL - pL * unL + pStar * sM) * invSLmSs;
+      F[0] = rhoSL * sM;
+      for (int i = 0; i < 3; ++i) F[1 + i] = rhouSL[i] * sM + pStar * unit_n[i];
+      F[4] = (eSL + pStar) * sM;
+    }
+  } else {
+    if (sR >= 0.0) {
+      const double invSRmSs = 1.0 / (sR - sM);
+      const double sRmuR = sR - unR;
+      const double rhoSR = rhoR * sRmuR * invSRmSs;
+      double rhouSR[3];
+      for (int i = 0; i < 3; ++i) rhouSR[i] = (rhoR * ub[i] * sRmuR + (pStar - pR) * unit_n[i]) * invSRmSs;
+      const double eSR = (sRmuR * ER - pR * unR + pStar * sM) * invSRmSs;
+      F[0] = rhoSR * sM;
+      for (int i = 0; i < 3; ++i) F[1 + i] = rhouSR[i] * sM + pStar * unit_n[i];
+      F[4] = (eSR + pStar) * sM;
+    } else {
+      F[0] = rhoR * unR;
+      for (int i = 0; i < 3; ++i) F[1 + i] = rhoR * ub[i] * unR + pR * unit_n[i];
+      F[4] = (ER + pR) * unR;
+    }
+  }
+  F[0] *= area;
+  for (int i = 0; i < 3; ++i) F[1 + i] *= area;
+  F[4] *= area;
+}
+
+// The constant boundary state of a CBC zone at face n_bf (CbcUpt / CbcMpt /
+// CbcRunp / CbcRup::initData, IdealGasSolverBcs.cpp:3182-3300)
+__device__ __forceinline__ void cbc_state(const Params& P, int z, int type, const double n_bf[3], double u[3],
+                                          double& sp_vol, double& p, double& h) {
+#pragma clang fp contract(off)  // FP64: rounded as the reference rounds
+  const double R = P.R_gas, gamma = P.gamma;
+  p = P.bc_p[z];
+  double T;
+  if (type == BC_CBC_UPT) {
+    T = P.bc_T[z];
+    for (int i = 0; i < 3; ++i) u[i] = P.bc_u[z][i];
+    sp_vol = R * T / p;
+  } else if (type == BC_CBC_MPT) {
+    T = P.bc_T[z];
+    const double rho = p / T / R;
+    const double un = -P.bc_q[z] / (rho * P.bc_area[z]);
+    const double bf_area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+    for (int i = 0; i < 3; ++i) u[i] = un * n_bf[i] / bf_area;
+    sp_vol = 1.0 / rho;
+  } else if (type == BC_CBC_RUNP) {
+    const double rho = P.bc_rho[z];
+    T = p / rho / R;
+    const double bf_area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+    for (int i = 0; i < 3; ++i) u[i] = -P.bc_q[z] * n_bf[i] / bf_area;
+    sp_vol = 1.0 / rho;
+  } else {
+    const double rho = P.bc_rho[z];
+    T = p / rho / R;
+    for (int i = 0; i < 3; ++i) u[i] = P.bc_u[z][i];
+    sp_vol = 1.0 / rho;
+  }
+  h = R * T * gamma / (gamma - 1.0);
+}
+
 // The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
 // IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
 // face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
@@ -546,8 +655,8 @@ __device__ __forceinline__ int cell_at(const uint8_t* perm, int pos) {
 // the gauge's p_g n sums to zero over a cell's pair and boundary faces), mu_lam
 // and loc_lam the lagged laminar coefficients, T = p / (rho R) of the stage.
 template <class R>
-__device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>& w, const Params& P, int c,
-                                              const Light<R>& self, R acc[5]) {
+__device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st,
+                                              const Params& P, int c, const Light<R>& self, R acc[5]) {
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
   if (g.nbf == 0) return;
@@ -571,6 +680,38 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
         visc_work += ubc * tauijnj;
       }
       acc[4] -= visc_work;
+    } else if (type >= BC_CBC_UPT) {
+      // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
+      // between the cell's FP64 state and the zone's boundary state
+      const double nd[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
+      double ua[3], va, pa, ha;
+      if constexpr (kMixed<R>) {
+        const int ncv = g.ncv;
+        double uu[3];
+        for (int i = 0; i < 3; ++i) uu[i] = G(st.cons)[(size_t)(1 + i) * ncv + c];
+        double T;
+        const Light<double> L = eos(P, G(st.cons)[c], uu, G(st.cons)[(size_t)4 * ncv + c], T);
+        for (int i = 0; i < 3; ++i) ua[i] = L.u[i];
+        va = L.spv; pa = L.p; ha = L.h;
+      } else {
+        for (int i = 0; i < 3; ++i) ua[i] = self.u[i];
+        va = self.spv; pa = self.p; ha = self.h;
+      }
+      double ub[3], vb, pb, hb;
+      cbc_state(P, z, type, nd, ub, vb, pb, hb);
+      double F[5];
+      riemann_flux(F, nd, ua, va, pa, ha, ub, vb, pb, hb, P.gamma);
+      if constexpr (kMixed<R>) {
+        // the accumulation's gauges: momentum without p_g n, energy as F_rhoE - h_g F_rho
+        #pragma clang fp contract(off)
+        acc[0] -= R(F[0]);
+        for (int i = 0; i < 3; ++i) acc[1 + i] -= R(F[1 + i] - P.p_g * nd[i]);
+        acc[4] -= R(F[4] - P.h_g * F[0]);
+      } else {
+        acc[0] -= F[0];
+        for (int i = 0; i < 3; ++i) acc[1 + i] -= F[1 + i];
+        acc[4] -= F[4];
+      }
     } else {
       const R aod = R(G(g.bf_aod)[b]);
       const R visc_coeff = R(G(w.mu_lam)[c]) * aod;
@@ -659,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
 
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  boundary_flux(g, w, P, c, self, acc);
+  boundary_flux(g, w, in, P, c, self, acc);
   gather_compact(g, w, pos, acc);
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
@@ -730,7 +871,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   const int ncv = g.ncv;
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  boundary_flux(g, w, P, c, self, acc);
+  boundary_flux(g, w, in, P, c, self, acc);
   gather_compact(g, w, pos, acc);
   // slots are fetched NB at a time so their global loads overlap (NB is a
   // tuning choice); the accumulation order stays k = 0, 1, ...

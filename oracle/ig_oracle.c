@@ -865,6 +865,113 @@ static void pair_flux(const og_solver* s, int ief, og_rhs* out) {
   *out = flux;
 }
 
+/* IdealGasSolverFlux.hpp:218-306: HLLC flux between cv0 (inside) and cv1 */
+void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,
+                     double gammaL, double gammaR) {
+  const double area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+  double unit_n[3];
+  for (int i = 0; i < 3; ++i) unit_n[i] = n_bf[i] / area;
+  const double unL = cv0->u[0] * unit_n[0] + cv0->u[1] * unit_n[1] + cv0->u[2] * unit_n[2];
+  const double uLuL = cv0->u[0] * cv0->u[0] + cv0->u[1] * cv0->u[1] + cv0->u[2] * cv0->u[2];
+  const double cL = sqrt(gammaL * cv0->p * cv0->sp_vol);
+  const double pL = cv0->p;
+  const double HL = cv0->h + 0.5 * uLuL;
+  const double rhoL = 1.0 / cv0->sp_vol;
+  const double EL = rhoL * HL - cv0->p;
+  const double unR = cv1->u[0] * unit_n[0] + cv1->u[1] * unit_n[1] + cv1->u[2] * unit_n[2];
+  const double uRuR = cv1->u[0] * cv1->u[0] + cv1->u[1] * cv1->u[1] + cv1->u[2] * cv1->u[2];
+  const double cR = sqrt(gammaR * cv1->p * cv1->sp_vol);
+  const double pR = cv1->p;
+  const double HR = cv1->h + 0.5 * uRuR;
+  const double rhoR = 1.0 / cv1->sp_vol;
+  const double ER = rhoR * HR - cv1->p;
+  /* Roe averages */
+  const double Rrho = sqrt(rhoR / rhoL);
+  const double tmp = 1.0 / (1.0 + Rrho);
+  double velRoe[3];
+  for (int i = 0; i < 3; ++i) velRoe[i] = tmp * (cv0->u[i] + cv1->u[i] * Rrho);
+  const double uRoe = velRoe[0] * unit_n[0] + velRoe[1] * unit_n[1] + velRoe[2] * unit_n[2];
+  const double gamPdivRho = tmp * ((gammaL * pL / rhoL + 0.5 * (gammaL - 1.0) * uLuL) +
+                                   (gammaR * pR / rhoR + 0.5 * (gammaR - 1.0) * uRuR) * Rrho);
+  const double cRoe = sqrt(gamPdivRho - ((gammaL + gammaR) * 0.5 - 1.0) * 0.5 *
+                                            (velRoe[0] * velRoe[0] + velRoe[1] * velRoe[1] + velRoe[2] * velRoe[2]));
+  /* wave speeds, contact speed and star pressure */
+  const double sL = smin(uRoe - cRoe, unL - cL);
+  const double sR = smax(uRoe + cRoe, unR + cR);
+  const double sM = (pL - pR - rhoL * unL * (sL - unL) + rhoR * unR * (sR - unR)) / (rhoR * (sR - unR) - rhoL * (sL - unL));
+  const double pStar = rhoR * (unR - sR) * (unR - sM) + pR;
+  if (sM >= 0.0) {
+    if (sL > 0.0) {
+      flux[0] = rhoL * unL;
+      for (int i = 0; i < 3; ++i) flux[1 + i] = rhoL * cv0->u[i] * unL + pL * unit_n[i];
+      flux[4] = (EL + pL) * unL;
+    } else {
+      const double invSLmSs = 1.0 / (sL - sM);
+      const double sLmuL = sL - unL;
+      const double rhoSL = rhoL * sLmuL * invSLmSs;
+      double rhouSL[3];
+      for (int i = 0; i < 3; ++i) rhouSL[i] = (rhoL * cv0->u[i] * sLmuL + (pStar - pL) * unit_n[i]) * invSLmSs;
+      const double eSL = (sLmuL * EL - pL * unL + pStar * sM) * invSLmSs;
+      flux[0] = rhoSL * sM;
+      for (int i = 0; i < 3; ++i) flux[1 + i] = rhouSL[i] * sM + pStar * unit_n[i];
+      flux[4] = (eSL + pStar) * sM;
+    }
+  } else {
+    if (sR >= 0.0) {
+      const double invSRmSs = 1.0 / (sR - sM);
+      const double sRmuR = sR - unR;
+      const double rhoSR = rhoR * sRmuR * invSRmSs;
+      double rhouSR[3];
+      for (int i = 0; i < 3; ++i) rhouSR[i] = (rhoR * cv1->u[i] * sRmuR + (pStar - pR) * unit_n[i]) * invSRmSs;
+      const double eSR = (sRmuR * ER - pR * unR + pStar * sM) * invSRmSs;
+      flux[0] = rhoSR * sM;
+      for (int i = 0; i < 3; ++i) flux[1 + i] = rhouSR[i] * sM + pStar * unit_n[i];
+      flux[4] = (eSR + pStar) * sM;
+    } else {
+      flux[0] = rhoR * unR;
+      for (int i = 0; i < 3; ++i) flux[1 + i] = rhoR * cv1->u[i] * unR + pR * unit_n[i];
+      flux[4] = (ER + pR) * unR;
+    }
+  }
+  flux[0] *= area;
+  for (int i = 0; i < 3; ++i) flux[1 + i] *= area;
+  flux[4] *= area;
+}
+
+/* the constant boundary state of a CBC zone at face n_bf (CbcUpt / CbcMpt /
+ * CbcRunp / CbcRup::initData, IdealGasSolverBcs.cpp:3182-3300) */
+static void cbc_state(const og_solver* s, int z, const double n_bf[3], og_state* b) {
+  const og_params* P = &s->p;
+  const double R = s->R_gas, gamma = P->gamma;
+  const double p = P->bc_p[z];
+  double T;
+  b->p = p;
+  if (P->bc_type[z] == 4) {
+    T = P->bc_T[z];
+    for (int i = 0; i < 3; ++i) b->u[i] = P->bc_u[z][i];
+    b->sp_vol = R * T / p;
+  } else if (P->bc_type[z] == 5) {
+    T = P->bc_T[z];
+    const double rho = p / T / R;
+    const double un = -P->bc_q[z] / (rho * P->bc_area[z]);
+    const double bf_area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+    for (int i = 0; i < 3; ++i) b->u[i] = un * n_bf[i] / bf_area;
+    b->sp_vol = 1.0 / rho;
+  } else if (P->bc_type[z] == 6) {
+    const double rho = P->bc_rho[z];
+    T = p / rho / R;
+    const double bf_area = sqrt(n_bf[0] * n_bf[0] + n_bf[1] * n_bf[1] + n_bf[2] * n_bf[2]);
+    for (int i = 0; i < 3; ++i) b->u[i] = -P->bc_q[z] * n_bf[i] / bf_area;
+    b->sp_vol = 1.0 / rho;
+  } else {
+    const double rho = P->bc_rho[z];
+    T = p / rho / R;
+    for (int i = 0; i < 3; ++i) b->u[i] = P->bc_u[z][i];
+    b->sp_vol = 1.0 / rho;
+  }
+  b->h = R * T * gamma / (gamma - 1.0);
+}
+
 /* addBoundaryFlux of every zone in zone order (IdealGasSolverBcs.cpp:14-47 SlipWallBc /
  * WallAdiabatic, :1485-1510 WallIsothermal), first in calcRhsBase
  * (IdealGasSolver.cpp:261-262): stage p, u, T; lagged mu_lam, loc_lam */
@@ -889,6 +996,15 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
         visc_work += P->bc_u[z][i] * tauijnj;
       }
       rhs[icv].rhoE -= visc_work;
+    } else if (P->bc_type[z] >= 4) {
+      /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976) */
+      og_state b;
+      cbc_state(s, z, n, &b);
+      double F[5];
+      og_flux_riemann(F, n, &s->cv_light[icv], &b, P->gamma, P->gamma);
+      rhs[icv].rho -= F[0];
+      for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= F[1 + i];
+      rhs[icv].rhoE -= F[4];
     } else {
       const double visc_coeff = (s->mu_lam[icv]) * m->aod_bf[ibf];
       const double k_coeff = (s->loc_lam[icv]) * m->aod_bf[ibf] * cp;

@@ -95,9 +95,11 @@ typedef struct og_params {
   int res_time;       /* index of the RES_TIME scalar, or -1 */
   /* boundary condition of each boundary zone (by zone index): 1 SLIP/SYMMETRY,
    * 2 WALL_ADIABATIC (wall velocity bc_u), 3 WALL_ISOTHERMAL (wall temperature bc_T) */
-  int bc_type[16];
+  int bc_type[16];   /* 4 CBC_UPT, 5 CBC_MPT, 6 CBC_RUNP, 7 CBC_RUP (Riemann flux against a constant state) */
   double bc_T[16];
   double bc_u[16][3];
+  double bc_p[16], bc_rho[16], bc_q[16]; /* CBC p, rho, and mdot (MPT) or un (RUNP) */
+  double bc_area[16];                    /* each zone's area, sum of |n_bf| in face order (CBC_MPT) */
 } og_params;
 
 typedef struct og_diag {
@@ -120,6 +122,10 @@ void og_flux_compact(og_rhs* flux, const og_fc* fa, const og_state* cv0, const o
                      const og_aux* a0, const og_aux* a1, const double u_avg[3], double inv_v_avg,
                      double h_stag, const double g0[9], const double g1[9]);
 double og_fast_pow_posx(double x, double p);
+/* calcRiemannFlux (IdealGasSolverFlux.hpp:218-306): flux[5] through n_bf between
+ * cv0 and cv1 */
+void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,
+                     double gammaL, double gammaR);
 /* calcSvd33 on a column-major A: U, V column-major, sigma unsorted; returns the
  * iteration count */
 int og_svd33(const double A[9], double U[9], double V[9], double sigma[3]);

@@ -35,6 +35,8 @@ class og_params(C.Structure):
         ("dt_data", C.c_double * 2), ("check_interval", C.c_int), ("sigma_coeff", C.c_double),
         ("full_gr", C.c_int), ("delta_r_dsm", C.c_double), ("nsc", C.c_int), ("res_time", C.c_int),
         ("bc_type", C.c_int * 16), ("bc_T", C.c_double * 16), ("bc_u", (C.c_double * 3) * 16),
+        ("bc_p", C.c_double * 16), ("bc_rho", C.c_double * 16), ("bc_q", C.c_double * 16),
+        ("bc_area", C.c_double * 16),
     ]
 
 
@@ -103,6 +105,8 @@ def lib(path=None):
         L.og_set_source_hook.restype = None
         L.og_gr_func.restype = C.c_double
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
+        L.og_flux_riemann.restype = None
+        L.og_flux_riemann.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double]
         _libs[path] = L
     return _libs[path]
 
@@ -121,6 +125,7 @@ def ref():
         R.ref_svd33.restype = C.c_int
         R.ref_svd33.argtypes = [dp, dp, dp, dp, dp]
         R.ref_erk_wgt3.argtypes = [dp]
+        R.ref_flux_riemann.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double]
         _ref = R
     return _ref
 
@@ -147,9 +152,10 @@ def filter_weights(mesh):
     return np.ascontiguousarray(area * dist / 6.0)
 
 
-def params_from(cgp, zones=()) -> og_params:
+def params_from(cgp, zones=(), zone_area=()) -> og_params:
     """The oracle's parameters; `zones` are the mesh's boundary zone names, each
-    given the condition cgp holds under that name."""
+    given the condition cgp holds under that name; `zone_area` each zone's area
+    (its faces' |n_bf| summed in face order)."""
     p = og_params()
     bcs = {cgp.bc[i].zone.decode(): cgp.bc[i] for i in range(cgp.nbc)}
     for z, name in enumerate(zones):
@@ -158,8 +164,10 @@ def params_from(cgp, zones=()) -> og_params:
         p.bc_T[z] = b.T
         for d in range(3):
             p.bc_u[z][d] = b.u[d]
+        p.bc_p[z], p.bc_rho[z], p.bc_q[z] = b.p, b.rho, b.q
+        p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
     for name, _ in og_params._fields_:
-        if name in ("bc_type", "bc_T", "bc_u"):
+        if name.startswith("bc_"):
             continue
         if name == "full_gr":
             p.full_gr = cgp.use_full_gr
@@ -198,7 +206,12 @@ class OracleSolver:
                 self._keep[field] = a
                 setattr(m, field, a.ctypes.data)
         self._m = m
-        self._p = params_from(cgparams, getattr(mesh, "zones", ()))
+        zone_area = ()
+        if m.nbf:
+            # area_global: each zone's face areas summed in face order (bincount adds in index order)
+            zone_area = np.bincount(self._keep["zone_bf"], weights=mesh.get("area_bf"),
+                                    minlength=len(mesh.zones))
+        self._p = params_from(cgparams, getattr(mesh, "zones", ()), zone_area)
         self.ncv = mesh.ncv
         self._h = self._L.og_create(C.byref(self._m), C.byref(self._p))
         self._L.og_set_threads(self._h, int(threads))

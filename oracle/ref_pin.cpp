@@ -4,6 +4,7 @@
 //
 //   calcInternalFluxNew / calcInternalFluxSymmetricNew /
 //   calcInternalCompactFluxNewGrad   src/charles/ideal_gas/IdealGasSolverFlux.hpp:9-215
+//   calcRiemannFlux                  src/charles/ideal_gas/IdealGasSolverFlux.hpp:218-306
 //   calcSvd33                        src/core/common/SvdNew.hpp:24
 //   MiscUtils::fast_pow_posx         src/core/common/MiscUtils.hpp:273-276
 //   RkWeights::erk_wgt3              src/charles/RkWeights.hpp:6-8
@@ -37,6 +38,14 @@ static void rhs_out(const IdealGasRhs& r, double* o) {
 }
 
 extern "C" {
+
+void ref_flux_riemann(double* flux, const double* n_bf, const double* cv0, const double* cv1, double gammaL,
+                      double gammaR) {
+  IdealGasState a = state_of(cv0), b = state_of(cv1);
+  IdealGasRhs f;
+  calcRiemannFlux(f, n_bf, a, b, gammaL, gammaR);
+  rhs_out(f, flux);
+}
 
 // aux_out = u_avg[3], inv_v_avg, h_stag, p_avg
 void ref_flux_skew(double* flux, const double* nc, const double* cv0, const double* cv1, double* aux_out) {

@@ -1,5 +1,5 @@
 """Boundary conditions of walls (SLIP / SYMMETRY, WALL_ADIABATIC, WALL_ISOTHERMAL)
-on channel meshes (periodic in x and z, walls at y = 0 and y = L), CPU side:
+and characteristic conditions (CBC_UPT / MPT / RUNP / RUP) on channel meshes (periodic in x and z, walls at y = 0 and y = L), CPU side:
 the mesh and operator geometry of boundary faces and the oracle's boundary
 terms, pinned by closed forms.
 
@@ -14,6 +14,11 @@ terms, pinned by closed forms.
   wall cell, and gains u_w mu 2h (U - u_w) of energy with TRANSLATE u_w; a
   WALL_ISOTHERMAL wall at T_w takes loc cp 2h (T - T_w) of energy; the CFL of a
   fluid at rest counts the wall faces, 0.5 dt c (sum |n_fa| + |n_bf|) / V.
+* CBC (IdealGasSolverBcs.cpp:2960-2976, 3182-3300): a uniform stream through
+  both walls whose CBC states are that stream keeps a zero RHS for every CBC
+  type (the Riemann flux of equal states is the physical flux, pinned against
+  the reference's calcRiemannFlux in test_oracle_pins.py); a gas at rest
+  facing an inflow state gains the Riemann flux's mass in the wall cells.
 * The parser: zone conditions by name, the reference's other types refused, a
   zone without a condition refused when the solver is built.
 """
@@ -160,3 +165,53 @@ def test_zone_without_condition_is_refused():
     if rc == -3:  # no HIP device here: the check comes first on a GPU
         pytest.skip("no HIP device")
     assert rc == -1 and b"y1" in load_library().cg_error_string()
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_cbc_with_the_interior_state_preserves_a_uniform_stream(jitter):
+    # a uniform stream V through the channel's y walls, each wall a CBC whose
+    # boundary state is that stream: the Riemann flux is then the physical flux,
+    # which the interior fluxes balance (the extended operator closes with the
+    # boundary faces), so the RHS vanishes; one deck per CBC type
+    m = Mesh.channel(10, jitter=jitter, seed=6)
+    R, gamma, T = 71.42857142857143, 1.4, 1.0
+    p = R * 1.0 * T
+    rho = p / (R * T)
+    V = 3.0
+    A = (2 * np.pi) ** 2
+    decks = {
+        "UPT": {"y0": f"CBC_UPT 0 {V} 0 {p!r} {T}", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"},
+        "MPT": {"y0": f"CBC_MPT {rho * V * A!r} {p!r} {T}", "y1": f"CBC_MPT {-rho * V * A!r} {p!r} {T}"},
+        "RUNP": {"y0": f"CBC_RUNP {rho} {V} {p!r}", "y1": f"CBC_RUNP {rho} {-V} {p!r}"},
+        "RUP": {"y0": f"CBC_RUP {rho} 0 {V} 0 {p!r}", "y1": f"CBC_RUP {rho} 0 {V} 0 {p!r} STATIONARY_FRAME"},
+    }
+    for name, walls in decks.items():
+        o = oracle(m, deck(walls), uniform(m, (0.0, V, 0.0), p=p, T=T, R=R, gamma=gamma))
+        r = o.calc_rhs()
+        scale = rho * V * V * H * H + p * H * H
+        assert np.abs(r).max() < 1e-11 * scale, (name, np.abs(r).max())
+
+
+def test_cbc_inflow_closed_form_at_rest():
+    # a gas at rest facing a CBC_UPT state (U, p0, T0) of the same p and T: on the
+    # lattice the wall cell's mass RHS is minus the Riemann flux's mass flux
+    # through its face, computed here from the flux alone
+    from oracle.oracle import lib
+    import ctypes as C
+    m = Mesh.channel(10, jitter=0.0)
+    R, p, T = 71.42857142857143, 71.42857142857143, 1.0
+    U = 2.0
+    o = oracle(m, deck({"y0": f"CBC_UPT 0 {U} 0 {p!r} {T}", "y1": "SLIP"}), uniform(m, p=p, T=T, R=R))
+    r = o.calc_rhs()
+    nb = np.array([0.0, -H * H, 0.0])
+    a = np.array([0.0, 0.0, 0.0, R * T / p, p, R * T * 1.4 / 0.4])
+    b = np.array([0.0, U, 0.0, R * T / p, p, R * T * 1.4 / 0.4])
+    f = np.zeros(5)
+    D = C.POINTER(C.c_double)
+    lib().og_flux_riemann(f.ctypes.data_as(D), nb.ctypes.data_as(D), a.ctypes.data_as(D), b.ctypes.data_as(D),
+                          1.4, 1.4)
+    lower = wall_cells(m, 0)
+    np.testing.assert_allclose(r[lower, 0], -f[0], rtol=1e-12)
+    # gas enters: the mass flux through the outward normal is negative, the
+    # wall cells gain mass
+    assert f[0] < 0.0 and np.all(r[lower, 0] > 0.0)

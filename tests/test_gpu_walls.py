@@ -1,8 +1,10 @@
-"""GPU parity of wall boundary conditions against the oracle (FP64 CPU
+"""GPU parity of boundary conditions against the oracle (FP64 CPU
 restatement; its wall terms are pinned by closed forms in
-test_boundary_conditions.py): SLIP / SYMMETRY, WALL_ADIABATIC (with a
-TRANSLATE wall velocity) and WALL_ISOTHERMAL on jittered channel meshes and on
-the reference's own Stitch mesh orifice.mles (six boundary zones).
+test_boundary_conditions.py, its Riemann flux against the reference's
+calcRiemannFlux in test_oracle_pins.py): SLIP / SYMMETRY, WALL_ADIABATIC (with
+a TRANSLATE wall velocity), WALL_ISOTHERMAL and the characteristic conditions
+CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP on jittered channel meshes and on the
+reference's own Stitch mesh orifice.mles (six boundary zones).
 
 FP64: p, T, sos and dudx bit-identical, the lagged coefficients within 1e-14,
 the stage RHS within 1e-12 of each component's largest magnitude, the CFL
@@ -22,13 +24,20 @@ from oracle.oracle import OracleSolver
 
 pytestmark = pytest.mark.gpu
 
-ORIFICE = "/root/reference/src/core/common/tests/assets/orifice.mles"
+# the geometry of the reference's own test mesh orifice.mles (written by
+# tests/golden/make_orifice_fixture.py)
+ORIFICE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "orifice_geometry.mles")
 RHS_TOL = 1e-12
 HIST_TOL = 1e-10
 
+P0 = 1.0 / (1.4 * 0.1 ** 2)  # the channel state's mean pressure (Ma 0.1, rho 1, T 1)
 WALLS = {
     "slip+translate": {"y0": "SLIP", "y1": "WALL_ADIABATIC TRANSLATE 0.3 0 0.1"},
     "isothermal+adiabatic": {"y0": "WALL_ISOTHERMAL 1.05", "y1": "WALL_ADIABATIC"},
+    # blowing through y0 and out through y1
+    "cbc_upt+runp": {"y0": f"CBC_UPT 0.1 0.2 0 {P0!r} 1.02", "y1": f"CBC_RUNP 1.0 -0.2 {P0 * 0.999!r}"},
+    "cbc_mpt+rup": {"y0": f"CBC_MPT {0.15 * (2 * np.pi) ** 2!r} {P0!r} 0.98",
+                    "y1": f"CBC_RUP 0.99 0.05 0.15 0 {P0 * 1.001!r} STATIONARY_FRAME"},
 }
 
 
@@ -112,7 +121,6 @@ def test_lattice_channel_walls_match_oracle():
     check_history(g, o, 20, "lattice channel")
 
 
-@pytest.mark.skipif(not os.path.exists(ORIFICE), reason="reference mesh fixture not present")
 def test_orifice_mesh_with_walls_matches_oracle():
     # the reference's own Stitch mesh (an orifice plate in a pipe), every zone a
     # wall: the pipe and plate no-slip, inlet and outlet closed by an isothermal
@@ -137,9 +145,33 @@ def test_orifice_mesh_with_walls_matches_oracle():
     check_history(g, o, 10, "orifice")
 
 
-def test_mixed_precision_channel_close_to_oracle():
+def test_orifice_mesh_with_inflow_and_outflow_matches_oracle():
+    # the orifice with a characteristic inlet (mass flow, p, T) and outlet (an
+    # ambient state), the pipe and plate no-slip, a gas at rest
+    m = Mesh.read_mles(ORIFICE)
+    gamma, p0 = 1.4, 1.0 / 1.4
+    zones = dict(zip(m.zones, range(len(m.zones))))
+    area = np.bincount(m.get("zone_bf"), weights=m.get("area_bf"), minlength=len(m.zones))
+    mdot = float(0.1 * area[zones["inlet"]])  # rho 1, 0.1 c
+    walls = {"inlet": f"CBC_MPT {mdot!r} {p0!r} 1.0", "outlet": f"CBC_RUP 1.0 0 0 0 {p0!r}",
+             "oriface-pipe": "WALL_ADIABATIC", "wall": "WALL_ADIABATIC", "oriface-wall-0": "WALL_ADIABATIC",
+             "oriface-wall-1": "WALL_ADIABATIC"}
+    rho = np.ones(m.ncv)
+    u = np.zeros((m.ncv, 3))
+    rhoE = np.full(m.ncv, p0 / (gamma - 1.0))
+    text = (f"GAMMA 1.4\nP_REF {p0!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF 1e-5\nSGS_MODEL VREMAN\nTIMESTEP CFL 0.8\n" +
+            "".join(f"{z} {w}\n" for z, w in walls.items()))
+    g, o = pair(m, text, (rho, u, rhoE))
+    check_state_and_rhs(g, o, "orifice cbc")
+    check_history(g, o, 10, "orifice cbc")
+    # the inlet pushes mass in: the mass grows
+    assert o.diag()["sums"][0] > rho @ m.get("vol")
+
+
+@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp"])
+def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
-    walls = {"y0": "WALL_ISOTHERMAL 1.05", "y1": "WALL_ADIABATIC TRANSLATE 0.3 0 0"}
+    walls = WALLS[walls]
     st = channel_state(m)
     g = IdealGasSolver(m, parse_params(deck(walls, prec="MIXED")))
     g.set_state(*st)

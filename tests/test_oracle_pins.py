@@ -205,3 +205,67 @@ def test_oracle_svd33_bitwise_equal_to_reference():
         itr = R.ref_svd33(dp(Acm), dp(Ur), dp(Vr), dp(sr), dp(err))
         assert it == itr
         assert np.array_equal(s, sr) and np.array_equal(U, Ur) and np.array_equal(V, Vr), A
+
+
+def _riemann_states(rng, n):
+    """Interior / boundary state pairs spanning every branch of the HLLC flux:
+    subsonic both ways (contact left or right of the face) and supersonic
+    inflow and outflow (normal Mach up to 3)."""
+    out = []
+    for k in range(n):
+        nb = rng.normal(size=3) * rng.uniform(0.1, 2.0)
+        un = nb / np.linalg.norm(nb)
+        a, b = np.empty(6), np.empty(6)
+        for s in (a, b):
+            s[3] = rng.uniform(0.5, 2.0)          # sp_vol
+            s[4] = rng.uniform(0.5, 2.0)          # p
+            T = s[4] * s[3]                       # R = 1
+            s[5] = 1.4 / 0.4 * T                  # h = gamma R T / (gamma - 1)
+            c = math.sqrt(1.4 * s[4] * s[3])
+            mach = rng.uniform(-3.0, 3.0) if k % 2 else rng.uniform(-0.8, 0.8)
+            s[0:3] = rng.normal(scale=0.3 * c, size=3) + mach * c * un
+        out.append((nb, a, b))
+    return out
+
+
+@needs_ref
+def test_riemann_flux_bitwise_equal_to_reference():
+    rng = np.random.default_rng(6)
+    L, R = orc.lib(), orc.ref()
+    for nb, a, b in _riemann_states(rng, 3000):
+        f_ref, f = np.zeros(5), np.zeros(5)
+        R.ref_flux_riemann(dp(f_ref), dp(nb), dp(a), dp(b), 1.4, 1.4)
+        L.og_flux_riemann(dp(f), dp(nb), dp(a), dp(b), 1.4, 1.4)
+        assert np.array_equal(f, f_ref)
+
+
+def test_riemann_flux_closed_forms():
+    # equal states: the physical flux (rho u.n, rho u u.n + p n, (rhoE + p) u.n) |n|;
+    # supersonic outflow takes the inside state alone, supersonic inflow the outside one
+    rng = np.random.default_rng(7)
+    L = orc.lib()
+
+    def phys(s, nb):
+        A = np.linalg.norm(nb)
+        n = nb / A
+        rho, u, p, h = 1.0 / s[3], s[0:3], s[4], s[5]
+        un = u @ n
+        E = rho * (h + 0.5 * u @ u) - p
+        return np.concatenate([[rho * un], rho * u * un + p * n, [(E + p) * un]]) * A
+
+    for nb, a, b in _riemann_states(rng, 400):
+        f = np.zeros(5)
+        L.og_flux_riemann(dp(f), dp(nb), dp(a), dp(a), 1.4, 1.4)
+        np.testing.assert_allclose(f, phys(a, nb), rtol=1e-12, atol=1e-12 * np.abs(phys(a, nb)).max())
+        n = nb / np.linalg.norm(nb)
+        ca = math.sqrt(1.4 * a[4] * a[3])
+        cb = math.sqrt(1.4 * b[4] * b[3])
+        a2, b2 = a.copy(), b.copy()
+        a2[0:3] += (4 * max(ca, cb) - a2[0:3] @ n) * n    # both leave through the face at Mach > 2
+        b2[0:3] += (4 * max(ca, cb) - b2[0:3] @ n) * n
+        L.og_flux_riemann(dp(f), dp(nb), dp(a2), dp(b2), 1.4, 1.4)
+        np.testing.assert_allclose(f, phys(a2, nb), rtol=1e-12, atol=1e-12 * np.abs(phys(a2, nb)).max())
+        a2[0:3] -= 8 * max(ca, cb) * n                    # both enter
+        b2[0:3] -= 8 * max(ca, cb) * n
+        L.og_flux_riemann(dp(f), dp(nb), dp(a2), dp(b2), 1.4, 1.4)
+        np.testing.assert_allclose(f, phys(b2, nb), rtol=1e-12, atol=1e-12 * np.abs(phys(b2, nb)).max())
