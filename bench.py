@@ -191,14 +191,21 @@ def kernel_rooflines(ncv, Ge, Gc, Gg, kms, mixed):
     return b_step, out
 
 
-def recorded_traffic(key, precision, n, ic):
-    """HBM bytes per launch recorded from rocprofv3 passes of this workload."""
+def traffic_key(key, precision, n, ic, sgs, nsc):
+    """The workload a recorded traffic figure belongs to: kernel role,
+    precision, lattice side, initial field, SGS model and scalar count."""
+    return f"{key}/{precision}/n{n}/{ic}/{sgs.lower()}/sc{nsc}"
+
+
+def recorded_traffic(key, precision, n, ic, sgs="VREMAN", nsc=0):
+    """HBM bytes per launch recorded from rocprofv3 passes of exactly this
+    workload (None when no pass covered it)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
-    return rec.get(f"{key}/{precision}/n{n}/{ic}", {}).get("bytes_per_launch")
+    return rec.get(traffic_key(key, precision, n, ic, sgs, nsc), {}).get("bytes_per_launch")
 
 
 def host_cores():
@@ -325,26 +332,30 @@ def accuracy(args):
 
 def ensemble_leg(count, steps, seed, tune_steps=2, sample_count=16):
     """Batched ensemble: `count` solves of uneven size (n ~ U{16..48}, jitter
-    0.3h, Re, Ma, TGV or k^4 as the ensemble draws them) in one batch (one fused
-    launch per kernel, graph replay, autotuned variant), FP64 and mixed, against
-    the same solves run one at a time in FP64 (a uniform sample, scaled). All
-    solves are resident and advance together, so each completes at the batch's
-    makespan: the per-solve completion times are recorded and their p50 / p99
-    reported."""
+    0.3h, Re, Ma, TGV or k^4 as the ensemble draws them) and uneven horizon
+    (U{steps/2..steps} steps) in one batch (one fused launch per kernel, graph
+    replay, autotuned variant), FP64 and mixed. All solves start together; the
+    batch is stepped to the next horizon, where the solves that reach it are
+    read back and retired, so each solve's latency is its own completion time
+    (p50 / p99 over the solves). Against the same solves run one at a time in
+    FP64 (a uniform sample, scaled)."""
     from nextgen_amd.batch import Batch
     from nextgen_amd.ensemble import ensemble_specs, run_alone
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
     import numpy as np
 
     specs = ensemble_specs(count, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    horizons = [int(h) for h in rng.integers(max(1, steps // 2), steps + 1, size=len(specs))]
     t0 = time.time()
     meshes = [sp.mesh() for sp in specs]
     cells = sum(m.ncv for m in meshes)
+    cell_steps = sum(m.ncv * h for m, h in zip(meshes, horizons))
     log(f"ensemble: {len(specs)} meshes, {cells} cells, built in {time.time() - t0:.1f}s")
     states = [sp.state(m) for sp, m in zip(specs, meshes)]
-    out = {"instances": len(specs), "cells": cells, "steps": steps,
+    out = {"instances": len(specs), "cells": cells, "steps": [min(horizons), max(horizons)],
            "spec": f"n ~ U{{16..48}} (jitter 0.3h), Re ~ U[400,3200], Ma ~ U[0.05,0.3], TGV or k^4 by coin flip, "
-                   f"seed {seed}"}
+                   f"horizon ~ U{{{max(1, steps // 2)}..{steps}}} steps, seed {seed}"}
     kes = {}
     for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
         t0 = time.time()
@@ -359,25 +370,38 @@ def ensemble_leg(count, steps, seed, tune_steps=2, sample_count=16):
         report = b.tune(tune_steps)
         chosen = [ln for ln in report.splitlines() if ln.startswith("chosen")]
         log(f"{name}: tuner {chosen[0] if chosen else report}")
-        ms = b.time_steps(steps)
-        done = [ms * 1e-3 for _ in ids]  # lockstep: every solve's last step ends with the batch's
-        kes[name] = [b.diag(i) for i in ids]
+        left = dict(zip(ids, horizons))
+        done, kes[name] = {}, {}
+        tstart = time.perf_counter()
+        while left:
+            k = min(left.values())
+            b.step(k)
+            t = time.perf_counter() - tstart
+            for iid in [i for i in left if left[i] == k]:
+                kes[name][iid] = b.diag(iid)
+                done[iid] = t
+                b.remove(iid)
+                del left[iid]
+            for iid in left:
+                left[iid] -= k
+        sec = time.perf_counter() - tstart
         b.close()
-        sec = ms * 1e-3
+        lat = [done[i] for i in ids]
         out[name] = {"seconds": sec, "solves_per_s": len(specs) / sec,
-                     "cell_stage_per_s": 3.0 * cells * steps / sec,
-                     "latency_p50_s": float(np.percentile(done, 50)), "latency_p99_s": float(np.percentile(done, 99)),
+                     "cell_stage_per_s": 3.0 * cell_steps / sec,
+                     "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
                      "setup_s": setup, "device_bytes": cnt["device_bytes"],
                      "tuner": chosen[0] if chosen else report}
-        log(f"{name}: {steps} steps of all instances in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s")
-    errs = [abs(a.ke - b.ke) / b.ke for a, b in zip(kes["mixed"], kes["fp64"])]
+        log(f"{name}: all solves to their horizons in {sec:.2f}s -> {len(specs) / sec:.2f} solves/s, "
+            f"latency p50 {out[name]['latency_p50_s']:.3f}s p99 {out[name]['latency_p99_s']:.3f}s")
+    errs = [abs(kes["mixed"][i].ke - kes["fp64"][i].ke) / kes["fp64"][i].ke for i in kes["fp64"]]
     out["mixed"]["ke_rel_err_vs_fp64_max"] = max(errs)
     out["tune_steps_per_variant"] = tune_steps
     # one at a time, FP64 (a uniform sample, scaled to the whole set)
     sample = list(range(0, len(specs), max(1, len(specs) // sample_count)))
     t_alone = 0.0
     for i in sample:
-        sec, _ = run_alone(specs[i], CG_PREC_FP64, steps)
+        sec, _ = run_alone(specs[i], CG_PREC_FP64, horizons[i])
         t_alone += sec
     seq = t_alone * len(specs) / len(sample)
     out["sequential_fp64"] = {"seconds_estimate": seq, "solves_per_s": len(specs) / seq,
@@ -393,12 +417,14 @@ def ensemble_bench(args):
     return 0
 
 
-def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True):
+def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True, rate=None):
     """Serving loop (continuous batching, mixed precision) against the same
     requests one at a time (FP64, and in the server's own precision), over
     arrival rates given as multiples of the sequential FP64 service rate. Each
     request has its own mesh (built before the clock); its setup (device
-    layout, state, initComplete) counts in both executions."""
+    layout, state, initComplete) counts in both executions. With `rate`
+    (requests/s) only the server runs, at that rate (for a kernel trace of the
+    serving loop alone)."""
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
     from nextgen_amd.serve import MeshPool, Server, requests_poisson, sequential, summarize
 
@@ -408,6 +434,12 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True):
     pool.build([r.spec for r in base])
     cells = [r.spec.n ** 3 for r in base]
     log(f"serving: {len(base)} request meshes ({min(cells)}..{max(cells)} cells) built in {time.time() - t0:.1f}s")
+    if rate is not None:
+        reqs = [type(r)(r.rid, r.arrival / rate, r.spec) for r in base]
+        srv = Server(pool, CG_PREC_MIXED, hbm_budget=hbm_budget, workers=workers)
+        res, info = srv.run(reqs)
+        return {"requests": len(base), "cells": cells, "rate_per_s": rate,
+                "served_mixed": summarize(res, info["makespan"]), **info}
     seq = {}
     for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
         if name == "mixed" and not seq_mixed:
@@ -439,8 +471,8 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True):
         errs = [abs(r.ke - ke64[r.rid]) / ke64[r.rid] for r in res]
         row = {"load": load, "rate_per_s": rate, "served_mixed": summarize(res, info["makespan"]),
                "sequential_fp64": summarize(queue(reqs, service["fp64"])),
-               "ke_rel_err_vs_fp64_max": max(errs), "batch_steps": info["batch_steps"],
-               "peak_device_bytes": srv.peak_bytes}
+               "ke_rel_err_vs_fp64_max": max(errs), "peak_device_bytes": srv.peak_bytes,
+               **{k: v for k, v in info.items() if k != "makespan"}}
         if "mixed" in service:
             row["sequential_mixed"] = summarize(queue(reqs, service["mixed"]))
             row["solves_per_s_ratio_vs_sequential_mixed"] = (row["served_mixed"]["solves_per_s"] /
@@ -454,7 +486,8 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True):
 
 
 def serve_bench(args):
-    out = {"mode": "serving", **serve_leg(args.serve, args.loads, args.seed, args.hbm_budget * 1e9)}
+    out = {"mode": "serving", **serve_leg(args.serve, args.loads, args.seed, args.hbm_budget * 1e9,
+                                          rate=args.serve_rate)}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -480,6 +513,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0, help="serving loop over this many requests")
     ap.add_argument("--loads", type=float, nargs="+", default=[0.5, 1.0, 2.0],
                     help="arrival rates as multiples of the sequential FP64 service rate")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="run only the server, at this many requests/s (no sequential baselines)")
     ap.add_argument("--hbm-budget", type=float, default=96.0, help="GB of device memory for admitted solves")
     ap.add_argument("--no-extras", action="store_true", help="skip the bounded ensemble and serving legs")
     ap.add_argument("--extra-ensemble", type=int, default=48, help="solves of the default run's ensemble leg")
@@ -520,7 +555,7 @@ def main():
 
     dom = max(r["kr"], key=lambda k: r["kr"][k]["ms_per_step"])
     kd = r["kr"][dom]
-    traffic = recorded_traffic(dom, main_prec, args.n, args.ic)
+    traffic = recorded_traffic(dom, main_prec, args.n, args.ic, args.sgs, len(args.scalars))
     copy_gbs = copy_bandwidth() / 1e9
     log(f"device copy: {copy_gbs:.0f} GB/s")
     out = {

@@ -168,6 +168,10 @@ class Server:
         now = lambda: time.perf_counter() - t0
         steps_run = 0
         cell_steps = 0
+        # host-side time accounting of the loop: inside batch steps (the GPU is
+        # busy for about this long), attaching, retiring, waiting with nothing
+        # resident
+        t_step = t_attach = t_retire = t_idle = 0.0
         while reqs or waiting or preparing or active:
             t = now()
             while reqs and reqs[0].arrival <= t:
@@ -183,6 +187,7 @@ class Server:
                 self.peak_bytes = max(self.peak_bytes, used)
                 preparing[pool.submit(self.prepare, r)] = (r, need)
             # attach what is ready
+            ta = time.perf_counter()
             for fut in [f for f in preparing if f.done()]:
                 r, need = preparing.pop(fut)
                 inst = fut.result()
@@ -199,17 +204,23 @@ class Server:
                 self.bpc = max(self.bpc, hold / r.spec.n ** 3)
                 iid = batch.attach(inst)
                 active[iid] = [r, r.spec.steps, now(), hold, r.spec.n ** 3]
+            t_attach += time.perf_counter() - ta
             if not active:
+                ti = time.perf_counter()
                 nxt = [f for f in preparing]
                 if nxt:
                     cf.wait(nxt, return_when=cf.FIRST_COMPLETED)
                 elif reqs:
                     time.sleep(max(0.0, min(reqs[0].arrival - now(), 0.05)))
+                t_idle += time.perf_counter() - ti
                 continue
             k = min(self.max_chunk, min(a[1] for a in active.values()))
+            ts = time.perf_counter()
             batch.step(k)
+            t_step += time.perf_counter() - ts
             steps_run += k
             cell_steps += k * sum(a[4] for a in active.values())
+            tr = time.perf_counter()
             for iid in list(active):
                 a = active[iid]
                 a[1] -= k
@@ -220,10 +231,13 @@ class Server:
                     batch.remove(iid)
                     used -= a[3]
                     del active[iid]
+            t_retire += time.perf_counter() - tr
         makespan = now()
         pool.shutdown()
         batch.close()
-        return results, {"makespan": makespan, "batch_steps": steps_run, "cell_steps": cell_steps}
+        return results, {"makespan": makespan, "batch_steps": steps_run, "cell_steps": cell_steps,
+                         "host_seconds": {"stepping": t_step, "attaching": t_attach, "retiring": t_retire,
+                                          "idle_nothing_resident": t_idle}}
 
 
 def sequential(pool: MeshPool, requests, precision=CG_PREC_FP64, layouts=None):
