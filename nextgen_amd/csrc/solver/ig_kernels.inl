@@ -212,6 +212,40 @@ __device__ __forceinline__ void flux_symmetric(R cF[5], const R c[3], const R u_
   cF[4] = h_stag * dFrho + Frho * dH;
 }
 
+// x with its sign flipped when neg (exact: one bit operation)
+__device__ __forceinline__ double flip(double x, bool neg) {
+  return __longlong_as_double(__double_as_longlong(x) ^ (long long)((unsigned long long)neg << 63));
+}
+__device__ __forceinline__ float flip(float x, bool neg) {
+  return __int_as_float(__float_as_int(x) ^ (int)((unsigned)neg << 31));
+}
+
+// One extended half-face's pair flux (skew + symmetric, in the pair's lo -> hi
+// orientation) added to the cell's accumulation: -F on the lo side, +F on the
+// hi side. Evaluated from the cell's own side, (self, other), with no select
+// of the two records: every operation of flux_skew is symmetric in its two
+// states (sums, products, averages), so its result is the same bits for
+// (lo, hi) and (hi, lo); every one of flux_symmetric is antisymmetric
+// (differences, and products and sums of them with symmetric terms), so its
+// (hi, lo) result is exactly the negation of its (lo, hi) one, and
+// round-to-nearest is symmetric under negation. Hence
+//   F(lo, hi) = skew(self, other) + (self is hi ? -1 : 1) symmetric(self, other)
+// bit for bit, and acc - F == acc + (-F).
+template <class R>
+__device__ __forceinline__ void add_half_face(R acc[5], const R n[3], const R cv[3], const Light<R>& self,
+                                              const Light<R>& other, bool self_hi) {
+  R F[5], u_avg[3], inv_v_avg, h_stag;
+  flux_skew(F, n, self, other, u_avg, inv_v_avg, h_stag);
+  if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
+    R cF[5];
+    flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, self, other);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) F[q] += flip(cF[q], self_hi);
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) acc[q] += flip(F[q], !self_hi);
+}
+
 // calcInternalCompactFluxNewGrad, IdealGasSolverFlux.hpp:93-215. In mixed
 // precision p is gauged, so the p v terms add the gauge back:
 //   p1 v1 - p0 v0 = (p1' v1 - p0' v0) + p_g (v1 - v0), and likewise the average.
@@ -808,25 +842,9 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
     const int nb = code & 0x7fffffff;
     const bool self_hi = code < 0;
     const Light<R> other = load_light(in.light, nb);
-    const Light<R>& a = self_hi ? other : self;
-    const Light<R>& b = self_hi ? self : other;
     const R n[3] = {ldnt(g.xsn0 + slot), ldnt(g.xsn1 + slot), ldnt(g.xsn2 + slot)};
     const R cv[3] = {ldnt(g.xsc0 + slot), ldnt(g.xsc1 + slot), ldnt(g.xsc2 + slot)};
-    R F[5], u_avg[3], inv_v_avg, h_stag;
-    flux_skew(F, n, a, b, u_avg, inv_v_avg, h_stag);
-    if (cv[0] != R(0) || cv[1] != R(0) || cv[2] != R(0)) {
-      R cF[5];
-      flux_symmetric(cF, cv, u_avg, inv_v_avg, h_stag, a, b);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) F[q] += cF[q];
-    }
-    if (self_hi) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] += F[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) acc[q] -= F[q];
-    }
+    add_half_face(acc, n, cv, self, other, self_hi);
   }
 
   if (MODE == 0) {
@@ -897,23 +915,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
       Light<R> other;
       other.u[0] = r0.x; other.u[1] = r0.y; other.u[2] = r1.x; other.spv = r1.y; other.p = r2.x; other.h = r2.y;
       other.rho = R(0); other.sos = R(0);
-      const Light<R>& a = self_hi ? other : self;
-      const Light<R>& b = self_hi ? self : other;
-      R F[5], u_avg[3], inv_v_avg, h_stag;
-      flux_skew(F, n[j], a, b, u_avg, inv_v_avg, h_stag);
-      if (cv[j][0] != R(0) || cv[j][1] != R(0) || cv[j][2] != R(0)) {
-        R cF[5];
-        flux_symmetric(cF, cv[j], u_avg, inv_v_avg, h_stag, a, b);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) F[q] += cF[q];
-      }
-      if (self_hi) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) acc[q] += F[q];
-      } else {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) acc[q] -= F[q];
-      }
+      add_half_face(acc, n[j], cv[j], self, other, self_hi);
     }
   }
   if (MODE == 0) {
@@ -1081,14 +1083,13 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
         for (int isc = 0; isc < kMaxScalars; ++isc)
           if (isc < nsc) phin[isc] = phi[(size_t)isc * ncv + nb];
       }
-      const Light<R>& a = self_hi ? other : self;
-      const Light<R>& b = self_hi ? self : other;
+      // the pair's mass flux (lo -> hi) from the cell's own side, as add_half_face
       R F[5], u_avg[3], inv_v_avg, h_stag;
-      flux_skew(F, gn[j], a, b, u_avg, inv_v_avg, h_stag);
+      flux_skew(F, gn[j], self, other, u_avg, inv_v_avg, h_stag);
       if (gcv[j][0] != R(0) || gcv[j][1] != R(0) || gcv[j][2] != R(0)) {
         R cF[5];
-        flux_symmetric(cF, gcv[j], u_avg, inv_v_avg, h_stag, a, b);
-        F[0] += cF[0];
+        flux_symmetric(cF, gcv[j], u_avg, inv_v_avg, h_stag, self, other);
+        F[0] += flip(cF[0], self_hi);
       }
       add(double(F[0]), 0.0, false, self_hi);
     }
