@@ -421,8 +421,9 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True, ra
     """Serving loop (continuous batching, mixed precision) against the same
     requests one at a time (FP64, and in the server's own precision), over
     arrival rates given as multiples of the sequential FP64 service rate. Each
-    request has its own mesh (built before the clock); its setup (device
-    layout, state, initComplete) counts in both executions. With `rate`
+    request has its own mesh and initial field (built before the clock, as a
+    request's files would be read); its setup (device layout, state upload,
+    initComplete) counts in both executions. With `rate`
     (requests/s) only the server runs, at that rate (for a kernel trace of the
     serving loop alone)."""
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
@@ -433,13 +434,17 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True, ra
     t0 = time.time()
     pool.build([r.spec for r in base])
     cells = [r.spec.n ** 3 for r in base]
-    log(f"serving: {len(base)} request meshes ({min(cells)}..{max(cells)} cells) built in {time.time() - t0:.1f}s")
+    log(f"serving: {len(base)} request meshes and fields ({min(cells)}..{max(cells)} cells) built in "
+        f"{time.time() - t0:.1f}s")
     if rate is not None:
         reqs = [type(r)(r.rid, r.arrival / rate, r.spec) for r in base]
         srv = Server(pool, CG_PREC_MIXED, hbm_budget=hbm_budget, workers=workers)
         res, info = srv.run(reqs)
         return {"requests": len(base), "cells": cells, "rate_per_s": rate,
-                "served_mixed": summarize(res, info["makespan"]), **info}
+                "served_mixed": summarize(res, info["makespan"]), **info,
+                "timeline": [{"rid": r.rid, "cells": r.cells, "steps": r.steps, "arrival": r.arrival,
+                              "attached": r.start, "done": r.done, "prep_s": srv.prep_seconds.get(r.rid)}
+                             for r in sorted(res, key=lambda q: q.rid)]}
     seq = {}
     for prec, name in ((CG_PREC_FP64, "fp64"), (CG_PREC_MIXED, "mixed")):
         if name == "mixed" and not seq_mixed:
@@ -451,7 +456,7 @@ def serve_leg(count, loads, seed, hbm_budget=96e9, workers=3, seq_mixed=True, ra
     mean_service = sum(service["fp64"].values()) / len(base)
     ke64 = {r.rid: r.ke for r in seq["fp64"]}
     out = {"requests": len(base), "cells": cells, "horizons": [r.spec.steps for r in base],
-           "spec": "cells log-uniform [2e4, 2e6] (own mesh per request), horizon U{200..2000} steps, "
+           "spec": "cells log-uniform [2e4, 2e6] (own mesh and initial field per request), horizon U{200..2000} steps, "
                    "Re U[400,3200], Ma U[0.05,0.3], TGV or k^4; Poisson arrivals; setup counted in both",
            "sequential_fp64_mean_service_s": mean_service, "workers": workers, "sweep": []}
 
@@ -520,7 +525,8 @@ def main():
     ap.add_argument("--extra-ensemble", type=int, default=48, help="solves of the default run's ensemble leg")
     ap.add_argument("--extra-ens-steps", type=int, default=100)
     ap.add_argument("--extra-serve", type=int, default=10, help="requests of the default run's serving leg")
-    ap.add_argument("--extra-load", type=float, default=4.0)
+    ap.add_argument("--extra-load", type=float, nargs="+", default=[4.0, 8.0],
+                    help="arrival rates of the default run's serving leg, as multiples of the sequential FP64 rate")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -615,7 +621,7 @@ def main():
         # bounded forms of the ensemble and serving workloads (the metric's
         # second half); full runs: --ensemble 256, --serve N
         out["ensemble"] = ensemble_leg(args.extra_ensemble, args.extra_ens_steps, args.seed, sample_count=8)
-        out["serving"] = serve_leg(args.extra_serve, [args.extra_load], args.seed, args.hbm_budget * 1e9,
+        out["serving"] = serve_leg(args.extra_serve, args.extra_load, args.seed, args.hbm_budget * 1e9,
                                    seq_mixed=True)
     print(json.dumps(out), flush=True)
     return 0

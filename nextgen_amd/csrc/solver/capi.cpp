@@ -473,6 +473,16 @@ void lane_order(int ncv, const std::vector<int64_t>& key, bool sorted, std::vect
 // SELL-64 half-face regions, LDS halos, gradient stencil, block maps.
 template <class R>
 int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
+  // CG_VERBOSE: the time of each phase on stderr
+  const bool verbose = std::getenv("CG_VERBOSE") != nullptr;
+  auto t_last = std::chrono::steady_clock::now();
+  auto tick = [&](const char* what) {
+    if (!verbose) return;
+    const auto t = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "build_layout(%d cells): %-24s %.3f s\n", m->op.ncv, what,
+                 std::chrono::duration<double>(t - t_last).count());
+    t_last = t;
+  };
   Alloc* h = &Lay->mem;
   const cgmesh::Operators& o = m->op;
   const int ncv = o.ncv, nfa = o.nfa;
@@ -579,6 +589,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   }
   if (rc) return rc;
 
+  tick("per-cell arrays");
   // SELL lane orders (CG_LANE_ORDER=plain: lane = cell, for comparison)
   const char* lane_env = std::getenv("CG_LANE_ORDER");
   const bool sort_lanes = !(lane_env && std::string(lane_env) == "plain");
@@ -662,6 +673,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
         }
       }
     });
+  tick("SELL slot codes");
     // compact pairs (pair index = face index): per-pair SoA for the thread-per-pair kernel
     {
       std::vector<int> plo(nfa), phi(nfa);
@@ -686,6 +698,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
                             &g.pu0, &g.pu1, &g.pu2, &g.parea, &g.paod};
       for (int q = 0; q < 11; ++q) rc |= upload(s, h, pdst[q], p11[q]);
     }
+  tick("compact pair arrays");
     // per-block halos of the extended half-faces for LDS staging
     {
       const int nblk = (ncv + cgdev::kBlock - 1) / cgdev::kBlock;
@@ -754,6 +767,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     rc |= upload(s, h, &g.xf_deg, xcnt);
     rc |= upload(s, h, &g.xf_nbr, xnbr);
     if (rc) return rc;
+  tick("halos, slot uploads");
     // per-slot geometry, gathered on the device from the per-pair arrays
     // (uploaded once, freed below): extended n, c in R; compact n_fa in R and
     // the filter weight
@@ -788,6 +802,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
       g.cfw = cw;
     }
   }
+  tick("slot geometry (device)");
   // gradient stencil without the diagonal
   {
     std::vector<int64_t> key(ncv);
@@ -827,6 +842,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     rc |= upload(s, h, &g.gc2, c2);
     if (rc) return rc;
   }
+  tick("gradient stencil");
   // XCD-ordered block maps for launches over one instance
   rc |= dev_alloc(s, h, &Lay->solo_bmap, (size_t)Lay->nblocks);
   rc |= dev_alloc(s, h, &Lay->solo_pbmap, (size_t)std::max(Lay->npblocks, 1));

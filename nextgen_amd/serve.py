@@ -12,9 +12,9 @@ Ma, horizon in steps) arrive over time. The server
     kernel launch covers all resident solves (cg_batch_step);
   * retires a solve at its horizon (its result read back, its memory freed)
     and records its time to solution = completion - arrival.
-Each request brings its own mesh (built before the clock, as a request's mesh
-file would be read); its device layout is built with the request and freed
-when it retires. Layouts given to the server up front (`layouts`) stay
+Each request brings its own mesh and initial field (built before the clock,
+as a request's mesh and data files would be read); its device layout is built
+with the request and freed when it retires. Layouts given to the server up front (`layouts`) stay
 resident and are shared by every request on that mesh (cg_layout_create /
 cg_instance_create_on).
 A solve's result does not depend on what else is in the batch (the batch
@@ -92,10 +92,13 @@ def requests_poisson(rate, count, sizes=None, seed=7, steps_lo=200, steps_hi=200
 
 
 class MeshPool:
-    """The requests' meshes by (lattice side, seed), each built once."""
+    """The requests' data: meshes by (lattice side, seed), each built once, and
+    initial fields by request spec (a request arrives with both, as a mesh file
+    and a data file would be read before it is submitted)."""
 
     def __init__(self):
         self.meshes = {}
+        self.states = {}
 
     def get(self, sp: Spec):
         key = (sp.n, sp.mesh_seed)
@@ -103,9 +106,15 @@ class MeshPool:
             self.meshes[key] = sp.mesh()
         return self.meshes[key]
 
+    def state(self, sp: Spec):
+        key = (sp.n, sp.mesh_seed, sp.ic, sp.Ma, sp.ic_seed)
+        if key not in self.states:
+            self.states[key] = sp.state(self.get(sp))
+        return self.states[key]
+
     def build(self, specs, threads=4):
-        """Build the meshes of `specs` (several at a time: the builder releases
-        the interpreter while it works)."""
+        """Build the meshes and initial fields of `specs` (several at a time: the
+        builders release the interpreter while they work)."""
         todo = {}
         for sp in specs:
             todo.setdefault((sp.n, sp.mesh_seed), sp)
@@ -113,6 +122,7 @@ class MeshPool:
         with cf.ThreadPoolExecutor(max_workers=threads) as ex:
             for key, m in zip(todo, ex.map(lambda sp: sp.mesh(), todo.values())):
                 self.meshes[key] = m
+            list(ex.map(self.state, specs))
 
 
 def build_layouts(pool: MeshPool, precision):
@@ -143,14 +153,22 @@ class Server:
         self.prepare = prepare or self._prepare
         self.layouts = layouts if layouts is not None else {}
         self.peak_bytes = 0.0
+        self.prep_seconds = {}  # request id -> seconds its preparation took
+
+    def _timed_prepare(self, req: Request):
+        t0 = time.perf_counter()
+        inst = self.prepare(req)
+        self.prep_seconds[req.rid] = time.perf_counter() - t0
+        return inst
 
     def _prepare(self, req: Request):
         m = self.pool.get(req.spec)
         key = (req.spec.n, req.spec.mesh_seed)
         deck = req.spec.deck(self.precision, check_interval=100)
+        st = self.pool.state(req.spec)
         if key in self.layouts:
-            return Instance(m, deck, *req.spec.state(m), layout=self.layouts[key])
-        inst = Instance(m, deck, *req.spec.state(m))  # its own layout, freed with it
+            return Instance(m, deck, *st, layout=self.layouts[key])
+        inst = Instance(m, deck, *st)  # its own layout, freed with it
         inst.private_layout = True
         return inst
 
@@ -185,7 +203,7 @@ class Server:
                 waiting.popleft()
                 used += need
                 self.peak_bytes = max(self.peak_bytes, used)
-                preparing[pool.submit(self.prepare, r)] = (r, need)
+                preparing[pool.submit(self._timed_prepare, r)] = (r, need)
             # attach what is ready
             ta = time.perf_counter()
             for fut in [f for f in preparing if f.done()]:
@@ -206,12 +224,15 @@ class Server:
                 active[iid] = [r, r.spec.steps, now(), hold, r.spec.n ** 3]
             t_attach += time.perf_counter() - ta
             if not active:
+                # nothing to step: wait for a preparation to finish or the next
+                # request to arrive, whichever comes first
                 ti = time.perf_counter()
                 nxt = [f for f in preparing]
+                until = max(0.0, reqs[0].arrival - now()) if reqs else None
                 if nxt:
-                    cf.wait(nxt, return_when=cf.FIRST_COMPLETED)
+                    cf.wait(nxt, timeout=until, return_when=cf.FIRST_COMPLETED)
                 elif reqs:
-                    time.sleep(max(0.0, min(reqs[0].arrival - now(), 0.05)))
+                    time.sleep(min(until, 0.05))
                 t_idle += time.perf_counter() - ti
                 continue
             k = min(self.max_chunk, min(a[1] for a in active.values()))
@@ -253,7 +274,7 @@ def sequential(pool: MeshPool, requests, precision=CG_PREC_FP64, layouts=None):
         t0 = time.perf_counter()
         lay = layouts.get(key) if layouts else None
         b = Batch(precision)
-        iid = b.attach(Instance(m, r.spec.deck(precision, check_interval=100), *r.spec.state(m), layout=lay))
+        iid = b.attach(Instance(m, r.spec.deck(precision, check_interval=100), *pool.state(r.spec), layout=lay))
         b.step(r.spec.steps)
         d = b.diag(iid)
         b.close()
