@@ -160,6 +160,7 @@ struct Instance : HostInst {
   Bufs<R> b;
   std::shared_ptr<Layout<R>> lay;    // its mesh's device layout
   cgdev::Inst<R>* solo = nullptr;    // one-entry device table of this instance
+  int slot = -1;                     // its entry in its engine's device table
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
             prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
@@ -194,6 +195,11 @@ struct Engine {
   bool force_global = false;
   int tile = 4;           // extended slots per batch in the LDS pass
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two captured steps from parity cur (which they return to)
+  // host copies of the device table and block maps; a removed solve leaves
+  // its entries empty (-1, nblocks 0) until the next rebuild
+  std::vector<cgdev::Inst<R>> h_tab;
+  std::vector<int> h_bmap, h_pbmap;
+  int64_t hole_blocks = 0;  // cell blocks of removed solves still in the block map
 };
 
 }  // namespace
@@ -1006,6 +1012,7 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   for (int i = 0; i < n; ++i) {
     Instance<R>* I = E.inst[i];
     tab[i] = I->record();
+    I->slot = i;
     for (int b = 0; b < I->nblocks; ++b) logical.push_back((i << cgdev::kBlkBits) | b);
     for (int b = 0; b < I->npblocks; ++b) plogical.push_back((i << cgdev::kBlkBits) | b);
     lds = std::max(lds, I->lds);
@@ -1052,6 +1059,10 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
     max_nsc = std::max(max_nsc, I->prm.nsc);
     lds_max = I->lay->lds_max;
   }
+  E.h_tab = tab;
+  E.h_bmap = bmap;
+  E.h_pbmap = pbmap;
+  E.hole_blocks = 0;
   const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
   E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
          any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf};
@@ -2628,10 +2639,31 @@ int cg_batch_remove(cg_batch* b, int id) {
   return with_engine(b, [&](auto& E) -> int {
     for (size_t i = 0; i < E.inst.size(); ++i) {
       if (E.inst[i]->id != id) continue;
+      auto* I = E.inst[i];
       HIPCHK(b, hipStreamSynchronize(b->stream));
-      free_instance(E.inst[i]);
+      // while the table and block maps are current, the solve's entries are
+      // emptied in place (its blocks return at once, the step passes skip its
+      // table entry) and the captured step graphs stay valid; the maps are
+      // rebuilt when the removed blocks reach half of the launch
+      if (!E.dirty && I->slot >= 0 && (size_t)I->slot < E.h_tab.size() && E.inst.size() > 1) {
+        const int sl = I->slot;
+        for (int& e : E.h_bmap)
+          if (e >= 0 && (e >> cgdev::kBlkBits) == sl) e = -1;
+        for (int& e : E.h_pbmap)
+          if (e >= 0 && (e >> cgdev::kBlkBits) == sl) e = -1;
+        E.h_tab[sl].nblocks = 0;
+        E.h_tab[sl].npblocks = 0;
+        HIPCHK(b, hipMemcpy(E.bmap, E.h_bmap.data(), sizeof(int) * E.h_bmap.size(), hipMemcpyHostToDevice));
+        if (!E.h_pbmap.empty())
+          HIPCHK(b, hipMemcpy(E.pbmap, E.h_pbmap.data(), sizeof(int) * E.h_pbmap.size(), hipMemcpyHostToDevice));
+        HIPCHK(b, hipMemcpy(E.tab + sl, &E.h_tab[sl], sizeof(E.h_tab[sl]), hipMemcpyHostToDevice));
+        E.hole_blocks += I->nblocks;
+        if (2 * E.hole_blocks > (int64_t)E.h_bmap.size()) E.dirty = true;
+      } else {
+        E.dirty = true;
+      }
+      free_instance(I);
       E.inst.erase(E.inst.begin() + i);
-      E.dirty = true;
       return 0;
     }
     return b->set(CG_EINVAL, "no instance " + std::to_string(id));

@@ -99,15 +99,18 @@ __device__ __forceinline__ void cfl_max(unsigned long long* p, unsigned long lon
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Instance and instance-local block of this workgroup. The host orders the
+// Instance and instance-local block of this workgroup, or null for an empty
+// entry of the block map (-1: a block of a solve removed from the batch since
+// the launch was captured; the whole workgroup returns). The host orders the
 // block map so that each XCD (workgroups are dealt round-robin over the 8
 // XCDs) works on one contiguous stretch of the concatenated Morton orders.
 template <class R>
-__device__ __forceinline__ const Inst<R>& inst_of(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+__device__ __forceinline__ const Inst<R>* inst_of(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                   int& lblk) {
   const int e = bmap[blockIdx.x];
+  if (e < 0) return nullptr;
   lblk = e & ((1 << kBlkBits) - 1);
-  return I[e >> kBlkBits];
+  return I + (e >> kBlkBits);
 }
 
 // streamed once per launch: keep them from evicting reusable neighbour records
@@ -480,7 +483,9 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_cpair(const Inst<R>* __restrict__ I, const int* __restrict__ pbmap,
                                                   int cur) {
   int lblk;
-  const Inst<R>& X = inst_of(I, pbmap, lblk);
+  const Inst<R>* Xp = inst_of(I, pbmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> in = X.st[cur];
   const WorkT<R> w = X.w;
@@ -819,7 +824,9 @@ template <class R, int STAGE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur,
                                                  double* rhs_out) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
@@ -866,7 +873,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   extern __shared__ __align__(16) unsigned char smem[];
   T2* halo = reinterpret_cast<T2*>(smem);  // [hsize][3]
   int blk;
-  const Inst<R>& X = inst_of(I, bmap, blk);
+  const Inst<R>* Xp = inst_of(I, bmap, blk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
@@ -934,7 +943,9 @@ template <class R, int STAGE>
 __global__ __launch_bounds__(kBlock) void k_rk_apply(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                      int cur, const double* __restrict__ rhs) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> in = X.st[cur], out = X.st[cur ^ 1];
   const WorkT<R> w = X.w;
@@ -974,7 +985,9 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
 #pragma clang fp contract(off)
   extern __shared__ __align__(16) unsigned char smem[];
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const Params& P = X.P;
   const int nsc = P.nsc;
   if (nsc <= 0) return;
@@ -1128,7 +1141,9 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
@@ -1199,7 +1214,9 @@ __global__ __launch_bounds__(kBlock) void k_grad(const Inst<R>* __restrict__ I, 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_eos(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
@@ -1274,7 +1291,9 @@ __global__ __launch_bounds__(kBlock) void k_sgs_sigma(const Inst<R>* __restrict_
                                                       int cur) {
 #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const Params& P = X.P;
   if (P.sgs_model != 2) return;
   const GeomT<R> g = X.g;
@@ -1372,7 +1391,9 @@ __global__ __launch_bounds__(kBlock) void k_dsm_filter(const Inst<R>* __restrict
                                                        int cur) {
 #pragma clang fp contract(off)
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const Params& P = X.P;
   if (P.sgs_model != 3) return;
   const GeomT<R> g = X.g;
@@ -1423,7 +1444,9 @@ __global__ __launch_bounds__(kBlock) void k_dsm_finish(const Inst<R>* __restrict
                                                        int cur, int relax) {
 #pragma clang fp contract(off)
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const Params& P = X.P;
   if (P.sgs_model != 3) return;
   const GeomT<R> g = X.g;
@@ -1611,7 +1634,9 @@ template <class R, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_sensor_full(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                         int cur) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const Params& P = X.P;
   if (!P.full_gr) return;  // the whole block belongs to one instance
   const GeomT<R> g = X.g;
@@ -1663,7 +1688,9 @@ template <class R, int FIELDS, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_props(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
                                                   int cur) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
@@ -1900,7 +1927,7 @@ __device__ __forceinline__ void begin_step(StepInfo* s, const Params& P) {
 template <class R>
 __global__ void k_begin_step(const Inst<R>* __restrict__ I, int ninst) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x)
-    begin_step(I[i].w.info, I[i].P);
+    if (I[i].nblocks > 0) begin_step(I[i].w.info, I[i].P);  // 0: removed from the batch
 }
 
 // per-cell convective CFL at a given dt
@@ -1908,7 +1935,9 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur,
                                                 double dt, unsigned long long* max_bits) {
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
@@ -1954,7 +1983,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_partial(const Inst<R>* __restri
   #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   int lblk;
-  const Inst<R>& X = inst_of(I, bmap, lblk);
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
   const GeomT<R> g = X.g;
   const StateT<R> st = X.st[cur];
   const WorkT<R> w = X.w;
@@ -2010,6 +2041,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict
   #pragma clang fp contract(off)  // FP64: rounded as the reference rounds
   __shared__ double sh[8][kBlock];
   const Inst<R>& X = I[blockIdx.x];
+  if (X.nblocks == 0) return;  // removed from the batch
   const auto partial = G(partial_in ? partial_in : X.w.partial);
   const auto diag = G(out ? out : X.w.diag);
   const int nblocks = X.nblocks;

@@ -91,6 +91,32 @@ def test_batch_join_and_leave():
     b.close()
 
 
+@pytest.mark.parametrize("precision", [CG_PREC_FP64, CG_PREC_MIXED])
+def test_removed_solves_leave_the_step_graphs_in_place(precision):
+    # solves removed between graph replays: their block-map and table entries
+    # are emptied in place (the captured graphs stay) until half of the blocks
+    # are empty, then the maps are rebuilt; the others stay bit-identical to
+    # their solo runs throughout, and a solve can join afterwards
+    b = Batch(precision)
+    ids = [add_case(b, i, precision) for i in range(4)]
+    b.step(4)
+    b.remove(ids[0])
+    b.step(4)
+    for i in (1, 2, 3):
+        assert_same(b.state(ids[i]), alone(i, precision, 8)[0])
+    b.remove(ids[2])  # 12^3 of 8^3 + 10^3 + 12^3 + 9^3 cells: past half, a rebuild
+    b.step(2)
+    b.remove(ids[1])
+    b.step(3)
+    assert_same(b.state(ids[3]), alone(3, precision, 13)[0])
+    late = add_case(b, 0, precision)
+    b.step(4)
+    assert_same(b.state(ids[3]), alone(3, precision, 17)[0])
+    assert_same(b.state(late), alone(0, precision, 4)[0])
+    assert b.diag(ids[3]).step == 17
+    b.close()
+
+
 @pytest.mark.parametrize("option,values", [("graph", (1, 0)), ("lds", (1, 0)), ("tile", (4, 2, 8))])
 def test_batch_variants_bitwise(option, values):
     prec = CG_PREC_FP64
