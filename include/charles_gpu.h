@@ -81,6 +81,15 @@ typedef struct cg_solver cg_solver;
  *   CBC_RUNP <rho> <un> <p>           (IdealGasSolverFlux.hpp:218-306):
  *   CBC_RUP <rho> <u> <v> <w> <p>     Cbc::addBoundaryFlux (:2960-2976),
  *                                    CbcUpt/Mpt/Runp/Rup::initData (:3182-3300)
+ *   NSCBC_OUTLET_P <p_ref> <sigma> <L>
+ *   NSCBC_OUTLET_PRESSURE P_REF <p> [SIGMA <s>] [L <l>] [CONSTANT_PRESSURE]
+ *                                    NscbcOutletP (:3846-3932, 4009-4068): per
+ *                                    face a boundary state (rho, p, u) evolved
+ *                                    with the step's RK3 weights by
+ *                                    calcNscbcIgOutletRhs (BcCommon.hpp:419-470)
+ *                                    and the Riemann flux against it (Nscbc,
+ *                                    :3578-3693); SIGMA 0.2 and L 1 by default,
+ *                                    CONSTANT_PRESSURE sets sigma -1
  * The other types of the reference are refused by cg_params_parse. */
 #define CG_MAX_BC_ZONES 16
 #define CG_BC_SLIP 1
@@ -90,6 +99,7 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_CBC_MPT 5
 #define CG_BC_CBC_RUNP 6
 #define CG_BC_CBC_RUP 7
+#define CG_BC_NSCBC_OUTLET_P 8
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
@@ -98,6 +108,8 @@ typedef struct cg_bc {
   double p;       /* CBC pressure */
   double rho;     /* CBC_RUNP / CBC_RUP density */
   double q;       /* CBC_MPT mass flow mdot, CBC_RUNP inflow normal velocity un */
+  double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
+  double L;       /* NSCBC outlet length scale */
 } cg_bc;
 
 /* Input-file parameters of the path; the names in comments are the reference's

@@ -138,6 +138,8 @@ struct HostInst {
   double* store_buf = nullptr;       // older snapshot
   double* store_recent = nullptr;    // newer snapshot
   size_t snap_n = 0;                 // doubles per snapshot
+  size_t snap_bc = 0;                // of them, the NSCBC faces' states at the end (5 per boundary face)
+  bool nscbc = false;                // some zone is an NSCBC outlet
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
   bool have_store = false;
@@ -895,6 +897,9 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_rho[z] = p->bc[k].rho;
       hi->P.bc_q[z] = p->bc[k].q;
       hi->P.bc_area[z] = lay->zone_area[z];
+      hi->P.bc_sigma[z] = p->bc[k].sigma;
+      hi->P.bc_L[z] = p->bc[k].L;
+      hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
     }
     if (!missing.empty()) return s->set(CG_EINVAL, "no boundary condition for zone(s) " + missing);
     if (lay->g.nbf > 0 && (p->sgs_model == CG_SGS_DSM_LOCAL || p->nsc > 0))
@@ -946,6 +951,14 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   rc |= dev_alloc(s, h, &w.partial, 8 * (size_t)hi->nblocks);
   rc |= dev_alloc(s, h, &w.diag, 8);
   hi->snap_n = mixed ? (size_t)ncv * 5 : (size_t)ncv * (cgdev::L_N + 1);
+  if (hi->nscbc) {
+    // NSCBC outlet faces: their states (in the snapshots too) and stage rates
+    const size_t nbf = (size_t)lay->g.nbf;
+    rc |= dev_alloc(s, h, &w.nscbc, 5 * nbf);
+    rc |= dev_alloc(s, h, &w.nscbc_rhs, 15 * nbf);
+    hi->snap_bc = 5 * nbf;
+    hi->snap_n += hi->snap_bc;
+  }
   rc |= dev_alloc(s, h, &hi->store_buf, hi->snap_n);
   rc |= dev_alloc(s, h, &hi->store_recent, hi->snap_n);
   rc |= dev_alloc(s, h, &hi->rhs_tmp, 5 * (size_t)ncv);
@@ -984,6 +997,11 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   }
   rc |= cgk::launch_fill(w.rhs0, R(0), 5 * (int64_t)ncv, stream);
   rc |= cgk::launch_fill(w.rhs1, R(0), 5 * (int64_t)ncv, stream);
+  if (hi->nscbc) {
+    // Nscbc::initData: zero rates; the states are set from the cells by initComplete
+    rc |= cgk::launch_fill(w.nscbc, 0.0, 5 * (int64_t)lay->g.nbf, stream);
+    rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
+  }
   if (rc) return s->set(CG_EHIP, "initial fill failed");
   HIPCHK(s, hipStreamSynchronize(stream));
   return upload_solo(s, I);
@@ -1154,12 +1172,27 @@ int refresh(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   return 0;
 }
 
+// Nscbc::initialHook (IdealGasSolverBcs.cpp:3630-3649): the NSCBC outlet faces'
+// states from their cells (initComplete; a restart file does not carry them)
+template <class R>
+int nscbc_init(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
+  if (!I->nscbc) return 0;
+  if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, stream)) return launch_fail(s, "nscbc init", le);
+  return 0;
+}
+
 // snapshot layout: FP64, light records [ncv][8] followed by rhoE [ncv]; mixed
 // precision, the FP64 state cons[5][ncv] (the records are recomputed on restore)
 template <class R>
 int snapshot_copy(Err* s, Instance<R>* I, int cur, double* dst, bool to_snapshot, hipStream_t stream) {
   const auto dir = hipMemcpyDeviceToDevice;
   const cgdev::StateT<R>& st = I->b.st[cur];
+  if (I->snap_bc) {
+    // Nscbc::storeData / restoreData: the NSCBC faces' states after the flow's
+    double* bc = dst + (I->snap_n - I->snap_bc);
+    HIPCHK(s, hipMemcpyAsync(to_snapshot ? bc : I->b.w.nscbc, to_snapshot ? I->b.w.nscbc : bc,
+                             sizeof(double) * I->snap_bc, dir, stream));
+  }
   if constexpr (std::is_same<R, float>::value) {
     const size_t n = (size_t)I->ncv * 5;
     HIPCHK(s, hipMemcpyAsync(to_snapshot ? dst : st.cons, to_snapshot ? st.cons : dst, sizeof(double) * n, dir, stream));
@@ -1613,9 +1646,36 @@ int cg_params_parse(cg_params* p, const char* text) {
             !num(5, &b.u[2]) || !num(6, &b.p) || !(b.p > 0.0 && b.rho > 0.0))
           return bad("expected CBC_RUP <rho> <u> <v> <w> <p> [STATIONARY_FRAME]");
         // STATIONARY_FRAME matters only in a rotating frame, which this path does not have
+      } else if (type == "NSCBC_OUTLET_P") {
+        // NscbcOutletP::initData (IdealGasSolverBcs.cpp:3851-3858): p_ref, sigma, L
+        b.type = CG_BC_NSCBC_OUTLET_P;
+        if (tok.size() != 5 || !num(2, &b.p) || !num(3, &b.sigma) || !num(4, &b.L) || !(b.L > 0.0))
+          return bad("expected NSCBC_OUTLET_P <p_ref> <sigma> <L>");
+      } else if (type == "NSCBC_OUTLET_PRESSURE") {
+        // (:3859-3887): tokens SIGMA (0.2), L (1.0), P_REF (required), CONSTANT_PRESSURE (sigma -1)
+        b.type = CG_BC_NSCBC_OUTLET_P;
+        b.sigma = 0.2;
+        b.L = 1.0;
+        b.p = -1.0;
+        for (size_t i = 2; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          if (opt == "SIGMA") {
+            if (!num(++i, &b.sigma)) return bad("expected SIGMA <value>");
+          } else if (opt == "L") {
+            if (!num(++i, &b.L) || !(b.L > 0.0)) return bad("expected L <value>");
+          } else if (opt == "P_REF") {
+            if (!num(++i, &b.p)) return bad("expected P_REF <value>");
+          } else if (opt == "CONSTANT_PRESSURE") {
+            b.sigma = -1.0;
+          } else {
+            return bad("unknown NSCBC_OUTLET_PRESSURE option " + opt);
+          }
+        }
+        if (b.p < 0.0) return bad("NSCBC_OUTLET_PRESSURE needs P_REF");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP and CBC_RUP are)");
+                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P and "
+                   "NSCBC_OUTLET_PRESSURE are)");
       }
       int slot = -1;
       for (int i = 0; i < p->nbc; ++i)
@@ -2040,6 +2100,7 @@ int cg_init_complete(cg_solver* s) {
     if (int rc = write_info(s, I, s->info_host, s->stream)) return rc;
     if (int rc = dsm_reset(s, I, s->stream)) return rc;
     if (int rc = refresh(s, I, E.cur, s->stream)) return rc;
+    if (int rc = nscbc_init(s, I, E.cur, s->stream)) return rc;
     return do_store(s, I, E.cur, s->info_host, s->stream);
   });
 }
@@ -2057,6 +2118,9 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
   if (!s) return fail(CG_EINVAL, "null solver");
   return with_inst(s, [&](auto& E, auto* I) -> int {
     (void)E;
+    // the hooked stage is an RHS pass and a separate update, which do not
+    // carry the NSCBC faces' own RK update
+    if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC outlet zones");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;
     if (!fn) std::vector<double>().swap(I->hook_buf);
@@ -2532,6 +2596,7 @@ int create_instance(cg_instance* x, std::shared_ptr<Layout<R>> lay, const cg_mes
   if (!rc) rc = set_state(x, *out, 0, rho, u, rhoE, stream);
   if (!rc) rc = dsm_reset(x, *out, stream);
   if (!rc) rc = refresh(x, *out, 0, stream);
+  if (!rc) rc = nscbc_init(x, *out, 0, stream);
   if (!rc) rc = do_store(x, *out, 0, host, stream);
   if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = x->set(CG_EHIP, "stream synchronize failed");
   if (host) (void)hipHostFree(host);

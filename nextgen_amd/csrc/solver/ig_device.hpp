@@ -138,12 +138,15 @@ struct WorkT {
   double *lijmij, *mijmij;
   // passive scalars: RK3 registers of the scalar RHS [nsc][ncv] (null without scalars)
   double *rhs_sc0, *rhs_sc1;
+  // NSCBC outlet faces (null without them): per boundary face (device order)
+  // its state rho, p, u[3] and the rates of the step's three stages [3][5]
+  double *nscbc, *nscbc_rhs;
 };
 
 constexpr int kMaxBcZones = 16;
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
-  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7
+  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8
 };
 
 struct Params {
@@ -163,6 +166,7 @@ struct Params {
   double bc_u[kMaxBcZones][3];
   double bc_p[kMaxBcZones], bc_rho[kMaxBcZones], bc_q[kMaxBcZones];
   double bc_area[kMaxBcZones];  // the zone's area (sum of its faces' |n_bf|), CBC_MPT
+  double bc_sigma[kMaxBcZones], bc_L[kMaxBcZones];  // NSCBC outlet sigma and L (p_ref in bc_p)
 };
 
 using Geom = GeomT<double>;
@@ -242,6 +246,9 @@ template <class R>
 int launch_rk_apply(int stage, const Launch<R>& L, int cur, const double* rhs, void* stream);
 template <class R>
 int launch_begin_step(const Launch<R>& L, void* stream);
+// Nscbc::initialHook: every NSCBC outlet face's state from its cell's
+template <class R>
+int launch_nscbc_init(const Launch<R>& L, int cur, void* stream);
 // FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]
 template <class R>
 int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, void* stream);

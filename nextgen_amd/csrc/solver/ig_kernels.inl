@@ -687,15 +687,56 @@ __device__ __forceinline__ void cbc_state(const Params& P, int z, int type, cons
   h = R * T * gamma / (gamma - 1.0);
 }
 
+// calcNscbcIgOutletRhs (BcCommon.hpp:419-470), FP64 in its operation order:
+// rhs = d/dt (rho_bc, p_bc, u_bc[3]) of an NSCBC outlet face from its state
+// bc = rho, p, u[3] and its cell's cv = rho, p, u[3]
+__device__ __forceinline__ void nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta,
+                                                 const double bc[5], const double cv[5], double gamma,
+                                                 double L_ref, double sigma, double p_ref) {
+#pragma clang fp contract(off)  // FP64: rounded as the reference rounds
+  const double rho_bc = bc[0], p_bc = bc[1];
+  const double un_bc = smax(0.0, bc[2] * unit_n[0] + bc[3] * unit_n[1] + bc[4] * unit_n[2]);
+  const double u0 = smax(0.0, cv[2] * unit_n[0] + cv[3] * unit_n[1] + cv[4] * unit_n[2]);
+  const double sos_bc = sqrt(gamma * p_bc / rho_bc);
+  const double Ma = un_bc / sos_bc;
+  const double dudn = (un_bc - u0) / delta;
+  const double dpdn = (p_bc - cv[1]) / delta;
+  const double L5 = (un_bc + sos_bc) * (dpdn + rho_bc * sos_bc * dudn);
+  double L1 = sigma >= 0.0 ? sigma * (1.0 - Ma * Ma) * sos_bc / L_ref * (p_bc - p_ref) : -L5;
+  const double drhodn = (rho_bc - cv[0]) / delta;
+  const double L2 = un_bc * (sos_bc * sos_bc * drhodn - dpdn);
+  if (Ma >= 1.0) L1 = sos_bc * (Ma - 1.0) * (dpdn - rho_bc * sos_bc * dudn);
+  rhs[0] = -(L2 + 0.5 * (L5 + L1)) / sos_bc / sos_bc;
+  rhs[1] = -0.5 * (L5 + L1);
+  for (int i = 0; i < 3; ++i) {
+    const double dudn_i = (bc[2 + i] - cv[2 + i]) / delta;
+    rhs[2 + i] = -(rho_bc * un_bc * dudn_i + 0.5 * rho_bc * sos_bc * (1.0 - Ma) * dudn * unit_n[i]) / rho_bc -
+                 0.5 * (1.0 + Ma) * dpdn * unit_n[i] / rho_bc + 0.5 * L1 * unit_n[i] / sos_bc / rho_bc;
+  }
+}
+
+// RkWeights::erk_wgt3 (RkWeights.hpp:6-8), as the NSCBC faces' rk3Step uses them
+__device__ __forceinline__ double erk_wgt3(int stage, int i) {
+  if (stage == 1) return i == 0 ? 1.0 : 0.0;
+  if (stage == 2) return i == 0 ? -0.75 : (i == 1 ? 0.25 : 0.0);
+  return i == 2 ? 8.0 / 12.0 : -1.0 / 12.0;
+}
+
 // The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
 // IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
 // face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
 // p is the stage pressure (gauged in mixed precision, as in the pair fluxes:
 // the gauge's p_g n sums to zero over a cell's pair and boundary faces), mu_lam
 // and loc_lam the lagged laminar coefficients, T = p / (rho R) of the stage.
+// NSCBC outlet faces (Nscbc, IdealGasSolverBcs.cpp:3578-3693) in stage `stage`:
+// the face's rate from its state and the cell's (NscbcOutletP::calcRhs), the
+// Riemann flux against the face's state, then the face's RK3 update
+// (Nscbc::rk3Step), the order of advanceSolution (IdealGasSolver.hpp:457-468);
+// with update false (an RHS evaluation alone) only the flux.
 template <class R>
 __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st,
-                                              const Params& P, int c, const Light<R>& self, R acc[5]) {
+                                              const Params& P, int c, const Light<R>& self, R acc[5],
+                                              int stage = 1, bool update = false) {
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
   if (g.nbf == 0) return;
@@ -723,23 +764,58 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
       // between the cell's FP64 state and the zone's boundary state
       const double nd[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
-      double ua[3], va, pa, ha;
+      double ua[3], va, pa, ha, rho_a;
       if constexpr (kMixed<R>) {
         const int ncv = g.ncv;
         double uu[3];
         for (int i = 0; i < 3; ++i) uu[i] = G(st.cons)[(size_t)(1 + i) * ncv + c];
         double T;
-        const Light<double> L = eos(P, G(st.cons)[c], uu, G(st.cons)[(size_t)4 * ncv + c], T);
+        rho_a = G(st.cons)[c];
+        const Light<double> L = eos(P, rho_a, uu, G(st.cons)[(size_t)4 * ncv + c], T);
         for (int i = 0; i < 3; ++i) ua[i] = L.u[i];
         va = L.spv; pa = L.p; ha = L.h;
       } else {
         for (int i = 0; i < 3; ++i) ua[i] = self.u[i];
         va = self.spv; pa = self.p; ha = self.h;
+        rho_a = self.rho;
       }
       double ub[3], vb, pb, hb;
-      cbc_state(P, z, type, nd, ub, vb, pb, hb);
+      __attribute__((address_space(1))) double* q = nullptr;  // an NSCBC face's state
+      if (type == BC_NSCBC_OUTLET_P) {
+        #pragma clang fp contract(off)
+        q = G(w.nscbc) + 5 * (size_t)b;
+        if (update) {
+          const double area = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+          const double unit_n[3] = {nd[0] / area, nd[1] / area, nd[2] / area};
+          const double delta = area / G(g.bf_aod)[b];
+          const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
+          const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
+          double r[5];
+          nscbc_outlet_rhs(r, unit_n, delta, qs, cv, P.gamma, P.bc_L[z], P.bc_sigma[z], P.bc_p[z]);
+          const auto rr = G(w.nscbc_rhs) + 15 * (size_t)b + 5 * (size_t)(stage - 1);
+          for (int k = 0; k < 5; ++k) rr[k] = r[k];
+        }
+        const double gogm1 = P.gamma / (P.gamma - 1.0);
+        for (int i = 0; i < 3; ++i) ub[i] = q[2 + i];
+        vb = 1.0 / q[0];
+        pb = q[1];
+        hb = gogm1 * q[1] * (1.0 / q[0]);
+      } else {
+        cbc_state(P, z, type, nd, ub, vb, pb, hb);
+      }
       double F[5];
       riemann_flux(F, nd, ua, va, pa, ha, ub, vb, pb, hb, P.gamma);
+      if (q && update) {
+        #pragma clang fp contract(off)
+        const double dt = G(w.info)->dt;
+        const auto rr = G(w.nscbc_rhs) + 15 * (size_t)b;
+        for (int i = 0; i < stage; ++i) {
+          const double wi = erk_wgt3(stage, i);
+          q[0] += wi * dt * rr[5 * i + 0];
+          q[1] += wi * dt * rr[5 * i + 1];
+          for (int j = 0; j < 3; ++j) q[2 + j] += wi * dt * rr[5 * i + 2 + j];
+        }
+      }
       if constexpr (kMixed<R>) {
         // the accumulation's gauges: momentum without p_g n, energy as F_rhoE - h_g F_rho
         #pragma clang fp contract(off)
@@ -841,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(const Inst<R>* __restrict__ I, 
 
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  boundary_flux(g, w, in, P, c, self, acc);
+  boundary_flux(g, w, in, P, c, self, acc, STAGE, MODE == 1);
   gather_compact(g, w, pos, acc);
   for (int k = 0; k < deg; ++k) {
     const int slot = base + k * kSlice;
@@ -898,7 +974,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_lds(const Inst<R>* __restrict__
   const int ncv = g.ncv;
   const Light<R> self = load_light(in.light, c);
   R acc[5];
-  boundary_flux(g, w, in, P, c, self, acc);
+  boundary_flux(g, w, in, P, c, self, acc, STAGE, MODE == 1);
   gather_compact(g, w, pos, acc);
   // slots are fetched NB at a time so their global loads overlap (NB is a
   // tuning choice); the accumulation order stays k = 0, 1, ...
@@ -1930,6 +2006,44 @@ __global__ void k_begin_step(const Inst<R>* __restrict__ I, int ninst) {
     if (I[i].nblocks > 0) begin_step(I[i].w.info, I[i].P);  // 0: removed from the batch
 }
 
+// Nscbc::initialHook (IdealGasSolverBcs.cpp:3630-3649): each NSCBC outlet
+// face's state from its cell's rho, p, u (FP64)
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
+                                                       int cur) {
+  int lblk;
+  const Inst<R>* Xp = inst_of(I, bmap, lblk);
+  if (!Xp) return;
+  const Inst<R>& X = *Xp;
+  const GeomT<R>& g = X.g;
+  const int c = lblk * kBlock + threadIdx.x;
+  if (c >= g.ncv || g.nbf == 0 || !X.w.nscbc) return;
+  const int b0 = G(g.bf_ptr)[c], b1 = G(g.bf_ptr)[c + 1];
+  for (int b = b0; b < b1; ++b) {
+    if (X.P.bc_type[G(g.bf_zone)[b]] != BC_NSCBC_OUTLET_P) continue;
+    double rho, p, u[3];
+    if constexpr (kMixed<R>) {
+      const StateT<R>& st = X.st[cur];
+      const int ncv = g.ncv;
+      for (int i = 0; i < 3; ++i) u[i] = G(st.cons)[(size_t)(1 + i) * ncv + c];
+      double T;
+      rho = G(st.cons)[c];
+      const Light<double> L = eos(X.P, rho, u, G(st.cons)[(size_t)4 * ncv + c], T);
+      for (int i = 0; i < 3; ++i) u[i] = L.u[i];
+      p = L.p;
+    } else {
+      const Light<R> L = load_light(X.st[cur].light, c);
+      rho = L.rho;
+      p = L.p;
+      for (int i = 0; i < 3; ++i) u[i] = L.u[i];
+    }
+    const auto q = G(X.w.nscbc) + 5 * (size_t)b;
+    q[0] = rho;
+    q[1] = p;
+    for (int i = 0; i < 3; ++i) q[2 + i] = u[i];
+  }
+}
+
 // per-cell convective CFL at a given dt
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_cfl(const Inst<R>* __restrict__ I, const int* __restrict__ bmap, int cur,
@@ -2183,6 +2297,12 @@ int launch_begin_step(const Launch<R>& L, void* stream) {
 }
 
 template <class R>
+int launch_nscbc_init(const Launch<R>& L, int cur, void* stream) {
+  hipLaunchKernelGGL(k_nscbc_init<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur);
+  return check();
+}
+
+template <class R>
 int launch_cfl(const Launch<R>& L, int cur, double dt, void* stream, unsigned long long* max_bits) {
   hipLaunchKernelGGL(k_cfl<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur, dt,
                      max_bits);
@@ -2290,6 +2410,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_prims<R>(const Launch<R>&, int, int, int, double*, void*);                  \
   template int launch_props<R>(const Launch<R>&, int, int, void*);                                \
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
+  template int launch_nscbc_init<R>(const Launch<R>&, int, void*);                                \
   template int launch_cfl<R>(const Launch<R>&, int, double, void*, unsigned long long*);          \
   template int launch_diag<R>(const Launch<R>&, int, void*, double*, double*);                   \
   template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);               \

@@ -35,11 +35,13 @@ class CharlesError(RuntimeError):
 CG_MAX_BC_ZONES = 16
 CG_BC_SLIP, CG_BC_WALL_ADIABATIC, CG_BC_WALL_ISOTHERMAL = 1, 2, 3
 CG_BC_CBC_UPT, CG_BC_CBC_MPT, CG_BC_CBC_RUNP, CG_BC_CBC_RUP = 4, 5, 6, 7
+CG_BC_NSCBC_OUTLET_P = 8
 
 
 class cg_bc(C.Structure):
     _fields_ = [("zone", C.c_char * 48), ("type", C.c_int), ("T", C.c_double), ("u", C.c_double * 3),
-                ("p", C.c_double), ("rho", C.c_double), ("q", C.c_double)]
+                ("p", C.c_double), ("rho", C.c_double), ("q", C.c_double), ("sigma", C.c_double),
+                ("L", C.c_double)]
 
 
 class cg_params(C.Structure):

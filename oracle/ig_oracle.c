@@ -183,6 +183,9 @@ struct og_solver {
    * (cell is the pair's hi side) << 31, and a per-pair value buffer */
   int *cp_i, *cp_v;
   og_rhs* pair_buf;
+  /* NSCBC outlet faces (Nscbc, IdealGasSolverBcs.hpp:576-612): per boundary
+   * face rho_bc, p_bc, u_bc[3], the stage rates [3][5] and the two snapshots */
+  double *nscbc, *nscbc_rhs, *nscbc_store, *nscbc_recent;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -223,6 +226,13 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   s->store_buf = NULL;
   s->store_recent = NULL;
   s->nthreads = 1;
+  {
+    const size_t nb = (size_t)(m->nbf > 0 ? m->nbf : 0);
+    s->nscbc = (double*)calloc(5 * nb + 1, sizeof(double));
+    s->nscbc_rhs = (double*)calloc(15 * nb + 1, sizeof(double)); /* Nscbc::initData: zero */
+    s->nscbc_store = (double*)calloc(5 * nb + 1, sizeof(double));
+    s->nscbc_recent = (double*)calloc(5 * nb + 1, sizeof(double));
+  }
   /* IdealGasSolver::initData: reference state (IdealGasSolver.cpp:58-62) */
   for (int i = 0; i < n; ++i) {
     s->rho[i] = p->rho_ref;
@@ -240,6 +250,7 @@ void og_destroy(og_solver* s) {
   free(s->lijmij); free(s->mijmij);
   free(s->sc); free(s->rhs0_sc); free(s->rhs1_sc); free(s->rhs2_sc); free(s->rhs_sc_tmp);
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
+  free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
   free(s);
 }
 
@@ -972,6 +983,85 @@ static void cbc_state(const og_solver* s, int z, const double n_bf[3], og_state*
   b->h = R * T * gamma / (gamma - 1.0);
 }
 
+/* calcNscbcIgOutletRhs (BcCommon.hpp:419-470), in its operation order */
+double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                           double gamma, double L_ref, double sigma, double p_ref) {
+  const double rho_bc = bc[0], p_bc = bc[1], *u_bc = bc + 2;
+  const double rho_cv = cv[0], p_cv = cv[1], *u_cv = cv + 2;
+  const double un_bc = smax(0.0, u_bc[0] * unit_n[0] + u_bc[1] * unit_n[1] + u_bc[2] * unit_n[2]);
+  const double u0 = smax(0.0, u_cv[0] * unit_n[0] + u_cv[1] * unit_n[1] + u_cv[2] * unit_n[2]);
+  const double sos_bc = sqrt(gamma * p_bc / rho_bc);
+  const double Ma = un_bc / sos_bc;
+  const double dudn = (un_bc - u0) / delta;
+  const double dpdn = (p_bc - p_cv) / delta;
+  const double L5 = (un_bc + sos_bc) * (dpdn + rho_bc * sos_bc * dudn);
+  double dudn_[3];
+  for (int i = 0; i < 3; ++i) dudn_[i] = (u_bc[i] - u_cv[i]) / delta;
+  /* the incoming acoustic characteristic; sigma < 0: perfectly reflecting */
+  double L1;
+  if (sigma >= 0.0) L1 = sigma * (1.0 - Ma * Ma) * sos_bc / L_ref * (p_bc - p_ref);
+  else L1 = -L5;
+  /* un_bc >= 0, so Ma >= 0 (the reference asserts it) */
+  const double drhodn = (rho_bc - rho_cv) / delta;
+  const double L2 = un_bc * (sos_bc * sos_bc * drhodn - dpdn);
+  if (Ma >= 1.0) L1 = sos_bc * (Ma - 1.0) * (dpdn - rho_bc * sos_bc * dudn);
+  rhs[0] = -(L2 + 0.5 * (L5 + L1)) / sos_bc / sos_bc;
+  rhs[1] = -0.5 * (L5 + L1);
+  for (int i = 0; i < 3; ++i)
+    rhs[2 + i] = -(rho_bc * un_bc * dudn_[i] + 0.5 * rho_bc * sos_bc * (1.0 - Ma) * dudn * unit_n[i]) / rho_bc -
+                 0.5 * (1.0 + Ma) * dpdn * unit_n[i] / rho_bc + 0.5 * L1 * unit_n[i] / sos_bc / rho_bc;
+  return un_bc;
+}
+
+/* NscbcOutletP::calcRhsPressure (IdealGasSolverBcs.cpp:4009-4024): each NSCBC
+ * face's rate of stage rk_stage from its state and its cell's rho, p, u */
+static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
+  const og_mesh* m = &s->m;
+  const og_params* P = &s->p;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (P->bc_type[z] != 8) continue;
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double bf_area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double unit_n[3] = {n[0] / bf_area, n[1] / bf_area, n[2] / bf_area};
+    const int icv = m->cvobf[ibf];
+    const double delta = bf_area / m->aod_bf[ibf];
+    const double cv[5] = {s->rho[icv], s->pr[icv], s->u[3 * (size_t)icv], s->u[3 * (size_t)icv + 1],
+                          s->u[3 * (size_t)icv + 2]};
+    og_nscbc_outlet_rhs(&s->nscbc_rhs[15 * (size_t)ibf + 5 * (size_t)(rk_stage - 1)], unit_n, delta,
+                        &s->nscbc[5 * (size_t)ibf], cv, P->gamma, P->bc_L[z], P->bc_sigma[z], P->bc_p[z]);
+  }
+}
+
+/* Nscbc::rk3Step (IdealGasSolverBcs.cpp:3651-3660) */
+static void nscbc_rk3(og_solver* s, int rk_stage) {
+  const og_mesh* m = &s->m;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    if (s->p.bc_type[m->zone_bf[ibf]] != 8) continue;
+    double* b = &s->nscbc[5 * (size_t)ibf];
+    const double* r = &s->nscbc_rhs[15 * (size_t)ibf];
+    for (int i = 0; i < rk_stage; ++i) {
+      const double w = og_erk_wgt3[rk_stage - 1][i];
+      b[0] += w * s->dt * r[5 * i + 0];
+      b[1] += w * s->dt * r[5 * i + 1];
+      for (int j = 0; j < 3; ++j) b[2 + j] += w * s->dt * r[5 * i + 2 + j];
+    }
+  }
+}
+
+/* Nscbc::initialHook (IdealGasSolverBcs.cpp:3630-3649): the faces' states from their cells */
+static void nscbc_init(og_solver* s) {
+  const og_mesh* m = &s->m;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    if (s->p.bc_type[m->zone_bf[ibf]] != 8) continue;
+    const int icv = m->cvobf[ibf];
+    double* b = &s->nscbc[5 * (size_t)ibf];
+    b[0] = s->rho[icv];
+    b[1] = s->pr[icv];
+    for (int i = 0; i < 3; ++i) b[2 + i] = s->u[3 * (size_t)icv + i];
+  }
+}
+
 /* addBoundaryFlux of every zone in zone order (IdealGasSolverBcs.cpp:14-47 SlipWallBc /
  * WallAdiabatic, :1485-1510 WallIsothermal), first in calcRhsBase
  * (IdealGasSolver.cpp:261-262): stage p, u, T; lagged mu_lam, loc_lam */
@@ -997,9 +1087,19 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
       }
       rhs[icv].rhoE -= visc_work;
     } else if (P->bc_type[z] >= 4) {
-      /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976) */
+      /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976); an NSCBC face
+       * (Nscbc::addBoundaryFlux, :3662-3693) uses its evolved state */
       og_state b;
-      cbc_state(s, z, n, &b);
+      if (P->bc_type[z] == 8) {
+        const double* q = &s->nscbc[5 * (size_t)ibf];
+        const double gogm1 = P->gamma / (P->gamma - 1.0);
+        for (int i = 0; i < 3; ++i) b.u[i] = q[2 + i];
+        b.sp_vol = 1.0 / q[0];
+        b.p = q[1];
+        b.h = gogm1 * q[1] * (1.0 / q[0]);
+      } else {
+        cbc_state(s, z, n, &b);
+      }
       double F[5];
       og_flux_riemann(F, n, &s->cv_light[icv], &b, P->gamma, P->gamma);
       rhs[icv].rho -= F[0];
@@ -1220,6 +1320,7 @@ void og_init_complete(og_solver* s) {
   }
   s->dsm_kept = 0;
   og_update_state(s);
+  nscbc_init(s);
   og_sgs_props(s);
   og_store(s);
 }
@@ -1242,12 +1343,17 @@ void og_store(og_solver* s) {
   for (int i = 0; i < n; ++i)
     for (int d = 0; d < 3; ++d) s->store_recent[c++] = s->u[3 * (size_t)i + d];
   for (int i = 0; i < n; ++i) s->store_recent[c++] = s->rhoE[i];
+  const size_t nb = 5 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0);
   if (!s->store_buf) {
     s->store_buf = (double*)malloc(sizeof(double) * cnt);
     memcpy(s->store_buf, s->store_recent, sizeof(double) * cnt);
     s->store_step = s->store_step_recent;
     s->store_time = s->store_time_recent;
+    memcpy(s->nscbc_store, s->nscbc, sizeof(double) * nb);
+  } else {
+    memcpy(s->nscbc_store, s->nscbc_recent, sizeof(double) * nb);
   }
+  memcpy(s->nscbc_recent, s->nscbc, sizeof(double) * nb); /* Nscbc::storeData */
 }
 
 void og_restore(og_solver* s) {
@@ -1259,6 +1365,7 @@ void og_restore(og_solver* s) {
   for (int i = 0; i < n; ++i)
     for (int d = 0; d < 3; ++d) s->u[3 * (size_t)i + d] = s->store_buf[c++];
   for (int i = 0; i < n; ++i) s->rhoE[i] = s->store_buf[c++];
+  memcpy(s->nscbc, s->nscbc_store, sizeof(double) * 5 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
   og_update_state(s);
   og_sgs_props(s);
   s->p.dt_data[0] *= 0.8;
@@ -1280,18 +1387,26 @@ int og_step(og_solver* s) {
   }
   /* calcRhs = calcRhsBase + calcRhsScalars + addSourceHook(rhs, time, rk_stage)
    * (IdealGasSolver.hpp:697-705), at the stage times t, t + dt, t + dt/2 */
+  /* each stage: the NSCBC faces' rates (bc calcRhs), the flow RHS with the
+   * faces' states, the faces' RK update, the flow's (IdealGasSolver.hpp:457-468) */
+  nscbc_calc_rhs(s, 1);
   calc_rhs_into(s, s->rhs0, s->rhs0_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs0, n, s->time, 1);
+  nscbc_rk3(s, 1);
   rk3_step(s, og_erk_wgt3[0]);
   s->time += s->dt;
   og_update_state(s);
+  nscbc_calc_rhs(s, 2);
   calc_rhs_into(s, s->rhs1, s->rhs1_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs1, n, s->time, 2);
+  nscbc_rk3(s, 2);
   rk3_step(s, og_erk_wgt3[1]);
   s->time -= 0.5 * s->dt;
   og_update_state(s);
+  nscbc_calc_rhs(s, 3);
   calc_rhs_into(s, s->rhs2, s->rhs2_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs2, n, s->time, 3);
+  nscbc_rk3(s, 3);
   rk3_step(s, og_erk_wgt3[2]);
   s->time += 0.5 * s->dt;
   og_update_state(s);

@@ -100,6 +100,9 @@ typedef struct og_params {
   double bc_u[16][3];
   double bc_p[16], bc_rho[16], bc_q[16]; /* CBC p, rho, and mdot (MPT) or un (RUNP) */
   double bc_area[16];                    /* each zone's area, sum of |n_bf| in face order (CBC_MPT) */
+  /* 8 NSCBC_OUTLET_P p_ref (bc_p), sigma, L: per-face boundary states evolved
+   * with the step's RK3 weights */
+  double bc_sigma[16], bc_L[16];
 } og_params;
 
 typedef struct og_diag {
@@ -124,6 +127,10 @@ void og_flux_compact(og_rhs* flux, const og_fc* fa, const og_state* cv0, const o
 double og_fast_pow_posx(double x, double p);
 /* calcRiemannFlux (IdealGasSolverFlux.hpp:218-306): flux[5] through n_bf between
  * cv0 and cv1 */
+/* calcNscbcIgOutletRhs (BcCommon.hpp:419-470): rhs[5] = d/dt (rho_bc, p_bc,
+ * u_bc[3]); bc and cv = rho, p, u[3]; returns un_bc */
+double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                           double gamma, double L_ref, double sigma, double p_ref);
 void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,
                      double gammaL, double gammaR);
 /* calcSvd33 on a column-major A: U, V column-major, sigma unsorted; returns the

@@ -36,7 +36,7 @@ class og_params(C.Structure):
         ("full_gr", C.c_int), ("delta_r_dsm", C.c_double), ("nsc", C.c_int), ("res_time", C.c_int),
         ("bc_type", C.c_int * 16), ("bc_T", C.c_double * 16), ("bc_u", (C.c_double * 3) * 16),
         ("bc_p", C.c_double * 16), ("bc_rho", C.c_double * 16), ("bc_q", C.c_double * 16),
-        ("bc_area", C.c_double * 16),
+        ("bc_area", C.c_double * 16), ("bc_sigma", C.c_double * 16), ("bc_L", C.c_double * 16),
     ]
 
 
@@ -107,6 +107,9 @@ def lib(path=None):
         L.og_gr_func.argtypes = [C.c_double, C.c_double]
         L.og_flux_riemann.restype = None
         L.og_flux_riemann.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double]
+        L.og_nscbc_outlet_rhs.restype = C.c_double
+        L.og_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
+                                          C.c_double]
         _libs[path] = L
     return _libs[path]
 
@@ -126,6 +129,9 @@ def ref():
         R.ref_svd33.argtypes = [dp, dp, dp, dp, dp]
         R.ref_erk_wgt3.argtypes = [dp]
         R.ref_flux_riemann.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double]
+        R.ref_nscbc_outlet_rhs.restype = C.c_double
+        R.ref_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
+                                           C.c_double]
         _ref = R
     return _ref
 
@@ -166,6 +172,7 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
             p.bc_u[z][d] = b.u[d]
         p.bc_p[z], p.bc_rho[z], p.bc_q[z] = b.p, b.rho, b.q
         p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
+        p.bc_sigma[z], p.bc_L[z] = b.sigma, b.L
     for name, _ in og_params._fields_:
         if name.startswith("bc_"):
             continue

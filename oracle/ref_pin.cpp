@@ -5,10 +5,12 @@
 //   calcInternalFluxNew / calcInternalFluxSymmetricNew /
 //   calcInternalCompactFluxNewGrad   src/charles/ideal_gas/IdealGasSolverFlux.hpp:9-215
 //   calcRiemannFlux                  src/charles/ideal_gas/IdealGasSolverFlux.hpp:218-306
+//   calcNscbcIgOutletRhs             src/charles/BcCommon.hpp:419-470
 //   calcSvd33                        src/core/common/SvdNew.hpp:24
 //   MiscUtils::fast_pow_posx         src/core/common/MiscUtils.hpp:273-276
 //   RkWeights::erk_wgt3              src/charles/RkWeights.hpp:6-8
 #include "FlowSolver.hpp"
+#include "BcCommon.hpp"
 #include "IdealGasSolverFlux.hpp"
 #include "RkWeights.hpp"
 #include "SvdNew.hpp"
@@ -45,6 +47,14 @@ void ref_flux_riemann(double* flux, const double* n_bf, const double* cv0, const
   IdealGasRhs f;
   calcRiemannFlux(f, n_bf, a, b, gammaL, gammaR);
   rhs_out(f, flux);
+}
+
+// rhs[5] = d/dt (rho_bc, p_bc, u_bc[3]) of an NSCBC outlet face; bc = rho, p,
+// u[3] of the face; cv = rho, p, u[3] of its cell; returns un_bc
+double ref_nscbc_outlet_rhs(double* rhs, const double* unit_n, double delta, const double* bc, const double* cv,
+                            double gamma, double L_ref, double sigma, double p_ref) {
+  return calcNscbcIgOutletRhs(rhs, unit_n, delta, bc[0], bc + 2, bc[1], cv[0], cv + 2, cv[1], gamma, L_ref, sigma,
+                              p_ref);
 }
 
 // aux_out = u_avg[3], inv_v_avg, h_stag, p_avg

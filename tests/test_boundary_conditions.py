@@ -215,3 +215,38 @@ def test_cbc_inflow_closed_form_at_rest():
     # gas enters: the mass flux through the outward normal is negative, the
     # wall cells gain mass
     assert f[0] < 0.0 and np.all(r[lower, 0] > 0.0)
+
+
+def test_nscbc_outlet_parser():
+    p = parse_params(deck({"y0": "SLIP", "y1": "NSCBC_OUTLET_P 71.4 0.25 3.0"}))
+    b = p.bc[1]
+    assert (b.zone, b.type, b.p, b.sigma, b.L) == (b"y1", 8, 71.4, 0.25, 3.0)
+    p = parse_params(deck({"y1": "NSCBC_OUTLET_PRESSURE P_REF 70.0 L 2.0"}))
+    assert (p.bc[0].p, p.bc[0].sigma, p.bc[0].L) == (70.0, 0.2, 2.0)  # SIGMA 0.2 by default
+    p = parse_params(deck({"y1": "NSCBC_OUTLET_PRESSURE CONSTANT_PRESSURE P_REF 70.0"}))
+    assert (p.bc[0].sigma, p.bc[0].L) == (-1.0, 1.0)
+    for bad in ("y1 NSCBC_OUTLET_P 70 0.2", "y1 NSCBC_OUTLET_PRESSURE SIGMA 0.1", "y1 NSCBC_OUTLET_MDOT MDOT 1"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_nscbc_outlet_keeps_a_matching_uniform_stream(jitter):
+    # a uniform stream V through the channel, in at y0 (CBC_UPT of that state),
+    # out at y1 (NSCBC_OUTLET_P at the stream's pressure): the outlet faces'
+    # states equal their cells' and every characteristic rate vanishes, so the
+    # RHS is zero and five steps keep the stream (the faces' states too)
+    m = Mesh.channel(10, jitter=jitter, seed=8)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    text = deck({"y0": f"CBC_UPT 0 {V} 0 {p!r} {T}", "y1": f"NSCBC_OUTLET_P {p!r} 0.25 6.28"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    r = o.calc_rhs()
+    assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
+    np.testing.assert_allclose(u[:, [0, 2]], 0.0, atol=1e-10)

@@ -28,20 +28,17 @@ def test_library_exports_every_declared_symbol():
     assert sorted(ig.EXPORTS) == names
 
 
-def test_ctypes_structs_match_the_header(tmp_path):
-    # every field of the ctypes mirrors sits where the C compiler puts it: a
-    # missing field lets the library write past the Python-owned struct
-    structs = {"cg_bc": ig.cg_bc, "cg_params": ig.cg_params, "cg_diag": ig.cg_diag}
+def check_struct_layout(tmp_path, header, incdir, structs):
     lines = []
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f in py._fields_:
             lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
     src = tmp_path / "layout.c"
-    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"charles_gpu.h\"\nint main(void) {\n" +
+    src.write_text(f"#include <stddef.h>\n#include <stdio.h>\n#include \"{header}\"\nint main(void) {{\n" +
                    "\n".join(lines) + "\nreturn 0;\n}\n")
     exe = tmp_path / "layout"
-    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    subprocess.run(["gcc", "-std=c11", "-I", incdir, "-o", str(exe), str(src)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     seen = 0
     for line in out:
@@ -53,6 +50,19 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert int(value) == want, (cname, field, int(value), want)
         seen += 1
     assert seen == sum(len(p._fields_) + 1 for p in structs.values())
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    # every field of the ctypes mirrors sits where the C compiler puts it: a
+    # missing field lets the library write past the Python-owned struct
+    check_struct_layout(tmp_path, "charles_gpu.h", os.path.join(ROOT, "include"),
+                        {"cg_bc": ig.cg_bc, "cg_params": ig.cg_params, "cg_diag": ig.cg_diag})
+
+
+def test_oracle_ctypes_structs_match_its_header(tmp_path):
+    from oracle import oracle as orc
+    check_struct_layout(tmp_path, "ig_oracle.h", os.path.join(ROOT, "oracle"),
+                        {"og_params": orc.og_params, "og_mesh": orc.og_mesh, "og_diag": orc.og_diag})
 
 
 def test_param_defaults_are_the_reference_defaults():

@@ -2,9 +2,11 @@
 restatement; its wall terms are pinned by closed forms in
 test_boundary_conditions.py, its Riemann flux against the reference's
 calcRiemannFlux in test_oracle_pins.py): SLIP / SYMMETRY, WALL_ADIABATIC (with
-a TRANSLATE wall velocity), WALL_ISOTHERMAL and the characteristic conditions
-CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP on jittered channel meshes and on the
-reference's own Stitch mesh orifice.mles (six boundary zones).
+a TRANSLATE wall velocity), WALL_ISOTHERMAL, the characteristic conditions
+CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP and the NSCBC pressure outlet (per-face
+states evolved with the step, restored with it after a NaN) on jittered
+channel meshes and on the reference's own Stitch mesh orifice.mles (six
+boundary zones).
 
 FP64: p, T, sos and dudx bit-identical, the lagged coefficients within 1e-14,
 the stage RHS within 1e-12 of each component's largest magnitude, the CFL
@@ -38,6 +40,8 @@ WALLS = {
     "cbc_upt+runp": {"y0": f"CBC_UPT 0.1 0.2 0 {P0!r} 1.02", "y1": f"CBC_RUNP 1.0 -0.2 {P0 * 0.999!r}"},
     "cbc_mpt+rup": {"y0": f"CBC_MPT {0.15 * (2 * np.pi) ** 2!r} {P0!r} 0.98",
                     "y1": f"CBC_RUP 0.99 0.05 0.15 0 {P0 * 1.001!r} STATIONARY_FRAME"},
+    # in through a CBC, out through an NSCBC outlet relaxing to a lower pressure
+    "cbc_upt+nscbc": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0", "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
 }
 
 
@@ -168,7 +172,59 @@ def test_orifice_mesh_with_inflow_and_outflow_matches_oracle():
     assert o.diag()["sums"][0] > rho @ m.get("vol")
 
 
-@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp"])
+def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
+    # the orifice with a mass-flow inlet and a non-reflecting pressure outlet
+    # (per-face states evolved with the step), walls no-slip
+    m = Mesh.read_mles(ORIFICE)
+    p0 = 1.0 / 1.4
+    zones = dict(zip(m.zones, range(len(m.zones))))
+    area = np.bincount(m.get("zone_bf"), weights=m.get("area_bf"), minlength=len(m.zones))
+    mdot = float(0.1 * area[zones["inlet"]])
+    walls = {"inlet": f"CBC_MPT {mdot!r} {p0!r} 1.0", "outlet": f"NSCBC_OUTLET_PRESSURE P_REF {p0 * 0.99!r} L 0.05",
+             "oriface-pipe": "WALL_ADIABATIC", "wall": "WALL_ADIABATIC", "oriface-wall-0": "WALL_ADIABATIC",
+             "oriface-wall-1": "WALL_ADIABATIC"}
+    rho = np.ones(m.ncv)
+    u = np.zeros((m.ncv, 3))
+    rhoE = np.full(m.ncv, p0 / 0.4)
+    text = (f"GAMMA 1.4\nP_REF {p0!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF 1e-5\nSGS_MODEL VREMAN\nTIMESTEP CFL 0.8\n" +
+            "".join(f"{z} {w}\n" for z, w in walls.items()))
+    g, o = pair(m, text, (rho, u, rhoE))
+    check_state_and_rhs(g, o, "orifice nscbc")
+    check_history(g, o, 20, "orifice nscbc")
+
+
+def test_nscbc_faces_roll_back_with_the_flow():
+    # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
+    # faces' states with it (Nscbc::storeData / restoreData), and goes on with
+    # dt * 0.8, as the oracle
+    m = Mesh.channel(10, jitter=0.3, seed=12)
+    g, o = pair(m, deck(WALLS["cbc_upt+nscbc"], "NONE") + "CHECK_INTERVAL 1\n", channel_state(m))
+    g.advance_solution(3)
+    for _ in range(3):
+        o.step()
+    g.store_data()
+    o.store()
+    g.advance_solution(2)
+    for _ in range(2):
+        o.step()
+    rho, u, rhoE = g.get_state()
+    rho[37] = np.nan
+    for x in (g, o):
+        x.set_state(rho, u, rhoE)
+        x.sync_post_state()
+    g.advance_solution(4)
+    # the reference run loop: 4 net steps, a rollback steps on from the restored step
+    target, rolled = o.diag()["step"] + 4, 0
+    while o.diag()["step"] < target:
+        rolled += o.step() != 0
+    assert rolled == 1
+    d, od = g.report(), o.diag()
+    assert d.step == od["step"] and d.nan_flag == 0
+    assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"]
+    check_history(g, o, 5, "after rollback")
+
+
+@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]

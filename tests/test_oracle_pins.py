@@ -1,9 +1,10 @@
 """Pins of the oracle (CPU restatement) against the reference's own code and
 against closed forms.
 
-* Face fluxes, fast_pow_posx, the RK3 weights: bit-identical to the
-  reference's header code compiled from its sources (oracle/_ref, built by
-  oracle/Makefile where the reference checkout exists).
+* Face fluxes, the boundary Riemann flux, the NSCBC outlet rates,
+  fast_pow_posx, the RK3 weights: bit-identical to the reference's header code
+  compiled from its sources (oracle/_ref, built by oracle/Makefile where the
+  reference checkout exists).
 * gr_func (IdealGasSolver.cpp:736-745 lives in a .cpp and is not compiled
   here): pinned by its closed-form properties.
 """
@@ -269,3 +270,46 @@ def test_riemann_flux_closed_forms():
         b2[0:3] -= 8 * max(ca, cb) * n
         L.og_flux_riemann(dp(f), dp(nb), dp(a2), dp(b2), 1.4, 1.4)
         np.testing.assert_allclose(f, phys(b2, nb), rtol=1e-12, atol=1e-12 * np.abs(phys(b2, nb)).max())
+
+
+@needs_ref
+def test_nscbc_outlet_rhs_bit_identical_to_reference():
+    # calcNscbcIgOutletRhs (BcCommon.hpp:419-470) on random faces: subsonic and
+    # supersonic outflow (the Ma >= 1 branch), inflow clipped to un = 0, the
+    # relaxed (sigma >= 0) and constant-pressure (sigma < 0) forms
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(31)
+    for k in range(3000):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        bc = np.array([rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0), *rng.normal(scale=0.8, size=3)])
+        cv = bc * (1.0 + 0.05 * rng.normal(size=5))
+        if k % 5 == 0:
+            bc[2:] = 2.5 * n  # supersonic outflow
+        sigma = -1.0 if k % 7 == 0 else rng.uniform(0.0, 1.0)
+        a, b = np.zeros(5), np.zeros(5)
+        ua = L.og_nscbc_outlet_rhs(dp(a), dp(n), 0.03, dp(bc), dp(cv), 1.4, 0.7, sigma, 1.1)
+        ub = R.ref_nscbc_outlet_rhs(dp(b), dp(n), 0.03, dp(bc), dp(cv), 1.4, 0.7, sigma, 1.1)
+        assert ua == ub
+        np.testing.assert_array_equal(a, b)
+
+
+def test_nscbc_outlet_rhs_closed_forms():
+    # at rest with the face state equal to its cell's, only the incoming
+    # characteristic acts: L1 = sigma c (p_bc - p_ref) / L, so dp_bc/dt =
+    # -L1 / 2 and drho_bc/dt = -L1 / (2 c^2) (isentropic), and u_bc moves along
+    # n by L1 / (2 rho c); with p_bc = p_ref the face is at rest in every rate
+    L = orc.lib()
+    n = np.array([0.0, 0.6, 0.8])
+    gamma, sigma, Lr, rho, p, p_ref = 1.4, 0.3, 0.5, 1.2, 1.05, 1.0
+    c = math.sqrt(gamma * p / rho)
+    bc = np.array([rho, p, 0.0, 0.0, 0.0])
+    r = np.zeros(5)
+    L.og_nscbc_outlet_rhs(dp(r), dp(n), 0.1, dp(bc), dp(bc.copy()), gamma, Lr, sigma, p_ref)
+    L1 = sigma * c / Lr * (p - p_ref)
+    np.testing.assert_allclose(r[1], -0.5 * L1, rtol=1e-14)
+    np.testing.assert_allclose(r[0], -0.5 * L1 / c ** 2, rtol=1e-14)
+    np.testing.assert_allclose(r[2:], 0.5 * L1 * n / c / rho, rtol=1e-14, atol=1e-300)
+    bc[1] = p_ref
+    L.og_nscbc_outlet_rhs(dp(r), dp(n), 0.1, dp(bc), dp(bc.copy()), gamma, Lr, sigma, p_ref)
+    assert np.all(r == 0.0)
