@@ -371,19 +371,26 @@ def ensemble_leg(count, steps, seed, tune_steps=2, sample_count=16):
         chosen = [ln for ln in report.splitlines() if ln.startswith("chosen")]
         log(f"{name}: tuner {chosen[0] if chosen else report}")
         left = dict(zip(ids, horizons))
+        base = {iid: b.diag(iid).step for iid in ids}  # the horizon counts net steps after tuning
         done, kes[name] = {}, {}
         tstart = time.perf_counter()
         while left:
             k = min(left.values())
             b.step(k)
             t = time.perf_counter() - tstart
-            for iid in [i for i in left if left[i] == k]:
-                kes[name][iid] = b.diag(iid)
+            for iid in list(left):
+                left[iid] -= k
+                if left[iid] > 0:
+                    continue
+                d = b.diag(iid)
+                target = base[iid] + horizons[ids.index(iid)]
+                if d.step < target:
+                    left[iid] = target - d.step  # a NaN rollback: net steps, as the reference run loop
+                    continue
+                kes[name][iid] = d
                 done[iid] = t
                 b.remove(iid)
                 del left[iid]
-            for iid in left:
-                left[iid] -= k
         sec = time.perf_counter() - tstart
         b.close()
         lat = [done[i] for i in ids]

@@ -399,8 +399,11 @@ int cg_layout_bytes(const cg_layout* L, int64_t* bytes);
 void cg_layout_destroy(cg_layout* L);
 int cg_instance_create_on(const cg_layout* L, const cg_params* p, const double* rho, const double* u,
                           const double* rhoE, cg_instance** out);
-/* nsteps of setDt + advanceSolution for every instance, NaN checks at each
- * instance's CHECK_INTERVAL. */
+/* nsteps iterations of setDt + advanceSolution for every instance, NaN checks at
+ * each instance's CHECK_INTERVAL. A NaN rollback takes that instance's step counter
+ * back (the older snapshot) and the call does not make up the lost steps: a caller
+ * that runs solves to a horizon reads the step from cg_batch_get_diag (the serving
+ * loop does, as the reference's run loop steps to NSTEPS). */
 int cg_batch_step(cg_batch* b, int nsteps);
 int cg_batch_get_diag(cg_batch* b, int id, cg_diag* d);
 int cg_batch_get_state(cg_batch* b, int id, double* rho, double* u, double* rhoE);

@@ -248,6 +248,11 @@ class Server:
                 if a[1] == 0:
                     d = batch.diag(iid)
                     r = a[0]
+                    if d.step < r.spec.steps:
+                        # a NaN rollback took the solve back: the horizon counts net
+                        # steps, as the reference run loop does
+                        a[1] = r.spec.steps - d.step
+                        continue
                     results.append(Result(r.rid, r.arrival, a[2], now(), a[4], r.spec.steps, d.ke, d.enstrophy, d.time))
                     batch.remove(iid)
                     used -= a[3]

@@ -40,7 +40,7 @@ class FakeBatch:
 
     def diag(self, i):
         req, steps = self.members[i]
-        return SimpleNamespace(ke=float(req.rid), enstrophy=float(steps), time=float(steps))
+        return SimpleNamespace(ke=float(req.rid), enstrophy=float(steps), time=float(steps), step=steps)
 
     def remove(self, i):
         self.members.pop(i)
@@ -99,3 +99,25 @@ def test_poisson_requests_and_summary():
     res = [Result(i, float(i), float(i), float(i) + 2.0, 1, 1, 0.0, 0.0, 0.0) for i in range(10)]
     s = summarize(res)
     assert s["tts_p50_s"] == 2.0 and s["solves_per_s"] == 10 / 11.0
+
+
+def test_server_counts_net_steps_after_a_rollback():
+    # a solve that a NaN rollback takes back 3 steps is stepped on to its
+    # horizon in net steps, as the reference run loop does
+    class RollbackBatch(FakeBatch):
+        rolled = False
+
+        def step(self, k):
+            super().step(k)
+            if not self.rolled:
+                for m in self.members.values():
+                    if m[1] >= 6:
+                        m[1] -= 3
+                        RollbackBatch.rolled = True
+                        break
+
+    reqs = [Request(0, 0.0, Spec(n=10, mesh_seed=1, ic="tgv", Re=1e3, Ma=0.1, ic_seed=1, steps=20))]
+    res, info = Server(None, max_chunk=4, batch_factory=RollbackBatch, prepare=FakeInstance).run(reqs)
+    assert RollbackBatch.rolled
+    assert res[0].enstrophy == 20  # the device's own step count at retirement
+    assert info["batch_steps"] == 23
