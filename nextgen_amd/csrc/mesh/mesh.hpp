@@ -41,6 +41,14 @@ struct Voronoi {
   // per boundary face as a mesh file carries them (empty: computed from the
   // geometry, StripedMesh.hpp:2760-2794)
   std::vector<double> aod_bf_file, gij_bf_file;  // [nbf], [nbf][3][3]
+  // the domain's largest x, y, z, sqrt(y^2 + z^2) and sqrt(x^2 + y^2) over its
+  // nodes (Sponge::initData, IdealGasSolverBcs.cpp:4094-4114): from a mesh
+  // file's node records, or the box [0, L] of a mesh built here
+  bool has_x_max = false;
+  double x_max[5] = {0, 0, 0, 0, 0};
+  // nodes [k][3] that attain those maxima (at most 5): a written mesh file
+  // carries them as its node records, so reading it back gives the same extents
+  std::vector<double> x_ext_nodes;
 };
 
 struct OperatorParams {
@@ -148,6 +156,7 @@ struct SlesData {
   double time = 0.0, dt = 0.0;
   const double *rho = nullptr, *u = nullptr, *rhoE = nullptr, *p = nullptr, *T = nullptr;  // p, T optional
   const double *lijmij = nullptr, *mijmij = nullptr;  // DSM_LOCAL relaxation state (optional)
+  const double *T_sponge = nullptr, *u_sponge = nullptr;  // a sponge's relaxed state (optional)
   std::vector<std::string> sc_names;                  // passive scalars by name (optional)
   std::vector<const double*> sc;
 };
@@ -155,7 +164,7 @@ struct SlesRead {
   int step = 0;
   double time = 0.0, dt = 0.0;
   bool have_step = false, have_time = false, have_dt = false;
-  std::vector<double> rho, u, rhoE, p, T, lijmij, mijmij;
+  std::vector<double> rho, u, rhoE, p, T, lijmij, mijmij, T_sponge, u_sponge;
   std::vector<std::pair<std::string, std::vector<double>>> other;  // other per-cell records by name
 };
 std::string write_sles(const std::string& path, const SlesData& s);

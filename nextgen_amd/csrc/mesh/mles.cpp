@@ -403,6 +403,30 @@ std::string voronoi_from_mles(const Mles& f, Voronoi& m) {
     if (f.area_over_delta_bf.size() == size_t(nbf)) m.aod_bf_file = f.area_over_delta_bf;
     if (f.Gij_bf.size() == size_t(9 * nbf)) m.gij_bf_file = f.Gij_bf;
   }
+  // the domain extents over the node records (Sponge::initData,
+  // IdealGasSolverBcs.cpp:4104-4113), when the file holds them
+  if (!f.x_no.empty()) {
+    double mx[5] = {-1.0E-20, -1.0E-20, -1.0E-20, -1.0E-20, -1.0E-20};
+    size_t arg[5] = {0, 0, 0, 0, 0};
+    for (size_t i = 0; i + 2 < f.x_no.size(); i += 3) {
+      const double* x = &f.x_no[i];
+      const double v[5] = {x[0], x[1], x[2], std::sqrt(x[1] * x[1] + x[2] * x[2]),
+                           std::sqrt(x[0] * x[0] + x[1] * x[1])};
+      for (int k = 0; k < 5; ++k)
+        if (mx[k] < v[k]) {
+          mx[k] = v[k];
+          arg[k] = i;
+        }
+    }
+    for (int k = 0; k < 5; ++k) m.x_max[k] = mx[k];
+    m.has_x_max = true;
+    m.x_ext_nodes.clear();
+    for (int k = 0; k < 5; ++k) {
+      bool seen = false;
+      for (int j = 0; j < k; ++j) seen = seen || arg[j] == arg[k];
+      if (!seen) m.x_ext_nodes.insert(m.x_ext_nodes.end(), &f.x_no[arg[k]], &f.x_no[arg[k]] + 3);
+    }
+  }
   return "";
 }
 
@@ -499,12 +523,15 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
                   m.x_bf.size() != size_t(3 * nbf) || m.bf_zone_i.size() != m.bf_zone_name.size() + 1))
     return "inconsistent boundary faces";
 
+  // node records: only the nodes that attain the domain's extents
+  const int64_t nno = m.has_x_max ? int64_t(m.x_ext_nodes.size() / 3) : 0;
+
   Writer w(path);
   if (!w.ok()) return "cannot create " + path;
   w.start(kMagic, 5);
   {
     RecHeader h = w.header("no_fa_bf_cv_counts", kCounts, 2 * 27 * 8);
-    set_count(h.idata + 0, 0);  // nno: geometry-only file, no node records
+    set_count(h.idata + 0, nno);  // nno: the extents' nodes, no connectivity
     set_count(h.idata + 2, nbf);
     set_count(h.idata + 4, nfa);
     set_count(h.idata + 6, ncv);
@@ -559,6 +586,7 @@ std::string write_mles(const Voronoi& m, const std::string& path) {
     set_count(h.idata + 4, 0);
     w.put_header(h);
   }
+  if (nno > 0) w.put_array("x_no", kXNo, m.x_ext_nodes.data(), nno, 3);
   w.put_array("vol_cv", kCvD1, m.vol.data(), ncv, 1);
   w.put_array("x_vv", kCvD2, m.x_vv.data(), ncv, 3);
   if (m.r_vv.size() == size_t(ncv)) w.put_array("r_vv", kCvD1, m.r_vv.data(), ncv, 1);
@@ -599,6 +627,9 @@ std::string write_sles(const std::string& path, const SlesData& s) {
   if (s.T) w.put_array("T", kCvD1, s.T, ncv, 1);
   if (s.lijmij) w.put_array("lijmij", kCvD1, s.lijmij, ncv, 1);
   if (s.mijmij) w.put_array("mijmij", kCvD1, s.mijmij, ncv, 1);
+  // Sponge's registered cv data (IdealGasSolverBcs.cpp:4089-4090)
+  if (s.T_sponge) w.put_array("T_sponge", kCvD1, s.T_sponge, ncv, 1);
+  if (s.u_sponge) w.put_array("u_sponge", kCvD2, s.u_sponge, ncv, 3);
   for (size_t i = 0; i < s.sc_names.size() && i < s.sc.size(); ++i)
     w.put_array(s.sc_names[i].c_str(), kCvD1, s.sc[i], ncv, 1);
   w.put_header(w.header("EOF", kEof, 0));
@@ -647,6 +678,8 @@ std::string read_sles(const std::string& path, int64_t ncv, SlesRead& s) {
       if (h.id == kCvD1 && name == "T") dst = &s.T;
       if (h.id == kCvD1 && name == "lijmij") dst = &s.lijmij;
       if (h.id == kCvD1 && name == "mijmij") dst = &s.mijmij;
+      if (h.id == kCvD1 && name == "T_sponge") dst = &s.T_sponge;
+      if (h.id == kCvD2 && name == "u_sponge") dst = &s.u_sponge;
       if (h.id == kCvD1 && !dst) {
         s.other.emplace_back(name, std::vector<double>());
         dst = &s.other.back().second;

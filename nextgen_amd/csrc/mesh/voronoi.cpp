@@ -516,9 +516,25 @@ std::string build_jittered(Voronoi& m, int nx, int ny, int nz, double Lx, double
   return build_clipped(m, nx, ny, nz, Lx, Ly, Lz, jitter, seed, nthreads, false);
 }
 
+// the domain extents of a box [0, L]: its corner node (Lx, Ly, Lz) holds every maximum
+static void set_box_x_max(Voronoi& m) {
+  const double* L = m.L;
+  m.x_max[0] = L[0];
+  m.x_max[1] = L[1];
+  m.x_max[2] = L[2];
+  m.x_max[3] = std::sqrt(L[1] * L[1] + L[2] * L[2]);
+  m.x_max[4] = std::sqrt(L[0] * L[0] + L[1] * L[1]);
+  m.has_x_max = true;
+  m.x_ext_nodes.assign(L, L + 3);
+}
+
 std::string build_channel(Voronoi& m, int nx, int ny, int nz, double Lx, double Ly, double Lz, double jitter,
                           uint64_t seed, int nthreads) {
-  if (jitter > 0.0) return build_clipped(m, nx, ny, nz, Lx, Ly, Lz, jitter, seed, nthreads, true);
+  if (jitter > 0.0) {
+    std::string e = build_clipped(m, nx, ny, nz, Lx, Ly, Lz, jitter, seed, nthreads, true);
+    if (e.empty()) set_box_x_max(m);
+    return e;
+  }
   // the uniform lattice of box cells: periodic faces in x and z, interior faces
   // in y, a wall face on each cell of the first and last layer
   std::string e = build_lattice(m, nx, ny, nz, Lx, Ly, Lz);
@@ -553,6 +569,7 @@ std::string build_channel(Voronoi& m, int nx, int ny, int nz, double Lx, double 
     m.bf_zone_i[z + 1] = int(m.cvobf.size());
   }
   m.nbf = int(m.cvobf.size());
+  set_box_x_max(m);
   return "";
 }
 

@@ -117,6 +117,11 @@ struct Layout {
   int lds_max = 0;            // the device's LDS bytes per workgroup
   std::vector<std::string> bf_zone_names;  // the mesh's boundary zones (index = zone)
   std::vector<double> zone_area;           // sum of each zone's |n_bf| in face order
+  // meshes with boundary faces: cell centres [ncv][3] in device order and the
+  // domain's extents (max x, y, z, sqrt(y^2+z^2), sqrt(x^2+y^2)), for a sponge
+  std::vector<double> x_cv;
+  bool has_x_max = false;
+  double x_max[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 };
 
 // LDS bytes of the scalar pass's halo variant: the flux pass's records of a
@@ -138,8 +143,10 @@ struct HostInst {
   double* store_buf = nullptr;       // older snapshot
   double* store_recent = nullptr;    // newer snapshot
   size_t snap_n = 0;                 // doubles per snapshot
-  size_t snap_bc = 0;                // of them, the NSCBC faces' states at the end (5 per boundary face)
+  size_t snap_bc = 0;                // of them, the NSCBC faces' states (5 per boundary face)
+  size_t snap_sp = 0;                // and, at the end, the sponge's relaxed T, u (4 per cell)
   bool nscbc = false;                // some zone is an NSCBC outlet
+  bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
   bool have_store = false;
@@ -150,6 +157,8 @@ struct HostInst {
   const int* ext_of = nullptr;       // device cell -> mesh cell (its layout's)
   Alloc mem;                         // its own device memory (state, work, snapshots)
   int id = -1;
+  bool sponge_kept = false;          // the sponge's T, u set from a data file: the next
+                                     // cg_init_complete keeps them (Sponge::initialHook)
   bool dsm_kept = false;             // DSM_LOCAL L:M, M:M set from a data file: the next
                                      // cg_init_complete keeps them
   cg_source_fn hook = nullptr;       // addSourceHook: the stages run unfused while it is set
@@ -567,6 +576,13 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
       for (int c = 0; c < ncv; ++c) r_vv[c] = rv[ext_of[c]];
     rc |= upload(s, h, &g.r_vv, r_vv);
   }
+  if (o.nbf > 0) {
+    Lay->x_cv.resize(3 * (size_t)ncv);
+    for (int c = 0; c < ncv; ++c)
+      for (int d = 0; d < 3; ++d) Lay->x_cv[3 * (size_t)c + d] = m->vor.x_cv[3 * (size_t)ext_of[c] + d];
+    Lay->has_x_max = m->vor.has_x_max;
+    for (int d = 0; d < 5; ++d) Lay->x_max[d] = m->vor.x_max[d];
+  }
   // boundary faces in device cell order (each cell's in zone-major order)
   g.nbf = o.nbf;
   Lay->bf_zone_names = m->vor.bf_zone_name;
@@ -876,6 +892,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   b.g = lay->g;
   hi->prm = *p;
   fill_dev_params(hi);
+  hi->P.sponge_zone = -1;
   // every boundary zone of the mesh needs its condition (registerBoundaryConditions,
   // IdealGasSolver.hpp:162-238); conditions for zones the mesh lacks are unused
   {
@@ -900,8 +917,20 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_sigma[z] = p->bc[k].sigma;
       hi->P.bc_L[z] = p->bc[k].L;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
+      if (p->bc[k].type == CG_BC_SPONGE) {
+        // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
+        // u_sponge are solver-wide cell data, so one sponge zone per solve
+        if (hi->sponge) return s->set(CG_EINVAL, "more than one SPONGE zone");
+        hi->sponge = true;
+        hi->P.sponge_zone = (int)z;
+        hi->P.sponge_p = p->bc[k].p < 0.0 ? p->p_ref : p->bc[k].p;
+        hi->P.sponge_trelax = p->bc[k].T;
+      }
     }
     if (!missing.empty()) return s->set(CG_EINVAL, "no boundary condition for zone(s) " + missing);
+    if (hi->sponge && !lay->has_x_max)
+      return s->set(CG_EINVAL, "a SPONGE zone needs the mesh's extents (node coordinates or box), which a mesh "
+                               "from raw arrays does not carry");
     if (lay->g.nbf > 0 && (p->sgs_model == CG_SGS_DSM_LOCAL || p->nsc > 0))
       return s->set(CG_EINVAL, "SGS_MODEL DSM_LOCAL and passive scalars need a periodic mesh (their boundary "
                                "terms are not on this path)");
@@ -959,6 +988,43 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     hi->snap_bc = 5 * nbf;
     hi->snap_n += hi->snap_bc;
   }
+  std::vector<double> sp_coeff;
+  if (hi->sponge) {
+    // Sponge::addBoundaryFlux's profile (IdealGasSolverBcs.cpp:4208-4282): per
+    // cell strength x_sp^2 with x_sp = (x - (Lmax - length)) / length where
+    // x > Lmax - length (x: the cell's x, y or z, or its distance from the x or
+    // z axis; Lmax: the domain's extent in it), -1 outside the sponge
+    const cg_bc* bc = nullptr;
+    for (int i = 0; i < p->nbc; ++i)
+      if (p->bc[i].type == CG_BC_SPONGE && lay->bf_zone_names[(size_t)hi->P.sponge_zone] == p->bc[i].zone)
+        bc = &p->bc[i];
+    const int mode = bc->mode;
+    const double Lmax = lay->x_max[mode], length = bc->L, strength = bc->q;
+    sp_coeff.assign(ncv, -1.0);
+    for (int c = 0; c < ncv; ++c) {
+      const double* x = &lay->x_cv[3 * (size_t)c];
+      double r;
+      if (mode < 3) {
+        r = x[mode];
+      } else if (mode == 3) {
+        const double yy = x[1] * x[1];
+        const double zz = x[2] * x[2];
+        r = std::sqrt(yy + zz);
+      } else {
+        const double xx = x[0] * x[0];
+        const double yy = x[1] * x[1];
+        r = std::sqrt(xx + yy);
+      }
+      if (r > Lmax - length) {
+        const double x_sp = (r - (Lmax - length)) / length;
+        sp_coeff[c] = strength * x_sp * x_sp;
+      }
+    }
+    rc |= dev_alloc(s, h, &w.sponge, 4 * (size_t)ncv);
+    rc |= dev_alloc(s, h, &w.sponge_coeff, (size_t)ncv);
+    hi->snap_sp = 4 * (size_t)ncv;
+    hi->snap_n += hi->snap_sp;
+  }
   rc |= dev_alloc(s, h, &hi->store_buf, hi->snap_n);
   rc |= dev_alloc(s, h, &hi->store_recent, hi->snap_n);
   rc |= dev_alloc(s, h, &hi->rhs_tmp, 5 * (size_t)ncv);
@@ -1001,6 +1067,11 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     // Nscbc::initData: zero rates; the states are set from the cells by initComplete
     rc |= cgk::launch_fill(w.nscbc, 0.0, 5 * (int64_t)lay->g.nbf, stream);
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
+  }
+  if (hi->sponge) {
+    // the relaxed state is set from the cells by initComplete
+    rc |= cgk::launch_fill(w.sponge, 0.0, 4 * (int64_t)ncv, stream);
+    HIPCHK(s, hipMemcpy(w.sponge_coeff, sp_coeff.data(), sizeof(double) * ncv, hipMemcpyHostToDevice));
   }
   if (rc) return s->set(CG_EHIP, "initial fill failed");
   HIPCHK(s, hipStreamSynchronize(stream));
@@ -1173,11 +1244,16 @@ int refresh(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
 }
 
 // Nscbc::initialHook (IdealGasSolverBcs.cpp:3630-3649): the NSCBC outlet faces'
-// states from their cells (initComplete; a restart file does not carry them)
+// states from their cells; Sponge::initialHook (IdealGasSolverBcs.cpp:4311-4346):
+// the sponge's T and u from the cells (initComplete; a restart file does not
+// carry them)
 template <class R>
 int nscbc_init(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
-  if (!I->nscbc) return 0;
-  if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, stream)) return launch_fail(s, "nscbc init", le);
+  const bool sponge = I->sponge && !I->sponge_kept;
+  I->sponge_kept = false;
+  if (!I->nscbc && !sponge) return 0;
+  if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, sponge, stream))
+    return launch_fail(s, "boundary state init", le);
   return 0;
 }
 
@@ -1189,9 +1265,15 @@ int snapshot_copy(Err* s, Instance<R>* I, int cur, double* dst, bool to_snapshot
   const cgdev::StateT<R>& st = I->b.st[cur];
   if (I->snap_bc) {
     // Nscbc::storeData / restoreData: the NSCBC faces' states after the flow's
-    double* bc = dst + (I->snap_n - I->snap_bc);
+    double* bc = dst + (I->snap_n - I->snap_sp - I->snap_bc);
     HIPCHK(s, hipMemcpyAsync(to_snapshot ? bc : I->b.w.nscbc, to_snapshot ? I->b.w.nscbc : bc,
                              sizeof(double) * I->snap_bc, dir, stream));
+  }
+  if (I->snap_sp) {
+    // Sponge::storeData / restoreData (IdealGasSolverBcs.cpp:4130-4159): its T, u
+    double* sp = dst + (I->snap_n - I->snap_sp);
+    HIPCHK(s, hipMemcpyAsync(to_snapshot ? sp : I->b.w.sponge, to_snapshot ? I->b.w.sponge : sp,
+                             sizeof(double) * I->snap_sp, dir, stream));
   }
   if constexpr (std::is_same<R, float>::value) {
     const size_t n = (size_t)I->ncv * 5;
@@ -1672,10 +1754,45 @@ int cg_params_parse(cg_params* p, const char* text) {
           }
         }
         if (b.p < 0.0) return bad("NSCBC_OUTLET_PRESSURE needs P_REF");
+      } else if (type == "SPONGE") {
+        // parseSpongeParam (SpongeCommon.hpp:11-63): RELAX_TIME, STRENGTH, TYPE,
+        // LENGTH, EPS_P (unused), SPONGE_PRESSURE (P_REF by default)
+        b.type = CG_BC_SPONGE;
+        b.mode = -1;
+        b.T = 0.0;
+        b.q = 0.0;
+        b.L = 0.0;
+        b.p = -1.0;
+        double eps_p = 0.01;
+        for (size_t i = 2; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          if (opt == "RELAX_TIME") {
+            if (!num(++i, &b.T)) return bad("expected RELAX_TIME <t>");
+          } else if (opt == "STRENGTH") {
+            if (!num(++i, &b.q)) return bad("expected STRENGTH <s>");
+          } else if (opt == "LENGTH") {
+            if (!num(++i, &b.L)) return bad("expected LENGTH <l>");
+          } else if (opt == "EPS_P") {
+            if (!num(++i, &eps_p)) return bad("expected EPS_P <e>");
+          } else if (opt == "SPONGE_PRESSURE") {
+            if (!num(++i, &b.p) || b.p < 0.0) return bad("SPONGE_PRESSURE: nonpositive value specified");
+          } else if (opt == "TYPE") {
+            const std::string t = ++i < tok.size() ? upper(tok[i]) : "";
+            b.mode = t == "L_X" ? 0 : t == "L_Y" ? 1 : t == "L_Z" ? 2 : t == "L_RX" ? 3 : t == "L_RZ" ? 4 : -2;
+            if (b.mode == -2) return bad("unrecognized sponge type " + t);
+          } else {
+            return bad("unrecognized sponge token " + opt);
+          }
+        }
+        if (eps_p < 0.0) return bad("EPS_P: negative value specified");
+        if (!(b.L > 0.0)) return bad("LENGTH: nonpositive value specified or unspecified");
+        if (b.q < 0.0) return bad("STRENGTH: negative value specified");
+        if (!(b.T > 0.0)) return bad("RELAX_TIME: nonpositive value specified or unspecified");
+        if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P and "
-                   "NSCBC_OUTLET_PRESSURE are)");
+                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P, "
+                   "NSCBC_OUTLET_PRESSURE and SPONGE are)");
       }
       int slot = -1;
       for (int i = 0; i < p->nbc; ++i)
@@ -1892,6 +2009,11 @@ int cg_mesh_get(const cg_mesh* m, const char* name, void* out, int64_t nbytes) {
     dtmp.assign(v.L, v.L + 3);
     src = dtmp.data();
     sz = 3 * sizeof(double);
+  } else if (n == "domain_max") {
+    if (!v.has_x_max) return fail(CG_EINVAL, "the mesh has no node coordinates or box extents");
+    dtmp.assign(v.x_max, v.x_max + 5);
+    src = dtmp.data();
+    sz = 5 * sizeof(double);
   } else if (n == "max_gcl") {
     dtmp.assign(1, o.max_gcl);
     src = dtmp.data();
@@ -2119,8 +2241,9 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
   return with_inst(s, [&](auto& E, auto* I) -> int {
     (void)E;
     // the hooked stage is an RHS pass and a separate update, which do not
-    // carry the NSCBC faces' own RK update
+    // carry the NSCBC faces' own RK update or the sponge's relaxation
     if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC outlet zones");
+    if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;
     if (!fn) std::vector<double>().swap(I->hook_buf);
@@ -2139,6 +2262,13 @@ int cg_get_field(cg_solver* s, const char* name, double* out, int64_t n) {
     if (nm == "dudx") {
       if (n != 9 * (int64_t)ncv) return s->set(CG_EINVAL, "dudx needs 9*ncv values");
       return get_rec(s, ncv, I->ext_of, w.aux, cgdev::A_N, cgdev::A_D, 9, out, s->stream);
+    }
+    if (nm == "T_sponge" || nm == "u_sponge") {
+      // the sponge's relaxed state (cv data T_sponge, u_sponge)
+      if (!w.sponge) return s->set(CG_EINVAL, nm + " exists with a SPONGE zone only");
+      const int nf = nm == "T_sponge" ? 1 : 3;
+      if (n != nf * (int64_t)ncv) return s->set(CG_EINVAL, nm + (nf == 1 ? " needs ncv values" : " needs 3*ncv values"));
+      return get_rec(s, ncv, I->ext_of, w.sponge, 4, nf == 1 ? 0 : 1, nf, out, s->stream);
     }
     if (n != ncv) return s->set(CG_EINVAL, "field needs ncv values");
     struct { const char* nm; int light; int f; int prim; } recf[] = {
@@ -2350,6 +2480,15 @@ int cg_write_sles(cg_solver* s, const char* path) {
       d.lijmij = lm.data();
       d.mijmij = mm.data();
     }
+    std::vector<double> ts, us;
+    if (I->b.w.sponge) {
+      ts.resize(n);
+      us.resize(3 * n);
+      if (int rc = get_rec(s, (int)n, I->ext_of, I->b.w.sponge, 4, 0, 1, ts.data(), s->stream)) return rc;
+      if (int rc = get_rec(s, (int)n, I->ext_of, I->b.w.sponge, 4, 1, 3, us.data(), s->stream)) return rc;
+      d.T_sponge = ts.data();
+      d.u_sponge = us.data();
+    }
     const std::string e = cgmesh::write_sles(path, d);
     return e.empty() ? 0 : s->set(CG_EIO, e);
   });
@@ -2389,6 +2528,18 @@ int cg_read_sles(cg_solver* s, const char* path) {
       HIPCHK(s, hipMemcpy(I->b.w.lijmij, lm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
       HIPCHK(s, hipMemcpy(I->b.w.mijmij, mm.data(), sizeof(double) * n, hipMemcpyHostToDevice));
       I->dsm_kept = true;
+    }
+    if (I->b.w.sponge && !r.T_sponge.empty() && !r.u_sponge.empty()) {
+      // the sponge's registered T_sponge, u_sponge in device cell order
+      const size_t n = (size_t)I->ncv;
+      std::vector<double> sp(4 * n);
+      for (size_t c = 0; c < n; ++c) {
+        const size_t e = (size_t)I->ext_of[c];
+        sp[4 * c] = r.T_sponge[e];
+        for (int i = 0; i < 3; ++i) sp[4 * c + 1 + i] = r.u_sponge[3 * e + i];
+      }
+      HIPCHK(s, hipMemcpy(I->b.w.sponge, sp.data(), sizeof(double) * sp.size(), hipMemcpyHostToDevice));
+      I->sponge_kept = true;
     }
     return write_info(s, I, s->info_host, s->stream);
   });

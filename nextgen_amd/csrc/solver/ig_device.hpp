@@ -141,12 +141,15 @@ struct WorkT {
   // NSCBC outlet faces (null without them): per boundary face (device order)
   // its state rho, p, u[3] and the rates of the step's three stages [3][5]
   double *nscbc, *nscbc_rhs;
+  // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
+  // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
+  double *sponge, *sponge_coeff;
 };
 
 constexpr int kMaxBcZones = 16;
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
-  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8
+  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9
 };
 
 struct Params {
@@ -167,6 +170,8 @@ struct Params {
   double bc_p[kMaxBcZones], bc_rho[kMaxBcZones], bc_q[kMaxBcZones];
   double bc_area[kMaxBcZones];  // the zone's area (sum of its faces' |n_bf|), CBC_MPT
   double bc_sigma[kMaxBcZones], bc_L[kMaxBcZones];  // NSCBC outlet sigma and L (p_ref in bc_p)
+  int sponge_zone;                 // the SPONGE zone, or -1
+  double sponge_p, sponge_trelax;  // its pressure and relaxation time
 };
 
 using Geom = GeomT<double>;
@@ -246,9 +251,10 @@ template <class R>
 int launch_rk_apply(int stage, const Launch<R>& L, int cur, const double* rhs, void* stream);
 template <class R>
 int launch_begin_step(const Launch<R>& L, void* stream);
-// Nscbc::initialHook: every NSCBC outlet face's state from its cell's
+// Nscbc::initialHook / Sponge::initialHook: every NSCBC outlet face's state
+// and (sponge true) the sponge's relaxed state from their cells
 template <class R>
-int launch_nscbc_init(const Launch<R>& L, int cur, void* stream);
+int launch_nscbc_init(const Launch<R>& L, int cur, bool sponge, void* stream);
 // FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]
 template <class R>
 int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, void* stream);

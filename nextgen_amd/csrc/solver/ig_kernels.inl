@@ -722,6 +722,57 @@ __device__ __forceinline__ double erk_wgt3(int stage, int i) {
   return i == 2 ? 8.0 / 12.0 : -1.0 / 12.0;
 }
 
+// The FP64 state of cell c: rho, u, rhoE and the primitives (in mixed
+// precision recovered from the FP64 conserved state, as the CBC faces use it)
+template <class R>
+__device__ __forceinline__ void cell_state64(const GeomT<R>& g, const StateT<R>& st, const Params& P, int c,
+                                             const Light<R>& self, double& rho, double u[3], double& rhoE,
+                                             double& spv, double& p, double& h, double& T) {
+  #pragma clang fp contract(off)
+  if constexpr (kMixed<R>) {
+    const int ncv = g.ncv;
+    for (int i = 0; i < 3; ++i) u[i] = G(st.cons)[(size_t)(1 + i) * ncv + c];
+    rho = G(st.cons)[c];
+    rhoE = G(st.cons)[(size_t)4 * ncv + c];
+    const Light<double> L = eos(P, rho, u, rhoE, T);
+    spv = L.spv; p = L.p; h = L.h;
+  } else {
+    for (int i = 0; i < 3; ++i) u[i] = self.u[i];
+    rho = self.rho;
+    rhoE = G(st.rhoE)[c];
+    spv = self.spv; p = self.p; h = self.h;
+    T = self.p / (self.rho * P.R_gas);
+  }
+}
+
+// Sponge::addBoundaryFlux's volume term (IdealGasSolverBcs.cpp:4203-4285) of a
+// cell in the sponge (coefficient >= 0): - vol coeff (q - q_sponge) for rho,
+// rho u and rhoE, q_sponge from the relaxed T, u at the sponge pressure
+template <class R>
+__device__ __forceinline__ void sponge_volume(const GeomT<R>& g, const WorkT<R>& w, const Params& P, int c,
+                                              double rho, const double u[3], double rhoE, R acc[5]) {
+  #pragma clang fp contract(off)
+  const double coeff = G(w.sponge_coeff)[c];
+  if (coeff < 0.0) return;
+  const auto sp = G(w.sponge) + 4 * (size_t)c;
+  const double T_s = sp[0], us[3] = {sp[1], sp[2], sp[3]};
+  const double vol = G(g.vol)[c];
+  const double rho_s = P.sponge_p / (T_s * P.R_gas);
+  double r[5];
+  r[0] = vol * coeff * (rho - rho_s);
+  for (int i = 0; i < 3; ++i) r[1 + i] = vol * coeff * (rho * u[i] - rho_s * us[i]);
+  const double rhoE_s = P.sponge_p / P.gm1 + 0.5 * rho_s * (us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+  r[4] = vol * coeff * (rhoE - rhoE_s);
+  if constexpr (kMixed<R>) {
+    // the accumulation's energy gauge: F_rhoE - h_g F_rho
+    acc[0] -= R(r[0]);
+    for (int i = 0; i < 3; ++i) acc[1 + i] -= R(r[1 + i]);
+    acc[4] -= R(r[4] - P.h_g * r[0]);
+  } else {
+    for (int q = 0; q < 5; ++q) acc[q] -= r[q];
+  }
+}
+
 // The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
 // IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
 // face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
@@ -740,10 +791,23 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = R(0);
   if (g.nbf == 0) return;
+  // a sponge's volume term follows its zone's faces, before the next zone's
+  // (addBoundaryFlux zone by zone, IdealGasSolverBcs.cpp:4166-4287)
+  const int zs = P.sponge_zone;
+  bool sponge_done = zs < 0;
+  double rho64 = 0.0, u64[3] = {0.0, 0.0, 0.0}, rhoE64 = 0.0, T64 = 0.0;
+  if (!sponge_done) {
+    double spv, pp, hh;
+    cell_state64(g, st, P, c, self, rho64, u64, rhoE64, spv, pp, hh, T64);
+  }
   const int b1 = G(g.bf_ptr)[c + 1];
   for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
     const int z = G(g.bf_zone)[b];
     const int type = P.bc_type[z];
+    if (!sponge_done && z > zs) {
+      sponge_volume(g, w, P, c, rho64, u64, rhoE64, acc);
+      sponge_done = true;
+    }
     const R n[3] = {R(G(g.bfn0)[b]), R(G(g.bfn1)[b]), R(G(g.bfn2)[b])};
     const R p = self.p;
     if (type == BC_SLIP) {
@@ -781,7 +845,16 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       }
       double ub[3], vb, pb, hb;
       __attribute__((address_space(1))) double* q = nullptr;  // an NSCBC face's state
-      if (type == BC_NSCBC_OUTLET_P) {
+      if (type == BC_SPONGE) {
+        // Sponge::addBoundaryFlux (IdealGasSolverBcs.cpp:4166-4201): the face
+        // cell's relaxed T, u at the sponge pressure
+        #pragma clang fp contract(off)
+        const auto sp = G(w.sponge) + 4 * (size_t)c;
+        vb = sp[0] * P.R_gas / P.sponge_p;
+        for (int i = 0; i < 3; ++i) ub[i] = sp[1 + i];
+        pb = P.sponge_p;
+        hb = P.Rgogm1 * sp[0];
+      } else if (type == BC_NSCBC_OUTLET_P) {
         #pragma clang fp contract(off)
         q = G(w.nscbc) + 5 * (size_t)b;
         if (update) {
@@ -836,6 +909,17 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       const R T = (kMixed<R> ? self.p + R(P.p_g) : self.p) / (self.rho * R(P.R_gas));
       acc[4] -= k_coeff * (T - R(P.bc_T[z]));
     }
+  }
+  if (zs < 0) return;
+  if (!sponge_done) sponge_volume(g, w, P, c, rho64, u64, rhoE64, acc);
+  if (update && stage == 1) {
+    // Sponge::rk3Step (IdealGasSolverBcs.cpp:4289-4309): the relaxed state moves
+    // toward the cell's at the start of the step, after the stage-1 RHS
+    #pragma clang fp contract(off)
+    const double eps = 1.0 / (P.sponge_trelax / G(w.info)->dt + 1.0);
+    const auto sp = G(w.sponge) + 4 * (size_t)c;
+    sp[0] += eps * (T64 - sp[0]);
+    for (int i = 0; i < 3; ++i) sp[1 + i] += eps * (u64[i] - sp[1 + i]);
   }
 }
 
@@ -2010,14 +2094,24 @@ __global__ void k_begin_step(const Inst<R>* __restrict__ I, int ninst) {
 // face's state from its cell's rho, p, u (FP64)
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict__ I, const int* __restrict__ bmap,
-                                                       int cur) {
+                                                       int cur, int init_sponge) {
   int lblk;
   const Inst<R>* Xp = inst_of(I, bmap, lblk);
   if (!Xp) return;
   const Inst<R>& X = *Xp;
   const GeomT<R>& g = X.g;
   const int c = lblk * kBlock + threadIdx.x;
-  if (c >= g.ncv || g.nbf == 0 || !X.w.nscbc) return;
+  if (c >= g.ncv || g.nbf == 0) return;
+  if (X.w.sponge && init_sponge) {
+    // Sponge::initialHook (IdealGasSolverBcs.cpp:4311-4346): the cells' T and u
+    double rho, u[3], rhoE, spv, p, h, T;
+    const Light<R> self = load_light(X.st[cur].light, c);
+    cell_state64(g, X.st[cur], X.P, c, self, rho, u, rhoE, spv, p, h, T);
+    const auto sp = G(X.w.sponge) + 4 * (size_t)c;
+    sp[0] = T;
+    for (int i = 0; i < 3; ++i) sp[1 + i] = u[i];
+  }
+  if (!X.w.nscbc) return;
   const int b0 = G(g.bf_ptr)[c], b1 = G(g.bf_ptr)[c + 1];
   for (int b = b0; b < b1; ++b) {
     if (X.P.bc_type[G(g.bf_zone)[b]] != BC_NSCBC_OUTLET_P) continue;
@@ -2297,8 +2391,9 @@ int launch_begin_step(const Launch<R>& L, void* stream) {
 }
 
 template <class R>
-int launch_nscbc_init(const Launch<R>& L, int cur, void* stream) {
-  hipLaunchKernelGGL(k_nscbc_init<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur);
+int launch_nscbc_init(const Launch<R>& L, int cur, bool sponge, void* stream) {
+  hipLaunchKernelGGL(k_nscbc_init<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur,
+                     sponge ? 1 : 0);
   return check();
 }
 
@@ -2410,7 +2505,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_prims<R>(const Launch<R>&, int, int, int, double*, void*);                  \
   template int launch_props<R>(const Launch<R>&, int, int, void*);                                \
   template int launch_begin_step<R>(const Launch<R>&, void*);                                     \
-  template int launch_nscbc_init<R>(const Launch<R>&, int, void*);                                \
+  template int launch_nscbc_init<R>(const Launch<R>&, int, bool, void*);                          \
   template int launch_cfl<R>(const Launch<R>&, int, double, void*, unsigned long long*);          \
   template int launch_diag<R>(const Launch<R>&, int, void*, double*, double*);                   \
   template int launch_rk_apply<R>(int, const Launch<R>&, int, const double*, void*);               \

@@ -35,13 +35,13 @@ class CharlesError(RuntimeError):
 CG_MAX_BC_ZONES = 16
 CG_BC_SLIP, CG_BC_WALL_ADIABATIC, CG_BC_WALL_ISOTHERMAL = 1, 2, 3
 CG_BC_CBC_UPT, CG_BC_CBC_MPT, CG_BC_CBC_RUNP, CG_BC_CBC_RUP = 4, 5, 6, 7
-CG_BC_NSCBC_OUTLET_P = 8
+CG_BC_NSCBC_OUTLET_P, CG_BC_SPONGE = 8, 9
 
 
 class cg_bc(C.Structure):
     _fields_ = [("zone", C.c_char * 48), ("type", C.c_int), ("T", C.c_double), ("u", C.c_double * 3),
                 ("p", C.c_double), ("rho", C.c_double), ("q", C.c_double), ("sigma", C.c_double),
-                ("L", C.c_double)]
+                ("L", C.c_double), ("mode", C.c_int)]
 
 
 class cg_params(C.Structure):
@@ -316,7 +316,7 @@ class Mesh:
              "cvocv_i": (self.ncv + 1,), "cvocv_v": (self.nnz,), "grad_coeff": (self.nnz, 3),
              "inv_vol": (self.ncv,), "sa_to_vol": (self.ncv,), "dcmag_hat": (self.ncv,), "cmag": (self.ncv,),
              "faocv_i": (self.ncv + 1,), "faocv_v": (2 * self.nfa,), "efocv_i": (self.ncv + 1,),
-             "efocv_v": (2 * self.nef,), "grad_flag": (self.ncv,), "L": (3,), "max_gcl": (1,),
+             "efocv_v": (2 * self.nef,), "grad_flag": (self.ncv,), "L": (3,), "domain_max": (5,), "max_gcl": (1,),
              "r_vv": (self.ncv,), "cvobf": (self.nbf,), "zone_bf": (self.nbf,), "n_bf": (self.nbf, 3),
              "x_bf": (self.nbf, 3), "area_bf": (self.nbf,), "aod_bf": (self.nbf,), "bfocv_i": (self.ncv + 1,),
              "bfocv_v": (self.nbf,)}
@@ -395,7 +395,7 @@ class IdealGasSolver:
         return rho, u, rhoE
 
     def get_field(self, name):
-        shape = (self.ncv, 3, 3) if name == "dudx" else (self.ncv,)
+        shape = {"dudx": (self.ncv, 3, 3), "u_sponge": (self.ncv, 3)}.get(name, (self.ncv,))
         out = np.empty(shape)
         self._chk(load_library().cg_get_field(self._h, name.encode(), _dptr(out), out.size))
         return out

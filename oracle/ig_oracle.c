@@ -186,6 +186,9 @@ struct og_solver {
   /* NSCBC outlet faces (Nscbc, IdealGasSolverBcs.hpp:576-612): per boundary
    * face rho_bc, p_bc, u_bc[3], the stage rates [3][5] and the two snapshots */
   double *nscbc, *nscbc_rhs, *nscbc_store, *nscbc_recent;
+  /* Sponge (IdealGasSolverBcs.hpp): T_sponge, u_sponge [ncv][4] and the two
+   * snapshots */
+  double *sponge, *sponge_store, *sponge_recent;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -233,6 +236,9 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
     s->nscbc_store = (double*)calloc(5 * nb + 1, sizeof(double));
     s->nscbc_recent = (double*)calloc(5 * nb + 1, sizeof(double));
   }
+  s->sponge = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
+  s->sponge_store = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
+  s->sponge_recent = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   /* IdealGasSolver::initData: reference state (IdealGasSolver.cpp:58-62) */
   for (int i = 0; i < n; ++i) {
     s->rho[i] = p->rho_ref;
@@ -251,6 +257,7 @@ void og_destroy(og_solver* s) {
   free(s->sc); free(s->rhs0_sc); free(s->rhs1_sc); free(s->rhs2_sc); free(s->rhs_sc_tmp);
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
+  free(s->sponge); free(s->sponge_store); free(s->sponge_recent);
   free(s);
 }
 
@@ -311,6 +318,15 @@ int og_get_field(const og_solver* s, const char* name, double* out) {
   else if (!strcmp(name, "lijmij")) src = s->lijmij;
   else if (!strcmp(name, "mijmij")) src = s->mijmij;
   else if (!strcmp(name, "dudx")) { memcpy(out, s->dudx, sizeof(double) * 9 * (size_t)n); return 0; }
+  else if (!strcmp(name, "T_sponge") || !strcmp(name, "u_sponge")) {
+    if (s->p.sponge_zone < 0) return -1;
+    const int t = !strcmp(name, "T_sponge");
+    for (int i = 0; i < n; ++i) {
+      if (t) out[i] = s->sponge[4 * (size_t)i];
+      else for (int d = 0; d < 3; ++d) out[3 * (size_t)i + d] = s->sponge[4 * (size_t)i + 1 + d];
+    }
+    return 0;
+  }
   if (src) { memcpy(out, src, sizeof(double) * n); return 0; }
   for (int i = 0; i < n; ++i) {
     const og_aux* a = &s->cv_compact[i];
@@ -1062,6 +1078,59 @@ static void nscbc_init(og_solver* s) {
   }
 }
 
+/* Sponge::initialHook (IdealGasSolverBcs.cpp:4311-4346): T_sponge, u_sponge
+ * from the cells */
+static void sponge_init(og_solver* s) {
+  if (s->p.sponge_zone < 0) return;
+  for (int icv = 0; icv < s->m.ncv; ++icv) {
+    s->sponge[4 * (size_t)icv] = s->T[icv];
+    for (int i = 0; i < 3; ++i) s->sponge[4 * (size_t)icv + 1 + i] = s->u[3 * (size_t)icv + i];
+  }
+}
+
+/* Sponge::rk3Step (IdealGasSolverBcs.cpp:4289-4309): at stage 1 the relaxed
+ * state moves toward the cells' (time level n) by eps = 1 / (t_relax / dt + 1) */
+static void sponge_rk3(og_solver* s, int rk_stage) {
+  if (s->p.sponge_zone < 0 || rk_stage != 1) return;
+  const double eps = 1.0 / (s->p.sponge_trelax / s->dt + 1.0);
+  for (int icv = 0; icv < s->m.ncv; ++icv) {
+    double* sp = &s->sponge[4 * (size_t)icv];
+    sp[0] += eps * (s->T[icv] - sp[0]);
+    for (int i = 0; i < 3; ++i) sp[1 + i] += eps * (s->u[3 * (size_t)icv + i] - sp[1 + i]);
+  }
+}
+
+/* Sponge::addBoundaryFlux's volume term (IdealGasSolverBcs.cpp:4203-4285):
+ * every cell whose coordinate r (x, y or z; sqrt(y^2+z^2) for L_RX;
+ * sqrt(x^2+y^2) for L_RZ) is beyond Lmax - length relaxes toward the sponge
+ * state with strength x_sp^2 */
+static void sponge_volume(const og_solver* s, og_rhs* rhs) {
+  const og_mesh* m = &s->m;
+  const og_params* P = &s->p;
+  const double gm1 = P->gamma - 1.0;
+  const int type = P->sponge_type;
+  const double Lmax = m->x_max[type];
+  for (int icv = 0; icv < m->ncv; ++icv) {
+    const double* x = &m->x_cv[3 * (size_t)icv];
+    double r;
+    if (type < 3) r = x[type];
+    else if (type == 3) r = sqrt(x[1] * x[1] + x[2] * x[2]);
+    else r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    if (r > Lmax - P->sponge_length) {
+      const double x_sp = (r - (Lmax - P->sponge_length)) / P->sponge_length;
+      const double coeff = P->sponge_strength * x_sp * x_sp;
+      const double* sp = &s->sponge[4 * (size_t)icv];
+      const double rho_s = P->sponge_p / (sp[0] * s->R_gas);
+      const double vol = m->vol[icv];
+      rhs[icv].rho -= vol * coeff * (s->rho[icv] - rho_s);
+      for (int i = 0; i < 3; ++i)
+        rhs[icv].rhou[i] -= vol * coeff * (s->rho[icv] * s->u[3 * (size_t)icv + i] - rho_s * sp[1 + i]);
+      const double rhoE_s = P->sponge_p / gm1 + 0.5 * rho_s * (sp[1] * sp[1] + sp[2] * sp[2] + sp[3] * sp[3]);
+      rhs[icv].rhoE -= vol * coeff * (s->rhoE[icv] - rhoE_s);
+    }
+  }
+}
+
 /* addBoundaryFlux of every zone in zone order (IdealGasSolverBcs.cpp:14-47 SlipWallBc /
  * WallAdiabatic, :1485-1510 WallIsothermal), first in calcRhsBase
  * (IdealGasSolver.cpp:261-262): stage p, u, T; lagged mu_lam, loc_lam */
@@ -1069,9 +1138,14 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
   const og_mesh* m = &s->m;
   const og_params* P = &s->p;
   const double cp = s->R_gas * P->gamma / (P->gamma - 1.0);
+  int sponge_done = P->sponge_zone < 0;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int icv = m->cvobf[ibf];
     const int z = m->zone_bf[ibf];
+    if (!sponge_done && z > P->sponge_zone) {
+      sponge_volume(s, rhs); /* after the sponge zone's faces */
+      sponge_done = 1;
+    }
     const double* n = &m->n_bf[3 * (size_t)ibf];
     const double* u = &s->u[3 * (size_t)icv];
     const double p = s->pr[icv];
@@ -1090,7 +1164,15 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
       /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976); an NSCBC face
        * (Nscbc::addBoundaryFlux, :3662-3693) uses its evolved state */
       og_state b;
-      if (P->bc_type[z] == 8) {
+      if (P->bc_type[z] == 9) {
+        /* Sponge::addBoundaryFlux (IdealGasSolverBcs.cpp:4166-4201) */
+        const double Rgogm1 = s->R_gas * P->gamma * (1.0 / (P->gamma - 1.0));
+        const double* sp = &s->sponge[4 * (size_t)icv];
+        b.sp_vol = sp[0] * s->R_gas / P->sponge_p;
+        for (int i = 0; i < 3; ++i) b.u[i] = sp[1 + i];
+        b.p = P->sponge_p;
+        b.h = Rgogm1 * sp[0];
+      } else if (P->bc_type[z] == 8) {
         const double* q = &s->nscbc[5 * (size_t)ibf];
         const double gogm1 = P->gamma / (P->gamma - 1.0);
         for (int i = 0; i < 3; ++i) b.u[i] = q[2 + i];
@@ -1112,6 +1194,7 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
       rhs[icv].rhoE -= k_coeff * (s->T[icv] - P->bc_T[z]);
     }
   }
+  if (!sponge_done) sponge_volume(s, rhs);
 }
 
 static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
@@ -1321,6 +1404,7 @@ void og_init_complete(og_solver* s) {
   s->dsm_kept = 0;
   og_update_state(s);
   nscbc_init(s);
+  sponge_init(s);
   og_sgs_props(s);
   og_store(s);
 }
@@ -1350,10 +1434,13 @@ void og_store(og_solver* s) {
     s->store_step = s->store_step_recent;
     s->store_time = s->store_time_recent;
     memcpy(s->nscbc_store, s->nscbc, sizeof(double) * nb);
+    memcpy(s->sponge_store, s->sponge, sizeof(double) * 4 * (size_t)n);
   } else {
     memcpy(s->nscbc_store, s->nscbc_recent, sizeof(double) * nb);
+    memcpy(s->sponge_store, s->sponge_recent, sizeof(double) * 4 * (size_t)n);
   }
   memcpy(s->nscbc_recent, s->nscbc, sizeof(double) * nb); /* Nscbc::storeData */
+  memcpy(s->sponge_recent, s->sponge, sizeof(double) * 4 * (size_t)n); /* Sponge::storeData */
 }
 
 void og_restore(og_solver* s) {
@@ -1366,6 +1453,7 @@ void og_restore(og_solver* s) {
     for (int d = 0; d < 3; ++d) s->u[3 * (size_t)i + d] = s->store_buf[c++];
   for (int i = 0; i < n; ++i) s->rhoE[i] = s->store_buf[c++];
   memcpy(s->nscbc, s->nscbc_store, sizeof(double) * 5 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
+  memcpy(s->sponge, s->sponge_store, sizeof(double) * 4 * (size_t)n);
   og_update_state(s);
   og_sgs_props(s);
   s->p.dt_data[0] *= 0.8;
@@ -1388,11 +1476,13 @@ int og_step(og_solver* s) {
   /* calcRhs = calcRhsBase + calcRhsScalars + addSourceHook(rhs, time, rk_stage)
    * (IdealGasSolver.hpp:697-705), at the stage times t, t + dt, t + dt/2 */
   /* each stage: the NSCBC faces' rates (bc calcRhs), the flow RHS with the
-   * faces' states, the faces' RK update, the flow's (IdealGasSolver.hpp:457-468) */
+   * faces' states, the faces' RK update and the sponge's relaxation (bc
+   * rk3Step), the flow's (IdealGasSolver.hpp:457-468) */
   nscbc_calc_rhs(s, 1);
   calc_rhs_into(s, s->rhs0, s->rhs0_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs0, n, s->time, 1);
   nscbc_rk3(s, 1);
+  sponge_rk3(s, 1);
   rk3_step(s, og_erk_wgt3[0]);
   s->time += s->dt;
   og_update_state(s);
@@ -1400,6 +1490,7 @@ int og_step(og_solver* s) {
   calc_rhs_into(s, s->rhs1, s->rhs1_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs1, n, s->time, 2);
   nscbc_rk3(s, 2);
+  sponge_rk3(s, 2);
   rk3_step(s, og_erk_wgt3[1]);
   s->time -= 0.5 * s->dt;
   og_update_state(s);
@@ -1407,6 +1498,7 @@ int og_step(og_solver* s) {
   calc_rhs_into(s, s->rhs2, s->rhs2_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs2, n, s->time, 3);
   nscbc_rk3(s, 3);
+  sponge_rk3(s, 3);
   rk3_step(s, og_erk_wgt3[2]);
   s->time += 0.5 * s->dt;
   og_update_state(s);

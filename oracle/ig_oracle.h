@@ -76,6 +76,10 @@ typedef struct og_mesh {
   const int* zone_bf;       /* [nbf] */
   const double* n_bf;       /* [nbf][3] */
   const double* aod_bf;     /* [nbf] */
+  /* SPONGE only (may be NULL otherwise): cell centres and the domain's extents
+   * from its nodes, max x, y, z, sqrt(y^2+z^2), sqrt(x^2+y^2) (sponge_data) */
+  const double* x_cv;       /* [ncv][3] */
+  double x_max[5];
 } og_mesh;
 
 typedef struct og_params {
@@ -103,6 +107,12 @@ typedef struct og_params {
   /* 8 NSCBC_OUTLET_P p_ref (bc_p), sigma, L: per-face boundary states evolved
    * with the step's RK3 weights */
   double bc_sigma[16], bc_L[16];
+  /* 9 SPONGE (zone sponge_zone, or -1 without one): relaxed T, u per cell,
+   * the Riemann flux at its faces against them at sponge_p, and a volume term
+   * over the last sponge_length of the domain in sponge_type (0 L_X, 1 L_Y,
+   * 2 L_Z, 3 L_RX, 4 L_RZ) with strength sponge_strength x_sp^2 */
+  int sponge_zone, sponge_type;
+  double sponge_strength, sponge_length, sponge_p, sponge_trelax;
 } og_params;
 
 typedef struct og_diag {

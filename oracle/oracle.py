@@ -22,7 +22,8 @@ class og_mesh(C.Structure):
         (n, C.c_void_p) for n in ("vol", "inv_vol", "sa_to_vol", "dcmag_hat", "lo", "hi", "n_use", "c_use",
                                   "unit_n_use", "area_use", "aod_use", "n_fa", "cvocv_i", "cvocv_v", "grad_coeff",
                                   "filt_w", "r_vv")
-    ] + [("nbf", C.c_int)] + [(n, C.c_void_p) for n in ("cvobf", "zone_bf", "n_bf", "aod_bf")]
+    ] + [("nbf", C.c_int)] + [(n, C.c_void_p) for n in ("cvobf", "zone_bf", "n_bf", "aod_bf", "x_cv")] + [
+        ("x_max", C.c_double * 5)]
 
 
 class og_params(C.Structure):
@@ -37,6 +38,8 @@ class og_params(C.Structure):
         ("bc_type", C.c_int * 16), ("bc_T", C.c_double * 16), ("bc_u", (C.c_double * 3) * 16),
         ("bc_p", C.c_double * 16), ("bc_rho", C.c_double * 16), ("bc_q", C.c_double * 16),
         ("bc_area", C.c_double * 16), ("bc_sigma", C.c_double * 16), ("bc_L", C.c_double * 16),
+        ("sponge_zone", C.c_int), ("sponge_type", C.c_int), ("sponge_strength", C.c_double),
+        ("sponge_length", C.c_double), ("sponge_p", C.c_double), ("sponge_trelax", C.c_double),
     ]
 
 
@@ -163,9 +166,16 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
     given the condition cgp holds under that name; `zone_area` each zone's area
     (its faces' |n_bf| summed in face order)."""
     p = og_params()
+    p.sponge_zone = -1
     bcs = {cgp.bc[i].zone.decode(): cgp.bc[i] for i in range(cgp.nbc)}
     for z, name in enumerate(zones):
         b = bcs[name]
+        if b.type == 9:
+            # SPONGE: RELAX_TIME in T, STRENGTH in q, LENGTH in L, TYPE in mode,
+            # SPONGE_PRESSURE in p (negative: P_REF)
+            p.sponge_zone, p.sponge_type = z, b.mode
+            p.sponge_strength, p.sponge_length, p.sponge_trelax = b.q, b.L, b.T
+            p.sponge_p = b.p if b.p >= 0.0 else cgp.p_ref
         p.bc_type[z] = b.type
         p.bc_T[z] = b.T
         for d in range(3):
@@ -174,7 +184,7 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
         p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
         p.bc_sigma[z], p.bc_L[z] = b.sigma, b.L
     for name, _ in og_params._fields_:
-        if name.startswith("bc_"):
+        if name.startswith("bc_") or name.startswith("sponge_"):
             continue
         if name == "full_gr":
             p.full_gr = cgp.use_full_gr
@@ -208,10 +218,15 @@ class OracleSolver:
             setattr(m, field, a.ctypes.data)
         m.nbf = getattr(mesh, "nbf", 0)
         if m.nbf:
-            for field in ("cvobf", "zone_bf", "n_bf", "aod_bf"):
+            for field in ("cvobf", "zone_bf", "n_bf", "aod_bf", "x_cv"):
                 a = np.ascontiguousarray(mesh.get(field))
                 self._keep[field] = a
                 setattr(m, field, a.ctypes.data)
+            try:
+                for d, v in enumerate(mesh.get("domain_max")):
+                    m.x_max[d] = float(v)
+            except RuntimeError:
+                pass  # a mesh from raw arrays: no extents (a SPONGE cannot run on it)
         self._m = m
         zone_area = ()
         if m.nbf:
@@ -235,7 +250,7 @@ class OracleSolver:
         return rho, u, rhoE
 
     def get_field(self, name):
-        out = np.empty((self.ncv, 3, 3) if name == "dudx" else self.ncv)
+        out = np.empty({"dudx": (self.ncv, 3, 3), "u_sponge": (self.ncv, 3)}.get(name, (self.ncv,)))
         if self._L.og_get_field(self._h, name.encode(), dptr(out)) != 0:
             raise KeyError(name)
         return out

@@ -19,6 +19,14 @@ terms, pinned by closed forms.
   type (the Riemann flux of equal states is the physical flux, pinned against
   the reference's calcRiemannFlux in test_oracle_pins.py); a gas at rest
   facing an inflow state gains the Riemann flux's mass in the wall cells.
+* SPONGE (IdealGasSolverBcs.cpp:4166-4346): a stream matching the sponge's
+  pressure and its relaxed state (set from the cells) keeps a zero RHS and
+  stays put for five steps, relaxed state included; at rest against a lower
+  sponge pressure, the sponge's volume term on the lattice is -vol s x_sp^2 (q -
+  q_sponge) with x_sp = 3/4 and 1/4 in the two cell rows of a sponge two cells
+  long (the wall row adding the Riemann flux's mass) and nothing below; the
+  stage-1 relaxation moves T_sponge, u_sponge by eps = 1 / (t_relax / dt + 1)
+  of the way to a changed state, in every cell.
 * The parser: zone conditions by name, the reference's other types refused, a
   zone without a condition refused when the solver is built.
 """
@@ -250,3 +258,98 @@ def test_nscbc_outlet_keeps_a_matching_uniform_stream(jitter):
     np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
     np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
     np.testing.assert_allclose(u[:, [0, 2]], 0.0, atol=1e-10)
+
+
+def test_sponge_parser():
+    p = parse_params(deck({"y0": "SLIP", "y1": "SPONGE TYPE L_Y LENGTH 1.5 STRENGTH 2.0 RELAX_TIME 0.5"}))
+    b = p.bc[1]
+    assert (b.zone, b.type, b.mode, b.L, b.q, b.T, b.p) == (b"y1", 9, 1, 1.5, 2.0, 0.5, -1.0)
+    p = parse_params(deck({"y1": "SPONGE TYPE L_RZ LENGTH 1 STRENGTH 0 RELAX_TIME 2 EPS_P 0.1 SPONGE_PRESSURE 70"}))
+    assert (p.bc[0].mode, p.bc[0].p, p.bc[0].q) == (4, 70.0, 0.0)
+    for bad in ("y1 SPONGE LENGTH 1 STRENGTH 1 RELAX_TIME 1", "y1 SPONGE TYPE L_Q LENGTH 1 STRENGTH 1 RELAX_TIME 1",
+                "y1 SPONGE TYPE L_X STRENGTH 1 RELAX_TIME 1", "y1 SPONGE TYPE L_X LENGTH 1 STRENGTH -1 RELAX_TIME 1",
+                "y1 SPONGE TYPE L_X LENGTH 1 STRENGTH 1", "y1 SPONGE TYPE L_X LENGTH 1 STRENGTH 1 RELAX_TIME 1 "
+                "SPONGE_PRESSURE -2", "y1 SPONGE TYPE L_X LENGTH 1 STRENGTH 1 RELAX_TIME 1 WIDTH 2"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_sponge_keeps_a_matching_uniform_stream(jitter):
+    # in at y0 (CBC_UPT), out through a sponge at y1 at the stream's pressure:
+    # its relaxed state starts as the cells' (initialHook), the Riemann flux
+    # against it is the physical flux and the volume term vanishes, so the RHS
+    # is zero and the stream and the relaxed state stay
+    m = Mesh.channel(10, jitter=jitter, seed=9)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    text = deck({"y0": f"CBC_UPT 0 {V} 0 {p!r} {T}",
+                 "y1": f"SPONGE TYPE L_Y LENGTH 2.5 STRENGTH 3.0 RELAX_TIME 0.1 SPONGE_PRESSURE {p!r}"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    r = o.calc_rhs()
+    assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
+    np.testing.assert_allclose(o.get_field("T_sponge"), T, rtol=1e-12)
+    np.testing.assert_allclose(o.get_field("u_sponge")[:, 1], V, rtol=1e-12)
+
+
+def test_sponge_volume_term_closed_form_at_rest():
+    # lattice channel at rest (p, T = 1) with a sponge at y1 of pressure 0.9 p,
+    # two cells long (LENGTH 2h): the top row (y = L - h/2) has x_sp = 3/4, the
+    # row below x_sp = 1/4, the rest lies outside
+    from oracle.oracle import lib
+    import ctypes as C
+    m = Mesh.channel(10, jitter=0.0)
+    R, T, gamma = 71.42857142857143, 1.0, 1.4
+    p = R * T
+    ps, S = 0.9 * p, 5.0
+    text = deck({"y0": "SLIP", "y1": f"SPONGE TYPE L_Y LENGTH {2 * H!r} STRENGTH {S} RELAX_TIME 0.1 "
+                 f"SPONGE_PRESSURE {ps!r}"})
+    o = oracle(m, text, uniform(m, p=p, T=T, R=R))
+    r = o.calc_rhs()
+    y = m.get("x_cv")[:, 1]
+    top = np.abs(y - (2 * np.pi - 0.5 * H)) < 1e-9
+    second = np.abs(y - (2 * np.pi - 1.5 * H)) < 1e-9
+    rest = ~(top | second)
+    assert top.sum() == 100 and second.sum() == 100
+    vol = H ** 3
+    rho, rho_s = p / (R * T), ps / (R * T)
+    np.testing.assert_allclose(r[second, 0], -vol * S / 16 * (rho - rho_s), rtol=1e-10)
+    np.testing.assert_allclose(r[second, 4], -vol * S / 16 * (p - ps) / (gamma - 1.0), rtol=1e-10)
+    assert np.abs(r[second, 1:4]).max() < 1e-11 * p * H * H
+    # the wall row: the Riemann flux against (rest, ps, T) and 9/16 of the strength
+    nb = np.array([0.0, H * H, 0.0])
+    a = np.array([0.0, 0.0, 0.0, R * T / p, p, R * T * gamma / (gamma - 1.0)])
+    b = np.array([0.0, 0.0, 0.0, R * T / ps, ps, R * T * gamma / (gamma - 1.0)])
+    f = np.zeros(5)
+    D = C.POINTER(C.c_double)
+    lib().og_flux_riemann(f.ctypes.data_as(D), nb.ctypes.data_as(D), a.ctypes.data_as(D), b.ctypes.data_as(D),
+                          gamma, gamma)
+    np.testing.assert_allclose(r[top, 0], -f[0] - vol * S * 9 / 16 * (rho - rho_s), rtol=1e-10)
+    assert np.abs(r[rest]).max() < 1e-11 * p * H * H
+
+
+def test_sponge_relaxes_toward_a_changed_state():
+    # the relaxed state is set from the cells at init (T = 1, at rest); the
+    # cells then become a uniform stream at T = 1.1: the first stage moves
+    # every cell's T_sponge and u_sponge by eps = 1 / (t_relax / dt + 1)
+    m = Mesh.channel(10, jitter=0.3, seed=4)
+    R = 71.42857142857143
+    t_relax = 0.02
+    o = oracle(m, deck({"y0": "SLIP", "y1": f"SPONGE TYPE L_Y LENGTH 1.0 STRENGTH 2.0 RELAX_TIME {t_relax}"}),
+               uniform(m, p=R, T=1.0, R=R))
+    np.testing.assert_array_equal(o.get_field("T_sponge"), o.get_field("T"))
+    U = np.array([0.4, 0.0, -0.2])
+    o.set_state(*uniform(m, tuple(U), p=1.1 * R, T=1.1, R=R))
+    o.sync_post_state()
+    T1 = o.get_field("T").copy()
+    assert o.step() == 0
+    dt = o.diag()["dt"]
+    eps = 1.0 / (t_relax / dt + 1.0)
+    np.testing.assert_allclose(o.get_field("T_sponge"), 1.0 + eps * (T1 - 1.0), rtol=1e-14)
+    np.testing.assert_allclose(o.get_field("u_sponge"), eps * np.tile(U, (m.ncv, 1)), rtol=1e-13, atol=1e-16)

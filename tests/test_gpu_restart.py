@@ -116,3 +116,41 @@ def test_data_file_of_another_mesh_is_rejected(tmp_path):
     with pytest.raises(CharlesError, match="cannot open"):
         b.read_sles(str(tmp_path / "missing.sles"))
     b.close()
+
+
+def test_sponge_restart_carries_its_relaxed_state(tmp_path):
+    # T_sponge and u_sponge are registered cv data (IdealGasSolverBcs.cpp:4089-4090):
+    # the data file carries them, the restart's initComplete keeps them
+    # (Sponge::initialHook resets them only when the file lacks them), and the
+    # restarted solve continues bit for bit; a restart that drops them does not
+    m = Mesh.channel(10, jitter=0.3, seed=3)
+    p0 = 1.0 / (1.4 * 0.1 ** 2)
+    text = (tgv_params(sgs="VREMAN") + f"y0 CBC_UPT 0.1 0.4 0 {p0!r} 1.0\n"
+            f"y1 SPONGE TYPE L_Y LENGTH 2.0 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {p0 * 0.99!r}\n")
+    x = m.get("x_cv")
+    y = (x[:, 1] - np.pi) / np.pi
+    u = np.zeros((m.ncv, 3))
+    u[:, 0] = 1.0 - y * y + 0.2 * np.sin(x[:, 0]) * np.cos(x[:, 2])
+    u[:, 1] = 0.1 * np.sin(x[:, 2]) * np.sin(x[:, 1])
+    rho = np.ones(m.ncv)
+    rhoE = p0 / 0.4 + 0.5 * (u * u).sum(1)
+    path = str(tmp_path / "sponge.sles")
+    a = IdealGasSolver(m, parse_params(text))
+    a.set_state(rho, u, rhoE)
+    a.init_complete()
+    a.advance_solution(5)
+    a.write_sles(path)
+    ts5 = a.get_field("T_sponge")
+    a.advance_solution(6)
+    ref_state, ref_ts = a.get_state(), a.get_field("T_sponge")
+    a.close()
+    b = IdealGasSolver(m, parse_params(text))
+    b.read_sles(path)
+    b.init_complete()
+    np.testing.assert_array_equal(b.get_field("T_sponge"), ts5)
+    assert not np.array_equal(ts5, b.get_field("T"))  # the relaxed state lags the cells
+    b.advance_solution(6)
+    for xg, yr in zip(b.get_state(), ref_state):
+        np.testing.assert_array_equal(xg, yr)
+    np.testing.assert_array_equal(b.get_field("T_sponge"), ref_ts)
+    b.close()

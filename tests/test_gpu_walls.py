@@ -3,10 +3,12 @@ restatement; its wall terms are pinned by closed forms in
 test_boundary_conditions.py, its Riemann flux against the reference's
 calcRiemannFlux in test_oracle_pins.py): SLIP / SYMMETRY, WALL_ADIABATIC (with
 a TRANSLATE wall velocity), WALL_ISOTHERMAL, the characteristic conditions
-CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP and the NSCBC pressure outlet (per-face
-states evolved with the step, restored with it after a NaN) on jittered
-channel meshes and on the reference's own Stitch mesh orifice.mles (six
-boundary zones).
+CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP, the NSCBC pressure outlet (per-face
+states evolved with the step, restored with it after a NaN) and the SPONGE
+(per-cell relaxed T, u, a volume term over the last LENGTH of the domain, the
+Riemann flux at its faces; L_Y and L_RX on channels, L_X on the orifice) on
+jittered channel meshes and on the reference's own Stitch mesh orifice.mles
+(six boundary zones).
 
 FP64: p, T, sos and dudx bit-identical, the lagged coefficients within 1e-14,
 the stage RHS within 1e-12 of each component's largest magnitude, the CFL
@@ -42,6 +44,12 @@ WALLS = {
                     "y1": f"CBC_RUP 0.99 0.05 0.15 0 {P0 * 1.001!r} STATIONARY_FRAME"},
     # in through a CBC, out through an NSCBC outlet relaxing to a lower pressure
     "cbc_upt+nscbc": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0", "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    # out through a sponge over the top quarter, relaxing to a lower pressure
+    "cbc_upt+sponge": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0",
+                       "y1": f"SPONGE TYPE L_Y LENGTH 1.5 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {P0 * 0.995!r}"},
+    # a radial sponge as the first zone (its volume term before the other zone's faces)
+    "sponge_rx+adiabatic": {"y0": "SPONGE TYPE L_RX LENGTH 3.0 STRENGTH 8.0 RELAX_TIME 0.2",
+                            "y1": "WALL_ADIABATIC"},
 }
 
 
@@ -97,7 +105,7 @@ def check_state_and_rhs(g, o, what):
     assert abs(cg - co) <= 1e-14 * co, (what, cg, co)
 
 
-def check_history(g, o, n, what):
+def check_history(g, o, n, what, sponge=False):
     for d in g.run_history(n):
         assert o.step() == 0
         od = o.diag()
@@ -107,6 +115,11 @@ def check_history(g, o, n, what):
         assert abs(d.enstrophy - od["enstrophy"]) <= HIST_TOL * od["enstrophy"], what
         for q in range(5):
             assert abs(d.sums[q] - od["sums"][q]) <= HIST_TOL * max(abs(od["sums"][q]), od["sums"][0]), (what, q)
+    if sponge:
+        # the sponge's relaxed state after the steps
+        for name in ("T_sponge", "u_sponge"):
+            a, b = g.get_field(name), o.get_field(name)
+            assert np.abs(a - b).max() <= HIST_TOL * np.abs(b).max(), (what, name)
 
 
 @pytest.mark.parametrize("walls", sorted(WALLS))
@@ -115,7 +128,7 @@ def test_channel_walls_match_oracle(walls, sgs):
     m = Mesh.channel(12, jitter=0.3, seed=7)
     g, o = pair(m, deck(WALLS[walls], sgs), channel_state(m))
     check_state_and_rhs(g, o, f"channel {walls} {sgs}")
-    check_history(g, o, 20, f"channel {walls} {sgs}")
+    check_history(g, o, 20, f"channel {walls} {sgs}", sponge="sponge" in walls)
 
 
 def test_lattice_channel_walls_match_oracle():
@@ -193,12 +206,13 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
     check_history(g, o, 20, "orifice nscbc")
 
 
-def test_nscbc_faces_roll_back_with_the_flow():
+@pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge"])
+def test_boundary_states_roll_back_with_the_flow(walls):
     # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
-    # faces' states with it (Nscbc::storeData / restoreData), and goes on with
-    # dt * 0.8, as the oracle
+    # faces' or the sponge's states with it (storeData / restoreData of the
+    # zone), and goes on with dt * 0.8, as the oracle
     m = Mesh.channel(10, jitter=0.3, seed=12)
-    g, o = pair(m, deck(WALLS["cbc_upt+nscbc"], "NONE") + "CHECK_INTERVAL 1\n", channel_state(m))
+    g, o = pair(m, deck(WALLS[walls], "NONE") + "CHECK_INTERVAL 1\n", channel_state(m))
     g.advance_solution(3)
     for _ in range(3):
         o.step()
@@ -221,10 +235,10 @@ def test_nscbc_faces_roll_back_with_the_flow():
     d, od = g.report(), o.diag()
     assert d.step == od["step"] and d.nan_flag == 0
     assert abs(d.ke - od["ke"]) <= HIST_TOL * od["ke"]
-    check_history(g, o, 5, "after rollback")
+    check_history(g, o, 5, f"{walls} after rollback", sponge="sponge" in walls)
 
 
-@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc"])
+@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]
@@ -243,6 +257,39 @@ def test_mixed_precision_channel_close_to_oracle(walls):
     d, od = g.report(), o.diag()
     assert abs(d.ke - od["ke"]) <= 1e-5 * od["ke"]
     assert abs(d.sums[0] - od["sums"][0]) <= 1e-9 * od["sums"][0]
+
+
+def test_orifice_mesh_with_sponge_outlet_matches_oracle():
+    # the orifice with a mass-flow inlet and a sponge over the last 0.1 of the
+    # pipe (L_X, the domain's extent from the mesh file's nodes) relaxing to a
+    # lower pressure, walls no-slip
+    m = Mesh.read_mles(ORIFICE)
+    p0 = 1.0 / 1.4
+    zones = dict(zip(m.zones, range(len(m.zones))))
+    area = np.bincount(m.get("zone_bf"), weights=m.get("area_bf"), minlength=len(m.zones))
+    mdot = float(0.1 * area[zones["inlet"]])
+    walls = {"inlet": f"CBC_MPT {mdot!r} {p0!r} 1.0",
+             "outlet": f"SPONGE TYPE L_X LENGTH 0.1 STRENGTH 30.0 RELAX_TIME 0.02 SPONGE_PRESSURE {p0 * 0.99!r}",
+             "oriface-pipe": "WALL_ADIABATIC", "wall": "WALL_ADIABATIC", "oriface-wall-0": "WALL_ADIABATIC",
+             "oriface-wall-1": "WALL_ADIABATIC"}
+    rho = np.ones(m.ncv)
+    u = np.zeros((m.ncv, 3))
+    u[:, 0] = 0.05
+    rhoE = p0 / 0.4 + 0.5 * rho * (u * u).sum(1)
+    text = (f"GAMMA 1.4\nP_REF {p0!r}\nRHO_REF 1.0\nT_REF 1.0\nMU_REF 1e-5\nSGS_MODEL VREMAN\nTIMESTEP CFL 0.8\n" +
+            "".join(f"{z} {w}\n" for z, w in walls.items()))
+    g, o = pair(m, text, (rho, u, rhoE))
+    check_state_and_rhs(g, o, "orifice sponge")
+    check_history(g, o, 20, "orifice sponge", sponge=True)
+
+
+def test_sponge_refuses_a_source_hook():
+    from nextgen_amd.ideal_gas import CharlesError
+    m = Mesh.channel(10, jitter=0.0)
+    g = IdealGasSolver(m, parse_params(deck(WALLS["cbc_upt+sponge"])))
+    with pytest.raises(CharlesError, match="SPONGE"):
+        g.set_source_hook(lambda rhs, time, stage: None)
+    g.close()
 
 
 @pytest.mark.parametrize("prec", ["FP64", "MIXED"])
