@@ -39,6 +39,7 @@ core-s/Mcv/step.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -520,6 +521,8 @@ def main():
     ap.add_argument("--accuracy", type=int, default=0, help="compare mixed with FP64 over this many steps")
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--copy-calibration", type=int, default=0, help="run only a device copy of this many bytes")
+    ap.add_argument("--hbm-probe", type=float, default=0.0,
+                    help="run only the HBM reference probe (streaming reads, copy, gather) on buffers of this many GiB")
     ap.add_argument("--ensemble", type=int, default=0, help="batched ensemble of this many solves")
     ap.add_argument("--ens-steps", type=int, default=300)
     ap.add_argument("--serve", type=int, default=0, help="serving loop over this many requests")
@@ -539,6 +542,13 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0
+    if args.hbm_probe:
+        # a child process (built with the library): this one has not touched the GPU
+        probe = os.path.join(ROOT, "nextgen_amd", "lib", "hbm_probe")
+        if not os.path.exists(probe):
+            print(f"{probe} is not built (make -C nextgen_amd/csrc)", file=sys.stderr)
+            return 1
+        return subprocess.run([probe, str(args.hbm_probe)]).returncode
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported by this build "
               "(the path is single-GPU: no sharding, no collectives)", file=sys.stderr)

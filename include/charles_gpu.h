@@ -122,6 +122,11 @@ typedef struct cg_bc {
   double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
   double L;       /* NSCBC outlet length scale, SPONGE LENGTH */
   int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ */
+  /* passive scalars entering through a CBC zone (parseScalarBc, BcCommon.hpp:213-274):
+   * the value of each registered scalar (by its index in cg_params.sc_names), given on
+   * the zone's line as "<name> <value>" pairs, every scalar required; an NSCBC
+   * outlet or a sponge brings in 0 on backflow, walls pass no scalar flux */
+  double sc[CG_MAX_SCALARS];
 } cg_bc;
 
 /* Input-file parameters of the path; the names in comments are the reference's
@@ -238,16 +243,17 @@ int cg_mesh_from_arrays(int ncv, const double* x_cv, const double* vol, int nfa,
  *   nno_p, nno_pb, nef -- the globals read_mles sets (the fields
  *   src/core/common/tests/testStripedMesh.cpp checks); nef is 0 (extended
  *   faces are built by the solver, not stored).
- * cg_mesh_read_mles: the mesh of a fully periodic file (three axis-aligned
- *   Cartesian periodic transforms, no boundary faces; face zones 0..25 give the
- *   periodic image of cvofa[f][1], StripedMesh.hpp:3043-3105), with cells in
- *   the file's order and operators built as in cg_mesh_from_arrays. A file
- *   with boundary faces is rejected (CG_EMESH): this solver has no boundary
- *   conditions.
- * cg_mesh_write_mles: a periodic mesh as a geometry-only file (counts, periodic
- *   transforms, cvofa, one fluid cell zone, vol_cv, x_vv, r_vv, x_cv, n_fa, x_fa;
- *   faces in zone order as StitchNS::writeMles, StitchNS.cpp:8122, but no node
- *   records) that cg_mesh_read_mles reads back to the same mesh. */
+ * cg_mesh_read_mles: the mesh of a file (axis-aligned Cartesian periodic
+ *   transforms, if any; face zones 0..25 give the periodic image of
+ *   cvofa[f][1], StripedMesh.hpp:3043-3105; boundary zones with their faces,
+ *   area_over_delta_bf and Gij_bf as stored), with cells in the file's order
+ *   and operators built as in cg_mesh_from_arrays; the domain's extents (for a
+ *   sponge) come from its node records.
+ * cg_mesh_write_mles: the mesh as a geometry-only file (counts, periodic
+ *   transforms, boundary zones and faces, cvofa, one fluid cell zone, vol_cv,
+ *   x_vv, r_vv, x_cv, n_fa, x_fa; faces in zone order as StitchNS::writeMles,
+ *   StitchNS.cpp:8122; as node records only the nodes that attain the domain's
+ *   extents) that cg_mesh_read_mles reads back to the same mesh. */
 int cg_mles_counts(const char* path, int64_t counts[8]);
 /* Copy a record array of a .mles file out, as stored (the StripedMesh arrays
  * read_mles fills): int64 cvofa [nfa][2], cvobf [nbf], noofa_i [nfa+1] (CSR

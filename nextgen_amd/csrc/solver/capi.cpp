@@ -916,6 +916,11 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_area[z] = lay->zone_area[z];
       hi->P.bc_sigma[z] = p->bc[k].sigma;
       hi->P.bc_L[z] = p->bc[k].L;
+      // backflow through an NSCBC outlet or a sponge brings in 0 (their
+      // initData, IdealGasSolverBcs.cpp:3928-3930,4124-4126)
+      const bool cbc = p->bc[k].type >= CG_BC_CBC_UPT && p->bc[k].type <= CG_BC_CBC_RUP;
+      for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
+        hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
       if (p->bc[k].type == CG_BC_SPONGE) {
         // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
@@ -931,9 +936,6 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     if (hi->sponge && !lay->has_x_max)
       return s->set(CG_EINVAL, "a SPONGE zone needs the mesh's extents (node coordinates or box), which a mesh "
                                "from raw arrays does not carry");
-    if (lay->g.nbf > 0 && (p->sgs_model == CG_SGS_DSM_LOCAL || p->nsc > 0))
-      return s->set(CG_EINVAL, "SGS_MODEL DSM_LOCAL and passive scalars need a periodic mesh (their boundary "
-                               "terms are not on this path)");
   }
   const int ncv = lay->ncv, nfa = lay->nfa;
   hi->ncv = ncv;
@@ -959,6 +961,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     for (int k = 0; k < 2; ++k) rc |= dev_alloc(s, h, &b.st[k].sc, (size_t)p->nsc * ncv);
     rc |= dev_alloc(s, h, &w.rhs_sc0, (size_t)p->nsc * ncv);
     rc |= dev_alloc(s, h, &w.rhs_sc1, (size_t)p->nsc * ncv);
+    if (lay->g.nbf > 0) rc |= dev_alloc(s, h, &w.mf_bf, (size_t)lay->g.nbf);
   }
   rc |= dev_alloc(s, h, &w.aux, (size_t)ncv * cgdev::A_N);
   if (!mixed) rc |= dev_alloc(s, h, &w.logp, ncv);
@@ -1060,6 +1063,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       HIPCHK(s, hipMemcpy(b.st[k].sc, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(w.rhs_sc0, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(w.rhs_sc1, zero.data(), sizeof(double) * zero.size(), hipMemcpyHostToDevice));
+    if (w.mf_bf) rc |= cgk::launch_fill(w.mf_bf, 0.0, (int64_t)lay->g.nbf, stream);
   }
   rc |= cgk::launch_fill(w.rhs0, R(0), 5 * (int64_t)ncv, stream);
   rc |= cgk::launch_fill(w.rhs1, R(0), 5 * (int64_t)ncv, stream);
@@ -1569,6 +1573,27 @@ static bool is_bc_type(const std::string& t) {
 
 int cg_params_parse(cg_params* p, const char* text) {
   if (!p || !text) return fail(CG_EINVAL, "null argument");
+  // the scalar names the text registers, wherever their lines sit: a CBC line
+  // names its scalars' inflow values (parseScalarBc runs after the whole deck
+  // is read)
+  std::vector<std::string> sc_known(p->sc_names, p->sc_names + std::max(0, std::min(p->nsc, CG_MAX_SCALARS)));
+  {
+    std::istringstream pre(text);
+    std::string ln;
+    while (std::getline(pre, ln)) {
+      const size_t hash = ln.find('#');
+      if (hash != std::string::npos) ln = ln.substr(0, hash);
+      std::istringstream ls(ln);
+      std::string k, t;
+      if (!(ls >> k)) continue;
+      if (upper(k) == "RES_TIME") sc_known.push_back("RES_TIME");
+      if (upper(k) == "REGISTER_SCALAR")
+        while (ls >> t)
+          if (t != "=") sc_known.push_back(t);
+    }
+  }
+  // per zone line: its "<scalar> <value>" pairs, resolved to scalar indices at the end
+  std::vector<std::pair<std::string, std::vector<std::pair<std::string, double>>>> zone_sc;
   std::istringstream in(text);
   std::string line;
   int lineno = 0;
@@ -1689,6 +1714,23 @@ int cg_params_parse(cg_params* p, const char* text) {
       std::memset(&b, 0, sizeof(b));
       if (tok[0].size() >= sizeof(b.zone)) return bad("zone name too long");
       std::snprintf(b.zone, sizeof(b.zone), "%s", tok[0].c_str());
+      // CBC zones: "<scalar> <value>" pairs anywhere after the type
+      // (BcTemplate::parseScalarBc, BcCommon.hpp:213-274)
+      std::vector<std::pair<std::string, double>> scv;
+      if (type.compare(0, 4, "CBC_") == 0) {
+        for (size_t i = 2; i < tok.size();) {
+          if (std::find(sc_known.begin(), sc_known.end(), tok[i]) == sc_known.end()) {
+            ++i;
+            continue;
+          }
+          double val = 0.0;
+          if (!num(i + 1, &val)) return bad("expected a value after scalar " + tok[i]);
+          for (const auto& e : scv)
+            if (e.first == tok[i]) return bad("multiply specified scalar bc for " + tok[i]);
+          scv.emplace_back(tok[i], val);
+          tok.erase(tok.begin() + long(i), tok.begin() + long(i) + 2);
+        }
+      }
       if (type == "SLIP" || type == "SYMMETRY") {
         b.type = CG_BC_SLIP;
       } else if (type == "WALL_ADIABATIC") {
@@ -1802,6 +1844,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         slot = p->nbc++;
       }
       p->bc[slot] = b;
+      if (b.type >= CG_BC_CBC_UPT && b.type <= CG_BC_CBC_RUP) zone_sc.emplace_back(b.zone, std::move(scv));
     } else if (key == "DISS_SPONGE" || key == "ROTATING_REFERENCE_FRAME" || key == "ADD_BODY_FORCE") {
       // body forces (addBodyForceSourceTerms, IdealGasSolver.cpp:2159-2202) and the
       // rotating frame are not on this path; a per-stage source can be added with
@@ -1809,6 +1852,21 @@ int cg_params_parse(cg_params* p, const char* text) {
       return bad("not available on this path");
     }
     // other keys (RESTART, NSTEPS, INIT, EOS, ...) belong to the driver
+  }
+  // every registered scalar needs its inflow value on each CBC line of this text
+  // (parseScalarBc: "failed to provide scalar bc"); the values by scalar index
+  for (const auto& zs : zone_sc) {
+    for (int i = 0; i < p->nbc; ++i) {
+      if (zs.first != p->bc[i].zone) continue;
+      for (int isc = 0; isc < p->nsc; ++isc) {
+        const char* nm = p->sc_names[isc];
+        auto it = std::find_if(zs.second.begin(), zs.second.end(), [&](const auto& e) { return e.first == nm; });
+        if (it == zs.second.end())
+          return fail(CG_EINVAL, std::string("zone ") + zs.first + ": no value for scalar " + nm +
+                                     " (give \"" + nm + " <value>\" on its line)");
+        p->bc[i].sc[isc] = it->second;
+      }
+    }
   }
   return CG_OK;
 }

@@ -144,9 +144,14 @@ struct WorkT {
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
+  // passive scalars on a mesh with boundary faces (null otherwise): each
+  // boundary face's mass flux of the stage (FP64, the Riemann flux's F_rho on
+  // CBC, NSCBC and sponge faces), written by the flux pass for the scalar pass
+  double* mf_bf;
 };
 
 constexpr int kMaxBcZones = 16;
+constexpr int kMaxScalars = 8;  // passive scalars per solve
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9
@@ -172,6 +177,7 @@ struct Params {
   double bc_sigma[kMaxBcZones], bc_L[kMaxBcZones];  // NSCBC outlet sigma and L (p_ref in bc_p)
   int sponge_zone;                 // the SPONGE zone, or -1
   double sponge_p, sponge_trelax;  // its pressure and relaxation time
+  double bc_sc[kMaxBcZones][kMaxScalars];  // scalars entering through a zone (CBC values, 0 otherwise)
 };
 
 using Geom = GeomT<double>;
@@ -193,8 +199,7 @@ struct Inst {
   int npblocks;  // blocks of kBlock compact pairs
 };
 
-constexpr int kBlkBits = 20;
-constexpr int kMaxScalars = 8;  // passive scalars per solve  // up to 2^20 blocks (268M cells) per instance, 2^11 instances
+constexpr int kBlkBits = 20;  // up to 2^20 blocks (268M cells) per instance, 2^11 instances
 
 }  // namespace cgdev
 

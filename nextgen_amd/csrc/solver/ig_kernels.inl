@@ -878,6 +878,7 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       }
       double F[5];
       riemann_flux(F, nd, ua, va, pa, ha, ub, vb, pb, hb, P.gamma);
+      if (w.mf_bf) G(w.mf_bf)[b] = F[0];  // mf[ibf] = flux.rho, for the scalar pass
       if (q && update) {
         #pragma clang fp contract(off)
         const double dt = G(w.info)->dt;
@@ -1195,6 +1196,20 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
       else acc[isc] -= fsc;
     }
   };
+  // boundary faces first, in zone order (calcRhsScalars, IdealGasSolver.cpp:546-549):
+  // _addScalarBoundaryFlux (BcCommon.hpp:276-291) on the faces that carry a
+  // mass flux, the zone's value coming in, the cell's going out; walls pass none
+  if (w.mf_bf) {
+    const int b1 = G(g.bf_ptr)[c + 1];
+    for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
+      const int z = G(g.bf_zone)[b];
+      if (P.bc_type[z] < BC_CBC_UPT) continue;
+      const double mf = G(w.mf_bf)[b];
+#pragma unroll
+      for (int isc = 0; isc < kMaxScalars; ++isc)
+        if (isc < nsc) acc[isc] -= mf * (mf < 0.0 ? P.bc_sc[z][isc] : phic[isc]);
+    }
+  }
   // compact pairs, pair order
   const int nc = G(g.ef_nc)[pos];
   const int cbase = G(g.cf_ptr)[pos >> 6] + lane;
@@ -1587,6 +1602,21 @@ __global__ __launch_bounds__(kBlock) void k_dsm_filter(const Inst<R>* __restrict
     wgt += fax;
 #pragma unroll
     for (int i = 0; i < 16; ++i) f[i] += (0.5 * (qc[i] + qn[i])) * fax;
+  }
+  // boundary faces after them (filterCvR1/R2, StaticSolver.cpp:9891-9902):
+  // the cell's own value over the pyramid |n_bf| delta / 3, delta = |n_bf| /
+  // area_over_delta_bf
+  if (g.nbf > 0) {
+    const int b1 = G(g.bf_ptr)[c + 1];
+    for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
+      const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
+      const double area = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      const double delta = area / G(g.bf_aod)[b];
+      const double vb = delta * area / 3.0;
+      wgt += vb;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) f[i] += qc[i] * vb;
+    }
   }
   const auto out = G(w.dsm + (size_t)c * 16);
 #pragma unroll

@@ -41,7 +41,7 @@ CG_BC_NSCBC_OUTLET_P, CG_BC_SPONGE = 8, 9
 class cg_bc(C.Structure):
     _fields_ = [("zone", C.c_char * 48), ("type", C.c_int), ("T", C.c_double), ("u", C.c_double * 3),
                 ("p", C.c_double), ("rho", C.c_double), ("q", C.c_double), ("sigma", C.c_double),
-                ("L", C.c_double), ("mode", C.c_int)]
+                ("L", C.c_double), ("mode", C.c_int), ("sc", C.c_double * 8)]
 
 
 class cg_params(C.Structure):
@@ -295,15 +295,17 @@ class Mesh:
 
     @classmethod
     def read_mles(cls, path, params=None):
-        """The periodic mesh of a .mles file (StripedMesh::read_mles,
-        src/core/common/StripedMesh.hpp:622), cells in the file's order."""
+        """The mesh of a .mles file (StripedMesh::read_mles,
+        src/core/common/StripedMesh.hpp:622), cells in the file's order, with its
+        periodic transforms and boundary zones."""
         lib = load_library()
         h = C.c_void_p()
         _check(lib.cg_mesh_read_mles(os.fsencode(path), C.byref(params or default_params()), C.byref(h)))
         return cls(h)
 
     def write_mles(self, path):
-        """Write the mesh as a geometry-only .mles file (no node records)."""
+        """Write the mesh as a geometry-only .mles file (its only node records are
+        the nodes that attain the domain's extents)."""
         _check(load_library().cg_mesh_write_mles(self._h, os.fsencode(path)))
 
     def _size(self, name):

@@ -189,6 +189,8 @@ struct og_solver {
   /* Sponge (IdealGasSolverBcs.hpp): T_sponge, u_sponge [ncv][4] and the two
    * snapshots */
   double *sponge, *sponge_store, *sponge_recent;
+  /* each boundary face's mass flux of the last RHS (mf, BcCommon.hpp) */
+  double* mf_bf;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -236,6 +238,7 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
     s->nscbc_store = (double*)calloc(5 * nb + 1, sizeof(double));
     s->nscbc_recent = (double*)calloc(5 * nb + 1, sizeof(double));
   }
+  s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->sponge = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_store = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_recent = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
@@ -257,7 +260,7 @@ void og_destroy(og_solver* s) {
   free(s->sc); free(s->rhs0_sc); free(s->rhs1_sc); free(s->rhs2_sc); free(s->rhs_sc_tmp);
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
-  free(s->sponge); free(s->sponge_store); free(s->sponge_recent);
+  free(s->sponge); free(s->sponge_store); free(s->sponge_recent); free(s->mf_bf);
   free(s);
 }
 
@@ -566,6 +569,17 @@ static void filter_cv(const og_mesh* m, const double* phi, double* phif, int nco
       phif[(size_t)icv0 * ncomp + k] += c_avg * fax;
       phif[(size_t)icv1 * ncomp + k] += c_avg * fax;
     }
+  }
+  /* boundary faces (StaticSolver.cpp:9891-9902): the cell's own value over the
+   * pyramid of the face, area * delta / 3 with delta = area / area_over_delta_bf */
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int icv = m->cvobf[ibf];
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double delta = area / m->aod_bf[ibf];
+    const double vol_ = delta * area / 3.0;
+    wgt[icv] += vol_;
+    for (int k = 0; k < ncomp; ++k) phif[(size_t)icv * ncomp + k] += phi[(size_t)icv * ncomp + k] * vol_;
   }
   for (int icv = 0; icv < m->ncv; ++icv)
     for (int k = 0; k < ncomp; ++k) phif[(size_t)icv * ncomp + k] /= wgt[icv];
@@ -1134,7 +1148,7 @@ static void sponge_volume(const og_solver* s, og_rhs* rhs) {
 /* addBoundaryFlux of every zone in zone order (IdealGasSolverBcs.cpp:14-47 SlipWallBc /
  * WallAdiabatic, :1485-1510 WallIsothermal), first in calcRhsBase
  * (IdealGasSolver.cpp:261-262): stage p, u, T; lagged mu_lam, loc_lam */
-static void boundary_flux(const og_solver* s, og_rhs* rhs) {
+static void boundary_flux(og_solver* s, og_rhs* rhs) {
   const og_mesh* m = &s->m;
   const og_params* P = &s->p;
   const double cp = s->R_gas * P->gamma / (P->gamma - 1.0);
@@ -1184,6 +1198,7 @@ static void boundary_flux(const og_solver* s, og_rhs* rhs) {
       }
       double F[5];
       og_flux_riemann(F, n, &s->cv_light[icv], &b, P->gamma, P->gamma);
+      s->mf_bf[ibf] = F[0]; /* mf[ibf] = flux.rho */
       rhs[icv].rho -= F[0];
       for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= F[1 + i];
       rhs[icv].rhoE -= F[4];
@@ -1205,6 +1220,19 @@ static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
   og_rhs* rhs = (og_rhs*)calloc(n, sizeof(og_rhs));
   if (nsc > 0) memset(rhs_sc, 0, sizeof(double) * (size_t)nsc * n);
   boundary_flux(s, rhs);
+  /* calcRhsScalars (IdealGasSolver.cpp:546-549): each zone's
+   * addScalarBoundaryFlux first, _addScalarBoundaryFlux (BcCommon.hpp:276-291)
+   * on the faces that carry a mass flux; walls pass no scalar flux */
+  for (int ibf = 0; ibf < m->nbf && nsc > 0; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] < 4) continue;
+    const int icv = m->cvobf[ibf];
+    const double mf = s->mf_bf[ibf];
+    for (int isc = 0; isc < nsc; ++isc) {
+      if (mf < 0.0) rhs_sc[(size_t)icv * nsc + isc] -= mf * s->p.bc_sc[z][isc];
+      else rhs_sc[(size_t)icv * nsc + isc] -= mf * s->sc[(size_t)icv * nsc + isc];
+    }
+  }
   if (nt_ > 1 && nsc == 0) {
     /* every pair's flux, then each cell adds its pairs' fluxes in pair order */
     build_cell_pairs(s);

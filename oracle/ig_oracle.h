@@ -113,6 +113,9 @@ typedef struct og_params {
    * 2 L_Z, 3 L_RX, 4 L_RZ) with strength sponge_strength x_sp^2 */
   int sponge_zone, sponge_type;
   double sponge_strength, sponge_length, sponge_p, sponge_trelax;
+  /* passive scalars entering through each zone's faces that carry a mass flux
+   * (CBC values from the zone's line; 0 for NSCBC outlets and the sponge) */
+  double bc_sc[16][8];
 } og_params;
 
 typedef struct og_diag {

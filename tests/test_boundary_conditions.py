@@ -27,6 +27,12 @@ terms, pinned by closed forms.
   long (the wall row adding the Riemann flux's mass) and nothing below; the
   stage-1 relaxation moves T_sponge, u_sponge by eps = 1 / (t_relax / dt + 1)
   of the way to a changed state, in every cell.
+* Passive scalars through boundaries (_addScalarBoundaryFlux, BcCommon.hpp:276-291):
+  at rest and scalar-free, a CBC inflow zone with value Z_in feeds the wall cells
+  -mf Z_in (mf the Riemann flux's mass, negative inflow); a uniform stream and a
+  uniform scalar equal to the inflow value keep a zero scalar RHS and stay uniform
+  over five steps, whatever the outflow zone's value (outflow carries the cell's);
+  walls pass no scalar flux; the parser takes "<name> <value>" pairs on CBC lines.
 * The parser: zone conditions by name, the reference's other types refused, a
   zone without a condition refused when the solver is built.
 """
@@ -353,3 +359,64 @@ def test_sponge_relaxes_toward_a_changed_state():
     eps = 1.0 / (t_relax / dt + 1.0)
     np.testing.assert_allclose(o.get_field("T_sponge"), 1.0 + eps * (T1 - 1.0), rtol=1e-14)
     np.testing.assert_allclose(o.get_field("u_sponge"), eps * np.tile(U, (m.ncv, 1)), rtol=1e-13, atol=1e-16)
+
+
+def test_scalar_inflow_values_parser():
+    text = "y0 CBC_UPT 0 1 0 71.4 1.0 Zb 2.5 Za 0.5\ny1 CBC_RUNP 1.0 -1 71.4 Za 0 Zb 0\nREGISTER_SCALAR Zb Za\n"
+    p = parse_params(deck({}) + text)
+    assert [p.sc_names[i].value for i in range(p.nsc)] == [b"Za", b"Zb"]
+    assert list(p.bc[0].sc[:2]) == [0.5, 2.5] and list(p.bc[1].sc[:2]) == [0.0, 0.0]
+    assert (p.bc[0].p, p.bc[0].T) == (71.4, 1.0)
+    p = parse_params("y1 NSCBC_OUTLET_P 71.4 0.2 1.0\ny0 SLIP\nREGISTER_SCALAR Za\n")
+    assert p.nbc == 2  # outlets and walls take no scalar values
+    for bad in ("y0 CBC_UPT 0 1 0 71.4 1.0 Za 0.5\nREGISTER_SCALAR Za Zb\n",
+                "y0 CBC_UPT 0 1 0 71.4 1.0 Za\nREGISTER_SCALAR Za\n",
+                "y0 CBC_UPT 0 1 0 71.4 1.0 Za 1 Za 2\nREGISTER_SCALAR Za\n"):
+        with pytest.raises(CharlesError):
+            parse_params(bad)
+
+
+def test_scalar_inflow_closed_form_at_rest():
+    # gas at rest, no scalar; y0 a CBC_UPT inflow state carrying Z = 1.5: the
+    # wall cells' scalar RHS is -mf 1.5 with mf the Riemann flux's mass through
+    # the face (negative: inflow); y1 a slip wall passes nothing
+    from oracle.oracle import lib
+    import ctypes as C
+    m = Mesh.channel(10, jitter=0.0)
+    R, p, T = 71.42857142857143, 71.42857142857143, 1.0
+    U = 2.0
+    text = deck({"y0": f"CBC_UPT 0 {U} 0 {p!r} {T} Z 1.5", "y1": "SLIP"}) + "REGISTER_SCALAR Z\n"
+    o = oracle(m, text, uniform(m, p=p, T=T, R=R))
+    rs = o.calc_rhs_scalars()[:, 0]
+    nb = np.array([0.0, -H * H, 0.0])
+    a = np.array([0.0, 0.0, 0.0, R * T / p, p, R * T * 1.4 / 0.4])
+    b = np.array([0.0, U, 0.0, R * T / p, p, R * T * 1.4 / 0.4])
+    f = np.zeros(5)
+    D = C.POINTER(C.c_double)
+    lib().og_flux_riemann(f.ctypes.data_as(D), nb.ctypes.data_as(D), a.ctypes.data_as(D), b.ctypes.data_as(D),
+                          1.4, 1.4)
+    assert f[0] < 0.0
+    lower = wall_cells(m, 0)
+    np.testing.assert_allclose(rs[lower], -f[0] * 1.5, rtol=1e-13)
+    rest = np.setdiff1d(np.arange(m.ncv), lower)
+    assert np.abs(rs[rest]).max() == 0.0
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_uniform_scalar_through_the_channel_stays_uniform(jitter):
+    # a uniform stream V through the channel (in at y0, out at y1, both CBC_UPT
+    # of the stream) and a uniform scalar equal to the inflow value: the scalar
+    # RHS vanishes (the outflow zone's own value, 9, is never used) and five
+    # steps keep Z = 3
+    m = Mesh.channel(10, jitter=jitter, seed=13)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    text = deck({"y0": f"CBC_UPT 0 {V} 0 {p!r} {T} Z 3.0", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T} Z 9.0"})
+    o = oracle(m, text + "REGISTER_SCALAR Z\n", uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    o.set_scalar(0, np.full(m.ncv, 3.0))
+    rs = o.calc_rhs_scalars()[:, 0]
+    assert np.abs(rs).max() < 1e-11 * 3.0 * V * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    np.testing.assert_allclose(o.get_scalar(0), 3.0, rtol=1e-12)

@@ -12,7 +12,11 @@ pinned by what the model's mathematics fixes rather than by its own formulas:
   x_n = LM (1 - r^n), r = 1/(1 + eps), toward the local Germano-Lilly value
   c_delta^2 = -L:M / (2 M:M); mu_sgs = c_delta^2 rho |S|;
 * a one-dimensional shear u = (A sin ky, 0, 0) has L:M = 0 (L is diagonal, M
-  off-diagonal), hence no eddy viscosity.
+  off-diagonal), hence no eddy viscosity;
+* next to a wall the filter adds the cell's own value over the pyramid of each
+  boundary face (area x delta / 3): on the lattice channel a unit spike at a wall
+  cell filters to (5 x 1/2 + 1) / 6 = 7/12 (1/2 away from the walls) and 1/12 at
+  each face neighbour, and the weights still sum to the cell volume.
 """
 import numpy as np
 import pytest
@@ -147,3 +151,26 @@ def test_one_dimensional_shear_has_no_eddy_viscosity():
     o = _dsm_after_relax(m, u, 0.02, n_props=5)
     assert np.max(np.abs(o.get_field("lijmij"))) < 1e-14
     assert np.max(o.get_field("mu_sgs")) < 1e-14
+
+
+def test_filter_boundary_faces_on_the_lattice_channel():
+    m = Mesh.channel(10, jitter=0.0)
+    text = tgv_params(Re=1600.0, Ma=0.1, sgs="DSM_LOCAL") + "y0 SLIP\ny1 WALL_ADIABATIC\n"
+    o = OracleSolver(m, parse_params(text))
+    cvobf, zone = m.get("cvobf"), m.get("zone_bf")
+    wall = int(cvobf[zone == 0][3])
+    cvofa = m.get("cvofa")
+    inner = int(cvofa[0, 0]) if cvofa[0, 0] not in cvobf else int(np.setdiff1d(np.arange(m.ncv), cvobf)[0])
+    for c, own in ((wall, 7.0 / 12.0), (inner, 0.5)):
+        phi = np.zeros(m.ncv)
+        phi[c] = 1.0
+        f = o.filter_cv(phi)
+        assert abs(f[c] - own) < 1e-14, (c, f[c], own)
+        nbrs = np.concatenate([cvofa[cvofa[:, 0] == c, 1], cvofa[cvofa[:, 1] == c, 0]])
+        np.testing.assert_allclose(f[nbrs], 1.0 / 12.0, rtol=1e-13)
+        rest = np.setdiff1d(np.arange(m.ncv), np.concatenate([[c], nbrs]))
+        assert np.abs(f[rest]).max() == 0.0
+    # constants survive on a jittered channel (weights of faces and boundary faces alike)
+    mj = Mesh.channel(10, jitter=0.3, seed=5)
+    oj = OracleSolver(mj, parse_params(text))
+    np.testing.assert_allclose(oj.filter_cv(np.full(mj.ncv, 2.5)), 2.5, rtol=1e-14)

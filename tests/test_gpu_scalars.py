@@ -1,6 +1,8 @@
 """Passive scalars on the device: against the oracle (FP64 to 1e-12 of the field's
 magnitude, mixed precision to 1e-5), without effect on the flow, the same in a
-batch as alone, and carried by data files."""
+batch as alone, and carried by data files; on channel meshes through CBC inflows
+(the zone's value coming in), CBC, NSCBC and sponge outflows (the cell's value
+going out, 0 on backflow) and walls (no scalar flux)."""
 import numpy as np
 import pytest
 
@@ -103,3 +105,60 @@ def test_scalars_travel_in_data_files(tmp_path):
     for n, v in ref.items():
         np.testing.assert_array_equal(b.get_scalar(n), v)
     b.close()
+
+
+P0 = 1.0 / (1.4 * 0.1 ** 2)
+CHANNEL_ZONES = {
+    "cbc": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0 Zmix 1.0 C2 0.25 RES_TIME 0.0",
+            "y1": f"CBC_RUNP 1.0 -0.2 {P0 * 0.999!r} Zmix 0.0 C2 0.0 RES_TIME 0.0"},
+    "nscbc": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0 Zmix 1.0 C2 0.25 RES_TIME 0.0",
+              "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    "sponge+wall": {"y0": f"SPONGE TYPE L_Y LENGTH 2.0 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {P0 * 1.002!r}",
+                    "y1": "WALL_ADIABATIC"},
+}
+
+
+def channel_state(m, Ma=0.1, gamma=1.4):
+    """A shear layer between the channel's walls with cellular perturbations."""
+    x = m.get("x_cv")
+    p0 = 1.0 / (gamma * Ma * Ma)
+    y = (x[:, 1] - np.pi) / np.pi
+    u = np.empty((m.ncv, 3))
+    u[:, 0] = 1.0 - y * y + 0.2 * np.sin(x[:, 0]) * np.cos(x[:, 2])
+    u[:, 1] = 0.1 * np.sin(x[:, 2]) * np.sin(x[:, 1])
+    u[:, 2] = -0.2 * np.cos(x[:, 0]) * np.sin(x[:, 2])
+    p = p0 + 0.05 * np.cos(x[:, 0]) * np.cos(2 * x[:, 1])
+    rho = p / p0
+    return rho, u, p / (gamma - 1.0) + 0.5 * rho * (u * u).sum(1)
+
+
+def channel_scalar_run(zones, prec, nsteps):
+    m = Mesh.channel(10, jitter=0.3, seed=21)
+    deck = (tgv_params(sgs="VREMAN") + "".join(f"{z} {w}\n" for z, w in zones.items()) + SC)
+    st = channel_state(m)
+    f = fields(m.get("x_cv"))
+    g = IdealGasSolver(m, parse_params(deck + f"PRECISION {prec}\n"))
+    g.set_state(*st)
+    for name, v in f.items():
+        g.set_scalar(name, v)
+    g.init_complete()
+    o = OracleSolver(m, parse_params(deck))
+    o.set_state(*st)
+    o.set_scalar(0, f["C2"])
+    o.set_scalar(2, f["Zmix"])
+    o.init_complete()
+    g.advance_solution(nsteps)
+    o.step(nsteps)
+    return m, g, o
+
+
+@pytest.mark.parametrize("zones", sorted(CHANNEL_ZONES))
+@pytest.mark.parametrize("prec,tol", [("FP64", 1e-12), ("MIXED", 1e-5)])
+def test_channel_scalars_through_boundaries_match_oracle(zones, prec, tol):
+    m, g, o = channel_scalar_run(CHANNEL_ZONES[zones], prec, 12)
+    for i, name in enumerate(["C2", "RES_TIME", "Zmix"]):
+        a, b = g.get_scalar(name), o.get_scalar(i)
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"{zones} {prec} {name}: {err:.3e}", flush=True)
+        assert err <= tol, name
+    g.close()
