@@ -100,6 +100,16 @@ typedef struct cg_solver cg_solver;
  *                                    LENGTH of the domain's largest x, y, z or
  *                                    radius, the state relaxed toward the
  *                                    cells' once per step; one sponge zone
+ *   WM_ALG_ADIABATIC [TRANSLATE u v w] [RELAX r] [Y_PLUS0 y] [TOL t]
+ *                                    WmAdiabatic with WM_ALGEBRAIC_ADIABATIC
+ *                                    (:996-1441): per face a wall stress
+ *                                    tau_wall, solved at the start of each step
+ *                                    from the law of the wall (AlgebraicWM::
+ *                                    solve_tau, wm/AlgebraicWM.hpp:26-113) for
+ *                                    the cell's wall-parallel velocity at
+ *                                    y1 = area / area_over_delta, applied as a
+ *                                    force along it; RELAX, Y_PLUS0, TOL belong
+ *                                    to the equilibrium model and are ignored
  * The other types of the reference are refused by cg_params_parse. */
 #define CG_MAX_BC_ZONES 16
 #define CG_BC_SLIP 1
@@ -111,6 +121,7 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_CBC_RUP 7
 #define CG_BC_NSCBC_OUTLET_P 8
 #define CG_BC_SPONGE 9
+#define CG_BC_WM_ALG_ADIABATIC 10
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */

@@ -144,8 +144,10 @@ struct HostInst {
   double* store_recent = nullptr;    // newer snapshot
   size_t snap_n = 0;                 // doubles per snapshot
   size_t snap_bc = 0;                // of them, the NSCBC faces' states (5 per boundary face)
+  size_t snap_wm = 0;                // then the wall-model faces' stresses (1 per boundary face)
   size_t snap_sp = 0;                // and, at the end, the sponge's relaxed T, u (4 per cell)
   bool nscbc = false;                // some zone is an NSCBC outlet
+  bool wm = false;                   // some zone is a WM_ALG_ADIABATIC wall
   bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
@@ -922,6 +924,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
+      hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC;
       if (p->bc[k].type == CG_BC_SPONGE) {
         // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
         // u_sponge are solver-wide cell data, so one sponge zone per solve
@@ -990,6 +993,12 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= dev_alloc(s, h, &w.nscbc_rhs, 15 * nbf);
     hi->snap_bc = 5 * nbf;
     hi->snap_n += hi->snap_bc;
+  }
+  if (hi->wm) {
+    // WmAdiabatic::storeData: the faces' tau_wall (in the snapshots too)
+    rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf);
+    hi->snap_wm = (size_t)lay->g.nbf;
+    hi->snap_n += hi->snap_wm;
   }
   std::vector<double> sp_coeff;
   if (hi->sponge) {
@@ -1072,6 +1081,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= cgk::launch_fill(w.nscbc, 0.0, 5 * (int64_t)lay->g.nbf, stream);
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
   }
+  if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)lay->g.nbf, stream);  // set by initComplete
   if (hi->sponge) {
     // the relaxed state is set from the cells by initComplete
     rc |= cgk::launch_fill(w.sponge, 0.0, 4 * (int64_t)ncv, stream);
@@ -1255,7 +1265,7 @@ template <class R>
 int nscbc_init(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   const bool sponge = I->sponge && !I->sponge_kept;
   I->sponge_kept = false;
-  if (!I->nscbc && !sponge) return 0;
+  if (!I->nscbc && !sponge && !I->wm) return 0;
   if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, sponge, stream))
     return launch_fail(s, "boundary state init", le);
   return 0;
@@ -1269,9 +1279,15 @@ int snapshot_copy(Err* s, Instance<R>* I, int cur, double* dst, bool to_snapshot
   const cgdev::StateT<R>& st = I->b.st[cur];
   if (I->snap_bc) {
     // Nscbc::storeData / restoreData: the NSCBC faces' states after the flow's
-    double* bc = dst + (I->snap_n - I->snap_sp - I->snap_bc);
+    double* bc = dst + (I->snap_n - I->snap_sp - I->snap_wm - I->snap_bc);
     HIPCHK(s, hipMemcpyAsync(to_snapshot ? bc : I->b.w.nscbc, to_snapshot ? I->b.w.nscbc : bc,
                              sizeof(double) * I->snap_bc, dir, stream));
+  }
+  if (I->snap_wm) {
+    // WmAdiabatic::storeData / restoreData (IdealGasSolverBcs.cpp:1122-1157): tau_wall
+    double* wm = dst + (I->snap_n - I->snap_sp - I->snap_wm);
+    HIPCHK(s, hipMemcpyAsync(to_snapshot ? wm : I->b.w.wm_tau, to_snapshot ? I->b.w.wm_tau : wm,
+                             sizeof(double) * I->snap_wm, dir, stream));
   }
   if (I->snap_sp) {
     // Sponge::storeData / restoreData (IdealGasSolverBcs.cpp:4130-4159): its T, u
@@ -1747,6 +1763,26 @@ int cg_params_parse(cg_params* p, const char* text) {
             return bad("unknown WALL_ADIABATIC option " + opt);
           }
         }
+      } else if (type == "WM_ALG_ADIABATIC") {
+        // WmAdiabatic::initData (IdealGasSolverBcs.cpp:996-1073): the wall
+        // velocity (TRANSLATE); RELAX, Y_PLUS0 and TOL are the equilibrium
+        // model's and take a value each
+        b.type = CG_BC_WM_ALG_ADIABATIC;
+        for (size_t i = 2; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          double ignored = 0.0;
+          if (opt == "TRANSLATE") {
+            for (int d = 0; d < 3; ++d)
+              if (!num(i + 1 + d, &b.u[d])) return bad("expected TRANSLATE <ux> <uy> <uz>");
+            i += 3;
+          } else if (opt == "STATIONARY" || opt == "STATIONARY_FRAME") {
+            // the wall velocity of a rotating frame; without one the wall is at rest
+          } else if (opt == "RELAX" || opt == "Y_PLUS0" || opt == "TOL") {
+            if (!num(++i, &ignored)) return bad("expected " + opt + " <value>");
+          } else {
+            return bad("unknown WM_ALG_ADIABATIC option " + opt);
+          }
+        }
       } else if (type == "WALL_ISOTHERMAL") {
         b.type = CG_BC_WALL_ISOTHERMAL;
         if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WALL_ISOTHERMAL <T>");
@@ -1833,7 +1869,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P, "
+                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE and SPONGE are)");
       }
       int slot = -1;
@@ -2299,9 +2335,11 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
   return with_inst(s, [&](auto& E, auto* I) -> int {
     (void)E;
     // the hooked stage is an RHS pass and a separate update, which do not
-    // carry the NSCBC faces' own RK update or the sponge's relaxation
+    // carry the NSCBC faces' own RK update, the sponge's relaxation or the wall
+    // model's stress update
     if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC outlet zones");
     if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
+    if (fn && I->wm) return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC zones");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;
     if (!fn) std::vector<double>().swap(I->hook_buf);

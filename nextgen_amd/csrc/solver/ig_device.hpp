@@ -144,6 +144,9 @@ struct WorkT {
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
+  // WM_ALG_ADIABATIC faces (null without them): per boundary face its wall
+  // stress tau_wall (FP64, in the snapshots)
+  double* wm_tau;
   // passive scalars on a mesh with boundary faces (null otherwise): each
   // boundary face's mass flux of the stage (FP64, the Riemann flux's F_rho on
   // CBC, NSCBC and sponge faces), written by the flux pass for the scalar pass
@@ -154,8 +157,11 @@ constexpr int kMaxBcZones = 16;
 constexpr int kMaxScalars = 8;  // passive scalars per solve
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
-  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9
+  BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
+  BC_WM_ALG_ADIABATIC = 10
 };
+// the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
+__host__ __device__ inline bool bc_riemann(int type) { return type >= BC_CBC_UPT && type <= BC_SPONGE; }
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;

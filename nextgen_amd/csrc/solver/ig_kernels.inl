@@ -773,6 +773,104 @@ __device__ __forceinline__ void sponge_volume(const GeomT<R>& g, const WorkT<R>&
   }
 }
 
+// AlgebraicWM::solve_utau / solve_tau (wm/AlgebraicWM.hpp:26-113), FP64: the
+// friction velocity of the piecewise law of the wall u+ = y+ + a1 y+^2 (y+ <=
+// y*), ln(y+)/kappa + B beyond, for the speed u1 at distance y1, by relaxed
+// Newton steps (relaxation 0.4, relative residual 1e-6, at most 1000 steps; a
+// vanishing derivative restarts from the laminar guess); rho u_tau^2
+__device__ __forceinline__ double wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess) {
+  #pragma clang fp contract(off)
+  const double ystar = 23.3244, kappa = 0.41, B = 5.2;
+  const double a1 = 0.5 * (1.0 / (kappa * ystar) - 1.0) / ystar;
+  const double nu = mu / rho;
+  double u_tau = u_tau_guess <= 0.0 ? sqrt(nu * u1 / y1) : u_tau_guess;
+  const double relax = 0.4, rel_tol = 1.0e-06;
+  const int max_iter = 1000;
+  int iter = 0;
+  while (true) {
+    ++iter;
+    double f, dfdutau;
+    const double yplus = u_tau * y1 / nu;
+    if (yplus <= ystar) {
+      f = u1 - u_tau * yplus - u_tau * a1 * yplus * yplus;
+      dfdutau = -2.0 * yplus - 3.0 * a1 * yplus * yplus;
+    } else {
+      f = u1 - u_tau / kappa * log(yplus) - u_tau * B;
+      dfdutau = -1.0 / kappa * log(yplus) - 1.0 / kappa - B;
+    }
+    if (fabs(f) < 1.0e-14 && fabs(dfdutau) < 1.0e-14) break;  // u_tau = 0
+    if (fabs(dfdutau) <= 1.0e-14 && iter < max_iter) {
+      u_tau = sqrt(nu * u1 / y1);
+      continue;
+    }
+    const double dutau = relax * f / dfdutau;
+    u_tau -= dutau;
+    double u_rel = fabs(u_tau);
+    if (u1 > 0.0) u_rel = fmin(u1, u_rel);
+    if (fabs(f / u_rel) < rel_tol || iter > max_iter) break;
+  }
+  return rho * u_tau * u_tau;
+}
+
+// A WM_ALG_ADIABATIC face b of cell c (WmAdiabatic, IdealGasSolverBcs.cpp:
+// 1256-1441): at the start of a step (stage 1 of a fused step) its wall stress
+// from the law of the wall for the cell's wall-parallel speed at y1 = area /
+// area_over_delta (calcRhs), then p n plus the stress along the parallel
+// velocity, and the wall's work (addBoundaryFlux). FP64 from the FP64 state.
+template <class R>
+__device__ __forceinline__ void wm_face(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st, const Params& P,
+                                        int c, int b, int z, const Light<R>& self, R acc[5], int stage, bool update) {
+  #pragma clang fp contract(off)
+  double rho, u[3], rhoE, spv, p, h, T;
+  cell_state64(g, st, P, c, self, rho, u, rhoE, spv, p, h, T);
+  const double n[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
+  const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double unit_n[3] = {0.0, 0.0, 0.0};
+  if (n_mag > 0.0)
+    for (int i = 0; i < 3; ++i) unit_n[i] = n[i] / n_mag;
+  const double* ubc = P.bc_u[z];
+  const auto tw = G(w.wm_tau) + b;
+  if (update && stage == 1) {
+    const double aod = G(g.bf_aod)[b];
+    double tau = 0.0;
+    if (aod > 0.0) {
+      const double y1 = n_mag / aod;
+      double u_par[3];
+      if (n_mag > 0.0) {
+        const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - un * unit_n[i] - ubc[i];
+      } else {
+        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - ubc[i];
+      }
+      const double u1 = sqrt(u_par[0] * u_par[0] + u_par[1] * u_par[1] + u_par[2] * u_par[2]);
+      const double u_tau = sqrt(*tw / rho);
+      tau = wm_solve_tau(u1, y1, rho, G(w.mu_lam)[c], u_tau);
+    }
+    *tw = tau;
+  }
+  const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+  double uph[3];
+  for (int i = 0; i < 3; ++i) uph[i] = u[i] - un * unit_n[i] - ubc[i];
+  const double u_mag = sqrt(uph[0] * uph[0] + uph[1] * uph[1] + uph[2] * uph[2]);
+  double force[3] = {0.0, 0.0, 0.0}, work = 0.0;
+  if (u_mag > 0.0) {
+    const double tau = *tw;
+    for (int i = 0; i < 3; ++i) force[i] = tau * uph[i] / u_mag * n_mag;
+    work = force[0] * ubc[0] + force[1] * ubc[1] + force[2] * ubc[2];
+  }
+  if constexpr (kMixed<R>) {
+    // momentum without the gauge's p_g n
+    for (int i = 0; i < 3; ++i) acc[1 + i] -= R((p - P.p_g) * n[i] + force[i]);
+    acc[4] -= R(work);
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      acc[1 + i] -= p * n[i];
+      if (u_mag > 0.0) acc[1 + i] -= force[i];
+    }
+    if (u_mag > 0.0) acc[4] -= work;
+  }
+}
+
 // The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
 // IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
 // face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
@@ -824,7 +922,9 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
         visc_work += ubc * tauijnj;
       }
       acc[4] -= visc_work;
-    } else if (type >= BC_CBC_UPT) {
+    } else if (type == BC_WM_ALG_ADIABATIC) {
+      wm_face(g, w, st, P, c, b, z, self, acc, stage, update);
+    } else if (bc_riemann(type)) {
       // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
       // between the cell's FP64 state and the zone's boundary state
       const double nd[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
@@ -1203,7 +1303,7 @@ __global__ __launch_bounds__(kBlock) void k_scalars(const Inst<R>* __restrict__ 
     const int b1 = G(g.bf_ptr)[c + 1];
     for (int b = G(g.bf_ptr)[c]; b < b1; ++b) {
       const int z = G(g.bf_zone)[b];
-      if (P.bc_type[z] < BC_CBC_UPT) continue;
+      if (!bc_riemann(P.bc_type[z])) continue;
       const double mf = G(w.mf_bf)[b];
 #pragma unroll
       for (int isc = 0; isc < kMaxScalars; ++isc)
@@ -2141,10 +2241,30 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
     sp[0] = T;
     for (int i = 0; i < 3; ++i) sp[1 + i] = u[i];
   }
-  if (!X.w.nscbc) return;
+  if (!X.w.nscbc && !X.w.wm_tau) return;
   const int b0 = G(g.bf_ptr)[c], b1 = G(g.bf_ptr)[c + 1];
   for (int b = b0; b < b1; ++b) {
-    if (X.P.bc_type[G(g.bf_zone)[b]] != BC_NSCBC_OUTLET_P) continue;
+    const int z = G(g.bf_zone)[b];
+    if (X.P.bc_type[z] == BC_WM_ALG_ADIABATIC) {
+      // WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1102): the
+      // laminar stress mu_lam |u - u_bc| / y1 (0 where area_over_delta is 0)
+      #pragma clang fp contract(off)
+      double rho, u[3], rhoE, spv, p, h, T;
+      const Light<R> self = load_light(X.st[cur].light, c);
+      cell_state64(g, X.st[cur], X.P, c, self, rho, u, rhoE, spv, p, h, T);
+      const double d[3] = {u[0] - X.P.bc_u[z][0], u[1] - X.P.bc_u[z][1], u[2] - X.P.bc_u[z][2]};
+      const double u1 = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double aod = G(g.bf_aod)[b];
+      double tau = 0.0;
+      if (aod > 0.0) {
+        const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
+        const double y1 = sqrt(n0 * n0 + n1 * n1 + n2 * n2) / aod;
+        tau = G(X.w.mu_lam)[c] * u1 / y1;
+      }
+      G(X.w.wm_tau)[b] = tau;
+      continue;
+    }
+    if (X.P.bc_type[z] != BC_NSCBC_OUTLET_P) continue;
     double rho, p, u[3];
     if constexpr (kMixed<R>) {
       const StateT<R>& st = X.st[cur];

@@ -191,6 +191,9 @@ struct og_solver {
   double *sponge, *sponge_store, *sponge_recent;
   /* each boundary face's mass flux of the last RHS (mf, BcCommon.hpp) */
   double* mf_bf;
+  /* WmAdiabatic (IdealGasSolverBcs.hpp): tau_wall per boundary face and the
+   * two snapshots */
+  double *wm_tau, *wm_store, *wm_recent;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -239,6 +242,9 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
     s->nscbc_recent = (double*)calloc(5 * nb + 1, sizeof(double));
   }
   s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_tau = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_store = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_recent = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->sponge = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_store = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_recent = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
@@ -261,6 +267,7 @@ void og_destroy(og_solver* s) {
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
   free(s->sponge); free(s->sponge_store); free(s->sponge_recent); free(s->mf_bf);
+  free(s->wm_tau); free(s->wm_store); free(s->wm_recent);
   free(s);
 }
 
@@ -1092,6 +1099,94 @@ static void nscbc_init(og_solver* s) {
   }
 }
 
+/* AlgebraicWM::solve_utau / solve_tau (wm/AlgebraicWM.hpp:26-113): piecewise
+ * law of the wall u+ = y+ + a1 y+^2 up to y* = 23.3244, ln(y+)/0.41 + 5.2
+ * beyond (a1 makes it C1), solved for u_tau by Newton steps relaxed by 0.4
+ * until |f| < 1e-6 min(u1, |u_tau|) (at most 1000; a vanishing derivative
+ * restarts from the laminar guess sqrt(nu u1 / y1)); returns rho u_tau^2 */
+double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess) {
+  const double ystar = 23.3244, kappa = 0.41, B = 5.2;
+  const double a1 = 0.5 * (1.0 / (kappa * ystar) - 1.0) / ystar;
+  const double nu = mu / rho;
+  double u_tau = u_tau_guess <= 0.0 ? sqrt(nu * u1 / y1) : u_tau_guess;
+  int iter = 0;
+  for (;;) {
+    ++iter;
+    double f, dfdutau;
+    const double yplus = u_tau * y1 / nu;
+    if (yplus <= ystar) {
+      f = u1 - u_tau * yplus - u_tau * a1 * yplus * yplus;
+      dfdutau = -2.0 * yplus - 3.0 * a1 * yplus * yplus;
+    } else {
+      f = u1 - u_tau / kappa * log(yplus) - u_tau * B;
+      dfdutau = -1.0 / kappa * log(yplus) - 1.0 / kappa - B;
+    }
+    if (fabs(f) < 1.0e-14 && fabs(dfdutau) < 1.0e-14) break;
+    if (fabs(dfdutau) <= 1.0e-14 && iter < 1000) {
+      u_tau = sqrt(nu * u1 / y1);
+      continue;
+    }
+    u_tau -= 0.4 * f / dfdutau;
+    double u_rel = fabs(u_tau);
+    if (u1 > 0.0 && u1 < u_rel) u_rel = u1;
+    if (fabs(f / u_rel) < 1.0e-06 || iter > 1000) break;
+  }
+  return rho * u_tau * u_tau;
+}
+
+/* the wall-parallel velocity of cell icv at face ibf: u - (u . n^) n^ - u_bc
+ * (n^ = 0 for a degenerate face) */
+static void wm_parallel(const og_solver* s, int ibf, int icv, const double* ubc, double up[3], double unit_n[3],
+                        double* n_mag) {
+  const double* n = &s->m.n_bf[3 * (size_t)ibf];
+  const double* u = &s->u[3 * (size_t)icv];
+  *n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int i = 0; i < 3; ++i) unit_n[i] = *n_mag > 0.0 ? n[i] / *n_mag : 0.0;
+  const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+  for (int i = 0; i < 3; ++i) up[i] = u[i] - un * unit_n[i] - ubc[i];
+}
+
+/* WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1102): the laminar
+ * stress mu_lam |u - u_bc| / y1, y1 = area / area_over_delta */
+static void wm_init(og_solver* s) {
+  const og_mesh* m = &s->m;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] != 10) continue;
+    const int icv = m->cvobf[ibf];
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = s->u[3 * (size_t)icv + i] - s->p.bc_u[z][i];
+    const double u1 = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    s->wm_tau[ibf] = 0.0;
+    if (m->aod_bf[ibf] > 0.0) {
+      const double* n = &m->n_bf[3 * (size_t)ibf];
+      const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
+      s->wm_tau[ibf] = s->mu_lam[icv] * u1 / y1;
+    }
+  }
+}
+
+/* WmAdiabatic::calcRhs at stage 1 (IdealGasSolverBcs.cpp:1256-1354): each
+ * face's stress from the law of the wall for the cell's parallel speed */
+static void wm_calc(og_solver* s) {
+  const og_mesh* m = &s->m;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] != 10) continue;
+    const int icv = m->cvobf[ibf];
+    if (m->aod_bf[ibf] > 0.0) {
+      double up[3], un_[3], n_mag;
+      wm_parallel(s, ibf, icv, s->p.bc_u[z], up, un_, &n_mag);
+      const double y1 = n_mag / m->aod_bf[ibf];
+      const double u1 = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+      const double u_tau = sqrt(s->wm_tau[ibf] / s->rho[icv]);
+      s->wm_tau[ibf] = og_wm_solve_tau(u1, y1, s->rho[icv], s->mu_lam[icv], u_tau);
+    } else {
+      s->wm_tau[ibf] = 0.0;
+    }
+  }
+}
+
 /* Sponge::initialHook (IdealGasSolverBcs.cpp:4311-4346): T_sponge, u_sponge
  * from the cells */
 static void sponge_init(og_solver* s) {
@@ -1174,7 +1269,23 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
         visc_work += P->bc_u[z][i] * tauijnj;
       }
       rhs[icv].rhoE -= visc_work;
-    } else if (P->bc_type[z] >= 4) {
+    } else if (P->bc_type[z] == 10) {
+      /* WmAdiabatic::addBoundaryFlux (IdealGasSolverBcs.cpp:1395-1441) */
+      double up[3], unit_n[3], n_mag;
+      wm_parallel(s, ibf, icv, P->bc_u[z], up, unit_n, &n_mag);
+      const double u_mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+      if (u_mag > 0.0) {
+        double force[3];
+        for (int i = 0; i < 3; ++i) force[i] = s->wm_tau[ibf] * up[i] / u_mag * n_mag;
+        for (int i = 0; i < 3; ++i) {
+          rhs[icv].rhou[i] -= p * n[i];
+          rhs[icv].rhou[i] -= force[i];
+        }
+        rhs[icv].rhoE -= force[0] * P->bc_u[z][0] + force[1] * P->bc_u[z][1] + force[2] * P->bc_u[z][2];
+      } else {
+        for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i];
+      }
+    } else if (P->bc_type[z] >= 4 && P->bc_type[z] <= 9) {
       /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976); an NSCBC face
        * (Nscbc::addBoundaryFlux, :3662-3693) uses its evolved state */
       og_state b;
@@ -1225,7 +1336,7 @@ static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
    * on the faces that carry a mass flux; walls pass no scalar flux */
   for (int ibf = 0; ibf < m->nbf && nsc > 0; ++ibf) {
     const int z = m->zone_bf[ibf];
-    if (s->p.bc_type[z] < 4) continue;
+    if (s->p.bc_type[z] < 4 || s->p.bc_type[z] > 9) continue;
     const int icv = m->cvobf[ibf];
     const double mf = s->mf_bf[ibf];
     for (int isc = 0; isc < nsc; ++isc) {
@@ -1434,6 +1545,7 @@ void og_init_complete(og_solver* s) {
   nscbc_init(s);
   sponge_init(s);
   og_sgs_props(s);
+  wm_init(s); /* after the properties: the laminar viscosity */
   og_store(s);
 }
 
@@ -1463,10 +1575,13 @@ void og_store(og_solver* s) {
     s->store_time = s->store_time_recent;
     memcpy(s->nscbc_store, s->nscbc, sizeof(double) * nb);
     memcpy(s->sponge_store, s->sponge, sizeof(double) * 4 * (size_t)n);
+    memcpy(s->wm_store, s->wm_tau, sizeof(double) * nb / 5);
   } else {
     memcpy(s->nscbc_store, s->nscbc_recent, sizeof(double) * nb);
     memcpy(s->sponge_store, s->sponge_recent, sizeof(double) * 4 * (size_t)n);
+    memcpy(s->wm_store, s->wm_recent, sizeof(double) * nb / 5);
   }
+  memcpy(s->wm_recent, s->wm_tau, sizeof(double) * nb / 5); /* WmAdiabatic::storeData */
   memcpy(s->nscbc_recent, s->nscbc, sizeof(double) * nb); /* Nscbc::storeData */
   memcpy(s->sponge_recent, s->sponge, sizeof(double) * 4 * (size_t)n); /* Sponge::storeData */
 }
@@ -1482,6 +1597,7 @@ void og_restore(og_solver* s) {
   for (int i = 0; i < n; ++i) s->rhoE[i] = s->store_buf[c++];
   memcpy(s->nscbc, s->nscbc_store, sizeof(double) * 5 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
   memcpy(s->sponge, s->sponge_store, sizeof(double) * 4 * (size_t)n);
+  memcpy(s->wm_tau, s->wm_store, sizeof(double) * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
   og_update_state(s);
   og_sgs_props(s);
   s->p.dt_data[0] *= 0.8;
@@ -1507,6 +1623,7 @@ int og_step(og_solver* s) {
    * faces' states, the faces' RK update and the sponge's relaxation (bc
    * rk3Step), the flow's (IdealGasSolver.hpp:457-468) */
   nscbc_calc_rhs(s, 1);
+  wm_calc(s);
   calc_rhs_into(s, s->rhs0, s->rhs0_sc);
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs0, n, s->time, 1);
   nscbc_rk3(s, 1);

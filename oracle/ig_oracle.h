@@ -107,6 +107,8 @@ typedef struct og_params {
   /* 8 NSCBC_OUTLET_P p_ref (bc_p), sigma, L: per-face boundary states evolved
    * with the step's RK3 weights */
   double bc_sigma[16], bc_L[16];
+  /* 10 WM_ALG_ADIABATIC (wall velocity bc_u): per face a wall stress solved
+   * from the law of the wall at the start of each step */
   /* 9 SPONGE (zone sponge_zone, or -1 without one): relaxed T, u per cell,
    * the Riemann flux at its faces against them at sponge_p, and a volume term
    * over the last sponge_length of the domain in sponge_type (0 L_X, 1 L_Y,
@@ -142,6 +144,9 @@ double og_fast_pow_posx(double x, double p);
  * cv0 and cv1 */
 /* calcNscbcIgOutletRhs (BcCommon.hpp:419-470): rhs[5] = d/dt (rho_bc, p_bc,
  * u_bc[3]); bc and cv = rho, p, u[3]; returns un_bc */
+/* AlgebraicWM::solve_tau (wm/AlgebraicWM.hpp:40-113): the wall stress of the
+ * law of the wall for speed u1 at distance y1 (10 WM_ALG_ADIABATIC faces) */
+double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess);
 double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
                            double gamma, double L_ref, double sigma, double p_ref);
 void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,

@@ -114,6 +114,8 @@ def lib(path=None):
         L.og_nscbc_outlet_rhs.restype = C.c_double
         L.og_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                           C.c_double]
+        L.og_wm_solve_tau.restype = C.c_double
+        L.og_wm_solve_tau.argtypes = [C.c_double] * 5
         _libs[path] = L
     return _libs[path]
 
@@ -136,6 +138,8 @@ def ref():
         R.ref_nscbc_outlet_rhs.restype = C.c_double
         R.ref_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                            C.c_double]
+        R.ref_wm_solve_tau.restype = C.c_double
+        R.ref_wm_solve_tau.argtypes = [C.c_double] * 5
         _ref = R
     return _ref
 

@@ -9,11 +9,13 @@
 //   calcSvd33                        src/core/common/SvdNew.hpp:24
 //   MiscUtils::fast_pow_posx         src/core/common/MiscUtils.hpp:273-276
 //   RkWeights::erk_wgt3              src/charles/RkWeights.hpp:6-8
+//   AlgebraicWM::solve_tau           src/charles/wm/AlgebraicWM.hpp:40-113
 #include "FlowSolver.hpp"
 #include "BcCommon.hpp"
 #include "IdealGasSolverFlux.hpp"
 #include "RkWeights.hpp"
 #include "SvdNew.hpp"
+#include "AlgebraicWM.hpp"
 
 #include <cstring>
 
@@ -113,6 +115,11 @@ int ref_svd33(const double* A, double* U, double* V, double* sigma, double* max_
 void ref_erk_wgt3(double* w) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) w[3 * i + j] = RkWeights::erk_wgt3[i][j];
+}
+
+// wall stress of the algebraic wall model
+double ref_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess) {
+  return AlgebraicWM::solve_tau(u1, y1, rho, mu, u_tau_guess);
 }
 
 }  // extern "C"

@@ -27,6 +27,11 @@ terms, pinned by closed forms.
   long (the wall row adding the Riemann flux's mass) and nothing below; the
   stage-1 relaxation moves T_sponge, u_sponge by eps = 1 / (t_relax / dt + 1)
   of the way to a changed state, in every cell.
+* WM_ALG_ADIABATIC (WmAdiabatic, IdealGasSolverBcs.cpp:996-1441): its stress
+  starts laminar (initialHook: mu |u - u_bc| / y1), so a uniform stream along
+  the lattice channel's wall loses mu 2h U of momentum per wall cell, the
+  no-slip wall's closed form; the law-of-the-wall solver is pinned in
+  test_oracle_pins.py.
 * Passive scalars through boundaries (_addScalarBoundaryFlux, BcCommon.hpp:276-291):
   at rest and scalar-free, a CBC inflow zone with value Z_in feeds the wall cells
   -mf Z_in (mf the Riemann flux's mass, negative inflow); a uniform stream and a
@@ -420,3 +425,26 @@ def test_uniform_scalar_through_the_channel_stays_uniform(jitter):
     for _ in range(5):
         assert o.step() == 0
     np.testing.assert_allclose(o.get_scalar(0), 3.0, rtol=1e-12)
+
+
+def test_wall_model_parser():
+    p = parse_params(deck({"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 1 0 0.5 RELAX 0.5 TOL 1e-5"}))
+    assert (p.bc[0].type, list(p.bc[0].u)) == (10, [0.0, 0.0, 0.0])
+    assert (p.bc[1].type, list(p.bc[1].u)) == (10, [1.0, 0.0, 0.5])
+    for bad in ("y0 WM_ALG_ADIABATIC TRANSLATE 1 0", "y0 WM_ALG_ADIABATIC RELAX", "y0 WM_ALG_ADIABATIC SOMETHING"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+def test_wall_model_starts_with_the_laminar_stress():
+    m = Mesh.channel(10, jitter=0.0)
+    U = 0.7
+    o = oracle(m, deck({"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.25 0 0"}), uniform(m, (U, 0, 0)))
+    mu = o.get_field("mu_lam")
+    r = o.calc_rhs()
+    lower, upper = wall_cells(m, 0), wall_cells(m, 1)
+    np.testing.assert_allclose(r[lower, 1], -mu[lower] * 2 * H * U, rtol=1e-10)
+    np.testing.assert_allclose(r[upper, 1], -mu[upper] * 2 * H * (U - 0.25), rtol=1e-10)
+    np.testing.assert_allclose(r[upper, 4], -0.25 * mu[upper] * 2 * H * (U - 0.25), rtol=1e-10)
+    # the law of the wall then takes over: one step moves the wall cells' shear
+    assert o.step() == 0

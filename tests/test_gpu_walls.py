@@ -4,7 +4,8 @@ test_boundary_conditions.py, its Riemann flux against the reference's
 calcRiemannFlux in test_oracle_pins.py): SLIP / SYMMETRY, WALL_ADIABATIC (with
 a TRANSLATE wall velocity), WALL_ISOTHERMAL, the characteristic conditions
 CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP, the NSCBC pressure outlet (per-face
-states evolved with the step, restored with it after a NaN) and the SPONGE
+states evolved with the step, restored with it after a NaN), the algebraic
+wall model WM_ALG_ADIABATIC (per-face stress from the law of the wall) and the SPONGE
 (per-cell relaxed T, u, a volume term over the last LENGTH of the domain, the
 Riemann flux at its faces; L_Y and L_RX on channels, L_X on the orifice) on
 jittered channel meshes and on the reference's own Stitch mesh orifice.mles
@@ -47,6 +48,8 @@ WALLS = {
     # out through a sponge over the top quarter, relaxing to a lower pressure
     "cbc_upt+sponge": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0",
                        "y1": f"SPONGE TYPE L_Y LENGTH 1.5 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {P0 * 0.995!r}"},
+    # the algebraic wall model on both walls, one of them moving
+    "wm+wm_translate": {"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.3 0 0.1"},
     # a radial sponge as the first zone (its volume term before the other zone's faces)
     "sponge_rx+adiabatic": {"y0": "SPONGE TYPE L_RX LENGTH 3.0 STRENGTH 8.0 RELAX_TIME 0.2",
                             "y1": "WALL_ADIABATIC"},
@@ -206,7 +209,7 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
     check_history(g, o, 20, "orifice nscbc")
 
 
-@pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge"])
+@pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate"])
 def test_boundary_states_roll_back_with_the_flow(walls):
     # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
     # faces' or the sponge's states with it (storeData / restoreData of the
@@ -238,7 +241,8 @@ def test_boundary_states_roll_back_with_the_flow(walls):
     check_history(g, o, 5, f"{walls} after rollback", sponge="sponge" in walls)
 
 
-@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge"])
+@pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge",
+                                   "wm+wm_translate"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]
@@ -302,11 +306,12 @@ def test_channel_dsm_matches_oracle(walls):
     g.close()
 
 
-def test_sponge_refuses_a_source_hook():
+@pytest.mark.parametrize("walls,what", [("cbc_upt+sponge", "SPONGE"), ("wm+wm_translate", "WM_ALG_ADIABATIC")])
+def test_stepped_boundary_states_refuse_a_source_hook(walls, what):
     from nextgen_amd.ideal_gas import CharlesError
     m = Mesh.channel(10, jitter=0.0)
-    g = IdealGasSolver(m, parse_params(deck(WALLS["cbc_upt+sponge"])))
-    with pytest.raises(CharlesError, match="SPONGE"):
+    g = IdealGasSolver(m, parse_params(deck(WALLS[walls])))
+    with pytest.raises(CharlesError, match=what):
         g.set_source_hook(lambda rhs, time, stage: None)
     g.close()
 

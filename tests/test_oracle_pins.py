@@ -313,3 +313,45 @@ def test_nscbc_outlet_rhs_closed_forms():
     bc[1] = p_ref
     L.og_nscbc_outlet_rhs(dp(r), dp(n), 0.1, dp(bc), dp(bc.copy()), gamma, Lr, sigma, p_ref)
     assert np.all(r == 0.0)
+
+
+def _wm_cases(rng, n):
+    # speeds, distances and viscosities from the viscous sublayer to y+ ~ 1e4,
+    # with the laminar guess (-1), the previous stress's u_tau and poor guesses
+    for k in range(n):
+        rho = rng.uniform(0.5, 2.0)
+        mu = 10.0 ** rng.uniform(-6, -3)
+        y1 = 10.0 ** rng.uniform(-4, -1)
+        u1 = 0.0 if k % 17 == 0 else 10.0 ** rng.uniform(-3, 1)
+        guess = (-1.0, 0.0, 10.0 ** rng.uniform(-4, 0))[k % 3]
+        yield u1, y1, rho, mu, guess
+
+
+@needs_ref
+def test_wall_model_bit_identical_to_reference():
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(31)
+    for case in _wm_cases(rng, 3000):
+        assert L.og_wm_solve_tau(*case) == R.ref_wm_solve_tau(*case), case
+
+
+def test_wall_model_closed_forms():
+    # the solved u_tau reproduces the law of the wall: u1 = u_tau u+(y1 u_tau / nu)
+    # with u+ = y+ + a1 y+^2 below y* = 23.3244 and ln(y+)/0.41 + 5.2 above (C1 at
+    # y*: a1 = (1/(kappa y*) - 1) / (2 y*)); in the sublayer limit tau -> mu u1 / y1
+    L = orc.lib()
+    kappa, B, ystar = 0.41, 5.2, 23.3244
+    a1 = 0.5 * (1.0 / (kappa * ystar) - 1.0) / ystar
+    assert abs((ystar + a1 * ystar ** 2) - (math.log(ystar) / kappa + B)) < 2e-4  # continuous at y*
+    assert abs((1 + 2 * a1 * ystar) - 1 / (kappa * ystar)) < 1e-14                # and its slope
+    rho, mu, y1 = 1.2, 1.8e-5, 1e-3
+    nu = mu / rho
+    for yplus_true in (0.5, 5.0, 20.0, 30.0, 300.0, 3000.0):
+        u_tau = yplus_true * nu / y1
+        up = yplus_true + a1 * yplus_true ** 2 if yplus_true <= ystar else math.log(yplus_true) / kappa + B
+        u1 = u_tau * up
+        tau = L.og_wm_solve_tau(u1, y1, rho, mu, -1.0)
+        np.testing.assert_allclose(math.sqrt(tau / rho), u_tau, rtol=2e-6)
+    u1 = 1e-6  # y+ ~ 0.008: the quadratic term is a 3e-4 correction
+    tau = L.og_wm_solve_tau(u1, y1, rho, mu, -1.0)
+    np.testing.assert_allclose(tau, mu * u1 / y1, rtol=1e-3)
