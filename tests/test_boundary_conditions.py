@@ -448,3 +448,25 @@ def test_wall_model_starts_with_the_laminar_stress():
     np.testing.assert_allclose(r[upper, 4], -0.25 * mu[upper] * 2 * H * (U - 0.25), rtol=1e-10)
     # the law of the wall then takes over: one step moves the wall cells' shear
     assert o.step() == 0
+
+
+def test_reference_deck_boundary_lines_parse():
+    # the boundary-line forms of the reference's own decks (charles__euler.in,
+    # charles.in, charles.soft_inlet.in, charles_scalar.in, test_charles_vg.in),
+    # with their parameter expressions written out as numbers
+    text = ("REGISTER_SCALAR foo\nRES_TIME\n"
+            "x0 CBC_UPT 0.1 0 0 0.7142857142857143 1.0 foo 1.0 RES_TIME 0.0\n"
+            "x1 NSCBC_OUTLET_PRESSURE P_REF 0.7142857142857143 L 1.0 SIGMA 0.2\n"
+            "y0 = CBC_UPT 0.25 0.43301270189 0.0 1.0 1.0 foo 0.5 RES_TIME 0.0\n"
+            "y1 = NSCBC_OUTLET_PRESSURE P_REF 1.0e5 SIGMA 0.2 L 1.0\n"
+            "z0 WM_ALG_ADIABATIC\n"
+            "z1 NSCBC_OUTLET_P 0.8e5 0.15 0.4\n")
+    p = parse_params(text)
+    bc = {p.bc[i].zone.decode(): p.bc[i] for i in range(p.nbc)}
+    names = [p.sc_names[i].value.decode() for i in range(p.nsc)]
+    assert names == ["RES_TIME", "foo"] and p.res_time == 0
+    assert (bc["x0"].type, list(bc["x0"].u), bc["x0"].p, bc["x0"].T) == (4, [0.1, 0.0, 0.0], 0.7142857142857143, 1.0)
+    assert list(bc["x0"].sc[:2]) == [0.0, 1.0] and list(bc["y0"].sc[:2]) == [0.0, 0.5]
+    assert (bc["x1"].type, bc["x1"].p, bc["x1"].sigma, bc["x1"].L) == (8, 0.7142857142857143, 0.2, 1.0)
+    assert (bc["y1"].p, bc["z1"].p, bc["z1"].sigma, bc["z1"].L) == (1.0e5, 0.8e5, 0.15, 0.4)
+    assert bc["z0"].type == 10
