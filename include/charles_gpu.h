@@ -81,6 +81,12 @@ typedef struct cg_solver cg_solver;
  *   CBC_RUNP <rho> <un> <p>           (IdealGasSolverFlux.hpp:218-306):
  *   CBC_RUP <rho> <u> <v> <w> <p>     Cbc::addBoundaryFlux (:2960-2976),
  *                                    CbcUpt/Mpt/Runp/Rup::initData (:3182-3300)
+ *   CBC_TOTAL_PT <p0> <T0> <t_relax> [STATIONARY_FRAME]
+ *                                    CbcTotalPt (:3372-3558): per face a normal
+ *                                    velocity un_bc (the cell's at
+ *                                    initComplete, relaxed toward it once per
+ *                                    step) and the isentropic state of total
+ *                                    p0, T0 at that speed; USE_AVG is refused
  *   NSCBC_OUTLET_P <p_ref> <sigma> <L>
  *   NSCBC_OUTLET_PRESSURE P_REF <p> [SIGMA <s>] [L <l>] [CONSTANT_PRESSURE]
  *                                    NscbcOutletP (:3846-3932, 4009-4068): per
@@ -122,16 +128,18 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_NSCBC_OUTLET_P 8
 #define CG_BC_SPONGE 9
 #define CG_BC_WM_ALG_ADIABATIC 10
+#define CG_BC_CBC_TOTAL_PT 11
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
-  double T;       /* wall temperature (WALL_ISOTHERMAL), T of CBC_UPT / CBC_MPT, SPONGE RELAX_TIME */
+  double T;       /* wall temperature (WALL_ISOTHERMAL), T of CBC_UPT / CBC_MPT, SPONGE RELAX_TIME,
+                     CBC_TOTAL_PT T0 */
   double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
-  double p;       /* CBC pressure, NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */
+  double p;       /* CBC pressure (CBC_TOTAL_PT: p0), NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */
   double rho;     /* CBC_RUNP / CBC_RUP density */
   double q;       /* CBC_MPT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
   double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
-  double L;       /* NSCBC outlet length scale, SPONGE LENGTH */
+  double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax */
   int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ */
   /* passive scalars entering through a CBC zone (parseScalarBc, BcCommon.hpp:213-274):
    * the value of each registered scalar (by its index in cg_params.sc_names), given on

@@ -148,6 +148,7 @@ struct HostInst {
   size_t snap_sp = 0;                // and, at the end, the sponge's relaxed T, u (4 per cell)
   bool nscbc = false;                // some zone is an NSCBC outlet
   bool wm = false;                   // some zone is a WM_ALG_ADIABATIC wall
+  bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
   double store_time = 0.0, store_time_recent = 0.0;
@@ -920,11 +921,13 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_L[z] = p->bc[k].L;
       // backflow through an NSCBC outlet or a sponge brings in 0 (their
       // initData, IdealGasSolverBcs.cpp:3928-3930,4124-4126)
-      const bool cbc = p->bc[k].type >= CG_BC_CBC_UPT && p->bc[k].type <= CG_BC_CBC_RUP;
+      const bool cbc = (p->bc[k].type >= CG_BC_CBC_UPT && p->bc[k].type <= CG_BC_CBC_RUP) ||
+                       p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
       hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC;
+      hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       if (p->bc[k].type == CG_BC_SPONGE) {
         // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
         // u_sponge are solver-wide cell data, so one sponge zone per solve
@@ -994,6 +997,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     hi->snap_bc = 5 * nbf;
     hi->snap_n += hi->snap_bc;
   }
+  if (hi->tpt) rc |= dev_alloc(s, h, &w.tpt_un, (size_t)lay->g.nbf);  // not snapshotted (no storeData)
   if (hi->wm) {
     // WmAdiabatic::storeData: the faces' tau_wall (in the snapshots too)
     rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf);
@@ -1082,6 +1086,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
   }
   if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)lay->g.nbf, stream);  // set by initComplete
+  if (hi->tpt) rc |= cgk::launch_fill(w.tpt_un, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->sponge) {
     // the relaxed state is set from the cells by initComplete
     rc |= cgk::launch_fill(w.sponge, 0.0, 4 * (int64_t)ncv, stream);
@@ -1265,7 +1270,7 @@ template <class R>
 int nscbc_init(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   const bool sponge = I->sponge && !I->sponge_kept;
   I->sponge_kept = false;
-  if (!I->nscbc && !sponge && !I->wm) return 0;
+  if (!I->nscbc && !sponge && !I->wm && !I->tpt) return 0;
   if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, sponge, stream))
     return launch_fail(s, "boundary state init", le);
   return 0;
@@ -1791,6 +1796,17 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (tok.size() != 7 || !num(2, &b.u[0]) || !num(3, &b.u[1]) || !num(4, &b.u[2]) || !num(5, &b.p) ||
             !num(6, &b.T) || !(b.p > 0.0 && b.T > 0.0))
           return bad("expected CBC_UPT <u> <v> <w> <p> <T>");
+      } else if (type == "CBC_TOTAL_PT") {
+        // CbcTotalPt::initData (IdealGasSolverBcs.cpp:3372-3401)
+        b.type = CG_BC_CBC_TOTAL_PT;
+        if (tok.size() < 5 || !num(2, &b.p) || !num(3, &b.T) || !num(4, &b.L) || !(b.p > 0.0 && b.T > 0.0) ||
+            !(b.L >= 0.0))
+          return bad("expected CBC_TOTAL_PT <p0> <T0> <t_relax>");
+        for (size_t i = 5; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          if (opt == "USE_AVG") return bad("CBC_TOTAL_PT USE_AVG (a zone-averaged inflow speed) is not available");
+          if (opt != "STATIONARY_FRAME") return bad("unknown CBC_TOTAL_PT option " + opt);
+        }
       } else if (type == "CBC_MPT") {
         b.type = CG_BC_CBC_MPT;
         if (tok.size() != 5 || !num(2, &b.q) || !num(3, &b.p) || !num(4, &b.T) || !(b.p > 0.0 && b.T > 0.0))
@@ -1869,7 +1885,8 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, NSCBC_OUTLET_P, "
+                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
+                   "NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE and SPONGE are)");
       }
       int slot = -1;
@@ -1880,7 +1897,8 @@ int cg_params_parse(cg_params* p, const char* text) {
         slot = p->nbc++;
       }
       p->bc[slot] = b;
-      if (b.type >= CG_BC_CBC_UPT && b.type <= CG_BC_CBC_RUP) zone_sc.emplace_back(b.zone, std::move(scv));
+      if ((b.type >= CG_BC_CBC_UPT && b.type <= CG_BC_CBC_RUP) || b.type == CG_BC_CBC_TOTAL_PT)
+        zone_sc.emplace_back(b.zone, std::move(scv));
     } else if (key == "DISS_SPONGE" || key == "ROTATING_REFERENCE_FRAME" || key == "ADD_BODY_FORCE") {
       // body forces (addBodyForceSourceTerms, IdealGasSolver.cpp:2159-2202) and the
       // rotating frame are not on this path; a per-stage source can be added with
@@ -2340,6 +2358,7 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
     if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC outlet zones");
     if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
     if (fn && I->wm) return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC zones");
+    if (fn && I->tpt) return s->set(CG_EINVAL, "a source hook cannot be combined with CBC_TOTAL_PT zones");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;
     if (!fn) std::vector<double>().swap(I->hook_buf);

@@ -147,6 +147,9 @@ struct WorkT {
   // WM_ALG_ADIABATIC faces (null without them): per boundary face its wall
   // stress tau_wall (FP64, in the snapshots)
   double* wm_tau;
+  // CBC_TOTAL_PT faces (null without them): per boundary face its inflow
+  // normal velocity un_bc (FP64; not in the snapshots, as in the reference)
+  double* tpt_un;
   // passive scalars on a mesh with boundary faces (null otherwise): each
   // boundary face's mass flux of the stage (FP64, the Riemann flux's F_rho on
   // CBC, NSCBC and sponge faces), written by the flux pass for the scalar pass
@@ -158,10 +161,12 @@ constexpr int kMaxScalars = 8;  // passive scalars per solve
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
-  BC_WM_ALG_ADIABATIC = 10
+  BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
-__host__ __device__ inline bool bc_riemann(int type) { return type >= BC_CBC_UPT && type <= BC_SPONGE; }
+__host__ __device__ inline bool bc_riemann(int type) {
+  return (type >= BC_CBC_UPT && type <= BC_SPONGE) || type == BC_CBC_TOTAL_PT;
+}
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;

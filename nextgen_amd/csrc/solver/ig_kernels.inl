@@ -973,12 +973,39 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
         vb = 1.0 / q[0];
         pb = q[1];
         hb = gogm1 * q[1] * (1.0 / q[0]);
+      } else if (type == BC_CBC_TOTAL_PT) {
+        // CbcTotalPt::calcBfState (IdealGasSolverBcs.cpp:3403-3425): the
+        // isentropic state of total p0 (bc_p), T0 (bc_T) at normal speed un_bc
+        #pragma clang fp contract(off)
+        const double area = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+        const double un = G(w.tpt_un)[b];
+        for (int i = 0; i < 3; ++i) ub[i] = un * (nd[i] / area) + 0.0;
+        const double T0 = P.bc_T[z], p0 = P.bc_p[z];
+        const double total_rh = p0 / P.R_gas / T0;
+        const double invgm1 = 1.0 / (P.gamma - 1.0);
+        const double cp = P.R_gas * P.gamma * invgm1;
+        const double T_bc = T0 - 0.5 * un * un / cp;
+        const double rho_bc = total_rh * pow(T_bc / T0, invgm1);
+        vb = 1.0 / rho_bc;
+        pb = p0 * pow(T_bc / T0, P.gamma * invgm1);
+        hb = cp * T_bc;
       } else {
         cbc_state(P, z, type, nd, ub, vb, pb, hb);
       }
       double F[5];
       riemann_flux(F, nd, ua, va, pa, ha, ub, vb, pb, hb, P.gamma);
       if (w.mf_bf) G(w.mf_bf)[b] = F[0];  // mf[ibf] = flux.rho, for the scalar pass
+      if (type == BC_CBC_TOTAL_PT && update && stage == 1) {
+        // CbcTotalPt::rk3Step (IdealGasSolverBcs.cpp:3521-3558): after the
+        // stage-1 RHS, un_bc relaxes toward the cell's normal velocity
+        #pragma clang fp contract(off)
+        const double area = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+        const double unit_n[3] = {nd[0] / area, nd[1] / area, nd[2] / area};
+        const double eps = 1.0 / (P.bc_L[z] / G(w.info)->dt + 1.0);
+        const double un0 = ua[0] * unit_n[0] + ua[1] * unit_n[1] + ua[2] * unit_n[2];
+        const auto un = G(w.tpt_un) + b;
+        *un = (1.0 - eps) * *un + eps * un0;
+      }
       if (q && update) {
         #pragma clang fp contract(off)
         const double dt = G(w.info)->dt;
@@ -2241,10 +2268,23 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
     sp[0] = T;
     for (int i = 0; i < 3; ++i) sp[1 + i] = u[i];
   }
-  if (!X.w.nscbc && !X.w.wm_tau) return;
+  if (!X.w.nscbc && !X.w.wm_tau && !X.w.tpt_un) return;
   const int b0 = G(g.bf_ptr)[c], b1 = G(g.bf_ptr)[c + 1];
   for (int b = b0; b < b1; ++b) {
     const int z = G(g.bf_zone)[b];
+    if (X.P.bc_type[z] == BC_CBC_TOTAL_PT) {
+      // CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3449): the cell's
+      // normal velocity
+      #pragma clang fp contract(off)
+      double rho, u[3], rhoE, spv, p, h, T;
+      const Light<R> self = load_light(X.st[cur].light, c);
+      cell_state64(g, X.st[cur], X.P, c, self, rho, u, rhoE, spv, p, h, T);
+      const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
+      const double area = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      const double unit_n[3] = {n0 / area, n1 / area, n2 / area};
+      G(X.w.tpt_un)[b] = unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
+      continue;
+    }
     if (X.P.bc_type[z] == BC_WM_ALG_ADIABATIC) {
       // WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1102): the
       // laminar stress mu_lam |u - u_bc| / y1 (0 where area_over_delta is 0)

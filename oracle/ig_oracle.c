@@ -194,6 +194,8 @@ struct og_solver {
   /* WmAdiabatic (IdealGasSolverBcs.hpp): tau_wall per boundary face and the
    * two snapshots */
   double *wm_tau, *wm_store, *wm_recent;
+  /* CbcTotalPt: un_bc per boundary face (not part of storeData) */
+  double* tpt_un;
 };
 
 og_solver* og_create(const og_mesh* m, const og_params* p) {
@@ -243,6 +245,7 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   }
   s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_tau = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->tpt_un = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_store = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_recent = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->sponge = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
@@ -267,7 +270,7 @@ void og_destroy(og_solver* s) {
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
   free(s->sponge); free(s->sponge_store); free(s->sponge_recent); free(s->mf_bf);
-  free(s->wm_tau); free(s->wm_store); free(s->wm_recent);
+  free(s->wm_tau); free(s->wm_store); free(s->wm_recent); free(s->tpt_un);
   free(s);
 }
 
@@ -1187,6 +1190,40 @@ static void wm_calc(og_solver* s) {
   }
 }
 
+static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11; }
+
+/* CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3449): un_bc = the
+ * cell's normal velocity */
+static void tpt_init(og_solver* s) {
+  const og_mesh* m = &s->m;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    if (s->p.bc_type[m->zone_bf[ibf]] != 11) continue;
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double* u = &s->u[3 * (size_t)m->cvobf[ibf]];
+    const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double unit_n[3] = {n[0] / area, n[1] / area, n[2] / area};
+    s->tpt_un[ibf] = unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
+  }
+}
+
+/* CbcTotalPt::rk3Step (IdealGasSolverBcs.cpp:3521-3558): at stage 1, after the
+ * RHS, un_bc <- (1 - eps) un_bc + eps (u . n^), eps = 1 / (t_relax / dt + 1) */
+static void tpt_rk3(og_solver* s, int rk_stage) {
+  const og_mesh* m = &s->m;
+  if (rk_stage != 1) return;
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] != 11) continue;
+    const double eps = 1.0 / (s->p.bc_L[z] / s->dt + 1.0);
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double* u = &s->u[3 * (size_t)m->cvobf[ibf]];
+    const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double unit_n[3] = {n[0] / area, n[1] / area, n[2] / area};
+    const double un0 = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+    s->tpt_un[ibf] = (1.0 - eps) * s->tpt_un[ibf] + eps * un0;
+  }
+}
+
 /* Sponge::initialHook (IdealGasSolverBcs.cpp:4311-4346): T_sponge, u_sponge
  * from the cells */
 static void sponge_init(og_solver* s) {
@@ -1285,11 +1322,24 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
       } else {
         for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i];
       }
-    } else if (P->bc_type[z] >= 4 && P->bc_type[z] <= 9) {
+    } else if (bc_riemann(P->bc_type[z])) {
       /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976); an NSCBC face
        * (Nscbc::addBoundaryFlux, :3662-3693) uses its evolved state */
       og_state b;
-      if (P->bc_type[z] == 9) {
+      if (P->bc_type[z] == 11) {
+        /* CbcTotalPt::calcBfState (IdealGasSolverBcs.cpp:3403-3425) */
+        const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const double un = s->tpt_un[ibf];
+        for (int i = 0; i < 3; ++i) b.u[i] = un * (n[i] / area) + 0.0;
+        const double total_rh = P->bc_p[z] / s->R_gas / P->bc_T[z];
+        const double invgm1 = 1.0 / (P->gamma - 1.0);
+        const double cp = s->R_gas * P->gamma * invgm1;
+        const double T_bc = P->bc_T[z] - 0.5 * un * un / cp;
+        const double rho_bc = total_rh * pow(T_bc / P->bc_T[z], invgm1);
+        b.sp_vol = 1.0 / rho_bc;
+        b.p = P->bc_p[z] * pow(T_bc / P->bc_T[z], P->gamma * invgm1);
+        b.h = cp * T_bc;
+      } else if (P->bc_type[z] == 9) {
         /* Sponge::addBoundaryFlux (IdealGasSolverBcs.cpp:4166-4201) */
         const double Rgogm1 = s->R_gas * P->gamma * (1.0 / (P->gamma - 1.0));
         const double* sp = &s->sponge[4 * (size_t)icv];
@@ -1336,7 +1386,7 @@ static void calc_rhs_full(og_solver* s, double* rhs_out, double* rhs_sc) {
    * on the faces that carry a mass flux; walls pass no scalar flux */
   for (int ibf = 0; ibf < m->nbf && nsc > 0; ++ibf) {
     const int z = m->zone_bf[ibf];
-    if (s->p.bc_type[z] < 4 || s->p.bc_type[z] > 9) continue;
+    if (!bc_riemann(s->p.bc_type[z])) continue;
     const int icv = m->cvobf[ibf];
     const double mf = s->mf_bf[ibf];
     for (int isc = 0; isc < nsc; ++isc) {
@@ -1543,6 +1593,7 @@ void og_init_complete(og_solver* s) {
   s->dsm_kept = 0;
   og_update_state(s);
   nscbc_init(s);
+  tpt_init(s);
   sponge_init(s);
   og_sgs_props(s);
   wm_init(s); /* after the properties: the laminar viscosity */
@@ -1628,6 +1679,7 @@ int og_step(og_solver* s) {
   if (s->hook) s->hook(s->hook_user, (double*)s->rhs0, n, s->time, 1);
   nscbc_rk3(s, 1);
   sponge_rk3(s, 1);
+  tpt_rk3(s, 1);
   rk3_step(s, og_erk_wgt3[0]);
   s->time += s->dt;
   og_update_state(s);

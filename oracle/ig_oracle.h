@@ -108,7 +108,9 @@ typedef struct og_params {
    * with the step's RK3 weights */
   double bc_sigma[16], bc_L[16];
   /* 10 WM_ALG_ADIABATIC (wall velocity bc_u): per face a wall stress solved
-   * from the law of the wall at the start of each step */
+   * from the law of the wall at the start of each step; 11 CBC_TOTAL_PT (p0 in
+   * bc_p, T0 in bc_T, t_relax in bc_L): the isentropic state at a per-face
+   * normal speed relaxed toward the cell's once per step */
   /* 9 SPONGE (zone sponge_zone, or -1 without one): relaxed T, u per cell,
    * the Riemann flux at its faces against them at sponge_p, and a volume term
    * over the last sponge_length of the domain in sponge_type (0 L_X, 1 L_Y,

@@ -188,7 +188,7 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
         p.bc_p[z], p.bc_rho[z], p.bc_q[z] = b.p, b.rho, b.q
         p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
         p.bc_sigma[z], p.bc_L[z] = b.sigma, b.L
-        if 4 <= b.type <= 7:  # CBC zones bring their scalars' values in; the others 0
+        if 4 <= b.type <= 7 or b.type == 11:  # CBC zones bring their scalars' values in; the others 0
             for i in range(min(cgp.nsc, 8)):
                 p.bc_sc[z][i] = b.sc[i]
     for name, _ in og_params._fields_:

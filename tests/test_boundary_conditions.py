@@ -470,3 +470,36 @@ def test_reference_deck_boundary_lines_parse():
     assert (bc["x1"].type, bc["x1"].p, bc["x1"].sigma, bc["x1"].L) == (8, 0.7142857142857143, 0.2, 1.0)
     assert (bc["y1"].p, bc["z1"].p, bc["z1"].sigma, bc["z1"].L) == (1.0e5, 0.8e5, 0.15, 0.4)
     assert bc["z0"].type == 10
+
+
+def test_total_pt_parser():
+    p = parse_params(deck({"y0": "CBC_TOTAL_PT 1.1e5 300 0.01 STATIONARY_FRAME", "y1": "SLIP"}))
+    b = p.bc[0]
+    assert (b.type, b.p, b.T, b.L) == (11, 1.1e5, 300.0, 0.01)
+    for bad in ("y0 CBC_TOTAL_PT 1e5 300", "y0 CBC_TOTAL_PT 1e5 300 0.1 USE_AVG", "y0 CBC_TOTAL_PT -1 300 0.1"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter):
+    # a uniform stream V in through a CBC_TOTAL_PT inlet whose totals are the
+    # stream's (T0 = T + V^2 / (2 cp), p0 = p (T0 / T)^(gamma / (gamma - 1))):
+    # un_bc starts as the cells' normal velocity, the isentropic state at that
+    # speed is the stream, so the RHS vanishes and five steps keep it
+    m = Mesh.channel(10, jitter=jitter, seed=14)
+    R, T, gamma = 71.42857142857143, 1.0, 1.4
+    p = R * T
+    V = 3.0
+    cp = R * gamma / (gamma - 1.0)
+    T0 = T + 0.5 * V * V / cp
+    p0 = p * (T0 / T) ** (gamma / (gamma - 1.0))
+    text = deck({"y0": f"CBC_TOTAL_PT {p0!r} {T0!r} 0.05", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    r = o.calc_rhs()
+    assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
