@@ -96,6 +96,12 @@ typedef struct cg_solver cg_solver;
  *                                    and the Riemann flux against it (Nscbc,
  *                                    :3578-3693); SIGMA 0.2 and L 1 by default,
  *                                    CONSTANT_PRESSURE sets sigma -1
+ *   NSCBC_MT <mdot> <T>              NscbcMt (:3718-3840): NSCBC inlet faces whose
+ *                                    velocity and pressure follow each stage from
+ *                                    their density, the zone's mass flux
+ *                                    -mdot / area and T; the density evolves
+ *                                    by calcNscbcIgSubsonicInletRhs
+ *                                    (BcCommon.hpp:311-337); T_TOTAL is refused
  *   SPONGE TYPE L_X|L_Y|L_Z|L_RX|L_RZ LENGTH <l> STRENGTH <s> RELAX_TIME <t>
  *          [SPONGE_PRESSURE <p>] [EPS_P <e>]
  *                                    Sponge (:4080-4346, SpongeCommon.hpp:11-63):
@@ -129,6 +135,7 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_SPONGE 9
 #define CG_BC_WM_ALG_ADIABATIC 10
 #define CG_BC_CBC_TOTAL_PT 11
+#define CG_BC_NSCBC_MT 12
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
@@ -137,7 +144,7 @@ typedef struct cg_bc {
   double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
   double p;       /* CBC pressure (CBC_TOTAL_PT: p0), NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */
   double rho;     /* CBC_RUNP / CBC_RUP density */
-  double q;       /* CBC_MPT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
+  double q;       /* CBC_MPT / NSCBC_MT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
   double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
   double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax */
   int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ */

@@ -922,10 +922,10 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       // backflow through an NSCBC outlet or a sponge brings in 0 (their
       // initData, IdealGasSolverBcs.cpp:3928-3930,4124-4126)
       const bool cbc = (p->bc[k].type >= CG_BC_CBC_UPT && p->bc[k].type <= CG_BC_CBC_RUP) ||
-                       p->bc[k].type == CG_BC_CBC_TOTAL_PT;
+                       p->bc[k].type == CG_BC_CBC_TOTAL_PT || p->bc[k].type == CG_BC_NSCBC_MT;
       for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
-      hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P;
+      hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P || p->bc[k].type == CG_BC_NSCBC_MT;
       hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       if (p->bc[k].type == CG_BC_SPONGE) {
@@ -1735,10 +1735,10 @@ int cg_params_parse(cg_params* p, const char* text) {
       std::memset(&b, 0, sizeof(b));
       if (tok[0].size() >= sizeof(b.zone)) return bad("zone name too long");
       std::snprintf(b.zone, sizeof(b.zone), "%s", tok[0].c_str());
-      // CBC zones: "<scalar> <value>" pairs anywhere after the type
-      // (BcTemplate::parseScalarBc, BcCommon.hpp:213-274)
+      // CBC zones and NSCBC inlets: "<scalar> <value>" pairs anywhere after the
+      // type (BcTemplate::parseScalarBc, BcCommon.hpp:213-274)
       std::vector<std::pair<std::string, double>> scv;
-      if (type.compare(0, 4, "CBC_") == 0) {
+      if (type.compare(0, 4, "CBC_") == 0 || type == "NSCBC_MT") {
         for (size_t i = 2; i < tok.size();) {
           if (std::find(sc_known.begin(), sc_known.end(), tok[i]) == sc_known.end()) {
             ++i;
@@ -1796,6 +1796,13 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (tok.size() != 7 || !num(2, &b.u[0]) || !num(3, &b.u[1]) || !num(4, &b.u[2]) || !num(5, &b.p) ||
             !num(6, &b.T) || !(b.p > 0.0 && b.T > 0.0))
           return bad("expected CBC_UPT <u> <v> <w> <p> <T>");
+      } else if (type == "NSCBC_MT") {
+        // NscbcMt::initData (IdealGasSolverBcs.cpp:3718-3759): mass flow and static T
+        b.type = CG_BC_NSCBC_MT;
+        if (tok.size() >= 3 && upper(tok[2]) == "T_TOTAL")
+          return bad("NSCBC_MT T_TOTAL (a zone-averaged Mach number per stage) is not available");
+        if (tok.size() != 4 || !num(2, &b.q) || !num(3, &b.T) || !(b.T > 0.0))
+          return bad("expected NSCBC_MT <mdot> <T>");
       } else if (type == "CBC_TOTAL_PT") {
         // CbcTotalPt::initData (IdealGasSolverBcs.cpp:3372-3401)
         b.type = CG_BC_CBC_TOTAL_PT;
@@ -1886,7 +1893,7 @@ int cg_params_parse(cg_params* p, const char* text) {
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
                    "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
-                   "NSCBC_OUTLET_P, "
+                   "NSCBC_MT, NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE and SPONGE are)");
       }
       int slot = -1;
@@ -1897,7 +1904,8 @@ int cg_params_parse(cg_params* p, const char* text) {
         slot = p->nbc++;
       }
       p->bc[slot] = b;
-      if ((b.type >= CG_BC_CBC_UPT && b.type <= CG_BC_CBC_RUP) || b.type == CG_BC_CBC_TOTAL_PT)
+      if ((b.type >= CG_BC_CBC_UPT && b.type <= CG_BC_CBC_RUP) || b.type == CG_BC_CBC_TOTAL_PT ||
+          b.type == CG_BC_NSCBC_MT)
         zone_sc.emplace_back(b.zone, std::move(scv));
     } else if (key == "DISS_SPONGE" || key == "ROTATING_REFERENCE_FRAME" || key == "ADD_BODY_FORCE") {
       // body forces (addBodyForceSourceTerms, IdealGasSolver.cpp:2159-2202) and the
@@ -2355,7 +2363,7 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
     // the hooked stage is an RHS pass and a separate update, which do not
     // carry the NSCBC faces' own RK update, the sponge's relaxation or the wall
     // model's stress update
-    if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC outlet zones");
+    if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC zones");
     if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
     if (fn && I->wm) return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC zones");
     if (fn && I->tpt) return s->set(CG_EINVAL, "a source hook cannot be combined with CBC_TOTAL_PT zones");

@@ -161,12 +161,14 @@ constexpr int kMaxScalars = 8;  // passive scalars per solve
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
-  BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11
+  BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
 __host__ __device__ inline bool bc_riemann(int type) {
-  return (type >= BC_CBC_UPT && type <= BC_SPONGE) || type == BC_CBC_TOTAL_PT;
+  return (type >= BC_CBC_UPT && type <= BC_SPONGE) || type == BC_CBC_TOTAL_PT || type == BC_NSCBC_MT;
 }
+// the zones whose faces carry an evolved NSCBC state
+__host__ __device__ inline bool bc_nscbc(int type) { return type == BC_NSCBC_OUTLET_P || type == BC_NSCBC_MT; }
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;

@@ -715,6 +715,27 @@ __device__ __forceinline__ void nscbc_outlet_rhs(double rhs[5], const double uni
   }
 }
 
+// calcNscbcIgSubsonicInletRhs (BcCommon.hpp:311-337), FP64: the density rate
+// of an NSCBC mass-flow inlet face (bc, cv = rho, p, u[3]) into rhs[0]; nothing
+// is written when the acoustic wave leaves the domain (u0 + c < 0)
+__device__ __forceinline__ void nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta,
+                                                const double bc[5], const double cv[5], double gamma) {
+  #pragma clang fp contract(off)
+  const double u0 = cv[2] * unit_n[0] + cv[3] * unit_n[1] + cv[4] * unit_n[2];
+  const double un_bc = bc[2] * unit_n[0] + bc[3] * unit_n[1] + bc[4] * unit_n[2];
+  const double dudn = (un_bc - u0) / delta;
+  const double dpdn = (bc[1] - cv[1]) / delta;
+  const double sos_bc = sqrt(gamma * bc[1] / bc[0]);
+  const double lam5 = u0 + sos_bc;
+  if (lam5 < 0.0) return;
+  const double L5 = lam5 * (dpdn + bc[0] * sos_bc * dudn);
+  const double Ma = un_bc / sos_bc;
+  double L1 = -(1.0 + Ma * gamma) / (Ma - 1.0) * L5;
+  L1 /= (1.0 + Ma / (Ma - 1.0) * (gamma - 1.0));
+  const double L2 = 0.5 * (gamma - 1.0) * (L5 + L1);
+  rhs[0] = -(L2 + 0.5 * (L5 + L1)) / sos_bc / sos_bc;
+}
+
 // RkWeights::erk_wgt3 (RkWeights.hpp:6-8), as the NSCBC faces' rk3Step uses them
 __device__ __forceinline__ double erk_wgt3(int stage, int i) {
   if (stage == 1) return i == 0 ? 1.0 : 0.0;
@@ -954,19 +975,34 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
         for (int i = 0; i < 3; ++i) ub[i] = sp[1 + i];
         pb = P.sponge_p;
         hb = P.Rgogm1 * sp[0];
-      } else if (type == BC_NSCBC_OUTLET_P) {
+      } else if (bc_nscbc(type)) {
         #pragma clang fp contract(off)
         q = G(w.nscbc) + 5 * (size_t)b;
         if (update) {
           const double area = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
           const double unit_n[3] = {nd[0] / area, nd[1] / area, nd[2] / area};
           const double delta = area / G(g.bf_aod)[b];
-          const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
-          const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
-          double r[5];
-          nscbc_outlet_rhs(r, unit_n, delta, qs, cv, P.gamma, P.bc_L[z], P.bc_sigma[z], P.bc_p[z]);
           const auto rr = G(w.nscbc_rhs) + 15 * (size_t)b + 5 * (size_t)(stage - 1);
-          for (int k = 0; k < 5; ++k) rr[k] = r[k];
+          if (type == BC_NSCBC_MT) {
+            // NscbcMt::calcRhs (IdealGasSolverBcs.cpp:3761-3840): the face's
+            // velocity (mass flux -mdot / area along n) and pressure (rho R T)
+            // from its density, then the density rate
+            const double rhoun_bc = -P.bc_q[z] / P.bc_area[z];
+            const double RT_bc = P.bc_T[z] * P.R_gas;
+            for (int i = 0; i < 3; ++i) q[2 + i] = (rhoun_bc * unit_n[i] + q[0] * 0.0) / q[0];
+            q[1] = q[0] * RT_bc;
+            const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
+            const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
+            double r[5] = {rr[0], rr[1], rr[2], rr[3], rr[4]};
+            nscbc_inlet_rhs(r, unit_n, delta, qs, cv, P.gamma);
+            rr[0] = r[0];
+          } else {
+            const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
+            const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
+            double r[5];
+            nscbc_outlet_rhs(r, unit_n, delta, qs, cv, P.gamma, P.bc_L[z], P.bc_sigma[z], P.bc_p[z]);
+            for (int k = 0; k < 5; ++k) rr[k] = r[k];
+          }
         }
         const double gogm1 = P.gamma / (P.gamma - 1.0);
         for (int i = 0; i < 3; ++i) ub[i] = q[2 + i];
@@ -2304,7 +2340,7 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
       G(X.w.wm_tau)[b] = tau;
       continue;
     }
-    if (X.P.bc_type[z] != BC_NSCBC_OUTLET_P) continue;
+    if (!bc_nscbc(X.P.bc_type[z])) continue;
     double rho, p, u[3];
     if constexpr (kMixed<R>) {
       const StateT<R>& st = X.st[cur];

@@ -1024,6 +1024,27 @@ static void cbc_state(const og_solver* s, int z, const double n_bf[3], og_state*
 }
 
 /* calcNscbcIgOutletRhs (BcCommon.hpp:419-470), in its operation order */
+/* calcNscbcIgSubsonicInletRhs (BcCommon.hpp:311-337): rhs[0] = d rho_bc / dt of
+ * an NSCBC mass-flow inlet face (mdot and T fixed: L1 and L2 follow from L5);
+ * nothing is written when u0 + c < 0. bc and cv = rho, p, u[3]; returns -1 then */
+int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                       double gamma) {
+  const double u0 = cv[2] * unit_n[0] + cv[3] * unit_n[1] + cv[4] * unit_n[2];
+  const double un_bc = bc[2] * unit_n[0] + bc[3] * unit_n[1] + bc[4] * unit_n[2];
+  const double dudn = (un_bc - u0) / delta;
+  const double dpdn = (bc[1] - cv[1]) / delta;
+  const double sos_bc = sqrt(gamma * bc[1] / bc[0]);
+  const double lam5 = u0 + sos_bc;
+  if (lam5 < 0.0) return -1;
+  const double L5 = lam5 * (dpdn + bc[0] * sos_bc * dudn);
+  const double Ma = un_bc / sos_bc;
+  double L1 = -(1.0 + Ma * gamma) / (Ma - 1.0) * L5;
+  L1 /= (1.0 + Ma / (Ma - 1.0) * (gamma - 1.0));
+  const double L2 = 0.5 * (gamma - 1.0) * (L5 + L1);
+  rhs[0] = -(L2 + 0.5 * (L5 + L1)) / sos_bc / sos_bc;
+  return 0;
+}
+
 double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
                            double gamma, double L_ref, double sigma, double p_ref) {
   const double rho_bc = bc[0], p_bc = bc[1], *u_bc = bc + 2;
@@ -1060,7 +1081,7 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
   const og_params* P = &s->p;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
-    if (P->bc_type[z] != 8) continue;
+    if (P->bc_type[z] != 8 && P->bc_type[z] != 12) continue;
     const double* n = &m->n_bf[3 * (size_t)ibf];
     const double bf_area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const double unit_n[3] = {n[0] / bf_area, n[1] / bf_area, n[2] / bf_area};
@@ -1068,8 +1089,19 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
     const double delta = bf_area / m->aod_bf[ibf];
     const double cv[5] = {s->rho[icv], s->pr[icv], s->u[3 * (size_t)icv], s->u[3 * (size_t)icv + 1],
                           s->u[3 * (size_t)icv + 2]};
-    og_nscbc_outlet_rhs(&s->nscbc_rhs[15 * (size_t)ibf + 5 * (size_t)(rk_stage - 1)], unit_n, delta,
-                        &s->nscbc[5 * (size_t)ibf], cv, P->gamma, P->bc_L[z], P->bc_sigma[z], P->bc_p[z]);
+    double* rhs = &s->nscbc_rhs[15 * (size_t)ibf + 5 * (size_t)(rk_stage - 1)];
+    double* b = &s->nscbc[5 * (size_t)ibf];
+    if (P->bc_type[z] == 12) {
+      /* NscbcMt::calcRhs (IdealGasSolverBcs.cpp:3761-3840): velocity and
+       * pressure from the density, mass flux -mdot / area, static T */
+      const double rhoun_bc = -P->bc_q[z] / P->bc_area[z];
+      const double RT_bc = P->bc_T[z] * s->R_gas;
+      for (int i = 0; i < 3; ++i) b[2 + i] = (rhoun_bc * unit_n[i] + b[0] * 0.0) / b[0];
+      b[1] = b[0] * RT_bc;
+      og_nscbc_inlet_rhs(rhs, unit_n, delta, b, cv, P->gamma);
+    } else {
+      og_nscbc_outlet_rhs(rhs, unit_n, delta, b, cv, P->gamma, P->bc_L[z], P->bc_sigma[z], P->bc_p[z]);
+    }
   }
 }
 
@@ -1077,7 +1109,7 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
 static void nscbc_rk3(og_solver* s, int rk_stage) {
   const og_mesh* m = &s->m;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
-    if (s->p.bc_type[m->zone_bf[ibf]] != 8) continue;
+    if (s->p.bc_type[m->zone_bf[ibf]] != 8 && s->p.bc_type[m->zone_bf[ibf]] != 12) continue;
     double* b = &s->nscbc[5 * (size_t)ibf];
     const double* r = &s->nscbc_rhs[15 * (size_t)ibf];
     for (int i = 0; i < rk_stage; ++i) {
@@ -1093,7 +1125,7 @@ static void nscbc_rk3(og_solver* s, int rk_stage) {
 static void nscbc_init(og_solver* s) {
   const og_mesh* m = &s->m;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
-    if (s->p.bc_type[m->zone_bf[ibf]] != 8) continue;
+    if (s->p.bc_type[m->zone_bf[ibf]] != 8 && s->p.bc_type[m->zone_bf[ibf]] != 12) continue;
     const int icv = m->cvobf[ibf];
     double* b = &s->nscbc[5 * (size_t)ibf];
     b[0] = s->rho[icv];
@@ -1190,7 +1222,7 @@ static void wm_calc(og_solver* s) {
   }
 }
 
-static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11; }
+static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11 || type == 12; }
 
 /* CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3449): un_bc = the
  * cell's normal velocity */
@@ -1347,7 +1379,7 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
         for (int i = 0; i < 3; ++i) b.u[i] = sp[1 + i];
         b.p = P->sponge_p;
         b.h = Rgogm1 * sp[0];
-      } else if (P->bc_type[z] == 8) {
+      } else if (P->bc_type[z] == 8 || P->bc_type[z] == 12) {
         const double* q = &s->nscbc[5 * (size_t)ibf];
         const double gogm1 = P->gamma / (P->gamma - 1.0);
         for (int i = 0; i < 3; ++i) b.u[i] = q[2 + i];

@@ -149,6 +149,10 @@ double og_fast_pow_posx(double x, double p);
 /* AlgebraicWM::solve_tau (wm/AlgebraicWM.hpp:40-113): the wall stress of the
  * law of the wall for speed u1 at distance y1 (10 WM_ALG_ADIABATIC faces) */
 double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess);
+/* calcNscbcIgSubsonicInletRhs (BcCommon.hpp:311-337): rhs[0] of an NSCBC
+ * mass-flow inlet face (12 NSCBC_MT: mdot in bc_q, T in bc_T); -1 when u0 + c < 0 */
+int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                       double gamma);
 double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
                            double gamma, double L_ref, double sigma, double p_ref);
 void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,

@@ -114,6 +114,8 @@ def lib(path=None):
         L.og_nscbc_outlet_rhs.restype = C.c_double
         L.og_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                           C.c_double]
+        L.og_nscbc_inlet_rhs.restype = C.c_int
+        L.og_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         L.og_wm_solve_tau.restype = C.c_double
         L.og_wm_solve_tau.argtypes = [C.c_double] * 5
         _libs[path] = L
@@ -138,6 +140,8 @@ def ref():
         R.ref_nscbc_outlet_rhs.restype = C.c_double
         R.ref_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                            C.c_double]
+        R.ref_nscbc_inlet_rhs.restype = C.c_int
+        R.ref_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         R.ref_wm_solve_tau.restype = C.c_double
         R.ref_wm_solve_tau.argtypes = [C.c_double] * 5
         _ref = R
@@ -188,7 +192,7 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
         p.bc_p[z], p.bc_rho[z], p.bc_q[z] = b.p, b.rho, b.q
         p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
         p.bc_sigma[z], p.bc_L[z] = b.sigma, b.L
-        if 4 <= b.type <= 7 or b.type == 11:  # CBC zones bring their scalars' values in; the others 0
+        if 4 <= b.type <= 7 or b.type in (11, 12):  # CBC zones and NSCBC inlets bring scalars in; others 0
             for i in range(min(cgp.nsc, 8)):
                 p.bc_sc[z][i] = b.sc[i]
     for name, _ in og_params._fields_:

@@ -6,6 +6,7 @@
 //   calcInternalCompactFluxNewGrad   src/charles/ideal_gas/IdealGasSolverFlux.hpp:9-215
 //   calcRiemannFlux                  src/charles/ideal_gas/IdealGasSolverFlux.hpp:218-306
 //   calcNscbcIgOutletRhs             src/charles/BcCommon.hpp:419-470
+//   calcNscbcIgSubsonicInletRhs      src/charles/BcCommon.hpp:311-337
 //   calcSvd33                        src/core/common/SvdNew.hpp:24
 //   MiscUtils::fast_pow_posx         src/core/common/MiscUtils.hpp:273-276
 //   RkWeights::erk_wgt3              src/charles/RkWeights.hpp:6-8
@@ -115,6 +116,12 @@ int ref_svd33(const double* A, double* U, double* V, double* sigma, double* max_
 void ref_erk_wgt3(double* w) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) w[3 * i + j] = RkWeights::erk_wgt3[i][j];
+}
+
+// rhs[0] of an NSCBC mass-flow inlet face; bc, cv = rho, p, u[3]
+int ref_nscbc_inlet_rhs(double* rhs, const double* unit_n, double delta, const double* bc, const double* cv,
+                        double gamma) {
+  return calcNscbcIgSubsonicInletRhs(rhs, unit_n, delta, bc[0], bc + 2, bc[1], cv[0], cv + 2, cv[1], gamma);
 }
 
 // wall stress of the algebraic wall model

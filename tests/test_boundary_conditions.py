@@ -503,3 +503,34 @@ def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter):
     rho, u, rhoE = o.get_state()
     np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
     np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
+
+
+def test_nscbc_mt_parser():
+    p = parse_params(deck({"y0": "NSCBC_MT 2.5 1.1 Z 0.3", "y1": "SLIP"}) + "REGISTER_SCALAR Z\n")
+    b = p.bc[0]
+    assert (b.type, b.q, b.T, b.sc[0]) == (12, 2.5, 1.1, 0.3)
+    for bad in ("y0 NSCBC_MT 2.5", "y0 NSCBC_MT 2.5 T_TOTAL 300", "y0 NSCBC_MT 2.5 -1"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_nscbc_mt_inlet_keeps_a_matching_uniform_stream(jitter):
+    # a uniform stream V in through an NSCBC_MT inlet of the stream's mass flow
+    # (mdot = rho V area) and temperature: the faces' velocity -mdot/area/rho n
+    # is the stream's, their density and pressure the cells', every rate
+    # vanishes, so the RHS is zero and five steps keep the stream
+    m = Mesh.channel(10, jitter=jitter, seed=15)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    area = float(np.bincount(m.get("zone_bf"), weights=m.get("area_bf"))[0])
+    text = deck({"y0": f"NSCBC_MT {1.0 * V * area!r} {T}", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    r = o.calc_rhs()
+    assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)

@@ -6,7 +6,7 @@ a TRANSLATE wall velocity), WALL_ISOTHERMAL, the characteristic conditions
 CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP, the NSCBC pressure outlet (per-face
 states evolved with the step, restored with it after a NaN), the algebraic
 wall model WM_ALG_ADIABATIC (per-face stress from the law of the wall), the
-total-pressure inlet CBC_TOTAL_PT and the SPONGE
+total-pressure inlet CBC_TOTAL_PT, the NSCBC mass-flow inlet NSCBC_MT and the SPONGE
 (per-cell relaxed T, u, a volume term over the last LENGTH of the domain, the
 Riemann flux at its faces; L_Y and L_RX on channels, L_X on the orifice) on
 jittered channel meshes and on the reference's own Stitch mesh orifice.mles
@@ -51,6 +51,9 @@ WALLS = {
                        "y1": f"SPONGE TYPE L_Y LENGTH 1.5 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {P0 * 0.995!r}"},
     # in through a total-pressure inlet, out through an NSCBC outlet
     "tpt+nscbc": {"y0": f"CBC_TOTAL_PT {P0 * 1.004!r} 1.002 0.05", "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    # in through an NSCBC mass-flow inlet (0.3 c0 / 10 into the channel), out through an NSCBC outlet
+    "nscbc_mt+nscbc": {"y0": f"NSCBC_MT {0.3 * (2 * np.pi) ** 2!r} 1.0",
+                       "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
     # the algebraic wall model on both walls, one of them moving
     "wm+wm_translate": {"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.3 0 0.1"},
     # a radial sponge as the first zone (its volume term before the other zone's faces)
@@ -212,7 +215,7 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
     check_history(g, o, 20, "orifice nscbc")
 
 
-@pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate"])
+@pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate", "nscbc_mt+nscbc"])
 def test_boundary_states_roll_back_with_the_flow(walls):
     # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
     # faces' or the sponge's states with it (storeData / restoreData of the
@@ -245,7 +248,7 @@ def test_boundary_states_roll_back_with_the_flow(walls):
 
 
 @pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge",
-                                   "wm+wm_translate", "tpt+nscbc"])
+                                   "wm+wm_translate", "tpt+nscbc", "nscbc_mt+nscbc"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]

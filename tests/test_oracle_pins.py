@@ -355,3 +355,43 @@ def test_wall_model_closed_forms():
     u1 = 1e-6  # y+ ~ 0.008: the quadratic term is a 3e-4 correction
     tau = L.og_wm_solve_tau(u1, y1, rho, mu, -1.0)
     np.testing.assert_allclose(tau, mu * u1 / y1, rtol=1e-3)
+
+
+@needs_ref
+def test_nscbc_inlet_rhs_bit_identical_to_reference():
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(47)
+    for k in range(3000):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        bc = np.array([rng.uniform(0.8, 1.2), rng.uniform(0.8, 1.2), *(-rng.uniform(0.0, 0.6) * n)])
+        cv = bc * (1.0 + 0.05 * rng.normal(size=5))
+        if k % 9 == 0:
+            cv[2:] = 3.0 * n * -1.0  # u0 + c < 0: nothing written
+        a = np.full(5, 7.0)
+        b = np.full(5, 7.0)
+        ra = L.og_nscbc_inlet_rhs(dp(a), dp(n), 0.03, dp(bc), dp(cv), 1.4)
+        rb = R.ref_nscbc_inlet_rhs(dp(b), dp(n), 0.03, dp(bc), dp(cv), 1.4)
+        assert ra == rb
+        np.testing.assert_array_equal(a, b)
+
+
+def test_nscbc_inlet_rhs_closed_form():
+    # with the face's normal speed equal to the cell's (no velocity gradient)
+    # only the pressure gradient drives it: L5 = (u0 + c) dp/dn and, with mdot
+    # and T fixed, d rho_bc / dt = gamma L5 / ((M gamma - 1) c^2), M = un / c
+    L = orc.lib()
+    n = np.array([0.0, -1.0, 0.0])
+    gamma, rho, p = 1.4, 1.1, 0.9
+    c = math.sqrt(gamma * p / rho)
+    un = -0.2 * c
+    bc = np.array([rho, p, 0.0, un * n[1], 0.0])
+    for dpc in (0.0, 0.02, -0.03):
+        cv = bc.copy()
+        cv[1] = p - dpc
+        r = np.zeros(5)
+        assert L.og_nscbc_inlet_rhs(dp(r), dp(n), 0.05, dp(bc), dp(cv), gamma) == 0
+        L5 = (un + c) * (dpc / 0.05)
+        M = un / c
+        np.testing.assert_allclose(r[0], gamma * L5 / ((M * gamma - 1.0) * c * c), rtol=1e-12, atol=1e-300)
+        assert np.all(r[1:] == 0.0)
