@@ -122,6 +122,17 @@ typedef struct cg_solver cg_solver;
  *                                    y1 = area / area_over_delta, applied as a
  *                                    force along it; RELAX, Y_PLUS0, TOL belong
  *                                    to the equilibrium model and are ignored
+ *   WM_ALG_ISOTHERMAL <T> [RELAX r] [Y_PLUS0 y] [TOL t]
+ *                                    WmIsothermal with WM_ALGEBRAIC_ISOTHERMAL
+ *                                    (:2054-2226): per face tau_wall from the
+ *                                    law of the wall for the cell's speed |u|
+ *                                    and a wall heat flux q_wall toward the
+ *                                    wall at T (AlgebraicWM::
+ *                                    compute_q_wall_approx, wm/AlgebraicWM.hpp:
+ *                                    115-162), both at the start of each step;
+ *                                    p n plus tau_wall along u, q_wall area out
+ *                                    of the energy; RELAX, Y_PLUS0, TOL are
+ *                                    ignored as for WM_ALG_ADIABATIC
  * The other types of the reference are refused by cg_params_parse. */
 #define CG_MAX_BC_ZONES 16
 #define CG_BC_SLIP 1
@@ -136,10 +147,11 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_WM_ALG_ADIABATIC 10
 #define CG_BC_CBC_TOTAL_PT 11
 #define CG_BC_NSCBC_MT 12
+#define CG_BC_WM_ALG_ISOTHERMAL 13
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
-  double T;       /* wall temperature (WALL_ISOTHERMAL), T of CBC_UPT / CBC_MPT, SPONGE RELAX_TIME,
+  double T;       /* wall temperature (WALL_ISOTHERMAL, WM_ALG_ISOTHERMAL), T of CBC_UPT / CBC_MPT, SPONGE RELAX_TIME,
                      CBC_TOTAL_PT T0 */
   double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
   double p;       /* CBC pressure (CBC_TOTAL_PT: p0), NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */

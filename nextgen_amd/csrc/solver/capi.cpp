@@ -147,7 +147,8 @@ struct HostInst {
   size_t snap_wm = 0;                // then the wall-model faces' stresses (1 per boundary face)
   size_t snap_sp = 0;                // and, at the end, the sponge's relaxed T, u (4 per cell)
   bool nscbc = false;                // some zone is an NSCBC outlet
-  bool wm = false;                   // some zone is a WM_ALG_ADIABATIC wall
+  bool wm = false;                   // some zone is a WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL wall
+  bool wm_iso = false;               // some zone is a WM_ALG_ISOTHERMAL wall
   bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
@@ -926,7 +927,8 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P || p->bc[k].type == CG_BC_NSCBC_MT;
-      hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC;
+      hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
+      hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       if (p->bc[k].type == CG_BC_SPONGE) {
         // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
@@ -999,8 +1001,10 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   }
   if (hi->tpt) rc |= dev_alloc(s, h, &w.tpt_un, (size_t)lay->g.nbf);  // not snapshotted (no storeData)
   if (hi->wm) {
-    // WmAdiabatic::storeData: the faces' tau_wall (in the snapshots too)
+    // WmAdiabatic::storeData, WmIsothermal::storeData: the faces' tau_wall
+    // (in the snapshots too); q_wall is not stored
     rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf);
+    if (hi->wm_iso) rc |= dev_alloc(s, h, &w.wm_q, (size_t)lay->g.nbf);
     hi->snap_wm = (size_t)lay->g.nbf;
     hi->snap_n += hi->snap_wm;
   }
@@ -1086,6 +1090,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
   }
   if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)lay->g.nbf, stream);  // set by initComplete
+  if (hi->wm_iso) rc |= cgk::launch_fill(w.wm_q, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->tpt) rc |= cgk::launch_fill(w.tpt_un, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->sponge) {
     // the relaxed state is set from the cells by initComplete
@@ -1788,6 +1793,21 @@ int cg_params_parse(cg_params* p, const char* text) {
             return bad("unknown WM_ALG_ADIABATIC option " + opt);
           }
         }
+      } else if (type == "WM_ALG_ISOTHERMAL") {
+        // WmIsothermal::initData (IdealGasSolverBcs.cpp:2054-2089): the wall
+        // temperature, then RELAX, Y_PLUS0 and TOL, the equilibrium model's,
+        // with a value each
+        b.type = CG_BC_WM_ALG_ISOTHERMAL;
+        if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WM_ALG_ISOTHERMAL <T>");
+        for (size_t i = 3; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          double ignored = 0.0;
+          if (opt == "RELAX" || opt == "Y_PLUS0" || opt == "TOL") {
+            if (!num(++i, &ignored)) return bad("expected " + opt + " <value>");
+          } else {
+            return bad("unknown WM_ALG_ISOTHERMAL option " + opt);
+          }
+        }
       } else if (type == "WALL_ISOTHERMAL") {
         b.type = CG_BC_WALL_ISOTHERMAL;
         if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WALL_ISOTHERMAL <T>");
@@ -1892,7 +1912,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
+                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_ALG_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
                    "NSCBC_MT, NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE and SPONGE are)");
       }
@@ -2365,7 +2385,8 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
     // model's stress update
     if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC zones");
     if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
-    if (fn && I->wm) return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC zones");
+    if (fn && I->wm)
+      return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL zones");
     if (fn && I->tpt) return s->set(CG_EINVAL, "a source hook cannot be combined with CBC_TOTAL_PT zones");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;

@@ -144,9 +144,11 @@ struct WorkT {
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
-  // WM_ALG_ADIABATIC faces (null without them): per boundary face its wall
-  // stress tau_wall (FP64, in the snapshots)
-  double* wm_tau;
+  // WM_ALG_ADIABATIC and WM_ALG_ISOTHERMAL faces (null without them): per
+  // boundary face its wall stress tau_wall (FP64, in the snapshots) and, with
+  // an isothermal zone, its wall heat flux q_wall (FP64, not in the snapshots,
+  // as in the reference: each step's stage 1 sets it before it is used)
+  double *wm_tau, *wm_q;
   // CBC_TOTAL_PT faces (null without them): per boundary face its inflow
   // normal velocity un_bc (FP64; not in the snapshots, as in the reference)
   double* tpt_un;
@@ -161,7 +163,8 @@ constexpr int kMaxScalars = 8;  // passive scalars per solve
 enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
-  BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12
+  BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12,
+  BC_WM_ALG_ISOTHERMAL = 13
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
 __host__ __device__ inline bool bc_riemann(int type) {

@@ -892,6 +892,92 @@ __device__ __forceinline__ void wm_face(const GeomT<R>& g, const WorkT<R>& w, co
   }
 }
 
+// AlgebraicWM::compute_q_wall_approx (wm/AlgebraicWM.hpp:115-162), FP64: the
+// wall heat flux (k_lam + k_t) (T1 - Tw) / y1 / (1 + kappa_t T*) of a
+// thermal law of the wall, k_t = cp rho kappa u_tau y1 / Pr_t (Pr_t 0.85,
+// kappa_t = kappa / Pr_t), T* = Re/2 up to Re = y* = 8.28314, beyond it
+// y*^2 / (2 Re) + C_t (1 - y*/Re) + (ln Re - 1 - y*/Re (ln y* - 1)) / kappa_t
+// (C_t 3.9), Re = u_tau rho y1 / mu, u_tau = sqrt(tau_w / rho)
+__device__ __forceinline__ double wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu,
+                                            double locp, double cp, double y1) {
+  #pragma clang fp contract(off)
+  const double kappa = 0.41, Pr_t = 0.85, C_t = 3.9, ystar = 8.28314;
+  const double kappa_t = kappa / Pr_t;
+  const double u_tau = sqrt(tau_w / rho);
+  const double mu_t = rho * kappa * u_tau * y1;
+  const double l_t = mu_t * cp / Pr_t;
+  const double l_tot = locp * cp + l_t;
+  const double rhs = l_tot * (T1 - Tw) / y1;
+  const double re_delta = u_tau * rho * y1 / mu;
+  double Tstar;
+  if (re_delta <= ystar) {
+    Tstar = 0.5 * re_delta;
+  } else {
+    Tstar = 0.5 * ystar * ystar / re_delta;
+    Tstar += C_t * (1.0 - ystar / re_delta);
+    double t = log(re_delta) - 1.0;
+    t -= ystar / re_delta * (log(ystar) - 1.0);
+    t /= kappa_t;
+    Tstar += t;
+  }
+  return rhs / (1.0 + kappa_t * Tstar);
+}
+
+// A WM_ALG_ISOTHERMAL face b of cell c (WmIsothermal, IdealGasSolverBcs.cpp:
+// 2054-2226): at the start of a step (stage 1 of a fused step) its wall stress
+// from the law of the wall for the cell's speed |u| at y1 = area /
+// area_over_delta and its heat flux toward the wall temperature from that
+// stress (calcRhs; both 0 where area_over_delta is 0), then p n plus the
+// stress along u and the heat flux out of the energy (addBoundaryFlux). FP64
+// from the FP64 state; T is the stage's, mu_lam and loc_lam the lagged ones.
+template <class R>
+__device__ __forceinline__ void wm_iso_face(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st,
+                                            const Params& P, int c, int b, int z, const Light<R>& self, R acc[5],
+                                            int stage, bool update) {
+  #pragma clang fp contract(off)
+  double rho, u[3], rhoE, spv, p, h, T;
+  cell_state64(g, st, P, c, self, rho, u, rhoE, spv, p, h, T);
+  const double n[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
+  const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const auto tw = G(w.wm_tau) + b;
+  const auto qw = G(w.wm_q) + b;
+  if (update && stage == 1) {
+    const double aod = G(g.bf_aod)[b];
+    double tau = 0.0, q = 0.0;
+    if (aod > 0.0) {
+      const double y1 = n_mag / aod;
+      const double u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      const double u_tau = sqrt(*tw / rho);
+      const double mu = G(w.mu_lam)[c];
+      tau = wm_solve_tau(u1, y1, rho, mu, u_tau);
+      const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
+      q = wm_q_wall(T, P.bc_T[z], tau, rho, mu, G(w.loc_lam)[c], cp, y1);
+    }
+    *tw = tau;
+    *qw = q;
+  }
+  double u_mag = 0.0;
+  for (int i = 0; i < 3; ++i) u_mag += u[i] * u[i];
+  u_mag = sqrt(fmax(0.0, u_mag));
+  double force[3] = {0.0, 0.0, 0.0};
+  if (u_mag > 0.0) {
+    const double tau = *tw;
+    for (int i = 0; i < 3; ++i) force[i] = tau * u[i] / u_mag * n_mag;
+  }
+  const double heat = *qw * n_mag;
+  if constexpr (kMixed<R>) {
+    // momentum without the gauge's p_g n
+    for (int i = 0; i < 3; ++i) acc[1 + i] -= R((p - P.p_g) * n[i] + force[i]);
+    acc[4] -= R(heat);
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      acc[1 + i] -= p * n[i];
+      if (u_mag > 0.0) acc[1 + i] -= force[i];
+    }
+    acc[4] -= heat;
+  }
+}
+
 // The boundary fluxes of cell c (addBoundaryFlux of each zone, in zone order,
 // IdealGasSolverBcs.cpp:14-47,1485-1510), which calcRhsBase adds before any
 // face (IdealGasSolver.cpp:261-262): the start of its stage RHS accumulation.
@@ -945,6 +1031,8 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       acc[4] -= visc_work;
     } else if (type == BC_WM_ALG_ADIABATIC) {
       wm_face(g, w, st, P, c, b, z, self, acc, stage, update);
+    } else if (type == BC_WM_ALG_ISOTHERMAL) {
+      wm_iso_face(g, w, st, P, c, b, z, self, acc, stage, update);
     } else if (bc_riemann(type)) {
       // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
       // between the cell's FP64 state and the zone's boundary state
@@ -2338,6 +2426,29 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
         tau = G(X.w.mu_lam)[c] * u1 / y1;
       }
       G(X.w.wm_tau)[b] = tau;
+      continue;
+    }
+    if (X.P.bc_type[z] == BC_WM_ALG_ISOTHERMAL) {
+      // WmIsothermal::initialHook (IdealGasSolverBcs.cpp:2090-2112): the
+      // laminar stress mu_lam |u| / y1 and heat flux mu_lam (cp T + |u|^2 / 2
+      // - cp T_wall) / y1 (0 where area_over_delta is 0)
+      #pragma clang fp contract(off)
+      double rho, u[3], rhoE, spv, p, h, T;
+      const Light<R> self = load_light(X.st[cur].light, c);
+      cell_state64(g, X.st[cur], X.P, c, self, rho, u, rhoE, spv, p, h, T);
+      const double aod = G(g.bf_aod)[b];
+      double tau = 0.0, q = 0.0;
+      if (aod > 0.0) {
+        const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
+        const double y1 = sqrt(n0 * n0 + n1 * n1 + n2 * n2) / aod;
+        const double cp = X.P.R_gas * X.P.gamma / (X.P.gamma - 1.0);
+        const double mu = G(X.w.mu_lam)[c];
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        tau = mu * sqrt(uu) / y1;
+        q = mu * (cp * T + 0.5 * uu - cp * X.P.bc_T[z]) / y1;
+      }
+      G(X.w.wm_tau)[b] = tau;
+      G(X.w.wm_q)[b] = q;
       continue;
     }
     if (!bc_nscbc(X.P.bc_type[z])) continue;

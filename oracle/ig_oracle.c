@@ -194,6 +194,7 @@ struct og_solver {
   /* WmAdiabatic (IdealGasSolverBcs.hpp): tau_wall per boundary face and the
    * two snapshots */
   double *wm_tau, *wm_store, *wm_recent;
+  double* wm_q; /* WM_ALG_ISOTHERMAL q_wall (not stored, as WmIsothermal::storeData) */
   /* CbcTotalPt: un_bc per boundary face (not part of storeData) */
   double* tpt_un;
 };
@@ -245,6 +246,7 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   }
   s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_tau = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_q = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->tpt_un = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_store = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_recent = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
@@ -270,7 +272,7 @@ void og_destroy(og_solver* s) {
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
   free(s->sponge); free(s->sponge_store); free(s->sponge_recent); free(s->mf_bf);
-  free(s->wm_tau); free(s->wm_store); free(s->wm_recent); free(s->tpt_un);
+  free(s->wm_tau); free(s->wm_q); free(s->wm_store); free(s->wm_recent); free(s->tpt_un);
   free(s);
 }
 
@@ -1182,11 +1184,28 @@ static void wm_parallel(const og_solver* s, int ibf, int icv, const double* ubc,
 }
 
 /* WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1102): the laminar
- * stress mu_lam |u - u_bc| / y1, y1 = area / area_over_delta */
+ * stress mu_lam |u - u_bc| / y1, y1 = area / area_over_delta;
+ * WmIsothermal::initialHook (:2090-2112): mu_lam |u| / y1 and the heat flux
+ * mu_lam (cp T + |u|^2 / 2 - cp T_wall) / y1 */
 static void wm_init(og_solver* s) {
   const og_mesh* m = &s->m;
+  const double cp = s->R_gas * s->p.gamma / (s->p.gamma - 1.0);
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] == 13) {
+      const int icv = m->cvobf[ibf];
+      const double* u = &s->u[3 * (size_t)icv];
+      const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      s->wm_tau[ibf] = 0.0;
+      s->wm_q[ibf] = 0.0;
+      if (m->aod_bf[ibf] > 0.0) {
+        const double* n = &m->n_bf[3 * (size_t)ibf];
+        const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
+        s->wm_tau[ibf] = s->mu_lam[icv] * sqrt(uu) / y1;
+        s->wm_q[ibf] = s->mu_lam[icv] * (cp * s->T[icv] + 0.5 * uu - cp * s->p.bc_T[z]) / y1;
+      }
+      continue;
+    }
     if (s->p.bc_type[z] != 10) continue;
     const int icv = m->cvobf[ibf];
     double d[3];
@@ -1201,12 +1220,53 @@ static void wm_init(og_solver* s) {
   }
 }
 
+/* AlgebraicWM::compute_q_wall_approx (wm/AlgebraicWM.hpp:115-162): the wall
+ * heat flux of the thermal law of the wall, (lambda + lambda_t)(T1 - Tw)/y1
+ * over 1 + kappa_t T*, lambda_t = cp rho kappa u_tau y1 / Pr_t, Pr_t = 0.85,
+ * kappa_t = 0.41 / Pr_t, T* linear (Re/2) up to Re = y* = 8.28314 and
+ * logarithmic with C_t = 3.9 beyond, Re = u_tau rho y1 / mu */
+double og_wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu, double locp, double cp, double y1) {
+  const double kappa = 0.41, Pr_t = 0.85, C_t = 3.9, ystar = 8.28314;
+  const double kappa_t = kappa / Pr_t;
+  const double u_tau = sqrt(tau_w / rho);
+  const double lam_t = rho * kappa * u_tau * y1 * cp / Pr_t;
+  const double num = (locp * cp + lam_t) * (T1 - Tw) / y1;
+  const double re = u_tau * rho * y1 / mu;
+  double Tstar;
+  if (re <= ystar) {
+    Tstar = 0.5 * re;
+  } else {
+    const double lg = (log(re) - 1.0 - ystar / re * (log(ystar) - 1.0)) / kappa_t;
+    Tstar = 0.5 * ystar * ystar / re + C_t * (1.0 - ystar / re) + lg;
+  }
+  return num / (1.0 + kappa_t * Tstar);
+}
+
 /* WmAdiabatic::calcRhs at stage 1 (IdealGasSolverBcs.cpp:1256-1354): each
- * face's stress from the law of the wall for the cell's parallel speed */
+ * face's stress from the law of the wall for the cell's parallel speed;
+ * WmIsothermal::calcRhs at stage 1 (:2173-2206): the stress for the cell's
+ * speed |u|, then the heat flux toward the wall temperature */
 static void wm_calc(og_solver* s) {
   const og_mesh* m = &s->m;
+  const double cp = s->R_gas * s->p.gamma / (s->p.gamma - 1.0);
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] == 13) {
+      const int icv = m->cvobf[ibf];
+      s->wm_tau[ibf] = 0.0;
+      s->wm_q[ibf] = 0.0;
+      if (m->aod_bf[ibf] > 0.0) {
+        const double* n = &m->n_bf[3 * (size_t)ibf];
+        const double* u = &s->u[3 * (size_t)icv];
+        const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
+        const double u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double u_tau = sqrt(s->wm_tau[ibf] / s->rho[icv]);
+        s->wm_tau[ibf] = og_wm_solve_tau(u1, y1, s->rho[icv], s->mu_lam[icv], u_tau);
+        s->wm_q[ibf] = og_wm_q_wall(s->T[icv], s->p.bc_T[z], s->wm_tau[ibf], s->rho[icv], s->mu_lam[icv],
+                                    s->loc_lam[icv], cp, y1);
+      }
+      continue;
+    }
     if (s->p.bc_type[z] != 10) continue;
     const int icv = m->cvobf[ibf];
     if (m->aod_bf[ibf] > 0.0) {
@@ -1354,6 +1414,18 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
       } else {
         for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i];
       }
+    } else if (P->bc_type[z] == 13) {
+      /* WmIsothermal::addBoundaryFlux (IdealGasSolverBcs.cpp:2208-2226): the
+       * stress along the full velocity, the heat flux out of the energy */
+      const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      double u_mag = 0.0;
+      for (int i = 0; i < 3; ++i) u_mag += u[i] * u[i];
+      u_mag = sqrt(u_mag > 0.0 ? u_mag : 0.0);
+      for (int i = 0; i < 3; ++i) {
+        rhs[icv].rhou[i] -= p * n[i];
+        if (u_mag > 0.0) rhs[icv].rhou[i] -= s->wm_tau[ibf] * u[i] / u_mag * area;
+      }
+      rhs[icv].rhoE -= s->wm_q[ibf] * area;
     } else if (bc_riemann(P->bc_type[z])) {
       /* Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976); an NSCBC face
        * (Nscbc::addBoundaryFlux, :3662-3693) uses its evolved state */

@@ -149,6 +149,10 @@ double og_fast_pow_posx(double x, double p);
 /* AlgebraicWM::solve_tau (wm/AlgebraicWM.hpp:40-113): the wall stress of the
  * law of the wall for speed u1 at distance y1 (10 WM_ALG_ADIABATIC faces) */
 double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess);
+/* AlgebraicWM::compute_q_wall_approx (wm/AlgebraicWM.hpp:115-162): the wall
+ * heat flux of cell temperature T1 toward Tw for stress tau_w (13
+ * WM_ALG_ISOTHERMAL faces; locp = loc_lam, the conductivity over cp) */
+double og_wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu, double locp, double cp, double y1);
 /* calcNscbcIgSubsonicInletRhs (BcCommon.hpp:311-337): rhs[0] of an NSCBC
  * mass-flow inlet face (12 NSCBC_MT: mdot in bc_q, T in bc_T); -1 when u0 + c < 0 */
 int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],

@@ -118,6 +118,8 @@ def lib(path=None):
         L.og_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         L.og_wm_solve_tau.restype = C.c_double
         L.og_wm_solve_tau.argtypes = [C.c_double] * 5
+        L.og_wm_q_wall.restype = C.c_double
+        L.og_wm_q_wall.argtypes = [C.c_double] * 8
         _libs[path] = L
     return _libs[path]
 
@@ -144,6 +146,8 @@ def ref():
         R.ref_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         R.ref_wm_solve_tau.restype = C.c_double
         R.ref_wm_solve_tau.argtypes = [C.c_double] * 5
+        R.ref_wm_q_wall.restype = C.c_double
+        R.ref_wm_q_wall.argtypes = [C.c_double] * 8
         _ref = R
     return _ref
 

@@ -129,4 +129,8 @@ double ref_wm_solve_tau(double u1, double y1, double rho, double mu, double u_ta
   return AlgebraicWM::solve_tau(u1, y1, rho, mu, u_tau_guess);
 }
 
+double ref_wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu, double locp, double cp, double y1) {
+  return AlgebraicWM::compute_q_wall_approx(T1, Tw, tau_w, rho, mu, locp, cp, y1);
+}
+
 }  // extern "C"

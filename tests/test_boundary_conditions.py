@@ -32,6 +32,13 @@ terms, pinned by closed forms.
   the lattice channel's wall loses mu 2h U of momentum per wall cell, the
   no-slip wall's closed form; the law-of-the-wall solver is pinned in
   test_oracle_pins.py.
+* WM_ALG_ISOTHERMAL (WmIsothermal, IdealGasSolverBcs.cpp:2054-2226): it also
+  starts laminar (initialHook: stress mu |u| / y1, heat flux mu (cp T + |u|^2/2 -
+  cp T_wall) / y1), so a uniform stream loses mu 2h U of momentum and mu 2h (cp
+  (T - T_wall) + U^2/2) of energy per wall cell; a gas at rest has no stress,
+  and the heat flux its first step's stage 1 sets is Fourier conduction to the
+  wall, the stage-1 RHS of the same deck with WALL_ISOTHERMAL walls; the
+  thermal law of the wall is pinned in test_oracle_pins.py.
 * Passive scalars through boundaries (_addScalarBoundaryFlux, BcCommon.hpp:276-291):
   at rest and scalar-free, a CBC inflow zone with value Z_in feeds the wall cells
   -mf Z_in (mf the Riemann flux's mass, negative inflow); a uniform stream and a
@@ -448,6 +455,46 @@ def test_wall_model_starts_with_the_laminar_stress():
     np.testing.assert_allclose(r[upper, 4], -0.25 * mu[upper] * 2 * H * (U - 0.25), rtol=1e-10)
     # the law of the wall then takes over: one step moves the wall cells' shear
     assert o.step() == 0
+
+
+def test_isothermal_wall_model_parser():
+    p = parse_params(deck({"y0": "WM_ALG_ISOTHERMAL 300", "y1": "WM_ALG_ISOTHERMAL 310.5 RELAX 0.5 Y_PLUS0 30 TOL 1e-5"}))
+    assert (p.bc[0].type, p.bc[0].T) == (13, 300.0)
+    assert (p.bc[1].type, p.bc[1].T) == (13, 310.5)
+    for bad in ("y0 WM_ALG_ISOTHERMAL", "y0 WM_ALG_ISOTHERMAL -1", "y0 WM_ALG_ISOTHERMAL 300 RELAX",
+                "y0 WM_ALG_ISOTHERMAL 300 TRANSLATE 1 0 0"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+def test_isothermal_wall_model_starts_laminar():
+    m = Mesh.channel(10, jitter=0.0)
+    U, R, gamma = 0.7, 71.42857142857143, 1.4
+    cp = R * gamma / (gamma - 1.0)
+    o = oracle(m, deck({"y0": "WM_ALG_ISOTHERMAL 1.1", "y1": "WM_ALG_ISOTHERMAL 0.9"}), uniform(m, (U, 0, 0)))
+    mu = o.get_field("mu_lam")
+    r = o.calc_rhs()
+    for zone, Tw in ((0, 1.1), (1, 0.9)):
+        c = wall_cells(m, zone)
+        np.testing.assert_allclose(r[c, 1], -mu[c] * 2 * H * U, rtol=1e-10)
+        np.testing.assert_allclose(r[c, 4], -mu[c] * 2 * H * (cp * (1.0 - Tw) + 0.5 * U * U), rtol=1e-10)
+    assert o.step() == 0
+
+
+def test_isothermal_wall_model_at_rest_conducts_like_the_isothermal_wall():
+    m = Mesh.channel(10, jitter=0.3, seed=6)
+    got = {}
+    for kind in ("WM_ALG_ISOTHERMAL", "WALL_ISOTHERMAL"):
+        o = oracle(m, deck({"y0": f"{kind} 1.1", "y1": f"{kind} 0.9"}), uniform(m))
+
+        def keep(rhs, time, stage, kind=kind):
+            if stage == 1 and kind not in got:
+                got[kind] = rhs.copy()
+        o.set_source_hook(keep)
+        assert o.step() == 0
+    a, b = got["WM_ALG_ISOTHERMAL"], got["WALL_ISOTHERMAL"]
+    assert np.abs(b[:, 4]).max() > 0.0
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
 
 
 def test_reference_deck_boundary_lines_parse():

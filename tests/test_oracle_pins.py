@@ -357,6 +357,51 @@ def test_wall_model_closed_forms():
     np.testing.assert_allclose(tau, mu * u1 / y1, rtol=1e-3)
 
 
+def _q_wall_cases(rng, n):
+    # friction Reynolds numbers u_tau rho y1 / mu from 0 through the branch
+    # point 8.28314 to ~1e5, wall hotter and colder than the cell
+    for k in range(n):
+        rho = rng.uniform(0.5, 2.0)
+        mu = 10.0 ** rng.uniform(-6, -3)
+        y1 = 10.0 ** rng.uniform(-4, -1)
+        re = 0.0 if k % 13 == 0 else 10.0 ** rng.uniform(-2, 5)
+        u_tau = re * mu / (rho * y1)
+        T1 = rng.uniform(250.0, 400.0)
+        yield T1, T1 + rng.uniform(-80.0, 80.0), rho * u_tau ** 2, rho, mu, mu / 0.7, 1004.5, y1
+
+
+@needs_ref
+def test_wall_heat_flux_bit_identical_to_reference():
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(37)
+    for case in _q_wall_cases(rng, 3000):
+        assert L.og_wm_q_wall(*case) == R.ref_wm_q_wall(*case), case
+
+
+def test_wall_heat_flux_closed_forms():
+    # no stress: Fourier conduction lambda (T1 - Tw) / y1 (lambda = loc cp);
+    # continuous where T* changes form (Re = y* = 8.28314); far out the thermal
+    # log law q = rho cp u_tau (T1 - Tw) / T+ with T+ = Pr_t/kappa ln Re + C_t,
+    # Pr_t = 0.85, kappa = 0.41, C_t = 3.9 (the laminar conductivity negligible)
+    L = orc.lib()
+    T1, Tw, rho, mu, cp, y1 = 320.0, 290.0, 1.1, 1.8e-5, 1004.5, 2e-3
+    loc = mu / 0.7
+    np.testing.assert_allclose(L.og_wm_q_wall(T1, Tw, 0.0, rho, mu, loc, cp, y1), loc * cp * (T1 - Tw) / y1,
+                               rtol=1e-15)
+    assert L.og_wm_q_wall(T1, T1, 0.3, rho, mu, loc, cp, y1) == 0.0
+    ystar = 8.28314
+
+    def q_at(re, locp=loc):
+        u_tau = re * mu / (rho * y1)
+        return L.og_wm_q_wall(T1, Tw, rho * u_tau ** 2, rho, mu, locp, cp, y1)
+    lo, hi = q_at(ystar), q_at(ystar * (1 + 1e-12))
+    assert abs(hi - lo) <= 1e-9 * abs(lo)
+    for re in (1e6, 1e8):
+        u_tau = re * mu / (rho * y1)
+        Tplus = 0.85 / 0.41 * math.log(re) + 3.9
+        np.testing.assert_allclose(q_at(re, 0.0), rho * cp * u_tau * (T1 - Tw) / Tplus, rtol=30 * ystar / re * math.log(re))
+
+
 @needs_ref
 def test_nscbc_inlet_rhs_bit_identical_to_reference():
     L, R = orc.lib(), orc.ref()
