@@ -96,6 +96,13 @@ typedef struct cg_solver cg_solver;
  *                                    and the Riemann flux against it (Nscbc,
  *                                    :3578-3693); SIGMA 0.2 and L 1 by default,
  *                                    CONSTANT_PRESSURE sets sigma -1
+ *   NSCBC_OUTLET_MDOT MDOT <mdot> T_RELAX <t>
+ *                                    NscbcOutletP with OUTLET_MDOT_WEAK (:3888-
+ *                                    3963): the NSCBC outlet faces with the
+ *                                    incoming wave of calcNscbcIgOutletMdotWeakRhs
+ *                                    (BcCommon.hpp:474-525), L1 = -c (rhou_avg -
+ *                                    mdot / area) / t from the zone's average
+ *                                    mass flux over its faces' states each stage
  *   NSCBC_MT <mdot> <T>              NscbcMt (:3718-3840): NSCBC inlet faces whose
  *                                    velocity and pressure follow each stage from
  *                                    their density, the zone's mass flux
@@ -148,15 +155,16 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_CBC_TOTAL_PT 11
 #define CG_BC_NSCBC_MT 12
 #define CG_BC_WM_ALG_ISOTHERMAL 13
+#define CG_BC_NSCBC_OUTLET_MDOT 14
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */
   double T;       /* wall temperature (WALL_ISOTHERMAL, WM_ALG_ISOTHERMAL), T of CBC_UPT / CBC_MPT, SPONGE RELAX_TIME,
-                     CBC_TOTAL_PT T0 */
+                     CBC_TOTAL_PT T0, NSCBC_OUTLET_MDOT T_RELAX */
   double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
   double p;       /* CBC pressure (CBC_TOTAL_PT: p0), NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */
   double rho;     /* CBC_RUNP / CBC_RUP density */
-  double q;       /* CBC_MPT / NSCBC_MT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
+  double q;       /* CBC_MPT / NSCBC_MT / NSCBC_OUTLET_MDOT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
   double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
   double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax */
   int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ */

@@ -149,6 +149,7 @@ struct HostInst {
   bool nscbc = false;                // some zone is an NSCBC outlet
   bool wm = false;                   // some zone is a WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL wall
   bool wm_iso = false;               // some zone is a WM_ALG_ISOTHERMAL wall
+  bool mdot = false;                 // some zone is an NSCBC_OUTLET_MDOT outlet
   bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
@@ -179,7 +180,7 @@ struct Instance : HostInst {
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
             prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
-            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0};
+            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0, mdot};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -926,7 +927,9 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
                        p->bc[k].type == CG_BC_CBC_TOTAL_PT || p->bc[k].type == CG_BC_NSCBC_MT;
       for (int isc = 0; isc < cgdev::kMaxScalars; ++isc)
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
-      hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P || p->bc[k].type == CG_BC_NSCBC_MT;
+      hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P || p->bc[k].type == CG_BC_NSCBC_MT ||
+                  p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT;
+      hi->mdot = hi->mdot || p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT;
       hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
@@ -996,6 +999,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     const size_t nbf = (size_t)lay->g.nbf;
     rc |= dev_alloc(s, h, &w.nscbc, 5 * nbf);
     rc |= dev_alloc(s, h, &w.nscbc_rhs, 15 * nbf);
+    if (hi->mdot) rc |= dev_alloc(s, h, &w.mdot, 3 * (size_t)cgdev::kMaxBcZones);  // set before each stage
     hi->snap_bc = 5 * nbf;
     hi->snap_n += hi->snap_bc;
   }
@@ -1162,10 +1166,11 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false, any_mdot = false;
   int max_nsc = 0, lds_max = 0;
   for (auto* I : E.inst) {
     any_bf = any_bf || I->lay->g.nbf > 0;
+    any_mdot = any_mdot || I->mdot;
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
@@ -1178,7 +1183,7 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   E.hole_blocks = 0;
   const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
   E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf};
+         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf, any_mdot};
   E.dirty = false;
   return 0;
 }
@@ -1875,6 +1880,24 @@ int cg_params_parse(cg_params* p, const char* text) {
           }
         }
         if (b.p < 0.0) return bad("NSCBC_OUTLET_PRESSURE needs P_REF");
+      } else if (type == "NSCBC_OUTLET_MDOT") {
+        // (:3888-3912): tokens MDOT (the zone's target mass flow) and T_RELAX
+        // (the relaxation time of its incoming wave), both required
+        b.type = CG_BC_NSCBC_OUTLET_MDOT;
+        b.q = -1.0;
+        b.T = -1.0;
+        for (size_t i = 2; i < tok.size(); ++i) {
+          const std::string opt = upper(tok[i]);
+          if (opt == "MDOT") {
+            if (!num(++i, &b.q)) return bad("expected MDOT <value>");
+          } else if (opt == "T_RELAX") {
+            if (!num(++i, &b.T)) return bad("expected T_RELAX <value>");
+          } else {
+            return bad("unknown NSCBC_OUTLET_MDOT option " + opt);
+          }
+        }
+        if (b.q < 0.0) return bad("NSCBC_OUTLET_MDOT needs MDOT");
+        if (!(b.T > 0.0)) return bad("NSCBC_OUTLET_MDOT needs a positive T_RELAX");
       } else if (type == "SPONGE") {
         // parseSpongeParam (SpongeCommon.hpp:11-63): RELAX_TIME, STRENGTH, TYPE,
         // LENGTH, EPS_P (unused), SPONGE_PRESSURE (P_REF by default)
@@ -1914,7 +1937,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
                    "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_ALG_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
                    "NSCBC_MT, NSCBC_OUTLET_P, "
-                   "NSCBC_OUTLET_PRESSURE and SPONGE are)");
+                   "NSCBC_OUTLET_PRESSURE, NSCBC_OUTLET_MDOT and SPONGE are)");
       }
       int slot = -1;
       for (int i = 0; i < p->nbc; ++i)

@@ -141,6 +141,9 @@ struct WorkT {
   // NSCBC outlet faces (null without them): per boundary face (device order)
   // its state rho, p, u[3] and the rates of the step's three stages [3][5]
   double *nscbc, *nscbc_rhs;
+  // NSCBC_OUTLET_MDOT zones (null without them): per zone the stage's sums over
+  // its faces' states of |n_bf|, rho_bc u_bc . n_bf and u_bc . n_bf [16][3]
+  double* mdot;
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
@@ -164,14 +167,17 @@ enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
   BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12,
-  BC_WM_ALG_ISOTHERMAL = 13
+  BC_WM_ALG_ISOTHERMAL = 13, BC_NSCBC_OUTLET_MDOT = 14
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
 __host__ __device__ inline bool bc_riemann(int type) {
-  return (type >= BC_CBC_UPT && type <= BC_SPONGE) || type == BC_CBC_TOTAL_PT || type == BC_NSCBC_MT;
+  return (type >= BC_CBC_UPT && type <= BC_SPONGE) || type == BC_CBC_TOTAL_PT || type == BC_NSCBC_MT ||
+         type == BC_NSCBC_OUTLET_MDOT;
 }
 // the zones whose faces carry an evolved NSCBC state
-__host__ __device__ inline bool bc_nscbc(int type) { return type == BC_NSCBC_OUTLET_P || type == BC_NSCBC_MT; }
+__host__ __device__ inline bool bc_nscbc(int type) {
+  return type == BC_NSCBC_OUTLET_P || type == BC_NSCBC_MT || type == BC_NSCBC_OUTLET_MDOT;
+}
 
 struct Params {
   double gamma, R_gas, gm1, Rgogm1, p_ref, inv_pref, inv_Tref, mu_ref, mu_power_law, invPr_lam;
@@ -241,6 +247,7 @@ struct Launch {
   int scalars;       // most passive scalars of an instance (0: no scalar pass after each stage's flux)
   size_t sc_lds;     // dynamic LDS bytes of the scalar pass's halo variant (0: global gathers)
   int boundary;      // some instance has boundary faces (the properties pass keeps mu_lam, loc_lam)
+  int mdot;          // some instance has NSCBC_OUTLET_MDOT zones (their sums precede each stage's flux pass)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

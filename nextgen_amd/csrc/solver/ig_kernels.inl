@@ -689,7 +689,11 @@ __device__ __forceinline__ void cbc_state(const Params& P, int z, int type, cons
 
 // calcNscbcIgOutletRhs (BcCommon.hpp:419-470), FP64 in its operation order:
 // rhs = d/dt (rho_bc, p_bc, u_bc[3]) of an NSCBC outlet face from its state
-// bc = rho, p, u[3] and its cell's cv = rho, p, u[3]
+// bc = rho, p, u[3] and its cell's cv = rho, p, u[3]. kMdot:
+// calcNscbcIgOutletMdotWeakRhs (BcCommon.hpp:474-525), the same rates with the
+// incoming wave L1 = -c (rhou_avg - rhou_target) / T_relax, the three last
+// arguments then being rhou_avg, rhou_target and T_relax
+template <bool kMdot = false>
 __device__ __forceinline__ void nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta,
                                                  const double bc[5], const double cv[5], double gamma,
                                                  double L_ref, double sigma, double p_ref) {
@@ -702,7 +706,9 @@ __device__ __forceinline__ void nscbc_outlet_rhs(double rhs[5], const double uni
   const double dudn = (un_bc - u0) / delta;
   const double dpdn = (p_bc - cv[1]) / delta;
   const double L5 = (un_bc + sos_bc) * (dpdn + rho_bc * sos_bc * dudn);
-  double L1 = sigma >= 0.0 ? sigma * (1.0 - Ma * Ma) * sos_bc / L_ref * (p_bc - p_ref) : -L5;
+  double L1;
+  if constexpr (kMdot) L1 = -sos_bc * (L_ref - sigma) / p_ref;
+  else L1 = sigma >= 0.0 ? sigma * (1.0 - Ma * Ma) * sos_bc / L_ref * (p_bc - p_ref) : -L5;
   const double drhodn = (rho_bc - cv[0]) / delta;
   const double L2 = un_bc * (sos_bc * sos_bc * drhodn - dpdn);
   if (Ma >= 1.0) L1 = sos_bc * (Ma - 1.0) * (dpdn - rho_bc * sos_bc * dudn);
@@ -1084,6 +1090,17 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
             double r[5] = {rr[0], rr[1], rr[2], rr[3], rr[4]};
             nscbc_inlet_rhs(r, unit_n, delta, qs, cv, P.gamma);
             rr[0] = r[0];
+          } else if (type == BC_NSCBC_OUTLET_MDOT) {
+            // NscbcOutletP::calcRhsMdot (IdealGasSolverBcs.cpp:3934-3963): the
+            // zone's average mass flux from the sums of its faces' states at
+            // the start of the stage (k_mdot_sums), the target mdot / area
+            const auto sm = G(w.mdot) + 3 * z;
+            const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
+            const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
+            double r[5];
+            nscbc_outlet_rhs<true>(r, unit_n, delta, qs, cv, P.gamma, sm[1] / sm[0], P.bc_q[z] / P.bc_area[z],
+                                   P.bc_T[z]);
+            for (int k = 0; k < 5; ++k) rr[k] = r[k];
           } else {
             const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
             const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
@@ -2606,6 +2623,45 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict
   if (threadIdx.x < 8) diag[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// NscbcOutletP::calcRhsMdot's zone sums (IdealGasSolverBcs.cpp:3937-3948):
+// for each NSCBC_OUTLET_MDOT zone of each solve, over its faces' states at the
+// start of the stage, the sums of |n_bf|, rho_bc u_bc . n_bf and u_bc . n_bf
+// into w.mdot[3 z ..]. One block per solve; each thread takes every kBlock-th
+// face and a fixed tree adds the threads' sums, so they repeat bit for bit.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_mdot_sums(const Inst<R>* __restrict__ I) {
+  #pragma clang fp contract(off)
+  const Inst<R>& X = I[blockIdx.x];
+  if (X.nblocks == 0 || !X.w.mdot) return;  // removed from the batch, or no such zone
+  __shared__ double sh[3][kBlock];
+  const GeomT<R>& g = X.g;
+  const int t = threadIdx.x;
+  for (int z = 0; z < kMaxBcZones; ++z) {
+    if (X.P.bc_type[z] != BC_NSCBC_OUTLET_MDOT) continue;  // the same for every thread
+    double a = 0.0, m = 0.0, v = 0.0;
+    for (int b = t; b < g.nbf; b += kBlock) {
+      if (G(g.bf_zone)[b] != z) continue;
+      const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
+      const auto q = G(X.w.nscbc) + 5 * (size_t)b;
+      const double un_dA = q[2] * n0 + q[3] * n1 + q[4] * n2;
+      a += sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      m += q[0] * un_dA;
+      v += un_dA;
+    }
+    sh[0][t] = a;
+    sh[1][t] = m;
+    sh[2][t] = v;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+      if (t < h)
+        for (int k = 0; k < 3; ++k) sh[k][t] += sh[k][t + h];
+      __syncthreads();
+    }
+    if (t < 3) G(X.w.mdot)[3 * z + t] = sh[t][0];
+    __syncthreads();
+  }
+}
+
 template <class T>
 __global__ void k_fill(T* p, T v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2654,6 +2710,10 @@ int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rh
       else hipLaunchKernelGGL((k_scalars<R, 3, false>), grid, block, 0, s, L.I, L.bmap, cur);
     }
     return check();
+  }
+  if (mode == 1 && L.mdot) {
+    hipLaunchKernelGGL(k_mdot_sums<R>, dim3(L.ninst), dim3(kBlock), 0, s, L.I);
+    if (int rc = check()) return rc;
   }
   if (L.lds) {
     auto go = [&](auto nb) {

@@ -1047,8 +1047,11 @@ int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, cons
   return 0;
 }
 
-double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
-                           double gamma, double L_ref, double sigma, double p_ref) {
+/* the rates of an NSCBC outlet face; mdot = 0: calcNscbcIgOutletRhs, a, b, c =
+ * L_ref, sigma, p_ref; mdot = 1: calcNscbcIgOutletMdotWeakRhs (BcCommon.hpp:
+ * 474-525), a, b, c = rhou_avg, rhou_target, T_relax */
+static double outlet_rates(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                           double gamma, int mdot, double a, double b, double c) {
   const double rho_bc = bc[0], p_bc = bc[1], *u_bc = bc + 2;
   const double rho_cv = cv[0], p_cv = cv[1], *u_cv = cv + 2;
   const double un_bc = smax(0.0, u_bc[0] * unit_n[0] + u_bc[1] * unit_n[1] + u_bc[2] * unit_n[2]);
@@ -1060,9 +1063,11 @@ double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, 
   const double L5 = (un_bc + sos_bc) * (dpdn + rho_bc * sos_bc * dudn);
   double dudn_[3];
   for (int i = 0; i < 3; ++i) dudn_[i] = (u_bc[i] - u_cv[i]) / delta;
-  /* the incoming acoustic characteristic; sigma < 0: perfectly reflecting */
+  /* the incoming acoustic characteristic: relaxing the pressure toward p_ref
+   * (sigma < 0: perfectly reflecting), or the zone's mass flux toward the target */
   double L1;
-  if (sigma >= 0.0) L1 = sigma * (1.0 - Ma * Ma) * sos_bc / L_ref * (p_bc - p_ref);
+  if (mdot) L1 = -sos_bc * (a - b) / c;
+  else if (b >= 0.0) L1 = b * (1.0 - Ma * Ma) * sos_bc / a * (p_bc - c);
   else L1 = -L5;
   /* un_bc >= 0, so Ma >= 0 (the reference asserts it) */
   const double drhodn = (rho_bc - rho_cv) / delta;
@@ -1076,14 +1081,39 @@ double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, 
   return un_bc;
 }
 
+double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                           double gamma, double L_ref, double sigma, double p_ref) {
+  return outlet_rates(rhs, unit_n, delta, bc, cv, gamma, 0, L_ref, sigma, p_ref);
+}
+
+double og_nscbc_mdot_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                         double gamma, double rhou_avg, double rhou_target, double T_relax) {
+  return outlet_rates(rhs, unit_n, delta, bc, cv, gamma, 1, rhou_avg, rhou_target, T_relax);
+}
+
+static int is_nscbc(int type) { return type == 8 || type == 12 || type == 14; }
+
 /* NscbcOutletP::calcRhsPressure (IdealGasSolverBcs.cpp:4009-4024): each NSCBC
- * face's rate of stage rk_stage from its state and its cell's rho, p, u */
+ * face's rate of stage rk_stage from its state and its cell's rho, p, u;
+ * NscbcOutletP::calcRhsMdot (:3934-3963) first sums |n_bf|, rho_bc u_bc . n_bf
+ * and u_bc . n_bf over each NSCBC_OUTLET_MDOT zone's faces, in face order */
 static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
   const og_mesh* m = &s->m;
   const og_params* P = &s->p;
+  double sums[16][3] = {{0.0}};
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
-    if (P->bc_type[z] != 8 && P->bc_type[z] != 12) continue;
+    if (P->bc_type[z] != 14) continue;
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const double* b = &s->nscbc[5 * (size_t)ibf];
+    const double un_dA = b[2] * n[0] + b[3] * n[1] + b[4] * n[2];
+    sums[z][0] += sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    sums[z][1] += b[0] * un_dA;
+    sums[z][2] += un_dA;
+  }
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (!is_nscbc(P->bc_type[z])) continue;
     const double* n = &m->n_bf[3 * (size_t)ibf];
     const double bf_area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const double unit_n[3] = {n[0] / bf_area, n[1] / bf_area, n[2] / bf_area};
@@ -1101,6 +1131,9 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
       for (int i = 0; i < 3; ++i) b[2 + i] = (rhoun_bc * unit_n[i] + b[0] * 0.0) / b[0];
       b[1] = b[0] * RT_bc;
       og_nscbc_inlet_rhs(rhs, unit_n, delta, b, cv, P->gamma);
+    } else if (P->bc_type[z] == 14) {
+      og_nscbc_mdot_rhs(rhs, unit_n, delta, b, cv, P->gamma, sums[z][1] / sums[z][0], P->bc_q[z] / P->bc_area[z],
+                        P->bc_T[z]);
     } else {
       og_nscbc_outlet_rhs(rhs, unit_n, delta, b, cv, P->gamma, P->bc_L[z], P->bc_sigma[z], P->bc_p[z]);
     }
@@ -1111,7 +1144,7 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
 static void nscbc_rk3(og_solver* s, int rk_stage) {
   const og_mesh* m = &s->m;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
-    if (s->p.bc_type[m->zone_bf[ibf]] != 8 && s->p.bc_type[m->zone_bf[ibf]] != 12) continue;
+    if (!is_nscbc(s->p.bc_type[m->zone_bf[ibf]])) continue;
     double* b = &s->nscbc[5 * (size_t)ibf];
     const double* r = &s->nscbc_rhs[15 * (size_t)ibf];
     for (int i = 0; i < rk_stage; ++i) {
@@ -1127,7 +1160,7 @@ static void nscbc_rk3(og_solver* s, int rk_stage) {
 static void nscbc_init(og_solver* s) {
   const og_mesh* m = &s->m;
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
-    if (s->p.bc_type[m->zone_bf[ibf]] != 8 && s->p.bc_type[m->zone_bf[ibf]] != 12) continue;
+    if (!is_nscbc(s->p.bc_type[m->zone_bf[ibf]])) continue;
     const int icv = m->cvobf[ibf];
     double* b = &s->nscbc[5 * (size_t)ibf];
     b[0] = s->rho[icv];
@@ -1282,7 +1315,7 @@ static void wm_calc(og_solver* s) {
   }
 }
 
-static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11 || type == 12; }
+static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11 || type == 12 || type == 14; }
 
 /* CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3449): un_bc = the
  * cell's normal velocity */
@@ -1451,7 +1484,7 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
         for (int i = 0; i < 3; ++i) b.u[i] = sp[1 + i];
         b.p = P->sponge_p;
         b.h = Rgogm1 * sp[0];
-      } else if (P->bc_type[z] == 8 || P->bc_type[z] == 12) {
+      } else if (is_nscbc(P->bc_type[z])) {
         const double* q = &s->nscbc[5 * (size_t)ibf];
         const double gogm1 = P->gamma / (P->gamma - 1.0);
         for (int i = 0; i < 3; ++i) b.u[i] = q[2 + i];

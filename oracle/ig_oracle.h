@@ -159,6 +159,11 @@ int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, cons
                        double gamma);
 double og_nscbc_outlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
                            double gamma, double L_ref, double sigma, double p_ref);
+/* calcNscbcIgOutletMdotWeakRhs (BcCommon.hpp:474-525): the outlet rates with
+ * the incoming wave L1 = -c (rhou_avg - rhou_target) / T_relax (14
+ * NSCBC_OUTLET_MDOT: MDOT in bc_q, T_RELAX in bc_T); returns un_bc */
+double og_nscbc_mdot_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],
+                         double gamma, double rhou_avg, double rhou_target, double T_relax);
 void og_flux_riemann(double flux[5], const double n_bf[3], const og_state* cv0, const og_state* cv1,
                      double gammaL, double gammaR);
 /* calcSvd33 on a column-major A: U, V column-major, sigma unsorted; returns the

@@ -114,6 +114,8 @@ def lib(path=None):
         L.og_nscbc_outlet_rhs.restype = C.c_double
         L.og_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                           C.c_double]
+        L.og_nscbc_mdot_rhs.restype = C.c_double
+        L.og_nscbc_mdot_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.og_nscbc_inlet_rhs.restype = C.c_int
         L.og_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         L.og_wm_solve_tau.restype = C.c_double
@@ -142,6 +144,8 @@ def ref():
         R.ref_nscbc_outlet_rhs.restype = C.c_double
         R.ref_nscbc_outlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double,
                                            C.c_double]
+        R.ref_nscbc_mdot_rhs.restype = C.c_double
+        R.ref_nscbc_mdot_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         R.ref_nscbc_inlet_rhs.restype = C.c_int
         R.ref_nscbc_inlet_rhs.argtypes = [dp, dp, C.c_double, dp, dp, C.c_double]
         R.ref_wm_solve_tau.restype = C.c_double

@@ -60,6 +60,13 @@ double ref_nscbc_outlet_rhs(double* rhs, const double* unit_n, double delta, con
                               p_ref);
 }
 
+// the same for an NSCBC_OUTLET_MDOT face (u_avg is not used by the rates)
+double ref_nscbc_mdot_rhs(double* rhs, const double* unit_n, double delta, const double* bc, const double* cv,
+                          double gamma, double rhou_avg, double rhou_target, double T_relax) {
+  return calcNscbcIgOutletMdotWeakRhs(rhs, unit_n, delta, bc[0], bc + 2, bc[1], cv[0], cv + 2, cv[1], gamma, rhou_avg,
+                                      0.0, rhou_target, T_relax);
+}
+
 // aux_out = u_avg[3], inv_v_avg, h_stag, p_avg
 void ref_flux_skew(double* flux, const double* nc, const double* cv0, const double* cv1, double* aux_out) {
   FaNC fa;

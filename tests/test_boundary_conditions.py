@@ -39,6 +39,10 @@ terms, pinned by closed forms.
   and the heat flux its first step's stage 1 sets is Fourier conduction to the
   wall, the stage-1 RHS of the same deck with WALL_ISOTHERMAL walls; the
   thermal law of the wall is pinned in test_oracle_pins.py.
+* NSCBC_OUTLET_MDOT (NscbcOutletP with OUTLET_MDOT_WEAK, IdealGasSolverBcs.cpp:
+  3888-3963): a uniform stream out through an outlet whose target is its own
+  mass flow keeps a zero RHS and stays put for five steps; a target above it
+  speeds up the outflow cells (the rates are pinned in test_oracle_pins.py).
 * Passive scalars through boundaries (_addScalarBoundaryFlux, BcCommon.hpp:276-291):
   at rest and scalar-free, a CBC inflow zone with value Z_in feeds the wall cells
   -mf Z_in (mf the Riemann flux's mass, negative inflow); a uniform stream and a
@@ -550,6 +554,58 @@ def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter):
     rho, u, rhoE = o.get_state()
     np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
     np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
+
+
+def test_nscbc_outlet_mdot_parser():
+    p = parse_params(deck({"y0": "SLIP", "y1": "NSCBC_OUTLET_MDOT MDOT 2.5 T_RELAX 0.3"}))
+    b = p.bc[1]
+    assert (b.type, b.q, b.T) == (14, 2.5, 0.3)
+    p = parse_params("y1 NSCBC_OUTLET_MDOT T_RELAX 1 MDOT 0\n")
+    assert (p.bc[0].q, p.bc[0].T) == (0.0, 1.0)
+    for bad in ("y0 NSCBC_OUTLET_MDOT MDOT 2.5", "y0 NSCBC_OUTLET_MDOT T_RELAX 0.3",
+                "y0 NSCBC_OUTLET_MDOT MDOT 2.5 T_RELAX 0", "y0 NSCBC_OUTLET_MDOT MDOT 2.5 T_RELAX 1 SIGMA 0.2"):
+        with pytest.raises(CharlesError):
+            parse_params(bad + "\n")
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_nscbc_outlet_mdot_keeps_a_matching_uniform_stream(jitter):
+    # a uniform stream V out through an NSCBC_OUTLET_MDOT zone whose target is
+    # the stream's own mass flow (rho V area): the faces start at their cells'
+    # state, the zone's average mass flux is the target, so the incoming wave
+    # and every other rate vanish; the RHS is zero and five steps keep the stream
+    m = Mesh.channel(10, jitter=jitter, seed=16)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    area = float(np.bincount(m.get("zone_bf"), weights=m.get("area_bf"))[1])
+    text = deck({"y0": f"CBC_UPT 0 {V} 0 {p!r} {T}", "y1": f"NSCBC_OUTLET_MDOT MDOT {1.0 * V * area!r} T_RELAX 0.2"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    r = o.calc_rhs()
+    assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
+    for _ in range(5):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    np.testing.assert_allclose(rho, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], V, rtol=1e-12)
+
+
+def test_nscbc_outlet_mdot_pulls_toward_its_target():
+    # the same stream with a target 20 % above its mass flow: the outlet faces'
+    # pressure falls (the incoming wave L1 = -c (rhou_avg - rhou_target) / T
+    # is positive), so the cells at the outlet accelerate and the outflow grows
+    m = Mesh.channel(10, jitter=0.0)
+    R, T = 71.42857142857143, 1.0
+    p = R * T
+    V = 3.0
+    area = float(np.bincount(m.get("zone_bf"), weights=m.get("area_bf"))[1])
+    text = deck({"y0": f"CBC_UPT 0 {V} 0 {p!r} {T}", "y1": f"NSCBC_OUTLET_MDOT MDOT {1.2 * V * area!r} T_RELAX 0.2"})
+    o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
+    for _ in range(10):
+        assert o.step() == 0
+    rho, u, rhoE = o.get_state()
+    top = wall_cells(m, 1)
+    assert np.all(rho[top] * u[top, 1] > 1.0 * V)
 
 
 def test_nscbc_mt_parser():

@@ -4,7 +4,8 @@ test_boundary_conditions.py, its Riemann flux against the reference's
 calcRiemannFlux in test_oracle_pins.py): SLIP / SYMMETRY, WALL_ADIABATIC (with
 a TRANSLATE wall velocity), WALL_ISOTHERMAL, the characteristic conditions
 CBC_UPT / CBC_MPT / CBC_RUNP / CBC_RUP, the NSCBC pressure outlet (per-face
-states evolved with the step, restored with it after a NaN), the algebraic
+states evolved with the step, restored with it after a NaN) and mass-flow
+outlet NSCBC_OUTLET_MDOT (its zone sums before each stage, also in a batch), the algebraic
 wall model WM_ALG_ADIABATIC (per-face stress from the law of the wall) and
 its isothermal form WM_ALG_ISOTHERMAL (with the thermal law's heat flux), the
 total-pressure inlet CBC_TOTAL_PT, the NSCBC mass-flow inlet NSCBC_MT and the SPONGE
@@ -55,6 +56,10 @@ WALLS = {
     # in through an NSCBC mass-flow inlet (0.3 c0 / 10 into the channel), out through an NSCBC outlet
     "nscbc_mt+nscbc": {"y0": f"NSCBC_MT {0.3 * (2 * np.pi) ** 2!r} 1.0",
                        "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    # in through a CBC, out through an NSCBC outlet driving the zone's mass flow
+    # toward a target above the inflow's
+    "cbc_upt+mdot": {"y0": f"CBC_UPT 0.1 0.4 0 {P0!r} 1.0",
+                     "y1": f"NSCBC_OUTLET_MDOT MDOT {0.5 * (2 * np.pi) ** 2!r} T_RELAX 0.5"},
     # the algebraic wall model on both walls, one of them moving
     "wm+wm_translate": {"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.3 0 0.1"},
     # the isothermal wall model, one wall hotter and one colder than the gas
@@ -219,7 +224,7 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
 
 
 @pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate", "nscbc_mt+nscbc",
-                                   "wmiso+wmiso"])
+                                   "wmiso+wmiso", "cbc_upt+mdot"])
 def test_boundary_states_roll_back_with_the_flow(walls):
     # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
     # faces' or the sponge's states with it (storeData / restoreData of the
@@ -252,7 +257,8 @@ def test_boundary_states_roll_back_with_the_flow(walls):
 
 
 @pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge",
-                                   "wm+wm_translate", "tpt+nscbc", "nscbc_mt+nscbc", "wmiso+wmiso"])
+                                   "wm+wm_translate", "tpt+nscbc", "nscbc_mt+nscbc", "wmiso+wmiso",
+                                   "cbc_upt+mdot"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]
@@ -327,14 +333,15 @@ def test_stepped_boundary_states_refuse_a_source_hook(walls, what):
     g.close()
 
 
+@pytest.mark.parametrize("walls", ["slip+translate", "cbc_upt+mdot"])
 @pytest.mark.parametrize("prec", ["FP64", "MIXED"])
-def test_channel_batched_with_periodic_solve_equals_solo(prec):
+def test_channel_batched_with_periodic_solve_equals_solo(prec, walls):
     from nextgen_amd.batch import Batch
     from nextgen_amd.ideal_gas import CG_PREC_FP64, CG_PREC_MIXED
     precision = CG_PREC_MIXED if prec == "MIXED" else CG_PREC_FP64
     mc = Mesh.channel(12, jitter=0.3, seed=9)
     mp = Mesh.jittered(10, 0.3, seed=10)
-    cases = [(mc, deck(WALLS["slip+translate"], prec=prec), channel_state(mc)),
+    cases = [(mc, deck(WALLS[walls], prec=prec), channel_state(mc)),
              (mp, tgv_params(sgs="VREMAN") + f"PRECISION {prec}\n", mp.ic_tgv(0.1, 1.4))]
     b = Batch(precision)
     ids = [b.add(m, parse_params(t), *st) for m, t, st in cases]

@@ -294,6 +294,63 @@ def test_nscbc_outlet_rhs_bit_identical_to_reference():
         np.testing.assert_array_equal(a, b)
 
 
+@needs_ref
+def test_nscbc_mdot_rhs_bit_identical_to_reference():
+    # calcNscbcIgOutletMdotWeakRhs (BcCommon.hpp:474-525) on random faces:
+    # subsonic and supersonic outflow, inflow clipped to un = 0, zone mass
+    # fluxes below, at and above the target
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(41)
+    for k in range(3000):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        bc = np.array([rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0), *rng.normal(scale=0.8, size=3)])
+        cv = bc * (1.0 + 0.05 * rng.normal(size=5))
+        if k % 5 == 0:
+            bc[2:] = 2.5 * n
+        target = rng.uniform(0.1, 2.0)
+        avg = target if k % 11 == 0 else target * rng.uniform(0.5, 1.5)
+        a, b = np.zeros(5), np.zeros(5)
+        T_relax = 0.37 + 0.1 * (k % 9)
+        ua = L.og_nscbc_mdot_rhs(dp(a), dp(n), 0.03, dp(bc), dp(cv), 1.4, avg, target, T_relax)
+        ub = R.ref_nscbc_mdot_rhs(dp(b), dp(n), 0.03, dp(bc), dp(cv), 1.4, avg, target, T_relax)
+        assert ua == ub
+        np.testing.assert_array_equal(a, b)
+
+
+def test_nscbc_mdot_rhs_closed_forms():
+    # at rest with the face state equal to its cell's only the incoming wave
+    # acts, L1 = -c (rhou_avg - rhou_target) / T_relax: a zone passing less
+    # than its target lowers the face pressure (dp/dt = -L1/2 < 0) and pulls
+    # the face along n; at the target every rate vanishes. On a moving
+    # subsonic face the rates equal the pressure outlet's whose L1 is the same
+    # (sigma (1 - Ma^2) c / L (p - p_ref) = -c (rhou_avg - rhou_target) / T)
+    L = orc.lib()
+    n = np.array([0.0, 0.6, 0.8])
+    gamma, rho, p, T = 1.4, 1.2, 1.05, 0.5
+    c = math.sqrt(gamma * p / rho)
+    bc = np.array([rho, p, 0.0, 0.0, 0.0])
+    r = np.zeros(5)
+    L.og_nscbc_mdot_rhs(dp(r), dp(n), 0.1, dp(bc), dp(bc.copy()), gamma, 0.8, 1.0, T)
+    L1 = -c * (0.8 - 1.0) / T
+    assert L1 > 0 and r[1] < 0
+    np.testing.assert_allclose(r[1], -0.5 * L1, rtol=1e-14)
+    np.testing.assert_allclose(r[0], -0.5 * L1 / c ** 2, rtol=1e-14)
+    np.testing.assert_allclose(r[2:], 0.5 * L1 * n / c / rho, rtol=1e-14, atol=1e-300)
+    L.og_nscbc_mdot_rhs(dp(r), dp(n), 0.1, dp(bc), dp(bc.copy()), gamma, 1.0, 1.0, T)
+    assert np.all(r == 0.0)
+    bc = np.array([rho, p, 0.1, 0.2, 0.3])
+    cv = bc * np.array([1.01, 0.98, 1.1, 0.9, 1.05])
+    un = bc[2:] @ n
+    Ma = un / c
+    avg, tgt, sigma, Lr = 0.9, 1.1, 0.4, 0.7
+    p_ref = p + (avg - tgt) * Lr / (T * sigma * (1.0 - Ma * Ma))
+    rm, rp = np.zeros(5), np.zeros(5)
+    L.og_nscbc_mdot_rhs(dp(rm), dp(n), 0.1, dp(bc), dp(cv), gamma, avg, tgt, T)
+    L.og_nscbc_outlet_rhs(dp(rp), dp(n), 0.1, dp(bc), dp(cv), gamma, Lr, sigma, p_ref)
+    np.testing.assert_allclose(rm, rp, rtol=1e-12, atol=1e-12 * np.abs(rp).max())
+
+
 def test_nscbc_outlet_rhs_closed_forms():
     # at rest with the face state equal to its cell's, only the incoming
     # characteristic acts: L1 = sigma c (p_bc - p_ref) / L, so dp_bc/dt =
