@@ -81,12 +81,14 @@ typedef struct cg_solver cg_solver;
  *   CBC_RUNP <rho> <un> <p>           (IdealGasSolverFlux.hpp:218-306):
  *   CBC_RUP <rho> <u> <v> <w> <p>     Cbc::addBoundaryFlux (:2960-2976),
  *                                    CbcUpt/Mpt/Runp/Rup::initData (:3182-3300)
- *   CBC_TOTAL_PT <p0> <T0> <t_relax> [STATIONARY_FRAME]
+ *   CBC_TOTAL_PT <p0> <T0> <t_relax> [USE_AVG] [STATIONARY_FRAME]
  *                                    CbcTotalPt (:3372-3558): per face a normal
  *                                    velocity un_bc (the cell's at
  *                                    initComplete, relaxed toward it once per
- *                                    step) and the isentropic state of total
- *                                    p0, T0 at that speed; USE_AVG is refused
+ *                                    step; USE_AVG: the zone's rho-weighted
+ *                                    average of u . n instead) and the
+ *                                    isentropic state of total p0, T0 at that
+ *                                    speed
  *   NSCBC_OUTLET_P <p_ref> <sigma> <L>
  *   NSCBC_OUTLET_PRESSURE P_REF <p> [SIGMA <s>] [L <l>] [CONSTANT_PRESSURE]
  *                                    NscbcOutletP (:3846-3932, 4009-4068): per
@@ -108,7 +110,11 @@ typedef struct cg_solver cg_solver;
  *                                    their density, the zone's mass flux
  *                                    -mdot / area and T; the density evolves
  *                                    by calcNscbcIgSubsonicInletRhs
- *                                    (BcCommon.hpp:311-337); T_TOTAL is refused
+ *                                    (BcCommon.hpp:311-337)
+ *   NSCBC_MT <mdot> T_TOTAL <T0>     the same with the static T of each stage
+ *                                    T0 / (1 + (gamma-1)/2 <Ma^2>), <Ma^2> the
+ *                                    zone's area-weighted average of
+ *                                    (rhoun_bc / rho / c)^2 over its cells
  *   SPONGE TYPE L_X|L_Y|L_Z|L_RX|L_RZ LENGTH <l> STRENGTH <s> RELAX_TIME <t>
  *          [SPONGE_PRESSURE <p>] [EPS_P <e>]
  *                                    Sponge (:4080-4346, SpongeCommon.hpp:11-63):
@@ -167,7 +173,8 @@ typedef struct cg_bc {
   double q;       /* CBC_MPT / NSCBC_MT / NSCBC_OUTLET_MDOT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
   double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
   double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax */
-  int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ */
+  int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ; CBC_TOTAL_PT 1: USE_AVG;
+                     NSCBC_MT 1: T_TOTAL (T is then the total temperature) */
   /* passive scalars entering through a CBC zone (parseScalarBc, BcCommon.hpp:213-274):
    * the value of each registered scalar (by its index in cg_params.sc_names), given on
    * the zone's line as "<name> <value>" pairs, every scalar required; an NSCBC

@@ -149,7 +149,7 @@ struct HostInst {
   bool nscbc = false;                // some zone is an NSCBC outlet
   bool wm = false;                   // some zone is a WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL wall
   bool wm_iso = false;               // some zone is a WM_ALG_ISOTHERMAL wall
-  bool mdot = false;                 // some zone is an NSCBC_OUTLET_MDOT outlet
+  bool zsum = false;                 // some zone needs zone sums (NSCBC_OUTLET_MDOT, USE_AVG, T_TOTAL)
   bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
   int64_t store_step = 0, store_step_recent = 0;
@@ -180,7 +180,7 @@ struct Instance : HostInst {
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
             prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
-            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0, mdot};
+            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0, zsum};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -595,7 +595,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
   Lay->zone_area.assign(Lay->bf_zone_names.size(), 0.0);
   for (int b = 0; b < o.nbf; ++b) Lay->zone_area[(size_t)o.zone_bf[b]] += o.area_bf[b];
   if (o.nbf > 0) {
-    std::vector<int> bptr(ncv + 1, 0), bzone(o.nbf);
+    std::vector<int> bptr(ncv + 1, 0), bzone(o.nbf), bcell(o.nbf);
     std::vector<double> bn[3], baod(o.nbf);
     for (auto& v : bn) v.resize(o.nbf);
     int k = 0;
@@ -604,6 +604,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
       for (int boc = o.bfocv_i[e]; boc < o.bfocv_i[e + 1]; ++boc, ++k) {
         const int b = o.bfocv_v[boc];
         bzone[k] = o.zone_bf[b];
+        bcell[k] = c;
         for (int d = 0; d < 3; ++d) bn[d][k] = o.n_bf[3 * size_t(b) + d];
         baod[k] = o.aod_bf[b];
       }
@@ -611,6 +612,7 @@ int build_layout(Err* s, Layout<R>* Lay, const cg_mesh* m, bool allow_lds) {
     }
     rc |= upload(s, h, &g.bf_ptr, bptr);
     rc |= upload(s, h, &g.bf_zone, bzone);
+    rc |= upload(s, h, &g.bf_cell, bcell);
     rc |= upload(s, h, &g.bfn0, bn[0]);
     rc |= upload(s, h, &g.bfn1, bn[1]);
     rc |= upload(s, h, &g.bfn2, bn[2]);
@@ -929,7 +931,8 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
         hi->P.bc_sc[z][isc] = cbc && isc < p->nsc ? p->bc[k].sc[isc] : 0.0;
       hi->nscbc = hi->nscbc || p->bc[k].type == CG_BC_NSCBC_OUTLET_P || p->bc[k].type == CG_BC_NSCBC_MT ||
                   p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT;
-      hi->mdot = hi->mdot || p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT;
+      hi->P.bc_mode[z] = (p->bc[k].type == CG_BC_CBC_TOTAL_PT || p->bc[k].type == CG_BC_NSCBC_MT) ? p->bc[k].mode : 0;
+      hi->zsum = hi->zsum || p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT || hi->P.bc_mode[z] != 0;
       hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
@@ -999,11 +1002,11 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     const size_t nbf = (size_t)lay->g.nbf;
     rc |= dev_alloc(s, h, &w.nscbc, 5 * nbf);
     rc |= dev_alloc(s, h, &w.nscbc_rhs, 15 * nbf);
-    if (hi->mdot) rc |= dev_alloc(s, h, &w.mdot, 3 * (size_t)cgdev::kMaxBcZones);  // set before each stage
     hi->snap_bc = 5 * nbf;
     hi->snap_n += hi->snap_bc;
   }
   if (hi->tpt) rc |= dev_alloc(s, h, &w.tpt_un, (size_t)lay->g.nbf);  // not snapshotted (no storeData)
+  if (hi->zsum) rc |= dev_alloc(s, h, &w.zsum, 3 * (size_t)cgdev::kMaxBcZones);  // set before its use
   if (hi->wm) {
     // WmAdiabatic::storeData, WmIsothermal::storeData: the faces' tau_wall
     // (in the snapshots too); q_wall is not stored
@@ -1166,11 +1169,11 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (nb) HIPCHK(s, hipMemcpy(E.bmap, bmap.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
-  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false, any_mdot = false;
+  bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false, any_zsum = false;
   int max_nsc = 0, lds_max = 0;
   for (auto* I : E.inst) {
     any_bf = any_bf || I->lay->g.nbf > 0;
-    any_mdot = any_mdot || I->mdot;
+    any_zsum = any_zsum || I->zsum;
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
@@ -1183,7 +1186,7 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   E.hole_blocks = 0;
   const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
   E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf, any_mdot};
+         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf, any_zsum};
   E.dirty = false;
   return 0;
 }
@@ -1281,6 +1284,9 @@ int nscbc_init(Err* s, Instance<R>* I, int cur, hipStream_t stream) {
   const bool sponge = I->sponge && !I->sponge_kept;
   I->sponge_kept = false;
   if (!I->nscbc && !sponge && !I->wm && !I->tpt) return 0;
+  // the zone sums the initial hooks use (CBC_TOTAL_PT USE_AVG)
+  if (I->zsum)
+    if (const int le = cgk::launch_zone_sums(I->solo_launch(), cur, 0, stream)) return launch_fail(s, "zone sums", le);
   if (const int le = cgk::launch_nscbc_init(I->solo_launch(), cur, sponge, stream))
     return launch_fail(s, "boundary state init", le);
   return 0;
@@ -1824,10 +1830,14 @@ int cg_params_parse(cg_params* p, const char* text) {
       } else if (type == "NSCBC_MT") {
         // NscbcMt::initData (IdealGasSolverBcs.cpp:3718-3759): mass flow and static T
         b.type = CG_BC_NSCBC_MT;
-        if (tok.size() >= 3 && upper(tok[2]) == "T_TOTAL")
-          return bad("NSCBC_MT T_TOTAL (a zone-averaged Mach number per stage) is not available");
-        if (tok.size() != 4 || !num(2, &b.q) || !num(3, &b.T) || !(b.T > 0.0))
-          return bad("expected NSCBC_MT <mdot> <T>");
+        // or T_TOTAL <T0>: the static T from the total temperature at the zone's
+        // area-averaged Mach number squared of each stage (mode 1)
+        if (tok.size() == 5 && upper(tok[3]) == "T_TOTAL") {
+          b.mode = 1;
+          if (!num(2, &b.q) || !num(4, &b.T) || !(b.T > 0.0)) return bad("expected NSCBC_MT <mdot> T_TOTAL <T0>");
+        } else if (tok.size() != 4 || !num(2, &b.q) || !num(3, &b.T) || !(b.T > 0.0)) {
+          return bad("expected NSCBC_MT <mdot> <T> or NSCBC_MT <mdot> T_TOTAL <T0>");
+        }
       } else if (type == "CBC_TOTAL_PT") {
         // CbcTotalPt::initData (IdealGasSolverBcs.cpp:3372-3401)
         b.type = CG_BC_CBC_TOTAL_PT;
@@ -1836,8 +1846,8 @@ int cg_params_parse(cg_params* p, const char* text) {
           return bad("expected CBC_TOTAL_PT <p0> <T0> <t_relax>");
         for (size_t i = 5; i < tok.size(); ++i) {
           const std::string opt = upper(tok[i]);
-          if (opt == "USE_AVG") return bad("CBC_TOTAL_PT USE_AVG (a zone-averaged inflow speed) is not available");
-          if (opt != "STATIONARY_FRAME") return bad("unknown CBC_TOTAL_PT option " + opt);
+          if (opt == "USE_AVG") b.mode = 1;  // the zone's mass-weighted average normal speed
+          else if (opt != "STATIONARY_FRAME") return bad("unknown CBC_TOTAL_PT option " + opt);
         }
       } else if (type == "CBC_MPT") {
         b.type = CG_BC_CBC_MPT;

@@ -99,6 +99,7 @@ struct GeomT {
   // area vector and area_over_delta_bf (FP64 in either precision)
   int nbf;
   const int *bf_ptr, *bf_zone;
+  const int* bf_cell;  // each face's cell (device order)
   const double *bfn0, *bfn1, *bfn2, *bf_aod;
 };
 
@@ -141,9 +142,12 @@ struct WorkT {
   // NSCBC outlet faces (null without them): per boundary face (device order)
   // its state rho, p, u[3] and the rates of the step's three stages [3][5]
   double *nscbc, *nscbc_rhs;
-  // NSCBC_OUTLET_MDOT zones (null without them): per zone the stage's sums over
-  // its faces' states of |n_bf|, rho_bc u_bc . n_bf and u_bc . n_bf [16][3]
-  double* mdot;
+  // zones whose condition needs sums over the zone (null without them), per
+  // zone [16][3], set by k_zone_sums before a stage: NSCBC_OUTLET_MDOT the sums
+  // over its faces' states of |n_bf|, rho_bc u_bc . n_bf, u_bc . n_bf; CBC_TOTAL_PT
+  // USE_AVG over its cells of rho |n_bf|, rho u . n_bf (at init and stage 1);
+  // NSCBC_MT T_TOTAL of |n_bf| Ma^2 and |n_bf|, Ma = (rhoun_bc / rho) / c
+  double* zsum;
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
@@ -200,6 +204,7 @@ struct Params {
   int sponge_zone;                 // the SPONGE zone, or -1
   double sponge_p, sponge_trelax;  // its pressure and relaxation time
   double bc_sc[kMaxBcZones][kMaxScalars];  // scalars entering through a zone (CBC values, 0 otherwise)
+  int bc_mode[kMaxBcZones];  // CBC_TOTAL_PT 1: USE_AVG; NSCBC_MT 1: T_TOTAL (bc_T then the total temperature)
 };
 
 using Geom = GeomT<double>;
@@ -247,7 +252,7 @@ struct Launch {
   int scalars;       // most passive scalars of an instance (0: no scalar pass after each stage's flux)
   size_t sc_lds;     // dynamic LDS bytes of the scalar pass's halo variant (0: global gathers)
   int boundary;      // some instance has boundary faces (the properties pass keeps mu_lam, loc_lam)
-  int mdot;          // some instance has NSCBC_OUTLET_MDOT zones (their sums precede each stage's flux pass)
+  int zsum;          // some instance has zones needing zone sums (k_zone_sums precedes each stage's flux pass)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);
@@ -283,6 +288,10 @@ int launch_begin_step(const Launch<R>& L, void* stream);
 // and (sponge true) the sponge's relaxed state from their cells
 template <class R>
 int launch_nscbc_init(const Launch<R>& L, int cur, bool sponge, void* stream);
+// the zone sums of stage `stage` (0: those of the initial hooks) into each
+// instance's zsum; the flux pass of a step launches it itself
+template <class R>
+int launch_zone_sums(const Launch<R>& L, int cur, int stage, void* stream);
 // FP64 primitives p, T, sos, h, sp_vol of one instance's state into out[5][ncv]
 template <class R>
 int launch_prims(const Launch<R>& L, int inst, int ncv, int cur, double* out, void* stream);

@@ -1082,7 +1082,14 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
             // velocity (mass flux -mdot / area along n) and pressure (rho R T)
             // from its density, then the density rate
             const double rhoun_bc = -P.bc_q[z] / P.bc_area[z];
-            const double RT_bc = P.bc_T[z] * P.R_gas;
+            double T_bc = P.bc_T[z];
+            if (P.bc_mode[z]) {
+              // T_TOTAL: the static T of the total temperature at the zone's
+              // area-averaged Mach number squared of the stage (k_zone_sums)
+              const auto sm = G(w.zsum) + 3 * z;
+              T_bc = P.bc_T[z] / (1.0 + 0.5 * (P.gamma - 1.0) * (sm[0] / sm[1]));
+            }
+            const double RT_bc = T_bc * P.R_gas;
             for (int i = 0; i < 3; ++i) q[2 + i] = (rhoun_bc * unit_n[i] + q[0] * 0.0) / q[0];
             q[1] = q[0] * RT_bc;
             const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
@@ -1093,8 +1100,8 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
           } else if (type == BC_NSCBC_OUTLET_MDOT) {
             // NscbcOutletP::calcRhsMdot (IdealGasSolverBcs.cpp:3934-3963): the
             // zone's average mass flux from the sums of its faces' states at
-            // the start of the stage (k_mdot_sums), the target mdot / area
-            const auto sm = G(w.mdot) + 3 * z;
+            // the start of the stage (k_zone_sums), the target mdot / area
+            const auto sm = G(w.zsum) + 3 * z;
             const double qs[5] = {q[0], q[1], q[2], q[3], q[4]};
             const double cv[5] = {rho_a, pa, ua[0], ua[1], ua[2]};
             double r[5];
@@ -1138,12 +1145,20 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       if (w.mf_bf) G(w.mf_bf)[b] = F[0];  // mf[ibf] = flux.rho, for the scalar pass
       if (type == BC_CBC_TOTAL_PT && update && stage == 1) {
         // CbcTotalPt::rk3Step (IdealGasSolverBcs.cpp:3521-3558): after the
-        // stage-1 RHS, un_bc relaxes toward the cell's normal velocity
+        // stage-1 RHS, un_bc relaxes toward the cell's normal velocity, or with
+        // USE_AVG toward the zone's mass-weighted average (k_zone_sums; every
+        // face then holds the zone's un_bc_avg)
         #pragma clang fp contract(off)
         const double area = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
         const double unit_n[3] = {nd[0] / area, nd[1] / area, nd[2] / area};
         const double eps = 1.0 / (P.bc_L[z] / G(w.info)->dt + 1.0);
-        const double un0 = ua[0] * unit_n[0] + ua[1] * unit_n[1] + ua[2] * unit_n[2];
+        double un0;
+        if (P.bc_mode[z]) {
+          const auto sm = G(w.zsum) + 3 * z;
+          un0 = sm[1] / sm[0];
+        } else {
+          un0 = ua[0] * unit_n[0] + ua[1] * unit_n[1] + ua[2] * unit_n[2];
+        }
         const auto un = G(w.tpt_un) + b;
         *un = (1.0 - eps) * *un + eps * un0;
       }
@@ -2423,7 +2438,13 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
       const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
       const double area = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
       const double unit_n[3] = {n0 / area, n1 / area, n2 / area};
-      G(X.w.tpt_un)[b] = unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
+      if (X.P.bc_mode[z]) {
+        // USE_AVG: the zone's mass-weighted normal velocity (k_zone_sums at stage 0)
+        const auto sm = G(X.w.zsum) + 3 * z;
+        G(X.w.tpt_un)[b] = sm[1] / sm[0];
+      } else {
+        G(X.w.tpt_un)[b] = unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
+      }
       continue;
     }
     if (X.P.bc_type[z] == BC_WM_ALG_ADIABATIC) {
@@ -2623,30 +2644,59 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict
   if (threadIdx.x < 8) diag[threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// NscbcOutletP::calcRhsMdot's zone sums (IdealGasSolverBcs.cpp:3937-3948):
-// for each NSCBC_OUTLET_MDOT zone of each solve, over its faces' states at the
-// start of the stage, the sums of |n_bf|, rho_bc u_bc . n_bf and u_bc . n_bf
-// into w.mdot[3 z ..]. One block per solve; each thread takes every kBlock-th
-// face and a fixed tree adds the threads' sums, so they repeat bit for bit.
+// The sums over a zone that some conditions need before a stage, for each
+// zone of each solve into w.zsum[3 z ..]: NSCBC_OUTLET_MDOT at stages 1-3
+// (NscbcOutletP::calcRhsMdot, IdealGasSolverBcs.cpp:3937-3948), over its faces'
+// states: |n_bf|, rho_bc u_bc . n_bf, u_bc . n_bf; CBC_TOTAL_PT USE_AVG at
+// stage 0 (initialHook, :3452-3460) and 1 (rk3Step, :3531-3541), over its
+// cells: rho |n_bf|, rho u . n_bf; NSCBC_MT T_TOTAL at stages 1-3 (calcRhs,
+// :3761-3775), over its cells: |n_bf| Ma^2, |n_bf| with Ma = (rhoun_bc / rho)
+// / c, rhoun_bc = -mdot / area. One block per solve; each thread takes every
+// kBlock-th face and a fixed tree adds the threads' sums, so they repeat bit
+// for bit. The cells are read from the stage's state st[cur].
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_mdot_sums(const Inst<R>* __restrict__ I) {
+__global__ __launch_bounds__(kBlock) void k_zone_sums(const Inst<R>* __restrict__ I, int cur, int stage) {
   #pragma clang fp contract(off)
   const Inst<R>& X = I[blockIdx.x];
-  if (X.nblocks == 0 || !X.w.mdot) return;  // removed from the batch, or no such zone
+  if (X.nblocks == 0 || !X.w.zsum) return;  // removed from the batch, or no such zone
   __shared__ double sh[3][kBlock];
   const GeomT<R>& g = X.g;
+  const Params& P = X.P;
   const int t = threadIdx.x;
   for (int z = 0; z < kMaxBcZones; ++z) {
-    if (X.P.bc_type[z] != BC_NSCBC_OUTLET_MDOT) continue;  // the same for every thread
+    // the kind of sum zone z needs at this stage (the same for every thread)
+    const int type = P.bc_type[z];
+    int kind = 0;
+    if (type == BC_NSCBC_OUTLET_MDOT && stage >= 1) kind = 1;
+    else if (type == BC_CBC_TOTAL_PT && P.bc_mode[z] && stage <= 1) kind = 2;
+    else if (type == BC_NSCBC_MT && P.bc_mode[z] && stage >= 1) kind = 3;
+    if (kind == 0) continue;
     double a = 0.0, m = 0.0, v = 0.0;
     for (int b = t; b < g.nbf; b += kBlock) {
       if (G(g.bf_zone)[b] != z) continue;
       const double n0 = G(g.bfn0)[b], n1 = G(g.bfn1)[b], n2 = G(g.bfn2)[b];
-      const auto q = G(X.w.nscbc) + 5 * (size_t)b;
-      const double un_dA = q[2] * n0 + q[3] * n1 + q[4] * n2;
-      a += sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-      m += q[0] * un_dA;
-      v += un_dA;
+      const double area = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      if (kind == 1) {
+        const auto q = G(X.w.nscbc) + 5 * (size_t)b;
+        const double un_dA = q[2] * n0 + q[3] * n1 + q[4] * n2;
+        a += area;
+        m += q[0] * un_dA;
+        v += un_dA;
+      } else {
+        const int c = G(g.bf_cell)[b];
+        double rho, u[3], rhoE, spv, p, h, T;
+        const Light<R> self = load_light(X.st[cur].light, c);
+        cell_state64(g, X.st[cur], P, c, self, rho, u, rhoE, spv, p, h, T);
+        if (kind == 2) {
+          a += rho * area;
+          m += rho * (u[0] * n0 + u[1] * n1 + u[2] * n2);
+        } else {
+          const double un_bc = -P.bc_q[z] / P.bc_area[z] / rho;
+          const double Ma = un_bc / sqrt(P.gamma * P.R_gas * T);
+          a += area * Ma * Ma;
+          m += area;
+        }
+      }
     }
     sh[0][t] = a;
     sh[1][t] = m;
@@ -2657,7 +2707,8 @@ __global__ __launch_bounds__(kBlock) void k_mdot_sums(const Inst<R>* __restrict_
         for (int k = 0; k < 3; ++k) sh[k][t] += sh[k][t + h];
       __syncthreads();
     }
-    if (t < 3) G(X.w.mdot)[3 * z + t] = sh[t][0];
+    // MDOT: |n| sum, rho un sum, un sum; USE_AVG: rho |n| sum, rho un sum; T_TOTAL: |n| Ma^2 sum, |n| sum
+    if (t < 3) G(X.w.zsum)[3 * z + t] = sh[t][0];
     __syncthreads();
   }
 }
@@ -2711,8 +2762,8 @@ int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rh
     }
     return check();
   }
-  if (mode == 1 && L.mdot) {
-    hipLaunchKernelGGL(k_mdot_sums<R>, dim3(L.ninst), dim3(kBlock), 0, s, L.I);
+  if (mode == 1 && L.zsum) {
+    hipLaunchKernelGGL(k_zone_sums<R>, dim3(L.ninst), dim3(kBlock), 0, s, L.I, cur, stage);
     if (int rc = check()) return rc;
   }
   if (L.lds) {
@@ -2791,6 +2842,12 @@ template <class R>
 int launch_nscbc_init(const Launch<R>& L, int cur, bool sponge, void* stream) {
   hipLaunchKernelGGL(k_nscbc_init<R>, dim3(L.nblk), dim3(kBlock), 0, (hipStream_t)stream, L.I, L.bmap, cur,
                      sponge ? 1 : 0);
+  return check();
+}
+
+template <class R>
+int launch_zone_sums(const Launch<R>& L, int cur, int stage, void* stream) {
+  hipLaunchKernelGGL(k_zone_sums<R>, dim3(L.ninst), dim3(kBlock), 0, (hipStream_t)stream, L.I, cur, stage);
   return check();
 }
 
@@ -2897,6 +2954,7 @@ int launch_fill(float* p, float v, int64_t n, void* stream) {
   template int launch_flux<R>(int, int, const Launch<R>&, int, double*, void*);                   \
   template int launch_flux_compact<R>(const Launch<R>&, int, void*);                              \
   template int launch_flux_ext<R>(int, int, const Launch<R>&, int, double*, void*);               \
+  template int launch_zone_sums<R>(const Launch<R>&, int, int, void*);                            \
   template int launch_grad<R>(const Launch<R>&, int, void*);                                      \
   template int launch_eos<R>(const Launch<R>&, int, void*);                                       \
   template int launch_prims<R>(const Launch<R>&, int, int, int, double*, void*);                  \

@@ -1103,6 +1103,18 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
   double sums[16][3] = {{0.0}};
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
+    if (P->bc_type[z] == 12 && P->bc_mode[z]) {
+      /* NscbcMt::calcRhs with T_TOTAL (:3761-3775): |n_bf| Ma^2 and |n_bf|
+       * over the zone's cells, Ma = (rhoun_bc / rho) / c */
+      const double* n = &m->n_bf[3 * (size_t)ibf];
+      const int icv = m->cvobf[ibf];
+      const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      const double un_bc = -P->bc_q[z] / P->bc_area[z] / s->rho[icv];
+      const double Ma = un_bc / s->sos[icv];
+      sums[z][0] += area * Ma * Ma;
+      sums[z][1] += area;
+      continue;
+    }
     if (P->bc_type[z] != 14) continue;
     const double* n = &m->n_bf[3 * (size_t)ibf];
     const double* b = &s->nscbc[5 * (size_t)ibf];
@@ -1127,7 +1139,9 @@ static void nscbc_calc_rhs(og_solver* s, int rk_stage) {
       /* NscbcMt::calcRhs (IdealGasSolverBcs.cpp:3761-3840): velocity and
        * pressure from the density, mass flux -mdot / area, static T */
       const double rhoun_bc = -P->bc_q[z] / P->bc_area[z];
-      const double RT_bc = P->bc_T[z] * s->R_gas;
+      const double T_bc =
+          P->bc_mode[z] ? P->bc_T[z] / (1.0 + 0.5 * (P->gamma - 1.0) * (sums[z][0] / sums[z][1])) : P->bc_T[z];
+      const double RT_bc = T_bc * s->R_gas;
       for (int i = 0; i < 3; ++i) b[2 + i] = (rhoun_bc * unit_n[i] + b[0] * 0.0) / b[0];
       b[1] = b[0] * RT_bc;
       og_nscbc_inlet_rhs(rhs, unit_n, delta, b, cv, P->gamma);
@@ -1317,17 +1331,38 @@ static void wm_calc(og_solver* s) {
 
 static int bc_riemann(int type) { return (type >= 4 && type <= 9) || type == 11 || type == 12 || type == 14; }
 
-/* CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3449): un_bc = the
- * cell's normal velocity */
+/* USE_AVG's zone average (CbcTotalPt, IdealGasSolverBcs.cpp:3452-3460,
+ * 3531-3541): sum rho u . n_bf over sum rho |n_bf| over the zone's cells, in
+ * face order */
+static void tpt_avg(const og_solver* s, double avg[16]) {
+  const og_mesh* m = &s->m;
+  double sums[16][2] = {{0.0}};
+  for (int ibf = 0; ibf < m->nbf; ++ibf) {
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] != 11 || !s->p.bc_mode[z]) continue;
+    const double* n = &m->n_bf[3 * (size_t)ibf];
+    const int icv = m->cvobf[ibf];
+    const double* u = &s->u[3 * (size_t)icv];
+    sums[z][0] += s->rho[icv] * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    sums[z][1] += s->rho[icv] * (u[0] * n[0] + u[1] * n[1] + u[2] * n[2]);
+  }
+  for (int z = 0; z < 16; ++z) avg[z] = sums[z][0] != 0.0 ? sums[z][1] / sums[z][0] : 0.0;
+}
+
+/* CbcTotalPt::initialHook (IdealGasSolverBcs.cpp:3427-3470): un_bc = the
+ * cell's normal velocity, or with USE_AVG the zone's average */
 static void tpt_init(og_solver* s) {
   const og_mesh* m = &s->m;
+  double avg[16];
+  tpt_avg(s, avg);
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
-    if (s->p.bc_type[m->zone_bf[ibf]] != 11) continue;
+    const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] != 11) continue;
     const double* n = &m->n_bf[3 * (size_t)ibf];
     const double* u = &s->u[3 * (size_t)m->cvobf[ibf]];
     const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const double unit_n[3] = {n[0] / area, n[1] / area, n[2] / area};
-    s->tpt_un[ibf] = unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
+    s->tpt_un[ibf] = s->p.bc_mode[z] ? avg[z] : unit_n[0] * u[0] + unit_n[1] * u[1] + unit_n[2] * u[2];
   }
 }
 
@@ -1336,6 +1371,8 @@ static void tpt_init(og_solver* s) {
 static void tpt_rk3(og_solver* s, int rk_stage) {
   const og_mesh* m = &s->m;
   if (rk_stage != 1) return;
+  double avg[16];
+  tpt_avg(s, avg); /* USE_AVG: un_bc_avg <- (1 - eps) un_bc_avg + eps avg0, held by every face */
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
     if (s->p.bc_type[z] != 11) continue;
@@ -1344,7 +1381,7 @@ static void tpt_rk3(og_solver* s, int rk_stage) {
     const double* u = &s->u[3 * (size_t)m->cvobf[ibf]];
     const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     const double unit_n[3] = {n[0] / area, n[1] / area, n[2] / area};
-    const double un0 = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+    const double un0 = s->p.bc_mode[z] ? avg[z] : u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
     s->tpt_un[ibf] = (1.0 - eps) * s->tpt_un[ibf] + eps * un0;
   }
 }

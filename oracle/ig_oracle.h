@@ -120,6 +120,7 @@ typedef struct og_params {
   /* passive scalars entering through each zone's faces that carry a mass flux
    * (CBC values from the zone's line; 0 for NSCBC outlets and the sponge) */
   double bc_sc[16][8];
+  int bc_mode[16]; /* CBC_TOTAL_PT 1: USE_AVG; NSCBC_MT 1: T_TOTAL (bc_T the total temperature) */
 } og_params;
 
 typedef struct og_diag {

@@ -40,7 +40,7 @@ class og_params(C.Structure):
         ("bc_area", C.c_double * 16), ("bc_sigma", C.c_double * 16), ("bc_L", C.c_double * 16),
         ("sponge_zone", C.c_int), ("sponge_type", C.c_int), ("sponge_strength", C.c_double),
         ("sponge_length", C.c_double), ("sponge_p", C.c_double), ("sponge_trelax", C.c_double),
-        ("bc_sc", (C.c_double * 8) * 16),
+        ("bc_sc", (C.c_double * 8) * 16), ("bc_mode", C.c_int * 16),
     ]
 
 
@@ -200,6 +200,7 @@ def params_from(cgp, zones=(), zone_area=()) -> og_params:
         p.bc_p[z], p.bc_rho[z], p.bc_q[z] = b.p, b.rho, b.q
         p.bc_area[z] = zone_area[z] if len(zone_area) else 0.0
         p.bc_sigma[z], p.bc_L[z] = b.sigma, b.L
+        p.bc_mode[z] = b.mode if b.type in (11, 12) else 0
         if 4 <= b.type <= 7 or b.type in (11, 12):  # CBC zones and NSCBC inlets bring scalars in; others 0
             for i in range(min(cgp.nsc, 8)):
                 p.bc_sc[z][i] = b.sc[i]

@@ -526,18 +526,21 @@ def test_reference_deck_boundary_lines_parse():
 def test_total_pt_parser():
     p = parse_params(deck({"y0": "CBC_TOTAL_PT 1.1e5 300 0.01 STATIONARY_FRAME", "y1": "SLIP"}))
     b = p.bc[0]
-    assert (b.type, b.p, b.T, b.L) == (11, 1.1e5, 300.0, 0.01)
-    for bad in ("y0 CBC_TOTAL_PT 1e5 300", "y0 CBC_TOTAL_PT 1e5 300 0.1 USE_AVG", "y0 CBC_TOTAL_PT -1 300 0.1"):
+    assert (b.type, b.p, b.T, b.L, b.mode) == (11, 1.1e5, 300.0, 0.01, 0)
+    assert parse_params("y0 CBC_TOTAL_PT 1e5 300 0.1 USE_AVG\n").bc[0].mode == 1
+    for bad in ("y0 CBC_TOTAL_PT 1e5 300", "y0 CBC_TOTAL_PT 1e5 300 0.1 SOMETHING", "y0 CBC_TOTAL_PT -1 300 0.1"):
         with pytest.raises(CharlesError):
             parse_params(bad + "\n")
 
 
+@pytest.mark.parametrize("avg", ["", " USE_AVG"])
 @pytest.mark.parametrize("jitter", [0.0, 0.3])
-def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter):
+def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter, avg):
     # a uniform stream V in through a CBC_TOTAL_PT inlet whose totals are the
     # stream's (T0 = T + V^2 / (2 cp), p0 = p (T0 / T)^(gamma / (gamma - 1))):
-    # un_bc starts as the cells' normal velocity, the isentropic state at that
-    # speed is the stream, so the RHS vanishes and five steps keep it
+    # un_bc starts as the cells' normal velocity (USE_AVG: the zone's average
+    # sum rho u . n_bf / sum rho |n_bf|, the same here), the isentropic state at
+    # that speed is the stream, so the RHS vanishes and five steps keep it
     m = Mesh.channel(10, jitter=jitter, seed=14)
     R, T, gamma = 71.42857142857143, 1.0, 1.4
     p = R * T
@@ -545,7 +548,7 @@ def test_total_pt_inlet_keeps_a_matching_uniform_stream(jitter):
     cp = R * gamma / (gamma - 1.0)
     T0 = T + 0.5 * V * V / cp
     p0 = p * (T0 / T) ** (gamma / (gamma - 1.0))
-    text = deck({"y0": f"CBC_TOTAL_PT {p0!r} {T0!r} 0.05", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
+    text = deck({"y0": f"CBC_TOTAL_PT {p0!r} {T0!r} 0.05{avg}", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
     o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
     r = o.calc_rhs()
     assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H
@@ -611,24 +614,30 @@ def test_nscbc_outlet_mdot_pulls_toward_its_target():
 def test_nscbc_mt_parser():
     p = parse_params(deck({"y0": "NSCBC_MT 2.5 1.1 Z 0.3", "y1": "SLIP"}) + "REGISTER_SCALAR Z\n")
     b = p.bc[0]
-    assert (b.type, b.q, b.T, b.sc[0]) == (12, 2.5, 1.1, 0.3)
-    for bad in ("y0 NSCBC_MT 2.5", "y0 NSCBC_MT 2.5 T_TOTAL 300", "y0 NSCBC_MT 2.5 -1"):
+    assert (b.type, b.q, b.T, b.sc[0], b.mode) == (12, 2.5, 1.1, 0.3, 0)
+    b = parse_params("y0 NSCBC_MT 2.5 T_TOTAL 300\n").bc[0]
+    assert (b.q, b.T, b.mode) == (2.5, 300.0, 1)
+    for bad in ("y0 NSCBC_MT 2.5", "y0 NSCBC_MT 2.5 T_TOTAL", "y0 NSCBC_MT 2.5 -1", "y0 NSCBC_MT 2.5 T_TOTAL -3"):
         with pytest.raises(CharlesError):
             parse_params(bad + "\n")
 
 
+@pytest.mark.parametrize("total", [False, True])
 @pytest.mark.parametrize("jitter", [0.0, 0.3])
-def test_nscbc_mt_inlet_keeps_a_matching_uniform_stream(jitter):
+def test_nscbc_mt_inlet_keeps_a_matching_uniform_stream(jitter, total):
     # a uniform stream V in through an NSCBC_MT inlet of the stream's mass flow
     # (mdot = rho V area) and temperature: the faces' velocity -mdot/area/rho n
     # is the stream's, their density and pressure the cells', every rate
-    # vanishes, so the RHS is zero and five steps keep the stream
+    # vanishes, so the RHS is zero and five steps keep the stream. With
+    # T_TOTAL T0 = T + V^2 / (2 cp) the static T0 / (1 + (gamma-1)/2 Ma^2) of
+    # the zone's average Ma^2 = V^2 / (gamma R T) is the stream's T again
     m = Mesh.channel(10, jitter=jitter, seed=15)
-    R, T = 71.42857142857143, 1.0
+    R, T, gamma = 71.42857142857143, 1.0, 1.4
     p = R * T
     V = 3.0
     area = float(np.bincount(m.get("zone_bf"), weights=m.get("area_bf"))[0])
-    text = deck({"y0": f"NSCBC_MT {1.0 * V * area!r} {T}", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
+    Tspec = f"T_TOTAL {T + 0.5 * V * V * (gamma - 1.0) / (gamma * R)!r}" if total else f"{T}"
+    text = deck({"y0": f"NSCBC_MT {1.0 * V * area!r} {Tspec}", "y1": f"CBC_UPT 0 {V} 0 {p!r} {T}"})
     o = oracle(m, text, uniform(m, (0.0, V, 0.0), p=p, T=T, R=R))
     r = o.calc_rhs()
     assert np.abs(r).max() < 1e-11 * (V * V + p) * H * H

@@ -8,7 +8,8 @@ states evolved with the step, restored with it after a NaN) and mass-flow
 outlet NSCBC_OUTLET_MDOT (its zone sums before each stage, also in a batch), the algebraic
 wall model WM_ALG_ADIABATIC (per-face stress from the law of the wall) and
 its isothermal form WM_ALG_ISOTHERMAL (with the thermal law's heat flux), the
-total-pressure inlet CBC_TOTAL_PT, the NSCBC mass-flow inlet NSCBC_MT and the SPONGE
+total-pressure inlet CBC_TOTAL_PT, the NSCBC mass-flow inlet NSCBC_MT (each also
+with its zone-averaged form, USE_AVG and T_TOTAL) and the SPONGE
 (per-cell relaxed T, u, a volume term over the last LENGTH of the domain, the
 Riemann flux at its faces; L_Y and L_RX on channels, L_X on the orifice) on
 jittered channel meshes and on the reference's own Stitch mesh orifice.mles
@@ -53,6 +54,11 @@ WALLS = {
                        "y1": f"SPONGE TYPE L_Y LENGTH 1.5 STRENGTH 5.0 RELAX_TIME 0.05 SPONGE_PRESSURE {P0 * 0.995!r}"},
     # in through a total-pressure inlet, out through an NSCBC outlet
     "tpt+nscbc": {"y0": f"CBC_TOTAL_PT {P0 * 1.004!r} 1.002 0.05", "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    # the same two inlets with their zone-averaged forms (USE_AVG, T_TOTAL)
+    "tpt_avg+nscbc": {"y0": f"CBC_TOTAL_PT {P0 * 1.004!r} 1.002 0.05 USE_AVG",
+                      "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
+    "nscbc_mt_t0+nscbc": {"y0": f"NSCBC_MT {0.3 * (2 * np.pi) ** 2!r} T_TOTAL 1.001",
+                          "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
     # in through an NSCBC mass-flow inlet (0.3 c0 / 10 into the channel), out through an NSCBC outlet
     "nscbc_mt+nscbc": {"y0": f"NSCBC_MT {0.3 * (2 * np.pi) ** 2!r} 1.0",
                        "y1": f"NSCBC_OUTLET_P {P0 * 0.995!r} 0.3 6.28"},
@@ -223,8 +229,10 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
     check_history(g, o, 20, "orifice nscbc")
 
 
+# (not CBC_TOTAL_PT: its un_bc is not in the snapshots, as in the reference, so a
+# NaN that reaches it stays, and with USE_AVG any NaN in an inlet cell does)
 @pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate", "nscbc_mt+nscbc",
-                                   "wmiso+wmiso", "cbc_upt+mdot"])
+                                   "wmiso+wmiso", "cbc_upt+mdot", "nscbc_mt_t0+nscbc"])
 def test_boundary_states_roll_back_with_the_flow(walls):
     # a NaN after a snapshot: the step restores the older snapshot, the NSCBC
     # faces' or the sponge's states with it (storeData / restoreData of the
@@ -258,7 +266,7 @@ def test_boundary_states_roll_back_with_the_flow(walls):
 
 @pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge",
                                    "wm+wm_translate", "tpt+nscbc", "nscbc_mt+nscbc", "wmiso+wmiso",
-                                   "cbc_upt+mdot"])
+                                   "cbc_upt+mdot", "tpt_avg+nscbc", "nscbc_mt_t0+nscbc"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]
