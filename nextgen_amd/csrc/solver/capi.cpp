@@ -180,7 +180,7 @@ struct Instance : HostInst {
   cgk::Launch<R> solo_launch() const {
     return {solo, lay->solo_bmap, nblocks, 1, lds, lay->solo_pbmap, npblocks, 4, prm.sgs_model == CG_SGS_SIGMA,
             prm.use_full_gr != 0, prm.sgs_model == CG_SGS_DSM_LOCAL, prm.nsc,
-            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0, zsum};
+            scalar_lds<R>(lds, prm.nsc, lay->lds_max), lay->g.nbf > 0, zsum, wm ? lay->g.nbf : 0};
   }
   cgdev::Inst<R> record() const {
     cgdev::Inst<R> r{};
@@ -1170,10 +1170,11 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   if (npb) HIPCHK(s, hipMemcpy(E.pbmap, pbmap.data(), sizeof(int) * npb, hipMemcpyHostToDevice));
   // the LDS variant of the extended pass needs a halo on every instance
   bool any_sigma = false, any_full_gr = false, any_dsm = false, any_bf = false, any_zsum = false;
-  int max_nsc = 0, lds_max = 0;
+  int max_nsc = 0, lds_max = 0, wm_nbf = 0;
   for (auto* I : E.inst) {
     any_bf = any_bf || I->lay->g.nbf > 0;
     any_zsum = any_zsum || I->zsum;
+    if (I->wm) wm_nbf = std::max(wm_nbf, I->lay->g.nbf);
     any_sigma = any_sigma || I->prm.sgs_model == CG_SGS_SIGMA;
     any_full_gr = any_full_gr || I->prm.use_full_gr != 0;
     any_dsm = any_dsm || I->prm.sgs_model == CG_SGS_DSM_LOCAL;
@@ -1186,7 +1187,8 @@ int engine_sync(Err* s, Engine<R>& E, hipStream_t stream) {
   E.hole_blocks = 0;
   const size_t flux_lds = (all_halo && !E.force_global) ? lds : 0;
   E.L = {E.tab, E.bmap, nb, n, flux_lds, E.pbmap, npb, E.tile, any_sigma,
-         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf, any_zsum};
+         any_full_gr, any_dsm, max_nsc, scalar_lds<R>(flux_lds, max_nsc, lds_max), any_bf, any_zsum,
+         wm_nbf};
   E.dirty = false;
   return 0;
 }

@@ -253,6 +253,7 @@ struct Launch {
   size_t sc_lds;     // dynamic LDS bytes of the scalar pass's halo variant (0: global gathers)
   int boundary;      // some instance has boundary faces (the properties pass keeps mu_lam, loc_lam)
   int zsum;          // some instance has zones needing zone sums (k_zone_sums precedes each stage's flux pass)
+  int wm_nbf;        // most boundary faces of an instance with wall-model zones (0: none; k_wm_solve precedes stage 1)
 };
 template <class R>
 int launch_flux(int stage, int mode, const Launch<R>& L, int cur, double* rhs_out, void* stream);

@@ -839,14 +839,13 @@ __device__ __forceinline__ double wm_solve_tau(double u1, double y1, double rho,
   return rho * u_tau * u_tau;
 }
 
-// A WM_ALG_ADIABATIC face b of cell c (WmAdiabatic, IdealGasSolverBcs.cpp:
-// 1256-1441): at the start of a step (stage 1 of a fused step) its wall stress
-// from the law of the wall for the cell's wall-parallel speed at y1 = area /
-// area_over_delta (calcRhs), then p n plus the stress along the parallel
-// velocity, and the wall's work (addBoundaryFlux). FP64 from the FP64 state.
+// A WM_ALG_ADIABATIC face b of cell c (WmAdiabatic::addBoundaryFlux,
+// IdealGasSolverBcs.cpp:1395-1441): p n plus the face's wall stress (k_wm_solve
+// sets it at the start of each step) along the cell's wall-parallel velocity,
+// and the wall's work. FP64 from the FP64 state.
 template <class R>
 __device__ __forceinline__ void wm_face(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st, const Params& P,
-                                        int c, int b, int z, const Light<R>& self, R acc[5], int stage, bool update) {
+                                        int c, int b, int z, const Light<R>& self, R acc[5]) {
   #pragma clang fp contract(off)
   double rho, u[3], rhoE, spv, p, h, T;
   cell_state64(g, st, P, c, self, rho, u, rhoE, spv, p, h, T);
@@ -857,24 +856,6 @@ __device__ __forceinline__ void wm_face(const GeomT<R>& g, const WorkT<R>& w, co
     for (int i = 0; i < 3; ++i) unit_n[i] = n[i] / n_mag;
   const double* ubc = P.bc_u[z];
   const auto tw = G(w.wm_tau) + b;
-  if (update && stage == 1) {
-    const double aod = G(g.bf_aod)[b];
-    double tau = 0.0;
-    if (aod > 0.0) {
-      const double y1 = n_mag / aod;
-      double u_par[3];
-      if (n_mag > 0.0) {
-        const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
-        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - un * unit_n[i] - ubc[i];
-      } else {
-        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - ubc[i];
-      }
-      const double u1 = sqrt(u_par[0] * u_par[0] + u_par[1] * u_par[1] + u_par[2] * u_par[2]);
-      const double u_tau = sqrt(*tw / rho);
-      tau = wm_solve_tau(u1, y1, rho, G(w.mu_lam)[c], u_tau);
-    }
-    *tw = tau;
-  }
   const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
   double uph[3];
   for (int i = 0; i < 3; ++i) uph[i] = u[i] - un * unit_n[i] - ubc[i];
@@ -929,17 +910,13 @@ __device__ __forceinline__ double wm_q_wall(double T1, double Tw, double tau_w, 
   return rhs / (1.0 + kappa_t * Tstar);
 }
 
-// A WM_ALG_ISOTHERMAL face b of cell c (WmIsothermal, IdealGasSolverBcs.cpp:
-// 2054-2226): at the start of a step (stage 1 of a fused step) its wall stress
-// from the law of the wall for the cell's speed |u| at y1 = area /
-// area_over_delta and its heat flux toward the wall temperature from that
-// stress (calcRhs; both 0 where area_over_delta is 0), then p n plus the
-// stress along u and the heat flux out of the energy (addBoundaryFlux). FP64
-// from the FP64 state; T is the stage's, mu_lam and loc_lam the lagged ones.
+// A WM_ALG_ISOTHERMAL face b of cell c (WmIsothermal::addBoundaryFlux,
+// IdealGasSolverBcs.cpp:2208-2226): p n plus the face's wall stress along u and
+// its heat flux out of the energy (k_wm_solve sets both at the start of each
+// step). FP64 from the FP64 state.
 template <class R>
 __device__ __forceinline__ void wm_iso_face(const GeomT<R>& g, const WorkT<R>& w, const StateT<R>& st,
-                                            const Params& P, int c, int b, int z, const Light<R>& self, R acc[5],
-                                            int stage, bool update) {
+                                            const Params& P, int c, int b, int z, const Light<R>& self, R acc[5]) {
   #pragma clang fp contract(off)
   double rho, u[3], rhoE, spv, p, h, T;
   cell_state64(g, st, P, c, self, rho, u, rhoE, spv, p, h, T);
@@ -947,21 +924,6 @@ __device__ __forceinline__ void wm_iso_face(const GeomT<R>& g, const WorkT<R>& w
   const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   const auto tw = G(w.wm_tau) + b;
   const auto qw = G(w.wm_q) + b;
-  if (update && stage == 1) {
-    const double aod = G(g.bf_aod)[b];
-    double tau = 0.0, q = 0.0;
-    if (aod > 0.0) {
-      const double y1 = n_mag / aod;
-      const double u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-      const double u_tau = sqrt(*tw / rho);
-      const double mu = G(w.mu_lam)[c];
-      tau = wm_solve_tau(u1, y1, rho, mu, u_tau);
-      const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
-      q = wm_q_wall(T, P.bc_T[z], tau, rho, mu, G(w.loc_lam)[c], cp, y1);
-    }
-    *tw = tau;
-    *qw = q;
-  }
   double u_mag = 0.0;
   for (int i = 0; i < 3; ++i) u_mag += u[i] * u[i];
   u_mag = sqrt(fmax(0.0, u_mag));
@@ -1036,9 +998,9 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       }
       acc[4] -= visc_work;
     } else if (type == BC_WM_ALG_ADIABATIC) {
-      wm_face(g, w, st, P, c, b, z, self, acc, stage, update);
+      wm_face(g, w, st, P, c, b, z, self, acc);
     } else if (type == BC_WM_ALG_ISOTHERMAL) {
-      wm_iso_face(g, w, st, P, c, b, z, self, acc, stage, update);
+      wm_iso_face(g, w, st, P, c, b, z, self, acc);
     } else if (bc_riemann(type)) {
       // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
       // between the cell's FP64 state and the zone's boundary state
@@ -2644,6 +2606,64 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict
   if (threadIdx.x < 8) diag[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// The wall models' calcRhs at stage 1 (WmAdiabatic, IdealGasSolverBcs.cpp:
+// 1256-1381; WmIsothermal, :2173-2206), before the step's first flux pass,
+// one thread per boundary face (blockIdx.y the solve): WM_ALG_ADIABATIC faces
+// solve the law of the wall (wm_solve_tau) for the cell's wall-parallel speed
+// relative to the wall at y1 = area / area_over_delta, from the previous
+// stress's u_tau; WM_ALG_ISOTHERMAL faces for the cell's speed |u|, then
+// their heat flux toward the wall temperature (wm_q_wall); both 0 where
+// area_over_delta is 0. The state is the step's start (st[cur]), T = p / (rho
+// R) of it, mu_lam and loc_lam the lagged coefficients. FP64 throughout.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__ I, int cur) {
+  #pragma clang fp contract(off)
+  const Inst<R>& X = I[blockIdx.y];
+  if (X.nblocks == 0 || !X.w.wm_tau) return;  // removed from the batch, or no wall model
+  const GeomT<R>& g = X.g;
+  const Params& P = X.P;
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= g.nbf) return;
+  const int z = G(g.bf_zone)[b];
+  const int type = P.bc_type[z];
+  if (type != BC_WM_ALG_ADIABATIC && type != BC_WM_ALG_ISOTHERMAL) return;
+  const int c = G(g.bf_cell)[b];
+  double rho, u[3], rhoE, spv, p, h, T;
+  const Light<R> self = load_light(X.st[cur].light, c);
+  cell_state64(g, X.st[cur], P, c, self, rho, u, rhoE, spv, p, h, T);
+  const double n[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
+  const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const double aod = G(g.bf_aod)[b];
+  const auto tw = G(X.w.wm_tau) + b;
+  double tau = 0.0, q = 0.0;
+  if (aod > 0.0) {
+    const double y1 = n_mag / aod;
+    const double mu = G(X.w.mu_lam)[c];
+    double u1;
+    if (type == BC_WM_ALG_ADIABATIC) {
+      const double* ubc = P.bc_u[z];
+      double u_par[3];
+      if (n_mag > 0.0) {
+        const double unit_n[3] = {n[0] / n_mag, n[1] / n_mag, n[2] / n_mag};
+        const double un = u[0] * unit_n[0] + u[1] * unit_n[1] + u[2] * unit_n[2];
+        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - un * unit_n[i] - ubc[i];
+      } else {
+        for (int i = 0; i < 3; ++i) u_par[i] = u[i] - ubc[i];
+      }
+      u1 = sqrt(u_par[0] * u_par[0] + u_par[1] * u_par[1] + u_par[2] * u_par[2]);
+    } else {
+      u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    }
+    tau = wm_solve_tau(u1, y1, rho, mu, sqrt(*tw / rho));
+    if (type == BC_WM_ALG_ISOTHERMAL) {
+      const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
+      q = wm_q_wall(T, P.bc_T[z], tau, rho, mu, G(X.w.loc_lam)[c], cp, y1);
+    }
+  }
+  *tw = tau;
+  if (type == BC_WM_ALG_ISOTHERMAL) G(X.w.wm_q)[b] = q;
+}
+
 // The sums over a zone that some conditions need before a stage, for each
 // zone of each solve into w.zsum[3 z ..]: NSCBC_OUTLET_MDOT at stages 1-3
 // (NscbcOutletP::calcRhsMdot, IdealGasSolverBcs.cpp:3937-3948), over its faces'
@@ -2761,6 +2781,11 @@ int launch_flux_ext(int stage, int mode, const Launch<R>& L, int cur, double* rh
       else hipLaunchKernelGGL((k_scalars<R, 3, false>), grid, block, 0, s, L.I, L.bmap, cur);
     }
     return check();
+  }
+  if (mode == 1 && stage == 1 && L.wm_nbf > 0) {
+    hipLaunchKernelGGL(k_wm_solve<R>, dim3((L.wm_nbf + kBlock - 1) / kBlock, L.ninst), dim3(kBlock), 0, s, L.I,
+                       cur);
+    if (int rc = check()) return rc;
   }
   if (mode == 1 && L.zsum) {
     hipLaunchKernelGGL(k_zone_sums<R>, dim3(L.ninst), dim3(kBlock), 0, s, L.I, cur, stage);
