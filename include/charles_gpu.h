@@ -135,6 +135,16 @@ typedef struct cg_solver cg_solver;
  *                                    y1 = area / area_over_delta, applied as a
  *                                    force along it; RELAX, Y_PLUS0, TOL belong
  *                                    to the equilibrium model and are ignored
+ *   WM_EQ_ADIABATIC [TRANSLATE u v w] [RELAX r] [Y_PLUS0 y] [TOL t]
+ *                                    WmAdiabatic with WM_EQUILIBRIUM_ADIABATIC:
+ *                                    the stress from the equilibrium boundary
+ *                                    layer integrated across y1 (EquilibriumWM::
+ *                                    solve_qt, wm/EquilibriumWM.hpp:36-205) with
+ *                                    an adiabatic wall temperature per face kept
+ *                                    between steps; RELAX (0.7), Y_PLUS0 (1),
+ *                                    TOL (1e-4) are its relaxation, first step
+ *                                    in wall units and tolerance (cg_bc sigma,
+ *                                    L, q)
  *   WM_ALG_ISOTHERMAL <T> [RELAX r] [Y_PLUS0 y] [TOL t]
  *                                    WmIsothermal with WM_ALGEBRAIC_ISOTHERMAL
  *                                    (:2054-2226): per face tau_wall from the
@@ -162,6 +172,7 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_NSCBC_MT 12
 #define CG_BC_WM_ALG_ISOTHERMAL 13
 #define CG_BC_NSCBC_OUTLET_MDOT 14
+#define CG_BC_WM_EQ_ADIABATIC 15
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */

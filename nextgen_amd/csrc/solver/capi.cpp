@@ -149,6 +149,7 @@ struct HostInst {
   bool nscbc = false;                // some zone is an NSCBC outlet
   bool wm = false;                   // some zone is a WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL wall
   bool wm_iso = false;               // some zone is a WM_ALG_ISOTHERMAL wall
+  bool wm_eq = false;                // some zone is a WM_EQ_ADIABATIC wall (T_wall follows tau_wall)
   bool zsum = false;                 // some zone needs zone sums (NSCBC_OUTLET_MDOT, USE_AVG, T_TOTAL)
   bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
@@ -366,6 +367,7 @@ void fill_dev_params(HostInst* h) {
   P.mu_ref = p.mu_ref;
   P.mu_power_law = p.mu_power_law;
   P.invPr_lam = 1.0 / p.Pr_lam;
+  P.Pr_lam = p.Pr_lam;
   P.vreman_coeff = p.vreman_coeff;
   P.sigma_coeff = p.sigma_coeff;
   P.a_sgs_coeff = p.a_sgs_coeff;
@@ -933,7 +935,9 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
                   p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT;
       hi->P.bc_mode[z] = (p->bc[k].type == CG_BC_CBC_TOTAL_PT || p->bc[k].type == CG_BC_NSCBC_MT) ? p->bc[k].mode : 0;
       hi->zsum = hi->zsum || p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT || hi->P.bc_mode[z] != 0;
-      hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
+      hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL ||
+               p->bc[k].type == CG_BC_WM_EQ_ADIABATIC;
+      hi->wm_eq = hi->wm_eq || p->bc[k].type == CG_BC_WM_EQ_ADIABATIC;
       hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       if (p->bc[k].type == CG_BC_SPONGE) {
@@ -1010,9 +1014,9 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
   if (hi->wm) {
     // WmAdiabatic::storeData, WmIsothermal::storeData: the faces' tau_wall
     // (in the snapshots too); q_wall is not stored
-    rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf);
+    rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf * (hi->wm_eq ? 2 : 1));
     if (hi->wm_iso) rc |= dev_alloc(s, h, &w.wm_q, (size_t)lay->g.nbf);
-    hi->snap_wm = (size_t)lay->g.nbf;
+    hi->snap_wm = (size_t)lay->g.nbf * (hi->wm_eq ? 2 : 1);
     hi->snap_n += hi->snap_wm;
   }
   std::vector<double> sp_coeff;
@@ -1096,7 +1100,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= cgk::launch_fill(w.nscbc, 0.0, 5 * (int64_t)lay->g.nbf, stream);
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
   }
-  if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)lay->g.nbf, stream);  // set by initComplete
+  if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)hi->snap_wm, stream);  // set by initComplete
   if (hi->wm_iso) rc |= cgk::launch_fill(w.wm_q, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->tpt) rc |= cgk::launch_fill(w.tpt_un, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->sponge) {
@@ -1786,11 +1790,16 @@ int cg_params_parse(cg_params* p, const char* text) {
             return bad("unknown WALL_ADIABATIC option " + opt);
           }
         }
-      } else if (type == "WM_ALG_ADIABATIC") {
+      } else if (type == "WM_ALG_ADIABATIC" || type == "WM_EQ_ADIABATIC") {
         // WmAdiabatic::initData (IdealGasSolverBcs.cpp:996-1073): the wall
-        // velocity (TRANSLATE); RELAX, Y_PLUS0 and TOL are the equilibrium
-        // model's and take a value each
-        b.type = CG_BC_WM_ALG_ADIABATIC;
+        // velocity (TRANSLATE); RELAX, Y_PLUS0 and TOL take a value each, the
+        // equilibrium model's relaxation (sigma), first step in wall units (L)
+        // and tolerance (q), defaults 0.7, 1, 1e-4 (IdealGasSolverBcs.hpp:200-202)
+        const bool eq = type == "WM_EQ_ADIABATIC";
+        b.type = eq ? CG_BC_WM_EQ_ADIABATIC : CG_BC_WM_ALG_ADIABATIC;
+        b.sigma = 0.7;
+        b.L = 1.0;
+        b.q = 1.0e-4;
         for (size_t i = 2; i < tok.size(); ++i) {
           const std::string opt = upper(tok[i]);
           double ignored = 0.0;
@@ -1801,9 +1810,10 @@ int cg_params_parse(cg_params* p, const char* text) {
           } else if (opt == "STATIONARY" || opt == "STATIONARY_FRAME") {
             // the wall velocity of a rotating frame; without one the wall is at rest
           } else if (opt == "RELAX" || opt == "Y_PLUS0" || opt == "TOL") {
-            if (!num(++i, &ignored)) return bad("expected " + opt + " <value>");
+            double* dst = !eq ? &ignored : opt == "RELAX" ? &b.sigma : opt == "Y_PLUS0" ? &b.L : &b.q;
+            if (!num(++i, dst)) return bad("expected " + opt + " <value>");
           } else {
-            return bad("unknown WM_ALG_ADIABATIC option " + opt);
+            return bad("unknown " + type + " option " + opt);
           }
         }
       } else if (type == "WM_ALG_ISOTHERMAL") {
@@ -1947,7 +1957,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_ALG_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
+                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_EQ_ADIABATIC, WM_ALG_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
                    "NSCBC_MT, NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE, NSCBC_OUTLET_MDOT and SPONGE are)");
       }
@@ -2421,7 +2431,8 @@ int cg_set_source_hook(cg_solver* s, cg_source_fn fn, void* user) {
     if (fn && I->nscbc) return s->set(CG_EINVAL, "a source hook cannot be combined with NSCBC zones");
     if (fn && I->sponge) return s->set(CG_EINVAL, "a source hook cannot be combined with a SPONGE zone");
     if (fn && I->wm)
-      return s->set(CG_EINVAL, "a source hook cannot be combined with WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL zones");
+      return s->set(CG_EINVAL, "a source hook cannot be combined with wall-model zones (WM_ALG_ADIABATIC, "
+                               "WM_EQ_ADIABATIC, WM_ALG_ISOTHERMAL)");
     if (fn && I->tpt) return s->set(CG_EINVAL, "a source hook cannot be combined with CBC_TOTAL_PT zones");
     I->hook = fn;
     I->hook_user = fn ? user : nullptr;

@@ -151,8 +151,10 @@ struct WorkT {
   // SPONGE (null without one): per cell the relaxed state T, u[3] [ncv][4] and
   // the volume coefficient strength x_sp^2 (negative outside the sponge) [ncv]
   double *sponge, *sponge_coeff;
-  // WM_ALG_ADIABATIC and WM_ALG_ISOTHERMAL faces (null without them): per
-  // boundary face its wall stress tau_wall (FP64, in the snapshots) and, with
+  // WM_ALG_ADIABATIC, WM_EQ_ADIABATIC and WM_ALG_ISOTHERMAL faces (null without
+  // them): per boundary face its wall stress tau_wall (FP64, in the snapshots;
+  // with a WM_EQ_ADIABATIC zone followed by each face's adiabatic wall
+  // temperature T_wall, [2][nbf], both snapshotted as WmAdiabatic::storeData) and, with
   // an isothermal zone, its wall heat flux q_wall (FP64, not in the snapshots,
   // as in the reference: each step's stage 1 sets it before it is used)
   double *wm_tau, *wm_q;
@@ -171,7 +173,7 @@ enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
   BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12,
-  BC_WM_ALG_ISOTHERMAL = 13, BC_NSCBC_OUTLET_MDOT = 14
+  BC_WM_ALG_ISOTHERMAL = 13, BC_NSCBC_OUTLET_MDOT = 14, BC_WM_EQ_ADIABATIC = 15
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
 __host__ __device__ inline bool bc_riemann(int type) {
@@ -204,6 +206,7 @@ struct Params {
   int sponge_zone;                 // the SPONGE zone, or -1
   double sponge_p, sponge_trelax;  // its pressure and relaxation time
   double bc_sc[kMaxBcZones][kMaxScalars];  // scalars entering through a zone (CBC values, 0 otherwise)
+  double Pr_lam;                           // PR_LAM (the equilibrium wall model's laminar conductivity)
   int bc_mode[kMaxBcZones];  // CBC_TOTAL_PT 1: USE_AVG; NSCBC_MT 1: T_TOTAL (bc_T then the total temperature)
 };
 

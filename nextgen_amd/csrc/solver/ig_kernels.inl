@@ -997,7 +997,7 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
         visc_work += ubc * tauijnj;
       }
       acc[4] -= visc_work;
-    } else if (type == BC_WM_ALG_ADIABATIC) {
+    } else if (type == BC_WM_ALG_ADIABATIC || type == BC_WM_EQ_ADIABATIC) {
       wm_face(g, w, st, P, c, b, z, self, acc);
     } else if (type == BC_WM_ALG_ISOTHERMAL) {
       wm_iso_face(g, w, st, P, c, b, z, self, acc);
@@ -2409,9 +2409,11 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
       }
       continue;
     }
-    if (X.P.bc_type[z] == BC_WM_ALG_ADIABATIC) {
-      // WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1102): the
-      // laminar stress mu_lam |u - u_bc| / y1 (0 where area_over_delta is 0)
+    if (X.P.bc_type[z] == BC_WM_ALG_ADIABATIC || X.P.bc_type[z] == BC_WM_EQ_ADIABATIC) {
+      // WmAdiabatic::initialHook (IdealGasSolverBcs.cpp:1076-1110): the
+      // laminar stress mu_lam |u - u_bc| / y1 (0 where area_over_delta is 0);
+      // the equilibrium model's wall temperature T + u1^2 / (2 cp), u1 that
+      // speed (0 where area_over_delta is 0)
       #pragma clang fp contract(off)
       double rho, u[3], rhoE, spv, p, h, T;
       const Light<R> self = load_light(X.st[cur].light, c);
@@ -2426,6 +2428,11 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
         tau = G(X.w.mu_lam)[c] * u1 / y1;
       }
       G(X.w.wm_tau)[b] = tau;
+      if (X.P.bc_type[z] == BC_WM_EQ_ADIABATIC) {
+        const double cp = X.P.R_gas * X.P.gamma / (X.P.gamma - 1.0);
+        const double v = aod > 0.0 ? u1 : 0.0;
+        G(X.w.wm_tau)[g.nbf + b] = T + 0.5 * v * v / cp;
+      }
       continue;
     }
     if (X.P.bc_type[z] == BC_WM_ALG_ISOTHERMAL) {
@@ -2606,6 +2613,89 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const Inst<R>* __restrict
   if (threadIdx.x < 8) diag[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// EquilibriumWM::solve_qt (wm/EquilibriumWM.hpp:36-205), FP64 in its operation
+// order: the wall stress (and heat flux) for which the thin-layer equations,
+// integrated from the wall to y1 by implicit midpoint steps (first step
+// y_plus0 viscous lengths, then growing with the solution's scale), reach the
+// cell's speed u1 and temperature T1 within tol; Newton updates of tau_w
+// relaxed by `relax`, at most 500. T_w < 0 asks for the adiabatic case with
+// -T_w the wall-temperature guess (use_guess) and returns the new -T_wall on
+// convergence; T_w > 0 is an isothermal wall. Mixing-length eddy viscosity
+// with van Driest damping (A+ = 17), kappa 0.41, Pr_t 0.9, mu ~ T^0.76.
+// Returns 0 on convergence, 1 otherwise; the integration also gives up (1)
+// after 2^22 steps in all, a bound the reference does not have, reached only
+// by inputs on which its loop would not end either.
+__device__ __noinline__ int eq_solve_qt(double& tau_w, double& q_w, double u1, double T1, double rho1, double y1,
+                                        double mu1, double Pr_lam, double cp, double& T_w, double relax,
+                                        double y_plus0, double tol, bool use_guess) {
+  #pragma clang fp contract(off)
+  const double kappa = 0.41, Pr_t = 0.9, mpl = 0.76, fax = 0.1, inv_Aplus = 1.0 / 17.0;
+  auto lo = [](double a, double b) { return b < a ? b : a; };  // std::min / std::max on NaN as well
+  auto hi = [](double a, double b) { return a < b ? b : a; };
+  double T_wall;
+  if (T_w < 0.0) {
+    q_w = 0.0;
+    T_wall = use_guess ? -T_w : T1 + 0.5 * u1 * u1 / cp;
+  } else {
+    T_wall = T_w;
+  }
+  if (fabs(u1) < 1.0e-12) {
+    tau_w = 0.0;
+    q_w = cp * mu1 * (T1 - T_wall) / (Pr_lam * y1);
+    return 0;
+  }
+  if (!use_guess) {
+    tau_w = mu1 * u1 / y1;
+    q_w = mu1 * (cp * T1 + 0.5 * u1 * u1 - cp * T_wall) / y1;
+  }
+  long steps = 0;
+  int iter;
+  for (iter = 0; iter < 500; ++iter) {
+    const double rho_w = rho1 * T1 / T_wall;
+    const double mu_w = mu1 * pow(T_wall / T1, mpl);
+    const double u_tau = sqrt(tau_w / rho_w);
+    const double l_visc = mu_w / (rho_w * u_tau);
+    double y = 0.0, int1 = 0.0, int2 = 0.0, int3 = 0.0, int4 = 0.0, u = 0.0, T = T_wall;
+    double dy = y_plus0 * l_visc;
+    while (y < y1) {
+      if (++steps > (1L << 22)) return 1;
+      dy = lo(dy, y1 - y);
+      const double y_half = y + 0.5 * dy;
+      const double fd = 1.0 - exp(-(y_half / l_visc) * inv_Aplus);
+      const double A = kappa * sqrt(rho_w) * sqrt(T_wall / T) * y_half * (fd * fd);
+      const double mu_t = A * sqrt(tau_w);
+      const double mu_lam = mu1 * pow(T / T1, mpl);
+      const double mu_tot = mu_lam + mu_t;
+      const double k_tot = cp * mu_lam / Pr_lam + cp * mu_t / Pr_t;
+      const double unew = u + dy * tau_w / mu_tot;
+      const double Tnew = T + dy / k_tot * (q_w - 0.5 * tau_w * u - 0.5 * tau_w * unew);
+      const double mu_prime = Pr_t * mu_lam / Pr_lam;
+      int1 += dy / (mu_prime + mu_t);
+      int2 += 0.5 * (u + unew) * dy / (mu_prime + mu_t);
+      int3 += dy / (mu_lam + mu_t);
+      int4 += 0.5 * A * dy / (mu_tot * mu_tot);
+      const double dudy = (unew - u) / dy;
+      const double dTdy = (Tnew - T) / dy;
+      y += dy;
+      u = unew;
+      T = Tnew;
+      dy = hi(dy, lo(fax * u / fabs(dudy), fax * T / fabs(dTdy)));
+    }
+    if ((fabs(T - T1) / T1 < tol) && ((fabs(u1) < 1.0e-12) || (fabs(u - u1) / u1 < tol))) break;
+    const double J = int3 - sqrt(tau_w) * int4;
+    tau_w += relax * ((u1 - u) / J);
+    if (T_w < 0.0) {
+      q_w = 0.0;
+      T_wall = (1.0 - relax) * T_wall + relax * (T1 + tau_w * Pr_t / cp * int2);
+    } else {
+      q_w = relax * ((cp * (T1 - T_w) / Pr_t + tau_w * int2) / int1) + (1.0 - relax) * q_w;
+    }
+  }
+  if (iter == 500) return 1;
+  if (T_w < 0.0) T_w = -T_wall;
+  return 0;
+}
+
 // The wall models' calcRhs at stage 1 (WmAdiabatic, IdealGasSolverBcs.cpp:
 // 1256-1381; WmIsothermal, :2173-2206), before the step's first flux pass,
 // one thread per boundary face (blockIdx.y the solve): WM_ALG_ADIABATIC faces
@@ -2626,7 +2716,7 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
   if (b >= g.nbf) return;
   const int z = G(g.bf_zone)[b];
   const int type = P.bc_type[z];
-  if (type != BC_WM_ALG_ADIABATIC && type != BC_WM_ALG_ISOTHERMAL) return;
+  if (type != BC_WM_ALG_ADIABATIC && type != BC_WM_ALG_ISOTHERMAL && type != BC_WM_EQ_ADIABATIC) return;
   const int c = G(g.bf_cell)[b];
   double rho, u[3], rhoE, spv, p, h, T;
   const Light<R> self = load_light(X.st[cur].light, c);
@@ -2640,7 +2730,7 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
     const double y1 = n_mag / aod;
     const double mu = G(X.w.mu_lam)[c];
     double u1;
-    if (type == BC_WM_ALG_ADIABATIC) {
+    if (type != BC_WM_ALG_ISOTHERMAL) {
       const double* ubc = P.bc_u[z];
       double u_par[3];
       if (n_mag > 0.0) {
@@ -2654,7 +2744,17 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
     } else {
       u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     }
-    tau = wm_solve_tau(u1, y1, rho, mu, sqrt(*tw / rho));
+    if (type == BC_WM_EQ_ADIABATIC) {
+      // the face's wall temperature as the adiabatic guess (T_w < 0), kept
+      // when the solve does not converge (WmAdiabatic::calcRhs, :1360-1381)
+      const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
+      double q_w = 0.0, Tw = -G(X.w.wm_tau)[g.nbf + b];
+      tau = *tw;
+      eq_solve_qt(tau, q_w, u1, T, rho, y1, mu, P.Pr_lam, cp, Tw, P.bc_sigma[z], P.bc_L[z], P.bc_q[z], true);
+      G(X.w.wm_tau)[g.nbf + b] = -Tw;
+    } else {
+      tau = wm_solve_tau(u1, y1, rho, mu, sqrt(*tw / rho));
+    }
     if (type == BC_WM_ALG_ISOTHERMAL) {
       const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
       q = wm_q_wall(T, P.bc_T[z], tau, rho, mu, G(X.w.loc_lam)[c], cp, y1);

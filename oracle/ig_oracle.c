@@ -245,11 +245,11 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
     s->nscbc_recent = (double*)calloc(5 * nb + 1, sizeof(double));
   }
   s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
-  s->wm_tau = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_tau = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double)); /* tau, T_wall */
   s->wm_q = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->tpt_un = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
-  s->wm_store = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
-  s->wm_recent = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_store = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_recent = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->sponge = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_store = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
   s->sponge_recent = (double*)calloc(4 * (size_t)n + 1, sizeof(double));
@@ -1218,6 +1218,80 @@ double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau
   return rho * u_tau * u_tau;
 }
 
+/* EquilibriumWM::solve_qt (wm/EquilibriumWM.hpp:36-205): the wall stress
+ * (and heat flux) for which the thin-layer momentum and energy equations,
+ * integrated wall to y1 with implicit midpoint steps (first y_plus0 viscous
+ * lengths, then 0.1 of the solution's own scale), reach u1 and T1 within tol;
+ * relaxed Newton updates of tau_w, at most 500. *T_w < 0: adiabatic wall,
+ * -*T_w the guess of its temperature (use_guess), -T_wall written back on
+ * convergence; *T_w > 0: isothermal. Mixing length kappa y sqrt(rho_w T_w/T)
+ * with van Driest damping (A+ 17), Pr_t 0.9, mu ~ T^0.76. Returns 0 when it
+ * converged; 1 otherwise, also after 2^22 integration steps in all. */
+static double dmin(double a, double b) { return b < a ? b : a; }
+static double dmax(double a, double b) { return a < b ? b : a; }
+int og_wm_eq_solve_qt(double* tau_w, double* q_w, double u1, double T1, double rho1, double y1, double mu1,
+                      double Pr_lam, double cp, double* T_w, double relax, double y_plus0, double tol, int use_guess) {
+  const double kappa = 0.41, Pr_t = 0.9, mpl = 0.76, fax = 0.1, inv_Aplus = 1.0 / 17.0;
+  double T_wall;
+  if (*T_w < 0.0) {
+    *q_w = 0.0;
+    T_wall = use_guess ? -*T_w : T1 + 0.5 * u1 * u1 / cp;
+  } else {
+    T_wall = *T_w;
+  }
+  if (fabs(u1) < 1.0e-12) {
+    *tau_w = 0.0;
+    *q_w = cp * mu1 * (T1 - T_wall) / (Pr_lam * y1);
+    return 0;
+  }
+  if (!use_guess) {
+    *tau_w = mu1 * u1 / y1;
+    *q_w = mu1 * (cp * T1 + 0.5 * u1 * u1 - cp * T_wall) / y1;
+  }
+  long steps = 0;
+  int iter;
+  for (iter = 0; iter < 500; ++iter) {
+    const double rho_w = rho1 * T1 / T_wall;
+    const double l_visc = mu1 * pow(T_wall / T1, mpl) / (rho_w * sqrt(*tau_w / rho_w));
+    double y = 0.0, i1 = 0.0, i2 = 0.0, i3 = 0.0, i4 = 0.0, u = 0.0, T = T_wall;
+    double dy = y_plus0 * l_visc;
+    while (y < y1) {
+      if (++steps > (1L << 22)) return 1;
+      dy = dmin(dy, y1 - y);
+      const double yh = y + 0.5 * dy;
+      const double d = 1.0 - exp(-(yh / l_visc) * inv_Aplus);
+      const double A = kappa * sqrt(rho_w) * sqrt(T_wall / T) * yh * (d * d);
+      const double mu_t = A * sqrt(*tau_w);
+      const double mu_l = mu1 * pow(T / T1, mpl);
+      const double mu_tot = mu_l + mu_t;
+      const double k_tot = cp * mu_l / Pr_lam + cp * mu_t / Pr_t;
+      const double un = u + dy * *tau_w / mu_tot;
+      const double Tn = T + dy / k_tot * (*q_w - 0.5 * *tau_w * u - 0.5 * *tau_w * un);
+      const double mu_p = Pr_t * mu_l / Pr_lam;
+      i1 += dy / (mu_p + mu_t);
+      i2 += 0.5 * (u + un) * dy / (mu_p + mu_t);
+      i3 += dy / (mu_l + mu_t);
+      i4 += 0.5 * A * dy / (mu_tot * mu_tot);
+      const double dudy = (un - u) / dy, dTdy = (Tn - T) / dy;
+      y += dy;
+      u = un;
+      T = Tn;
+      dy = dmax(dy, dmin(fax * u / fabs(dudy), fax * T / fabs(dTdy)));
+    }
+    if (fabs(T - T1) / T1 < tol && (fabs(u1) < 1.0e-12 || fabs(u - u1) / u1 < tol)) break;
+    *tau_w += relax * ((u1 - u) / (i3 - sqrt(*tau_w) * i4));
+    if (*T_w < 0.0) {
+      *q_w = 0.0;
+      T_wall = (1.0 - relax) * T_wall + relax * (T1 + *tau_w * Pr_t / cp * i2);
+    } else {
+      *q_w = relax * ((cp * (T1 - *T_w) / Pr_t + *tau_w * i2) / i1) + (1.0 - relax) * *q_w;
+    }
+  }
+  if (iter == 500) return 1;
+  if (*T_w < 0.0) *T_w = -T_wall;
+  return 0;
+}
+
 /* the wall-parallel velocity of cell icv at face ibf: u - (u . n^) n^ - u_bc
  * (n^ = 0 for a degenerate face) */
 static void wm_parallel(const og_solver* s, int ibf, int icv, const double* ubc, double up[3], double unit_n[3],
@@ -1253,7 +1327,7 @@ static void wm_init(og_solver* s) {
       }
       continue;
     }
-    if (s->p.bc_type[z] != 10) continue;
+    if (s->p.bc_type[z] != 10 && s->p.bc_type[z] != 15) continue;
     const int icv = m->cvobf[ibf];
     double d[3];
     for (int i = 0; i < 3; ++i) d[i] = s->u[3 * (size_t)icv + i] - s->p.bc_u[z][i];
@@ -1263,6 +1337,10 @@ static void wm_init(og_solver* s) {
       const double* n = &m->n_bf[3 * (size_t)ibf];
       const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
       s->wm_tau[ibf] = s->mu_lam[icv] * u1 / y1;
+    }
+    if (s->p.bc_type[z] == 15) { /* the equilibrium model's T_wall (:1103-1110) */
+      const double v = m->aod_bf[ibf] > 0.0 ? u1 : 0.0;
+      s->wm_tau[m->nbf + ibf] = s->T[icv] + 0.5 * v * v / cp;
     }
   }
 }
@@ -1314,15 +1392,24 @@ static void wm_calc(og_solver* s) {
       }
       continue;
     }
-    if (s->p.bc_type[z] != 10) continue;
+    if (s->p.bc_type[z] != 10 && s->p.bc_type[z] != 15) continue;
     const int icv = m->cvobf[ibf];
     if (m->aod_bf[ibf] > 0.0) {
       double up[3], un_[3], n_mag;
       wm_parallel(s, ibf, icv, s->p.bc_u[z], up, un_, &n_mag);
       const double y1 = n_mag / m->aod_bf[ibf];
       const double u1 = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
-      const double u_tau = sqrt(s->wm_tau[ibf] / s->rho[icv]);
-      s->wm_tau[ibf] = og_wm_solve_tau(u1, y1, s->rho[icv], s->mu_lam[icv], u_tau);
+      if (s->p.bc_type[z] == 15) {
+        /* WmAdiabatic::calcRhs (:1360-1381): the equilibrium model from the
+         * previous stress and wall temperature; T_wall kept unless it converges */
+        double q_w = 0.0, Tw = -s->wm_tau[m->nbf + ibf];
+        og_wm_eq_solve_qt(&s->wm_tau[ibf], &q_w, u1, s->T[icv], s->rho[icv], y1, s->mu_lam[icv], s->p.Pr_lam, cp,
+                          &Tw, s->p.bc_sigma[z], s->p.bc_L[z], s->p.bc_q[z], 1);
+        s->wm_tau[m->nbf + ibf] = -Tw;
+      } else {
+        const double u_tau = sqrt(s->wm_tau[ibf] / s->rho[icv]);
+        s->wm_tau[ibf] = og_wm_solve_tau(u1, y1, s->rho[icv], s->mu_lam[icv], u_tau);
+      }
     } else {
       s->wm_tau[ibf] = 0.0;
     }
@@ -1468,7 +1555,7 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
         visc_work += P->bc_u[z][i] * tauijnj;
       }
       rhs[icv].rhoE -= visc_work;
-    } else if (P->bc_type[z] == 10) {
+    } else if (P->bc_type[z] == 10 || P->bc_type[z] == 15) {
       /* WmAdiabatic::addBoundaryFlux (IdealGasSolverBcs.cpp:1395-1441) */
       double up[3], unit_n[3], n_mag;
       wm_parallel(s, ibf, icv, P->bc_u[z], up, unit_n, &n_mag);
@@ -1800,13 +1887,13 @@ void og_store(og_solver* s) {
     s->store_time = s->store_time_recent;
     memcpy(s->nscbc_store, s->nscbc, sizeof(double) * nb);
     memcpy(s->sponge_store, s->sponge, sizeof(double) * 4 * (size_t)n);
-    memcpy(s->wm_store, s->wm_tau, sizeof(double) * nb / 5);
+    memcpy(s->wm_store, s->wm_tau, sizeof(double) * 2 * nb / 5);
   } else {
     memcpy(s->nscbc_store, s->nscbc_recent, sizeof(double) * nb);
     memcpy(s->sponge_store, s->sponge_recent, sizeof(double) * 4 * (size_t)n);
-    memcpy(s->wm_store, s->wm_recent, sizeof(double) * nb / 5);
+    memcpy(s->wm_store, s->wm_recent, sizeof(double) * 2 * nb / 5);
   }
-  memcpy(s->wm_recent, s->wm_tau, sizeof(double) * nb / 5); /* WmAdiabatic::storeData */
+  memcpy(s->wm_recent, s->wm_tau, sizeof(double) * 2 * nb / 5); /* WmAdiabatic::storeData: tau, T_wall */
   memcpy(s->nscbc_recent, s->nscbc, sizeof(double) * nb); /* Nscbc::storeData */
   memcpy(s->sponge_recent, s->sponge, sizeof(double) * 4 * (size_t)n); /* Sponge::storeData */
 }
@@ -1822,7 +1909,7 @@ void og_restore(og_solver* s) {
   for (int i = 0; i < n; ++i) s->rhoE[i] = s->store_buf[c++];
   memcpy(s->nscbc, s->nscbc_store, sizeof(double) * 5 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
   memcpy(s->sponge, s->sponge_store, sizeof(double) * 4 * (size_t)n);
-  memcpy(s->wm_tau, s->wm_store, sizeof(double) * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
+  memcpy(s->wm_tau, s->wm_store, sizeof(double) * 2 * (size_t)(s->m.nbf > 0 ? s->m.nbf : 0));
   og_update_state(s);
   og_sgs_props(s);
   s->p.dt_data[0] *= 0.8;

@@ -154,6 +154,11 @@ double og_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau
  * heat flux of cell temperature T1 toward Tw for stress tau_w (13
  * WM_ALG_ISOTHERMAL faces; locp = loc_lam, the conductivity over cp) */
 double og_wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu, double locp, double cp, double y1);
+/* EquilibriumWM::solve_qt (wm/EquilibriumWM.hpp:36-205): the equilibrium wall
+ * model (15 WM_EQ_ADIABATIC faces: relax in bc_sigma, y_plus0 in bc_L, tol in
+ * bc_q); *T_w < 0 adiabatic with guess -*T_w; returns 0 when converged */
+int og_wm_eq_solve_qt(double* tau_w, double* q_w, double u1, double T1, double rho1, double y1, double mu1,
+                      double Pr_lam, double cp, double* T_w, double relax, double y_plus0, double tol, int use_guess);
 /* calcNscbcIgSubsonicInletRhs (BcCommon.hpp:311-337): rhs[0] of an NSCBC
  * mass-flow inlet face (12 NSCBC_MT: mdot in bc_q, T in bc_T); -1 when u0 + c < 0 */
 int og_nscbc_inlet_rhs(double rhs[5], const double unit_n[3], double delta, const double bc[5], const double cv[5],

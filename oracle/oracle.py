@@ -121,6 +121,8 @@ def lib(path=None):
         L.og_wm_solve_tau.restype = C.c_double
         L.og_wm_solve_tau.argtypes = [C.c_double] * 5
         L.og_wm_q_wall.restype = C.c_double
+        L.og_wm_eq_solve_qt.restype = C.c_int
+        L.og_wm_eq_solve_qt.argtypes = [dp, dp] + [C.c_double] * 7 + [dp] + [C.c_double] * 3 + [C.c_int]
         L.og_wm_q_wall.argtypes = [C.c_double] * 8
         _libs[path] = L
     return _libs[path]
@@ -151,6 +153,8 @@ def ref():
         R.ref_wm_solve_tau.restype = C.c_double
         R.ref_wm_solve_tau.argtypes = [C.c_double] * 5
         R.ref_wm_q_wall.restype = C.c_double
+        R.ref_wm_eq_solve_qt.restype = C.c_int
+        R.ref_wm_eq_solve_qt.argtypes = [dp, dp] + [C.c_double] * 7 + [dp] + [C.c_double] * 3 + [C.c_int]
         R.ref_wm_q_wall.argtypes = [C.c_double] * 8
         _ref = R
     return _ref

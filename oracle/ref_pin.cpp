@@ -17,6 +17,7 @@
 #include "RkWeights.hpp"
 #include "SvdNew.hpp"
 #include "AlgebraicWM.hpp"
+#include "EquilibriumWM.hpp"
 
 #include <cstring>
 
@@ -134,6 +135,12 @@ int ref_nscbc_inlet_rhs(double* rhs, const double* unit_n, double delta, const d
 // wall stress of the algebraic wall model
 double ref_wm_solve_tau(double u1, double y1, double rho, double mu, double u_tau_guess) {
   return AlgebraicWM::solve_tau(u1, y1, rho, mu, u_tau_guess);
+}
+
+int ref_wm_eq_solve_qt(double* tau_w, double* q_w, double u1, double T1, double rho1, double y1, double mu1,
+                       double Pr_lam, double cp, double* T_w, double relax, double y_plus0, double tol, int use_guess) {
+  return EquilibriumWM::solve_qt(*tau_w, *q_w, u1, T1, rho1, y1, mu1, Pr_lam, cp, *T_w, relax, y_plus0, tol,
+                                 use_guess != 0);
 }
 
 double ref_wm_q_wall(double T1, double Tw, double tau_w, double rho, double mu, double locp, double cp, double y1) {

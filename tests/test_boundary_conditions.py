@@ -447,10 +447,19 @@ def test_wall_model_parser():
             parse_params(bad + "\n")
 
 
-def test_wall_model_starts_with_the_laminar_stress():
+def test_equilibrium_wall_model_parser():
+    p = parse_params(deck({"y0": "WM_EQ_ADIABATIC", "y1": "WM_EQ_ADIABATIC TRANSLATE 1 0 0.5 RELAX 0.5 Y_PLUS0 2 TOL 1e-5"}))
+    assert (p.bc[0].type, p.bc[0].sigma, p.bc[0].L, p.bc[0].q) == (15, 0.7, 1.0, 1e-4)
+    assert (p.bc[1].type, list(p.bc[1].u), p.bc[1].sigma, p.bc[1].L, p.bc[1].q) == (15, [1.0, 0.0, 0.5], 0.5, 2.0, 1e-5)
+    with pytest.raises(CharlesError):
+        parse_params("y0 WM_EQ_ADIABATIC TOL\n")
+
+
+@pytest.mark.parametrize("kind", ["WM_ALG_ADIABATIC", "WM_EQ_ADIABATIC"])
+def test_wall_model_starts_with_the_laminar_stress(kind):
     m = Mesh.channel(10, jitter=0.0)
     U = 0.7
-    o = oracle(m, deck({"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.25 0 0"}), uniform(m, (U, 0, 0)))
+    o = oracle(m, deck({"y0": kind, "y1": f"{kind} TRANSLATE 0.25 0 0"}), uniform(m, (U, 0, 0)))
     mu = o.get_field("mu_lam")
     r = o.calc_rhs()
     lower, upper = wall_cells(m, 0), wall_cells(m, 1)

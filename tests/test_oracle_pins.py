@@ -351,6 +351,57 @@ def test_nscbc_mdot_rhs_closed_forms():
     np.testing.assert_allclose(rm, rp, rtol=1e-12, atol=1e-12 * np.abs(rp).max())
 
 
+def _eq_case(L_or_R, fn, case):
+    tau, q, Tw = C.c_double(case["tau"]), C.c_double(0.0), C.c_double(case["Tw"])
+    rc = getattr(L_or_R, fn)(C.byref(tau), C.byref(q), case["u1"], case["T1"], case["rho"], case["y1"], case["mu"],
+                             0.7, 1004.5, C.byref(Tw), case["relax"], case["yp0"], case["tol"], case["guess"])
+    return rc, tau.value, q.value, Tw.value
+
+
+def _eq_cases(rng, n):
+    # viscous sublayer to the log layer (y1+ ~ 0.5 .. 3e3), adiabatic walls with
+    # and without a guess of their temperature, isothermal walls hotter and
+    # colder than the cell, u1 = 0
+    for k in range(n):
+        rho, mu, T1 = rng.uniform(0.5, 2.0), 10.0 ** rng.uniform(-5.5, -4), rng.uniform(250.0, 400.0)
+        y1 = 10.0 ** rng.uniform(-4, -2)
+        u1 = 0.0 if k % 23 == 0 else 10.0 ** rng.uniform(-1, 2)
+        guess = k % 3 != 0
+        Tw = -(T1 + rng.uniform(0.0, 20.0)) if k % 4 != 3 else T1 + rng.uniform(-60.0, 60.0)
+        yield dict(tau=mu * u1 / y1 * rng.uniform(0.5, 3.0), Tw=Tw, u1=u1, T1=T1, rho=rho, y1=y1, mu=mu,
+                   relax=(0.7, 0.5)[k % 2], yp0=(1.0, 0.5)[k % 2], tol=(1e-4, 1e-6)[k % 2], guess=int(guess))
+
+
+@needs_ref
+def test_equilibrium_wall_model_bit_identical_to_reference():
+    L, R = orc.lib(), orc.ref()
+    rng = np.random.default_rng(43)
+    for case in _eq_cases(rng, 600):
+        assert _eq_case(L, "og_wm_eq_solve_qt", case) == _eq_case(R, "ref_wm_eq_solve_qt", case), case
+
+
+def test_equilibrium_wall_model_closed_forms():
+    # thin viscous layer (y1+ ~ 0.1, the damped eddy viscosity negligible) and
+    # nearly constant properties: plane Couette flow. Stress mu u1 / y1; an
+    # adiabatic wall recovers T1 + Pr u1^2 / (2 cp) (recovery factor Pr); an
+    # isothermal wall takes q_w = k (T1 - T_w) / y1 + tau u1 / 2, k = cp mu / Pr
+    L = orc.lib()
+    rho, mu, T1, cp, Pr = 1.2, 1.8e-5, 300.0, 1004.5, 0.7
+    y1, u1 = 1e-5, 5.0
+    case = dict(tau=mu * u1 / y1, Tw=-T1, u1=u1, T1=T1, rho=rho, y1=y1, mu=mu, relax=0.7, yp0=1.0, tol=1e-11,
+                guess=0)
+    rc, tau, q, Tw = _eq_case(L, "og_wm_eq_solve_qt", case)
+    assert rc == 0 and q == 0.0
+    np.testing.assert_allclose(tau, mu * u1 / y1, rtol=2e-3)
+    np.testing.assert_allclose(-Tw - T1, Pr * u1 ** 2 / (2 * cp), rtol=2e-2)
+    case.update(Tw=T1 - 0.5)
+    rc, tau, q, Tw = _eq_case(L, "og_wm_eq_solve_qt", case)
+    assert rc == 0 and Tw == T1 - 0.5
+    np.testing.assert_allclose(q, cp * mu / Pr * 0.5 / y1 + tau * u1 / 2, rtol=5e-3)
+    case.update(u1=0.0, Tw=-T1)
+    assert _eq_case(L, "og_wm_eq_solve_qt", case)[:2] == (0, 0.0)
+
+
 def test_nscbc_outlet_rhs_closed_forms():
     # at rest with the face state equal to its cell's, only the incoming
     # characteristic acts: L1 = sigma c (p_bc - p_ref) / L, so dp_bc/dt =
