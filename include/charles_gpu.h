@@ -156,6 +156,12 @@ typedef struct cg_solver cg_solver;
  *                                    p n plus tau_wall along u, q_wall area out
  *                                    of the energy; RELAX, Y_PLUS0, TOL are
  *                                    ignored as for WM_ALG_ADIABATIC
+ *   WM_EQ_ISOTHERMAL <T> [RELAX r] [Y_PLUS0 y] [TOL t]
+ *                                    WmIsothermal with WM_EQUILIBRIUM_ISOTHERMAL:
+ *                                    stress and heat flux from solve_qt toward
+ *                                    the wall at T each step (RELAX, Y_PLUS0,
+ *                                    TOL as for WM_EQ_ADIABATIC); like the
+ *                                    reference, neither is in the NaN snapshots
  * The other types of the reference are refused by cg_params_parse. */
 #define CG_MAX_BC_ZONES 16
 #define CG_BC_SLIP 1
@@ -173,6 +179,7 @@ typedef struct cg_solver cg_solver;
 #define CG_BC_WM_ALG_ISOTHERMAL 13
 #define CG_BC_NSCBC_OUTLET_MDOT 14
 #define CG_BC_WM_EQ_ADIABATIC 15
+#define CG_BC_WM_EQ_ISOTHERMAL 16
 typedef struct cg_bc {
   char zone[48];  /* boundary zone name of the mesh */
   int type;       /* CG_BC_* */

@@ -150,6 +150,7 @@ struct HostInst {
   bool wm = false;                   // some zone is a WM_ALG_ADIABATIC or WM_ALG_ISOTHERMAL wall
   bool wm_iso = false;               // some zone is a WM_ALG_ISOTHERMAL wall
   bool wm_eq = false;                // some zone is a WM_EQ_ADIABATIC wall (T_wall follows tau_wall)
+  bool wm_eqiso = false;             // some zone is a WM_EQ_ISOTHERMAL wall (its stress follows q_wall)
   bool zsum = false;                 // some zone needs zone sums (NSCBC_OUTLET_MDOT, USE_AVG, T_TOTAL)
   bool tpt = false;                  // some zone is a CBC_TOTAL_PT inlet
   bool sponge = false;               // some zone is a sponge
@@ -936,9 +937,10 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
       hi->P.bc_mode[z] = (p->bc[k].type == CG_BC_CBC_TOTAL_PT || p->bc[k].type == CG_BC_NSCBC_MT) ? p->bc[k].mode : 0;
       hi->zsum = hi->zsum || p->bc[k].type == CG_BC_NSCBC_OUTLET_MDOT || hi->P.bc_mode[z] != 0;
       hi->wm = hi->wm || p->bc[k].type == CG_BC_WM_ALG_ADIABATIC || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL ||
-               p->bc[k].type == CG_BC_WM_EQ_ADIABATIC;
+               p->bc[k].type == CG_BC_WM_EQ_ADIABATIC || p->bc[k].type == CG_BC_WM_EQ_ISOTHERMAL;
       hi->wm_eq = hi->wm_eq || p->bc[k].type == CG_BC_WM_EQ_ADIABATIC;
-      hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL;
+      hi->wm_eqiso = hi->wm_eqiso || p->bc[k].type == CG_BC_WM_EQ_ISOTHERMAL;
+      hi->wm_iso = hi->wm_iso || p->bc[k].type == CG_BC_WM_ALG_ISOTHERMAL || p->bc[k].type == CG_BC_WM_EQ_ISOTHERMAL;
       hi->tpt = hi->tpt || p->bc[k].type == CG_BC_CBC_TOTAL_PT;
       if (p->bc[k].type == CG_BC_SPONGE) {
         // Sponge (IdealGasSolverBcs.cpp:4080-4128): its cells' T_sponge and
@@ -1015,7 +1017,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     // WmAdiabatic::storeData, WmIsothermal::storeData: the faces' tau_wall
     // (in the snapshots too); q_wall is not stored
     rc |= dev_alloc(s, h, &w.wm_tau, (size_t)lay->g.nbf * (hi->wm_eq ? 2 : 1));
-    if (hi->wm_iso) rc |= dev_alloc(s, h, &w.wm_q, (size_t)lay->g.nbf);
+    if (hi->wm_iso) rc |= dev_alloc(s, h, &w.wm_q, (size_t)lay->g.nbf * (hi->wm_eqiso ? 2 : 1));
     hi->snap_wm = (size_t)lay->g.nbf * (hi->wm_eq ? 2 : 1);
     hi->snap_n += hi->snap_wm;
   }
@@ -1101,7 +1103,7 @@ int build_instance(Err* s, Instance<R>* I, std::shared_ptr<Layout<R>> lay, const
     rc |= cgk::launch_fill(w.nscbc_rhs, 0.0, 15 * (int64_t)lay->g.nbf, stream);
   }
   if (hi->wm) rc |= cgk::launch_fill(w.wm_tau, 0.0, (int64_t)hi->snap_wm, stream);  // set by initComplete
-  if (hi->wm_iso) rc |= cgk::launch_fill(w.wm_q, 0.0, (int64_t)lay->g.nbf, stream);
+  if (hi->wm_iso) rc |= cgk::launch_fill(w.wm_q, 0.0, (int64_t)lay->g.nbf * (hi->wm_eqiso ? 2 : 1), stream);
   if (hi->tpt) rc |= cgk::launch_fill(w.tpt_un, 0.0, (int64_t)lay->g.nbf, stream);
   if (hi->sponge) {
     // the relaxed state is set from the cells by initComplete
@@ -1816,19 +1818,24 @@ int cg_params_parse(cg_params* p, const char* text) {
             return bad("unknown " + type + " option " + opt);
           }
         }
-      } else if (type == "WM_ALG_ISOTHERMAL") {
+      } else if (type == "WM_ALG_ISOTHERMAL" || type == "WM_EQ_ISOTHERMAL") {
         // WmIsothermal::initData (IdealGasSolverBcs.cpp:2054-2089): the wall
-        // temperature, then RELAX, Y_PLUS0 and TOL, the equilibrium model's,
-        // with a value each
-        b.type = CG_BC_WM_ALG_ISOTHERMAL;
-        if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected WM_ALG_ISOTHERMAL <T>");
+        // temperature, then RELAX, Y_PLUS0 and TOL with a value each, the
+        // equilibrium model's (sigma, L, q; defaults 0.7, 1, 1e-4)
+        const bool eq = type == "WM_EQ_ISOTHERMAL";
+        b.type = eq ? CG_BC_WM_EQ_ISOTHERMAL : CG_BC_WM_ALG_ISOTHERMAL;
+        b.sigma = 0.7;
+        b.L = 1.0;
+        b.q = 1.0e-4;
+        if (!num(2, &b.T) || !(b.T > 0.0)) return bad("expected " + type + " <T>");
         for (size_t i = 3; i < tok.size(); ++i) {
           const std::string opt = upper(tok[i]);
           double ignored = 0.0;
           if (opt == "RELAX" || opt == "Y_PLUS0" || opt == "TOL") {
-            if (!num(++i, &ignored)) return bad("expected " + opt + " <value>");
+            double* dst = !eq ? &ignored : opt == "RELAX" ? &b.sigma : opt == "Y_PLUS0" ? &b.L : &b.q;
+            if (!num(++i, dst)) return bad("expected " + opt + " <value>");
           } else {
-            return bad("unknown WM_ALG_ISOTHERMAL option " + opt);
+            return bad("unknown " + type + " option " + opt);
           }
         }
       } else if (type == "WALL_ISOTHERMAL") {
@@ -1957,7 +1964,7 @@ int cg_params_parse(cg_params* p, const char* text) {
         if (b.mode < 0) return bad("TYPE: unspecified");
       } else {
         return bad("boundary condition " + type + " is not available (SLIP, SYMMETRY, WALL_ADIABATIC, "
-                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_EQ_ADIABATIC, WM_ALG_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
+                   "WALL_ISOTHERMAL, WM_ALG_ADIABATIC, WM_EQ_ADIABATIC, WM_ALG_ISOTHERMAL, WM_EQ_ISOTHERMAL, CBC_UPT, CBC_MPT, CBC_RUNP, CBC_RUP, CBC_TOTAL_PT, "
                    "NSCBC_MT, NSCBC_OUTLET_P, "
                    "NSCBC_OUTLET_PRESSURE, NSCBC_OUTLET_MDOT and SPONGE are)");
       }

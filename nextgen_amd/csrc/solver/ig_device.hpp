@@ -156,7 +156,9 @@ struct WorkT {
   // with a WM_EQ_ADIABATIC zone followed by each face's adiabatic wall
   // temperature T_wall, [2][nbf], both snapshotted as WmAdiabatic::storeData) and, with
   // an isothermal zone, its wall heat flux q_wall (FP64, not in the snapshots,
-  // as in the reference: each step's stage 1 sets it before it is used)
+  // as in the reference: each step's stage 1 sets it before it is used); with a
+  // WM_EQ_ISOTHERMAL zone wm_q is [2][nbf], its faces' stress in the second half
+  // (WmIsothermal::storeData stores nothing for that model)
   double *wm_tau, *wm_q;
   // CBC_TOTAL_PT faces (null without them): per boundary face its inflow
   // normal velocity un_bc (FP64; not in the snapshots, as in the reference)
@@ -173,7 +175,8 @@ enum {
   BC_NONE = 0, BC_SLIP = 1, BC_WALL_ADIABATIC = 2, BC_WALL_ISOTHERMAL = 3,
   BC_CBC_UPT = 4, BC_CBC_MPT = 5, BC_CBC_RUNP = 6, BC_CBC_RUP = 7, BC_NSCBC_OUTLET_P = 8, BC_SPONGE = 9,
   BC_WM_ALG_ADIABATIC = 10, BC_CBC_TOTAL_PT = 11, BC_NSCBC_MT = 12,
-  BC_WM_ALG_ISOTHERMAL = 13, BC_NSCBC_OUTLET_MDOT = 14, BC_WM_EQ_ADIABATIC = 15
+  BC_WM_ALG_ISOTHERMAL = 13, BC_NSCBC_OUTLET_MDOT = 14, BC_WM_EQ_ADIABATIC = 15,
+  BC_WM_EQ_ISOTHERMAL = 16
 };
 // the zones whose faces carry a mass flux (the Riemann flux against a boundary state)
 __host__ __device__ inline bool bc_riemann(int type) {

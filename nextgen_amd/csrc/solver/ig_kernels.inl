@@ -922,7 +922,7 @@ __device__ __forceinline__ void wm_iso_face(const GeomT<R>& g, const WorkT<R>& w
   cell_state64(g, st, P, c, self, rho, u, rhoE, spv, p, h, T);
   const double n[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
   const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  const auto tw = G(w.wm_tau) + b;
+  const auto tw = P.bc_type[z] == BC_WM_EQ_ISOTHERMAL ? G(w.wm_q) + g.nbf + b : G(w.wm_tau) + b;
   const auto qw = G(w.wm_q) + b;
   double u_mag = 0.0;
   for (int i = 0; i < 3; ++i) u_mag += u[i] * u[i];
@@ -999,7 +999,7 @@ __device__ __forceinline__ void boundary_flux(const GeomT<R>& g, const WorkT<R>&
       acc[4] -= visc_work;
     } else if (type == BC_WM_ALG_ADIABATIC || type == BC_WM_EQ_ADIABATIC) {
       wm_face(g, w, st, P, c, b, z, self, acc);
-    } else if (type == BC_WM_ALG_ISOTHERMAL) {
+    } else if (type == BC_WM_ALG_ISOTHERMAL || type == BC_WM_EQ_ISOTHERMAL) {
       wm_iso_face(g, w, st, P, c, b, z, self, acc);
     } else if (bc_riemann(type)) {
       // Cbc::addBoundaryFlux (IdealGasSolverBcs.cpp:2960-2976): the Riemann flux
@@ -2435,7 +2435,7 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
       }
       continue;
     }
-    if (X.P.bc_type[z] == BC_WM_ALG_ISOTHERMAL) {
+    if (X.P.bc_type[z] == BC_WM_ALG_ISOTHERMAL || X.P.bc_type[z] == BC_WM_EQ_ISOTHERMAL) {
       // WmIsothermal::initialHook (IdealGasSolverBcs.cpp:2090-2112): the
       // laminar stress mu_lam |u| / y1 and heat flux mu_lam (cp T + |u|^2 / 2
       // - cp T_wall) / y1 (0 where area_over_delta is 0)
@@ -2454,8 +2454,9 @@ __global__ __launch_bounds__(kBlock) void k_nscbc_init(const Inst<R>* __restrict
         tau = mu * sqrt(uu) / y1;
         q = mu * (cp * T + 0.5 * uu - cp * X.P.bc_T[z]) / y1;
       }
-      G(X.w.wm_tau)[b] = tau;
       G(X.w.wm_q)[b] = q;
+      if (X.P.bc_type[z] == BC_WM_EQ_ISOTHERMAL) G(X.w.wm_q)[g.nbf + b] = tau;
+      else G(X.w.wm_tau)[b] = tau;
       continue;
     }
     if (!bc_nscbc(X.P.bc_type[z])) continue;
@@ -2716,7 +2717,10 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
   if (b >= g.nbf) return;
   const int z = G(g.bf_zone)[b];
   const int type = P.bc_type[z];
-  if (type != BC_WM_ALG_ADIABATIC && type != BC_WM_ALG_ISOTHERMAL && type != BC_WM_EQ_ADIABATIC) return;
+  if (type != BC_WM_ALG_ADIABATIC && type != BC_WM_ALG_ISOTHERMAL && type != BC_WM_EQ_ADIABATIC &&
+      type != BC_WM_EQ_ISOTHERMAL)
+    return;
+  const bool iso = type == BC_WM_ALG_ISOTHERMAL || type == BC_WM_EQ_ISOTHERMAL;
   const int c = G(g.bf_cell)[b];
   double rho, u[3], rhoE, spv, p, h, T;
   const Light<R> self = load_light(X.st[cur].light, c);
@@ -2724,13 +2728,13 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
   const double n[3] = {G(g.bfn0)[b], G(g.bfn1)[b], G(g.bfn2)[b]};
   const double n_mag = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   const double aod = G(g.bf_aod)[b];
-  const auto tw = G(X.w.wm_tau) + b;
+  const auto tw = type == BC_WM_EQ_ISOTHERMAL ? G(X.w.wm_q) + g.nbf + b : G(X.w.wm_tau) + b;
   double tau = 0.0, q = 0.0;
   if (aod > 0.0) {
     const double y1 = n_mag / aod;
     const double mu = G(X.w.mu_lam)[c];
     double u1;
-    if (type != BC_WM_ALG_ISOTHERMAL) {
+    if (!iso) {
       const double* ubc = P.bc_u[z];
       double u_par[3];
       if (n_mag > 0.0) {
@@ -2752,6 +2756,13 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
       tau = *tw;
       eq_solve_qt(tau, q_w, u1, T, rho, y1, mu, P.Pr_lam, cp, Tw, P.bc_sigma[z], P.bc_L[z], P.bc_q[z], true);
       G(X.w.wm_tau)[g.nbf + b] = -Tw;
+    } else if (type == BC_WM_EQ_ISOTHERMAL) {
+      // WmIsothermal::calcRhs (:2194-2199): from the previous stress and heat flux
+      const double cp = P.R_gas * P.gamma / (P.gamma - 1.0);
+      double Tw = P.bc_T[z];
+      tau = *tw;
+      q = G(X.w.wm_q)[b];
+      eq_solve_qt(tau, q, u1, T, rho, y1, mu, P.Pr_lam, cp, Tw, P.bc_sigma[z], P.bc_L[z], P.bc_q[z], true);
     } else {
       tau = wm_solve_tau(u1, y1, rho, mu, sqrt(*tw / rho));
     }
@@ -2761,7 +2772,7 @@ __global__ __launch_bounds__(kBlock) void k_wm_solve(const Inst<R>* __restrict__
     }
   }
   *tw = tau;
-  if (type == BC_WM_ALG_ISOTHERMAL) G(X.w.wm_q)[b] = q;
+  if (iso) G(X.w.wm_q)[b] = q;
 }
 
 // The sums over a zone that some conditions need before a stage, for each

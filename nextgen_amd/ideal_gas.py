@@ -36,7 +36,7 @@ CG_MAX_BC_ZONES = 16
 CG_BC_SLIP, CG_BC_WALL_ADIABATIC, CG_BC_WALL_ISOTHERMAL = 1, 2, 3
 CG_BC_CBC_UPT, CG_BC_CBC_MPT, CG_BC_CBC_RUNP, CG_BC_CBC_RUP = 4, 5, 6, 7
 CG_BC_NSCBC_OUTLET_P, CG_BC_SPONGE, CG_BC_WM_ALG_ADIABATIC, CG_BC_CBC_TOTAL_PT, CG_BC_NSCBC_MT = 8, 9, 10, 11, 12
-CG_BC_WM_ALG_ISOTHERMAL, CG_BC_NSCBC_OUTLET_MDOT, CG_BC_WM_EQ_ADIABATIC = 13, 14, 15
+CG_BC_WM_ALG_ISOTHERMAL, CG_BC_NSCBC_OUTLET_MDOT, CG_BC_WM_EQ_ADIABATIC, CG_BC_WM_EQ_ISOTHERMAL = 13, 14, 15, 16
 
 
 class cg_bc(C.Structure):

@@ -195,6 +195,7 @@ struct og_solver {
    * two snapshots */
   double *wm_tau, *wm_store, *wm_recent;
   double* wm_q; /* WM_ALG_ISOTHERMAL q_wall (not stored, as WmIsothermal::storeData) */
+  double* wm_tq; /* WM_EQ_ISOTHERMAL tau_wall (not stored either: that model's storeData keeps nothing) */
   /* CbcTotalPt: un_bc per boundary face (not part of storeData) */
   double* tpt_un;
 };
@@ -247,6 +248,7 @@ og_solver* og_create(const og_mesh* m, const og_params* p) {
   s->mf_bf = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_tau = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double)); /* tau, T_wall */
   s->wm_q = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
+  s->wm_tq = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->tpt_un = (double*)calloc((size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_store = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
   s->wm_recent = (double*)calloc(2 * (size_t)(m->nbf > 0 ? m->nbf : 0) + 1, sizeof(double));
@@ -272,7 +274,7 @@ void og_destroy(og_solver* s) {
   free(s->cp_i); free(s->cp_v); free(s->pair_buf);
   free(s->nscbc); free(s->nscbc_rhs); free(s->nscbc_store); free(s->nscbc_recent);
   free(s->sponge); free(s->sponge_store); free(s->sponge_recent); free(s->mf_bf);
-  free(s->wm_tau); free(s->wm_q); free(s->wm_store); free(s->wm_recent); free(s->tpt_un);
+  free(s->wm_tau); free(s->wm_q); free(s->wm_tq); free(s->wm_store); free(s->wm_recent); free(s->tpt_un);
   free(s);
 }
 
@@ -1313,16 +1315,17 @@ static void wm_init(og_solver* s) {
   const double cp = s->R_gas * s->p.gamma / (s->p.gamma - 1.0);
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
-    if (s->p.bc_type[z] == 13) {
+    if (s->p.bc_type[z] == 13 || s->p.bc_type[z] == 16) {
       const int icv = m->cvobf[ibf];
       const double* u = &s->u[3 * (size_t)icv];
       const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-      s->wm_tau[ibf] = 0.0;
+      double* tw = s->p.bc_type[z] == 16 ? &s->wm_tq[ibf] : &s->wm_tau[ibf];
+      *tw = 0.0;
       s->wm_q[ibf] = 0.0;
       if (m->aod_bf[ibf] > 0.0) {
         const double* n = &m->n_bf[3 * (size_t)ibf];
         const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
-        s->wm_tau[ibf] = s->mu_lam[icv] * sqrt(uu) / y1;
+        *tw = s->mu_lam[icv] * sqrt(uu) / y1;
         s->wm_q[ibf] = s->mu_lam[icv] * (cp * s->T[icv] + 0.5 * uu - cp * s->p.bc_T[z]) / y1;
       }
       continue;
@@ -1376,6 +1379,24 @@ static void wm_calc(og_solver* s) {
   const double cp = s->R_gas * s->p.gamma / (s->p.gamma - 1.0);
   for (int ibf = 0; ibf < m->nbf; ++ibf) {
     const int z = m->zone_bf[ibf];
+    if (s->p.bc_type[z] == 16) {
+      /* WmIsothermal::calcRhs with the equilibrium model (:2194-2199), from
+       * the previous stress and heat flux */
+      const int icv = m->cvobf[ibf];
+      if (m->aod_bf[ibf] > 0.0) {
+        const double* n = &m->n_bf[3 * (size_t)ibf];
+        const double* u = &s->u[3 * (size_t)icv];
+        const double y1 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) / m->aod_bf[ibf];
+        const double u1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double Tw = s->p.bc_T[z];
+        og_wm_eq_solve_qt(&s->wm_tq[ibf], &s->wm_q[ibf], u1, s->T[icv], s->rho[icv], y1, s->mu_lam[icv],
+                          s->p.Pr_lam, cp, &Tw, s->p.bc_sigma[z], s->p.bc_L[z], s->p.bc_q[z], 1);
+      } else {
+        s->wm_tq[ibf] = 0.0;
+        s->wm_q[ibf] = 0.0;
+      }
+      continue;
+    }
     if (s->p.bc_type[z] == 13) {
       const int icv = m->cvobf[ibf];
       s->wm_tau[ibf] = 0.0;
@@ -1571,7 +1592,7 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
       } else {
         for (int i = 0; i < 3; ++i) rhs[icv].rhou[i] -= p * n[i];
       }
-    } else if (P->bc_type[z] == 13) {
+    } else if (P->bc_type[z] == 13 || P->bc_type[z] == 16) {
       /* WmIsothermal::addBoundaryFlux (IdealGasSolverBcs.cpp:2208-2226): the
        * stress along the full velocity, the heat flux out of the energy */
       const double area = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -1580,7 +1601,7 @@ static void boundary_flux(og_solver* s, og_rhs* rhs) {
       u_mag = sqrt(u_mag > 0.0 ? u_mag : 0.0);
       for (int i = 0; i < 3; ++i) {
         rhs[icv].rhou[i] -= p * n[i];
-        if (u_mag > 0.0) rhs[icv].rhou[i] -= s->wm_tau[ibf] * u[i] / u_mag * area;
+        if (u_mag > 0.0) rhs[icv].rhou[i] -= (P->bc_type[z] == 16 ? s->wm_tq : s->wm_tau)[ibf] * u[i] / u_mag * area;
       }
       rhs[icv].rhoE -= s->wm_q[ibf] * area;
     } else if (bc_riemann(P->bc_type[z])) {

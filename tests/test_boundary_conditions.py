@@ -480,11 +480,18 @@ def test_isothermal_wall_model_parser():
             parse_params(bad + "\n")
 
 
-def test_isothermal_wall_model_starts_laminar():
+def test_equilibrium_isothermal_wall_model_parser():
+    p = parse_params(deck({"y0": "WM_EQ_ISOTHERMAL 300", "y1": "WM_EQ_ISOTHERMAL 310 RELAX 0.5 Y_PLUS0 2 TOL 1e-6"}))
+    assert (p.bc[0].type, p.bc[0].T, p.bc[0].sigma, p.bc[0].L, p.bc[0].q) == (16, 300.0, 0.7, 1.0, 1e-4)
+    assert (p.bc[1].T, p.bc[1].sigma, p.bc[1].L, p.bc[1].q) == (310.0, 0.5, 2.0, 1e-6)
+
+
+@pytest.mark.parametrize("kind", ["WM_ALG_ISOTHERMAL", "WM_EQ_ISOTHERMAL"])
+def test_isothermal_wall_model_starts_laminar(kind):
     m = Mesh.channel(10, jitter=0.0)
     U, R, gamma = 0.7, 71.42857142857143, 1.4
     cp = R * gamma / (gamma - 1.0)
-    o = oracle(m, deck({"y0": "WM_ALG_ISOTHERMAL 1.1", "y1": "WM_ALG_ISOTHERMAL 0.9"}), uniform(m, (U, 0, 0)))
+    o = oracle(m, deck({"y0": f"{kind} 1.1", "y1": f"{kind} 0.9"}), uniform(m, (U, 0, 0)))
     mu = o.get_field("mu_lam")
     r = o.calc_rhs()
     for zone, Tw in ((0, 1.1), (1, 0.9)):
@@ -494,10 +501,11 @@ def test_isothermal_wall_model_starts_laminar():
     assert o.step() == 0
 
 
-def test_isothermal_wall_model_at_rest_conducts_like_the_isothermal_wall():
+@pytest.mark.parametrize("model", ["WM_ALG_ISOTHERMAL", "WM_EQ_ISOTHERMAL"])
+def test_isothermal_wall_model_at_rest_conducts_like_the_isothermal_wall(model):
     m = Mesh.channel(10, jitter=0.3, seed=6)
     got = {}
-    for kind in ("WM_ALG_ISOTHERMAL", "WALL_ISOTHERMAL"):
+    for kind in (model, "WALL_ISOTHERMAL"):
         o = oracle(m, deck({"y0": f"{kind} 1.1", "y1": f"{kind} 0.9"}), uniform(m))
 
         def keep(rhs, time, stage, kind=kind):
@@ -505,7 +513,7 @@ def test_isothermal_wall_model_at_rest_conducts_like_the_isothermal_wall():
                 got[kind] = rhs.copy()
         o.set_source_hook(keep)
         assert o.step() == 0
-    a, b = got["WM_ALG_ISOTHERMAL"], got["WALL_ISOTHERMAL"]
+    a, b = got[model], got["WALL_ISOTHERMAL"]
     assert np.abs(b[:, 4]).max() > 0.0
     np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
 

@@ -8,7 +8,8 @@ states evolved with the step, restored with it after a NaN) and mass-flow
 outlet NSCBC_OUTLET_MDOT (its zone sums before each stage, also in a batch), the algebraic
 wall model WM_ALG_ADIABATIC (per-face stress from the law of the wall) and
 its isothermal form WM_ALG_ISOTHERMAL (with the thermal law's heat flux), the
-equilibrium wall model WM_EQ_ADIABATIC (per-face stress and wall temperature), the
+equilibrium wall models WM_EQ_ADIABATIC (per-face stress and wall temperature) and
+WM_EQ_ISOTHERMAL (stress and heat flux, outside the snapshots as in the reference), the
 total-pressure inlet CBC_TOTAL_PT, the NSCBC mass-flow inlet NSCBC_MT (each also
 with its zone-averaged form, USE_AVG and T_TOTAL) and the SPONGE
 (per-cell relaxed T, u, a volume term over the last LENGTH of the domain, the
@@ -71,7 +72,8 @@ WALLS = {
     "wm+wm_translate": {"y0": "WM_ALG_ADIABATIC", "y1": "WM_ALG_ADIABATIC TRANSLATE 0.3 0 0.1"},
     # the equilibrium wall model on both walls, one of them moving
     "wmeq+wmeq_translate": {"y0": "WM_EQ_ADIABATIC", "y1": "WM_EQ_ADIABATIC TRANSLATE 0.3 0 0.1 RELAX 0.6"},
-    # the isothermal wall model, one wall hotter and one colder than the gas
+    # the isothermal wall models, one wall hotter and one colder than the gas
+    "wmeqiso+wmeqiso": {"y0": "WM_EQ_ISOTHERMAL 1.05", "y1": "WM_EQ_ISOTHERMAL 0.97 RELAX 0.5"},
     "wmiso+wmiso": {"y0": "WM_ALG_ISOTHERMAL 1.05", "y1": "WM_ALG_ISOTHERMAL 0.97 RELAX 0.5"},
     # a radial sponge as the first zone (its volume term before the other zone's faces)
     "sponge_rx+adiabatic": {"y0": "SPONGE TYPE L_RX LENGTH 3.0 STRENGTH 8.0 RELAX_TIME 0.2",
@@ -232,8 +234,9 @@ def test_orifice_mesh_with_nscbc_outlet_matches_oracle():
     check_history(g, o, 20, "orifice nscbc")
 
 
-# (not CBC_TOTAL_PT: its un_bc is not in the snapshots, as in the reference, so a
-# NaN that reaches it stays, and with USE_AVG any NaN in an inlet cell does)
+# (not CBC_TOTAL_PT or WM_EQ_ISOTHERMAL: their face states are not in the
+# snapshots, as in the reference, so a NaN that reaches them stays and every
+# retried step fails again)
 @pytest.mark.parametrize("walls", ["cbc_upt+nscbc", "cbc_upt+sponge", "wm+wm_translate", "nscbc_mt+nscbc",
                                    "wmiso+wmiso", "cbc_upt+mdot", "nscbc_mt_t0+nscbc", "wmeq+wmeq_translate"])
 def test_boundary_states_roll_back_with_the_flow(walls):
@@ -269,7 +272,8 @@ def test_boundary_states_roll_back_with_the_flow(walls):
 
 @pytest.mark.parametrize("walls", ["isothermal+adiabatic", "cbc_upt+runp", "cbc_upt+nscbc", "cbc_upt+sponge",
                                    "wm+wm_translate", "tpt+nscbc", "nscbc_mt+nscbc", "wmiso+wmiso",
-                                   "cbc_upt+mdot", "tpt_avg+nscbc", "nscbc_mt_t0+nscbc", "wmeq+wmeq_translate"])
+                                   "cbc_upt+mdot", "tpt_avg+nscbc", "nscbc_mt_t0+nscbc", "wmeq+wmeq_translate",
+                                   "wmeqiso+wmeqiso"])
 def test_mixed_precision_channel_close_to_oracle(walls):
     m = Mesh.channel(12, jitter=0.3, seed=8)
     walls = WALLS[walls]
