@@ -188,9 +188,10 @@ typedef struct cg_bc {
   double u[3];    /* wall velocity (WALL_ADIABATIC TRANSLATE), u of CBC_UPT / CBC_RUP */
   double p;       /* CBC pressure (CBC_TOTAL_PT: p0), NSCBC p_ref, SPONGE_PRESSURE (negative: P_REF) */
   double rho;     /* CBC_RUNP / CBC_RUP density */
-  double q;       /* CBC_MPT / NSCBC_MT / NSCBC_OUTLET_MDOT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH */
-  double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure) */
-  double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax */
+  double q;       /* CBC_MPT / NSCBC_MT / NSCBC_OUTLET_MDOT mass flow mdot, CBC_RUNP inflow normal velocity un, SPONGE STRENGTH,
+                     WM_EQ_* TOL */
+  double sigma;   /* NSCBC outlet relaxation coefficient (negative: constant pressure), WM_EQ_* RELAX */
+  double L;       /* NSCBC outlet length scale, SPONGE LENGTH, CBC_TOTAL_PT t_relax, WM_EQ_* Y_PLUS0 */
   int mode;       /* SPONGE TYPE: 0 L_X, 1 L_Y, 2 L_Z, 3 L_RX, 4 L_RZ; CBC_TOTAL_PT 1: USE_AVG;
                      NSCBC_MT 1: T_TOTAL (T is then the total temperature) */
   /* passive scalars entering through a CBC zone (parseScalarBc, BcCommon.hpp:213-274):
